@@ -1,0 +1,192 @@
+"""Benchmark of the SIMPLEC linear-algebra hot path on one MI355X.
+
+Workload: the 1026x514 pressure-correction (p') system on the wall-resolved
+ogive grid (near-wall aspect ratio 1e4, seeded Gaussian right-hand side),
+solved by MSI from a zero guess to a 1e-10 residual drop
+(||r||_2 < 1e-10 ||r0||_2, the stopping rule of fb2d9).  One step = one full
+solve: factorization (lu2d9) plus the iterations (fb2d9), with the
+coefficients and right-hand side already resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--mode fp64|fp32_ir]
+
+Prints one JSON line.  value = solves/s of the timed mode; gbps = the
+algorithmic bytes of the solve, s*N*(18 + 11 + 24*K_it) with s = 8 bytes,
+over the solve time; roofline = the dominant kernel's algorithmic bytes per
+launch over its average launch duration (HIP events on the solver's stream
+around each launch during the timed steps); cpu_baseline = the reference
+MSI (compiled Fortran, one core) timed on a bounded sample of the same
+solve on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mach-2d_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = ("p' solves/s and HBM GB/s to 1e-10 residual drop; learned+FP32 vs plain FP64")
+HBM_PEAK_GBS = 8000.0
+GAMMA = 1e-10
+NITM_CAP = 20000
+
+# algorithmic bytes per point of one launch, in units of the value size s
+BYTES_PER_POINT = {
+    "factor": 18,    # read a (9), write dl (5) and du (4)
+    "forward": 7,    # read r, dl (5); write w
+    "backward": 8,   # read w, du (4), var; write z (over w), var
+    "residual": 12,  # read a (9), rhs, var; write r
+}
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def make_system():
+    from mach2d import problems as P
+    c = P.CFG3
+    t = time.time()
+    s = P.pcorr_system(c["nx"], c["ny"], seed=c["seed"], aspect=c["aspect"], rhs=c["rhs"])
+    log("system %dx%d (q=%.5f, aspect %.3g) built in %.1fs" % (
+        s.nx, s.ny, s.q, c["aspect"], time.time() - t))
+    return s
+
+
+def gpu_run(s, mode, steps, warmup):
+    import torch
+
+    from mach2d import solvers as S
+    dev = S.DeviceMSI(s.nx, s.ny, precision=mode)
+    ta = torch.tensor(s.a, device="cuda")
+    tr = torch.tensor(s.rhs, device="cuda")
+    tv = torch.zeros(s.N, dtype=torch.float64, device="cuda")
+
+    def one():
+        tv.zero_()
+        dev.factor(ta)
+        return dev.solve(ta, tr, tv, GAMMA, NITM_CAP)
+
+    for w in range(warmup):
+        st = one()
+        log("warmup %d: status %d nit %d (%.2f ms)" % (w, st.status, st.nit, st.device_ms))
+    dev.enable_kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sts = [one() for _ in range(steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kt = dev.kernel_timing()
+    dev.enable_kernel_timing(False)
+    st = sts[-1]
+    ok = all(x.status == 0 and x.rn < x.gammarp for x in sts)
+    log("%s: %d steps in %.3fs, nit %d, status %d, r0 %.3e rn %.3e" % (
+        mode, steps, dt, st.nit, st.status, st.r0, st.rn))
+    x = tv.cpu().numpy()
+    dev.close()
+    return dict(dt=dt, st=st, ok=ok, kt=kt, x=x)
+
+
+def cpu_baseline(s, nit, budget_s=15.0):
+    """Reference lu2d9 + fb2d9 (compiled Fortran, one core) on a bounded
+    sample: the factorization plus M iterations; solves/s extrapolated to
+    the nit iterations the same algorithm needs for the 1e-10 drop."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    if kind == "reference":
+        lu = lambda: O.ref_lu2d9(s.a, s.nx, s.ny)
+        fb = lambda dl, du, m: O.ref_fb2d9(s.a, dl, du, s.nx, s.ny, np.zeros(s.N), 0.0, m,
+                                           s.rhs)
+    else:
+        lu = lambda: O.lu2d9(s.a, s.nx, s.ny)
+        fb = lambda dl, du, m: O.fb2d9(s.a, dl, du, s.nx, s.ny, np.zeros(s.N), 0.0, m, s.rhs)
+    t = time.perf_counter()
+    dl, du = lu()
+    t_lu = time.perf_counter() - t
+    t = time.perf_counter()
+    fb(dl, du, 2)
+    t2 = time.perf_counter() - t
+    m = int(max(2, min(nit, (budget_s - t_lu) / max(t2 / 2, 1e-6))))
+    t = time.perf_counter()
+    fb(dl, du, m)
+    t_m = time.perf_counter() - t
+    t_iter = t_m / m
+    t_solve = t_lu + nit * t_iter
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": kind,
+            "sample": "lu2d9 + %d fb2d9 iterations of the 1026x514 system (%.1fs); "
+                      "solve time = t_lu %.3fs + %d iterations x %.2f ms" % (
+                          m, t_lu + t_m, t_lu, nit, t_iter * 1e3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", default="fp64", choices=["fp64", "fp32_ir"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    s = make_system()
+    r = gpu_run(s, args.mode, args.steps, args.warmup)
+    st, dt = r["st"], r["dt"]
+    nit = st.nit if args.mode == "fp64" else st.nit
+    per = dt / args.steps
+    sz = 8
+    algo_bytes = sz * s.N * (18 + 11 + 24 * nit)
+    # dominant kernel by device time
+    kt = r["kt"]
+    dom = max(kt, key=lambda k: kt[k][0]) if kt else None
+    roof = None
+    if dom:
+        base = dom.replace("32", "")
+        size = 4 if dom.endswith("32") else 8
+        per_launch = BYTES_PER_POINT[base] * size * s.N
+        avg_ms = kt[dom][0] / kt[dom][1]
+        ach = per_launch / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
+                "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 2),
+                "bytes_per_launch": per_launch}
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(s, nit)
+        log("cpu: %.4g solves/s (%s)" % (cpu["value"], cpu["sample"]))
+    out = {
+        "metric": METRIC,
+        "value": round(args.steps / dt, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(per * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.mode == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "p' MSI solve, 1026x514 wall-resolved ogive grid "
+                               "(aspect 1e4), zero guess, 1e-10 residual drop",
+                   "mode": args.mode, "nx": s.nx, "ny": s.ny, "nit": nit,
+                   "gamma": GAMMA, "correct": bool(r["ok"])},
+        "gbps": round(algo_bytes / per / 1e9, 2),
+        "kernels_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

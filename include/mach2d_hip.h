@@ -1,0 +1,169 @@
+/*
+ * mach2d_hip.h -- C ABI of the MI355X (gfx950) SIMPLEC linear-algebra hot
+ * path: 9-diagonal MSI (modified strongly implicit) factorization and solve,
+ * alternating-direction line TDMA, and L1 residual norms.
+ *
+ * Layout everywhere: the reference's column-major Fortran arrays seen from C.
+ * A 9-diagonal coefficient array a(nij,9) is `double a[9][nij]` (diagonal k,
+ * 1-based, at a + (k-1)*nij), slots 1=SW 2=S 3=SE 4=W 5=P 6=E 7=NW 8=N 9=NE;
+ * the point index is np = ni*(j-1)+i (1-based, i fastest).
+ *
+ * Two families of entry points:
+ *
+ *  1. Reference-signature entry points (host arrays, synchronous, every
+ *     argument by reference, so a Fortran `bind(C)` interface with no VALUE
+ *     attributes binds them directly).  They replace the reference routines
+ *     named in each comment and keep their argument order and meaning.
+ *     Status, iteration count and residual norms of the last call, which the
+ *     reference signatures cannot return, are read with m2d_last_status().
+ *
+ *  2. A handle API for device-resident data (device pointers; no host
+ *     copies), used for batching, graph capture and the mixed-precision and
+ *     learned-guess extensions.
+ *
+ * Every entry point requires the invariant the reference states at
+ * src/solvers/msi2d9-v03.f90:16-20: coefficients that couple a point to a
+ * neighbour outside the grid are exactly zero.  Inputs that break it are
+ * rejected with M2D_ERR_OUTSIDE_COUPLING (the reference would silently couple
+ * across row ends through its flat index).
+ */
+#ifndef MACH2D_HIP_H
+#define MACH2D_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (m2d_status.status and the int return values). */
+enum {
+    M2D_OK = 0,
+    M2D_ERR_NONFINITE = 1,          /* zero/non-finite pivot or residual     */
+    M2D_ERR_HIP = 3,                /* a HIP runtime call failed             */
+    M2D_ERR_ARG = 4,                /* invalid argument (sizes, pointers)    */
+    M2D_ERR_OUTSIDE_COUPLING = 5,   /* a coupling to an outside neighbour    */
+    M2D_ERR_SYNC_TIMEOUT = 6,       /* inter-workgroup wait gave up          */
+    M2D_ERR_ITER_CAP = 7            /* nitm <= 0 and the safety cap reached  */
+};
+
+/* Precision of a handle. */
+enum {
+    M2D_FP64 = 0,       /* FP64 factorization and sweeps (parity mode)        */
+    M2D_FP32_IR = 1     /* FP32 factorization and sweeps inside FP64
+                           iterative refinement (FP64 residual and update)    */
+};
+
+typedef struct m2d_status {
+    int status;         /* M2D_OK or an error code above                      */
+    int nit;            /* MSI iterations run (fb2d9's nit)                   */
+    int nitm_reached;   /* 1 when the loop ended on nit == nitm               */
+    int outer;          /* refinement steps (M2D_FP32_IR), else 0             */
+    double r0;          /* ||A.var_in - rhs||_2                               */
+    double rn;          /* ||A.var_out - rhs||_2                              */
+    double gammarp;     /* gamma * r0, the stopping threshold                 */
+    double device_ms;   /* device time of the call's kernels                  */
+} m2d_status;
+
+/* ---- 1. Reference-signature entry points (host arrays) ----------------- */
+
+/* Replaces lu2d9(b,dl,du,nij,ni,nj), src/solvers/msi2d9-v03.f90:277-282.
+ * Note the argument order (nij, ni, nj) of the reference. */
+void m2d_lu2d9(const double *b, double *dl, double *du, const int *nij,
+               const int *ni, const int *nj);
+
+/* Replaces fb2d9(b,dl,du,nj,ni,nij,var,gamma,nitm,rhs),
+ * src/solvers/msi2d9-v03.f90:42-50.  Note the order (nj, ni, nij).
+ * Stops when ||r||_2 < gamma*||r0||_2 or after nitm iterations; at least one
+ * iteration always runs; nitm <= 0 means no count limit (a safety cap of
+ * 1,000,000 iterations then applies, status M2D_ERR_ITER_CAP). */
+void m2d_fb2d9(const double *b, const double *dl, const double *du,
+               const int *nj, const int *ni, const int *nij, double *var,
+               const double *gamma, const int *nitm, const double *rhs);
+
+/* Replaces tdma2d9(nx,ny,nitm,a,b,x), src/solvers/tdma2d9.f90:38-45. */
+void m2d_tdma2d9(const int *nx, const int *ny, const int *nitm,
+                 const double *a, const double *b, double *x);
+
+/* Replaces class_solver_msi9d%solve(a,b,sol) after init(nx,ny,nitm,tol),
+ * src/solvers/mod_solver_msi.f90:129-166: lu2d9 then fb2d9. */
+void m2d_solve_msi9d(const int *nx, const int *ny, const int *nitm,
+                     const double *tol, const double *a, const double *b,
+                     double *sol);
+
+/* Replace norm_l1_9d, norm_l1_9d_relative (accumulate into *norm) and
+ * norm_l1_b, src/solvers/mod_solvers.f90:128-170, 176-245, 250-276. */
+void m2d_norm_l1_9d(const int *nx, const int *ny, const double *var,
+                    const double *b, const double *a, double *norm);
+void m2d_norm_l1_9d_relative(const int *nx, const int *ny, const double *var,
+                             const double *b, const double *a, double *norm);
+double m2d_norm_l1_b(const int *nx, const int *ny, const double *b);
+
+/* Status of the last call made by this host thread (no reference
+ * counterpart: the reference routines return nothing and fail silently). */
+int m2d_last_status(m2d_status *out);
+
+/* ---- 2. Handle API (device pointers) ----------------------------------- */
+
+typedef struct m2d_system m2d_system;
+
+/* Create a solver for one nx*ny grid; precision is M2D_FP64 or M2D_FP32_IR.
+ * Allocates the factor workspace (dl, du: 9*nx*ny values) on the current
+ * device.  Mirrors class_solver_msi9d%init, mod_solver_msi.f90:129-148. */
+int m2d_create(int nx, int ny, int precision, m2d_system **out);
+int m2d_destroy(m2d_system *s);
+
+/* Run on a HIP stream (hipStream_t passed as void*; NULL = default). */
+int m2d_set_stream(m2d_system *s, void *stream);
+
+/* Factor the device array d_a (9*nx*ny doubles, layout above).  In FP32_IR
+ * mode the factor and an FP32 copy of A are kept in FP32. */
+int m2d_factor(m2d_system *s, const double *d_a);
+
+/* Solve A.var = rhs (device pointers) from the var passed in, with the
+ * stopping rule of fb2d9: ||r||_2 < gamma*||r0||_2 or nitm iterations.
+ * d_hist, if not NULL, receives ||r_k||_2 for k = 1..nit (capacity hist_cap).
+ * In FP32_IR mode, gamma and nitm bound the FP64 outer residual; inner FP32
+ * sweeps run until their own residual drops by `inner_gamma` (set with
+ * m2d_set_inner) or `inner_nitm` iterations. */
+int m2d_solve(m2d_system *s, const double *d_a, const double *d_rhs,
+              double *d_var, double gamma, int nitm, double *d_hist,
+              int hist_cap, m2d_status *st);
+
+int m2d_set_inner(m2d_system *s, double inner_gamma, int inner_nitm);
+
+/* Device copies of the factor (dl 5*nx*ny, du 4*nx*ny doubles, reference
+ * layout); FP64 handles only. */
+int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du);
+
+/* Launch-shape tuning of the wavefront kernels (autotuner hook):
+ * publish_every = columns between progress publications of a band;
+ * xcd_stride = 1 (bands on consecutive XCDs) or 8 (bands of one system on
+ * one XCD). */
+int m2d_set_tuning(m2d_system *s, int publish_every, int xcd_stride);
+
+/* Batched: solve `count` independent systems (each with its own handle,
+ * factor, and device arrays) with one wavefront launch per phase for the
+ * whole batch.  Arrays are indexed like the handles. */
+int m2d_solve_batch(m2d_system **systems, int count, const double **d_a,
+                    const double **d_rhs, double **d_var, double gamma,
+                    int nitm, m2d_status *st);
+
+/* TDMA on device pointers (tdma2d9 semantics). */
+int m2d_tdma2d9_device(int nx, int ny, int nitm, const double *d_a,
+                       const double *d_b, double *d_x, void *stream);
+
+/* Per-kernel device time of this handle's launches (HIP events on the
+ * handle's stream around each launch; launches after the stopping rule
+ * fired are not counted).  Kernel ids: 0 factor, 1 forward sweep,
+ * 2 backward sweep + update, 3 residual; +4 for the FP32 kernels of
+ * M2D_FP32_IR.  Enabling resets the totals.  No reference counterpart (the
+ * reference times only the whole run with cpu_time). */
+int m2d_enable_kernel_timing(m2d_system *s, int on);
+int m2d_kernel_timing(m2d_system *s, int kernel, double *total_ms, long *launches);
+
+/* Library version string. */
+const char *m2d_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

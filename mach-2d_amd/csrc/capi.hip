@@ -1,0 +1,662 @@
+// capi.hip -- the C ABI (include/mach2d_hip.h): solver handles, the device
+// side MSI iteration loop, and the reference-signature entry points.
+//
+// The iteration loop keeps the stopping decision on the device: every
+// iteration is [forward sweep, backward sweep + update, residual, norm and
+// stopping rule]; once the rule fires a flag turns the remaining kernels
+// into no-ops.  The host enqueues iterations in chunks and reads the flag
+// once per chunk, so the loop costs one host round trip per chunk instead of
+// one per iteration.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "m2d_internal.h"
+
+using namespace m2d;
+
+#define HIP_OK(call)                                                  \
+    do {                                                              \
+        hipError_t e_ = (call);                                       \
+        if (e_ != hipSuccess) {                                       \
+            fprintf(stderr, "mach2d_hip: %s failed: %s (%s:%d)\n", #call, \
+                    hipGetErrorString(e_), __FILE__, __LINE__);       \
+            return M2D_ERR_HIP;                                       \
+        }                                                             \
+    } while (0)
+
+struct m2d_system {
+    int nx = 0, ny = 0;
+    long N = 0;
+    int precision = M2D_FP64;
+    hipStream_t stream = nullptr;
+    int publish_every = 32;
+    int xcd_stride = 1;
+    int chunk = 8;
+    double inner_gamma = 1e-4;
+    int inner_nitm = 200;
+
+    // FP64 factor and sweep workspace
+    double *dl = nullptr, *du = nullptr, *r = nullptr, *w = nullptr;
+    // FP32 workspace (mixed precision)
+    float *a32 = nullptr, *dl32 = nullptr, *du32 = nullptr, *r32 = nullptr, *w32 = nullptr,
+          *c32 = nullptr, *rr32 = nullptr;
+    double *partial = nullptr;
+    int np = 0;
+    unsigned long long *flags = nullptr;
+    int nflags = 0;
+    SolveState *st = nullptr;     // device
+    SolveState *st_in = nullptr;  // device, inner solve (mixed precision)
+    SolveState *h_st = nullptr;   // pinned host mirror
+    unsigned gen = 0;
+    bool factored = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // optional per-kernel event timing (m2d_enable_kernel_timing)
+    struct Rec {
+        int kernel, k;
+        hipEvent_t a, b;
+    };
+    bool timing = false;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    std::vector<Rec> recs;
+    double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// Kernel ids of the timing facility: factor, forward, backward, residual;
+// +4 for the FP32 kernels of mixed precision.
+enum { KT_FACTOR = 0, KT_FORWARD = 1, KT_BACKWARD = 2, KT_RESIDUAL = 3, KT_FP32 = 4 };
+
+static hipEvent_t pool_event(m2d_system *s)
+{
+    if (s->pool_used == s->pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        s->pool.push_back(e);
+    }
+    return s->pool[s->pool_used++];
+}
+
+// Bracket the launches made by fn() with events when timing is on.
+template <typename F>
+static void timed(m2d_system *s, int kernel, int k, F fn)
+{
+    if (!s->timing) {
+        fn();
+        return;
+    }
+    hipEvent_t a = pool_event(s), b = pool_event(s);
+    if (!a || !b) {
+        fn();
+        return;
+    }
+    (void)hipEventRecord(a, s->stream);
+    fn();
+    (void)hipEventRecord(b, s->stream);
+    s->recs.push_back({kernel, k, a, b});
+}
+
+// Accumulate the recorded launches of iterations k <= nit (launches after
+// the stopping rule fired are no-ops and are not counted).
+static void harvest(m2d_system *s, int nit)
+{
+    if (!s->timing) return;
+    for (const auto &r : s->recs) {
+        if (r.k > nit) continue;
+        float ms = 0.f;
+        if (hipEventSynchronize(r.b) == hipSuccess &&
+            hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            s->kt_ms[r.kernel] += ms;
+            s->kt_n[r.kernel] += 1;
+        }
+    }
+    s->recs.clear();
+    s->pool_used = 0;
+}
+
+static thread_local m2d_status g_last = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0};
+
+static void set_last(int status)
+{
+    memset(&g_last, 0, sizeof(g_last));
+    g_last.status = status;
+}
+
+extern "C" const char *m2d_version(void) { return "mach2d_hip 0.1 (gfx950)"; }
+
+extern "C" int m2d_last_status(m2d_status *out)
+{
+    if (!out) return M2D_ERR_ARG;
+    *out = g_last;
+    return M2D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+template <typename T>
+static hipError_t dalloc(T **p, long n)
+{
+    return hipMalloc((void **)p, sizeof(T) * (size_t)std::max<long>(n, 1));
+}
+
+extern "C" int m2d_destroy(m2d_system *s)
+{
+    if (!s) return M2D_OK;
+    void *bufs[] = {s->dl,   s->du,   s->r,    s->w,   s->a32,     s->dl32,  s->du32,
+                    s->r32,  s->w32,  s->c32,  s->rr32, s->partial, s->flags, s->st,
+                    s->st_in};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (hipEvent_t e : s->pool) (void)hipEventDestroy(e);
+    if (s->h_st) (void)hipHostFree(s->h_st);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    delete s;
+    return M2D_OK;
+}
+
+extern "C" int m2d_create(int nx, int ny, int precision, m2d_system **out)
+{
+    if (!out || nx < 3 || ny < 3 || (precision != M2D_FP64 && precision != M2D_FP32_IR))
+        return M2D_ERR_ARG;
+    if ((long)nx * ny > (1L << 31) / 9) return M2D_ERR_ARG;
+    m2d_system *s = new m2d_system();
+    s->nx = nx;
+    s->ny = ny;
+    s->N = (long)nx * ny;
+    s->precision = precision;
+    s->np = residual_blocks(nx, ny);
+    s->nflags = (ny + kWave - 1) / kWave;
+    const long N = s->N;
+    bool ok = true;
+    ok &= dalloc(&s->r, N) == hipSuccess;
+    ok &= dalloc(&s->w, N) == hipSuccess;
+    ok &= dalloc(&s->partial, s->np) == hipSuccess;
+    ok &= dalloc(&s->flags, s->nflags) == hipSuccess;
+    ok &= dalloc(&s->st, 1) == hipSuccess;
+    ok &= dalloc(&s->st_in, 1) == hipSuccess;
+    if (precision == M2D_FP64) {
+        ok &= dalloc(&s->dl, 5 * N) == hipSuccess;
+        ok &= dalloc(&s->du, 4 * N) == hipSuccess;
+    } else {
+        ok &= dalloc(&s->a32, 9 * N) == hipSuccess;
+        ok &= dalloc(&s->dl32, 5 * N) == hipSuccess;
+        ok &= dalloc(&s->du32, 4 * N) == hipSuccess;
+        ok &= dalloc(&s->r32, N) == hipSuccess;
+        ok &= dalloc(&s->w32, N) == hipSuccess;
+        ok &= dalloc(&s->c32, N) == hipSuccess;
+        ok &= dalloc(&s->rr32, N) == hipSuccess;
+    }
+    ok &= hipHostMalloc((void **)&s->h_st, sizeof(SolveState), hipHostMallocDefault) ==
+          hipSuccess;
+    ok &= hipEventCreate(&s->ev0) == hipSuccess;
+    ok &= hipEventCreate(&s->ev1) == hipSuccess;
+    if (!ok) {
+        m2d_destroy(s);
+        return M2D_ERR_HIP;
+    }
+    *out = s;
+    return M2D_OK;
+}
+
+extern "C" int m2d_set_stream(m2d_system *s, void *stream)
+{
+    if (!s) return M2D_ERR_ARG;
+    s->stream = (hipStream_t)stream;
+    return M2D_OK;
+}
+
+extern "C" int m2d_set_tuning(m2d_system *s, int publish_every, int xcd_stride)
+{
+    if (!s || publish_every < 1 || (xcd_stride != 1 && xcd_stride != 8)) return M2D_ERR_ARG;
+    s->publish_every = publish_every;
+    s->xcd_stride = xcd_stride;
+    return M2D_OK;
+}
+
+extern "C" int m2d_set_inner(m2d_system *s, double inner_gamma, int inner_nitm)
+{
+    if (!s || !(inner_gamma > 0.0) || inner_nitm < 1) return M2D_ERR_ARG;
+    s->inner_gamma = inner_gamma;
+    s->inner_nitm = inner_nitm;
+    return M2D_OK;
+}
+
+static SweepSync make_sync(m2d_system *s, SolveState *st)
+{
+    SweepSync y;
+    y.flags = s->flags;
+    y.st = st;
+    y.gen = ++s->gen;
+    y.publish_every = s->publish_every;
+    return y;
+}
+
+static int read_state(m2d_system *s, SolveState *dev)
+{
+    HIP_OK(hipMemcpyAsync(s->h_st, dev, sizeof(SolveState), hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return M2D_OK;
+}
+
+// Reset state and flags; gens restart so a replayed sequence is valid.
+static void begin(m2d_system *s, SolveState *st)
+{
+    launch_zero_flags(s->flags, s->nflags, st, 1, s->stream);
+    s->gen = 0;
+}
+
+extern "C" int m2d_factor(m2d_system *s, const double *d_a)
+{
+    if (!s || !d_a) return M2D_ERR_ARG;
+    begin(s, s->st);
+    launch_check_outside(d_a, s->nx, s->ny, s->st, s->stream);
+    if (s->precision == M2D_FP64) {
+        timed(s, KT_FACTOR, 0, [&] {
+            launch_factor<double>(d_a, s->dl, s->du, s->nx, s->ny, 0.5, make_sync(s, s->st),
+                                  s->xcd_stride, s->stream);
+        });
+    } else {
+        launch_convert<double, float>(d_a, s->a32, 9 * s->N, s->stream);
+        timed(s, KT_FACTOR + KT_FP32, 0, [&] {
+            launch_factor<float>(s->a32, s->dl32, s->du32, s->nx, s->ny, 0.5f,
+                                 make_sync(s, s->st), s->xcd_stride, s->stream);
+        });
+    }
+    HIP_OK(hipGetLastError());
+    int rc = read_state(s, s->st);
+    if (rc) return rc;
+    harvest(s, 0);
+    if (s->h_st->outside) return M2D_ERR_OUTSIDE_COUPLING;
+    if (s->h_st->abort) return M2D_ERR_SYNC_TIMEOUT;
+    s->factored = true;
+    if (s->h_st->err_pivot) return M2D_ERR_NONFINITE;
+    return M2D_OK;
+}
+
+extern "C" int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du)
+{
+    if (!s || s->precision != M2D_FP64 || !s->factored) return M2D_ERR_ARG;
+    HIP_OK(hipMemcpyAsync(d_dl, s->dl, sizeof(double) * 5 * s->N, hipMemcpyDeviceToDevice,
+                          s->stream));
+    HIP_OK(hipMemcpyAsync(d_du, s->du, sizeof(double) * 4 * s->N, hipMemcpyDeviceToDevice,
+                          s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return M2D_OK;
+}
+
+// Install an externally computed FP64 factor (fb2d9 takes dl, du as input).
+static int put_factor(m2d_system *s, const double *d_dl, const double *d_du)
+{
+    HIP_OK(hipMemcpyAsync(s->dl, d_dl, sizeof(double) * 5 * s->N, hipMemcpyDeviceToDevice,
+                          s->stream));
+    HIP_OK(hipMemcpyAsync(s->du, d_du, sizeof(double) * 4 * s->N, hipMemcpyDeviceToDevice,
+                          s->stream));
+    s->factored = true;
+    return M2D_OK;
+}
+
+// One MSI iteration (k >= 1) in precision T on the system's workspace.
+template <typename T>
+static void enqueue_iteration(m2d_system *s, SolveState *st, const T *a, const T *rhs, T *x,
+                              const T *dl, const T *du, T *r, T *w, double *hist, int hist_cap,
+                              int k, double gamma, int nitm)
+{
+    const int off = sizeof(T) == 4 ? KT_FP32 : 0;
+    timed(s, KT_FORWARD + off, k, [&] {
+        launch_forward<T>(r, dl, w, s->nx, s->ny, make_sync(s, st), s->xcd_stride, s->stream);
+    });
+    timed(s, KT_BACKWARD + off, k, [&] {
+        launch_backward<T, T>(w, du, x, s->nx, s->ny, make_sync(s, st), s->xcd_stride,
+                              s->stream);
+    });
+    timed(s, KT_RESIDUAL + off, k, [&] {
+        launch_residual<T, T, T>(a, x, rhs, r, s->nx, s->ny, s->partial, st, s->stream);
+    });
+    launch_norm_finalize(s->partial, s->np, st, hist, hist_cap, k, gamma, nitm, s->stream);
+}
+
+// fb2d9 on the device, in precision T, from the current x: returns after the
+// stopping rule fired (state in s->h_st).
+template <typename T>
+static int run_msi(m2d_system *s, SolveState *st, const T *a, const T *rhs, T *x, const T *dl,
+                   const T *du, T *r, T *w, double gamma, int nitm, double *hist, int hist_cap)
+{
+    begin(s, st);
+    launch_residual<T, T, T>(a, x, rhs, r, s->nx, s->ny, s->partial, st, s->stream);
+    launch_norm_finalize(s->partial, s->np, st, nullptr, 0, 0, gamma, nitm, s->stream);
+    const int kmax = nitm > 0 ? nitm : kIterCap;
+    int k = 1;
+    int chunk = 2;  // grows geometrically up to s->chunk
+    while (k <= kmax) {
+        const int kend = std::min(kmax, k + chunk - 1);
+        for (; k <= kend; ++k)
+            enqueue_iteration<T>(s, st, a, rhs, x, dl, du, r, w, hist, hist_cap, k, gamma, nitm);
+        HIP_OK(hipGetLastError());
+        int rc = read_state(s, st);
+        if (rc) return rc;
+        if (s->h_st->done) break;
+        chunk = std::min(chunk * 2, s->chunk);
+    }
+    harvest(s, s->h_st->nit);
+    return M2D_OK;
+}
+
+static void fill_status(m2d_system *s, m2d_status *st, float ms)
+{
+    if (!st) return;
+    st->status = s->h_st->status;
+    st->nit = s->h_st->nit;
+    st->nitm_reached = s->h_st->nitm_hit;
+    st->outer = s->h_st->outer;
+    st->r0 = s->h_st->r0;
+    st->rn = s->h_st->rn;
+    st->gammarp = s->h_st->gammarp;
+    st->device_ms = ms;
+}
+
+static int solve_fp64(m2d_system *s, const double *d_a, const double *d_rhs, double *d_var,
+                      double gamma, int nitm, double *d_hist, int hist_cap, m2d_status *out)
+{
+    HIP_OK(hipEventRecord(s->ev0, s->stream));
+    int rc = run_msi<double>(s, s->st, d_a, d_rhs, d_var, s->dl, s->du, s->r, s->w, gamma, nitm,
+                             d_hist, hist_cap);
+    if (rc) return rc;
+    HIP_OK(hipEventRecord(s->ev1, s->stream));
+    HIP_OK(hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    fill_status(s, out, ms);
+    return s->h_st->status;
+}
+
+int solve_fp32_ir(m2d_system *s, const double *d_a, const double *d_rhs, double *d_var,
+                  double gamma, int nitm, double *d_hist, int hist_cap, m2d_status *out);
+
+extern "C" int m2d_solve(m2d_system *s, const double *d_a, const double *d_rhs, double *d_var,
+                         double gamma, int nitm, double *d_hist, int hist_cap, m2d_status *st)
+{
+    if (!s || !d_a || !d_rhs || !d_var || (d_hist && hist_cap < 1)) return M2D_ERR_ARG;
+    if (!s->factored) return M2D_ERR_ARG;
+    if (s->precision == M2D_FP64)
+        return solve_fp64(s, d_a, d_rhs, d_var, gamma, nitm, d_hist, hist_cap, st);
+    return solve_fp32_ir(s, d_a, d_rhs, d_var, gamma, nitm, d_hist, hist_cap, st);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: FP32 factor and FP32 MSI sweeps inside FP64 iterative
+// refinement.  Outer step: r = A x - rhs in FP64 (stopping rule of fb2d9 on
+// this FP64 residual), then the correction c solves A c = -r approximately by
+// FP32 MSI iterations from c = 0 (A, its factor, residuals and sweeps in
+// FP32) until its own residual drops by inner_gamma, then x += c in FP64.
+// ---------------------------------------------------------------------------
+__global__ void k_ir_prepare(const double *__restrict__ r64, float *__restrict__ rr32,
+                             float *__restrict__ c32, long n, const SolveState *st)
+{
+    if (st->done) return;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) {
+        rr32[q] = (float)(-r64[q]);
+        c32[q] = 0.f;
+    }
+}
+
+__global__ void k_ir_update(double *__restrict__ x, const float *__restrict__ c32, long n,
+                            const SolveState *st)
+{
+    if (st->done) return;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) x[q] = x[q] + (double)c32[q];
+}
+
+__global__ void k_copy_done(SolveState *outer, const SolveState *inner, int *flag_bad)
+{
+    // inner status propagates only as an error code
+    if (inner->status && inner->status != M2D_ERR_ITER_CAP) outer->status = inner->status;
+    (void)flag_bad;
+}
+
+int solve_fp32_ir(m2d_system *s, const double *d_a, const double *d_rhs, double *d_var,
+                  double gamma, int nitm, double *d_hist, int hist_cap, m2d_status *out)
+{
+    HIP_OK(hipEventRecord(s->ev0, s->stream));
+    const long N = s->N;
+    const unsigned nblk = (unsigned)((N + 255) / 256);
+    // outer state lives in s->st; its flags are untouched by the inner solves
+    launch_zero_flags(s->flags, s->nflags, s->st, 1, s->stream);
+    launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r, s->nx, s->ny, s->partial,
+                                            s->st, s->stream);
+    launch_norm_finalize(s->partial, s->np, s->st, nullptr, 0, 0, gamma, nitm, s->stream);
+    HIP_OK(hipGetLastError());
+    int rc = read_state(s, s->st);
+    if (rc) return rc;
+    SolveState outer = *s->h_st;
+    int total_inner = 0;
+    const int kmax = nitm > 0 ? nitm : kIterCap;
+    int k = 1;
+    for (; k <= kmax && !outer.done; ++k) {
+        hipLaunchKernelGGL(k_ir_prepare, dim3(nblk), dim3(256), 0, s->stream, s->r, s->rr32,
+                           s->c32, N, s->st);
+        rc = run_msi<float>(s, s->st_in, s->a32, s->rr32, s->c32, s->dl32, s->du32, s->r32,
+                            s->w32, s->inner_gamma, s->inner_nitm, nullptr, 0);
+        if (rc) return rc;
+        total_inner += s->h_st->nit;
+        const int inner_status = s->h_st->status;
+        hipLaunchKernelGGL(k_ir_update, dim3(nblk), dim3(256), 0, s->stream, d_var, s->c32, N,
+                           s->st);
+        launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r, s->nx, s->ny,
+                                                s->partial, s->st, s->stream);
+        launch_norm_finalize(s->partial, s->np, s->st, d_hist, hist_cap, k, gamma, nitm,
+                             s->stream);
+        HIP_OK(hipGetLastError());
+        rc = read_state(s, s->st);
+        if (rc) return rc;
+        outer = *s->h_st;
+        if (inner_status == M2D_ERR_NONFINITE || inner_status == M2D_ERR_SYNC_TIMEOUT) {
+            outer.status = inner_status;
+            outer.done = 1;
+        }
+    }
+    HIP_OK(hipEventRecord(s->ev1, s->stream));
+    HIP_OK(hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    *s->h_st = outer;
+    s->h_st->outer = outer.nit;
+    s->h_st->nit = total_inner;
+    fill_status(s, out, ms);
+    return outer.status;
+}
+
+// ---------------------------------------------------------------------------
+// Reference-signature entry points (host arrays).  A per-thread cache keeps
+// the device buffers and an FP64 handle for the last grid size.
+// ---------------------------------------------------------------------------
+struct HostCache {
+    int nx = 0, ny = 0;
+    m2d_system *sys = nullptr;
+    double *a = nullptr, *dl = nullptr, *du = nullptr, *rhs = nullptr, *var = nullptr;
+    ~HostCache() { release(); }
+    void release()
+    {
+        double *bufs[] = {a, dl, du, rhs, var};
+        for (double *b : bufs)
+            if (b) (void)hipFree(b);
+        a = dl = du = rhs = var = nullptr;
+        if (sys) m2d_destroy(sys);
+        sys = nullptr;
+        nx = ny = 0;
+    }
+    int ensure(int nx_, int ny_)
+    {
+        if (sys && nx == nx_ && ny == ny_) return M2D_OK;
+        release();
+        int rc = m2d_create(nx_, ny_, M2D_FP64, &sys);
+        if (rc) return rc;
+        const long N = (long)nx_ * ny_;
+        if (dalloc(&a, 9 * N) || dalloc(&dl, 5 * N) || dalloc(&du, 4 * N) || dalloc(&rhs, N) ||
+            dalloc(&var, N)) {
+            release();
+            return M2D_ERR_HIP;
+        }
+        nx = nx_;
+        ny = ny_;
+        return M2D_OK;
+    }
+};
+static thread_local HostCache g_cache;
+
+static bool sizes_ok(int nij, int ni, int nj)
+{
+    return ni >= 3 && nj >= 3 && (long)ni * nj == (long)nij;
+}
+
+extern "C" void m2d_lu2d9(const double *b, double *dl, double *du, const int *nij,
+                          const int *ni, const int *nj)
+{
+    if (!b || !dl || !du || !nij || !ni || !nj || !sizes_ok(*nij, *ni, *nj)) {
+        set_last(M2D_ERR_ARG);
+        return;
+    }
+    const long N = *nij;
+    int rc = g_cache.ensure(*ni, *nj);
+    if (rc) {
+        set_last(rc);
+        return;
+    }
+    m2d_system *s = g_cache.sys;
+    if (hipMemcpy(g_cache.a, b, sizeof(double) * 9 * N, hipMemcpyHostToDevice) != hipSuccess) {
+        set_last(M2D_ERR_HIP);
+        return;
+    }
+    rc = m2d_factor(s, g_cache.a);
+    set_last(rc);
+    if (rc == M2D_OK || rc == M2D_ERR_NONFINITE) {
+        if (hipMemcpy(dl, s->dl, sizeof(double) * 5 * N, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(du, s->du, sizeof(double) * 4 * N, hipMemcpyDeviceToHost) != hipSuccess)
+            set_last(M2D_ERR_HIP);
+    }
+}
+
+static void host_fb2d9(const double *b, const double *dl, const double *du, int nj, int ni,
+                       int nij, double *var, double gamma, int nitm, const double *rhs,
+                       bool factor_here)
+{
+    const long N = nij;
+    int rc = g_cache.ensure(ni, nj);
+    if (rc) {
+        set_last(rc);
+        return;
+    }
+    m2d_system *s = g_cache.sys;
+    bool ok = hipMemcpy(g_cache.a, b, sizeof(double) * 9 * N, hipMemcpyHostToDevice) ==
+                  hipSuccess &&
+              hipMemcpy(g_cache.rhs, rhs, sizeof(double) * N, hipMemcpyHostToDevice) ==
+                  hipSuccess &&
+              hipMemcpy(g_cache.var, var, sizeof(double) * N, hipMemcpyHostToDevice) ==
+                  hipSuccess;
+    if (!ok) {
+        set_last(M2D_ERR_HIP);
+        return;
+    }
+    if (factor_here) {
+        rc = m2d_factor(s, g_cache.a);
+        if (rc && rc != M2D_ERR_NONFINITE) {
+            set_last(rc);
+            return;
+        }
+    } else {
+        ok = hipMemcpy(g_cache.dl, dl, sizeof(double) * 5 * N, hipMemcpyHostToDevice) ==
+                 hipSuccess &&
+             hipMemcpy(g_cache.du, du, sizeof(double) * 4 * N, hipMemcpyHostToDevice) ==
+                 hipSuccess;
+        if (!ok) {
+            set_last(M2D_ERR_HIP);
+            return;
+        }
+        // fb2d9 does not factor; still reject couplings across the grid edge
+        begin(s, s->st);
+        launch_check_outside(g_cache.a, ni, nj, s->st, s->stream);
+        rc = read_state(s, s->st);
+        if (!rc && s->h_st->outside) rc = M2D_ERR_OUTSIDE_COUPLING;
+        if (!rc) rc = put_factor(s, g_cache.dl, g_cache.du);
+        if (rc) {
+            set_last(rc);
+            return;
+        }
+    }
+    m2d_status st;
+    memset(&st, 0, sizeof(st));
+    rc = m2d_solve(s, g_cache.a, g_cache.rhs, g_cache.var, gamma, nitm, nullptr, 0, &st);
+    g_last = st;
+    g_last.status = rc;
+    if (rc == M2D_ERR_HIP) return;
+    if (hipMemcpy(var, g_cache.var, sizeof(double) * N, hipMemcpyDeviceToHost) != hipSuccess)
+        g_last.status = M2D_ERR_HIP;
+}
+
+extern "C" void m2d_fb2d9(const double *b, const double *dl, const double *du, const int *nj,
+                          const int *ni, const int *nij, double *var, const double *gamma,
+                          const int *nitm, const double *rhs)
+{
+    if (!b || !dl || !du || !nj || !ni || !nij || !var || !gamma || !nitm || !rhs ||
+        !sizes_ok(*nij, *ni, *nj)) {
+        set_last(M2D_ERR_ARG);
+        return;
+    }
+    host_fb2d9(b, dl, du, *nj, *ni, *nij, var, *gamma, *nitm, rhs, false);
+}
+
+extern "C" void m2d_solve_msi9d(const int *nx, const int *ny, const int *nitm, const double *tol,
+                                const double *a, const double *b, double *sol)
+{
+    if (!nx || !ny || !nitm || !tol || !a || !b || !sol || *nx < 3 || *ny < 3) {
+        set_last(M2D_ERR_ARG);
+        return;
+    }
+    host_fb2d9(a, nullptr, nullptr, *ny, *nx, *nx * *ny, sol, *tol, *nitm, b, true);
+}
+
+void m2d_set_last_status_code(int status) { set_last(status); }
+
+// Batched entry: every system keeps its own handle, factor and arrays.
+extern "C" int m2d_solve_batch(m2d_system **systems, int count, const double **d_a,
+                               const double **d_rhs, double **d_var, double gamma, int nitm,
+                               m2d_status *st)
+{
+    if (!systems || count < 0 || !d_a || !d_rhs || !d_var) return M2D_ERR_ARG;
+    int worst = M2D_OK;
+    for (int q = 0; q < count; ++q) {
+        int rc = m2d_solve(systems[q], d_a[q], d_rhs[q], d_var[q], gamma, nitm, nullptr, 0,
+                           st ? &st[q] : nullptr);
+        if (rc && !worst) worst = rc;
+    }
+    return worst;
+}
+
+extern "C" int m2d_enable_kernel_timing(m2d_system *s, int on)
+{
+    if (!s) return M2D_ERR_ARG;
+    s->timing = on != 0;
+    for (int k = 0; k < 8; ++k) {
+        s->kt_ms[k] = 0.0;
+        s->kt_n[k] = 0;
+    }
+    s->recs.clear();
+    s->pool_used = 0;
+    return M2D_OK;
+}
+
+extern "C" int m2d_kernel_timing(m2d_system *s, int kernel, double *total_ms, long *launches)
+{
+    if (!s || kernel < 0 || kernel > 7 || !total_ms || !launches) return M2D_ERR_ARG;
+    *total_ms = s->kt_ms[kernel];
+    *launches = s->kt_n[kernel];
+    return M2D_OK;
+}

@@ -1,0 +1,109 @@
+// msi_stencil.h -- per-point arithmetic of the 9-diagonal MSI and TDMA paths,
+// shared by the gfx950 kernels and by the CPU wavefront emulator in tests/.
+//
+// Every formula is the general 2-D stencil form: a neighbour outside the grid
+// contributes a literal zero (its du / w / z value is 0), and alpha is alpha0
+// on real volumes (2 <= i <= nx-1, 2 <= j <= ny-1, 1-based) and 0 on
+// fictitious ones.  Term order follows the reference expressions, so with
+// contraction off the results equal the reference's special-row formulas
+// bit for bit whenever every coupling towards a neighbour outside the grid is
+// exactly zero (the assumption stated at src/solvers/msi2d9-v03.f90:16-20).
+#pragma once
+
+#if defined(__HIPCC__)
+#define M2D_HD __host__ __device__ __forceinline__
+#else
+#define M2D_HD inline
+#endif
+
+namespace m2d {
+
+// Diagonal slots, 0-based: the reference's 1=SW 2=S 3=SE 4=W 5=P 6=E 7=NW 8=N 9=NE.
+enum { SW = 0, S = 1, SE = 2, W = 3, P = 4, E = 5, NW = 6, N = 7, NE = 8 };
+
+// Incomplete factorization of one point: src/solvers/msi2d9-v03.f90:356-369.
+// b[9]: row of A at P.  du_w/du_s/du_se/du_sw: U entries (E, NW, N, NE) of
+// the W, S, SE, SW neighbours (zeros when the neighbour is outside the grid).
+// Out: dl[5] (P, W, SE, S, SW) and du[4] (E, NW, N, NE) at P.
+template <typename T>
+M2D_HD void factor_point(const T *b, T alpha, const T *du_w, const T *du_s,
+                         const T *du_se, const T *du_sw, T *dl, T *du)
+{
+    const T one = T(1), two = T(2);
+    T dl5 = b[SW];
+    T dl4 = (b[S] - dl5 * du_sw[0] - alpha * b[SE] * du_se[0]) /
+            (one - alpha * du_s[0] * du_se[0]);
+    T dl3 = b[SE] - dl4 * du_s[0];
+    T dl2 = (b[W] - dl4 * du_s[1] - dl5 * du_sw[2] - two * alpha * dl5 * du_sw[1]) /
+            (one + two * alpha * du_w[1]);
+    T phi1 = dl3 * du_se[0];
+    T phi2 = dl5 * du_sw[1];
+    T phi3 = dl3 * du_se[3];
+    T phi4 = dl2 * du_w[1];
+    T dl1 = b[P] - dl2 * du_w[0] - dl3 * du_se[1] - dl4 * du_s[2] - dl5 * du_sw[3] +
+            alpha * (two * phi1 + phi2 + phi3 + two * phi4);
+    du[0] = (b[E] - dl3 * du_se[2] - dl4 * du_s[3] - two * alpha * (phi1 + phi3)) / dl1;
+    du[1] = (b[NW] - dl2 * du_w[2]) / dl1;
+    du[2] = (b[N] - dl2 * du_w[3] - alpha * phi4) / dl1;
+    du[3] = b[NE] / dl1;
+    dl[0] = dl1;
+    dl[1] = dl2;
+    dl[2] = dl3;
+    dl[3] = dl4;
+    dl[4] = dl5;
+}
+
+// Forward substitution L w = -r at one point: msi2d9-v03.f90:155.
+template <typename T>
+M2D_HD T forward_point(T r, T dl1, T dl2, T dl3, T dl4, T dl5, T w_w, T w_se, T w_s,
+                       T w_sw)
+{
+    return -(r + dl2 * w_w + dl3 * w_se + dl4 * w_s + dl5 * w_sw) / dl1;
+}
+
+// Same, multiplying by a precomputed reciprocal of dl1 (fast precisions only).
+template <typename T>
+M2D_HD T forward_point_rcp(T r, T rdl1, T dl2, T dl3, T dl4, T dl5, T w_w, T w_se,
+                           T w_s, T w_sw)
+{
+    return -(r + dl2 * w_w + dl3 * w_se + dl4 * w_s + dl5 * w_sw) * rdl1;
+}
+
+// Backward substitution U z = w at one point: msi2d9-v03.f90:175.
+template <typename T>
+M2D_HD T backward_point(T w, T du1, T du2, T du3, T du4, T z_e, T z_nw, T z_n, T z_ne)
+{
+    return w - du1 * z_e - du2 * z_nw - du3 * z_n - du4 * z_ne;
+}
+
+// Residual r = A.x - rhs at (i, j) (0-based), 2-D aware: a term whose
+// neighbour lies outside the grid is omitted, as in the seven row classes of
+// msi2d9-v03.f90:59-131.  Terms are added in slot order starting from -rhs.
+// `a` points at diagonal 0; `ld` is the diagonal stride (nx*ny).
+template <typename TA, typename TX, typename TR>
+M2D_HD TR residual_point(const TA *a, long ld, const TX *x, TR rhs, int i, int j, int nx,
+                         int ny)
+{
+    const long p = (long)j * nx + i;
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    TR acc = -rhs;
+    if (hs && hw) acc += (TR)a[SW * ld + p] * (TR)x[p - nx - 1];
+    if (hs) acc += (TR)a[S * ld + p] * (TR)x[p - nx];
+    if (hs && he) acc += (TR)a[SE * ld + p] * (TR)x[p - nx + 1];
+    if (hw) acc += (TR)a[W * ld + p] * (TR)x[p - 1];
+    acc += (TR)a[P * ld + p] * (TR)x[p];
+    if (he) acc += (TR)a[E * ld + p] * (TR)x[p + 1];
+    if (hn && hw) acc += (TR)a[NW * ld + p] * (TR)x[p + nx - 1];
+    if (hn) acc += (TR)a[N * ld + p] * (TR)x[p + nx];
+    if (hn && he) acc += (TR)a[NE * ld + p] * (TR)x[p + nx + 1];
+    return acc;
+}
+
+// alpha of msi2d9-v03.f90:287,347-351: alpha0 on real volumes, 0 on fictitious.
+template <typename T>
+M2D_HD T alpha_at(int i, int j, int nx, int ny, T alpha0)
+{
+    return (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2) ? alpha0 : T(0);
+}
+
+}  // namespace m2d

@@ -1,0 +1,127 @@
+"""ctypes binding of the C ABI in include/mach2d_hip.h (libmach2d_hip.so).
+
+The shared library is built in-tree by mach-2d_amd/build.py (hipcc for
+gfx950).  Loading fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmach2d_hip.so")
+
+M2D_OK = 0
+M2D_ERR_NONFINITE = 1
+M2D_ERR_HIP = 3
+M2D_ERR_ARG = 4
+M2D_ERR_OUTSIDE_COUPLING = 5
+M2D_ERR_SYNC_TIMEOUT = 6
+M2D_ERR_ITER_CAP = 7
+
+M2D_FP64 = 0
+M2D_FP32_IR = 1
+
+STATUS_NAMES = {
+    M2D_OK: "ok",
+    M2D_ERR_NONFINITE: "non-finite pivot or residual",
+    M2D_ERR_HIP: "HIP runtime error",
+    M2D_ERR_ARG: "invalid argument",
+    M2D_ERR_OUTSIDE_COUPLING: "coupling to a neighbour outside the grid",
+    M2D_ERR_SYNC_TIMEOUT: "inter-workgroup wait timed out",
+    M2D_ERR_ITER_CAP: "iteration safety cap reached (nitm <= 0)",
+}
+
+
+class Status(C.Structure):
+    _fields_ = [("status", C.c_int), ("nit", C.c_int), ("nitm_reached", C.c_int),
+                ("outer", C.c_int), ("r0", C.c_double), ("rn", C.c_double),
+                ("gammarp", C.c_double), ("device_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# every symbol include/mach2d_hip.h declares
+EXPORTS = [
+    "m2d_lu2d9", "m2d_fb2d9", "m2d_tdma2d9", "m2d_solve_msi9d", "m2d_norm_l1_9d",
+    "m2d_norm_l1_9d_relative", "m2d_norm_l1_b", "m2d_last_status", "m2d_create",
+    "m2d_destroy", "m2d_set_stream", "m2d_factor", "m2d_solve", "m2d_set_inner",
+    "m2d_get_factor", "m2d_set_tuning", "m2d_solve_batch", "m2d_tdma2d9_device",
+    "m2d_version", "m2d_enable_kernel_timing", "m2d_kernel_timing",
+]
+
+_lib = None
+
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+def load():
+    """Load the library (does not touch the GPU)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libmach2d_hip.so is not built (%s); run `python mach-2d_amd/build.py` "
+            "or __graft_entry__.build()" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    L.m2d_lu2d9.argtypes = [_vp, _vp, _vp, _ip, _ip, _ip]
+    L.m2d_lu2d9.restype = None
+    L.m2d_fb2d9.argtypes = [_vp, _vp, _vp, _ip, _ip, _ip, _vp, _dp, _ip, _vp]
+    L.m2d_fb2d9.restype = None
+    L.m2d_tdma2d9.argtypes = [_ip, _ip, _ip, _vp, _vp, _vp]
+    L.m2d_tdma2d9.restype = None
+    L.m2d_solve_msi9d.argtypes = [_ip, _ip, _ip, _dp, _vp, _vp, _vp]
+    L.m2d_solve_msi9d.restype = None
+    for f in (L.m2d_norm_l1_9d, L.m2d_norm_l1_9d_relative):
+        f.argtypes = [_ip, _ip, _vp, _vp, _vp, _dp]
+        f.restype = None
+    L.m2d_norm_l1_b.argtypes = [_ip, _ip, _vp]
+    L.m2d_norm_l1_b.restype = C.c_double
+    L.m2d_last_status.argtypes = [C.POINTER(Status)]
+    L.m2d_last_status.restype = C.c_int
+    L.m2d_create.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]
+    L.m2d_create.restype = C.c_int
+    L.m2d_destroy.argtypes = [_vp]
+    L.m2d_destroy.restype = C.c_int
+    L.m2d_set_stream.argtypes = [_vp, _vp]
+    L.m2d_set_stream.restype = C.c_int
+    L.m2d_factor.argtypes = [_vp, _vp]
+    L.m2d_factor.restype = C.c_int
+    L.m2d_solve.argtypes = [_vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp, C.c_int,
+                            C.POINTER(Status)]
+    L.m2d_solve.restype = C.c_int
+    L.m2d_set_inner.argtypes = [_vp, C.c_double, C.c_int]
+    L.m2d_set_inner.restype = C.c_int
+    L.m2d_get_factor.argtypes = [_vp, _vp, _vp]
+    L.m2d_get_factor.restype = C.c_int
+    L.m2d_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
+    L.m2d_set_tuning.restype = C.c_int
+    L.m2d_solve_batch.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(_vp), C.POINTER(_vp),
+                                  C.POINTER(_vp), C.c_double, C.c_int, C.POINTER(Status)]
+    L.m2d_solve_batch.restype = C.c_int
+    L.m2d_tdma2d9_device.argtypes = [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]
+    L.m2d_tdma2d9_device.restype = C.c_int
+    L.m2d_enable_kernel_timing.argtypes = [_vp, C.c_int]
+    L.m2d_enable_kernel_timing.restype = C.c_int
+    L.m2d_kernel_timing.argtypes = [_vp, C.c_int, _dp, C.POINTER(C.c_long)]
+    L.m2d_kernel_timing.restype = C.c_int
+    L.m2d_version.argtypes = []
+    L.m2d_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def last_status() -> Status:
+    st = Status()
+    load().m2d_last_status(C.byref(st))
+    return st
+
+
+class SolverError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        super().__init__("%s: status %d (%s)" % (where, code, STATUS_NAMES.get(code, "?")))

@@ -1,0 +1,277 @@
+"""Synthetic 9-diagonal systems of the MACH-2D pressure-correction kind.
+
+The reference assembles its p' system as a 5-point stencil; the workloads
+here use the full 9-point curvilinear diffusion stencil instead, built from
+the same grid metrics, so that every MSI diagonal is exercised.
+
+Grid: the reference's g01 power-law ogive (src/user/external_flow/
+mod_extflow_geometry.f90:178-292 with mach2d_input/geometry_g01.txt): the
+south boundary is the body y = rb (x/lr)^lbd, the north boundary an ellipse
+of semi-axes la, lb; points along both follow power laws of exponent 1.5.
+Interior grid lines are straight segments from each south point to the
+north point of the same index, with node spacing in geometric progression
+of ratio q away from the wall (q chosen to reach a target near-wall aspect
+ratio for the wall-resolved grids).
+
+Metrics: src/grid/mod_grid.f90:304-558 (get_metrics), centroids by the
+simple mean of the corners (mod_grid.f90:236-272, opt = 1), restated with
+numpy.
+
+Coefficients: the Navier-Stokes diffusion part of get_u_coefficients
+(src/equations/mach2d_coef.f90:160-197) with vle = vln = re = rn = 1; then
+off-diagonals are forced to -|a_nb| and a_P = 1.001 * sum |a_nb| (an
+M-matrix, diagonally dominant).  Boundary rows follow intflow_set_bcpl
+(src/user/internal_flow/mod_intflow.f90:410-457) mapped to 9 slots:
+west P=1,E=1; east P=1,W=-1; south P=1,N=-1; north P=1,S=-1; corners P=1.
+Every coupling towards a neighbour outside the grid is exactly zero.
+
+Arrays are numpy float64, diagonal-major (9, nx*ny), point index
+j*nx + i (0-based; the reference's np = nx*(j-1)+i).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+# g01 parameters, mach2d_input/geometry_g01.txt
+G01 = dict(la=8.382e-2, lb=5.080e-2, lr=7.620e-2, rb=1.270e-2, lbd=1.0, akn=1.5, aks=1.5)
+
+
+@dataclass
+class Grid:
+    nx: int
+    ny: int
+    x: np.ndarray   # (ny, nx) coordinates of the north-east corner of volume (i, j)
+    y: np.ndarray
+    q: float
+
+
+def gp_fractions(m: int, q: float) -> np.ndarray:
+    """m+1 fractions 0..1 with interval lengths in geometric progression q."""
+    k = np.arange(m + 1, dtype=np.float64)
+    if abs(q - 1.0) < 1e-14:
+        return k / m
+    return (q ** k - 1.0) / (q ** m - 1.0)
+
+
+def ogive_grid(nx: int, ny: int, q: float = 1.05, geom=None) -> Grid:
+    g = dict(G01 if geom is None else geom)
+    x = np.zeros((ny, nx))
+    y = np.zeros((ny, nx))
+    t = np.arange(nx - 1, dtype=np.float64) / (nx - 2)
+    # south: body; north: ellipse (mod_extflow_geometry.f90:259-284)
+    xs = g["lr"] * t ** g["aks"]
+    ys = g["rb"] * (xs / g["lr"]) ** g["lbd"]
+    xn = g["la"] * t ** g["akn"] - (g["la"] - g["lr"])
+    yn = g["lb"] * np.sqrt(np.maximum(1.0 - ((xn - g["lr"]) / g["la"]) ** 2, 0.0))
+    f = gp_fractions(ny - 2, q)              # rows 0 .. ny-2 of corners
+    x[: ny - 1, : nx - 1] = xs[None, :] + f[:, None] * (xn - xs)[None, :]
+    y[: ny - 1, : nx - 1] = ys[None, :] + f[:, None] * (yn - ys)[None, :]
+    return Grid(nx, ny, x, y, q)
+
+
+def ratio_for_aspect(nx: int, ny: int, aspect: float, geom=None) -> float:
+    """Geometric ratio q giving a maximum near-wall cell aspect ratio
+    (tangential / wall-normal size) of about `aspect` (bisection)."""
+    lo, hi = 1.0 + 1e-9, 1.5
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        g = ogive_grid(nx, ny, mid, geom)
+        if max_wall_aspect(g) < aspect:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def max_wall_aspect(g: Grid) -> float:
+    x, y = g.x, g.y
+    dxi = np.hypot(x[0, 2: g.nx - 1] - x[0, 1: g.nx - 2], y[0, 2: g.nx - 1] - y[0, 1: g.nx - 2])
+    deta = np.hypot(x[1, 2: g.nx - 1] - x[0, 2: g.nx - 1], y[1, 2: g.nx - 1] - y[0, 2: g.nx - 1])
+    return float(np.max(dxi / np.maximum(deta, 1e-300)))
+
+
+def metrics(g: Grid) -> dict:
+    """mod_grid.f90:304-558 restated (0-based (j, i) arrays of shape (ny, nx))."""
+    nx, ny, x, y = g.nx, g.ny, g.x, g.y
+    z = lambda: np.zeros((ny, nx))
+    xp, yp = z(), z()
+    # centroids of real volumes, simple mean (mod_grid.f90:258-271)
+    xp[1:-1, 1:-1] = (x[1:-1, 1:-1] + x[1:-1, :-2] + x[:-2, :-2] + x[:-2, 1:-1]) / 4.0
+    yp[1:-1, 1:-1] = (y[1:-1, 1:-1] + y[1:-1, :-2] + y[:-2, :-2] + y[:-2, 1:-1]) / 4.0
+    xe, ye, xen, yen, xk, yk, xke, yke = z(), z(), z(), z(), z(), z(), z(), z()
+    Jp, Je, Jn, alphae, gamman, betae, betan = z(), z(), z(), z(), z(), z(), z()
+    # eta-derivatives at east faces, j = 2..ny-1, i = 1..nx-1 (1-based)
+    xe[1:-1, :-1] = x[1:-1, :-1] - x[:-2, :-1]
+    ye[1:-1, :-1] = y[1:-1, :-1] - y[:-2, :-1]
+    alphae[1:-1, :-1] = xe[1:-1, :-1] ** 2 + ye[1:-1, :-1] ** 2
+    # csi-derivatives at north faces, j = 1..ny-1, i = 2..nx-1
+    xk[:-1, 1:-1] = x[:-1, 1:-1] - x[:-1, :-2]
+    yk[:-1, 1:-1] = y[:-1, 1:-1] - y[:-1, :-2]
+    gamman[:-1, 1:-1] = xk[:-1, 1:-1] ** 2 + yk[:-1, 1:-1] ** 2
+    # csi-derivatives at inner east faces, i = 2..nx-2
+    xke[1:-1, 1:-2] = xp[1:-1, 2:-1] - xp[1:-1, 1:-2]
+    yke[1:-1, 1:-2] = yp[1:-1, 2:-1] - yp[1:-1, 1:-2]
+    # west boundary face (stored at i = 1), one-sided
+    fw = (x[1:-1, 0] + x[:-2, 0]) / 2.0
+    fe = (x[1:-1, 1] + x[:-2, 1]) / 2.0
+    xke[1:-1, 0] = -3.0 * fw + 4.0 * xp[1:-1, 1] - fe
+    fw = (y[1:-1, 0] + y[:-2, 0]) / 2.0
+    fe = (y[1:-1, 1] + y[:-2, 1]) / 2.0
+    yke[1:-1, 0] = -3.0 * fw + 4.0 * yp[1:-1, 1] - fe
+    # east boundary face (i = nx-1)
+    fw = (x[1:-1, nx - 3] + x[:-2, nx - 3]) / 2.0
+    fe = (x[1:-1, nx - 2] + x[:-2, nx - 2]) / 2.0
+    xke[1:-1, nx - 2] = fw - 4.0 * xp[1:-1, nx - 2] + 3.0 * fe
+    fw = (y[1:-1, nx - 3] + y[:-2, nx - 3]) / 2.0
+    fe = (y[1:-1, nx - 2] + y[:-2, nx - 2]) / 2.0
+    yke[1:-1, nx - 2] = fw - 4.0 * yp[1:-1, nx - 2] + 3.0 * fe
+    # eta-derivatives at inner north faces, j = 2..ny-2
+    xen[1:-2, 1:-1] = xp[2:-1, 1:-1] - xp[1:-2, 1:-1]
+    yen[1:-2, 1:-1] = yp[2:-1, 1:-1] - yp[1:-2, 1:-1]
+    # south boundary face (stored at j = 1)
+    fs = (x[0, 1:-1] + x[0, :-2]) / 2.0
+    fn = (x[1, 1:-1] + x[1, :-2]) / 2.0
+    xen[0, 1:-1] = -3.0 * fs + 4.0 * xp[1, 1:-1] - fn
+    fs = (y[0, 1:-1] + y[0, :-2]) / 2.0
+    fn = (y[1, 1:-1] + y[1, :-2]) / 2.0
+    yen[0, 1:-1] = -3.0 * fs + 4.0 * yp[1, 1:-1] - fn
+    # north boundary face (j = ny-1)
+    fs = (x[ny - 3, 1:-1] + x[ny - 3, :-2]) / 2.0
+    fn = (x[ny - 2, 1:-1] + x[ny - 2, :-2]) / 2.0
+    xen[ny - 2, 1:-1] = fs - 4.0 * xp[ny - 2, 1:-1] + 3.0 * fn
+    fs = (y[ny - 3, 1:-1] + y[ny - 3, :-2]) / 2.0
+    fn = (y[ny - 2, 1:-1] + y[ny - 2, :-2]) / 2.0
+    yen[ny - 2, 1:-1] = fs - 4.0 * yp[ny - 2, 1:-1] + 3.0 * fn
+    # beta, J at east faces (j = 2..ny-1, i = 1..nx-1) and north faces
+    sl = (slice(1, -1), slice(0, -1))
+    betae[sl] = xke[sl] * xe[sl] + yke[sl] * ye[sl]
+    Je[sl] = 1.0 / (xke[sl] * ye[sl] - xe[sl] * yke[sl])
+    sl = (slice(0, -1), slice(1, -1))
+    betan[sl] = xk[sl] * xen[sl] + yk[sl] * yen[sl]
+    Jn[sl] = 1.0 / (xk[sl] * yen[sl] - xen[sl] * yk[sl])
+    # J at volume centres
+    fw = (x[1:-1, :-2] + x[:-2, :-2]) / 2.0
+    fe = (x[1:-1, 1:-1] + x[:-2, 1:-1]) / 2.0
+    fs = (x[:-2, 1:-1] + x[:-2, :-2]) / 2.0
+    fn = (x[1:-1, 1:-1] + x[1:-1, :-2]) / 2.0
+    xkp, xep = fe - fw, fn - fs
+    fw = (y[1:-1, :-2] + y[:-2, :-2]) / 2.0
+    fe = (y[1:-1, 1:-1] + y[:-2, 1:-1]) / 2.0
+    fs = (y[:-2, 1:-1] + y[:-2, :-2]) / 2.0
+    fn = (y[1:-1, 1:-1] + y[1:-1, :-2]) / 2.0
+    ykp, yep = fe - fw, fn - fs
+    Jp[1:-1, 1:-1] = 1.0 / (xkp * yep - xep * ykp)
+    Je, Jn, Jp = np.abs(Je), np.abs(Jn), np.abs(Jp)
+    return dict(xp=xp, yp=yp, xe=xe, ye=ye, xen=xen, yen=yen, xk=xk, yk=yk, xke=xke, yke=yke,
+                Jp=Jp, Je=Je, Jn=Jn, alphae=alphae, gamman=gamman, betae=betae, betan=betan)
+
+
+def diffusion_coefficients(nx: int, ny: int, m: dict, dominance: float = 1.001) -> np.ndarray:
+    """9-point p' coefficients (9, nx*ny), see the module docstring."""
+    a = np.zeros((9, ny, nx))
+    Je, Jn = m["Je"], m["Jn"]
+    ce = Je * m["betae"]      # vle*re*Je*betae with vle = re = 1
+    cn = Jn * m["betan"]
+    ge = Je * m["alphae"]
+    gn = Jn * m["gamman"]
+    J, I = slice(1, ny - 1), slice(1, nx - 1)
+    Jm, Im = slice(0, ny - 2), slice(0, nx - 2)   # south / west neighbours
+    # mach2d_coef.f90:160-193 (advection terms dropped)
+    a[0, J, I] = (ce[J, Im] + cn[Jm, I]) / 4.0
+    a[1, J, I] = -gn[Jm, I] + (ce[J, Im] - ce[J, I]) / 4.0
+    a[2, J, I] = -(ce[J, I] + cn[Jm, I]) / 4.0
+    a[3, J, I] = -ge[J, Im] + (cn[Jm, I] - cn[J, I]) / 4.0
+    a[5, J, I] = -ge[J, I] + (-cn[Jm, I] + cn[J, I]) / 4.0
+    a[6, J, I] = (-ce[J, Im] - cn[J, I]) / 4.0
+    a[7, J, I] = -gn[J, I] - (ce[J, Im] - ce[J, I]) / 4.0
+    a[8, J, I] = (ce[J, I] + cn[J, I]) / 4.0
+    nb = [0, 1, 2, 3, 5, 6, 7, 8]
+    a[nb, J, I] = -np.abs(a[nb, J, I])
+    a[4, J, I] = dominance * np.sum(np.abs(a[nb, J, I]), axis=0)
+    # boundary rows (intflow_set_bcpl pattern), corners identity
+    a[:, :, 0] = 0.0
+    a[4, :, 0] = 1.0
+    a[5, 1:-1, 0] = 1.0                 # west: P + E
+    a[:, :, nx - 1] = 0.0
+    a[4, :, nx - 1] = 1.0
+    a[3, 1:-1, nx - 1] = -1.0           # east: P - W
+    a[:, 0, 1:-1] = 0.0
+    a[4, 0, 1:-1] = 1.0
+    a[7, 0, 1:-1] = -1.0                # south: P - N
+    a[:, ny - 1, 1:-1] = 0.0
+    a[4, ny - 1, 1:-1] = 1.0
+    a[1, ny - 1, 1:-1] = -1.0           # north: P - S
+    return a.reshape(9, nx * ny)
+
+
+def outside_mask(nx: int, ny: int) -> np.ndarray:
+    """(9, nx*ny) bool: slot couples to a neighbour outside the grid."""
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    w, e, s, n = i == 0, i == nx - 1, j == 0, j == ny - 1
+    m = np.stack([s | w, s, s | e, w, np.zeros_like(w), e, n | w, n, n | e])
+    return m.reshape(9, nx * ny)
+
+
+@dataclass
+class System:
+    nx: int
+    ny: int
+    a: np.ndarray      # (9, N)
+    rhs: np.ndarray    # (N,)
+    q: float
+    seed: int
+    metrics: dict | None = None
+
+    @property
+    def N(self) -> int:
+        return self.nx * self.ny
+
+
+def pcorr_system(nx: int, ny: int, seed: int = 0, q: float | None = None,
+                 aspect: float | None = None, rhs: str = "uniform",
+                 keep_metrics: bool = False) -> System:
+    """A seeded 9-diagonal p' system on the ogive grid.  rhs is drawn on
+    interior rows only ('uniform' on [-1,1] or 'gauss' N(0,1)); fictitious
+    rows get 0."""
+    if q is None:
+        q = ratio_for_aspect(nx, ny, aspect) if aspect else 1.05
+    g = ogive_grid(nx, ny, q)
+    m = metrics(g)
+    a = diffusion_coefficients(nx, ny, m)
+    rng = np.random.default_rng(seed)
+    b = np.zeros((ny, nx))
+    if rhs == "uniform":
+        b[1:-1, 1:-1] = rng.uniform(-1.0, 1.0, size=(ny - 2, nx - 2))
+    elif rhs == "gauss":
+        b[1:-1, 1:-1] = rng.standard_normal(size=(ny - 2, nx - 2))
+    else:
+        raise ValueError(rhs)
+    return System(nx, ny, np.ascontiguousarray(a), b.reshape(-1).copy(), q, seed,
+                  m if keep_metrics else None)
+
+
+def random_f3_system(nx: int, ny: int, seed: int = 0, dominance: float = 1.5) -> System:
+    """Random diagonally dominant 9-diagonal system honouring the zero
+    outside-coupling invariant (for tests on odd shapes)."""
+    rng = np.random.default_rng(seed)
+    a = -rng.uniform(0.05, 1.0, size=(9, nx * ny))
+    a[outside_mask(nx, ny)] = 0.0
+    nb = [0, 1, 2, 3, 5, 6, 7, 8]
+    a[4] = dominance * np.sum(np.abs(a[nb]), axis=0) + 1e-3
+    rhs = rng.uniform(-1.0, 1.0, size=nx * ny)
+    return System(nx, ny, np.ascontiguousarray(a), rhs, 1.0, seed)
+
+
+# Workload definitions -------------------------------------------------------
+CFG1 = dict(nx=130, ny=66, q=1.05, rhs="uniform", seed=1, gamma=1e-10, nitm=500)
+CFG3 = dict(nx=1026, ny=514, aspect=1e4, rhs="gauss", seed=3, gamma=1e-10)
+
+
+def cfg4_sizes(count: int = 256, seed: int = 4):
+    """Real sizes 64s x 32s with s log-uniform on [1, 8], plus fictitious rows."""
+    rng = np.random.default_rng(seed)
+    s = np.exp(rng.uniform(0.0, math.log(8.0), size=count))
+    return [(int(round(64 * v)) + 2, int(round(32 * v)) + 2) for v in s]

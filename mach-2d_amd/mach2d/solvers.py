@@ -1,0 +1,343 @@
+"""Host-side mirror of the reference's linear-solver plugin interface,
+backed by the gfx950 C ABI (include/mach2d_hip.h).
+
+Reference interface (src/solvers/):
+  * class_solver_abstract%solve(a, b, sol)       mod_solver_abstract.f90:12-31
+  * class_solver_msi9d%init(nx, ny, nitm, tol)   mod_solver_msi.f90:129-148
+    class_solver_msi9d%solve(a, b, sol)          mod_solver_msi.f90:152-166
+  * class_solver_msi5d%init / solve              mod_solver_msi.f90:89-126
+  * class_solver_tdma9d%init(nx, ny, nitm)       mod_solver_tdma.f90:244-268
+  * class_solver_tdma5d%init / solve             mod_solver_tdma.f90:216-240
+  * solvers_init(nx, ny, ifile) -> (solver5d, solver9d)   mod_solvers.f90:17-87
+  * norm_l1_9d / norm_l1_9d_relative / norm_l1_b          mod_solvers.f90:128-276
+  * lu2d9, fb2d9 (msi2d9-v03.f90:277, :42), tdma2d9 (tdma2d9.f90:38)
+
+Arrays: a coefficient array for an nx*ny grid is either a C-contiguous
+(9, nx*ny) float64 array or the Fortran-ordered (nx*ny, 9) array of the
+reference (the same memory).  `sol`/`var`/`x` are updated in place, like the
+reference's intent(inout) dummies.  numpy arrays take the synchronous
+host-array entry points; torch tensors on the GPU take the device-resident
+handle API without host copies.
+
+The 5-diagonal solvers embed the 5-point system (slots S, W, P, E, N) into
+the 9-diagonal slots 2, 4, 5, 6, 8 with zero corners: lu2d9/fb2d9 and
+tdma2d9 on that embedding perform the 5-point reference's operations plus
+exact zero terms, so they reproduce lu2d5/fb2d5 and tdma2d5.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import SolverError, Status, last_status
+
+# 5-point slots (S, W, P, E, N) -> 9-point slots, 0-based
+EMBED_5_IN_9 = (1, 3, 4, 5, 7)
+
+
+def _coef(a, n, k):
+    """Return a C-contiguous (k, n) float64 view/copy of a coefficient array."""
+    a = np.asarray(a)
+    if a.shape == (k, n):
+        return np.ascontiguousarray(a, dtype=np.float64)
+    if a.shape == (n, k):           # Fortran a(n, k): the transpose is the memory image
+        return np.ascontiguousarray(a.T, dtype=np.float64)
+    raise ValueError("coefficient array must have shape (%d, %d) or (%d, %d), got %s"
+                     % (k, n, n, k, a.shape))
+
+
+def _vec(v, n, name):
+    if not isinstance(v, np.ndarray) or v.dtype != np.float64 or not v.flags.c_contiguous:
+        raise TypeError("%s must be a C-contiguous float64 numpy array" % name)
+    if v.size != n:
+        raise ValueError("%s has %d entries, expected %d" % (name, v.size, n))
+    return v
+
+
+def _p(arr):
+    return arr.ctypes.data_as(C.c_void_p)
+
+
+def _i(v):
+    return C.byref(C.c_int(int(v)))
+
+
+def _d(v):
+    return C.byref(C.c_double(float(v)))
+
+
+def embed5(a5, n):
+    """(5, n) or (n, 5) 5-point coefficients -> (9, n) with zero corners."""
+    a5 = _coef(a5, n, 5)
+    a9 = np.zeros((9, n))
+    a9[list(EMBED_5_IN_9)] = a5
+    return a9
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+# ---------------------------------------------------------------------------
+# reference-signature functions
+# ---------------------------------------------------------------------------
+def lu2d9(b, dl, du, nij, ni, nj):
+    """msi2d9-v03.f90:277 lu2d9(b, dl, du, nij, ni, nj); dl (5,nij), du (4,nij) out."""
+    bb = _coef(b, nij, 9)
+    _vec(dl, 5 * nij, "dl")
+    _vec(du, 4 * nij, "du")
+    _lib.load().m2d_lu2d9(_p(bb), _p(dl), _p(du), _i(nij), _i(ni), _i(nj))
+    return last_status()
+
+
+def fb2d9(b, dl, du, nj, ni, nij, var, gamma, nitm, rhs):
+    """msi2d9-v03.f90:42 fb2d9(b, dl, du, nj, ni, nij, var, gamma, nitm, rhs); var inout."""
+    bb = _coef(b, nij, 9)
+    dl = _coef(dl, nij, 5)
+    du = _coef(du, nij, 4)
+    _vec(var, nij, "var")
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    _lib.load().m2d_fb2d9(_p(bb), _p(dl), _p(du), _i(nj), _i(ni), _i(nij), _p(var),
+                          _d(gamma), _i(nitm), _p(rhs))
+    return last_status()
+
+
+def tdma2d9(nx, ny, nitm, a, b, x):
+    """tdma2d9.f90:38 tdma2d9(nx, ny, nitm, a, b, x); x inout."""
+    n = nx * ny
+    aa = _coef(a, n, 9)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    _vec(x, n, "x")
+    _lib.load().m2d_tdma2d9(_i(nx), _i(ny), _i(nitm), _p(aa), _p(b), _p(x))
+    return last_status()
+
+
+def norm_l1_9d(nx, ny, var, b, a, norm=0.0):
+    """mod_solvers.f90:128; returns norm + sum over real volumes |A.var - b|."""
+    v = C.c_double(norm)
+    n = nx * ny
+    aa = _coef(a, n, 9)
+    _lib.load().m2d_norm_l1_9d(_i(nx), _i(ny), _p(np.ascontiguousarray(var, np.float64)),
+                               _p(np.ascontiguousarray(b, np.float64)), _p(aa), C.byref(v))
+    _check_last("norm_l1_9d")
+    return v.value
+
+
+def norm_l1_9d_relative(nx, ny, var, b, a, norm=0.0):
+    """mod_solvers.f90:176; relative to ||b||_1 when ||b||_1 > 10 eps."""
+    v = C.c_double(norm)
+    n = nx * ny
+    aa = _coef(a, n, 9)
+    _lib.load().m2d_norm_l1_9d_relative(_i(nx), _i(ny),
+                                        _p(np.ascontiguousarray(var, np.float64)),
+                                        _p(np.ascontiguousarray(b, np.float64)), _p(aa),
+                                        C.byref(v))
+    _check_last("norm_l1_9d_relative")
+    return v.value
+
+
+def norm_l1_b(nx, ny, b):
+    """mod_solvers.f90:250."""
+    r = _lib.load().m2d_norm_l1_b(_i(nx), _i(ny), _p(np.ascontiguousarray(b, np.float64)))
+    _check_last("norm_l1_b")
+    return r
+
+
+def _check_last(where):
+    st = last_status()
+    if st.status not in (_lib.M2D_OK,):
+        raise SolverError(st.status, where)
+    return st
+
+
+# ---------------------------------------------------------------------------
+# plugin classes
+# ---------------------------------------------------------------------------
+class ClassSolverAbstract:
+    """mod_solver_abstract.f90:12-31: solve(a, b, sol) with sol inout."""
+
+    def solve(self, a, b, sol):  # pragma: no cover - interface
+        raise NotImplementedError
+
+
+class ClassSolverMSI9d(ClassSolverAbstract):
+    """MSI for 9-diagonal systems (mod_solver_msi.f90:68-166).  Every solve
+    factors then iterates, as the reference (lu2d9 + fb2d9)."""
+
+    def init(self, nx, ny, nitm, tol):
+        self.nx, self.ny, self.nxy, self.nitm, self.tol = nx, ny, nx * ny, nitm, tol
+        self._dev = None
+        return self
+
+    def solve(self, a, b, sol):
+        if _is_torch(sol):
+            return self._solve_device(a, b, sol)
+        aa = _coef(a, self.nxy, 9)
+        _vec(sol, self.nxy, "sol")
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        _lib.load().m2d_solve_msi9d(_i(self.nx), _i(self.ny), _i(self.nitm), _d(self.tol),
+                                    _p(aa), _p(b), _p(sol))
+        self.status = last_status()
+        return self.status
+
+    def _solve_device(self, a, b, sol):
+        if self._dev is None:
+            self._dev = DeviceMSI(self.nx, self.ny)
+        self._dev.factor(a)
+        self.status = self._dev.solve(a, b, sol, self.tol, self.nitm)
+        return self.status
+
+
+class ClassSolverMSI5d(ClassSolverAbstract):
+    """MSI for 5-diagonal systems (mod_solver_msi.f90:50-126) through the
+    zero-corner embedding into the 9-diagonal solver."""
+
+    def init(self, nx, ny, nitm, tol):
+        self.inner = ClassSolverMSI9d().init(nx, ny, nitm, tol)
+        self.nxy = nx * ny
+        return self
+
+    def solve(self, a, b, sol):
+        self.status = self.inner.solve(embed5(a, self.nxy), b, sol)
+        return self.status
+
+
+class ClassSolverTDMA9d(ClassSolverAbstract):
+    """mod_solver_tdma.f90:199-268: nitm alternating line sweeps."""
+
+    def init(self, nx, ny, nitm):
+        self.nx, self.ny, self.nitm = nx, ny, nitm
+        return self
+
+    def solve(self, a, b, sol):
+        if _is_torch(sol):
+            rc = _lib.load().m2d_tdma2d9_device(self.nx, self.ny, self.nitm, a.data_ptr(),
+                                                b.data_ptr(), sol.data_ptr(), None)
+            if rc:
+                raise SolverError(rc, "tdma2d9")
+            return rc
+        self.status = tdma2d9(self.nx, self.ny, self.nitm, a, b, sol)
+        return self.status
+
+
+class ClassSolverTDMA5d(ClassSolverAbstract):
+    """mod_solver_tdma.f90:184-240 through the zero-corner embedding."""
+
+    def init(self, nx, ny, nitm):
+        self.inner = ClassSolverTDMA9d().init(nx, ny, nitm)
+        self.nxy = nx * ny
+        return self
+
+    def solve(self, a, b, sol):
+        return self.inner.solve(embed5(a, self.nxy), b, sol)
+
+
+def solvers_init(nx, ny, ifile):
+    """mod_solvers.f90:17-87.  `ifile` maps the keys solver, nitm_u, nitm_p,
+    tol_u, tol_p (a dict, or anything with get_value(name)).  Returns
+    (solver5d, solver9d).  An unknown solver name stops, as the reference
+    does (here: raises SystemExit)."""
+    get = ifile.get_value if hasattr(ifile, "get_value") else ifile.__getitem__
+    ksolver = str(get("solver")).strip()
+    nitm_u, nitm_p = int(get("nitm_u")), int(get("nitm_p"))
+    tol_u, tol_p = float(get("tol_u")), float(get("tol_p"))
+    if ksolver == "MSI":
+        return (ClassSolverMSI5d().init(nx, ny, nitm_p, tol_p),
+                ClassSolverMSI9d().init(nx, ny, nitm_u, tol_u))
+    if ksolver == "TDMA":
+        return (ClassSolverTDMA5d().init(nx, ny, nitm_p),
+                ClassSolverTDMA9d().init(nx, ny, nitm_u))
+    raise SystemExit("solvers_init:\nUnkown option: %s\nStopping..." % ksolver)
+
+
+# ---------------------------------------------------------------------------
+# device-resident handle (torch tensors on the GPU)
+# ---------------------------------------------------------------------------
+class DeviceMSI:
+    """One solver handle for an nx*ny grid on the current GPU.
+
+    precision: "fp64" (parity mode) or "fp32_ir" (FP32 factor and sweeps
+    inside FP64 iterative refinement)."""
+
+    def __init__(self, nx, ny, precision="fp64", stream=None):
+        L = _lib.load()
+        self.nx, self.ny, self.N = nx, ny, nx * ny
+        self.precision = precision
+        prec = {"fp64": _lib.M2D_FP64, "fp32_ir": _lib.M2D_FP32_IR}[precision]
+        h = C.c_void_p()
+        rc = L.m2d_create(nx, ny, prec, C.byref(h))
+        if rc:
+            raise SolverError(rc, "m2d_create")
+        self.h = h
+        if stream is not None:
+            L.m2d_set_stream(self.h, C.c_void_p(stream))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().m2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, t, numel, name):
+        import torch
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
+                and t.is_contiguous() and t.numel() == numel):
+            raise TypeError("%s must be a contiguous float64 CUDA tensor of %d values"
+                            % (name, numel))
+        return t.data_ptr()
+
+    def set_tuning(self, publish_every=32, xcd_stride=1):
+        rc = _lib.load().m2d_set_tuning(self.h, publish_every, xcd_stride)
+        if rc:
+            raise SolverError(rc, "m2d_set_tuning")
+
+    KERNELS = {"factor": 0, "forward": 1, "backward": 2, "residual": 3,
+               "factor32": 4, "forward32": 5, "backward32": 6, "residual32": 7}
+
+    def enable_kernel_timing(self, on=True):
+        _lib.load().m2d_enable_kernel_timing(self.h, 1 if on else 0)
+
+    def kernel_timing(self):
+        """{kernel: (total_ms, launches)} of the launches timed so far."""
+        out = {}
+        for name, kid in self.KERNELS.items():
+            ms, n = C.c_double(), C.c_long()
+            _lib.load().m2d_kernel_timing(self.h, kid, C.byref(ms), C.byref(n))
+            if n.value:
+                out[name] = (ms.value, n.value)
+        return out
+
+    def set_inner(self, inner_gamma=1e-4, inner_nitm=200):
+        rc = _lib.load().m2d_set_inner(self.h, inner_gamma, inner_nitm)
+        if rc:
+            raise SolverError(rc, "m2d_set_inner")
+
+    def factor(self, a):
+        rc = _lib.load().m2d_factor(self.h, C.c_void_p(self._chk(a, 9 * self.N, "a")))
+        if rc and rc != _lib.M2D_ERR_NONFINITE:
+            raise SolverError(rc, "m2d_factor")
+        return rc
+
+    def get_factor(self, dl, du):
+        rc = _lib.load().m2d_get_factor(self.h, C.c_void_p(self._chk(dl, 5 * self.N, "dl")),
+                                        C.c_void_p(self._chk(du, 4 * self.N, "du")))
+        if rc:
+            raise SolverError(rc, "m2d_get_factor")
+
+    def solve(self, a, rhs, var, gamma, nitm, hist=None) -> Status:
+        st = Status()
+        hp, cap = (None, 0)
+        if hist is not None:
+            hp, cap = C.c_void_p(self._chk(hist, hist.numel(), "hist")), hist.numel()
+        _lib.load().m2d_solve(self.h, C.c_void_p(self._chk(a, 9 * self.N, "a")),
+                              C.c_void_p(self._chk(rhs, self.N, "rhs")),
+                              C.c_void_p(self._chk(var, self.N, "var")), float(gamma),
+                              int(nitm), hp, cap, C.byref(st))
+        return st

@@ -1,0 +1,67 @@
+/*
+ * oracle.h -- CPU restatement of the MACH-2D SIMPLEC linear-algebra hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP product path
+ * is compared against.  Only tests, the smoke check and the CPU-baseline leg
+ * of the benchmark may load it; the product path never links or calls it.
+ *
+ * Every routine restates one reference subroutine, keeping its argument
+ * meaning, its special-case rows and its floating-point operation order, so
+ * that a build with -ffp-contract=off reproduces the reference bit for bit.
+ * Arrays are the reference's column-major Fortran arrays seen from C:
+ * a(nij,9) is `double a[9][nij]`, diagonal k (1-based) at a + (k-1)*nij.
+ * Point index np = nx*(j-1)+i (1-based, i fastest).
+ *
+ * 9-diagonal slots: 1=SW 2=S 3=SE 4=W 5=P 6=E 7=NW 8=N 9=NE.
+ * 5-diagonal slots: 1=S 2=W 3=P 4=E 5=N.
+ */
+#ifndef MACH2D_ORACLE_H
+#define MACH2D_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/solvers/msi2d9-v03.f90:277-432 (lu2d9).  alpha0 is 0.5 in the
+ * reference (:287); it is a parameter here so tests can force alpha = 0. */
+void orc_lu2d9(const double *b, double *dl, double *du, int nij, int ni,
+               int nj, double alpha0);
+
+/* src/solvers/msi2d9-v03.f90:42-262 (fb2d9).  Extra outputs, which the
+ * reference does not return: *nit_out (iterations run), *r0_out (||r0||_2),
+ * hist[nit-1] = ||r_nit||_2 (hist may be NULL; hist_cap bounds it).
+ * max_iter_cap guards nitm <= 0, which never exits on count in the
+ * reference (:259); pass 0 for no guard. */
+void orc_fb2d9(const double *b, const double *dl, const double *du, int nj,
+               int ni, int nij, double *var, double gamma, int nitm,
+               const double *rhs, int *nit_out, double *r0_out, double *hist,
+               int hist_cap, int max_iter_cap);
+
+/* src/solvers/msi2d5-v03.f90:247-330 (lu2d5) and :42-233 (fb2d5). */
+void orc_lu2d5(const double *b, double *dl, double *du, int nij, int ni,
+               int nj, double alpha0);
+void orc_fb2d5(const double *b, const double *dl, const double *du, int nj,
+               int ni, int nij, double *var, double gamma, int nitm,
+               const double *rhs, int *nit_out, double *r0_out, double *hist,
+               int hist_cap, int max_iter_cap);
+
+/* src/solvers/tdma2d9.f90:38-322 (tdma2d9); line-Jacobi, serial here
+ * (the reference's OpenMP split over lines does not change results). */
+void orc_tdma2d9(int nx, int ny, int nitm, const double *a, const double *b,
+                 double *x);
+
+/* src/solvers/mod_solvers.f90:128-170, 176-245, 250-276. */
+void orc_norm_l1_9d(int nx, int ny, const double *var, const double *b,
+                    const double *a, double *norm);
+void orc_norm_l1_9d_relative(int nx, int ny, const double *var,
+                             const double *b, const double *a, double *norm);
+double orc_norm_l1_b(int nx, int ny, const double *b);
+
+/* r = A.var - rhs over all nij rows, the residual of fb2d9 (:59-131). */
+void orc_residual9(const double *b, int ni, int nj, const double *var,
+                   const double *rhs, double *r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

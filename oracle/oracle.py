@@ -1,0 +1,279 @@
+"""ctypes bindings for the oracle (TEST INFRASTRUCTURE ONLY).
+
+Two libraries live in this directory:
+
+* ``liboracle.so``   -- the C restatement of the reference hot path
+                        (msi2d9.c, msi2d5.c, tdma2d9.c, coef.c);
+* ``_ref/libm2dref.so`` -- the reference's own Fortran sources compiled in
+                        place by ``make ref`` (only where the reference tree is
+                        present; the compiled file travels with the repo).
+
+Only tests, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this module.  The product path (the HIP library and
+its Python host mirror) never does.
+
+All arrays are numpy float64, diagonal-major: a coefficient array for an
+nx*ny grid has shape (9, nx*ny) (or (5, nx*ny)), which is the memory image of
+the reference's column-major ``a(nx*ny, 9)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libm2dref.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = C.POINTER(C.c_int)
+_dptr = C.POINTER(C.c_double)
+
+_lib = None
+_ref = None
+
+
+def build():
+    """Compile liboracle.so (and the reference library when the reference
+    sources are present)."""
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", HERE, "liboracle.so"])
+    if os.path.isdir(os.environ.get("MACH2D_REFERENCE", "/root/reference")):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = C.CDLL(ORACLE_SO)
+        L.orc_lu2d9.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.orc_lu2d5.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double]
+        for f in (L.orc_fb2d9, L.orc_fb2d5):
+            f.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_double,
+                          C.c_int, _dp, _ip, _dptr, C.c_void_p, C.c_int, C.c_int]
+        L.orc_tdma2d9.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_norm_l1_9d.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dptr]
+        L.orc_norm_l1_9d_relative.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dptr]
+        L.orc_norm_l1_b.argtypes = [C.c_int, C.c_int, _dp]
+        L.orc_norm_l1_b.restype = C.c_double
+        L.orc_residual9.argtypes = [_dp, C.c_int, C.c_int, _dp, _dp, _dp]
+        _bind_coef(L)
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise FileNotFoundError(REF_SO + " (run `make -C oracle ref` where the reference exists)")
+        R = C.CDLL(REF_SO)
+        R.ref_lu2d9.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
+        R.ref_lu2d5.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
+        for f in (R.ref_fb2d9, R.ref_fb2d5):
+            f.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_double, C.c_int, _dp]
+        R.ref_tdma2d9.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        R.ref_solver_msi9d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
+        R.ref_norm_l1_9d.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dptr]
+        R.ref_norm_l1_9d_relative.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dptr]
+        R.ref_norm_l1_b.argtypes = [C.c_int, C.c_int, _dp]
+        R.ref_norm_l1_b.restype = C.c_double
+        _bind_ref_coef(R)
+        _ref = R
+    return _ref
+
+
+def _bind_coef(L):
+    """Assembly restatements (coef.c), bound when present."""
+    if hasattr(L, "orc_coef_bind"):
+        L.orc_coef_bind()
+
+
+def _bind_ref_coef(R):
+    pass
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------
+# C restatement
+# --------------------------------------------------------------------------
+def lu2d9(a, nx, ny, alpha0=0.5):
+    """src/solvers/msi2d9-v03.f90:277-432 -> (dl (5,N), du (4,N))."""
+    a = _c(a)
+    n = nx * ny
+    dl = np.zeros((5, n))
+    du = np.zeros((4, n))
+    lib().orc_lu2d9(a, dl, du, n, nx, ny, alpha0)
+    return dl, du
+
+
+def fb2d9(a, dl, du, nx, ny, var, gamma, nitm, rhs, hist_cap=None, max_iter_cap=0):
+    """src/solvers/msi2d9-v03.f90:42-262.  Returns (var, nit, r0, hist)."""
+    var = _c(var).copy()
+    n = nx * ny
+    cap = hist_cap if hist_cap is not None else max(nitm, max_iter_cap, 1)
+    hist = np.zeros(max(cap, 1))
+    nit = C.c_int(0)
+    r0 = C.c_double(0.0)
+    lib().orc_fb2d9(_c(a), _c(dl), _c(du), ny, nx, n, var, gamma, nitm, _c(rhs),
+                    C.byref(nit), C.byref(r0), hist.ctypes.data_as(C.c_void_p), cap,
+                    max_iter_cap)
+    k = nit.value
+    return var, k, r0.value, hist[:min(k, cap)].copy()
+
+
+def solve_msi9d(a, rhs, var, nx, ny, nitm, tol, max_iter_cap=0):
+    """class_solver_msi9d%solve: lu2d9 then fb2d9 (mod_solver_msi.f90:152-166)."""
+    dl, du = lu2d9(a, nx, ny)
+    return fb2d9(a, dl, du, nx, ny, var, tol, nitm, rhs, max_iter_cap=max_iter_cap)
+
+
+def lu2d5(a, nx, ny, alpha0=0.5):
+    a = _c(a)
+    n = nx * ny
+    dl = np.zeros((4, n))
+    du = np.zeros((3, n))
+    lib().orc_lu2d5(a, dl, du, n, nx, ny, alpha0)
+    return dl, du
+
+
+def fb2d5(a, dl, du, nx, ny, var, gamma, nitm, rhs, hist_cap=None, max_iter_cap=0):
+    var = _c(var).copy()
+    n = nx * ny
+    cap = hist_cap if hist_cap is not None else max(nitm, max_iter_cap, 1)
+    hist = np.zeros(max(cap, 1))
+    nit = C.c_int(0)
+    r0 = C.c_double(0.0)
+    lib().orc_fb2d5(_c(a), _c(dl), _c(du), ny, nx, n, var, gamma, nitm, _c(rhs),
+                    C.byref(nit), C.byref(r0), hist.ctypes.data_as(C.c_void_p), cap,
+                    max_iter_cap)
+    k = nit.value
+    return var, k, r0.value, hist[:min(k, cap)].copy()
+
+
+def tdma2d9(a, b, x, nx, ny, nitm):
+    x = _c(x).copy()
+    lib().orc_tdma2d9(nx, ny, nitm, _c(a), _c(b), x)
+    return x
+
+
+def residual9(a, var, rhs, nx, ny):
+    r = np.zeros(nx * ny)
+    lib().orc_residual9(_c(a), nx, ny, _c(var), _c(rhs), r)
+    return r
+
+
+def norm_l1_9d(a, b, var, nx, ny, norm=0.0):
+    v = C.c_double(norm)
+    lib().orc_norm_l1_9d(nx, ny, _c(var), _c(b), _c(a), C.byref(v))
+    return v.value
+
+
+def norm_l1_9d_relative(a, b, var, nx, ny, norm=0.0):
+    v = C.c_double(norm)
+    lib().orc_norm_l1_9d_relative(nx, ny, _c(var), _c(b), _c(a), C.byref(v))
+    return v.value
+
+
+def norm_l1_b(b, nx, ny):
+    return lib().orc_norm_l1_b(nx, ny, _c(b))
+
+
+# --------------------------------------------------------------------------
+# The reference itself (compiled Fortran).  fb2d9 keeps its automatic arrays
+# on the stack (3*N doubles), so calls run on a thread with a large stack.
+# --------------------------------------------------------------------------
+def _big_stack_call(fn, stack_mb=512):
+    out = {}
+
+    def run():
+        try:
+            out["v"] = fn()
+        except BaseException as e:  # pragma: no cover - re-raised below
+            out["e"] = e
+
+    old = threading.stack_size()
+    threading.stack_size(stack_mb * 1024 * 1024)
+    try:
+        t = threading.Thread(target=run)
+        t.start()
+        t.join()
+    finally:
+        threading.stack_size(old)
+    if "e" in out:
+        raise out["e"]
+    return out.get("v")
+
+
+def ref_lu2d9(a, nx, ny):
+    a = _c(a)
+    n = nx * ny
+    dl = np.zeros((5, n))
+    du = np.zeros((4, n))
+    ref().ref_lu2d9(a, dl, du, n, nx, ny)
+    return dl, du
+
+
+def ref_fb2d9(a, dl, du, nx, ny, var, gamma, nitm, rhs):
+    var = _c(var).copy()
+    n = nx * ny
+    _big_stack_call(lambda: ref().ref_fb2d9(_c(a), _c(dl), _c(du), ny, nx, n, var,
+                                            gamma, nitm, _c(rhs)))
+    return var
+
+
+def ref_lu2d5(a, nx, ny):
+    a = _c(a)
+    n = nx * ny
+    dl = np.zeros((4, n))
+    du = np.zeros((3, n))
+    ref().ref_lu2d5(a, dl, du, n, nx, ny)
+    return dl, du
+
+
+def ref_fb2d5(a, dl, du, nx, ny, var, gamma, nitm, rhs):
+    var = _c(var).copy()
+    n = nx * ny
+    _big_stack_call(lambda: ref().ref_fb2d5(_c(a), _c(dl), _c(du), ny, nx, n, var,
+                                            gamma, nitm, _c(rhs)))
+    return var
+
+
+def ref_tdma2d9(a, b, x, nx, ny, nitm):
+    x = _c(x).copy()
+    _big_stack_call(lambda: ref().ref_tdma2d9(nx, ny, nitm, _c(a), _c(b), x))
+    return x
+
+
+def ref_solver_msi9d(a, b, sol, nx, ny, nitm, tol):
+    sol = _c(sol).copy()
+    _big_stack_call(lambda: ref().ref_solver_msi9d(nx, ny, nitm, tol, _c(a), _c(b), sol))
+    return sol
+
+
+def ref_norm_l1_9d(a, b, var, nx, ny, norm=0.0):
+    v = C.c_double(norm)
+    ref().ref_norm_l1_9d(nx, ny, _c(var), _c(b), _c(a), C.byref(v))
+    return v.value
+
+
+def ref_norm_l1_9d_relative(a, b, var, nx, ny, norm=0.0):
+    v = C.c_double(norm)
+    ref().ref_norm_l1_9d_relative(nx, ny, _c(var), _c(b), _c(a), C.byref(v))
+    return v.value
+
+
+def ref_norm_l1_b(b, nx, ny):
+    return ref().ref_norm_l1_b(nx, ny, _c(b))

@@ -1,0 +1,155 @@
+"""Write the golden vectors of the hot path by running the REFERENCE.
+
+The reference's Fortran sources (src/solvers/*.f90, src/equations/
+mach2d_coef.f90) are compiled in place by `make -C oracle ref` into
+oracle/_ref/libm2dref.so (with a C-callable shim of ours); this script calls
+them on small seeded inputs and stores inputs and outputs as .npz files next
+to it.  It needs the reference tree, so it runs only in the build container:
+
+    python tests/golden/make_golden.py
+
+Fixtures (float64, diagonal-major arrays (k, nx*ny)):
+  msi_small.npz   lu2d9 factors and fb2d9 iterates (nitm = 1,2,3,5,10 with
+                  gamma = 0, and a full solve at gamma = 1e-10) on 6x5, 10x6,
+                  34x18 random systems and on 10x6 / 34x18 p' systems
+  cfg1.npz        130x66 p' system (the CPU-runnable configuration): fb2d9
+                  iterates after 1, 10, 100 iterations and the full 500
+  tdma.npz        tdma2d9 after nitm = 1, 2, 5 on random systems whose
+                  outside couplings are NOT zero (exercises every branch)
+  norms.npz       norm_l1_9d, norm_l1_9d_relative (both branches), norm_l1_b
+  embed5.npz      lu2d5 + fb2d5 against lu2d9 + fb2d9 on the zero-corner
+                  embedding of the same 5-point system
+  solver_msi9d.npz  class_solver_msi9d init + solve (the plugin object)
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "mach-2d_amd"))
+
+import oracle as O  # noqa: E402
+from mach2d import problems as P  # noqa: E402
+
+
+def save(name, **arrs):
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, **arrs)
+    print("%-18s %8.1f KB" % (name, os.path.getsize(path) / 1024))
+
+
+def iterates(a, nx, ny, rhs, ks, x0=None):
+    dl, du = O.ref_lu2d9(a, nx, ny)
+    x0 = np.zeros(nx * ny) if x0 is None else x0
+    out = {}
+    for k in ks:
+        out["var_%d" % k] = O.ref_fb2d9(a, dl, du, nx, ny, x0, 0.0, k, rhs)
+    return dl, du, out
+
+
+def msi_small():
+    arrs = {}
+    cases = [("r6x5", P.random_f3_system(6, 5, seed=11)),
+             ("r10x6", P.random_f3_system(10, 6, seed=12)),
+             ("r34x18", P.random_f3_system(34, 18, seed=13)),
+             ("p10x6", P.pcorr_system(10, 6, seed=14)),
+             ("p34x18", P.pcorr_system(34, 18, seed=15))]
+    for name, s in cases:
+        dl, du, it = iterates(s.a, s.nx, s.ny, s.rhs, [1, 2, 3, 5, 10])
+        full = O.ref_fb2d9(s.a, dl, du, s.nx, s.ny, np.zeros(s.N), 1e-10, 500, s.rhs)
+        arrs.update({name + "_a": s.a, name + "_rhs": s.rhs, name + "_dl": dl, name + "_du": du,
+                     name + "_full": full, name + "_shape": np.array([s.nx, s.ny])})
+        arrs.update({name + "_" + k: v for k, v in it.items()})
+    arrs["cases"] = np.array([c for c, _ in cases])
+    save("msi_small.npz", **arrs)
+
+
+def cfg1():
+    c = P.CFG1
+    s = P.pcorr_system(c["nx"], c["ny"], seed=c["seed"], q=c["q"], rhs=c["rhs"])
+    dl, du = O.ref_lu2d9(s.a, s.nx, s.ny)
+    arrs = dict(a=s.a, rhs=s.rhs, shape=np.array([s.nx, s.ny]))
+    for k in (1, 10, 100):
+        arrs["var_%d" % k] = O.ref_fb2d9(s.a, dl, du, s.nx, s.ny, np.zeros(s.N), 0.0, k, s.rhs)
+    arrs["var_full"] = O.ref_fb2d9(s.a, dl, du, s.nx, s.ny, np.zeros(s.N), c["gamma"],
+                                   c["nitm"], s.rhs)
+    arrs["dl1"] = dl[0]
+    arrs["du1"] = du[0]
+    save("cfg1.npz", **arrs)
+
+
+def tdma():
+    arrs = {}
+    rng = np.random.default_rng(21)
+    for nx, ny in [(6, 5), (10, 6), (34, 18), (3, 7)]:
+        a = -rng.uniform(0.05, 1.0, size=(9, nx * ny))
+        a[4] = 1.5 * np.sum(np.abs(a[[0, 1, 2, 3, 5, 6, 7, 8]]), axis=0) + 0.1
+        b = rng.uniform(-1, 1, nx * ny)
+        x0 = rng.uniform(-1, 1, nx * ny)
+        name = "t%dx%d" % (nx, ny)
+        arrs.update({name + "_a": a, name + "_b": b, name + "_x0": x0,
+                     name + "_shape": np.array([nx, ny])})
+        for k in (1, 2, 5):
+            arrs[name + "_x%d" % k] = O.ref_tdma2d9(a, b, x0, nx, ny, k)
+    save("tdma.npz", **arrs)
+
+
+def norms():
+    arrs = {}
+    rng = np.random.default_rng(31)
+    for nx, ny in [(6, 5), (34, 18)]:
+        name = "n%dx%d" % (nx, ny)
+        a = rng.uniform(-1, 1, size=(9, nx * ny))
+        b = rng.uniform(-1, 1, nx * ny)
+        v = rng.uniform(-1, 1, nx * ny)
+        arrs.update({name + "_a": a, name + "_b": b, name + "_var": v,
+                     name + "_shape": np.array([nx, ny]),
+                     name + "_l1": np.array(O.ref_norm_l1_9d(a, b, v, nx, ny, 0.25)),
+                     name + "_rel": np.array(O.ref_norm_l1_9d_relative(a, b, v, nx, ny, 0.25)),
+                     name + "_rel0": np.array(O.ref_norm_l1_9d_relative(a, np.zeros_like(b), v,
+                                                                         nx, ny, 0.25)),
+                     name + "_nb": np.array(O.ref_norm_l1_b(b, nx, ny))})
+    save("norms.npz", **arrs)
+
+
+def embed5():
+    arrs = {}
+    for nx, ny, seed in [(10, 6, 41), (34, 18, 42)]:
+        s9 = P.pcorr_system(nx, ny, seed=seed)
+        a5 = s9.a[[1, 3, 4, 5, 7]].copy()        # the 5-point part (S, W, P, E, N)
+        a9 = np.zeros_like(s9.a)
+        a9[[1, 3, 4, 5, 7]] = a5
+        dl5, du5 = O.ref_lu2d5(a5, nx, ny)
+        dl9, du9 = O.ref_lu2d9(a9, nx, ny)
+        x5 = O.ref_fb2d5(a5, dl5, du5, nx, ny, np.zeros(s9.N), 0.0, 7, s9.rhs)
+        x9 = O.ref_fb2d9(a9, dl9, du9, nx, ny, np.zeros(s9.N), 0.0, 7, s9.rhs)
+        name = "e%dx%d" % (nx, ny)
+        arrs.update({name + "_a5": a5, name + "_rhs": s9.rhs, name + "_dl5": dl5,
+                     name + "_du5": du5, name + "_x5": x5, name + "_x9": x9,
+                     name + "_shape": np.array([nx, ny])})
+    save("embed5.npz", **arrs)
+
+
+def solver_msi9d():
+    s = P.pcorr_system(34, 18, seed=51)
+    sol0 = np.linspace(-1, 1, s.N)
+    sol = O.ref_solver_msi9d(s.a, s.rhs, sol0, s.nx, s.ny, 20, 1e-2)
+    save("solver_msi9d.npz", a=s.a, b=s.rhs, sol0=sol0, sol=sol, shape=np.array([s.nx, s.ny]),
+         nitm=np.array(20), tol=np.array(1e-2))
+
+
+if __name__ == "__main__":
+    O.build()
+    if not O.ref_available():
+        sys.exit("reference library not built (no reference tree?)")
+    msi_small()
+    cfg1()
+    tdma()
+    norms()
+    embed5()
+    solver_msi9d()

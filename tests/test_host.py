@@ -1,0 +1,116 @@
+"""Host logic without a GPU: workload generators, the C ABI's exported
+symbols, the solver factory and argument handling."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from mach2d import problems as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_generated_systems_honour_outside_invariant():
+    for nx, ny, q in [(10, 6, 1.05), (130, 66, 1.05), (66, 34, 1.2)]:
+        s = P.pcorr_system(nx, ny, seed=1, q=q)
+        assert np.count_nonzero(s.a[P.outside_mask(nx, ny)]) == 0
+        # interior rows: M-matrix with a_P = 1.001 * sum |a_nb|
+        a = s.a.reshape(9, ny, nx)[:, 1:-1, 1:-1]
+        nb = a[[0, 1, 2, 3, 5, 6, 7, 8]]
+        assert np.all(nb <= 0)
+        assert np.allclose(a[4], 1.001 * np.abs(nb).sum(axis=0), rtol=1e-15)
+        # fictitious rows have zero rhs
+        b = s.rhs.reshape(ny, nx)
+        assert not b[0].any() and not b[-1].any() and not b[:, 0].any() and not b[:, -1].any()
+
+
+def test_uniform_grid_has_no_cross_terms():
+    """Cartesian grid: betae = betan = 0, so the corners vanish (before the
+    M-matrix sign change every corner is +-0)."""
+    nx, ny = 12, 9
+    g = P.Grid(nx, ny, *np.meshgrid(np.arange(nx) * 0.1, np.arange(ny) * 0.2)[::1], q=1.0)
+    m = P.metrics(g)
+    assert np.allclose(m["betae"][1:-1, :-1], 0) and np.allclose(m["betan"][:-1, 1:-1], 0)
+    a = P.diffusion_coefficients(nx, ny, m).reshape(9, ny, nx)
+    assert np.allclose(a[[0, 2, 6, 8], 1:-1, 1:-1], 0)
+
+
+def test_wall_aspect_ratio_target():
+    q = P.ratio_for_aspect(130, 66, 1e3)
+    asp = P.max_wall_aspect(P.ogive_grid(130, 66, q))
+    assert 0.9e3 < asp < 1.1e3
+
+
+def test_cfg4_sizes():
+    sizes = P.cfg4_sizes()
+    assert len(sizes) == 256
+    for nx, ny in sizes:
+        assert 66 <= nx <= 514 and 34 <= ny <= 258
+
+
+def _declared_symbols():
+    syms = []
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            syms += re.findall(r"\b(m2d_\w+)\s*\(", txt)
+    return sorted(set(syms))
+
+
+def test_library_exports_every_declared_symbol():
+    """Loads libmach2d_hip.so (no GPU call) and resolves every entry point
+    include/*.h declares."""
+    import ctypes
+
+    from mach2d import _lib
+    L = _lib.load()
+    declared = _declared_symbols()
+    assert declared, "no declarations found"
+    for name in declared:
+        assert hasattr(L, name), name
+        ctypes.cast(getattr(L, name), ctypes.c_void_p)
+    assert set(declared) == set(_lib.EXPORTS)
+    assert b"gfx950" in L.m2d_version()
+
+
+def test_solvers_init_factory():
+    from mach2d import solvers as S
+    cfg = dict(solver="MSI", nitm_u=5, nitm_p=20, tol_u=1e-2, tol_p=1e-2)
+    s5, s9 = S.solvers_init(10, 6, cfg)
+    assert isinstance(s5, S.ClassSolverMSI5d) and isinstance(s9, S.ClassSolverMSI9d)
+    assert (s9.nitm, s9.tol, s5.inner.nitm) == (5, 1e-2, 20)
+    cfg["solver"] = "TDMA"
+    s5, s9 = S.solvers_init(10, 6, cfg)
+    assert isinstance(s9, S.ClassSolverTDMA9d) and s9.nitm == 5
+    cfg["solver"] = "CG"
+    with pytest.raises(SystemExit):
+        S.solvers_init(10, 6, cfg)
+
+
+def test_coefficient_layouts_and_embedding():
+    from mach2d import solvers as S
+    n = 12
+    a = np.arange(9 * n, dtype=np.float64).reshape(9, n)
+    assert np.array_equal(S._coef(a, n, 9), a)
+    fortran = np.asfortranarray(a.T)          # a(n, 9) as the reference holds it
+    assert np.array_equal(S._coef(fortran, n, 9), a)
+    with pytest.raises(ValueError):
+        S._coef(a[:5], n, 9)
+    a5 = np.arange(5 * n, dtype=np.float64).reshape(5, n)
+    a9 = S.embed5(a5, n)
+    assert np.array_equal(a9[[1, 3, 4, 5, 7]], a5) and not a9[[0, 2, 6, 8]].any()
+
+
+def test_fortran_dropin_module_compiles(tmp_path):
+    """The iso_c_binding drop-in module (same module and routine names as
+    the reference) compiles with the image's Fortran compiler."""
+    import shutil
+    import subprocess
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        pytest.skip("no Fortran compiler")
+    src = os.path.join(ROOT, "mach-2d_amd", "fortran", "msi2d9_hip.f90")
+    r = subprocess.run([fc, "-c", "-module-dir", str(tmp_path), "-o",
+                        str(tmp_path / "m.o"), src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

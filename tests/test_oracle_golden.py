@@ -1,0 +1,122 @@
+"""The oracle (C restatement) against golden vectors written by the compiled
+reference (tests/golden/make_golden.py).  Bit-exact: the oracle keeps the
+reference's operation order and is built without FMA contraction."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+
+def _cases(g):
+    return [str(c) for c in g["cases"]]
+
+
+def test_lu2d9_matches_reference(orc):
+    g = load_golden("msi_small.npz")
+    for c in _cases(g):
+        nx, ny = (int(v) for v in g[c + "_shape"])
+        dl, du = orc.lu2d9(g[c + "_a"], nx, ny)
+        assert np.array_equal(dl, g[c + "_dl"]), c
+        assert np.array_equal(du, g[c + "_du"]), c
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5, 10])
+def test_fb2d9_iterates_match_reference(orc, k):
+    g = load_golden("msi_small.npz")
+    for c in _cases(g):
+        nx, ny = (int(v) for v in g[c + "_shape"])
+        var, nit, _, _ = orc.fb2d9(g[c + "_a"], g[c + "_dl"], g[c + "_du"], nx, ny,
+                                   np.zeros(nx * ny), 0.0, k, g[c + "_rhs"])
+        assert nit == k
+        assert np.array_equal(var, g[c + "_var_%d" % k]), c
+
+
+def test_fb2d9_full_solve_matches_reference(orc):
+    g = load_golden("msi_small.npz")
+    for c in _cases(g):
+        nx, ny = (int(v) for v in g[c + "_shape"])
+        var, nit, r0, hist = orc.fb2d9(g[c + "_a"], g[c + "_dl"], g[c + "_du"], nx, ny,
+                                       np.zeros(nx * ny), 1e-10, 500, g[c + "_rhs"])
+        assert np.array_equal(var, g[c + "_full"]), c
+        # the stopping rule fired on the strict test of msi2d9-v03.f90:258
+        assert hist[-1] < 1e-10 * r0 or nit == 500
+
+
+def test_cfg1_iterates_match_reference(orc):
+    g = load_golden("cfg1.npz")
+    nx, ny = (int(v) for v in g["shape"])
+    dl, du = orc.lu2d9(g["a"], nx, ny)
+    assert np.array_equal(dl[0], g["dl1"]) and np.array_equal(du[0], g["du1"])
+    for k in (1, 10, 100):
+        var, nit, _, _ = orc.fb2d9(g["a"], dl, du, nx, ny, np.zeros(nx * ny), 0.0, k, g["rhs"])
+        assert np.array_equal(var, g["var_%d" % k]), k
+    var, nit, r0, hist = orc.fb2d9(g["a"], dl, du, nx, ny, np.zeros(nx * ny), 1e-10, 500,
+                                   g["rhs"])
+    assert np.array_equal(var, g["var_full"])
+    assert len(hist) == nit
+
+
+@pytest.mark.parametrize("k", [1, 2, 5])
+def test_tdma2d9_matches_reference(orc, k):
+    g = load_golden("tdma.npz")
+    names = sorted({n.rsplit("_", 1)[0] for n in g if n.endswith("_shape")})
+    for c in names:
+        nx, ny = (int(v) for v in g[c + "_shape"])
+        x = orc.tdma2d9(g[c + "_a"], g[c + "_b"], g[c + "_x0"], nx, ny, k)
+        assert np.array_equal(x, g[c + "_x%d" % k]), c
+
+
+def test_norms_match_reference(orc):
+    g = load_golden("norms.npz")
+    names = sorted({n.rsplit("_", 1)[0] for n in g if n.endswith("_shape")})
+    for c in names:
+        nx, ny = (int(v) for v in g[c + "_shape"])
+        a, b, v = g[c + "_a"], g[c + "_b"], g[c + "_var"]
+        assert orc.norm_l1_9d(a, b, v, nx, ny, 0.25) == float(g[c + "_l1"])
+        assert orc.norm_l1_9d_relative(a, b, v, nx, ny, 0.25) == float(g[c + "_rel"])
+        assert orc.norm_l1_9d_relative(a, np.zeros_like(b), v, nx, ny, 0.25) == float(
+            g[c + "_rel0"])
+        assert orc.norm_l1_b(b, nx, ny) == float(g[c + "_nb"])
+
+
+def test_msi5d_and_embedding_match_reference(orc):
+    """F1 identity on the reference itself: lu2d9+fb2d9 on the zero-corner
+    embedding reproduce lu2d5+fb2d5; the oracle reproduces both."""
+    g = load_golden("embed5.npz")
+    names = sorted({n.rsplit("_", 1)[0] for n in g if n.endswith("_shape")})
+    for c in names:
+        nx, ny = (int(v) for v in g[c + "_shape"])
+        a5, rhs = g[c + "_a5"], g[c + "_rhs"]
+        dl5, du5 = orc.lu2d5(a5, nx, ny)
+        assert np.array_equal(dl5, g[c + "_dl5"]) and np.array_equal(du5, g[c + "_du5"])
+        x5, _, _, _ = orc.fb2d5(a5, dl5, du5, nx, ny, np.zeros(nx * ny), 0.0, 7, rhs)
+        assert np.array_equal(x5, g[c + "_x5"])
+        a9 = np.zeros((9, nx * ny))
+        a9[[1, 3, 4, 5, 7]] = a5
+        dl9, du9 = orc.lu2d9(a9, nx, ny)
+        x9, _, _, _ = orc.fb2d9(a9, dl9, du9, nx, ny, np.zeros(nx * ny), 0.0, 7, rhs)
+        assert np.array_equal(x9, g[c + "_x9"])
+        assert np.array_equal(x9, x5)
+
+
+def test_solver_object_matches_reference(orc):
+    g = load_golden("solver_msi9d.npz")
+    nx, ny = (int(v) for v in g["shape"])
+    var, _, _, _ = orc.solve_msi9d(g["a"], g["b"], g["sol0"], nx, ny, int(g["nitm"]),
+                                   float(g["tol"]))
+    assert np.array_equal(var, g["sol"])
+
+
+def test_oracle_matches_live_reference_when_built(orc):
+    """Where oracle/_ref exists (it travels with the tree), compare live on a
+    system not in the fixtures."""
+    if not orc.ref_available():
+        pytest.skip("reference library not built")
+    from mach2d import problems as P
+    s = P.pcorr_system(41, 23, seed=77)
+    dl, du = orc.lu2d9(s.a, s.nx, s.ny)
+    rdl, rdu = orc.ref_lu2d9(s.a, s.nx, s.ny)
+    assert np.array_equal(dl, rdl) and np.array_equal(du, rdu)
+    v, nit, _, _ = orc.fb2d9(s.a, dl, du, s.nx, s.ny, np.zeros(s.N), 1e-8, 300, s.rhs)
+    rv = orc.ref_fb2d9(s.a, dl, du, s.nx, s.ny, np.zeros(s.N), 1e-8, 300, s.rhs)
+    assert np.array_equal(v, rv)
