@@ -1,12 +1,13 @@
 // capi.hip -- the C ABI (include/mach2d_hip.h): solver handles, the device
 // side MSI iteration loop, and the reference-signature entry points.
 //
-// The iteration loop keeps the stopping decision on the device: every
-// iteration is [forward sweep, backward sweep + update, residual, norm and
-// stopping rule]; once the rule fires a flag turns the remaining kernels
-// into no-ops.  The host enqueues iterations in chunks and reads the flag
-// once per chunk, so the loop costs one host round trip per chunk instead of
-// one per iteration.
+// A handle owns the factor and sweep vectors in the wavefront layout of
+// skew.h (a skewed copy of A for the factorization, dl, du, r, w/z).  One MSI
+// iteration is [forward sweep, backward sweep, var = z + var, residual with
+// partial sums of r.r, norm and stopping rule]; the stopping rule runs on the
+// device and, once it fires, turns the remaining enqueued kernels into
+// no-ops, so the host enqueues iterations in chunks and reads the state once
+// per chunk instead of once per iteration.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -15,43 +16,80 @@
 #include <vector>
 
 #include "m2d_internal.h"
+#include "skew.h"
+#include "wavefront.h"
 
 using namespace m2d;
 
-#define HIP_OK(call)                                                  \
-    do {                                                              \
-        hipError_t e_ = (call);                                       \
-        if (e_ != hipSuccess) {                                       \
-            fprintf(stderr, "mach2d_hip: %s failed: %s (%s:%d)\n", #call, \
-                    hipGetErrorString(e_), __FILE__, __LINE__);       \
-            return M2D_ERR_HIP;                                       \
-        }                                                             \
+#define HIP_OK(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            fprintf(stderr, "mach2d_hip: %s failed: %s (%s:%d)\n", #call,     \
+                    hipGetErrorString(e_), __FILE__, __LINE__);               \
+            return M2D_ERR_HIP;                                               \
+        }                                                                     \
     } while (0)
+
+// Kernel ids of the timing facility: factor, forward, backward, residual;
+// +4 for the FP32 kernels of mixed precision.
+enum { KT_FACTOR = 0, KT_FORWARD = 1, KT_BACKWARD = 2, KT_RESIDUAL = 3, KT_FP32 = 4 };
+
+// Skewed buffers of one precision.
+template <typename T>
+struct Work {
+    Skew sk{};
+    T *a = nullptr;   // 9 arrays: skewed copy of A (factor input)
+    T *dl = nullptr;  // 5 arrays
+    T *du = nullptr;  // 4 arrays
+    T *r = nullptr;   // residual
+    T *w = nullptr;   // forward result, then z
+    void release()
+    {
+        T *b[] = {a, dl, du, r, w};
+        for (T *p : b)
+            if (p) (void)hipFree(p);
+        a = dl = du = r = w = nullptr;
+    }
+    bool alloc(int nx, int ny)
+    {
+        sk = skew_for(nx, ny, (int)sizeof(T));
+        const long L = sk.size();
+        const size_t z = sizeof(T) * (size_t)L;
+        bool ok = hipMalloc((void **)&a, 9 * z) == hipSuccess &&
+                  hipMalloc((void **)&dl, 5 * z) == hipSuccess &&
+                  hipMalloc((void **)&du, 4 * z) == hipSuccess &&
+                  hipMalloc((void **)&r, z) == hipSuccess && hipMalloc((void **)&w, z) == hipSuccess;
+        if (ok)  // padding entries are never consumed; start them finite
+            ok = hipMemset(r, 0, z) == hipSuccess && hipMemset(w, 0, z) == hipSuccess;
+        return ok;
+    }
+};
 
 struct m2d_system {
     int nx = 0, ny = 0;
     long N = 0;
     int precision = M2D_FP64;
     hipStream_t stream = nullptr;
-    int publish_every = 32;
     int xcd_stride = 1;
     int chunk = 8;
     double inner_gamma = 1e-4;
     int inner_nitm = 200;
 
-    // FP64 factor and sweep workspace
-    double *dl = nullptr, *du = nullptr, *r = nullptr, *w = nullptr;
-    // FP32 workspace (mixed precision)
-    float *a32 = nullptr, *dl32 = nullptr, *du32 = nullptr, *r32 = nullptr, *w32 = nullptr,
-          *c32 = nullptr, *rr32 = nullptr;
+    Work<double> w64;
+    Work<float> w32;
+    // mixed precision: FP32 A (reference layout), outer FP64 residual, FP32
+    // correction and its right-hand side
+    float *a32 = nullptr, *c32 = nullptr, *rr32 = nullptr;
+    double *r64 = nullptr;
+
     double *partial = nullptr;
     int np = 0;
-    unsigned long long *flags = nullptr;
-    int nflags = 0;
+    u64 *bnd = nullptr;  // boundary words, then per-lane scratch words
+    long nbnd = 0;
     SolveState *st = nullptr;     // device
     SolveState *st_in = nullptr;  // device, inner solve (mixed precision)
     SolveState *h_st = nullptr;   // pinned host mirror
-    unsigned gen = 0;
     bool factored = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -67,10 +105,6 @@ struct m2d_system {
     double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     long kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
-
-// Kernel ids of the timing facility: factor, forward, backward, residual;
-// +4 for the FP32 kernels of mixed precision.
-enum { KT_FACTOR = 0, KT_FORWARD = 1, KT_BACKWARD = 2, KT_RESIDUAL = 3, KT_FP32 = 4 };
 
 static hipEvent_t pool_event(m2d_system *s)
 {
@@ -127,7 +161,9 @@ static void set_last(int status)
     g_last.status = status;
 }
 
-extern "C" const char *m2d_version(void) { return "mach2d_hip 0.1 (gfx950)"; }
+void m2d_set_last_status_code(int status) { set_last(status); }
+
+extern "C" const char *m2d_version(void) { return "mach2d_hip 0.2 (gfx950)"; }
 
 extern "C" int m2d_last_status(m2d_status *out)
 {
@@ -148,9 +184,9 @@ static hipError_t dalloc(T **p, long n)
 extern "C" int m2d_destroy(m2d_system *s)
 {
     if (!s) return M2D_OK;
-    void *bufs[] = {s->dl,   s->du,   s->r,    s->w,   s->a32,     s->dl32,  s->du32,
-                    s->r32,  s->w32,  s->c32,  s->rr32, s->partial, s->flags, s->st,
-                    s->st_in};
+    s->w64.release();
+    s->w32.release();
+    void *bufs[] = {s->a32, s->c32, s->rr32, s->r64, s->partial, s->bnd, s->st, s->st_in};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : s->pool) (void)hipEventDestroy(e);
@@ -165,38 +201,26 @@ extern "C" int m2d_create(int nx, int ny, int precision, m2d_system **out)
 {
     if (!out || nx < 3 || ny < 3 || (precision != M2D_FP64 && precision != M2D_FP32_IR))
         return M2D_ERR_ARG;
-    if ((long)nx * ny > (1L << 31) / 9) return M2D_ERR_ARG;
+    if ((long)nx * ny > (1L << 31) / 16) return M2D_ERR_ARG;
     m2d_system *s = new m2d_system();
     s->nx = nx;
     s->ny = ny;
     s->N = (long)nx * ny;
     s->precision = precision;
     s->np = residual_blocks(nx, ny);
-    s->nflags = (ny + kWave - 1) / kWave;
+    s->nbnd = boundary_words(nx, ny) + scratch_words(ny);
     const long N = s->N;
-    bool ok = true;
-    ok &= dalloc(&s->r, N) == hipSuccess;
-    ok &= dalloc(&s->w, N) == hipSuccess;
-    ok &= dalloc(&s->partial, s->np) == hipSuccess;
-    ok &= dalloc(&s->flags, s->nflags) == hipSuccess;
-    ok &= dalloc(&s->st, 1) == hipSuccess;
-    ok &= dalloc(&s->st_in, 1) == hipSuccess;
-    if (precision == M2D_FP64) {
-        ok &= dalloc(&s->dl, 5 * N) == hipSuccess;
-        ok &= dalloc(&s->du, 4 * N) == hipSuccess;
-    } else {
-        ok &= dalloc(&s->a32, 9 * N) == hipSuccess;
-        ok &= dalloc(&s->dl32, 5 * N) == hipSuccess;
-        ok &= dalloc(&s->du32, 4 * N) == hipSuccess;
-        ok &= dalloc(&s->r32, N) == hipSuccess;
-        ok &= dalloc(&s->w32, N) == hipSuccess;
-        ok &= dalloc(&s->c32, N) == hipSuccess;
-        ok &= dalloc(&s->rr32, N) == hipSuccess;
+    bool ok = dalloc(&s->partial, s->np) == hipSuccess && dalloc(&s->bnd, s->nbnd) == hipSuccess &&
+              dalloc(&s->st, 1) == hipSuccess && dalloc(&s->st_in, 1) == hipSuccess;
+    if (ok && precision == M2D_FP64) ok = s->w64.alloc(nx, ny);
+    if (ok && precision == M2D_FP32_IR) {
+        ok = s->w32.alloc(nx, ny) && dalloc(&s->a32, 9 * N) == hipSuccess &&
+             dalloc(&s->c32, N) == hipSuccess && dalloc(&s->rr32, N) == hipSuccess &&
+             dalloc(&s->r64, N) == hipSuccess;
     }
-    ok &= hipHostMalloc((void **)&s->h_st, sizeof(SolveState), hipHostMallocDefault) ==
-          hipSuccess;
-    ok &= hipEventCreate(&s->ev0) == hipSuccess;
-    ok &= hipEventCreate(&s->ev1) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&s->h_st, sizeof(SolveState), hipHostMallocDefault) ==
+                   hipSuccess;
+    ok = ok && hipEventCreate(&s->ev0) == hipSuccess && hipEventCreate(&s->ev1) == hipSuccess;
     if (!ok) {
         m2d_destroy(s);
         return M2D_ERR_HIP;
@@ -214,8 +238,9 @@ extern "C" int m2d_set_stream(m2d_system *s, void *stream)
 
 extern "C" int m2d_set_tuning(m2d_system *s, int publish_every, int xcd_stride)
 {
+    // publish_every is accepted for interface stability: the wavefront
+    // kernels publish once per chunk of steps.
     if (!s || publish_every < 1 || (xcd_stride != 1 && xcd_stride != 8)) return M2D_ERR_ARG;
-    s->publish_every = publish_every;
     s->xcd_stride = xcd_stride;
     return M2D_OK;
 }
@@ -228,13 +253,14 @@ extern "C" int m2d_set_inner(m2d_system *s, double inner_gamma, int inner_nitm)
     return M2D_OK;
 }
 
-static SweepSync make_sync(m2d_system *s, SolveState *st)
+static Link make_link(m2d_system *s, SolveState *st, int phase)
 {
-    SweepSync y;
-    y.flags = s->flags;
+    Link y;
+    y.bnd = s->bnd;
+    y.scratch = s->bnd + boundary_words(s->nx, s->ny);
     y.st = st;
-    y.gen = ++s->gen;
-    y.publish_every = s->publish_every;
+    y.phase = phase;
+    y.gen = 0;
     return y;
 }
 
@@ -245,11 +271,12 @@ static int read_state(m2d_system *s, SolveState *dev)
     return M2D_OK;
 }
 
-// Reset state and flags; gens restart so a replayed sequence is valid.
+// Reset the solve state and clear the boundary words: generations restart
+// with every solve (they follow the device iteration counter).
 static void begin(m2d_system *s, SolveState *st)
 {
-    launch_zero_flags(s->flags, s->nflags, st, 1, s->stream);
-    s->gen = 0;
+    launch_reset_state(st, s->stream);
+    launch_clear_boundary(s->bnd, s->nx, s->ny, s->stream);
 }
 
 extern "C" int m2d_factor(m2d_system *s, const double *d_a)
@@ -258,15 +285,19 @@ extern "C" int m2d_factor(m2d_system *s, const double *d_a)
     begin(s, s->st);
     launch_check_outside(d_a, s->nx, s->ny, s->st, s->stream);
     if (s->precision == M2D_FP64) {
+        Work<double> &W = s->w64;
+        launch_to_skew<double, double>(d_a, W.a, 9, W.sk, s->stream);
         timed(s, KT_FACTOR, 0, [&] {
-            launch_factor<double>(d_a, s->dl, s->du, s->nx, s->ny, 0.5, make_sync(s, s->st),
-                                  s->xcd_stride, s->stream);
+            launch_factor<double>(W.a, W.dl, W.du, 0.5, W.sk, make_link(s, s->st, 0), s->xcd_stride,
+                                  s->stream);
         });
     } else {
+        Work<float> &W = s->w32;
         launch_convert<double, float>(d_a, s->a32, 9 * s->N, s->stream);
+        launch_to_skew<double, float>(d_a, W.a, 9, W.sk, s->stream);
         timed(s, KT_FACTOR + KT_FP32, 0, [&] {
-            launch_factor<float>(s->a32, s->dl32, s->du32, s->nx, s->ny, 0.5f,
-                                 make_sync(s, s->st), s->xcd_stride, s->stream);
+            launch_factor<float>(W.a, W.dl, W.du, 0.5f, W.sk, make_link(s, s->st, 0),
+                                 s->xcd_stride, s->stream);
         });
     }
     HIP_OK(hipGetLastError());
@@ -282,11 +313,10 @@ extern "C" int m2d_factor(m2d_system *s, const double *d_a)
 
 extern "C" int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du)
 {
-    if (!s || s->precision != M2D_FP64 || !s->factored) return M2D_ERR_ARG;
-    HIP_OK(hipMemcpyAsync(d_dl, s->dl, sizeof(double) * 5 * s->N, hipMemcpyDeviceToDevice,
-                          s->stream));
-    HIP_OK(hipMemcpyAsync(d_du, s->du, sizeof(double) * 4 * s->N, hipMemcpyDeviceToDevice,
-                          s->stream));
+    if (!s || s->precision != M2D_FP64 || !s->factored || !d_dl || !d_du) return M2D_ERR_ARG;
+    launch_from_skew<double>(s->w64.dl, d_dl, 5, s->w64.sk, s->stream);
+    launch_from_skew<double>(s->w64.du, d_du, 4, s->w64.sk, s->stream);
+    HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(s->stream));
     return M2D_OK;
 }
@@ -294,50 +324,48 @@ extern "C" int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du)
 // Install an externally computed FP64 factor (fb2d9 takes dl, du as input).
 static int put_factor(m2d_system *s, const double *d_dl, const double *d_du)
 {
-    HIP_OK(hipMemcpyAsync(s->dl, d_dl, sizeof(double) * 5 * s->N, hipMemcpyDeviceToDevice,
-                          s->stream));
-    HIP_OK(hipMemcpyAsync(s->du, d_du, sizeof(double) * 4 * s->N, hipMemcpyDeviceToDevice,
-                          s->stream));
+    launch_to_skew<double, double>(d_dl, s->w64.dl, 5, s->w64.sk, s->stream);
+    launch_to_skew<double, double>(d_du, s->w64.du, 4, s->w64.sk, s->stream);
+    HIP_OK(hipGetLastError());
     s->factored = true;
     return M2D_OK;
 }
 
-// One MSI iteration (k >= 1) in precision T on the system's workspace.
+// One MSI iteration (k >= 1) in precision T.
 template <typename T>
-static void enqueue_iteration(m2d_system *s, SolveState *st, const T *a, const T *rhs, T *x,
-                              const T *dl, const T *du, T *r, T *w, double *hist, int hist_cap,
-                              int k, double gamma, int nitm)
+static void enqueue_iteration(m2d_system *s, Work<T> &W, SolveState *st, const T *a, const T *rhs,
+                              T *x, double *hist, int hist_cap, int k, double gamma, int nitm)
 {
     const int off = sizeof(T) == 4 ? KT_FP32 : 0;
     timed(s, KT_FORWARD + off, k, [&] {
-        launch_forward<T>(r, dl, w, s->nx, s->ny, make_sync(s, st), s->xcd_stride, s->stream);
+        launch_forward<T>(W.r, W.dl, W.w, W.sk, make_link(s, st, 1), s->xcd_stride, s->stream);
     });
     timed(s, KT_BACKWARD + off, k, [&] {
-        launch_backward<T, T>(w, du, x, s->nx, s->ny, make_sync(s, st), s->xcd_stride,
-                              s->stream);
+        launch_backward<T>(W.w, W.du, W.sk, make_link(s, st, 2), s->xcd_stride, s->stream);
     });
     timed(s, KT_RESIDUAL + off, k, [&] {
-        launch_residual<T, T, T>(a, x, rhs, r, s->nx, s->ny, s->partial, st, s->stream);
+        launch_update<T>(x, W.w, W.sk, st, s->stream);
+        launch_residual<T, T, T>(a, x, rhs, W.r, s->nx, s->ny, s->partial, st, &W.sk, s->stream);
     });
-    launch_norm_finalize(s->partial, s->np, st, hist, hist_cap, k, gamma, nitm, s->stream);
+    launch_norm_finalize(s->partial, s->np, st, hist, hist_cap, 0, gamma, nitm, s->stream);
 }
 
-// fb2d9 on the device, in precision T, from the current x: returns after the
-// stopping rule fired (state in s->h_st).
+// fb2d9 on the device, in precision T, from the current x (reference
+// layout); returns after the stopping rule fired (state in s->h_st).
 template <typename T>
-static int run_msi(m2d_system *s, SolveState *st, const T *a, const T *rhs, T *x, const T *dl,
-                   const T *du, T *r, T *w, double gamma, int nitm, double *hist, int hist_cap)
+static int run_msi(m2d_system *s, Work<T> &W, SolveState *st, const T *a, const T *rhs, T *x,
+                   double gamma, int nitm, double *hist, int hist_cap)
 {
     begin(s, st);
-    launch_residual<T, T, T>(a, x, rhs, r, s->nx, s->ny, s->partial, st, s->stream);
-    launch_norm_finalize(s->partial, s->np, st, nullptr, 0, 0, gamma, nitm, s->stream);
+    launch_residual<T, T, T>(a, x, rhs, W.r, s->nx, s->ny, s->partial, st, &W.sk, s->stream);
+    launch_norm_finalize(s->partial, s->np, st, nullptr, 0, 1, gamma, nitm, s->stream);
     const int kmax = nitm > 0 ? nitm : kIterCap;
     int k = 1;
     int chunk = 2;  // grows geometrically up to s->chunk
     while (k <= kmax) {
         const int kend = std::min(kmax, k + chunk - 1);
         for (; k <= kend; ++k)
-            enqueue_iteration<T>(s, st, a, rhs, x, dl, du, r, w, hist, hist_cap, k, gamma, nitm);
+            enqueue_iteration<T>(s, W, st, a, rhs, x, hist, hist_cap, k, gamma, nitm);
         HIP_OK(hipGetLastError());
         int rc = read_state(s, st);
         if (rc) return rc;
@@ -365,8 +393,7 @@ static int solve_fp64(m2d_system *s, const double *d_a, const double *d_rhs, dou
                       double gamma, int nitm, double *d_hist, int hist_cap, m2d_status *out)
 {
     HIP_OK(hipEventRecord(s->ev0, s->stream));
-    int rc = run_msi<double>(s, s->st, d_a, d_rhs, d_var, s->dl, s->du, s->r, s->w, gamma, nitm,
-                             d_hist, hist_cap);
+    int rc = run_msi<double>(s, s->w64, s->st, d_a, d_rhs, d_var, gamma, nitm, d_hist, hist_cap);
     if (rc) return rc;
     HIP_OK(hipEventRecord(s->ev1, s->stream));
     HIP_OK(hipEventSynchronize(s->ev1));
@@ -376,25 +403,13 @@ static int solve_fp64(m2d_system *s, const double *d_a, const double *d_rhs, dou
     return s->h_st->status;
 }
 
-int solve_fp32_ir(m2d_system *s, const double *d_a, const double *d_rhs, double *d_var,
-                  double gamma, int nitm, double *d_hist, int hist_cap, m2d_status *out);
-
-extern "C" int m2d_solve(m2d_system *s, const double *d_a, const double *d_rhs, double *d_var,
-                         double gamma, int nitm, double *d_hist, int hist_cap, m2d_status *st)
-{
-    if (!s || !d_a || !d_rhs || !d_var || (d_hist && hist_cap < 1)) return M2D_ERR_ARG;
-    if (!s->factored) return M2D_ERR_ARG;
-    if (s->precision == M2D_FP64)
-        return solve_fp64(s, d_a, d_rhs, d_var, gamma, nitm, d_hist, hist_cap, st);
-    return solve_fp32_ir(s, d_a, d_rhs, d_var, gamma, nitm, d_hist, hist_cap, st);
-}
-
 // ---------------------------------------------------------------------------
 // Mixed precision: FP32 factor and FP32 MSI sweeps inside FP64 iterative
-// refinement.  Outer step: r = A x - rhs in FP64 (stopping rule of fb2d9 on
-// this FP64 residual), then the correction c solves A c = -r approximately by
-// FP32 MSI iterations from c = 0 (A, its factor, residuals and sweeps in
-// FP32) until its own residual drops by inner_gamma, then x += c in FP64.
+// refinement.  Outer step: r = A x - rhs in FP64 (the stopping rule of fb2d9
+// on this FP64 residual), then the correction c solves A c = -r
+// approximately by FP32 MSI iterations from c = 0 (A, its factor, residuals
+// and sweeps in FP32) until its own residual drops by inner_gamma, then
+// x += c in FP64.
 // ---------------------------------------------------------------------------
 __global__ void k_ir_prepare(const double *__restrict__ r64, float *__restrict__ rr32,
                              float *__restrict__ c32, long n, const SolveState *st)
@@ -415,44 +430,35 @@ __global__ void k_ir_update(double *__restrict__ x, const float *__restrict__ c3
     if (q < n) x[q] = x[q] + (double)c32[q];
 }
 
-__global__ void k_copy_done(SolveState *outer, const SolveState *inner, int *flag_bad)
-{
-    // inner status propagates only as an error code
-    if (inner->status && inner->status != M2D_ERR_ITER_CAP) outer->status = inner->status;
-    (void)flag_bad;
-}
-
-int solve_fp32_ir(m2d_system *s, const double *d_a, const double *d_rhs, double *d_var,
-                  double gamma, int nitm, double *d_hist, int hist_cap, m2d_status *out)
+static int solve_fp32_ir(m2d_system *s, const double *d_a, const double *d_rhs, double *d_var,
+                         double gamma, int nitm, double *d_hist, int hist_cap, m2d_status *out)
 {
     HIP_OK(hipEventRecord(s->ev0, s->stream));
     const long N = s->N;
     const unsigned nblk = (unsigned)((N + 255) / 256);
-    // outer state lives in s->st; its flags are untouched by the inner solves
-    launch_zero_flags(s->flags, s->nflags, s->st, 1, s->stream);
-    launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r, s->nx, s->ny, s->partial,
-                                            s->st, s->stream);
-    launch_norm_finalize(s->partial, s->np, s->st, nullptr, 0, 0, gamma, nitm, s->stream);
+    launch_reset_state(s->st, s->stream);
+    launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r64, s->nx, s->ny, s->partial,
+                                            s->st, nullptr, s->stream);
+    launch_norm_finalize(s->partial, s->np, s->st, nullptr, 0, 1, gamma, nitm, s->stream);
     HIP_OK(hipGetLastError());
     int rc = read_state(s, s->st);
     if (rc) return rc;
     SolveState outer = *s->h_st;
     int total_inner = 0;
     const int kmax = nitm > 0 ? nitm : kIterCap;
-    int k = 1;
-    for (; k <= kmax && !outer.done; ++k) {
-        hipLaunchKernelGGL(k_ir_prepare, dim3(nblk), dim3(256), 0, s->stream, s->r, s->rr32,
+    for (int k = 1; k <= kmax && !outer.done; ++k) {
+        hipLaunchKernelGGL(k_ir_prepare, dim3(nblk), dim3(256), 0, s->stream, s->r64, s->rr32,
                            s->c32, N, s->st);
-        rc = run_msi<float>(s, s->st_in, s->a32, s->rr32, s->c32, s->dl32, s->du32, s->r32,
-                            s->w32, s->inner_gamma, s->inner_nitm, nullptr, 0);
+        rc = run_msi<float>(s, s->w32, s->st_in, s->a32, s->rr32, s->c32, s->inner_gamma,
+                            s->inner_nitm, nullptr, 0);
         if (rc) return rc;
         total_inner += s->h_st->nit;
         const int inner_status = s->h_st->status;
         hipLaunchKernelGGL(k_ir_update, dim3(nblk), dim3(256), 0, s->stream, d_var, s->c32, N,
                            s->st);
-        launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r, s->nx, s->ny,
-                                                s->partial, s->st, s->stream);
-        launch_norm_finalize(s->partial, s->np, s->st, d_hist, hist_cap, k, gamma, nitm,
+        launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r64, s->nx, s->ny,
+                                                s->partial, s->st, nullptr, s->stream);
+        launch_norm_finalize(s->partial, s->np, s->st, d_hist, hist_cap, 0, gamma, nitm,
                              s->stream);
         HIP_OK(hipGetLastError());
         rc = read_state(s, s->st);
@@ -472,6 +478,16 @@ int solve_fp32_ir(m2d_system *s, const double *d_a, const double *d_rhs, double 
     s->h_st->nit = total_inner;
     fill_status(s, out, ms);
     return outer.status;
+}
+
+extern "C" int m2d_solve(m2d_system *s, const double *d_a, const double *d_rhs, double *d_var,
+                         double gamma, int nitm, double *d_hist, int hist_cap, m2d_status *st)
+{
+    if (!s || !d_a || !d_rhs || !d_var || (d_hist && hist_cap < 1)) return M2D_ERR_ARG;
+    if (!s->factored) return M2D_ERR_ARG;
+    if (s->precision == M2D_FP64)
+        return solve_fp64(s, d_a, d_rhs, d_var, gamma, nitm, d_hist, hist_cap, st);
+    return solve_fp32_ir(s, d_a, d_rhs, d_var, gamma, nitm, d_hist, hist_cap, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -538,8 +554,11 @@ extern "C" void m2d_lu2d9(const double *b, double *dl, double *du, const int *ni
     rc = m2d_factor(s, g_cache.a);
     set_last(rc);
     if (rc == M2D_OK || rc == M2D_ERR_NONFINITE) {
-        if (hipMemcpy(dl, s->dl, sizeof(double) * 5 * N, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(du, s->du, sizeof(double) * 4 * N, hipMemcpyDeviceToHost) != hipSuccess)
+        int rc2 = m2d_get_factor(s, g_cache.dl, g_cache.du);
+        if (rc2 ||
+            hipMemcpy(dl, g_cache.dl, sizeof(double) * 5 * N, hipMemcpyDeviceToHost) !=
+                hipSuccess ||
+            hipMemcpy(du, g_cache.du, sizeof(double) * 4 * N, hipMemcpyDeviceToHost) != hipSuccess)
             set_last(M2D_ERR_HIP);
     }
 }
@@ -622,8 +641,6 @@ extern "C" void m2d_solve_msi9d(const int *nx, const int *ny, const int *nitm, c
     }
     host_fb2d9(a, nullptr, nullptr, *ny, *nx, *nx * *ny, sol, *tol, *nitm, b, true);
 }
-
-void m2d_set_last_status_code(int status) { set_last(status); }
 
 // Batched entry: every system keeps its own handle, factor and arrays.
 extern "C" int m2d_solve_batch(m2d_system **systems, int count, const double **d_a,

@@ -23,40 +23,48 @@ struct SolveState {
     int err_pivot;   // factorization met a zero / non-finite pivot
     int outside;     // number of non-zero couplings to outside neighbours
     int outer;       // refinement steps (mixed precision)
+    int iter;        // iterations completed (drives the launch generations)
+    int pad0;
     double r0;       // ||r0||
     double gammarp;  // gamma * ||r0||
     double rn;       // latest ||r||
     double pad;
 };
 
-// Progress flags of a wavefront launch: one 64-bit word per band,
-// (generation << 32) | columns published.  Zeroed at the start of a solve.
-struct SweepSync {
-    unsigned long long *flags;
-    SolveState *st;
-    unsigned gen;
-    int publish_every;
-};
+struct Link;
+struct Skew;
 
-// ---- launchers (msi9.hip) --------------------------------------------------
+// ---- wavefront kernels and layout (sweep.hip) --------------------------------
+long boundary_words(int nx, int ny);
+long scratch_words(int ny);
+Skew skew_for(int nx, int ny, int value_bytes);
+void launch_clear_boundary(unsigned long long *bnd, int nx, int ny, hipStream_t s);
 template <typename T>
-void launch_factor(const T *a, T *dl, T *du, int nx, int ny, T alpha0, SweepSync sync,
+void launch_factor(const T *a_sk, T *dl, T *du, T alpha0, const Skew &sk, Link lk,
                    int xcd_stride, hipStream_t s);
 template <typename T>
-void launch_forward(const T *r, const T *dl, T *w, int nx, int ny, SweepSync sync,
-                    int xcd_stride, hipStream_t s);
-template <typename T, typename TX>
-void launch_backward(T *wz, const T *du, TX *x, int nx, int ny, SweepSync sync,
-                     int xcd_stride, hipStream_t s);
+void launch_forward(const T *r, const T *dl, T *w, const Skew &sk, Link lk, int xcd_stride,
+                    hipStream_t s);
+template <typename T>
+void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s);
+template <typename TI, typename TO>
+void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t s);
+template <typename T>
+void launch_from_skew(const T *src, T *dst, int nk, const Skew &sk, hipStream_t s);
+template <typename T>
+void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipStream_t s);
+
+// ---- parallel kernels (msi9.hip) ---------------------------------------------
 template <typename TA, typename TX, typename TR>
 void launch_residual(const TA *a, const TX *x, const TA *rhs, TR *r, int nx, int ny,
-                     double *partial, const SolveState *st, hipStream_t s);
+                     double *partial, const SolveState *st, const Skew *out_skew, hipStream_t s);
 int residual_blocks(int nx, int ny);
+// initial = 1: the residual of the incoming var (k = 0); otherwise the
+// residual after iteration k = st->iter + 1, which becomes st->iter.
 void launch_norm_finalize(const double *partial, int np, SolveState *st, double *hist,
-                          int hist_cap, int k, double gamma, int nitm, hipStream_t s);
+                          int hist_cap, int initial, double gamma, int nitm, hipStream_t s);
 void launch_check_outside(const double *a, int nx, int ny, SolveState *st, hipStream_t s);
-void launch_zero_flags(unsigned long long *flags, int n, SolveState *st, int reset_state,
-                       hipStream_t s);
+void launch_reset_state(SolveState *st, hipStream_t s);
 template <typename TI, typename TO>
 void launch_convert(const TI *in, TO *out, long n, hipStream_t s);
 

@@ -1,0 +1,602 @@
+// sweep.hip -- pipelined wavefront kernels of the 9-diagonal MSI:
+// factorization lu2d9 (src/solvers/msi2d9-v03.f90:277-432) and the forward /
+// backward sweeps of fb2d9 (:139-181), plus the layout conversions and the
+// update var = z + var (:181).
+//
+// Schedule.  The factorization and the forward sweep of point (i,j) read the
+// W, SW, S and SE neighbours (:357-366, :155), the backward sweep E, NW, N
+// and NE (:175), so points with equal k = i + 2j are independent.  A wave of
+// 64 lanes owns a band of 64 rows; lane l (row 64*band + l) works on column
+// i at stored step u = i + 2l + pad (layout in skew.h).  S/SE/SW (N/NW/NE)
+// values come from lane l-1 (l+1) through a one-lane DPP shift of its latest
+// result and two registers of history.
+//
+// Chunks.  A kernel advances F steps per chunk.  Every lane loads its F
+// consecutive values of each input array with 16-byte loads one chunk ahead,
+// keeps its F results in registers and stores them with 16-byte stores after
+// the chunk, so the loop body has a fixed, small number of vector memory
+// operations and the only wait per chunk is for loads issued a chunk
+// earlier.  Steps whose column is outside the grid compute on padding and
+// are never consumed: every neighbour value is masked by the grid test.
+//
+// Bands.  A band's edge row travels to the neighbouring band through
+// generation-tagged words (wavefront.h), F columns per chunk: lanes 0..F-1
+// of the producer store the values of the edge lane (every lane stores; the
+// others write a private scratch word, which keeps the store count fixed);
+// lanes 0..F of the consumer load the F+1 columns its next chunk needs, one
+// chunk ahead, and the edge lane takes them with lane reads.  A band only
+// waits on a band with a lower launch index, so the waits cannot deadlock
+// under in-order workgroup dispatch, and every wait is bounded.
+#include <math.h>
+
+#include "m2d_internal.h"
+#include "msi_stencil.h"
+#include "skew.h"
+#include "wavefront.h"
+
+namespace m2d {
+
+template <typename T, int F>
+__device__ __forceinline__ void vload(T (&dst)[F], const T *__restrict__ src)
+{
+    static_assert((F * sizeof(T)) % 16 == 0, "a chunk is a whole number of 16-byte words");
+    const uint4 *p = reinterpret_cast<const uint4 *>(src);
+#pragma unroll
+    for (int q = 0; q < (int)(F * sizeof(T) / 16); ++q) {
+        const uint4 v = p[q];
+        __builtin_memcpy(&dst[q * (int)(16 / sizeof(T))], &v, 16);
+    }
+}
+
+template <typename T, int F>
+__device__ __forceinline__ void vstore(T *__restrict__ dst, const T (&src)[F])
+{
+    uint4 *p = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+    for (int q = 0; q < (int)(F * sizeof(T) / 16); ++q) {
+        uint4 v;
+        __builtin_memcpy(&v, &src[q * (int)(16 / sizeof(T))], 16);
+        p[q] = v;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ bool bad_pivot(T d)
+{
+    return !(d == d) || d == T(0) || isinf(d);
+}
+
+// Slow path of a boundary wait: sleep, reload, re-check, bounded.  Returns
+// false when it gave up or another band did (abort flag).
+template <typename FOK, typename FRE>
+__device__ bool slow_wait(const Link &lk, FOK ok, FRE reload)
+{
+    for (int spin = 0; spin < kMaxSpin; ++spin) {
+        if (__hip_atomic_load(&lk.st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return false;
+        __builtin_amdgcn_s_sleep(1);
+        reload();
+        if (__all(ok())) return true;
+    }
+    __hip_atomic_store(&lk.st->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return false;
+}
+
+// Value of lane `sel` of a per-lane array held by one lane: lane q < F gets
+// element q of `v` as held by lane `src`.
+template <typename T, int F>
+__device__ __forceinline__ T spread_from_lane(const T (&v)[F], int src, int lane)
+{
+    T out = read_lane(v[0], src);
+#pragma unroll
+    for (int q = 1; q < F; ++q) {
+        const T x = read_lane(v[q], src);
+        out = lane == q ? x : out;
+    }
+    return out;
+}
+
+// --------------------------------------------------------------------------
+// forward sweep  L w = -r  (msi2d9-v03.f90:139-156); r, dl, w in the skewed
+// layout.
+// --------------------------------------------------------------------------
+template <typename T, int F>
+__global__ __launch_bounds__(kWave) void k_forward(const T *__restrict__ r,
+                                                   const T *__restrict__ dl, T *__restrict__ w,
+                                                   Skew sk, Link lk0, int xcd_stride)
+{
+    if (lk0.st->done) return;
+    Link lk = lk0;
+    lk.gen = launch_gen(lk0);
+    const int blk = blockIdx.x;
+    if (blk % xcd_stride) return;
+    const int band = blk / xcd_stride;
+    const int lane = threadIdx.x;
+    const int nx = sk.nx, pad = sk.pad(band);
+    const int nf = sk.nst / F;  // chunks of F steps
+    const long L = sk.size();
+    const int j = band * kWave + lane;
+    const bool hs = j > 0, has_up = band > 0, has_down = band < sk.nb - 1;
+    constexpr int NW = TagWords<T>::n;
+    const u64 *bin = lk.bnd + (long)(has_up ? band - 1 : 0) * nx * NW;
+    u64 *bout = lk.bnd + (long)band * nx * NW;
+    u64 *scratch = lk.scratch + ((long)band * kWave + lane) * 8;
+
+    T A[6][F], B[6][F];
+    Raw<T> bn{};
+    T bv[F + 1];
+    T cur = 0, se = 0, sv = 0, sw = 0;
+
+    auto load_data = [&](T(&buf)[6][F], int c) {
+        const long e = sk.index(band, lane, min(c, nf - 1) * F);
+        vload<T, F>(buf[0], r + e);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) vload<T, F>(buf[1 + k], dl + k * L + e);
+    };
+    auto bcol = [&](int c) { return c * F - pad + lane; };  // column held by this lane
+    auto bneed = [&](int c) {
+        const int col = bcol(c);
+        return has_up && lane <= F && col >= 0 && col < nx;
+    };
+    auto load_bnd = [&](Raw<T> &b, int c) {
+        raw_load<T>(b, bin + (long)min(max(bcol(c), 0), nx - 1) * NW);
+    };
+    // The slow path reloads into a temporary and copies it back, so the fast
+    // path's loads keep a fixed number of younger memory operations.
+    auto ready = [&](Raw<T> &b, int c) -> bool {
+        if (!has_up || __all(!bneed(c) || raw_ok<T>(b, lk.gen))) return true;
+        Raw<T> t{};
+        const bool ok = slow_wait(lk, [&] { return !bneed(c) || raw_ok<T>(t, lk.gen); },
+                                  [&] { load_bnd(t, c); });
+        b = t;
+        return ok;
+    };
+    auto take = [&](const Raw<T> &b) {
+        const T val = raw_value(b);
+#pragma unroll
+        for (int q = 0; q <= F; ++q) bv[q] = read_lane(val, q);
+    };
+    // Chunk c: issue the loads of chunk c+1, run the F steps of chunk c, then
+    // take chunk c+1's boundary (any wait there covers only operations issued
+    // a chunk of compute earlier), then store and publish chunk c.
+    auto chunk = [&](T(&cb)[6][F], T(&nbuf)[6][F], int c) -> bool {
+        load_data(nbuf, c + 1);
+        load_bnd(bn, c + 1);
+        T outv[F];
+#pragma unroll
+        for (int s = 0; s < F; ++s) {
+            const int i = c * F + s - pad - 2 * lane;
+            const T up = lane_from_below(cur);
+            sw = sv;
+            sv = se;
+            se = lane == 0 ? (has_up ? bv[s + 1] : T(0)) : up;  // column i+1 above
+            const bool hw = i > 0, he = i + 1 < nx;
+            const T v = forward_point<T>(cb[0][s], cb[1][s], cb[2][s], cb[3][s], cb[4][s],
+                                         cb[5][s], hw ? cur : T(0), (hs && he) ? se : T(0),
+                                         hs ? sv : T(0), (hs && hw) ? sw : T(0));
+            outv[s] = v;
+            cur = v;
+        }
+        if (c + 1 < nf) {
+            if (!ready(bn, c + 1)) return false;
+            take(bn);
+        }
+        vstore<T, F>(w + sk.index(band, lane, c * F), outv);
+        {  // publish the edge lane's F results (columns c*F + q - pad - 126)
+            const T pv = spread_from_lane<T, F>(outv, kWave - 1, lane);
+            const int col = c * F + lane - pad - 2 * (kWave - 1);
+            const bool ok = has_down && lane < F && col >= 0 && col < nx;
+            put_tagged(ok ? bout + (long)col * NW : scratch, lk.gen, pv);
+        }
+        return true;
+    };
+
+    load_data(A, 0);
+    load_bnd(bn, 0);
+    if (!ready(bn, 0)) return;
+    take(bn);
+    se = (lane == 0 && has_up) ? bv[0] : T(0);  // column -pad above (used only if pad = 0)
+    for (int c = 0; c < nf; c += 2) {
+        if (!chunk(A, B, c)) return;
+        if (c + 1 >= nf) break;
+        if (!chunk(B, A, c + 1)) return;
+    }
+}
+
+// --------------------------------------------------------------------------
+// backward sweep  U z = w  (msi2d9-v03.f90:158-176): chunks in reverse
+// stored order, z over w in place; bands top to bottom.
+// --------------------------------------------------------------------------
+template <typename T, int F>
+__global__ __launch_bounds__(kWave) void k_backward(T *__restrict__ wz, const T *__restrict__ du,
+                                                    Skew sk, Link lk0, int xcd_stride)
+{
+    if (lk0.st->done) return;
+    Link lk = lk0;
+    lk.gen = launch_gen(lk0);
+    const int blk = blockIdx.x;
+    if (blk % xcd_stride) return;
+    const int band = sk.nb - 1 - blk / xcd_stride;
+    const int lane = threadIdx.x;
+    const int nx = sk.nx, pad = sk.pad(band), last = sk.rows(band) - 1;
+    const int nf = sk.nst / F;
+    const long L = sk.size();
+    const int j = band * kWave + lane;
+    const bool hn = j + 1 < sk.ny, has_above = band < sk.nb - 1, has_below = band > 0;
+    constexpr int NW = TagWords<T>::n;
+    const u64 *bin = lk.bnd + (long)(has_above ? band : 0) * nx * NW;
+    u64 *bout = lk.bnd + (long)(has_below ? band - 1 : 0) * nx * NW;
+    u64 *scratch = lk.scratch + ((long)band * kWave + lane) * 8;
+
+    T A[5][F], B[5][F];
+    Raw<T> bn{};
+    T bv[F + 1];
+    T cur = 0, nw = 0, nv = 0, ne = 0;
+
+    // backward chunk c covers stored chunk nf-1-c, visited from its end
+    auto load_data = [&](T(&buf)[5][F], int c) {
+        const long e = sk.index(band, lane, max(nf - 1 - c, 0) * F);
+        vload<T, F>(buf[0], wz + e);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) vload<T, F>(buf[1 + k], du + k * L + e);
+    };
+    auto bcol = [&](int c) { return nx - 1 - c * F - lane; };
+    auto bneed = [&](int c) {
+        const int col = bcol(c);
+        return has_above && lane <= F && col >= 0 && col < nx;
+    };
+    auto load_bnd = [&](Raw<T> &b, int c) {
+        raw_load<T>(b, bin + (long)min(max(bcol(c), 0), nx - 1) * NW);
+    };
+    // The slow path reloads into a temporary and copies it back, so the fast
+    // path's loads keep a fixed number of younger memory operations.
+    auto ready = [&](Raw<T> &b, int c) -> bool {
+        if (!has_above || __all(!bneed(c) || raw_ok<T>(b, lk.gen))) return true;
+        Raw<T> t{};
+        const bool ok = slow_wait(lk, [&] { return !bneed(c) || raw_ok<T>(t, lk.gen); },
+                                  [&] { load_bnd(t, c); });
+        b = t;
+        return ok;
+    };
+    auto take = [&](const Raw<T> &b) {
+        const T val = raw_value(b);
+#pragma unroll
+        for (int q = 0; q <= F; ++q) bv[q] = read_lane(val, q);
+    };
+    auto chunk = [&](T(&cb)[5][F], T(&nbuf)[5][F], int c) -> bool {
+        load_data(nbuf, c + 1);
+        load_bnd(bn, c + 1);
+        const int sc = nf - 1 - c;
+        T outv[F];
+#pragma unroll
+        for (int s = 0; s < F; ++s) {
+            const int q = F - 1 - s;
+            const int i = sc * F + q - pad - 2 * lane;
+            const T dn = lane_from_above(cur);
+            ne = nv;
+            nv = nw;
+            nw = lane == last ? (has_above ? bv[s + 1] : T(0)) : dn;  // column i-1 above
+            const bool hw = i > 0, he = i + 1 < nx;
+            const T v = backward_point<T>(cb[0][q], cb[1][q], cb[2][q], cb[3][q], cb[4][q],
+                                          he ? cur : T(0), (hn && hw) ? nw : T(0),
+                                          hn ? nv : T(0), (hn && he) ? ne : T(0));
+            outv[q] = v;
+            cur = v;
+        }
+        if (c + 1 < nf) {
+            if (!ready(bn, c + 1)) return false;
+            take(bn);
+        }
+        vstore<T, F>(wz + sk.index(band, lane, sc * F), outv);
+        {  // publish lane 0's results: step s = lane, column sc*F + F-1-lane - pad
+            T rev[F];
+#pragma unroll
+            for (int s = 0; s < F; ++s) rev[s] = outv[F - 1 - s];
+            const T pv = spread_from_lane<T, F>(rev, 0, lane);
+            const int col = sc * F + F - 1 - lane - pad;
+            const bool ok = has_below && lane < F && col >= 0 && col < nx;
+            put_tagged(ok ? bout + (long)col * NW : scratch, lk.gen, pv);
+        }
+        return true;
+    };
+
+    load_data(A, 0);
+    load_bnd(bn, 0);
+    if (!ready(bn, 0)) return;
+    take(bn);
+    nw = (lane == last && has_above) ? bv[0] : T(0);  // column nx-1 of the row above
+    for (int c = 0; c < nf; c += 2) {
+        if (!chunk(A, B, c)) return;
+        if (c + 1 >= nf) break;
+        if (!chunk(B, A, c + 1)) return;
+    }
+}
+
+// --------------------------------------------------------------------------
+// lu2d9 factorization: A (skewed copy) -> dl, du (skewed).  The edge row
+// carries the four du values of each column.
+// --------------------------------------------------------------------------
+template <typename T, int F>
+__global__ __launch_bounds__(kWave) void k_factor(const T *__restrict__ a, T *__restrict__ dl,
+                                                  T *__restrict__ du, T alpha0, Skew sk, Link lk0,
+                                                  int xcd_stride)
+{
+    Link lk = lk0;
+    lk.gen = launch_gen(lk0);
+    const int blk = blockIdx.x;
+    if (blk % xcd_stride) return;
+    const int band = blk / xcd_stride;
+    const int lane = threadIdx.x;
+    const int nx = sk.nx, ny = sk.ny, pad = sk.pad(band), rows = sk.rows(band);
+    const int nf = sk.nst / F;
+    const long L = sk.size();
+    const int j = band * kWave + lane;
+    const bool hs = j > 0, has_up = band > 0, has_down = band < sk.nb - 1;
+    constexpr int NW = TagWords<T>::n;
+    constexpr int CW = 4 * NW;
+    const u64 *bin = lk.bnd + (long)(has_up ? band - 1 : 0) * nx * CW;
+    u64 *bout = lk.bnd + (long)band * nx * CW;
+    u64 *scratch = lk.scratch + ((long)band * kWave + lane) * 8;
+
+    T A[9][F], B[9][F];
+    Raw<T> bn[4] = {};
+    T bv[4][F + 1];
+    T cur[4] = {0, 0, 0, 0}, se[4] = {0, 0, 0, 0}, sv[4] = {0, 0, 0, 0}, sw[4] = {0, 0, 0, 0};
+    int bad = 0;
+
+    auto load_data = [&](T(&buf)[9][F], int c) {
+        const long e = sk.index(band, lane, min(c, nf - 1) * F);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) vload<T, F>(buf[k], a + k * L + e);
+    };
+    auto bcol = [&](int c) { return c * F - pad + lane; };
+    auto bneed = [&](int c) {
+        const int col = bcol(c);
+        return has_up && lane <= F && col >= 0 && col < nx;
+    };
+    auto load_bnd = [&](Raw<T>(&b)[4], int c) {
+        const u64 *p = bin + (long)min(max(bcol(c), 0), nx - 1) * CW;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) raw_load<T>(b[k], p + k * NW);
+    };
+    auto ok4 = [&](const Raw<T>(&b)[4]) {
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) ok = ok && raw_ok<T>(b[k], lk.gen);
+        return ok;
+    };
+    auto ready = [&](Raw<T>(&b)[4], int c) -> bool {
+        if (!has_up || __all(!bneed(c) || ok4(b))) return true;
+        Raw<T> t[4] = {};
+        const bool ok = slow_wait(lk, [&] { return !bneed(c) || ok4(t); }, [&] { load_bnd(t, c); });
+#pragma unroll
+        for (int k = 0; k < 4; ++k) b[k] = t[k];
+        return ok;
+    };
+    auto take = [&](const Raw<T>(&b)[4]) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const T val = raw_value(b[k]);
+#pragma unroll
+            for (int q = 0; q <= F; ++q) bv[k][q] = read_lane(val, q);
+        }
+    };
+    auto chunk = [&](T(&cb)[9][F], T(&nbuf)[9][F], int c) -> bool {
+        load_data(nbuf, c + 1);
+        load_bnd(bn, c + 1);
+        T oL[5][F], oU[4][F];
+#pragma unroll
+        for (int s = 0; s < F; ++s) {
+            const int i = c * F + s - pad - 2 * lane;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const T up = lane_from_below(cur[k]);
+                sw[k] = sv[k];
+                sv[k] = se[k];
+                se[k] = lane == 0 ? (has_up ? bv[k][s + 1] : T(0)) : up;
+            }
+            const bool hw = i > 0, he = i + 1 < nx;
+            T b9[9], dw[4], ds[4], dse[4], dsw[4], Lv[5], Uv[4];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) b9[k] = cb[k][s];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                dw[k] = hw ? cur[k] : T(0);
+                ds[k] = hs ? sv[k] : T(0);
+                dse[k] = (hs && he) ? se[k] : T(0);
+                dsw[k] = (hs && hw) ? sw[k] : T(0);
+            }
+            factor_point<T>(b9, alpha_at<T>(i, j, nx, ny, alpha0), dw, ds, dse, dsw, Lv, Uv);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) oL[k][s] = Lv[k];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                oU[k][s] = Uv[k];
+                cur[k] = Uv[k];
+            }
+            bad |= (lane < rows && i >= 0 && i < nx && bad_pivot(Lv[0])) ? 1 : 0;
+        }
+        if (c + 1 < nf) {
+            if (!ready(bn, c + 1)) return false;
+            take(bn);
+        }
+        const long e = sk.index(band, lane, c * F);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) vstore<T, F>(dl + k * L + e, oL[k]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) vstore<T, F>(du + k * L + e, oU[k]);
+        {
+            const int col = c * F + lane - pad - 2 * (kWave - 1);
+            const bool ok = has_down && lane < F && col >= 0 && col < nx;
+            u64 *p = ok ? bout + (long)col * CW : scratch;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                put_tagged(p + (ok ? k * NW : 0), lk.gen,
+                           spread_from_lane<T, F>(oU[k], kWave - 1, lane));
+        }
+        return true;
+    };
+
+    load_data(A, 0);
+    load_bnd(bn, 0);
+    bool ok = ready(bn, 0);
+    if (ok) {
+        take(bn);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) se[k] = (lane == 0 && has_up) ? bv[k][0] : T(0);
+        for (int c = 0; c < nf; c += 2) {
+            ok = chunk(A, B, c);
+            if (!ok || c + 1 >= nf) break;
+            ok = chunk(B, A, c + 1);
+            if (!ok) break;
+        }
+    }
+    if (__any(bad))
+        if (bad) __hip_atomic_store(&lk.st->err_pivot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// --------------------------------------------------------------------------
+// layout conversions and the update
+// --------------------------------------------------------------------------
+// nk arrays, reference layout (stride nx*ny) -> skewed (stride sk.size());
+// padding entries get zero.  One thread per skewed entry.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_to_skew(const TI *__restrict__ src, TO *__restrict__ dst,
+                                                 int nk, Skew sk)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const long L = sk.size();
+    if (e >= L) return;
+    const int s = (int)(e % sk.S);
+    const long t1 = e / sk.S;
+    const int l = (int)(t1 % 64);
+    const long t2 = t1 / 64;
+    const int c = (int)(t2 % sk.nch);
+    const int b = (int)(t2 / sk.nch);
+    const int u = c * sk.S + s;
+    const int j = b * 64 + l;
+    const int i = u - sk.pad(b) - 2 * l;
+    const bool real = j < sk.ny && i >= 0 && i < sk.nx;
+    const long N = (long)sk.nx * sk.ny;
+    const long p = real ? (long)j * sk.nx + i : 0;
+    for (int k = 0; k < nk; ++k) dst[k * L + e] = real ? (TO)src[k * N + p] : TO(0);
+}
+
+// skewed -> reference layout, one thread per point.
+template <typename T>
+__global__ __launch_bounds__(256) void k_from_skew(const T *__restrict__ src, T *__restrict__ dst,
+                                                   int nk, Skew sk)
+{
+    const long N = (long)sk.nx * sk.ny;
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= N) return;
+    const int j = (int)(p / sk.nx), i = (int)(p - (long)j * sk.nx);
+    const long e = sk.of_point(i, j);
+    const long L = sk.size();
+    for (int k = 0; k < nk; ++k) dst[k * N + p] = src[k * L + e];
+}
+
+// var = z + var (msi2d9-v03.f90:181), z skewed, var in the reference layout.
+template <typename T>
+__global__ __launch_bounds__(256) void k_update(T *__restrict__ x, const T *__restrict__ z, Skew sk,
+                                                const SolveState *st)
+{
+    if (st->done) return;
+    const long N = (long)sk.nx * sk.ny;
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p >= N) return;
+    const int j = (int)(p / sk.nx), i = (int)(p - (long)j * sk.nx);
+    x[p] = z[sk.of_point(i, j)] + x[p];
+}
+
+// Clear the boundary words (generations restart with every solve).
+__global__ void k_clear_words(u64 *p, long n)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = 0ull;
+}
+
+// ---- launchers -------------------------------------------------------------
+static inline unsigned blocks_for(long n) { return (unsigned)((n + 255) / 256); }
+
+long boundary_words(int nx, int ny) { return (long)((ny + 63) / 64) * nx * 8; }
+long scratch_words(int ny) { return (long)((ny + 63) / 64) * 64 * 8; }
+
+void launch_clear_boundary(u64 *bnd, int nx, int ny, hipStream_t s)
+{
+    const long n = boundary_words(nx, ny);
+    hipLaunchKernelGGL(k_clear_words, dim3(blocks_for(n)), dim3(256), 0, s, bnd, n);
+}
+
+template <typename T>
+constexpr int sweep_steps()
+{
+    return 64 / (int)sizeof(T);  // 64 bytes per lane and array per chunk
+}
+template <typename T>
+constexpr int factor_steps()
+{
+    return 32 / (int)sizeof(T);
+}
+
+Skew skew_for(int nx, int ny, int value_bytes) { return Skew::make(nx, ny, 64 / value_bytes); }
+
+template <typename T>
+void launch_factor(const T *a_sk, T *dl, T *du, T alpha0, const Skew &sk, Link lk, int xcd_stride,
+                   hipStream_t s)
+{
+    hipLaunchKernelGGL((k_factor<T, factor_steps<T>()>), dim3(sk.nb * xcd_stride), dim3(kWave), 0,
+                       s, a_sk, dl, du, alpha0, sk, lk, xcd_stride);
+}
+
+template <typename T>
+void launch_forward(const T *r, const T *dl, T *w, const Skew &sk, Link lk, int xcd_stride,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL((k_forward<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride), dim3(kWave), 0,
+                       s, r, dl, w, sk, lk, xcd_stride);
+}
+
+template <typename T>
+void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_backward<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride), dim3(kWave),
+                       0, s, wz, du, sk, lk, xcd_stride);
+}
+
+template <typename TI, typename TO>
+void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_to_skew<TI, TO>), dim3(blocks_for(sk.size())), dim3(256), 0, s, src, dst,
+                       nk, sk);
+}
+
+template <typename T>
+void launch_from_skew(const T *src, T *dst, int nk, const Skew &sk, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_from_skew<T>, dim3(blocks_for((long)sk.nx * sk.ny)), dim3(256), 0, s, src,
+                       dst, nk, sk);
+}
+
+template <typename T>
+void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_update<T>, dim3(blocks_for((long)sk.nx * sk.ny)), dim3(256), 0, s, x, z,
+                       sk, st);
+}
+
+#define M2D_INST(T)                                                                           \
+    template void launch_factor<T>(const T *, T *, T *, T, const Skew &, Link, int, hipStream_t); \
+    template void launch_forward<T>(const T *, const T *, T *, const Skew &, Link, int,          \
+                                    hipStream_t);                                             \
+    template void launch_backward<T>(T *, const T *, const Skew &, Link, int, hipStream_t);     \
+    template void launch_from_skew<T>(const T *, T *, int, const Skew &, hipStream_t);          \
+    template void launch_update<T>(T *, const T *, const Skew &, const SolveState *, hipStream_t);
+M2D_INST(double)
+M2D_INST(float)
+template void launch_to_skew<double, double>(const double *, double *, int, const Skew &,
+                                             hipStream_t);
+template void launch_to_skew<double, float>(const double *, float *, int, const Skew &,
+                                            hipStream_t);
+
+}  // namespace m2d

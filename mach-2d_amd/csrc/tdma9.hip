@@ -292,6 +292,11 @@ int norm_l1_device(const double *a, const double *b, const double *var, int nx, 
     hipLaunchKernelGGL(k_norm_l1_partial, dim3(gx, gy), dim3(256), 0, s, a, b, var, nx, ny, pa,
                        pb);
     hipLaunchKernelGGL(k_sum2, dim3(1), dim3(256), 0, s, pa, pb, np, res);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        fprintf(stderr, "mach2d_hip: norm kernels failed to launch: %s\n", hipGetErrorString(e));
+        return M2D_ERR_HIP;
+    }
     if (hipMemcpyAsync(h_out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
         return M2D_ERR_HIP;
     if (hipStreamSynchronize(s) != hipSuccess) return M2D_ERR_HIP;

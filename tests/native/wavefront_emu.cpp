@@ -1,19 +1,23 @@
 // wavefront_emu.cpp -- CPU emulation of the gfx950 wavefront kernels
-// (mach-2d_amd/csrc/msi9.hip: k_factor, k_forward, k_backward).
+// (mach-2d_amd/csrc/sweep.hip: k_factor, k_forward, k_backward, k_to_skew,
+// k_from_skew, k_update).
 //
-// Test infrastructure.  Each band (one wave of 64 lanes) is a state machine
-// stepping through exactly the kernel's loop: the one-lane shuffle of the
-// previous step's result, the history shift, the boundary-row read of the
-// first/last lane, the per-point formula (shared header msi_stencil.h) and
-// the progress publication.  A scheduler interleaves bands in a seeded
-// random order and lets a band take a step only when the wait condition the
-// kernel would spin on holds, so the emulation checks that
-//   * every boundary-row value a band reads was covered by a publication of
-//     its producer band (else: abort with a message),
+// Test infrastructure.  The solver's own arrays live in the skewed layout of
+// skew.h (same header as the kernels).  Each band (one wave of 64 lanes)
+// runs the kernel's chunk loop: per-lane vector loads of F consecutive
+// stored steps, F steps with the one-lane shift of the previous result and
+// two registers of history, the neighbouring band's boundary values held by
+// lanes 0..F (step s uses lane s+1), per-lane stores of the F results, and
+// the generation-tagged publication of the edge lane's F results at the end
+// of the chunk.  A scheduler interleaves bands chunk by chunk in a seeded
+// random order; a band may start a chunk only when every boundary word it
+// needs carries the current generation (the kernel's bounded wait), so the
+// emulation checks that
+//   * no value is consumed before its producer published it,
 //   * the protocol cannot deadlock (some band can always move),
-//   * every array index stays inside its buffer,
-// and produces the same numbers the kernels compute, for a bitwise
-// comparison with the oracle.
+//   * every skewed index, boundary word and scratch word stays in range,
+// and yields the numbers the kernels compute, for a bitwise comparison with
+// the oracle.
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -25,234 +29,402 @@
 #include <vector>
 
 #include "msi_stencil.h"
+#include "skew.h"
 
 using namespace m2d;
 
 namespace {
 
 constexpr int W64 = 64;
+constexpr int FS = 8;  // sweep steps per chunk (FP64)
+constexpr int FF = 4;  // factor steps per chunk (FP64)
 
-struct Fail {
-    char msg[256];
-};
-Fail g_fail;
+char g_msg[256];
 bool g_failed = false;
 
 void fail(const char *m, long a = 0, long b = 0)
 {
-    if (!g_failed) snprintf(g_fail.msg, sizeof(g_fail.msg), "%s (%ld, %ld)", m, a, b);
+    if (!g_failed) snprintf(g_msg, sizeof(g_msg), "%s (%ld, %ld)", m, a, b);
     g_failed = true;
 }
 
-template <typename T>
 struct Buf {
-    std::vector<T> v;
+    std::vector<double> v;
     const char *name;
-    Buf(size_t n, const char *nm) : v(n, T(0)), name(nm) {}
-    T &at(long k)
+    Buf(size_t n, const char *nm) : v(n, 0.0), name(nm) {}
+    double &at(long k)
     {
+        static double dummy;
         if (k < 0 || k >= (long)v.size()) {
             fail(name, k, (long)v.size());
-            static T dummy;
             return dummy;
         }
         return v[(size_t)k];
     }
 };
 
-// Progress publications: progress[band] = columns published.
-struct Sync {
-    std::vector<int> progress;
-    int K;
-};
-
-// ---- factor ----------------------------------------------------------------
-struct FactorBand {
-    int band, nb, rows, tend, t = 0;
-    double cur[W64][4], se[W64][4], s[W64][4], sw[W64][4];
-    FactorBand(int b, int nb_, int nx, int ny) : band(b), nb(nb_)
+// Boundary words (value + generation) per (slot, column, k) plus per-lane
+// scratch, as the kernels' buffer.
+struct Bnd {
+    int nx, per, nb;
+    std::vector<double> val;
+    std::vector<unsigned> gen;
+    long scratch_writes = 0;
+    Bnd(int nb_, int nx_, int per_)
+        : nx(nx_), per(per_), nb(nb_), val((size_t)nb_ * nx_ * per_, 0.0),
+          gen((size_t)nb_ * nx_ * per_, 0u) {}
+    long idx(int slot, int col, int k) const
     {
-        rows = std::min(W64, ny - b * W64);
-        tend = nx - 1 + 2 * (rows - 1);
-        memset(cur, 0, sizeof(cur));
-        memset(se, 0, sizeof(se));
-        memset(s, 0, sizeof(s));
-        memset(sw, 0, sizeof(sw));
+        if (slot < 0 || slot >= nb || col < 0 || col >= nx || k < 0 || k >= per) return -1;
+        return ((long)slot * nx + col) * per + k;
     }
-    bool done() const { return t > tend; }
-    bool can_step(const Sync &sy, int nx) const
+    void put(int slot, int col, int k, unsigned g, double v)
     {
-        if (band == 0) return true;
-        return sy.progress[band - 1] >= std::min(t + 2, nx);
+        const long q = idx(slot, col, k);
+        if (q < 0) return fail("boundary put out of range", slot, col);
+        val[(size_t)q] = v;
+        gen[(size_t)q] = g;
     }
-    void step(Buf<double> &a, Buf<double> &dl, Buf<double> &du, int nx, int ny, double alpha0,
-              Sync &sy)
+    void put_scratch(int band, int lane)
     {
-        const long N = (long)nx * ny;
-        if (t == 0 && band > 0) {  // the kernel primes before its loop
-            if (sy.progress[band - 1] < 1) fail("factor: unpublished prime", band, 0);
-            for (int k = 0; k < 4; ++k) se[0][k] = du.at(k * N + (long)(band * W64) * nx - nx);
+        if (band < 0 || band >= nb || lane < 0 || lane >= W64) fail("scratch out of range");
+        ++scratch_writes;
+    }
+    bool ok(int slot, int col, int k, unsigned g) const
+    {
+        const long q = idx(slot, col, k);
+        return q >= 0 && gen[(size_t)q] == g;
+    }
+    double get_clamped(int slot, int col, int k) const
+    {
+        col = std::min(std::max(col, 0), nx - 1);
+        const long q = idx(slot, col, k);
+        if (q < 0) {
+            fail("boundary get out of range", slot, col);
+            return 0.0;
         }
-        double up[W64][4];
-        for (int l = 0; l < W64; ++l)
-            for (int k = 0; k < 4; ++k) up[l][k] = (l == 0) ? cur[0][k] : cur[l - 1][k];
-        for (int l = 0; l < W64; ++l)
-            for (int k = 0; k < 4; ++k) {
-                sw[l][k] = s[l][k];
-                s[l][k] = se[l][k];
-                se[l][k] = up[l][k];
-            }
-        {
-            const int j = band * W64;
-            const bool ok = band > 0 && t + 1 < nx;
-            for (int k = 0; k < 4; ++k) {
-                if (ok) {
-                    if (sy.progress[band - 1] < t + 2) fail("factor: unpublished read", band, t);
-                    se[0][k] = du.at(k * N + (long)j * nx - nx + t + 1);
-                } else {
-                    se[0][k] = 0.0;
-                }
-            }
-        }
-        for (int l = 0; l < rows; ++l) {
-            const int i = t - 2 * l, j = band * W64 + l;
-            if (i < 0 || i >= nx) continue;
-            const long p = (long)j * nx + i;
-            double b[9], dw[4], ds[4], dse[4], dsw[4], L[5], U[4];
-            for (int k = 0; k < 9; ++k) b[k] = a.at(k * N + p);
-            const bool hw = i > 0, hs = j > 0, he = i + 1 < nx;
-            for (int k = 0; k < 4; ++k) {
-                dw[k] = hw ? cur[l][k] : 0.0;
-                ds[k] = hs ? s[l][k] : 0.0;
-                dse[k] = (hs && he) ? se[l][k] : 0.0;
-                dsw[k] = (hs && hw) ? sw[l][k] : 0.0;
-            }
-            factor_point<double>(b, alpha_at<double>(i, j, nx, ny, alpha0), dw, ds, dse, dsw, L,
-                                 U);
-            for (int k = 0; k < 5; ++k) dl.at(k * N + p) = L[k];
-            for (int k = 0; k < 4; ++k) {
-                du.at(k * N + p) = U[k];
-                cur[l][k] = U[k];
-            }
-            if (l == W64 - 1 && band < nb - 1 && ((i + 1) % sy.K == 0 || i == nx - 1))
-                sy.progress[band] = i + 1;
-        }
-        ++t;
+        return val[(size_t)q];
     }
 };
 
 // ---- forward ---------------------------------------------------------------
-struct FwdBand {
-    int band, nb, rows, tend, t = 0;
-    double cur[W64], se[W64], s[W64], sw[W64];
-    FwdBand(int b, int nb_, int nx, int ny) : band(b), nb(nb_)
+struct Fwd {
+    const Skew &sk;
+    int band, pad, nf, c = 0, phase = -1;
+    bool has_up, has_down;
+    double cur[W64], se[W64], sv[W64], sw[W64], bv[FS + 1], outv[W64][FS];
+    Fwd(const Skew &k, int b) : sk(k), band(b)
     {
-        rows = std::min(W64, ny - b * W64);
-        tend = nx - 1 + 2 * (rows - 1);
-        for (int l = 0; l < W64; ++l) cur[l] = se[l] = s[l] = sw[l] = 0.0;
+        pad = sk.pad(b);
+        nf = sk.nst / FS;
+        has_up = b > 0;
+        has_down = b < sk.nb - 1;
+        for (int l = 0; l < W64; ++l) cur[l] = se[l] = sv[l] = sw[l] = 0.0;
     }
-    bool done() const { return t > tend; }
-    bool can_step(const Sync &sy, int nx) const
+    bool done() const { return c >= nf; }
+    int bcol(int cc, int lane) const { return cc * FS - pad + lane; }
+    bool bneed(int cc, int lane) const
     {
-        return band == 0 || sy.progress[band - 1] >= std::min(t + 2, nx);
+        const int col = bcol(cc, lane);
+        return has_up && lane <= FS && col >= 0 && col < sk.nx;
     }
-    void step(Buf<double> &r, Buf<double> &dl, Buf<double> &w, int nx, int ny, Sync &sy)
+    bool valid(const Bnd &bd, unsigned g, int cc) const
     {
-        const long N = (long)nx * ny;
-        if (t == 0 && band > 0) {
-            if (sy.progress[band - 1] < 1) fail("forward: unpublished prime", band, 0);
-            se[0] = w.at((long)(band * W64) * nx - nx);
+        if (cc >= nf) return true;
+        for (int lane = 0; lane <= FS; ++lane)
+            if (bneed(cc, lane) && !bd.ok(band - 1, bcol(cc, lane), 0, g)) return false;
+        return true;
+    }
+    // phase -1: the prime waits for chunk 0's boundary; phase 0: loads and
+    // steps of chunk c; phase 1: waits for chunk c+1's boundary, then
+    // takes it, stores and publishes chunk c.
+    bool can_run(const Bnd &bd, unsigned g) const
+    {
+        return phase == 0 || valid(bd, g, phase < 0 ? 0 : c + 1);
+    }
+    void take(const Bnd &bd, int cc)
+    {
+        for (int q = 0; q <= FS; ++q) bv[q] = bd.get_clamped(has_up ? band - 1 : 0, bcol(cc, q), 0);
+    }
+    void run(Buf &r, Buf &dl, Buf &w, Bnd &bd, unsigned g)
+    {
+        const long L = sk.size();
+        if (phase < 0) {
+            take(bd, 0);
+            se[0] = has_up ? bv[0] : 0.0;
+            phase = 0;
+            return;
         }
-        double up[W64];
-        for (int l = 0; l < W64; ++l) up[l] = (l == 0) ? cur[0] : cur[l - 1];
+        if (phase == 1) return finish(w, bd, g);
+        double cb[W64][6][FS];
         for (int l = 0; l < W64; ++l) {
-            sw[l] = s[l];
-            s[l] = se[l];
-            se[l] = up[l];
+            const long e = sk.index(band, l, std::min(c, nf - 1) * FS);
+            for (int s = 0; s < FS; ++s) {
+                cb[l][0][s] = r.at(e + s);
+                for (int k = 0; k < 5; ++k) cb[l][1 + k][s] = dl.at(k * L + e + s);
+            }
         }
-        if (band > 0 && t + 1 < nx) {
-            if (sy.progress[band - 1] < t + 2) fail("forward: unpublished read", band, t);
-            se[0] = w.at((long)(band * W64) * nx - nx + t + 1);
-        } else {
-            se[0] = 0.0;
+        for (int s = 0; s < FS; ++s) {
+            double up[W64];
+            for (int l = 0; l < W64; ++l) up[l] = l == 0 ? 0.0 : cur[l - 1];
+            for (int l = 0; l < W64; ++l) {
+                sw[l] = sv[l];
+                sv[l] = se[l];
+                se[l] = l == 0 ? (has_up ? bv[s + 1] : 0.0) : up[l];
+            }
+            for (int l = 0; l < W64; ++l) {
+                const int j = band * W64 + l, i = c * FS + s - pad - 2 * l;
+                const bool hs = j > 0, hw = i > 0, he = i + 1 < sk.nx;
+                const double v = forward_point<double>(
+                    cb[l][0][s], cb[l][1][s], cb[l][2][s], cb[l][3][s], cb[l][4][s], cb[l][5][s],
+                    hw ? cur[l] : 0.0, (hs && he) ? se[l] : 0.0, hs ? sv[l] : 0.0,
+                    (hs && hw) ? sw[l] : 0.0);
+                outv[l][s] = v;
+                cur[l] = v;
+            }
         }
-        for (int l = 0; l < rows; ++l) {
-            const int i = t - 2 * l, j = band * W64 + l;
-            if (i < 0 || i >= nx) continue;
-            const long p = (long)j * nx + i;
-            const bool hw = i > 0, hs = j > 0, he = i + 1 < nx;
-            const double v = forward_point<double>(
-                r.at(p), dl.at(p), dl.at(N + p), dl.at(2 * N + p), dl.at(3 * N + p),
-                dl.at(4 * N + p), hw ? cur[l] : 0.0, (hs && he) ? se[l] : 0.0, hs ? s[l] : 0.0,
-                (hs && hw) ? sw[l] : 0.0);
-            w.at(p) = v;
-            cur[l] = v;
-            if (l == W64 - 1 && band < nb - 1 && ((i + 1) % sy.K == 0 || i == nx - 1))
-                sy.progress[band] = i + 1;
+        phase = 1;
+    }
+    void finish(Buf &w, Bnd &bd, unsigned g)
+    {
+        if (c + 1 < nf) take(bd, c + 1);  // chunk c+1's boundary, before storing chunk c
+        const int cc = c++;
+        phase = 0;
+        for (int l = 0; l < W64; ++l) {
+            const long e = sk.index(band, l, cc * FS);
+            for (int s = 0; s < FS; ++s) w.at(e + s) = outv[l][s];
         }
-        ++t;
+        for (int lane = 0; lane < W64; ++lane) {
+            const int col = cc * FS + lane - pad - 2 * (W64 - 1);
+            const bool ok = has_down && lane < FS && col >= 0 && col < sk.nx;
+            if (ok)
+                bd.put(band, col, 0, g, outv[W64 - 1][lane]);
+            else
+                bd.put_scratch(band, lane);
+        }
     }
 };
 
 // ---- backward --------------------------------------------------------------
-struct BwdBand {
-    int band, nb, rows, last, send, s_ = 0;
-    double cur[W64], nw[W64], n[W64], ne[W64];
-    BwdBand(int b, int nb_, int nx, int ny) : band(b), nb(nb_)
+struct Bwd {
+    const Skew &sk;
+    int band, pad, nf, last, c = 0, phase = -1;
+    bool has_above, has_below;
+    double cur[W64], nw[W64], nv[W64], ne[W64], bv[FS + 1], outv[W64][FS];
+    Bwd(const Skew &k, int b) : sk(k), band(b)
     {
-        rows = std::min(W64, ny - b * W64);
-        last = rows - 1;
-        send = nx - 1 + 2 * last;
-        for (int l = 0; l < W64; ++l) cur[l] = nw[l] = n[l] = ne[l] = 0.0;
+        pad = sk.pad(b);
+        nf = sk.nst / FS;
+        last = sk.rows(b) - 1;
+        has_above = b < sk.nb - 1;
+        has_below = b > 0;
+        for (int l = 0; l < W64; ++l) cur[l] = nw[l] = nv[l] = ne[l] = 0.0;
     }
-    bool done() const { return s_ > send; }
-    bool can_step(const Sync &sy, int nx) const
+    bool done() const { return c >= nf; }
+    int bcol(int cc, int lane) const { return sk.nx - 1 - cc * FS - lane; }
+    bool bneed(int cc, int lane) const
     {
-        return band == nb - 1 || sy.progress[band + 1] >= std::min(s_ + 2, nx);
+        const int col = bcol(cc, lane);
+        return has_above && lane <= FS && col >= 0 && col < sk.nx;
     }
-    void step(Buf<double> &wz, Buf<double> &du, Buf<double> &x, int nx, int ny, Sync &sy)
+    bool valid(const Bnd &bd, unsigned g, int cc) const
     {
-        const long N = (long)nx * ny;
-        const int s = s_;
-        if (s == 0 && band < nb - 1) {
-            if (sy.progress[band + 1] < 1) fail("backward: unpublished prime", band, 0);
-            nw[last] = wz.at((long)(band * W64 + last) * nx + nx + (nx - 1));
+        if (cc >= nf) return true;
+        for (int lane = 0; lane <= FS; ++lane)
+            if (bneed(cc, lane) && !bd.ok(band, bcol(cc, lane), 0, g)) return false;
+        return true;
+    }
+    bool can_run(const Bnd &bd, unsigned g) const
+    {
+        return phase == 0 || valid(bd, g, phase < 0 ? 0 : c + 1);
+    }
+    void take(const Bnd &bd, int cc)
+    {
+        for (int q = 0; q <= FS; ++q) bv[q] = bd.get_clamped(has_above ? band : 0, bcol(cc, q), 0);
+    }
+    void run(Buf &wz, Buf &du, Bnd &bd, unsigned g)
+    {
+        const long L = sk.size();
+        if (phase < 0) {
+            take(bd, 0);
+            nw[last] = has_above ? bv[0] : 0.0;
+            phase = 0;
+            return;
         }
-        double dn[W64];
-        for (int l = 0; l < W64; ++l) dn[l] = (l == W64 - 1) ? cur[W64 - 1] : cur[l + 1];
+        if (phase == 1) return finish(wz, bd, g);
+        const int sc = nf - 1 - c;
+        double cb[W64][5][FS];
         for (int l = 0; l < W64; ++l) {
-            ne[l] = n[l];
-            n[l] = nw[l];
-            nw[l] = dn[l];
+            const long e = sk.index(band, l, std::max(nf - 1 - c, 0) * FS);
+            for (int s = 0; s < FS; ++s) {
+                cb[l][0][s] = wz.at(e + s);
+                for (int k = 0; k < 4; ++k) cb[l][1 + k][s] = du.at(k * L + e + s);
+            }
         }
-        if (band < nb - 1 && nx - 2 - s >= 0) {
-            if (sy.progress[band + 1] < s + 2) fail("backward: unpublished read", band, s);
-            const long rowbase = (long)(band * W64 + last) * nx;
-            nw[last] = wz.at(rowbase + nx + (nx - 2 - s));
-        } else {
-            nw[last] = 0.0;
+        for (int s = 0; s < FS; ++s) {
+            const int q = FS - 1 - s;
+            double dn[W64];
+            for (int l = 0; l < W64; ++l) dn[l] = l == W64 - 1 ? 0.0 : cur[l + 1];
+            for (int l = 0; l < W64; ++l) {
+                ne[l] = nv[l];
+                nv[l] = nw[l];
+                nw[l] = l == last ? (has_above ? bv[s + 1] : 0.0) : dn[l];
+            }
+            for (int l = 0; l < W64; ++l) {
+                const int j = band * W64 + l, i = sc * FS + q - pad - 2 * l;
+                const bool hn = j + 1 < sk.ny, hw = i > 0, he = i + 1 < sk.nx;
+                const double v = backward_point<double>(
+                    cb[l][0][q], cb[l][1][q], cb[l][2][q], cb[l][3][q], cb[l][4][q],
+                    he ? cur[l] : 0.0, (hn && hw) ? nw[l] : 0.0, hn ? nv[l] : 0.0,
+                    (hn && he) ? ne[l] : 0.0);
+                outv[l][q] = v;
+                cur[l] = v;
+            }
         }
-        for (int l = 0; l < rows; ++l) {
-            const int i = nx - 1 - s + 2 * (last - l), j = band * W64 + l;
-            if (i < 0 || i >= nx) continue;
-            const long p = (long)j * nx + i;
-            const bool hn = j + 1 < ny, hw = i > 0, he = i + 1 < nx;
-            const double v = backward_point<double>(
-                wz.at(p), du.at(p), du.at(N + p), du.at(2 * N + p), du.at(3 * N + p),
-                he ? cur[l] : 0.0, (hn && hw) ? nw[l] : 0.0, hn ? n[l] : 0.0,
-                (hn && he) ? ne[l] : 0.0);
-            wz.at(p) = v;
-            x.at(p) = v + x.at(p);
-            cur[l] = v;
-            if (l == 0 && band > 0 && ((nx - i) % sy.K == 0 || i == 0))
-                sy.progress[band] = nx - i;
+        phase = 1;
+    }
+    void finish(Buf &wz, Bnd &bd, unsigned g)
+    {
+        const int sc = nf - 1 - c;
+        if (c + 1 < nf) take(bd, c + 1);
+        ++c;
+        phase = 0;
+        for (int l = 0; l < W64; ++l) {
+            const long e = sk.index(band, l, sc * FS);
+            for (int s = 0; s < FS; ++s) wz.at(e + s) = outv[l][s];
         }
-        ++s_;
+        for (int lane = 0; lane < W64; ++lane) {
+            const int col = sc * FS + FS - 1 - lane - pad;
+            const bool ok = has_below && lane < FS && col >= 0 && col < sk.nx;
+            if (ok)
+                bd.put(band - 1, col, 0, g, outv[0][FS - 1 - lane]);
+            else
+                bd.put_scratch(band, lane);
+        }
     }
 };
 
-template <typename Band, typename StepFn>
-bool schedule(std::vector<Band> &bands, Sync &sy, int nx, unsigned seed, StepFn fn)
+// ---- factor ----------------------------------------------------------------
+struct Fac {
+    const Skew &sk;
+    int band, pad, nf, rows, c = 0, phase = -1;
+    bool has_up, has_down;
+    double cur[W64][4], se[W64][4], sv[W64][4], sw[W64][4], bv[4][FF + 1];
+    double oL[W64][5][FF], oU[W64][4][FF];
+    Fac(const Skew &k, int b) : sk(k), band(b)
+    {
+        pad = sk.pad(b);
+        nf = sk.nst / FF;
+        rows = sk.rows(b);
+        has_up = b > 0;
+        has_down = b < sk.nb - 1;
+        memset(cur, 0, sizeof(cur));
+        memset(se, 0, sizeof(se));
+        memset(sv, 0, sizeof(sv));
+        memset(sw, 0, sizeof(sw));
+    }
+    bool done() const { return c >= nf; }
+    int bcol(int cc, int lane) const { return cc * FF - pad + lane; }
+    bool bneed(int cc, int lane) const
+    {
+        const int col = bcol(cc, lane);
+        return has_up && lane <= FF && col >= 0 && col < sk.nx;
+    }
+    bool valid(const Bnd &bd, unsigned g, int cc) const
+    {
+        if (cc >= nf) return true;
+        for (int lane = 0; lane <= FF; ++lane)
+            for (int k = 0; k < 4; ++k)
+                if (bneed(cc, lane) && !bd.ok(band - 1, bcol(cc, lane), k, g)) return false;
+        return true;
+    }
+    bool can_run(const Bnd &bd, unsigned g) const
+    {
+        return phase == 0 || valid(bd, g, phase < 0 ? 0 : c + 1);
+    }
+    void take(const Bnd &bd, int cc)
+    {
+        for (int k = 0; k < 4; ++k)
+            for (int q = 0; q <= FF; ++q)
+                bv[k][q] = bd.get_clamped(has_up ? band - 1 : 0, bcol(cc, q), k);
+    }
+    void run(Buf &a, Buf &dl, Buf &du, double alpha0, Bnd &bd, unsigned g)
+    {
+        const long L = sk.size();
+        if (phase < 0) {
+            take(bd, 0);
+            for (int k = 0; k < 4; ++k) se[0][k] = has_up ? bv[k][0] : 0.0;
+            phase = 0;
+            return;
+        }
+        if (phase == 1) return finish(dl, du, bd, g);
+        double cb[W64][9][FF];
+        for (int l = 0; l < W64; ++l) {
+            const long e = sk.index(band, l, std::min(c, nf - 1) * FF);
+            for (int s = 0; s < FF; ++s)
+                for (int k = 0; k < 9; ++k) cb[l][k][s] = a.at(k * L + e + s);
+        }
+        for (int s = 0; s < FF; ++s) {
+            double up[W64][4];
+            for (int l = 0; l < W64; ++l)
+                for (int k = 0; k < 4; ++k) up[l][k] = l == 0 ? 0.0 : cur[l - 1][k];
+            for (int l = 0; l < W64; ++l)
+                for (int k = 0; k < 4; ++k) {
+                    sw[l][k] = sv[l][k];
+                    sv[l][k] = se[l][k];
+                    se[l][k] = l == 0 ? (has_up ? bv[k][s + 1] : 0.0) : up[l][k];
+                }
+            for (int l = 0; l < W64; ++l) {
+                const int j = band * W64 + l, i = c * FF + s - pad - 2 * l;
+                const bool hs = j > 0, hw = i > 0, he = i + 1 < sk.nx;
+                double b9[9], dw[4], ds[4], dse[4], dsw[4], Lv[5], Uv[4];
+                for (int k = 0; k < 9; ++k) b9[k] = cb[l][k][s];
+                for (int k = 0; k < 4; ++k) {
+                    dw[k] = hw ? cur[l][k] : 0.0;
+                    ds[k] = hs ? sv[l][k] : 0.0;
+                    dse[k] = (hs && he) ? se[l][k] : 0.0;
+                    dsw[k] = (hs && hw) ? sw[l][k] : 0.0;
+                }
+                factor_point<double>(b9, alpha_at<double>(i, j, sk.nx, sk.ny, alpha0), dw, ds,
+                                     dse, dsw, Lv, Uv);
+                for (int k = 0; k < 5; ++k) oL[l][k][s] = Lv[k];
+                for (int k = 0; k < 4; ++k) {
+                    oU[l][k][s] = Uv[k];
+                    cur[l][k] = Uv[k];
+                }
+                (void)rows;
+            }
+        }
+        phase = 1;
+    }
+    void finish(Buf &dl, Buf &du, Bnd &bd, unsigned g)
+    {
+        const long L = sk.size();
+        if (c + 1 < nf) take(bd, c + 1);
+        const int cc = c++;
+        phase = 0;
+        for (int l = 0; l < W64; ++l) {
+            const long e = sk.index(band, l, cc * FF);
+            for (int s = 0; s < FF; ++s) {
+                for (int k = 0; k < 5; ++k) dl.at(k * L + e + s) = oL[l][k][s];
+                for (int k = 0; k < 4; ++k) du.at(k * L + e + s) = oU[l][k][s];
+            }
+        }
+        for (int lane = 0; lane < W64; ++lane) {
+            const int col = cc * FF + lane - pad - 2 * (W64 - 1);
+            const bool ok = has_down && lane < FF && col >= 0 && col < sk.nx;
+            for (int k = 0; k < 4; ++k) {
+                if (ok)
+                    bd.put(band, col, k, g, oU[W64 - 1][k][lane]);
+                else
+                    bd.put_scratch(band, lane);
+            }
+        }
+    }
+};
+
+template <typename Band, typename Run>
+void schedule(std::vector<Band> &bands, const Bnd &bd, unsigned g, unsigned seed, Run run)
 {
     std::mt19937 rng(seed);
     for (;;) {
@@ -264,76 +436,109 @@ bool schedule(std::vector<Band> &bands, Sync &sy, int nx, unsigned seed, StepFn 
             Band &b = bands[(size_t)q];
             if (b.done()) continue;
             all_done = false;
-            const int burst = 1 + (int)(rng() % 7);
-            for (int m = 0; m < burst && !b.done() && b.can_step(sy, nx); ++m) {
-                fn(b);
+            const int burst = 1 + (int)(rng() % 5);
+            for (int m = 0; m < burst && !b.done() && b.can_run(bd, g); ++m) {
+                run(b);
                 moved = true;
-                if (g_failed) return false;
+                if (g_failed) return;
             }
         }
-        if (all_done) return true;
-        if (!moved) {
-            fail("deadlock: no band can move");
-            return false;
-        }
+        if (all_done) return;
+        if (!moved) return fail("deadlock: no band can move");
     }
+}
+
+// reference layout <-> skewed layout (k_to_skew / k_from_skew)
+void to_skew(const double *src, Buf &dst, int nk, const Skew &sk)
+{
+    const long L = sk.size(), N = (long)sk.nx * sk.ny;
+    for (long e = 0; e < L; ++e) {
+        const int s = (int)(e % sk.S);
+        const long t1 = e / sk.S;
+        const int l = (int)(t1 % 64);
+        const long t2 = t1 / 64;
+        const int c = (int)(t2 % sk.nch), b = (int)(t2 / sk.nch);
+        const int u = c * sk.S + s, j = b * 64 + l, i = u - sk.pad(b) - 2 * l;
+        const bool real = j < sk.ny && i >= 0 && i < sk.nx;
+        for (int k = 0; k < nk; ++k)
+            dst.at(k * L + e) = real ? src[k * N + (long)j * sk.nx + i] : 0.0;
+    }
+}
+void from_skew(Buf &src, double *dst, int nk, const Skew &sk)
+{
+    const long L = sk.size(), N = (long)sk.nx * sk.ny;
+    for (int j = 0; j < sk.ny; ++j)
+        for (int i = 0; i < sk.nx; ++i) {
+            const long e = sk.of_point(i, j);
+            for (int k = 0; k < nk; ++k) dst[k * N + (long)j * sk.nx + i] = src.at(k * L + e);
+        }
 }
 
 }  // namespace
 
 extern "C" {
 
-const char *emu_error(void) { return g_failed ? g_fail.msg : ""; }
+const char *emu_error(void) { return g_failed ? g_msg : ""; }
 
-// lu2d9 through the emulated wavefront kernel.  Returns 0 on success.
+// lu2d9 through the emulated factor kernel (A and the factor round-trip
+// through the skewed layout exactly as the handle does).
 int emu_lu2d9(const double *a_in, double *dl_out, double *du_out, int nx, int ny, double alpha0,
-              int K, unsigned seed)
+              int unused, unsigned seed)
 {
+    (void)unused;
     g_failed = false;
-    const long N = (long)nx * ny;
-    Buf<double> a(9 * N, "a"), dl(5 * N, "dl"), du(4 * N, "du");
-    memcpy(a.v.data(), a_in, sizeof(double) * 9 * N);
-    const int nb = (ny + W64 - 1) / W64;
-    std::vector<FactorBand> bands;
-    for (int b = 0; b < nb; ++b) bands.emplace_back(b, nb, nx, ny);
-    Sync sy{std::vector<int>(nb, 0), K};
-    schedule(bands, sy, nx, seed, [&](FactorBand &b) { b.step(a, dl, du, nx, ny, alpha0, sy); });
-    memcpy(dl_out, dl.v.data(), sizeof(double) * 5 * N);
-    memcpy(du_out, du.v.data(), sizeof(double) * 4 * N);
+    const Skew sk = Skew::make(nx, ny, 8);
+    const long L = sk.size();
+    Buf a(9 * L, "a_sk"), dl(5 * L, "dl_sk"), du(4 * L, "du_sk");
+    to_skew(a_in, a, 9, sk);
+    Bnd bd(sk.nb, nx, 4);
+    std::vector<Fac> bands;
+    for (int b = 0; b < sk.nb; ++b) bands.emplace_back(sk, b);
+    schedule(bands, bd, 1u, seed, [&](Fac &b) { b.run(a, dl, du, alpha0, bd, 1u); });
+    from_skew(dl, dl_out, 5, sk);
+    from_skew(du, du_out, 4, sk);
     return g_failed ? 1 : 0;
 }
 
-// One MSI iteration (forward + backward + update) through the emulated
-// kernels, given r.  w_out receives z (in place over w, as the kernel).
+// One MSI iteration: r (reference layout) -> skewed, forward, backward,
+// var = z + var.  The boundary buffer persists across calls for one grid
+// (stale words of earlier generations stay in it); generations 2*gen+1 and
+// 2*gen+2.
 int emu_msi_iteration(const double *r_in, const double *dl_in, const double *du_in,
-                      double *x_io, double *w_out, int nx, int ny, int K, unsigned seed)
+                      double *x_io, double *w_out, int nx, int ny, int gen, unsigned seed)
 {
     g_failed = false;
-    const long N = (long)nx * ny;
-    Buf<double> r(N, "r"), dl(5 * N, "dl"), du(4 * N, "du"), w(N, "w"), x(N, "x");
-    memcpy(r.v.data(), r_in, sizeof(double) * N);
-    memcpy(dl.v.data(), dl_in, sizeof(double) * 5 * N);
-    memcpy(du.v.data(), du_in, sizeof(double) * 4 * N);
-    memcpy(x.v.data(), x_io, sizeof(double) * N);
-    const int nb = (ny + W64 - 1) / W64;
+    const Skew sk = Skew::make(nx, ny, 8);
+    const long L = sk.size(), N = (long)nx * ny;
+    Buf r(L, "r_sk"), dl(5 * L, "dl_sk"), du(4 * L, "du_sk"), w(L, "w_sk");
+    to_skew(r_in, r, 1, sk);
+    to_skew(dl_in, dl, 5, sk);
+    to_skew(du_in, du, 4, sk);
+    static std::vector<Bnd> keep;
+    if (keep.empty() || keep[0].nx != nx || keep[0].nb != sk.nb) keep.assign(1, Bnd(sk.nb, nx, 1));
+    Bnd &bd = keep[0];
+    const unsigned g1 = (unsigned)(2 * gen + 1), g2 = g1 + 1;
     {
-        std::vector<FwdBand> bands;
-        for (int b = 0; b < nb; ++b) bands.emplace_back(b, nb, nx, ny);
-        Sync sy{std::vector<int>(nb, 0), K};
-        schedule(bands, sy, nx, seed, [&](FwdBand &b) { b.step(r, dl, w, nx, ny, sy); });
+        std::vector<Fwd> bands;
+        for (int b = 0; b < sk.nb; ++b) bands.emplace_back(sk, b);
+        schedule(bands, bd, g1, seed, [&](Fwd &b) { b.run(r, dl, w, bd, g1); });
     }
     if (!g_failed) {
-        std::vector<BwdBand> bands;
-        for (int b = 0; b < nb; ++b) bands.emplace_back(b, nb, nx, ny);
-        Sync sy{std::vector<int>(nb, 0), K};
-        schedule(bands, sy, nx, seed + 1, [&](BwdBand &b) { b.step(w, du, x, nx, ny, sy); });
+        std::vector<Bwd> bands;
+        for (int b = 0; b < sk.nb; ++b) bands.emplace_back(sk, b);
+        schedule(bands, bd, g2, seed + 1, [&](Bwd &b) { b.run(w, du, bd, g2); });
     }
-    memcpy(x_io, x.v.data(), sizeof(double) * N);
-    memcpy(w_out, w.v.data(), sizeof(double) * N);
+    // k_update: var = z + var
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const long p = (long)j * nx + i;
+            x_io[p] = w.at(sk.of_point(i, j)) + x_io[p];
+        }
+    from_skew(w, w_out, 1, sk);
+    (void)N;
     return g_failed ? 1 : 0;
 }
 
-// Residual of the 2-D aware kernel formula.
 void emu_residual(const double *a, const double *x, const double *rhs, double *r, int nx, int ny)
 {
     const long N = (long)nx * ny;
