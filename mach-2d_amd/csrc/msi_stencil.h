@@ -19,7 +19,7 @@
 namespace m2d {
 
 // Diagonal slots, 0-based: the reference's 1=SW 2=S 3=SE 4=W 5=P 6=E 7=NW 8=N 9=NE.
-enum { SW = 0, S = 1, SE = 2, W = 3, P = 4, E = 5, NW = 6, N = 7, NE = 8 };
+enum Slot { kSW = 0, kS = 1, kSE = 2, kW = 3, kP = 4, kE = 5, kNW = 6, kN = 7, kNE = 8 };
 
 // Incomplete factorization of one point: src/solvers/msi2d9-v03.f90:356-369.
 // b[9]: row of A at P.  du_w/du_s/du_se/du_sw: U entries (E, NW, N, NE) of
@@ -30,22 +30,22 @@ M2D_HD void factor_point(const T *b, T alpha, const T *du_w, const T *du_s,
                          const T *du_se, const T *du_sw, T *dl, T *du)
 {
     const T one = T(1), two = T(2);
-    T dl5 = b[SW];
-    T dl4 = (b[S] - dl5 * du_sw[0] - alpha * b[SE] * du_se[0]) /
+    T dl5 = b[kSW];
+    T dl4 = (b[kS] - dl5 * du_sw[0] - alpha * b[kSE] * du_se[0]) /
             (one - alpha * du_s[0] * du_se[0]);
-    T dl3 = b[SE] - dl4 * du_s[0];
-    T dl2 = (b[W] - dl4 * du_s[1] - dl5 * du_sw[2] - two * alpha * dl5 * du_sw[1]) /
+    T dl3 = b[kSE] - dl4 * du_s[0];
+    T dl2 = (b[kW] - dl4 * du_s[1] - dl5 * du_sw[2] - two * alpha * dl5 * du_sw[1]) /
             (one + two * alpha * du_w[1]);
     T phi1 = dl3 * du_se[0];
     T phi2 = dl5 * du_sw[1];
     T phi3 = dl3 * du_se[3];
     T phi4 = dl2 * du_w[1];
-    T dl1 = b[P] - dl2 * du_w[0] - dl3 * du_se[1] - dl4 * du_s[2] - dl5 * du_sw[3] +
+    T dl1 = b[kP] - dl2 * du_w[0] - dl3 * du_se[1] - dl4 * du_s[2] - dl5 * du_sw[3] +
             alpha * (two * phi1 + phi2 + phi3 + two * phi4);
-    du[0] = (b[E] - dl3 * du_se[2] - dl4 * du_s[3] - two * alpha * (phi1 + phi3)) / dl1;
-    du[1] = (b[NW] - dl2 * du_w[2]) / dl1;
-    du[2] = (b[N] - dl2 * du_w[3] - alpha * phi4) / dl1;
-    du[3] = b[NE] / dl1;
+    du[0] = (b[kE] - dl3 * du_se[2] - dl4 * du_s[3] - two * alpha * (phi1 + phi3)) / dl1;
+    du[1] = (b[kNW] - dl2 * du_w[2]) / dl1;
+    du[2] = (b[kN] - dl2 * du_w[3] - alpha * phi4) / dl1;
+    du[3] = b[kNE] / dl1;
     dl[0] = dl1;
     dl[1] = dl2;
     dl[2] = dl3;
@@ -87,15 +87,15 @@ M2D_HD TR residual_point(const TA *a, long ld, const TX *x, TR rhs, int i, int j
     const long p = (long)j * nx + i;
     const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
     TR acc = -rhs;
-    if (hs && hw) acc += (TR)a[SW * ld + p] * (TR)x[p - nx - 1];
-    if (hs) acc += (TR)a[S * ld + p] * (TR)x[p - nx];
-    if (hs && he) acc += (TR)a[SE * ld + p] * (TR)x[p - nx + 1];
-    if (hw) acc += (TR)a[W * ld + p] * (TR)x[p - 1];
-    acc += (TR)a[P * ld + p] * (TR)x[p];
-    if (he) acc += (TR)a[E * ld + p] * (TR)x[p + 1];
-    if (hn && hw) acc += (TR)a[NW * ld + p] * (TR)x[p + nx - 1];
-    if (hn) acc += (TR)a[N * ld + p] * (TR)x[p + nx];
-    if (hn && he) acc += (TR)a[NE * ld + p] * (TR)x[p + nx + 1];
+    if (hs && hw) acc += (TR)a[kSW * ld + p] * (TR)x[p - nx - 1];
+    if (hs) acc += (TR)a[kS * ld + p] * (TR)x[p - nx];
+    if (hs && he) acc += (TR)a[kSE * ld + p] * (TR)x[p - nx + 1];
+    if (hw) acc += (TR)a[kW * ld + p] * (TR)x[p - 1];
+    acc += (TR)a[kP * ld + p] * (TR)x[p];
+    if (he) acc += (TR)a[kE * ld + p] * (TR)x[p + 1];
+    if (hn && hw) acc += (TR)a[kNW * ld + p] * (TR)x[p + nx - 1];
+    if (hn) acc += (TR)a[kN * ld + p] * (TR)x[p + nx];
+    if (hn && he) acc += (TR)a[kNE * ld + p] * (TR)x[p + nx + 1];
     return acc;
 }
 

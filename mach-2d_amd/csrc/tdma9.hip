@@ -74,9 +74,9 @@ __global__ __launch_bounds__(kTile) void k_tdma_x(const T *__restrict__ a, const
             const long rb = (long)j * nx;
             for (int ci = 0; ci < cols; ++ci) {
                 const int i = c0 + ci;
-                const T a1 = ta[SW][lane][ci], a2 = ta[S][lane][ci], a3 = ta[SE][lane][ci];
-                const T a4 = ta[W][lane][ci], a5 = ta[P][lane][ci], a6 = ta[E][lane][ci];
-                const T a7 = ta[NW][lane][ci], a8 = ta[N][lane][ci], a9 = ta[NE][lane][ci];
+                const T a1 = ta[kSW][lane][ci], a2 = ta[kS][lane][ci], a3 = ta[kSE][lane][ci];
+                const T a4 = ta[kW][lane][ci], a5 = ta[kP][lane][ci], a6 = ta[kE][lane][ci];
+                const T a7 = ta[kNW][lane][ci], a8 = ta[kN][lane][ci], a9 = ta[kNE][lane][ci];
                 const T bp = tb[lane][ci];
                 // neighbour x: tx[row - j0 + 1][col - c0 + 1]; entries outside the
                 // grid are never staged and never used by the branch taken
@@ -146,9 +146,9 @@ __global__ __launch_bounds__(kTile) void k_tdma_y(const T *__restrict__ a, const
     T Pm = 0, Qm = 0;
     for (int j = 0; j < ny; ++j) {
         const long p = (long)j * nx + i;
-        const T a1 = a[SW * nn + p], a2 = a[S * nn + p], a3 = a[SE * nn + p];
-        const T a4 = a[W * nn + p], a5 = a[P * nn + p], a6 = a[E * nn + p];
-        const T a7 = a[NW * nn + p], a8 = a[N * nn + p], a9 = a[NE * nn + p];
+        const T a1 = a[kSW * nn + p], a2 = a[kS * nn + p], a3 = a[kSE * nn + p];
+        const T a4 = a[kW * nn + p], a5 = a[kP * nn + p], a6 = a[kE * nn + p];
+        const T a7 = a[kNW * nn + p], a8 = a[kN * nn + p], a9 = a[kNE * nn + p];
         const T bp = b[p];
         // neighbour x (the pass input); out-of-grid reads are never used
         const T xw = west ? T(0) : xin[p - 1];
@@ -230,11 +230,11 @@ __global__ __launch_bounds__(256) void k_norm_l1_partial(const double *__restric
     double ra = 0.0, rbv = 0.0;
     if (i <= nx - 2) {
         const long p = (long)j * nx + i;
-        const double s = a[SW * nn + p] * var[p - nx - 1] + a[S * nn + p] * var[p - nx] +
-                         a[SE * nn + p] * var[p - nx + 1] + a[W * nn + p] * var[p - 1] +
-                         a[P * nn + p] * var[p] + a[E * nn + p] * var[p + 1] +
-                         a[NW * nn + p] * var[p + nx - 1] + a[N * nn + p] * var[p + nx] +
-                         a[NE * nn + p] * var[p + nx + 1] - b[p];
+        const double s = a[kSW * nn + p] * var[p - nx - 1] + a[kS * nn + p] * var[p - nx] +
+                         a[kSE * nn + p] * var[p - nx + 1] + a[kW * nn + p] * var[p - 1] +
+                         a[kP * nn + p] * var[p] + a[kE * nn + p] * var[p + 1] +
+                         a[kNW * nn + p] * var[p + nx - 1] + a[kN * nn + p] * var[p + nx] +
+                         a[kNE * nn + p] * var[p + nx + 1] - b[p];
         ra = fabs(s);
         rbv = fabs(b[p]);
     }
@@ -362,12 +362,12 @@ static int host_norm(int nx, int ny, const double *var, const double *b, const d
     const long nn = (long)nx * ny;
     const int nw = norm_l1_work_size(nx, ny);
     double *d = nullptr;
-    if (hipMalloc((void **)&d, sizeof(double) * (11 * N + nw)) != hipSuccess) return M2D_ERR_HIP;
-    double *da = d, *db = d + 9 * N, *dv = d + 10 * N, *dw = d + 11 * N;
+    if (hipMalloc((void **)&d, sizeof(double) * (11 * nn + nw)) != hipSuccess) return M2D_ERR_HIP;
+    double *da = d, *db = d + 9 * nn, *dv = d + 10 * nn, *dw = d + 11 * nn;
     int rc = M2D_OK;
-    if (hipMemcpy(da, a, sizeof(double) * 9 * N, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(db, b, sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dv, var, sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(da, a, sizeof(double) * 9 * nn, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(db, b, sizeof(double) * nn, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dv, var, sizeof(double) * nn, hipMemcpyHostToDevice) != hipSuccess)
         rc = M2D_ERR_HIP;
     if (!rc) rc = norm_l1_device(da, db, dv, nx, ny, dw, out, nullptr);
     (void)hipFree(d);
