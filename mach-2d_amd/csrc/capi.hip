@@ -40,7 +40,7 @@ template <typename T>
 struct Work {
     Skew sk{};
     T *a = nullptr;   // 9 arrays: skewed copy of A (factor input)
-    T *dl = nullptr;  // 5 arrays
+    T *dl = nullptr;  // 6 arrays: dl1..dl5, 1/dl1
     T *du = nullptr;  // 4 arrays
     T *r = nullptr;   // residual
     T *w = nullptr;   // forward result, then z
@@ -57,7 +57,7 @@ struct Work {
         const long L = sk.size();
         const size_t z = sizeof(T) * (size_t)L;
         bool ok = hipMalloc((void **)&a, 9 * z) == hipSuccess &&
-                  hipMalloc((void **)&dl, 5 * z) == hipSuccess &&
+                  hipMalloc((void **)&dl, 6 * z) == hipSuccess &&
                   hipMalloc((void **)&du, 4 * z) == hipSuccess &&
                   hipMalloc((void **)&r, z) == hipSuccess && hipMalloc((void **)&w, z) == hipSuccess;
         if (ok)  // padding entries are never consumed; start them finite
@@ -291,6 +291,7 @@ extern "C" int m2d_factor(m2d_system *s, const double *d_a)
             launch_factor<double>(W.a, W.dl, W.du, 0.5, W.sk, make_link(s, s->st, 0), s->xcd_stride,
                                   s->stream);
         });
+        launch_prepare_factor<double>(W.dl, W.du, W.sk, s->stream);
     } else {
         Work<float> &W = s->w32;
         launch_convert<double, float>(d_a, s->a32, 9 * s->N, s->stream);
@@ -299,6 +300,7 @@ extern "C" int m2d_factor(m2d_system *s, const double *d_a)
             launch_factor<float>(W.a, W.dl, W.du, 0.5f, W.sk, make_link(s, s->st, 0),
                                  s->xcd_stride, s->stream);
         });
+        launch_prepare_factor<float>(W.dl, W.du, W.sk, s->stream);
     }
     HIP_OK(hipGetLastError());
     int rc = read_state(s, s->st);
@@ -326,6 +328,7 @@ static int put_factor(m2d_system *s, const double *d_dl, const double *d_du)
 {
     launch_to_skew<double, double>(d_dl, s->w64.dl, 5, s->w64.sk, s->stream);
     launch_to_skew<double, double>(d_du, s->w64.du, 4, s->w64.sk, s->stream);
+    launch_prepare_factor<double>(s->w64.dl, s->w64.du, s->w64.sk, s->stream);
     HIP_OK(hipGetLastError());
     s->factored = true;
     return M2D_OK;

@@ -49,6 +49,9 @@ template <typename T>
 void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s);
 template <typename TI, typename TO>
 void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t s);
+// dl: 6 arrays (dl1..dl5, then 1/dl1, written here); padding made finite.
+template <typename T>
+void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s);
 template <typename T>
 void launch_from_skew(const T *src, T *dst, int nk, const Skew &sk, hipStream_t s);
 template <typename T>

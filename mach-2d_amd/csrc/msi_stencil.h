@@ -61,12 +61,33 @@ M2D_HD T forward_point(T r, T dl1, T dl2, T dl3, T dl4, T dl5, T w_w, T w_se, T 
     return -(r + dl2 * w_w + dl3 * w_se + dl4 * w_s + dl5 * w_sw) / dl1;
 }
 
-// Same, multiplying by a precomputed reciprocal of dl1 (fast precisions only).
+M2D_HD double fma_t(double a, double b, double c) { return fma(a, b, c); }
+M2D_HD float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+
+// n / d from y = 1/d (correctly rounded): q0 = n*y, then two corrections
+// q <- q + (n - q*d)*y with exact remainders.  After the first correction q
+// is within one ulp of n/d, and Markstein's theorem (y within half an ulp of
+// 1/d, q within one ulp of n/d) makes the second one return the correctly
+// rounded quotient, i.e. exactly n / d, for normal operands and quotient.
+// Only the sign of a zero quotient may differ (n = -0 gives +0).  Five
+// dependent operations once y is known, where the quotient with its own
+// scaling and reciprocal refinement needs about thirteen.
 template <typename T>
-M2D_HD T forward_point_rcp(T r, T rdl1, T dl2, T dl3, T dl4, T dl5, T w_w, T w_se,
-                           T w_s, T w_sw)
+M2D_HD T div_by_rcp(T n, T d, T y)
 {
-    return -(r + dl2 * w_w + dl3 * w_se + dl4 * w_s + dl5 * w_sw) * rdl1;
+    T q = n * y;
+    T e = fma_t(-q, d, n);
+    q = fma_t(e, y, q);
+    e = fma_t(-q, d, n);
+    return fma_t(e, y, q);
+}
+
+// forward_point with the quotient taken through y = 1/dl1.
+template <typename T>
+M2D_HD T forward_point_y(T r, T dl1, T y, T dl2, T dl3, T dl4, T dl5, T w_w, T w_se, T w_s,
+                         T w_sw)
+{
+    return div_by_rcp<T>(-(r + dl2 * w_w + dl3 * w_se + dl4 * w_s + dl5 * w_sw), dl1, y);
 }
 
 // Backward substitution U z = w at one point: msi2d9-v03.f90:175.

@@ -16,8 +16,17 @@
 // keeps its F results in registers and stores them with 16-byte stores after
 // the chunk, so the loop body has a fixed, small number of vector memory
 // operations and the only wait per chunk is for loads issued a chunk
-// earlier.  Steps whose column is outside the grid compute on padding and
-// are never consumed: every neighbour value is masked by the grid test.
+// earlier.  The sweeps reload each 16-byte vector of the chunk buffer as
+// soon as its steps are done, so one buffer serves both the current and the
+// next chunk.
+//
+// Padding.  Steps whose column (or row) is outside the grid compute on
+// padding.  In the sweeps the padding holds r = 0, dl1 = 1, 1/dl1 = 1 and
+// zero couplings (k_prepare_factor), so padding results are finite and no
+// neighbour value needs a mask: every factor entry that couples a point to
+// a neighbour outside the grid is an exact zero (the couplings of A to
+// outside neighbours are zero, and the factorization keeps them zero), and
+// zero times a finite value adds nothing.  The factorization still masks.
 //
 // Bands.  A band's edge row travels to the neighbouring band through
 // generation-tagged words (wavefront.h), F columns per chunk: lanes 0..F-1
@@ -96,9 +105,21 @@ __device__ __forceinline__ T spread_from_lane(const T (&v)[F], int src, int lane
     return out;
 }
 
+// Half-chunk buffers of the sweeps.  The F steps of a chunk are two halves
+// of H = F/2 steps; three buffers hold halves h, h+1 and h+2, and the loads
+// of half h+2 are issued at the start of half h into the buffer half h-1
+// used, so every load has a chunk of compute to arrive and never overwrites
+// a live value.  The loop body covers three chunks, after which the buffer
+// roles repeat.  Chunk length equals the skew segment (F == sk.S), so a
+// chunk starts at sk.chunk_base.
+template <typename T, int NA, int H>
+struct Halves {
+    T v[NA][H];
+};
+
 // --------------------------------------------------------------------------
-// forward sweep  L w = -r  (msi2d9-v03.f90:139-156); r, dl, w in the skewed
-// layout.
+// forward sweep  L w = -r  (msi2d9-v03.f90:139-156); r, dl (dl1..dl5, then
+// 1/dl1), w in the skewed layout.
 // --------------------------------------------------------------------------
 template <typename T, int F>
 __global__ __launch_bounds__(kWave) void k_forward(const T *__restrict__ r,
@@ -115,23 +136,31 @@ __global__ __launch_bounds__(kWave) void k_forward(const T *__restrict__ r,
     const int nx = sk.nx, pad = sk.pad(band);
     const int nf = sk.nst / F;  // chunks of F steps
     const long L = sk.size();
-    const int j = band * kWave + lane;
-    const bool hs = j > 0, has_up = band > 0, has_down = band < sk.nb - 1;
+    const bool has_up = band > 0, has_down = band < sk.nb - 1;
     constexpr int NW = TagWords<T>::n;
+    constexpr int H = F / 2;
+    constexpr int V = 16 / (int)sizeof(T);  // values per 16-byte vector
+    typedef Halves<T, 7, H> Buf;            // r, dl1..dl5, 1/dl1
     const u64 *bin = lk.bnd + (long)(has_up ? band - 1 : 0) * nx * NW;
     u64 *bout = lk.bnd + (long)band * nx * NW;
     u64 *scratch = lk.scratch + ((long)band * kWave + lane) * 8;
 
-    T A[6][F], B[6][F];
+    Buf X, Y, Z;
     Raw<T> bn{};
     T bv[F + 1];
     T cur = 0, se = 0, sv = 0, sw = 0;
 
-    auto load_data = [&](T(&buf)[6][F], int c) {
-        const long e = sk.index(band, lane, min(c, nf - 1) * F);
-        vload<T, F>(buf[0], r + e);
+    auto load_half = [&](Buf &b, int h) {  // half h = chunk h/2, elements (h%2)*H..
+        const long e = sk.chunk_base(band, min(h >> 1, nf - 1), lane) + (h & 1) * H;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) vload<T, F>(buf[1 + k], dl + k * L + e);
+        for (int k = 0; k < 7; ++k) {
+            const T *p = (k == 0 ? r : dl + (long)(k - 1) * L) + e;
+#pragma unroll
+            for (int q = 0; q < H / V; ++q) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(p + q * V);
+                __builtin_memcpy(&b.v[k][q * V], &v, 16);
+            }
+        }
     };
     auto bcol = [&](int c) { return c * F - pad + lane; };  // column held by this lane
     auto bneed = [&](int c) {
@@ -141,47 +170,53 @@ __global__ __launch_bounds__(kWave) void k_forward(const T *__restrict__ r,
     auto load_bnd = [&](Raw<T> &b, int c) {
         raw_load<T>(b, bin + (long)min(max(bcol(c), 0), nx - 1) * NW);
     };
-    // The slow path reloads into a temporary and copies it back, so the fast
-    // path's loads keep a fixed number of younger memory operations.
-    auto ready = [&](Raw<T> &b, int c) -> bool {
-        if (!has_up || __all(!bneed(c) || raw_ok<T>(b, lk.gen))) return true;
-        Raw<T> t{};
-        const bool ok = slow_wait(lk, [&] { return !bneed(c) || raw_ok<T>(t, lk.gen); },
-                                  [&] { load_bnd(t, c); });
-        b = t;
-        return ok;
-    };
-    auto take = [&](const Raw<T> &b) {
-        const T val = raw_value(b);
+    // Lane q of chunk c's boundary holds column bcol(c); columns outside the
+    // grid (and every column of the bottom band) read as zero.
+    auto take = [&](const Raw<T> &b, int c) {
+        const T val = bneed(c) ? raw_value(b) : T(0);
 #pragma unroll
         for (int q = 0; q <= F; ++q) bv[q] = read_lane(val, q);
     };
-    // Chunk c: issue the loads of chunk c+1, run the F steps of chunk c, then
-    // take chunk c+1's boundary (any wait there covers only operations issued
-    // a chunk of compute earlier), then store and publish chunk c.
-    auto chunk = [&](T(&cb)[6][F], T(&nbuf)[6][F], int c) -> bool {
-        load_data(nbuf, c + 1);
-        load_bnd(bn, c + 1);
+    // Wait for chunk c's boundary words and take them.  The slow path reloads
+    // into a temporary and takes from it, so no register it loads is live
+    // after the two paths join and the fast path's wait covers only loads
+    // issued before the chunk's data loads.
+    auto sync = [&](const Raw<T> &b, int c) -> bool {
+        if (!has_up || __all(!bneed(c) || raw_ok<T>(b, lk.gen))) {
+            take(b, c);
+            return true;
+        }
+        Raw<T> t{};
+        const bool ok = slow_wait(lk, [&] { return !bneed(c) || raw_ok<T>(t, lk.gen); },
+                                  [&] { load_bnd(t, c); });
+        take(t, c);
+        return ok;
+    };
+    // Chunk c: halves 2c (in h0) and 2c+1 (in h1); loads halves 2c+2 into h2
+    // and 2c+3 into h0.
+    auto chunk = [&](Buf &h0, Buf &h1, Buf &h2, int c) -> bool {
+        load_bnd(bn, c + 1);  // chunk c+1's boundary words, before any data load
         T outv[F];
 #pragma unroll
-        for (int s = 0; s < F; ++s) {
-            const int i = c * F + s - pad - 2 * lane;
-            const T up = lane_from_below(cur);
-            sw = sv;
-            sv = se;
-            se = lane == 0 ? (has_up ? bv[s + 1] : T(0)) : up;  // column i+1 above
-            const bool hw = i > 0, he = i + 1 < nx;
-            const T v = forward_point<T>(cb[0][s], cb[1][s], cb[2][s], cb[3][s], cb[4][s],
-                                         cb[5][s], hw ? cur : T(0), (hs && he) ? se : T(0),
-                                         hs ? sv : T(0), (hs && hw) ? sw : T(0));
-            outv[s] = v;
-            cur = v;
+        for (int hh = 0; hh < 2; ++hh) {
+            Buf &cb = hh == 0 ? h0 : h1;
+            load_half(hh == 0 ? h2 : h0, 2 * c + 2 + hh);
+#pragma unroll
+            for (int t = 0; t < H; ++t) {
+                const int s = hh * H + t;
+                const T up = lane_from_below_or(cur, bv[s + 1]);  // lane 0: column i+1 above
+                sw = sv;
+                sv = se;
+                se = up;
+                const T v = forward_point_y<T>(cb.v[0][t], cb.v[1][t], cb.v[6][t], cb.v[2][t],
+                                               cb.v[3][t], cb.v[4][t], cb.v[5][t], cur, se, sv,
+                                               sw);
+                outv[s] = v;
+                cur = v;
+            }
         }
-        if (c + 1 < nf) {
-            if (!ready(bn, c + 1)) return false;
-            take(bn);
-        }
-        vstore<T, F>(w + sk.index(band, lane, c * F), outv);
+        if (c + 1 < nf && !sync(bn, c + 1)) return false;
+        vstore<T, F>(w + sk.chunk_base(band, c, lane), outv);
         {  // publish the edge lane's F results (columns c*F + q - pad - 126)
             const T pv = spread_from_lane<T, F>(outv, kWave - 1, lane);
             const int col = c * F + lane - pad - 2 * (kWave - 1);
@@ -191,15 +226,15 @@ __global__ __launch_bounds__(kWave) void k_forward(const T *__restrict__ r,
         return true;
     };
 
-    load_data(A, 0);
+    load_half(X, 0);
+    load_half(Y, 1);
     load_bnd(bn, 0);
-    if (!ready(bn, 0)) return;
-    take(bn);
-    se = (lane == 0 && has_up) ? bv[0] : T(0);  // column -pad above (used only if pad = 0)
-    for (int c = 0; c < nf; c += 2) {
-        if (!chunk(A, B, c)) return;
-        if (c + 1 >= nf) break;
-        if (!chunk(B, A, c + 1)) return;
+    if (!sync(bn, 0)) return;
+    se = lane == 0 ? bv[0] : T(0);  // column -pad above (used only if pad = 0)
+    for (int c = 0; c < nf; c += 3) {
+        if (!chunk(X, Y, Z, c) || c + 1 >= nf) break;
+        if (!chunk(Z, X, Y, c + 1) || c + 2 >= nf) break;
+        if (!chunk(Y, Z, X, c + 2)) break;
     }
 }
 
@@ -218,27 +253,35 @@ __global__ __launch_bounds__(kWave) void k_backward(T *__restrict__ wz, const T 
     if (blk % xcd_stride) return;
     const int band = sk.nb - 1 - blk / xcd_stride;
     const int lane = threadIdx.x;
-    const int nx = sk.nx, pad = sk.pad(band), last = sk.rows(band) - 1;
+    const int nx = sk.nx, pad = sk.pad(band);
     const int nf = sk.nst / F;
     const long L = sk.size();
-    const int j = band * kWave + lane;
-    const bool hn = j + 1 < sk.ny, has_above = band < sk.nb - 1, has_below = band > 0;
+    const bool has_above = band < sk.nb - 1, has_below = band > 0;
     constexpr int NW = TagWords<T>::n;
+    constexpr int H = F / 2;
+    constexpr int V = 16 / (int)sizeof(T);
+    typedef Halves<T, 5, H> Buf;  // w, du1..du4
     const u64 *bin = lk.bnd + (long)(has_above ? band : 0) * nx * NW;
     u64 *bout = lk.bnd + (long)(has_below ? band - 1 : 0) * nx * NW;
     u64 *scratch = lk.scratch + ((long)band * kWave + lane) * 8;
 
-    T A[5][F], B[5][F];
+    Buf X, Y, Z;
     Raw<T> bn{};
     T bv[F + 1];
     T cur = 0, nw = 0, nv = 0, ne = 0;
 
-    // backward chunk c covers stored chunk nf-1-c, visited from its end
-    auto load_data = [&](T(&buf)[5][F], int c) {
-        const long e = sk.index(band, lane, max(nf - 1 - c, 0) * F);
-        vload<T, F>(buf[0], wz + e);
+    // backward half h: stored chunk nf-1-h/2, its upper half first
+    auto load_half = [&](Buf &b, int h) {
+        const long e = sk.chunk_base(band, max(nf - 1 - (h >> 1), 0), lane) + (1 - (h & 1)) * H;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) vload<T, F>(buf[1 + k], du + k * L + e);
+        for (int k = 0; k < 5; ++k) {
+            const T *p = (k == 0 ? wz : du + (long)(k - 1) * L) + e;
+#pragma unroll
+            for (int q = 0; q < H / V; ++q) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(p + q * V);
+                __builtin_memcpy(&b.v[k][q * V], &v, 16);
+            }
+        }
     };
     auto bcol = [&](int c) { return nx - 1 - c * F - lane; };
     auto bneed = [&](int c) {
@@ -248,46 +291,48 @@ __global__ __launch_bounds__(kWave) void k_backward(T *__restrict__ wz, const T 
     auto load_bnd = [&](Raw<T> &b, int c) {
         raw_load<T>(b, bin + (long)min(max(bcol(c), 0), nx - 1) * NW);
     };
-    // The slow path reloads into a temporary and copies it back, so the fast
-    // path's loads keep a fixed number of younger memory operations.
-    auto ready = [&](Raw<T> &b, int c) -> bool {
-        if (!has_above || __all(!bneed(c) || raw_ok<T>(b, lk.gen))) return true;
-        Raw<T> t{};
-        const bool ok = slow_wait(lk, [&] { return !bneed(c) || raw_ok<T>(t, lk.gen); },
-                                  [&] { load_bnd(t, c); });
-        b = t;
-        return ok;
-    };
-    auto take = [&](const Raw<T> &b) {
-        const T val = raw_value(b);
+    auto take = [&](const Raw<T> &b, int c) {
+        const T val = bneed(c) ? raw_value(b) : T(0);
 #pragma unroll
         for (int q = 0; q <= F; ++q) bv[q] = read_lane(val, q);
     };
-    auto chunk = [&](T(&cb)[5][F], T(&nbuf)[5][F], int c) -> bool {
-        load_data(nbuf, c + 1);
+    auto sync = [&](const Raw<T> &b, int c) -> bool {
+        if (!has_above || __all(!bneed(c) || raw_ok<T>(b, lk.gen))) {
+            take(b, c);
+            return true;
+        }
+        Raw<T> t{};
+        const bool ok = slow_wait(lk, [&] { return !bneed(c) || raw_ok<T>(t, lk.gen); },
+                                  [&] { load_bnd(t, c); });
+        take(t, c);
+        return ok;
+    };
+    auto chunk = [&](Buf &h0, Buf &h1, Buf &h2, int c) -> bool {
         load_bnd(bn, c + 1);
         const int sc = nf - 1 - c;
         T outv[F];
 #pragma unroll
-        for (int s = 0; s < F; ++s) {
-            const int q = F - 1 - s;
-            const int i = sc * F + q - pad - 2 * lane;
-            const T dn = lane_from_above(cur);
-            ne = nv;
-            nv = nw;
-            nw = lane == last ? (has_above ? bv[s + 1] : T(0)) : dn;  // column i-1 above
-            const bool hw = i > 0, he = i + 1 < nx;
-            const T v = backward_point<T>(cb[0][q], cb[1][q], cb[2][q], cb[3][q], cb[4][q],
-                                          he ? cur : T(0), (hn && hw) ? nw : T(0),
-                                          hn ? nv : T(0), (hn && he) ? ne : T(0));
-            outv[q] = v;
-            cur = v;
+        for (int hh = 0; hh < 2; ++hh) {
+            Buf &cb = hh == 0 ? h0 : h1;
+            load_half(hh == 0 ? h2 : h0, 2 * c + 2 + hh);
+#pragma unroll
+            for (int t = 0; t < H; ++t) {
+                const int s = hh * H + t;
+                const int q = F - 1 - s, qh = H - 1 - t;  // element in the chunk, in the half
+                // lane 63: column i-1 of the band above; the top lane of a
+                // partial band reads a padding row above it (zero couplings)
+                const T dn = lane_from_above_or(cur, bv[s + 1]);
+                ne = nv;
+                nv = nw;
+                nw = dn;
+                const T v = backward_point<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh], cb.v[3][qh],
+                                              cb.v[4][qh], cur, nw, nv, ne);
+                outv[q] = v;
+                cur = v;
+            }
         }
-        if (c + 1 < nf) {
-            if (!ready(bn, c + 1)) return false;
-            take(bn);
-        }
-        vstore<T, F>(wz + sk.index(band, lane, sc * F), outv);
+        if (c + 1 < nf && !sync(bn, c + 1)) return false;
+        vstore<T, F>(wz + sk.chunk_base(band, sc, lane), outv);
         {  // publish lane 0's results: step s = lane, column sc*F + F-1-lane - pad
             T rev[F];
 #pragma unroll
@@ -300,15 +345,15 @@ __global__ __launch_bounds__(kWave) void k_backward(T *__restrict__ wz, const T 
         return true;
     };
 
-    load_data(A, 0);
+    load_half(X, 0);
+    load_half(Y, 1);
     load_bnd(bn, 0);
-    if (!ready(bn, 0)) return;
-    take(bn);
-    nw = (lane == last && has_above) ? bv[0] : T(0);  // column nx-1 of the row above
-    for (int c = 0; c < nf; c += 2) {
-        if (!chunk(A, B, c)) return;
-        if (c + 1 >= nf) break;
-        if (!chunk(B, A, c + 1)) return;
+    if (!sync(bn, 0)) return;
+    nw = lane == kWave - 1 ? bv[0] : T(0);  // column nx-1 of the row above
+    for (int c = 0; c < nf; c += 3) {
+        if (!chunk(X, Y, Z, c) || c + 1 >= nf) break;
+        if (!chunk(Z, X, Y, c + 1) || c + 2 >= nf) break;
+        if (!chunk(Y, Z, X, c + 2)) break;
     }
 }
 
@@ -482,6 +527,36 @@ __global__ __launch_bounds__(256) void k_to_skew(const TI *__restrict__ src, TO 
     for (int k = 0; k < nk; ++k) dst[k * L + e] = real ? (TO)src[k * N + p] : TO(0);
 }
 
+// Sweep-ready factor: dl[5] = 1/dl1 at grid points; at padding entries
+// (outside the grid or beyond the last row) dl1 = 1/dl1 = 1 and every
+// coupling zero, so the sweeps compute finite padding values.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prepare_factor(T *__restrict__ dl, T *__restrict__ du,
+                                                        Skew sk)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const long L = sk.size();
+    if (e >= L) return;
+    const int s = (int)(e % sk.S);
+    const long t1 = e / sk.S;
+    const int l = (int)(t1 % 64);
+    const long t2 = t1 / 64;
+    const int c = (int)(t2 % sk.nch);
+    const int b = (int)(t2 / sk.nch);
+    const int j = b * 64 + l;
+    const int i = c * sk.S + s - sk.pad(b) - 2 * l;
+    if (j < sk.ny && i >= 0 && i < sk.nx) {
+        dl[5 * L + e] = T(1) / dl[e];
+        return;
+    }
+    dl[e] = T(1);
+    dl[5 * L + e] = T(1);
+#pragma unroll
+    for (int k = 1; k < 5; ++k) dl[k * L + e] = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) du[k * L + e] = T(0);
+}
+
 // skewed -> reference layout, one thread per point.
 template <typename T>
 __global__ __launch_bounds__(256) void k_from_skew(const T *__restrict__ src, T *__restrict__ dst,
@@ -564,6 +639,13 @@ void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride
                        0, s, wz, du, sk, lk, xcd_stride);
 }
 
+template <typename T>
+void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_prepare_factor<T>, dim3(blocks_for(sk.size())), dim3(256), 0, s, dl, du,
+                       sk);
+}
+
 template <typename TI, typename TO>
 void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t s)
 {
@@ -591,6 +673,7 @@ void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipSt
                                     hipStream_t);                                             \
     template void launch_backward<T>(T *, const T *, const Skew &, Link, int, hipStream_t);     \
     template void launch_from_skew<T>(const T *, T *, int, const Skew &, hipStream_t);          \
+    template void launch_prepare_factor<T>(T *, T *, const Skew &, hipStream_t);                \
     template void launch_update<T>(T *, const T *, const Skew &, const SolveState *, hipStream_t);
 M2D_INST(double)
 M2D_INST(float)

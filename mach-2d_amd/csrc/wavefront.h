@@ -50,6 +50,40 @@ __device__ __forceinline__ float lane_from_above(float v)
     return __int_as_float(dpp_shl1(__float_as_int(v)));
 }
 
+// Same shifts, with the lane that has no source (lane 0 for the shift from
+// below, lane 63 from above) receiving `edge` instead of 0: the DPP move
+// keeps the destination's old value there (bound_ctrl off).
+__device__ __forceinline__ int dpp_shr1_or(int v, int edge)
+{
+    return __builtin_amdgcn_update_dpp(edge, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_shl1_or(int v, int edge)
+{
+    return __builtin_amdgcn_update_dpp(edge, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ double lane_from_below_or(double v, double edge)
+{
+    const long long b = __double_as_longlong(v), e = __double_as_longlong(edge);
+    const int lo = dpp_shr1_or((int)(b & 0xffffffff), (int)(e & 0xffffffff));
+    const int hi = dpp_shr1_or((int)(b >> 32), (int)(e >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float lane_from_below_or(float v, float edge)
+{
+    return __int_as_float(dpp_shr1_or(__float_as_int(v), __float_as_int(edge)));
+}
+__device__ __forceinline__ double lane_from_above_or(double v, double edge)
+{
+    const long long b = __double_as_longlong(v), e = __double_as_longlong(edge);
+    const int lo = dpp_shl1_or((int)(b & 0xffffffff), (int)(e & 0xffffffff));
+    const int hi = dpp_shl1_or((int)(b >> 32), (int)(e >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float lane_from_above_or(float v, float edge)
+{
+    return __int_as_float(dpp_shl1_or(__float_as_int(v), __float_as_int(edge)));
+}
+
 // Value of lane `src` (wave-uniform constant) broadcast to every lane.
 __device__ __forceinline__ double read_lane(double v, int src)
 {

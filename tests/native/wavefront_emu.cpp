@@ -142,43 +142,44 @@ struct Fwd {
     {
         return phase == 0 || valid(bd, g, phase < 0 ? 0 : c + 1);
     }
+    // columns outside the grid, and every column of the bottom band, read 0
     void take(const Bnd &bd, int cc)
     {
-        for (int q = 0; q <= FS; ++q) bv[q] = bd.get_clamped(has_up ? band - 1 : 0, bcol(cc, q), 0);
+        for (int q = 0; q <= FS; ++q) {
+            const double v = bd.get_clamped(has_up ? band - 1 : 0, bcol(cc, q), 0);
+            bv[q] = bneed(cc, q) ? v : 0.0;
+        }
     }
     void run(Buf &r, Buf &dl, Buf &w, Bnd &bd, unsigned g)
     {
         const long L = sk.size();
         if (phase < 0) {
             take(bd, 0);
-            se[0] = has_up ? bv[0] : 0.0;
+            se[0] = bv[0];
             phase = 0;
             return;
         }
         if (phase == 1) return finish(w, bd, g);
-        double cb[W64][6][FS];
+        double cb[W64][7][FS];  // r, dl1..dl5, 1/dl1
         for (int l = 0; l < W64; ++l) {
-            const long e = sk.index(band, l, std::min(c, nf - 1) * FS);
+            const long e = sk.chunk_base(band, std::min(c, nf - 1), l);
             for (int s = 0; s < FS; ++s) {
                 cb[l][0][s] = r.at(e + s);
-                for (int k = 0; k < 5; ++k) cb[l][1 + k][s] = dl.at(k * L + e + s);
+                for (int k = 0; k < 6; ++k) cb[l][1 + k][s] = dl.at(k * L + e + s);
             }
         }
         for (int s = 0; s < FS; ++s) {
-            double up[W64];
-            for (int l = 0; l < W64; ++l) up[l] = l == 0 ? 0.0 : cur[l - 1];
+            double up[W64];  // one-lane shift; lane 0 keeps the boundary value
+            for (int l = 0; l < W64; ++l) up[l] = l == 0 ? bv[s + 1] : cur[l - 1];
             for (int l = 0; l < W64; ++l) {
                 sw[l] = sv[l];
                 sv[l] = se[l];
-                se[l] = l == 0 ? (has_up ? bv[s + 1] : 0.0) : up[l];
+                se[l] = up[l];
             }
-            for (int l = 0; l < W64; ++l) {
-                const int j = band * W64 + l, i = c * FS + s - pad - 2 * l;
-                const bool hs = j > 0, hw = i > 0, he = i + 1 < sk.nx;
-                const double v = forward_point<double>(
-                    cb[l][0][s], cb[l][1][s], cb[l][2][s], cb[l][3][s], cb[l][4][s], cb[l][5][s],
-                    hw ? cur[l] : 0.0, (hs && he) ? se[l] : 0.0, hs ? sv[l] : 0.0,
-                    (hs && hw) ? sw[l] : 0.0);
+            for (int l = 0; l < W64; ++l) {  // no masks: padding values are finite
+                const double v = forward_point_y<double>(
+                    cb[l][0][s], cb[l][1][s], cb[l][6][s], cb[l][2][s], cb[l][3][s],
+                    cb[l][4][s], cb[l][5][s], cur[l], se[l], sv[l], sw[l]);
                 outv[l][s] = v;
                 cur[l] = v;
             }
@@ -191,7 +192,7 @@ struct Fwd {
         const int cc = c++;
         phase = 0;
         for (int l = 0; l < W64; ++l) {
-            const long e = sk.index(band, l, cc * FS);
+            const long e = sk.chunk_base(band, cc, l);
             for (int s = 0; s < FS; ++s) w.at(e + s) = outv[l][s];
         }
         for (int lane = 0; lane < W64; ++lane) {
@@ -208,14 +209,13 @@ struct Fwd {
 // ---- backward --------------------------------------------------------------
 struct Bwd {
     const Skew &sk;
-    int band, pad, nf, last, c = 0, phase = -1;
+    int band, pad, nf, c = 0, phase = -1;
     bool has_above, has_below;
     double cur[W64], nw[W64], nv[W64], ne[W64], bv[FS + 1], outv[W64][FS];
     Bwd(const Skew &k, int b) : sk(k), band(b)
     {
         pad = sk.pad(b);
         nf = sk.nst / FS;
-        last = sk.rows(b) - 1;
         has_above = b < sk.nb - 1;
         has_below = b > 0;
         for (int l = 0; l < W64; ++l) cur[l] = nw[l] = nv[l] = ne[l] = 0.0;
@@ -240,22 +240,24 @@ struct Bwd {
     }
     void take(const Bnd &bd, int cc)
     {
-        for (int q = 0; q <= FS; ++q) bv[q] = bd.get_clamped(has_above ? band : 0, bcol(cc, q), 0);
+        for (int q = 0; q <= FS; ++q) {
+            const double v = bd.get_clamped(has_above ? band : 0, bcol(cc, q), 0);
+            bv[q] = bneed(cc, q) ? v : 0.0;
+        }
     }
     void run(Buf &wz, Buf &du, Bnd &bd, unsigned g)
     {
         const long L = sk.size();
         if (phase < 0) {
             take(bd, 0);
-            nw[last] = has_above ? bv[0] : 0.0;
+            nw[W64 - 1] = bv[0];
             phase = 0;
             return;
         }
         if (phase == 1) return finish(wz, bd, g);
-        const int sc = nf - 1 - c;
         double cb[W64][5][FS];
         for (int l = 0; l < W64; ++l) {
-            const long e = sk.index(band, l, std::max(nf - 1 - c, 0) * FS);
+            const long e = sk.chunk_base(band, std::max(nf - 1 - c, 0), l);
             for (int s = 0; s < FS; ++s) {
                 cb[l][0][s] = wz.at(e + s);
                 for (int k = 0; k < 4; ++k) cb[l][1 + k][s] = du.at(k * L + e + s);
@@ -263,20 +265,18 @@ struct Bwd {
         }
         for (int s = 0; s < FS; ++s) {
             const int q = FS - 1 - s;
-            double dn[W64];
-            for (int l = 0; l < W64; ++l) dn[l] = l == W64 - 1 ? 0.0 : cur[l + 1];
+            double dn[W64];  // lane 63 keeps the boundary value; the top lane of a
+                             // partial band reads the padding row above it
+            for (int l = 0; l < W64; ++l) dn[l] = l == W64 - 1 ? bv[s + 1] : cur[l + 1];
             for (int l = 0; l < W64; ++l) {
                 ne[l] = nv[l];
                 nv[l] = nw[l];
-                nw[l] = l == last ? (has_above ? bv[s + 1] : 0.0) : dn[l];
+                nw[l] = dn[l];
             }
             for (int l = 0; l < W64; ++l) {
-                const int j = band * W64 + l, i = sc * FS + q - pad - 2 * l;
-                const bool hn = j + 1 < sk.ny, hw = i > 0, he = i + 1 < sk.nx;
-                const double v = backward_point<double>(
-                    cb[l][0][q], cb[l][1][q], cb[l][2][q], cb[l][3][q], cb[l][4][q],
-                    he ? cur[l] : 0.0, (hn && hw) ? nw[l] : 0.0, hn ? nv[l] : 0.0,
-                    (hn && he) ? ne[l] : 0.0);
+                const double v = backward_point<double>(cb[l][0][q], cb[l][1][q], cb[l][2][q],
+                                                        cb[l][3][q], cb[l][4][q], cur[l], nw[l],
+                                                        nv[l], ne[l]);
                 outv[l][q] = v;
                 cur[l] = v;
             }
@@ -290,7 +290,7 @@ struct Bwd {
         ++c;
         phase = 0;
         for (int l = 0; l < W64; ++l) {
-            const long e = sk.index(band, l, sc * FS);
+            const long e = sk.chunk_base(band, sc, l);
             for (int s = 0; s < FS; ++s) wz.at(e + s) = outv[l][s];
         }
         for (int lane = 0; lane < W64; ++lane) {
@@ -448,6 +448,29 @@ void schedule(std::vector<Band> &bands, const Bnd &bd, unsigned g, unsigned seed
     }
 }
 
+// k_prepare_factor: 1/dl1 at grid points; dl1 = 1/dl1 = 1 and zero
+// couplings at padding entries.
+void prepare_factor(Buf &dl, Buf &du, const Skew &sk)
+{
+    const long L = sk.size();
+    for (long e = 0; e < L; ++e) {
+        const int s = (int)(e % sk.S);
+        const long t1 = e / sk.S;
+        const int l = (int)(t1 % 64);
+        const long t2 = t1 / 64;
+        const int c = (int)(t2 % sk.nch), b = (int)(t2 / sk.nch);
+        const int j = b * 64 + l, i = c * sk.S + s - sk.pad(b) - 2 * l;
+        if (j < sk.ny && i >= 0 && i < sk.nx) {
+            dl.at(5 * L + e) = 1.0 / dl.at(e);
+            continue;
+        }
+        dl.at(e) = 1.0;
+        dl.at(5 * L + e) = 1.0;
+        for (int k = 1; k < 5; ++k) dl.at(k * L + e) = 0.0;
+        for (int k = 0; k < 4; ++k) du.at(k * L + e) = 0.0;
+    }
+}
+
 // reference layout <-> skewed layout (k_to_skew / k_from_skew)
 void to_skew(const double *src, Buf &dst, int nk, const Skew &sk)
 {
@@ -510,10 +533,13 @@ int emu_msi_iteration(const double *r_in, const double *dl_in, const double *du_
     g_failed = false;
     const Skew sk = Skew::make(nx, ny, 8);
     const long L = sk.size(), N = (long)nx * ny;
-    Buf r(L, "r_sk"), dl(5 * L, "dl_sk"), du(4 * L, "du_sk"), w(L, "w_sk");
+    Buf r(L, "r_sk"), dl(6 * L, "dl_sk"), du(4 * L, "du_sk"), w(L, "w_sk");
     to_skew(r_in, r, 1, sk);
     to_skew(dl_in, dl, 5, sk);
     to_skew(du_in, du, 4, sk);
+    prepare_factor(dl, du, sk);
+    for (long e = 0; e < L; ++e)  // padding of w starts as garbage, never consumed
+        w.at(e) = 1e300;
     static std::vector<Bnd> keep;
     if (keep.empty() || keep[0].nx != nx || keep[0].nb != sk.nb) keep.assign(1, Bnd(sk.nb, nx, 1));
     Bnd &bd = keep[0];
