@@ -7,15 +7,23 @@ solved by MSI from a zero guess to a 1e-10 residual drop
 solve: factorization (lu2d9) plus the iterations (fb2d9), with the
 coefficients and right-hand side already resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--mode fp64|fp32_ir]
+    python bench.py [--steps K] [--warmup W] [--mode best|fp64|fp32_ir]
 
-Prints one JSON line.  value = solves/s of the timed mode; gbps = the
-algorithmic bytes of the solve, s*N*(18 + 11 + 24*K_it) with s = 8 bytes,
-over the solve time; roofline = the dominant kernel's algorithmic bytes per
-launch over its average launch duration (HIP events on the solver's stream
-around each launch during the timed steps); cpu_baseline = the reference
-MSI (compiled Fortran, one core) timed on a bounded sample of the same
-solve on this host.
+Modes: fp64 = plain FP64 MSI (bit-identical to the reference); fp32_ir =
+FP32 factor and sweeps inside FP64 iterative refinement (FP64 residuals and
+stopping rule).  "best" runs both; value is the FP32-refined solves/s when
+its solution is within 1e-8 relative of the FP64 one, else the FP64 figure,
+and speedup_vs_plain_fp64 compares the two.  The batch entry times 256
+independent p' systems of uneven size (64s x 32s, s log-uniform on [1, 8])
+solved together to the same drop (autotuned placement, graph replays).
+
+Prints one JSON line.  gbps = algorithmic bytes of the solve over its time
+(FP64: 8 N (18 + 11 + 24 K); refinement: 4 N (18 + 24 K_inner) +
+8 N (11 (outer + 1) + 3 outer)); roofline = the dominant kernel's
+algorithmic bytes per launch over its average launch duration (HIP events on
+the solver's stream around each launch during the timed steps);
+cpu_baseline = the reference MSI (compiled Fortran, one core) timed on a
+bounded sample of the same solve on this host.
 """
 from __future__ import annotations
 
@@ -35,12 +43,12 @@ HBM_PEAK_GBS = 8000.0
 GAMMA = 1e-10
 NITM_CAP = 20000
 
-# algorithmic bytes per point of one launch, in units of the value size s
+# algorithmic bytes per point of one launch, in units of the value size
 BYTES_PER_POINT = {
     "factor": 18,    # read a (9), write dl (5) and du (4)
     "forward": 7,    # read r, dl (5); write w
-    "backward": 8,   # read w, du (4), var; write z (over w), var
-    "residual": 12,  # read a (9), rhs, var; write r
+    "backward": 6,   # read w, du (4); write z
+    "residual": 15,  # update: read z, var, write var; residual: read a (9), rhs, var, write r
 }
 
 
@@ -124,13 +132,78 @@ def cpu_baseline(s, nit, budget_s=15.0):
                           m, t_lu + t_m, t_lu, nit, t_iter * 1e3)}
 
 
+def solve_bytes(mode, N, st):
+    if mode == "fp64":
+        return 8 * N * (18 + 11 + 24 * st.nit)
+    return 4 * N * (18 + 24 * st.nit) + 8 * N * (11 * (st.outer + 1) + 3 * st.outer)
+
+
+def roofline(kt, N):
+    if not kt:
+        return None
+    dom = max(kt, key=lambda k: kt[k][0])
+    base = dom.replace("32", "")
+    size = 4 if dom.endswith("32") else 8
+    per_launch = BYTES_PER_POINT[base] * size * N
+    avg_ms = kt[dom][0] / kt[dom][1]
+    ach = per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+            "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": per_launch}
+
+
+def batch_run(mode, steps, warmup, count=256):
+    """256 independent p' systems (own grid size, stretching and RHS) solved
+    together to the 1e-10 drop."""
+    import torch
+
+    from mach2d import problems as P
+    from mach2d import solvers as S
+    sizes = P.cfg4_sizes(count)
+    t = time.time()
+    systems = [P.pcorr_system(nx, ny, seed=100 + q, q=1.02 + 0.06 * ((q * 0.618) % 1.0),
+                              rhs="gauss") for q, (nx, ny) in enumerate(sizes)]
+    log("batch: %d systems, N = %d in total, built in %.1fs" % (
+        count, sum(x.N for x in systems), time.time() - t))
+    a = [torch.tensor(x.a.reshape(-1), device="cuda") for x in systems]
+    rhs = [torch.tensor(x.rhs, device="cuda") for x in systems]
+    var = [torch.zeros(x.N, dtype=torch.float64, device="cuda") for x in systems]
+    b = S.BatchMSI(sizes, precision=mode)
+    b.factor(a)
+    layout, ms = b.tune(a, rhs, iters=4)
+    log("batch %s: layout %d (%s), ms/iteration per layout %s" % (
+        mode, layout, S.BatchMSI.LAYOUTS[layout], [round(v, 3) for v in ms]))
+
+    def one():
+        for v in var:
+            v.zero_()
+        b.factor(a)
+        return b.solve(a, rhs, var, GAMMA, NITM_CAP)
+
+    for _ in range(warmup):
+        one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = [one() for _ in range(steps)]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    rc, st = res[-1]
+    ok = rc == 0 and all(x.status == 0 and x.rn < x.gammarp for x in st)
+    nbytes = sum(solve_bytes(mode, x.N, y) for x, y in zip(systems, st))
+    b.close()
+    return {"mode": mode, "systems": count, "solves_per_s": round(count / dt, 2),
+            "ms_per_batch": round(dt * 1e3, 3), "gbps": round(nbytes / dt / 1e9, 2),
+            "iterations_max": max(y.nit for y in st), "layout": layout, "correct": bool(ok)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="fp64", choices=["fp64", "fp32_ir"])
+    ap.add_argument("--mode", default="best", choices=["best", "fp64", "fp32_ir"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -138,52 +211,59 @@ def main():
         sys.exit(2)
 
     s = make_system()
-    r = gpu_run(s, args.mode, args.steps, args.warmup)
-    st, dt = r["st"], r["dt"]
-    nit = st.nit if args.mode == "fp64" else st.nit
-    per = dt / args.steps
-    sz = 8
-    algo_bytes = sz * s.N * (18 + 11 + 24 * nit)
-    # dominant kernel by device time
-    kt = r["kt"]
-    dom = max(kt, key=lambda k: kt[k][0]) if kt else None
-    roof = None
-    if dom:
-        base = dom.replace("32", "")
-        size = 4 if dom.endswith("32") else 8
-        per_launch = BYTES_PER_POINT[base] * size * s.N
-        avg_ms = kt[dom][0] / kt[dom][1]
-        ach = per_launch / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
-                "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 2),
-                "bytes_per_launch": per_launch}
+    modes = ["fp64", "fp32_ir"] if args.mode == "best" else [args.mode]
+    runs = {m: gpu_run(s, m, args.steps, args.warmup) for m in modes}
+    x64 = runs["fp64"]["x"] if "fp64" in runs else None
+    summary = {}
+    for m, r in runs.items():
+        st = r["st"]
+        per = r["dt"] / args.steps
+        e = {"solves_per_s": round(1.0 / per, 4), "ms_per_solve": round(per * 1e3, 3),
+             "nit": st.nit, "outer": st.outer, "correct": bool(r["ok"]),
+             "gbps": round(solve_bytes(m, s.N, st) / per / 1e9, 2)}
+        if m != "fp64" and x64 is not None:
+            e["rel_error_vs_fp64"] = float(np.linalg.norm(r["x"] - x64) / np.linalg.norm(x64))
+            e["correct"] = bool(r["ok"] and e["rel_error_vs_fp64"] <= 1e-8)
+        summary[m] = e
+    head = "fp32_ir" if "fp32_ir" in summary and summary["fp32_ir"]["correct"] else modes[0]
+    r, e = runs[head], summary[head]
+    st = r["st"]
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(s, nit)
+        nit64 = runs["fp64"]["st"].nit if "fp64" in runs else st.nit
+        cpu = cpu_baseline(s, nit64)
         log("cpu: %.4g solves/s (%s)" % (cpu["value"], cpu["sample"]))
+    batch = None
+    if not args.no_batch:
+        batch = {m: batch_run(m, max(1, args.steps - 1), 1) for m in ("fp64", "fp32_ir")}
+        for m, v in batch.items():
+            log("batch %s: %s" % (m, v))
     out = {
         "metric": METRIC,
-        "value": round(args.steps / dt, 4),
+        "value": e["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(per * 1e3, 3),
+        "ms_per_step": e["ms_per_solve"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.mode == "fp64" else "f32+f64",
+        "dtype": "f64" if head == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": "p' MSI solve, 1026x514 wall-resolved ogive grid "
                                "(aspect 1e4), zero guess, 1e-10 residual drop",
-                   "mode": args.mode, "nx": s.nx, "ny": s.ny, "nit": nit,
-                   "gamma": GAMMA, "correct": bool(r["ok"])},
-        "gbps": round(algo_bytes / per / 1e9, 2),
-        "kernels_ms": {k: [round(v[0], 3), v[1]] for k, v in kt.items()},
-        "roofline": roof,
+                   "mode": head, "nx": s.nx, "ny": s.ny, "nit": st.nit, "outer": st.outer,
+                   "gamma": GAMMA, "correct": e["correct"]},
+        "gbps": e["gbps"],
+        "speedup_vs_plain_fp64": (round(e["solves_per_s"] / summary["fp64"]["solves_per_s"], 3)
+                                  if "fp64" in summary else None),
+        "modes": summary,
+        "kernels_ms": {k: [round(v[0], 3), v[1]] for k, v in r["kt"].items()},
+        "roofline": roofline(r["kt"], s.N),
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 

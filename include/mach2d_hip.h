@@ -140,12 +140,48 @@ int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du);
  * one XCD). */
 int m2d_set_tuning(m2d_system *s, int publish_every, int xcd_stride);
 
-/* Batched: solve `count` independent systems (each with its own handle,
- * factor, and device arrays) with one wavefront launch per phase for the
- * whole batch.  Arrays are indexed like the handles. */
-int m2d_solve_batch(m2d_system **systems, int count, const double **d_a,
-                    const double **d_rhs, double **d_var, double gamma,
-                    int nitm, m2d_status *st);
+/* ---- 3. Batched API (device pointers) ---------------------------------- */
+/* Many independent systems of uneven grid size solved together: one launch
+ * per phase for the whole batch, iterations replayed from a captured
+ * hipGraph, launch placement picked by an autotuner.  No reference
+ * counterpart (the reference solves one system per call); each system
+ * follows fb2d9's semantics exactly as m2d_solve does.  Arrays of device
+ * pointers are host arrays with `count` entries, indexed like nx/ny. */
+typedef struct m2d_batch m2d_batch;
+
+int m2d_batch_create(int count, const int *nx, const int *ny, int precision,
+                     m2d_batch **out);
+int m2d_batch_destroy(m2d_batch *b);
+
+/* A non-NULL stream for the batch's work; NULL restores the batch's own
+ * stream, which first waits for the legacy default stream.  Calls return
+ * after the work finished. */
+int m2d_batch_set_stream(m2d_batch *b, void *stream);
+
+/* Factor every system (lu2d9 on each d_a[q]). */
+int m2d_batch_factor(m2d_batch *b, const double *const *d_a);
+
+/* Solve every system from its var: stopping rule of fb2d9 per system
+ * (gamma, nitm as in m2d_solve); st (count entries, may be NULL) receives
+ * each system's status; device_ms is the batch's time.  Returns the first
+ * non-zero status. */
+int m2d_batch_solve(m2d_batch *b, const double *const *d_a, const double *const *d_rhs,
+                    double *const *d_var, double gamma, int nitm, m2d_status *st);
+
+/* FP32_IR: the inner stopping rule (as m2d_set_inner). */
+int m2d_batch_set_inner(m2d_batch *b, double inner_gamma, int inner_nitm);
+
+/* Launch placement: layout 0 = systems in order, 1 = each system's bands on
+ * one XCD (longest-processing-time assignment), 2 = largest systems first;
+ * chunk_iters = iterations per graph replay (1..64). */
+int m2d_batch_set_tuning(m2d_batch *b, int layout, int chunk_iters);
+int m2d_batch_get_tuning(m2d_batch *b, int *layout, int *chunk_iters);
+
+/* Autotuner: time `iters` iterations of each layout on these systems (from
+ * a zero iterate in scratch memory; the caller's var is not touched) and
+ * keep the fastest.  ms_out (3 entries, may be NULL): time per iteration. */
+int m2d_batch_tune(m2d_batch *b, const double *const *d_a, const double *const *d_rhs,
+                   int iters, int *layout_out, double *ms_out);
 
 /* TDMA on device pointers (tdma2d9 semantics). */
 int m2d_tdma2d9_device(int nx, int ny, int nitm, const double *d_a,
@@ -154,7 +190,7 @@ int m2d_tdma2d9_device(int nx, int ny, int nitm, const double *d_a,
 /* Per-kernel device time of this handle's launches (HIP events on the
  * handle's stream around each launch; launches after the stopping rule
  * fired are not counted).  Kernel ids: 0 factor, 1 forward sweep,
- * 2 backward sweep + update, 3 residual; +4 for the FP32 kernels of
+ * 2 backward sweep, 3 update + residual; +4 for the FP32 kernels of
  * M2D_FP32_IR.  Enabling resets the totals.  No reference counterpart (the
  * reference times only the whole run with cpu_time). */
 int m2d_enable_kernel_timing(m2d_system *s, int on);

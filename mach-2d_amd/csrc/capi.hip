@@ -646,20 +646,6 @@ extern "C" void m2d_solve_msi9d(const int *nx, const int *ny, const int *nitm, c
 }
 
 // Batched entry: every system keeps its own handle, factor and arrays.
-extern "C" int m2d_solve_batch(m2d_system **systems, int count, const double **d_a,
-                               const double **d_rhs, double **d_var, double gamma, int nitm,
-                               m2d_status *st)
-{
-    if (!systems || count < 0 || !d_a || !d_rhs || !d_var) return M2D_ERR_ARG;
-    int worst = M2D_OK;
-    for (int q = 0; q < count; ++q) {
-        int rc = m2d_solve(systems[q], d_a[q], d_rhs[q], d_var[q], gamma, nitm, nullptr, 0,
-                           st ? &st[q] : nullptr);
-        if (rc && !worst) worst = rc;
-    }
-    return worst;
-}
-
 extern "C" int m2d_enable_kernel_timing(m2d_system *s, int on)
 {
     if (!s) return M2D_ERR_ARG;

@@ -57,6 +57,20 @@ void launch_from_skew(const T *src, T *dst, int nk, const Skew &sk, hipStream_t 
 template <typename T>
 void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipStream_t s);
 
+// ---- batched wavefront and layout kernels (sweep.hip) -------------------------
+template <typename T>
+struct BatchSys;
+template <typename T>
+void launch_factor_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s);
+template <typename T>
+void launch_forward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s);
+template <typename T>
+void launch_backward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s);
+template <typename T>
+void launch_to_skew_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s);
+template <typename T>
+void launch_prepare_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s);
+
 // ---- parallel kernels (msi9.hip) ---------------------------------------------
 template <typename TA, typename TX, typename TR>
 void launch_residual(const TA *a, const TX *x, const TA *rhs, TR *r, int nx, int ny,

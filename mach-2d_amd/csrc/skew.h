@@ -15,7 +15,8 @@
 // at the end (front padding pad(b)), so the backward sweep, which visits the
 // band's steps in exactly reverse order, walks the same chunks backwards.
 // Padding entries (steps whose column is outside 0..nx-1, lanes beyond the
-// last row) are read and written like real ones and never feed a result.
+// last row) are read and written like real ones; their values only ever
+// meet zero couplings.
 #pragma once
 
 #if defined(__HIPCC__)

@@ -38,6 +38,7 @@
 // under in-order workgroup dispatch, and every wait is bounded.
 #include <math.h>
 
+#include "batch.h"
 #include "m2d_internal.h"
 #include "msi_stencil.h"
 #include "skew.h"
@@ -122,16 +123,10 @@ struct Halves {
 // 1/dl1), w in the skewed layout.
 // --------------------------------------------------------------------------
 template <typename T, int F>
-__global__ __launch_bounds__(kWave) void k_forward(const T *__restrict__ r,
-                                                   const T *__restrict__ dl, T *__restrict__ w,
-                                                   Skew sk, Link lk0, int xcd_stride)
+__device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *__restrict__ dl,
+                                             T *__restrict__ w, const Skew &sk, const Link &lk,
+                                             int band)
 {
-    if (lk0.st->done) return;
-    Link lk = lk0;
-    lk.gen = launch_gen(lk0);
-    const int blk = blockIdx.x;
-    if (blk % xcd_stride) return;
-    const int band = blk / xcd_stride;
     const int lane = threadIdx.x;
     const int nx = sk.nx, pad = sk.pad(band);
     const int nf = sk.nst / F;  // chunks of F steps
@@ -243,15 +238,9 @@ __global__ __launch_bounds__(kWave) void k_forward(const T *__restrict__ r,
 // stored order, z over w in place; bands top to bottom.
 // --------------------------------------------------------------------------
 template <typename T, int F>
-__global__ __launch_bounds__(kWave) void k_backward(T *__restrict__ wz, const T *__restrict__ du,
-                                                    Skew sk, Link lk0, int xcd_stride)
+__device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__restrict__ du,
+                                              const Skew &sk, const Link &lk, int band)
 {
-    if (lk0.st->done) return;
-    Link lk = lk0;
-    lk.gen = launch_gen(lk0);
-    const int blk = blockIdx.x;
-    if (blk % xcd_stride) return;
-    const int band = sk.nb - 1 - blk / xcd_stride;
     const int lane = threadIdx.x;
     const int nx = sk.nx, pad = sk.pad(band);
     const int nf = sk.nst / F;
@@ -362,15 +351,10 @@ __global__ __launch_bounds__(kWave) void k_backward(T *__restrict__ wz, const T 
 // carries the four du values of each column.
 // --------------------------------------------------------------------------
 template <typename T, int F>
-__global__ __launch_bounds__(kWave) void k_factor(const T *__restrict__ a, T *__restrict__ dl,
-                                                  T *__restrict__ du, T alpha0, Skew sk, Link lk0,
-                                                  int xcd_stride)
+__device__ __forceinline__ void factor_band(const T *__restrict__ a, T *__restrict__ dl,
+                                            T *__restrict__ du, T alpha0, const Skew &sk,
+                                            const Link &lk, int band)
 {
-    Link lk = lk0;
-    lk.gen = launch_gen(lk0);
-    const int blk = blockIdx.x;
-    if (blk % xcd_stride) return;
-    const int band = blk / xcd_stride;
     const int lane = threadIdx.x;
     const int nx = sk.nx, ny = sk.ny, pad = sk.pad(band), rows = sk.rows(band);
     const int nf = sk.nst / F;
@@ -501,51 +485,144 @@ __global__ __launch_bounds__(kWave) void k_factor(const T *__restrict__ a, T *__
 }
 
 // --------------------------------------------------------------------------
-// layout conversions and the update
+// Single-system kernels: workgroup g (g % xcd_stride == 0) runs band
+// g / xcd_stride, so with xcd_stride 8 every band lands on the same XCD
+// (workgroups are dealt to the XCDs round robin) and shares its L2.
 // --------------------------------------------------------------------------
-// nk arrays, reference layout (stride nx*ny) -> skewed (stride sk.size());
-// padding entries get zero.  One thread per skewed entry.
-template <typename TI, typename TO>
-__global__ __launch_bounds__(256) void k_to_skew(const TI *__restrict__ src, TO *__restrict__ dst,
-                                                 int nk, Skew sk)
+template <typename T, int F>
+__global__ __launch_bounds__(kWave) void k_forward(const T *__restrict__ r,
+                                                   const T *__restrict__ dl, T *__restrict__ w,
+                                                   Skew sk, Link lk0, int xcd_stride)
 {
-    const long e = (long)blockIdx.x * 256 + threadIdx.x;
-    const long L = sk.size();
-    if (e >= L) return;
-    const int s = (int)(e % sk.S);
-    const long t1 = e / sk.S;
-    const int l = (int)(t1 % 64);
-    const long t2 = t1 / 64;
-    const int c = (int)(t2 % sk.nch);
-    const int b = (int)(t2 / sk.nch);
-    const int u = c * sk.S + s;
-    const int j = b * 64 + l;
-    const int i = u - sk.pad(b) - 2 * l;
-    const bool real = j < sk.ny && i >= 0 && i < sk.nx;
-    const long N = (long)sk.nx * sk.ny;
-    const long p = real ? (long)j * sk.nx + i : 0;
-    for (int k = 0; k < nk; ++k) dst[k * L + e] = real ? (TO)src[k * N + p] : TO(0);
+    if (lk0.st->done) return;
+    const int blk = blockIdx.x;
+    if (blk % xcd_stride) return;
+    Link lk = lk0;
+    lk.gen = launch_gen(lk0);
+    forward_band<T, F>(r, dl, w, sk, lk, blk / xcd_stride);
 }
 
-// Sweep-ready factor: dl[5] = 1/dl1 at grid points; at padding entries
-// (outside the grid or beyond the last row) dl1 = 1/dl1 = 1 and every
-// coupling zero, so the sweeps compute finite padding values.
-template <typename T>
-__global__ __launch_bounds__(256) void k_prepare_factor(T *__restrict__ dl, T *__restrict__ du,
-                                                        Skew sk)
+template <typename T, int F>
+__global__ __launch_bounds__(kWave) void k_backward(T *__restrict__ wz, const T *__restrict__ du,
+                                                    Skew sk, Link lk0, int xcd_stride)
 {
-    const long e = (long)blockIdx.x * 256 + threadIdx.x;
-    const long L = sk.size();
-    if (e >= L) return;
+    if (lk0.st->done) return;
+    const int blk = blockIdx.x;
+    if (blk % xcd_stride) return;
+    Link lk = lk0;
+    lk.gen = launch_gen(lk0);
+    backward_band<T, F>(wz, du, sk, lk, sk.nb - 1 - blk / xcd_stride);
+}
+
+template <typename T, int F>
+__global__ __launch_bounds__(kWave) void k_factor(const T *__restrict__ a, T *__restrict__ dl,
+                                                  T *__restrict__ du, T alpha0, Skew sk, Link lk0,
+                                                  int xcd_stride)
+{
+    const int blk = blockIdx.x;
+    if (blk % xcd_stride) return;
+    Link lk = lk0;
+    lk.gen = launch_gen(lk0);
+    factor_band<T, F>(a, dl, du, alpha0, sk, lk, blk / xcd_stride);
+}
+
+// --------------------------------------------------------------------------
+// Batched kernels: workgroup g runs band map[g].y of system map[g].x (x < 0:
+// an idle slot).  The map lists the bands of every system in dependency
+// order (bottom to top for the factorization and the forward sweep, top to
+// bottom for the backward sweep), so a band again waits only on workgroups
+// with lower indices.
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ Link batch_link(const BatchSys<T> &d, int phase)
+{
+    Link lk;
+    lk.bnd = d.bnd;
+    lk.scratch = d.scratch;
+    lk.st = d.st;
+    lk.phase = phase;
+    lk.gen = 0;
+    lk.gen = launch_gen(lk);
+    return lk;
+}
+
+template <typename T, int F>
+__global__ __launch_bounds__(kWave) void k_forward_batch(const BatchSys<T> *__restrict__ sys,
+                                                         const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys<T> &d = sys[m.x];
+    if (d.st->done) return;
+    forward_band<T, F>(d.r, d.dl, d.w, d.sk, batch_link(d, 1), m.y);
+}
+
+template <typename T, int F>
+__global__ __launch_bounds__(kWave) void k_backward_batch(const BatchSys<T> *__restrict__ sys,
+                                                          const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys<T> &d = sys[m.x];
+    if (d.st->done) return;
+    backward_band<T, F>(d.w, d.du, d.sk, batch_link(d, 2), m.y);
+}
+
+template <typename T, int F>
+__global__ __launch_bounds__(kWave) void k_factor_batch(const BatchSys<T> *__restrict__ sys,
+                                                        const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys<T> &d = sys[m.x];
+    factor_band<T, F>(d.a, d.dl, d.du, T(0.5), d.sk, batch_link(d, 0), m.y);
+}
+
+// --------------------------------------------------------------------------
+// layout conversions and the update
+// --------------------------------------------------------------------------
+// Grid point of skewed entry e: i, j (0-based); real = inside the grid.
+struct SkewPoint {
+    int i, j;
+    bool real;
+};
+__device__ __forceinline__ SkewPoint skew_point(const Skew &sk, long e)
+{
     const int s = (int)(e % sk.S);
     const long t1 = e / sk.S;
     const int l = (int)(t1 % 64);
     const long t2 = t1 / 64;
     const int c = (int)(t2 % sk.nch);
     const int b = (int)(t2 / sk.nch);
-    const int j = b * 64 + l;
-    const int i = c * sk.S + s - sk.pad(b) - 2 * l;
-    if (j < sk.ny && i >= 0 && i < sk.nx) {
+    SkewPoint q;
+    q.j = b * 64 + l;
+    q.i = c * sk.S + s - sk.pad(b) - 2 * l;
+    q.real = q.j < sk.ny && q.i >= 0 && q.i < sk.nx;
+    return q;
+}
+
+// nk arrays, reference layout (stride nx*ny) -> skewed entry e (stride
+// sk.size()); padding entries get zero.
+template <typename TI, typename TO>
+__device__ __forceinline__ void to_skew_entry(const TI *__restrict__ src, TO *__restrict__ dst,
+                                              int nk, const Skew &sk, long e)
+{
+    const long L = sk.size();
+    const SkewPoint q = skew_point(sk, e);
+    const long N = (long)sk.nx * sk.ny;
+    const long p = q.real ? (long)q.j * sk.nx + q.i : 0;
+    for (int k = 0; k < nk; ++k) dst[k * L + e] = q.real ? (TO)src[k * N + p] : TO(0);
+}
+
+// Sweep-ready factor at skewed entry e: dl[5] = 1/dl1 at grid points; at
+// padding entries (outside the grid or beyond the last row) dl1 = 1/dl1 = 1
+// and every coupling zero, so the sweeps compute finite padding values.
+template <typename T>
+__device__ __forceinline__ void prepare_entry(T *__restrict__ dl, T *__restrict__ du,
+                                              const Skew &sk, long e)
+{
+    const long L = sk.size();
+    if (skew_point(sk, e).real) {
         dl[5 * L + e] = T(1) / dl[e];
         return;
     }
@@ -555,6 +632,44 @@ __global__ __launch_bounds__(256) void k_prepare_factor(T *__restrict__ dl, T *_
     for (int k = 1; k < 5; ++k) dl[k * L + e] = T(0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) du[k * L + e] = T(0);
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_to_skew(const TI *__restrict__ src, TO *__restrict__ dst,
+                                                 int nk, Skew sk)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e < sk.size()) to_skew_entry<TI, TO>(src, dst, nk, sk, e);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_prepare_factor(T *__restrict__ dl, T *__restrict__ du,
+                                                        Skew sk)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e < sk.size()) prepare_entry<T>(dl, du, sk, e);
+}
+
+// Batched: workgroup g covers skewed entries map[g].y*256 .. of system
+// map[g].x.  A (reference layout, precision T) -> its skewed copy.
+template <typename T>
+__global__ __launch_bounds__(256) void k_to_skew_batch(const BatchSys<T> *__restrict__ sys,
+                                                       const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    const BatchSys<T> &d = sys[m.x];
+    const long e = (long)m.y * 256 + threadIdx.x;
+    if (e < d.sk.size()) to_skew_entry<T, T>(d.ra, d.a, 9, d.sk, e);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_prepare_batch(const BatchSys<T> *__restrict__ sys,
+                                                       const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    const BatchSys<T> &d = sys[m.x];
+    const long e = (long)m.y * 256 + threadIdx.x;
+    if (e < d.sk.size()) prepare_entry<T>(d.dl, d.du, d.sk, e);
 }
 
 // skewed -> reference layout, one thread per point.
@@ -646,6 +761,35 @@ void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s)
                        sk);
 }
 
+template <typename T>
+void launch_factor_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_factor_batch<T, factor_steps<T>()>), dim3(nblk), dim3(kWave), 0, s,
+                       sys, map);
+}
+template <typename T>
+void launch_forward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_forward_batch<T, sweep_steps<T>()>), dim3(nblk), dim3(kWave), 0, s,
+                       sys, map);
+}
+template <typename T>
+void launch_backward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_backward_batch<T, sweep_steps<T>()>), dim3(nblk), dim3(kWave), 0, s,
+                       sys, map);
+}
+template <typename T>
+void launch_to_skew_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_to_skew_batch<T>, dim3(nblk), dim3(256), 0, s, sys, map);
+}
+template <typename T>
+void launch_prepare_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_prepare_batch<T>, dim3(nblk), dim3(256), 0, s, sys, map);
+}
+
 template <typename TI, typename TO>
 void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t s)
 {
@@ -674,6 +818,11 @@ void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipSt
     template void launch_backward<T>(T *, const T *, const Skew &, Link, int, hipStream_t);     \
     template void launch_from_skew<T>(const T *, T *, int, const Skew &, hipStream_t);          \
     template void launch_prepare_factor<T>(T *, T *, const Skew &, hipStream_t);                \
+    template void launch_factor_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t);  \
+    template void launch_forward_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t); \
+    template void launch_backward_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t);\
+    template void launch_to_skew_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t); \
+    template void launch_prepare_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t); \
     template void launch_update<T>(T *, const T *, const Skew &, const SolveState *, hipStream_t);
 M2D_INST(double)
 M2D_INST(float)

@@ -47,8 +47,11 @@ EXPORTS = [
     "m2d_lu2d9", "m2d_fb2d9", "m2d_tdma2d9", "m2d_solve_msi9d", "m2d_norm_l1_9d",
     "m2d_norm_l1_9d_relative", "m2d_norm_l1_b", "m2d_last_status", "m2d_create",
     "m2d_destroy", "m2d_set_stream", "m2d_factor", "m2d_solve", "m2d_set_inner",
-    "m2d_get_factor", "m2d_set_tuning", "m2d_solve_batch", "m2d_tdma2d9_device",
+    "m2d_get_factor", "m2d_set_tuning", "m2d_tdma2d9_device",
     "m2d_version", "m2d_enable_kernel_timing", "m2d_kernel_timing",
+    "m2d_batch_create", "m2d_batch_destroy", "m2d_batch_set_stream", "m2d_batch_factor",
+    "m2d_batch_solve", "m2d_batch_set_inner", "m2d_batch_set_tuning", "m2d_batch_get_tuning",
+    "m2d_batch_tune",
 ]
 
 _lib = None
@@ -100,9 +103,20 @@ def load():
     L.m2d_get_factor.restype = C.c_int
     L.m2d_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
     L.m2d_set_tuning.restype = C.c_int
-    L.m2d_solve_batch.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(_vp), C.POINTER(_vp),
-                                  C.POINTER(_vp), C.c_double, C.c_int, C.POINTER(Status)]
-    L.m2d_solve_batch.restype = C.c_int
+    _pp = C.POINTER(_vp)
+    L.m2d_batch_create.argtypes = [C.c_int, _ip, _ip, C.c_int, C.POINTER(_vp)]
+    L.m2d_batch_destroy.argtypes = [_vp]
+    L.m2d_batch_set_stream.argtypes = [_vp, _vp]
+    L.m2d_batch_factor.argtypes = [_vp, _pp]
+    L.m2d_batch_solve.argtypes = [_vp, _pp, _pp, _pp, C.c_double, C.c_int, C.POINTER(Status)]
+    L.m2d_batch_set_inner.argtypes = [_vp, C.c_double, C.c_int]
+    L.m2d_batch_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
+    L.m2d_batch_get_tuning.argtypes = [_vp, _ip, _ip]
+    L.m2d_batch_tune.argtypes = [_vp, _pp, _pp, C.c_int, _ip, _dp]
+    for f in (L.m2d_batch_create, L.m2d_batch_destroy, L.m2d_batch_set_stream,
+              L.m2d_batch_factor, L.m2d_batch_solve, L.m2d_batch_set_inner,
+              L.m2d_batch_set_tuning, L.m2d_batch_get_tuning, L.m2d_batch_tune):
+        f.restype = C.c_int
     L.m2d_tdma2d9_device.argtypes = [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]
     L.m2d_tdma2d9_device.restype = C.c_int
     L.m2d_enable_kernel_timing.argtypes = [_vp, C.c_int]

@@ -341,3 +341,88 @@ class DeviceMSI:
                               C.c_void_p(self._chk(var, self.N, "var")), float(gamma),
                               int(nitm), hp, cap, C.byref(st))
         return st
+
+
+class BatchMSI:
+    """Many independent 9-diagonal systems of uneven size solved together
+    (m2d_batch_*): one launch per phase for the whole batch, iterations
+    replayed from a captured hipGraph, placement chosen by an autotuner.
+    Each system follows fb2d9's semantics (as DeviceMSI.solve)."""
+
+    LAYOUTS = {0: "in order", 1: "one XCD per system", 2: "largest first"}
+
+    def __init__(self, sizes, precision="fp64"):
+        L = _lib.load()
+        self.sizes = [(int(nx), int(ny)) for nx, ny in sizes]
+        self.n = len(self.sizes)
+        nx = (C.c_int * self.n)(*[s[0] for s in self.sizes])
+        ny = (C.c_int * self.n)(*[s[1] for s in self.sizes])
+        prec = {"fp64": _lib.M2D_FP64, "fp32_ir": _lib.M2D_FP32_IR}[precision]
+        self.precision = precision
+        h = C.c_void_p()
+        rc = L.m2d_batch_create(self.n, nx, ny, prec, C.byref(h))
+        if rc:
+            raise SolverError(rc, "m2d_batch_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().m2d_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ptrs(self, tensors, per_point, name):
+        import torch
+        if len(tensors) != self.n:
+            raise ValueError("%s: %d tensors for %d systems" % (name, len(tensors), self.n))
+        arr = (C.c_void_p * self.n)()
+        for q, (t, (nx, ny)) in enumerate(zip(tensors, self.sizes)):
+            if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
+                    and t.is_contiguous() and t.numel() == per_point * nx * ny):
+                raise TypeError("%s[%d] must be a contiguous float64 CUDA tensor of %d values"
+                                % (name, q, per_point * nx * ny))
+            arr[q] = t.data_ptr()
+        return arr
+
+    def factor(self, a):
+        rc = _lib.load().m2d_batch_factor(self.h, self._ptrs(a, 9, "a"))
+        if rc and rc != _lib.M2D_ERR_NONFINITE:
+            raise SolverError(rc, "m2d_batch_factor")
+        return rc
+
+    def solve(self, a, rhs, var, gamma, nitm):
+        """Returns (first non-zero status, [Status per system])."""
+        st = (Status * self.n)()
+        rc = _lib.load().m2d_batch_solve(self.h, self._ptrs(a, 9, "a"), self._ptrs(rhs, 1, "rhs"),
+                                         self._ptrs(var, 1, "var"), float(gamma), int(nitm), st)
+        return rc, list(st)
+
+    def set_inner(self, inner_gamma=1e-4, inner_nitm=200):
+        rc = _lib.load().m2d_batch_set_inner(self.h, float(inner_gamma), int(inner_nitm))
+        if rc:
+            raise SolverError(rc, "m2d_batch_set_inner")
+
+    def set_tuning(self, layout=0, chunk_iters=8):
+        rc = _lib.load().m2d_batch_set_tuning(self.h, int(layout), int(chunk_iters))
+        if rc:
+            raise SolverError(rc, "m2d_batch_set_tuning")
+
+    def get_tuning(self):
+        lay, ch = C.c_int(), C.c_int()
+        _lib.load().m2d_batch_get_tuning(self.h, C.byref(lay), C.byref(ch))
+        return lay.value, ch.value
+
+    def tune(self, a, rhs, iters=4):
+        """Autotune the placement; returns (layout, [ms per iteration per layout])."""
+        lay = C.c_int()
+        ms = (C.c_double * 3)()
+        rc = _lib.load().m2d_batch_tune(self.h, self._ptrs(a, 9, "a"), self._ptrs(rhs, 1, "rhs"),
+                                        int(iters), C.byref(lay), ms)
+        if rc:
+            raise SolverError(rc, "m2d_batch_tune")
+        return lay.value, list(ms)
