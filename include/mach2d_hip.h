@@ -183,6 +183,84 @@ int m2d_batch_get_tuning(m2d_batch *b, int *layout, int *chunk_iters);
 int m2d_batch_tune(m2d_batch *b, const double *const *d_a, const double *const *d_rhs,
                    int iters, int *layout_out, double *ms_out);
 
+/* ---- 4. Coefficient assembly (src/equations/mach2d_coef.f90) ----------- */
+/* Each entry mirrors a routine of module `coefficients` with the same
+ * argument list and order, every argument by pointer (bind(C) from Fortran
+ * unchanged).  Arrays: N = nx*ny doubles, reference layout; coefficient
+ * arrays a(N,9) / a(N,5) column-major.  Arrays may be host memory (copied,
+ * call returns with the results) or device memory (all of one kind;
+ * processed on the assembly stream, call returns once enqueued).  Only the
+ * points the reference writes are written.  Status: m2d_last_status. */
+void m2d_assembly_set_stream(void *stream);
+
+/* get_density_at_faces, mach2d_coef.f90:20 */
+void m2d_get_density_at_faces(const int *nx, const int *ny, const double *beta,
+                              const double *ro, const double *Uce, const double *Vcn,
+                              double *roe, double *ron);
+/* get_u_coefficients, mach2d_coef.f90:63 */
+void m2d_get_u_coefficients(const int *nx, const int *ny, const int *modvis, const double *dt,
+                            const double *rp, const double *re, const double *rn,
+                            const double *Jp, const double *Je, const double *Jn,
+                            const double *ye, const double *yk, const double *alphae,
+                            const double *betae, const double *betan, const double *gamman,
+                            const double *vle, const double *vln, const double *roe,
+                            const double *ron, const double *roa, const double *Uce,
+                            const double *Vcn, double *a);
+/* get_u_source, mach2d_coef.f90:221 */
+void m2d_get_u_source(const int *nx, const int *ny, const int *modvis, const double *beta,
+                      const double *dt, const double *rp, const double *re, const double *rn,
+                      const double *xe, const double *ye, const double *xk, const double *yk,
+                      const double *xke, const double *yke, const double *xen,
+                      const double *yen, const double *Jp, const double *Je, const double *Jn,
+                      const double *roe, const double *ron, const double *roa, const double *p,
+                      const double *vle, const double *vln, const double *Uce,
+                      const double *Vcn, const double *ua, const double *u, const double *v,
+                      double *cup, double *sup, double *b);
+/* get_v_coefficients, mach2d_coef.f90:454 */
+void m2d_get_v_coefficients(const int *nx, const int *ny, const int *coord, const int *modvis,
+                            const double *dt, const double *rp, const double *re,
+                            const double *rn, const double *Jp, const double *Je,
+                            const double *Jn, const double *xe, const double *xk,
+                            const double *alphae, const double *betae, const double *betan,
+                            const double *gamman, const double *vle, const double *vln,
+                            const double *vlp, const double *roe, const double *ron,
+                            const double *roa, const double *Uce, const double *Vcn, double *a);
+/* get_v_source, mach2d_coef.f90:616 */
+void m2d_get_v_source(const int *nx, const int *ny, const int *coord, const int *modvis,
+                      const double *beta, const double *dt, const double *rp, const double *re,
+                      const double *rn, const double *xe, const double *ye, const double *xk,
+                      const double *yk, const double *xke, const double *yke,
+                      const double *xen, const double *yen, const double *Jp, const double *Je,
+                      const double *Jn, const double *roe, const double *ron, const double *roa,
+                      const double *p, const double *vle, const double *vln, const double *Uce,
+                      const double *Vcn, const double *va, const double *u, const double *v,
+                      double *cvp, double *svp, double *b);
+/* get_T_coefficients_and_source, mach2d_coef.f90:787 */
+void m2d_get_T_coefficients_and_source(
+    const int *nx, const int *ny, const int *modvis, const double *beta, const double *dt,
+    const double *rp, const double *re, const double *rn, const double *xe, const double *ye,
+    const double *xk, const double *yk, const double *xke, const double *yke, const double *xen,
+    const double *yen, const double *alphae, const double *betae, const double *betan,
+    const double *gamman, const double *Jp, const double *Je, const double *Jn,
+    const double *roe, const double *ron, const double *ro, const double *roa, const double *p,
+    const double *pa, const double *cp, const double *vle, const double *vln, const double *ke,
+    const double *kn, const double *Uce, const double *Vcn, const double *u, const double *v,
+    const double *ue, const double *ve, const double *un, const double *vn, const double *ua,
+    const double *va, const double *T, const double *Ta, const double *fbe, const double *fbn,
+    double *a, double *b);
+/* get_p_coefficients, mach2d_coef.f90:1207 (a: 5 diagonals) */
+void m2d_get_p_coefficients(const int *nx, const int *ny, const double *dt, const double *rp,
+                            const double *re, const double *rn, const double *Jp,
+                            const double *Uce, const double *Vcn, const double *roe,
+                            const double *ron, const double *g, const double *de,
+                            const double *dn, double *a);
+/* get_p_source, mach2d_coef.f90:1281 */
+void m2d_get_p_source(const int *nx, const int *ny, const double *dt, const double *rp,
+                      const double *re, const double *rn, const double *Jp, const double *roe,
+                      const double *ron, const double *ro, const double *roa, const double *Uce,
+                      const double *Vcn, const double *de, const double *dn, const double *g,
+                      const double *pl, double *b);
+
 /* TDMA on device pointers (tdma2d9 semantics). */
 int m2d_tdma2d9_device(int nx, int ny, int nitm, const double *d_a,
                        const double *d_b, double *d_x, void *stream);

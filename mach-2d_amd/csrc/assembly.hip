@@ -1,0 +1,849 @@
+// assembly.hip -- coefficient assembly of the SIMPLEC equations on the GPU:
+// the 9-point u, v, T and 5-point p' coefficients and sources of
+// src/equations/mach2d_coef.f90 (module `coefficients`), plus the face
+// densities.  One thread per point; every formula keeps the reference's
+// operation order and is compiled without contraction, so the results equal
+// the reference bit for bit.
+//
+// Memory: all inputs are N-vectors in the reference layout (flat index
+// np = nx*(j-1) + i); a point reads its own value and the W/E/S/N (and, for
+// the viscous terms, diagonal) neighbours of a few fields, so a 256-wide
+// row tile reads each field once from HBM with the neighbours from the same
+// or adjacent lines.  The kernels are HBM-bound; the algorithmic traffic of
+// each is (inputs + outputs) x 8 bytes per point.
+//
+// Entry points take the reference's argument lists, every argument by
+// pointer (a Fortran bind(C) interface calls them unchanged).  The arrays may
+// be host or device memory (all of one kind): device arrays are processed in
+// place on the assembly stream (m2d_assembly_set_stream) and the call
+// returns once the kernel is enqueued; host arrays are copied to the device
+// and back, and the call returns when the results are in place.  As in the
+// reference, only the points the routine defines are written.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "m2d_internal.h"
+
+namespace m2d {
+
+constexpr int kTx = 256;  // points per workgroup along i
+constexpr int kMaxIn = 48;
+
+struct Fields {
+    const double *in[kMaxIn];
+    double *out[3];
+    int nx, ny;
+    int modvis, coord;
+    double dt, beta;
+};
+
+__device__ __forceinline__ double dsign5(double x) { return copysign(0.5, x); }
+
+// Advection and diffusion part of get_u_coefficients / get_v_coefficients
+// (mach2d_coef.f90:100-197, :491-598).
+struct UV {
+    const double *rp, *re, *rn, *Jp, *Je, *Jn, *alphae, *betae, *betan, *gamman, *vle, *vln,
+        *roe, *ron, *roa, *Uce, *Vcn;
+};
+
+__device__ __forceinline__ void uv_point(const UV &f, int modvis, double dt, double *a, long N,
+                                         long np, long nx)
+{
+    const long nps = np - nx, npw = np - 1;
+    const double fme = f.roe[np] * f.re[np] * f.Uce[np];
+    const double fmw = f.roe[npw] * f.re[npw] * f.Uce[npw];
+    const double fmn = f.ron[np] * f.rn[np] * f.Vcn[np];
+    const double fms = f.ron[nps] * f.rn[nps] * f.Vcn[nps];
+    const double mpa = f.roa[np] * f.rp[np] / f.Jp[np];
+    const double as = dsign5(f.Vcn[nps]), an = dsign5(f.Vcn[np]);
+    const double aw = dsign5(f.Uce[npw]), ae = dsign5(f.Uce[np]);
+    double c[9];
+    if (modvis == 0) {
+        c[0] = c[2] = c[6] = c[8] = 0.0;
+        c[1] = -fms * (0.5 + as);
+        c[3] = -fmw * (0.5 + aw);
+        c[5] = fme * (0.5 - ae);
+        c[7] = fmn * (0.5 - an);
+    } else {
+        const double ew = f.vle[npw] * f.re[npw] * f.Je[npw] * f.betae[npw];
+        const double ep = f.vle[np] * f.re[np] * f.Je[np] * f.betae[np];
+        const double ns = f.vln[nps] * f.rn[nps] * f.Jn[nps] * f.betan[nps];
+        const double nq = f.vln[np] * f.rn[np] * f.Jn[np] * f.betan[np];
+        c[0] = (ew + ns) / 4.0;
+        c[1] = -(fms * (0.5 + as)) - f.vln[nps] * f.rn[nps] * f.Jn[nps] * f.gamman[nps] +
+               (ew - ep) / 4.0;
+        c[2] = -(ep + ns) / 4.0;
+        c[3] = -(fmw * (0.5 + aw)) - f.vle[npw] * f.re[npw] * f.Je[npw] * f.alphae[npw] +
+               (ns - nq) / 4.0;
+        c[5] = fme * (0.5 - ae) - f.vle[np] * f.re[np] * f.Je[np] * f.alphae[np] +
+               (-ns + nq) / 4.0;
+        c[6] = (-ew - nq) / 4.0;
+        c[7] = fmn * (0.5 - an) - f.vln[np] * f.rn[np] * f.Jn[np] * f.gamman[np] -
+               (ew - ep) / 4.0;
+        c[8] = (ep + nq) / 4.0;
+    }
+    c[4] = mpa / dt - (c[7] + c[1] + c[3] + c[5]);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k * N + np] = c[k];
+}
+
+// Interior point (i, j) of this thread (1-based, 2..nx-1 x 2..ny-1); false
+// outside.
+__device__ __forceinline__ bool interior(int nx, int ny, long &np)
+{
+    const int i = blockIdx.x * kTx + threadIdx.x + 2;
+    const int j = blockIdx.y + 2;
+    if (i > nx - 1 || j > ny - 1) return false;
+    np = (long)nx * (j - 1) + i - 1;
+    return true;
+}
+
+// get_density_at_faces: mach2d_coef.f90:20-60 (east faces j = 2..ny-1,
+// i = 1..nx-1; north faces j = 1..ny-1, i = 2..nx-1).  Grid over
+// i = 1..nx-1, j = 1..ny-1.
+__global__ __launch_bounds__(kTx) void k_density_at_faces(Fields F)
+{
+    const int nx = F.nx, ny = F.ny;
+    const int i = blockIdx.x * kTx + threadIdx.x + 1;
+    const int j = blockIdx.y + 1;
+    if (i > nx - 1 || j > ny - 1) return;
+    const double *ro = F.in[0], *Uce = F.in[1], *Vcn = F.in[2];
+    const double beta = F.beta;
+    const long np = (long)nx * (j - 1) + i - 1;
+    if (j >= 2) {
+        const long npe = np + 1;
+        const double ae = dsign5(Uce[np]);
+        F.out[0][np] = (0.5 + ae) * ro[np] + (0.5 - ae) * ro[npe] + beta * ae * (ro[npe] - ro[np]);
+    }
+    if (i >= 2) {
+        const long npn = np + nx;
+        const double an = dsign5(Vcn[np]);
+        F.out[1][np] = (0.5 + an) * ro[np] + (0.5 - an) * ro[npn] + beta * an * (ro[npn] - ro[np]);
+    }
+}
+
+// get_u_coefficients: mach2d_coef.f90:63-218
+__global__ __launch_bounds__(kTx) void k_u_coefficients(Fields F)
+{
+    long np;
+    if (!interior(F.nx, F.ny, np)) return;
+    const double *const *q = F.in;
+    // rp re rn Jp Je Jn ye yk alphae betae betan gamman vle vln roe ron roa Uce Vcn
+    const UV f = {q[0], q[1], q[2], q[3], q[4], q[5], q[8], q[9], q[10], q[11], q[12], q[13],
+                  q[14], q[15], q[16], q[17], q[18]};
+    const long N = (long)F.nx * F.ny, nps = np - F.nx, npw = np - 1;
+    double *a = F.out[0];
+    uv_point(f, F.modvis, F.dt, a, N, np, F.nx);
+    if (F.modvis == 0) return;
+    const double *rp = q[0], *Je = q[4], *Jn = q[5], *ye = q[6], *yk = q[7], *vle = q[12],
+                 *vln = q[13];
+    const double S1p = rp[np] * Je[np] * vle[np] * (ye[np] * ye[np]) / 3.0 +
+                       rp[np] * Je[npw] * vle[npw] * (ye[npw] * ye[npw]) / 3.0;
+    const double S2p = rp[np] * Jn[np] * vln[np] * (yk[np] * yk[np]) / 3.0 +
+                       rp[np] * Jn[nps] * vln[nps] * (yk[nps] * yk[nps]) / 3.0;
+    a[4 * N + np] = a[4 * N + np] + S1p + S2p;
+}
+
+// get_v_coefficients: mach2d_coef.f90:454-613
+__global__ __launch_bounds__(kTx) void k_v_coefficients(Fields F)
+{
+    long np;
+    if (!interior(F.nx, F.ny, np)) return;
+    const double *const *q = F.in;
+    // rp re rn Jp Je Jn xe xk alphae betae betan gamman vle vln vlp roe ron roa Uce Vcn
+    const UV f = {q[0], q[1], q[2], q[3], q[4], q[5], q[8], q[9], q[10], q[11], q[12], q[13],
+                  q[15], q[16], q[17], q[18], q[19]};
+    const long N = (long)F.nx * F.ny, nps = np - F.nx, npw = np - 1;
+    double *a = F.out[0];
+    uv_point(f, F.modvis, F.dt, a, N, np, F.nx);
+    if (F.modvis == 0) return;
+    const double *rp = q[0], *re = q[1], *rn = q[2], *Jp = q[3], *Je = q[4], *Jn = q[5],
+                 *xe = q[6], *xk = q[7], *vle = q[12], *vln = q[13], *vlp = q[14];
+    const double S1p = vle[np] * re[np] * Je[np] * (xe[np] * xe[np]) / 3.0 +
+                       vle[npw] * re[npw] * Je[npw] * (xe[npw] * xe[npw]) / 3.0;
+    const double S2p = vln[np] * rn[np] * Jn[np] * (xk[np] * xk[np]) / 3.0 +
+                       vln[nps] * rn[nps] * Jn[nps] * (xk[nps] * xk[nps]) / 3.0;
+    const double S7p = (double)F.coord * vlp[np] * 4.0 / (3.0 * rp[np] * Jp[np]);
+    a[4 * N + np] = a[4 * N + np] + S1p + S2p + S7p;
+}
+
+// get_p_coefficients: mach2d_coef.f90:1207-1278 (5 diagonals S W P E N)
+__global__ __launch_bounds__(kTx) void k_p_coefficients(Fields F)
+{
+    long np;
+    if (!interior(F.nx, F.ny, np)) return;
+    const double *const *q = F.in;
+    const double *rp = q[0], *re = q[1], *rn = q[2], *Jp = q[3], *Uce = q[4], *Vcn = q[5],
+                 *roe = q[6], *ron = q[7], *g = q[8], *de = q[9], *dn = q[10];
+    const long N = (long)F.nx * F.ny;
+    const long nps = np - F.nx, npn = np + F.nx, npw = np - 1, npe = np + 1;
+    const double dt = F.dt;
+    const double as = dsign5(Vcn[nps]), an = dsign5(Vcn[np]);
+    const double aw = dsign5(Uce[npw]), ae = dsign5(Uce[np]);
+    const double roem = roe[np], rowm = roe[npw], ronm = ron[np], rosm = ron[nps];
+    double *a = F.out[0];
+    a[np] = -rn[nps] * ((0.5 + as) * Vcn[nps] * g[nps] + rosm * dn[nps]);
+    a[N + np] = -re[npw] * ((0.5 + aw) * Uce[npw] * g[npw] + rowm * de[npw]);
+    a[2 * N + np] = g[np] * (rp[np] / (Jp[np] * dt) + (0.5 + ae) * re[np] * Uce[np] -
+                             (0.5 - aw) * re[npw] * Uce[npw] + (0.5 + an) * rn[np] * Vcn[np] -
+                             (0.5 - as) * rn[nps] * Vcn[nps]) +
+                    roem * re[np] * de[np] + rowm * re[npw] * de[npw] + ronm * rn[np] * dn[np] +
+                    rosm * rn[nps] * dn[nps];
+    a[3 * N + np] = re[np] * ((0.5 - ae) * Uce[np] * g[npe] - roem * de[np]);
+    a[4 * N + np] = rn[np] * ((0.5 - an) * Vcn[np] * g[npn] - ronm * dn[np]);
+}
+
+// get_p_source: mach2d_coef.f90:1281-1348
+__global__ __launch_bounds__(kTx) void k_p_source(Fields F)
+{
+    long np;
+    if (!interior(F.nx, F.ny, np)) return;
+    const double *const *q = F.in;
+    const double *rp = q[0], *re = q[1], *rn = q[2], *Jp = q[3], *roe = q[4], *ron = q[5],
+                 *ro = q[6], *roa = q[7], *Uce = q[8], *Vcn = q[9], *de = q[10], *dn = q[11],
+                 *g = q[12], *pl = q[13];
+    const long nps = np - F.nx, npn = np + F.nx, npw = np - 1, npe = np + 1;
+    const double dt = F.dt;
+    const double as = dsign5(Vcn[nps]), an = dsign5(Vcn[np]);
+    const double aw = dsign5(Uce[npw]), ae = dsign5(Uce[np]);
+    const double roel = g[np] * pl[np] * (0.5 + ae) + g[npe] * pl[npe] * (0.5 - ae);
+    const double rowl = g[npw] * pl[npw] * (0.5 + aw) + g[np] * pl[np] * (0.5 - aw);
+    const double ronl = g[np] * pl[np] * (0.5 + an) + g[npn] * pl[npn] * (0.5 - an);
+    const double rosl = g[nps] * pl[nps] * (0.5 + as) + g[np] * pl[np] * (0.5 - as);
+    const double Ucel = de[np] * (pl[np] - pl[npe]);
+    const double Ucwl = de[npw] * (pl[npw] - pl[np]);
+    const double Vcnl = dn[np] * (pl[np] - pl[npn]);
+    const double Vcsl = dn[nps] * (pl[nps] - pl[np]);
+    F.out[0][np] = -rp[np] * (ro[np] - roa[np]) / (Jp[np] * dt) - roe[np] * re[np] * Uce[np] +
+                   roe[npw] * re[npw] * Uce[npw] - ron[np] * rn[np] * Vcn[np] +
+                   ron[nps] * rn[nps] * Vcn[nps] - roel * re[np] * Ucel +
+                   rowl * re[npw] * Ucwl - ronl * rn[np] * Vcnl + rosl * rn[nps] * Vcsl;
+}
+
+// get_u_source: mach2d_coef.f90:221-451 (outputs cup, sup, b)
+__global__ __launch_bounds__(kTx) void k_u_source(Fields F)
+{
+    long np;
+    if (!interior(F.nx, F.ny, np)) return;
+    const double *const *q = F.in;
+    const double *rp = q[0], *re = q[1], *rn = q[2], *xe = q[3], *ye = q[4], *xk = q[5],
+                 *yk = q[6], *xke = q[7], *yke = q[8], *xen = q[9], *yen = q[10], *Jp = q[11],
+                 *Je = q[12], *Jn = q[13], *roe = q[14], *ron = q[15], *roa = q[16], *p = q[17],
+                 *vle = q[18], *vln = q[19], *Uce = q[20], *Vcn = q[21], *ua = q[22], *u = q[23],
+                 *v = q[24];
+    double *cup = F.out[0], *sup = F.out[1], *b = F.out[2];
+    const long nx = F.nx;
+    const long nps = np - nx, npn = np + nx, npw = np - 1, npe = np + 1;
+    const double fme = roe[np] * re[np] * Uce[np];
+    const double fmw = roe[npw] * re[npw] * Uce[npw];
+    const double fmn = ron[np] * rn[np] * Vcn[np];
+    const double fms = ron[nps] * rn[nps] * Vcn[nps];
+    const double mpa = roa[np] * rp[np] / Jp[np];
+    const double as = dsign5(Vcn[nps]), an = dsign5(Vcn[np]);
+    const double aw = dsign5(Uce[npw]), ae = dsign5(Uce[np]);
+    double bb = mpa * ua[np] / F.dt;
+    const double c = -(F.beta * (fme * ae * (u[npe] - u[np]) - fmw * aw * (u[np] - u[npw]) +
+                                 fmn * an * (u[npn] - u[np]) - fms * as * (u[np] - u[nps])));
+    cup[np] = c;
+    bb = bb + c;
+    bb = bb + 0.5 * rp[np] *
+                  (yk[np] * (p[npn] + p[np]) - yk[nps] * (p[nps] + p[np]) -
+                   ye[np] * (p[npe] + p[np]) + ye[npw] * (p[npw] + p[np]));
+    if (F.modvis == 1) {
+        const long npsw = nps - 1, npse = nps + 1, npnw = npn - 1, npne = npn + 1;
+        const double S1 =
+            rp[np] / 3.0 *
+            (Je[np] * vle[np] *
+                 (ye[np] * ye[np] * u[npe] -
+                  yke[np] * ye[np] * (u[npn] + u[npne] - u[nps] - u[npse]) / 4.0) -
+             Je[npw] * vle[npw] *
+                 (-(ye[npw] * ye[npw] * u[npw]) -
+                  yke[npw] * ye[npw] * (u[npn] + u[npnw] - u[nps] - u[npsw]) / 4.0));
+        const double S2 =
+            rp[np] / 3.0 *
+            (Jn[np] * vln[np] *
+                 (yk[np] * yk[np] * u[npn] -
+                  yk[np] * yen[np] * (u[npe] + u[npne] - u[npw] - u[npnw]) / 4.0) -
+             Jn[nps] * vln[nps] *
+                 (-(yk[nps] * yk[nps] * u[nps]) -
+                  yk[nps] * yen[nps] * (u[npe] + u[npse] - u[npw] - u[npsw]) / 4.0));
+        const double S3 = Je[np] * re[np] * vle[np] * xe[np] *
+                              (yke[np] * (v[npn] + v[npne] - v[nps] - v[npse]) / 4.0 -
+                               ye[np] * (v[npe] - v[np])) -
+                          Je[npw] * re[npw] * vle[npw] * xe[npw] *
+                              (yke[npw] * (v[npn] + v[npnw] - v[nps] - v[npsw]) / 4.0 -
+                               ye[npw] * (v[np] - v[npw]));
+        const double S4 = Jn[np] * rn[np] * vln[np] * xk[np] *
+                              (yen[np] * (v[npne] + v[npe] - v[npnw] - v[npw]) / 4.0 -
+                               yk[np] * (v[npn] - v[np])) -
+                          Jn[nps] * rn[nps] * vln[nps] * xk[nps] *
+                              (yen[nps] * (v[npse] + v[npe] - v[npsw] - v[npw]) / 4.0 -
+                               yk[nps] * (v[np] - v[nps]));
+        // face terms of S5 / S6; a face on the axis (r = 0) drops its term
+        // (mach2d_coef.f90:378-441).  Both terms are evaluated and the
+        // branch selects, which equals the reference also when a dropped
+        // term would divide by zero.
+        double S5, S6;
+        const bool axw = fabs(re[npw]) < 1e-15, axe = fabs(re[np]) < 1e-15;
+        const double X1 = 2.0 * rp[np] / 3.0 * Je[np] * vle[np] / re[np] * ye[np] *
+                                    (xke[np] *
+                                         (rp[npn] * v[npn] + rp[npne] * v[npne] -
+                                          rp[nps] * v[nps] - rp[npse] * v[npse]) / 4.0 -
+                                     xe[np] * (rp[npe] * v[npe] - rp[np] * v[np]));
+        const double X2 = 2.0 * rp[np] / 3.0 * Je[npw] * vle[npw] / re[npw] * ye[npw] *
+                                    (xke[npw] *
+                                         (rp[npn] * v[npn] + rp[npnw] * v[npnw] -
+                                          rp[nps] * v[nps] - rp[npsw] * v[npsw]) / 4.0 -
+                                     xe[npw] * (rp[np] * v[np] - rp[npw] * v[npw]));
+        if (axw)
+            S5 = -X1;
+        else if (axe)
+            S5 = X2;
+        else
+            S5 = -X1 + X2;
+        const bool axs = fabs(rn[nps]) < 1e-15, axn = fabs(rn[np]) < 1e-15;
+        const double Y1 = 2.0 * rp[np] / 3.0 * Jn[np] * vln[np] / rn[np] * yk[np] *
+                                    (xen[np] *
+                                         (rp[npne] * v[npne] + rp[npe] * v[npe] -
+                                          rp[npnw] * v[npnw] - rp[npw] * v[npw]) / 4.0 -
+                                     xk[np] * (rp[npn] * v[npn] - rp[np] * v[np]));
+        const double Y2 = 2.0 * rp[np] / 3.0 * Jn[nps] * vln[nps] / rn[nps] * yk[nps] *
+                                    (xen[nps] *
+                                         (rp[npe] * v[npe] + rp[npse] * v[npse] -
+                                          rp[npw] * v[npw] - rp[npsw] * v[npsw]) / 4.0 -
+                                     xk[nps] * (rp[np] * v[np] - rp[nps] * v[nps]));
+        if (axs)
+            S6 = -Y1;
+        else if (axn)
+            S6 = Y2;
+        else
+            S6 = -Y1 + Y2;
+        const double s = S1 + S2 + S3 + S4 + S5 + S6;
+        sup[np] = s;
+        bb = bb + s;
+    }
+    b[np] = bb;
+}
+
+// get_v_source: mach2d_coef.f90:616-783 (outputs cvp, svp, b)
+__global__ __launch_bounds__(kTx) void k_v_source(Fields F)
+{
+    long np;
+    if (!interior(F.nx, F.ny, np)) return;
+    const double *const *q = F.in;
+    const double *rp = q[0], *re = q[1], *rn = q[2], *xe = q[3], *ye = q[4], *xk = q[5],
+                 *yk = q[6], *xke = q[7], *yke = q[8], *xen = q[9], *yen = q[10], *Jp = q[11],
+                 *Je = q[12], *Jn = q[13], *roe = q[14], *ron = q[15], *roa = q[16], *p = q[17],
+                 *vle = q[18], *vln = q[19], *Uce = q[20], *Vcn = q[21], *va = q[22], *u = q[23],
+                 *v = q[24];
+    double *cvp = F.out[0], *svp = F.out[1], *b = F.out[2];
+    const long nx = F.nx;
+    const long nps = np - nx, npn = np + nx, npw = np - 1, npe = np + 1;
+    const double fme = roe[np] * re[np] * Uce[np];
+    const double fmw = roe[npw] * re[npw] * Uce[npw];
+    const double fmn = ron[np] * rn[np] * Vcn[np];
+    const double fms = ron[nps] * rn[nps] * Vcn[nps];
+    const double mpa = roa[np] * rp[np] / Jp[np];
+    const double as = dsign5(Vcn[nps]), an = dsign5(Vcn[np]);
+    const double aw = dsign5(Uce[npw]), ae = dsign5(Uce[np]);
+    double bb = mpa * va[np] / F.dt;
+    const double c = -(F.beta * (fme * ae * (v[npe] - v[np]) - fmw * aw * (v[np] - v[npw]) +
+                                 fmn * an * (v[npn] - v[np]) - fms * as * (v[np] - v[nps])));
+    cvp[np] = c;
+    bb = bb + c;
+    bb = bb + 0.5 * rp[np] *
+                  (xe[np] * (p[np] + p[npe]) - xe[npw] * (p[np] + p[npw]) -
+                   xk[np] * (p[np] + p[npn]) + xk[nps] * (p[np] + p[nps]));
+    if (F.modvis == 1) {
+        const long npsw = nps - 1, npse = nps + 1, npnw = npn - 1, npne = npn + 1;
+        const double S1 = vle[np] * re[np] * Je[np] / 3.0 *
+                              (xe[np] * xe[np] * v[npe] -
+                               xke[np] * xe[np] * (v[npn] + v[npne] - v[nps] - v[npse]) / 4.0) -
+                          vle[npw] * re[npw] * Je[npw] / 3.0 *
+                              (-(xe[npw] * xe[npw] * v[npw]) -
+                               xke[npw] * xe[npw] * (v[npn] + v[npnw] - v[nps] - v[npsw]) / 4.0);
+        const double S2 = vln[np] * rn[np] * Jn[np] / 3.0 *
+                              (xk[np] * xk[np] * v[npn] -
+                               xk[np] * xen[np] * (v[npe] + v[npne] - v[npnw] - v[npw]) / 4.0) -
+                          vln[nps] * rn[nps] * Jn[nps] / 3.0 *
+                              (-(xk[nps] * xk[nps] * v[nps]) -
+                               xk[nps] * xen[nps] * (v[npe] + v[npse] - v[npsw] - v[npw]) / 4.0);
+        const double S3 = rp[np] * vle[np] * Je[np] * ye[np] *
+                              (xke[np] * (u[npn] + u[npne] - u[nps] - u[npse]) / 4.0 -
+                               xe[np] * (u[npe] - u[np])) -
+                          rp[np] * vle[npw] * Je[npw] * ye[npw] *
+                              (xke[npw] * (u[npn] + u[npnw] - u[nps] - u[npsw]) / 4.0 -
+                               xe[npw] * (u[np] - u[npw]));
+        const double S4 = rp[np] * vln[np] * Jn[np] * yk[np] *
+                              (xen[np] * (u[npne] + u[npe] - u[npnw] - u[npw]) / 4.0 -
+                               xk[np] * (u[npn] - u[np])) -
+                          rp[np] * vln[nps] * Jn[nps] * yk[nps] *
+                              (xen[nps] * (u[npse] + u[npe] - u[npsw] - u[npw]) / 4.0 -
+                               xk[nps] * (u[np] - u[nps]));
+        const double S5 = -(2.0 / 3.0 * rp[np] * vle[np] * Je[np] * xe[np] *
+                            (yke[np] * (u[npn] + u[npne] - u[nps] - u[npse]) / 4.0 -
+                             ye[np] * (u[npe] - u[np]))) +
+                          2.0 / 3.0 * rp[np] * vle[npw] * Je[npw] * xe[npw] *
+                              (yke[npw] * (u[npn] + u[npnw] - u[nps] - u[npsw]) / 4.0 -
+                               ye[npw] * (u[np] - u[npw]));
+        const double S6 = -(2.0 / 3.0 * rp[np] * vln[np] * Jn[np] * xk[np] *
+                            (yen[np] * (u[npe] + u[npne] - u[npw] - u[npnw]) / 4.0 -
+                             yk[np] * (u[npn] - u[np]))) +
+                          2.0 / 3.0 * rp[np] * vln[nps] * Jn[nps] * xk[nps] *
+                              (yen[nps] * (u[npe] + u[npse] - u[npw] - u[npsw]) / 4.0 -
+                               yk[nps] * (u[np] - u[nps]));
+        const double S8 = 2.0 / 3.0 * (double)F.coord * v[np] *
+                          (-(vln[np] * xk[np]) + vln[nps] * xk[nps] + vle[np] * xe[np] -
+                           vle[npw] * xe[npw]);
+        const double s = S1 + S2 + S3 + S4 + S5 + S6 + S8;
+        svp[np] = s;
+        bb = bb + s;
+    }
+    b[np] = bb;
+}
+
+// get_T_coefficients_and_source: mach2d_coef.f90:787-1203
+__global__ __launch_bounds__(kTx) void k_T_coefficients_and_source(Fields F)
+{
+    long np;
+    if (!interior(F.nx, F.ny, np)) return;
+    const double *const *q = F.in;
+    const double *rp = q[0], *re = q[1], *rn = q[2], *xe = q[3], *ye = q[4], *xk = q[5],
+                 *yk = q[6], *xke = q[7], *yke = q[8], *xen = q[9], *yen = q[10],
+                 *alphae = q[11], *betae = q[12], *betan = q[13], *gamman = q[14], *Jp = q[15],
+                 *Je = q[16], *Jn = q[17], *roe = q[18], *ron = q[19], *ro = q[20],
+                 *roa = q[21], *p = q[22], *pa = q[23], *cp = q[24], *vle = q[25], *vln = q[26],
+                 *ke = q[27], *kn = q[28], *Uce = q[29], *Vcn = q[30], *u = q[31], *v = q[32],
+                 *ue = q[33], *ve = q[34], *un = q[35], *vn = q[36], *ua = q[37], *va = q[38],
+                 *T = q[39], *Ta = q[40], *fbe = q[41], *fbn = q[42];
+    double *a = F.out[0], *b = F.out[1];
+    const long nx = F.nx, N = (long)F.nx * F.ny;
+    const double dt = F.dt, beta = F.beta;
+    const double eps = 2.220446049250313080847e-16;  // epsilon(1.d0)
+    const long nps = np - nx, npn = np + nx, npw = np - 1, npe = np + 1;
+    const double fme = roe[np] * re[np] * Uce[np];
+    const double fmw = roe[npw] * re[npw] * Uce[npw];
+    const double fmn = ron[np] * rn[np] * Vcn[np];
+    const double fms = ron[nps] * rn[nps] * Vcn[nps];
+    const double mpa = roa[np] * rp[np] / Jp[np];
+    const double as = dsign5(Vcn[nps]), an = dsign5(Vcn[np]);
+    const double aw = dsign5(Uce[npw]), ae = dsign5(Uce[np]);
+    double c[9];
+    if (F.modvis == 0) {
+        c[0] = c[2] = c[6] = c[8] = 0.0;
+        c[1] = -fms * (0.5 + as) * cp[np];
+        c[3] = -fmw * (0.5 + aw) * cp[np];
+        c[5] = fme * (0.5 - ae) * cp[np];
+        c[7] = fmn * (0.5 - an) * cp[np];
+    } else {
+        const double ew = ke[npw] * re[npw] * Je[npw] * betae[npw];
+        const double ep = ke[np] * re[np] * Je[np] * betae[np];
+        const double ns = kn[nps] * rn[nps] * Jn[nps] * betan[nps];
+        const double nq = kn[np] * rn[np] * Jn[np] * betan[np];
+        c[0] = (ew + ns) / 4.0;
+        c[1] = -(fms * (0.5 + as) * cp[np]) - kn[nps] * rn[nps] * Jn[nps] * gamman[nps] +
+               (ew - ep) / 4.0;
+        c[2] = -(ep + ns) / 4.0;
+        c[3] = -(fmw * (0.5 + aw) * cp[np]) - ke[npw] * re[npw] * Je[npw] * alphae[npw] +
+               (ns - nq) / 4.0;
+        c[5] = fme * (0.5 - ae) * cp[np] - ke[np] * re[np] * Je[np] * alphae[np] +
+               (-ns + nq) / 4.0;
+        c[6] = (-ew - nq) / 4.0;
+        c[7] = fmn * (0.5 - an) * cp[np] - kn[np] * rn[np] * Jn[np] * gamman[np] - (ew - ep) / 4.0;
+        c[8] = (ep + nq) / 4.0;
+    }
+    c[4] = mpa * cp[np] / dt - (c[7] + c[1] + c[3] + c[5]);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k * N + np] = c[k];
+    double bb = mpa * Ta[np] * cp[np] / dt;
+    bb = bb - beta * cp[np] *
+                  (fme * ae * (T[npe] - T[np]) - fmw * aw * (T[np] - T[npw]) +
+                   fmn * an * (T[npn] - T[np]) - fms * as * (T[np] - T[nps]));
+    const double KpE = (u[npe] * u[npe] + v[npe] * v[npe]) / 2.0;
+    const double KpW = (u[npw] * u[npw] + v[npw] * v[npw]) / 2.0;
+    const double KpN = (u[npn] * u[npn] + v[npn] * v[npn]) / 2.0;
+    const double KpS = (u[nps] * u[nps] + v[nps] * v[nps]) / 2.0;
+    const double KpP = (u[np] * u[np] + v[np] * v[np]) / 2.0;
+    const double KpPa = (ua[np] * ua[np] + va[np] * va[np]) / 2.0;
+    double Kfe = (0.5 + ae) * KpP + (0.5 - ae) * KpE + beta * ae * (KpE - KpP);
+    double Kfw = (0.5 + aw) * KpW + (0.5 - aw) * KpP + beta * aw * (KpP - KpW);
+    double Kfn = (0.5 + an) * KpP + (0.5 - an) * KpN + beta * an * (KpN - KpP);
+    double Kfs = (0.5 + as) * KpS + (0.5 - as) * KpP + beta * as * (KpP - KpS);
+    const double Kfbe = (ue[np] * ue[np] + ve[np] * ve[np]) / 2.0;
+    const double Kfbw = (ue[npw] * ue[npw] + ve[npw] * ve[npw]) / 2.0;
+    const double Kfbn = (un[np] * un[np] + vn[np] * vn[np]) / 2.0;
+    const double Kfbs = (un[nps] * un[nps] + vn[nps] * vn[nps]) / 2.0;
+    Kfe = Kfe * (1.0 - fbe[np]) + Kfbe * fbe[np];
+    Kfw = Kfw * (1.0 - fbe[npw]) + Kfbw * fbe[npw];
+    Kfn = Kfn * (1.0 - fbn[np]) + Kfbn * fbn[np];
+    Kfs = Kfs * (1.0 - fbn[nps]) + Kfbs * fbn[nps];
+    double raux = rp[np] / Jp[np] * (ro[np] * KpP - roa[np] * KpPa) / dt + fme * Kfe -
+                  fmw * Kfw + fmn * Kfn - fms * Kfs;
+    bb = bb + rp[np] / Jp[np] * (p[np] - pa[np]) / dt - raux;
+    if (F.modvis != 0) {
+        const long npsw = nps - 1, npse = nps + 1, npnw = npn - 1, npne = npn + 1;
+        const double dudk_e = u[npe] - u[np];
+        const double dudk_w = u[np] - u[npw];
+        const double dudk_n = (u[npne] + u[npe] - u[npnw] - u[npw]) / 4.0;
+        const double dudk_s = (u[npse] + u[npe] - u[npsw] - u[npw]) / 4.0;
+        const double dude_e = (u[npne] + u[npn] - u[npse] - u[nps]) / 4.0;
+        const double dude_w = (u[npnw] + u[npn] - u[npsw] - u[nps]) / 4.0;
+        const double dude_n = u[npn] - u[np];
+        const double dude_s = u[np] - u[nps];
+        const double dvdk_e = v[npe] - v[np];
+        const double dvdk_w = v[np] - v[npw];
+        const double dvdk_n = (v[npne] + v[npe] - v[npnw] - v[npw]) / 4.0;
+        const double dvdk_s = (v[npse] + v[npe] - v[npsw] - v[npw]) / 4.0;
+        const double dvde_e = (v[npne] + v[npn] - v[npse] - v[nps]) / 4.0;
+        const double dvde_w = (v[npnw] + v[npn] - v[npsw] - v[nps]) / 4.0;
+        const double dvde_n = v[npn] - v[np];
+        const double dvde_s = v[np] - v[nps];
+        const double dudx_e = Je[np] * (ye[np] * dudk_e - yke[np] * dude_e);
+        const double dudx_w = Je[npw] * (ye[npw] * dudk_w - yke[npw] * dude_w);
+        const double dudx_n = Jn[np] * (yen[np] * dudk_n - yk[np] * dude_n);
+        const double dudx_s = Jn[nps] * (yen[nps] * dudk_s - yk[nps] * dude_s);
+        const double dudy_e = Je[np] * (xke[np] * dude_e - xe[np] * dudk_e);
+        const double dudy_w = Je[npw] * (xke[npw] * dude_w - xe[npw] * dudk_w);
+        const double dudy_n = Jn[np] * (xk[np] * dude_n - xen[np] * dudk_n);
+        const double dudy_s = Jn[nps] * (xk[nps] * dude_s - xen[nps] * dudk_s);
+        const double dvdx_e = Je[np] * (ye[np] * dvdk_e - yke[np] * dvde_e);
+        const double dvdx_w = Je[npw] * (ye[npw] * dvdk_w - yke[npw] * dvde_w);
+        const double dvdx_n = Jn[np] * (yen[np] * dvdk_n - yk[np] * dvde_n);
+        const double dvdx_s = Jn[nps] * (yen[nps] * dvdk_s - yk[nps] * dvde_s);
+        const double dvdy_e = Je[np] * (xke[np] * dvde_e - xe[np] * dvdk_e);
+        const double dvdy_w = Je[npw] * (xke[npw] * dvde_w - xe[npw] * dvdk_w);
+        const double dvdy_n = Jn[np] * (xk[np] * dvde_n - xen[np] * dvdk_n);
+        const double dvdy_s = Jn[nps] * (xk[nps] * dvde_s - xen[nps] * dvdk_s);
+        const double divU_e = dudx_e + dvdy_e + ve[np] / (fabs(re[np]) + eps);
+        const double divU_w = dudx_w + dvdy_w + ve[npw] / (fabs(re[npw]) + eps);
+        const double divU_n = dudx_n + dvdy_n + vn[np] / (fabs(rn[np]) + eps);
+        const double divU_s = dudx_s + dvdy_s + vn[nps] / (fabs(rn[nps]) + eps);
+        const double Txx_e = -2.0 * vle[np] * dudx_e + 2.0 / 3.0 * vle[np] * divU_e;
+        const double Txx_w = -2.0 * vle[npw] * dudx_w + 2.0 / 3.0 * vle[npw] * divU_w;
+        const double Txx_n = -2.0 * vln[np] * dudx_n + 2.0 / 3.0 * vln[np] * divU_n;
+        const double Txx_s = -2.0 * vln[nps] * dudx_s + 2.0 / 3.0 * vln[nps] * divU_s;
+        const double Txy_e = -vle[np] * (dvdx_e + dudy_e);
+        const double Txy_w = -vle[npw] * (dvdx_w + dudy_w);
+        const double Txy_n = -vln[np] * (dvdx_n + dudy_n);
+        const double Txy_s = -vln[nps] * (dvdx_s + dudy_s);
+        const double Tyy_e = -2.0 * vle[np] * dvdy_e + 2.0 / 3.0 * vle[np] * divU_e;
+        const double Tyy_w = -2.0 * vle[npw] * dvdy_w + 2.0 / 3.0 * vle[npw] * divU_w;
+        const double Tyy_n = -2.0 * vln[np] * dvdy_n + 2.0 / 3.0 * vln[np] * divU_n;
+        const double Tyy_s = -2.0 * vln[nps] * dvdy_s + 2.0 / 3.0 * vln[nps] * divU_s;
+        const double Lx_e = ve[np] * Txy_e + ue[np] * Txx_e;
+        const double Lx_w = ve[npw] * Txy_w + ue[npw] * Txx_w;
+        const double Lx_n = vn[np] * Txy_n + un[np] * Txx_n;
+        const double Lx_s = vn[nps] * Txy_s + un[nps] * Txx_s;
+        const double Ly_e = ve[np] * Tyy_e + ue[np] * Txy_e;
+        const double Ly_w = ve[npw] * Tyy_w + ue[npw] * Txy_w;
+        const double Ly_n = vn[np] * Tyy_n + un[np] * Txy_n;
+        const double Ly_s = vn[nps] * Tyy_s + un[nps] * Txy_s;
+        const double Lk_e = Lx_e * ye[np] - Ly_e * xe[np];
+        const double Lk_w = Lx_w * ye[npw] - Ly_w * xe[npw];
+        const double Le_n = Ly_n * xk[np] - Lx_n * yk[np];
+        const double Le_s = Ly_s * xk[nps] - Lx_s * yk[nps];
+        raux = re[np] * Lk_e - re[npw] * Lk_w + rn[np] * Le_n - rn[nps] * Le_s;
+        bb = bb - raux;
+    }
+    b[np] = bb;
+}
+
+}  // namespace m2d
+
+using namespace m2d;
+
+// ---------------------------------------------------------------------------
+// host side: one routine = kernel + field counts
+// ---------------------------------------------------------------------------
+namespace {
+
+thread_local hipStream_t g_stream = nullptr;
+
+struct Routine {
+    void (*kernel)(Fields);
+    int nin;
+    int nout;
+    int outk[3];  // N-vectors per output
+    int full;     // 1: grid over 1..nx-1 x 1..ny-1 (face densities)
+};
+
+bool is_device_ptr(const void *p)
+{
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();  // clear the error of an unknown host pointer
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice;
+}
+
+int run(const Routine &R, Fields F)
+{
+    const int nx = F.nx, ny = F.ny;
+    if (nx < 3 || ny < 3) return M2D_ERR_ARG;
+    for (int k = 0; k < R.nin; ++k)
+        if (!F.in[k]) return M2D_ERR_ARG;
+    for (int k = 0; k < R.nout; ++k)
+        if (!F.out[k]) return M2D_ERR_ARG;
+    const long N = (long)nx * ny;
+    const bool dev = is_device_ptr(F.in[0]);
+    for (int k = 0; k < R.nin; ++k)
+        if (is_device_ptr(F.in[k]) != dev) return M2D_ERR_ARG;
+    for (int k = 0; k < R.nout; ++k)
+        if (is_device_ptr(F.out[k]) != dev) return M2D_ERR_ARG;
+    dim3 grid, block(kTx);
+    if (R.full)
+        grid = dim3((unsigned)((nx - 1 + kTx - 1) / kTx), (unsigned)(ny - 1));
+    else
+        grid = dim3((unsigned)((nx - 2 + kTx - 1) / kTx), (unsigned)(ny - 2));
+    if (dev) {
+        hipLaunchKernelGGL(R.kernel, grid, block, 0, g_stream, F);
+        return hipGetLastError() == hipSuccess ? M2D_OK : M2D_ERR_HIP;
+    }
+    // host arrays: stage through one device buffer (outputs keep the values
+    // the routine does not write)
+    long tot = R.nin;
+    for (int k = 0; k < R.nout; ++k) tot += R.outk[k];
+    double *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(double) * N * tot) != hipSuccess) return M2D_ERR_HIP;
+    Fields G = F;
+    const double *hout[3] = {F.out[0], F.out[1], F.out[2]};
+    int rc = M2D_OK;
+    long off = 0;
+    for (int k = 0; k < R.nin && !rc; ++k, off += N) {
+        if (hipMemcpy(d + off, F.in[k], sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess)
+            rc = M2D_ERR_HIP;
+        G.in[k] = d + off;
+    }
+    for (int k = 0; k < R.nout && !rc; ++k) {
+        const long n = N * R.outk[k];
+        if (hipMemcpy(d + off, hout[k], sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
+            rc = M2D_ERR_HIP;
+        G.out[k] = d + off;
+        off += n;
+    }
+    if (!rc) {
+        hipLaunchKernelGGL(R.kernel, grid, block, 0, nullptr, G);
+        if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
+    }
+    for (int k = 0; k < R.nout && !rc; ++k)
+        if (hipMemcpy(F.out[k], G.out[k], sizeof(double) * N * R.outk[k],
+                      hipMemcpyDeviceToHost) != hipSuccess)
+            rc = M2D_ERR_HIP;
+    (void)hipFree(d);
+    return rc;
+}
+
+const Routine kDensity = {k_density_at_faces, 3, 2, {1, 1, 0}, 1};
+const Routine kU = {k_u_coefficients, 19, 1, {9, 0, 0}, 0};
+const Routine kV = {k_v_coefficients, 20, 1, {9, 0, 0}, 0};
+const Routine kUs = {k_u_source, 25, 3, {1, 1, 1}, 0};
+const Routine kVs = {k_v_source, 25, 3, {1, 1, 1}, 0};
+const Routine kT = {k_T_coefficients_and_source, 43, 2, {9, 1, 0}, 0};
+const Routine kPc = {k_p_coefficients, 11, 1, {5, 0, 0}, 0};
+const Routine kPs = {k_p_source, 14, 1, {1, 0, 0}, 0};
+
+Fields fields(const int *nx, const int *ny)
+{
+    Fields F;
+    memset(&F, 0, sizeof(F));
+    F.nx = *nx;
+    F.ny = *ny;
+    return F;
+}
+
+}  // namespace
+
+void m2d_set_last_status_code(int status);
+
+#define IN(k, x) F.in[k] = (x)
+#define FINISH(R)                               \
+    do {                                        \
+        const int rc_ = run(R, F);              \
+        m2d_set_last_status_code(rc_);          \
+    } while (0)
+#define CHECK_SCALARS(...)                                      \
+    do {                                                        \
+        const void *s_[] = {__VA_ARGS__};                       \
+        for (const void *p_ : s_)                               \
+            if (!p_) {                                          \
+                m2d_set_last_status_code(M2D_ERR_ARG);          \
+                return;                                         \
+            }                                                   \
+    } while (0)
+
+extern "C" void m2d_assembly_set_stream(void *stream) { g_stream = (hipStream_t)stream; }
+
+extern "C" void m2d_get_density_at_faces(const int *nx, const int *ny, const double *beta,
+                                         const double *ro, const double *Uce, const double *Vcn,
+                                         double *roe, double *ron)
+{
+    CHECK_SCALARS(nx, ny, beta);
+    Fields F = fields(nx, ny);
+    F.beta = *beta;
+    IN(0, ro);
+    IN(1, Uce);
+    IN(2, Vcn);
+    F.out[0] = roe;
+    F.out[1] = ron;
+    FINISH(kDensity);
+}
+
+extern "C" void m2d_get_u_coefficients(const int *nx, const int *ny, const int *modvis,
+                                       const double *dt, const double *rp, const double *re,
+                                       const double *rn, const double *Jp, const double *Je,
+                                       const double *Jn, const double *ye, const double *yk,
+                                       const double *alphae, const double *betae,
+                                       const double *betan, const double *gamman,
+                                       const double *vle, const double *vln, const double *roe,
+                                       const double *ron, const double *roa, const double *Uce,
+                                       const double *Vcn, double *a)
+{
+    CHECK_SCALARS(nx, ny, modvis, dt);
+    Fields F = fields(nx, ny);
+    F.modvis = *modvis;
+    F.dt = *dt;
+    const double *in[] = {rp, re, rn, Jp, Je, Jn, ye, yk, alphae, betae,
+                          betan, gamman, vle, vln, roe, ron, roa, Uce, Vcn};
+    for (int k = 0; k < 19; ++k) IN(k, in[k]);
+    F.out[0] = a;
+    FINISH(kU);
+}
+
+extern "C" void m2d_get_v_coefficients(const int *nx, const int *ny, const int *coord,
+                                       const int *modvis, const double *dt, const double *rp,
+                                       const double *re, const double *rn, const double *Jp,
+                                       const double *Je, const double *Jn, const double *xe,
+                                       const double *xk, const double *alphae,
+                                       const double *betae, const double *betan,
+                                       const double *gamman, const double *vle,
+                                       const double *vln, const double *vlp, const double *roe,
+                                       const double *ron, const double *roa, const double *Uce,
+                                       const double *Vcn, double *a)
+{
+    CHECK_SCALARS(nx, ny, coord, modvis, dt);
+    Fields F = fields(nx, ny);
+    F.coord = *coord;
+    F.modvis = *modvis;
+    F.dt = *dt;
+    const double *in[] = {rp,    re,    rn,     Jp,  Je,  Jn,  xe,  xk,  alphae, betae,
+                          betan, gamman, vle, vln, vlp, roe, ron, roa, Uce,    Vcn};
+    for (int k = 0; k < 20; ++k) IN(k, in[k]);
+    F.out[0] = a;
+    FINISH(kV);
+}
+
+extern "C" void m2d_get_u_source(const int *nx, const int *ny, const int *modvis,
+                                 const double *beta, const double *dt, const double *rp,
+                                 const double *re, const double *rn, const double *xe,
+                                 const double *ye, const double *xk, const double *yk,
+                                 const double *xke, const double *yke, const double *xen,
+                                 const double *yen, const double *Jp, const double *Je,
+                                 const double *Jn, const double *roe, const double *ron,
+                                 const double *roa, const double *p, const double *vle,
+                                 const double *vln, const double *Uce, const double *Vcn,
+                                 const double *ua, const double *u, const double *v, double *cup,
+                                 double *sup, double *b)
+{
+    CHECK_SCALARS(nx, ny, modvis, beta, dt);
+    Fields F = fields(nx, ny);
+    F.modvis = *modvis;
+    F.beta = *beta;
+    F.dt = *dt;
+    const double *in[] = {rp, re, rn, xe,  ye,  xk,  yk, xke, yke, xen, yen, Jp, Je,
+                          Jn, roe, ron, roa, p, vle, vln, Uce, Vcn, ua, u, v};
+    for (int k = 0; k < 25; ++k) IN(k, in[k]);
+    F.out[0] = cup;
+    F.out[1] = sup;
+    F.out[2] = b;
+    FINISH(kUs);
+}
+
+extern "C" void m2d_get_v_source(const int *nx, const int *ny, const int *coord,
+                                 const int *modvis, const double *beta, const double *dt,
+                                 const double *rp, const double *re, const double *rn,
+                                 const double *xe, const double *ye, const double *xk,
+                                 const double *yk, const double *xke, const double *yke,
+                                 const double *xen, const double *yen, const double *Jp,
+                                 const double *Je, const double *Jn, const double *roe,
+                                 const double *ron, const double *roa, const double *p,
+                                 const double *vle, const double *vln, const double *Uce,
+                                 const double *Vcn, const double *va, const double *u,
+                                 const double *v, double *cvp, double *svp, double *b)
+{
+    CHECK_SCALARS(nx, ny, coord, modvis, beta, dt);
+    Fields F = fields(nx, ny);
+    F.coord = *coord;
+    F.modvis = *modvis;
+    F.beta = *beta;
+    F.dt = *dt;
+    const double *in[] = {rp, re, rn, xe,  ye,  xk,  yk, xke, yke, xen, yen, Jp, Je,
+                          Jn, roe, ron, roa, p, vle, vln, Uce, Vcn, va, u, v};
+    for (int k = 0; k < 25; ++k) IN(k, in[k]);
+    F.out[0] = cvp;
+    F.out[1] = svp;
+    F.out[2] = b;
+    FINISH(kVs);
+}
+
+extern "C" void m2d_get_T_coefficients_and_source(
+    const int *nx, const int *ny, const int *modvis, const double *beta, const double *dt,
+    const double *rp, const double *re, const double *rn, const double *xe, const double *ye,
+    const double *xk, const double *yk, const double *xke, const double *yke, const double *xen,
+    const double *yen, const double *alphae, const double *betae, const double *betan,
+    const double *gamman, const double *Jp, const double *Je, const double *Jn,
+    const double *roe, const double *ron, const double *ro, const double *roa, const double *p,
+    const double *pa, const double *cp, const double *vle, const double *vln, const double *ke,
+    const double *kn, const double *Uce, const double *Vcn, const double *u, const double *v,
+    const double *ue, const double *ve, const double *un, const double *vn, const double *ua,
+    const double *va, const double *T, const double *Ta, const double *fbe, const double *fbn,
+    double *a, double *b)
+{
+    CHECK_SCALARS(nx, ny, modvis, beta, dt);
+    Fields F = fields(nx, ny);
+    F.modvis = *modvis;
+    F.beta = *beta;
+    F.dt = *dt;
+    const double *in[] = {rp,  re,  rn,  xe,  ye,  xk,  yk,  xke, yke, xen, yen,
+                          alphae, betae, betan, gamman, Jp, Je, Jn, roe, ron, ro, roa,
+                          p,   pa,  cp,  vle, vln, ke,  kn,  Uce, Vcn, u,   v,
+                          ue,  ve,  un,  vn,  ua,  va,  T,   Ta,  fbe, fbn};
+    for (int k = 0; k < 43; ++k) IN(k, in[k]);
+    F.out[0] = a;
+    F.out[1] = b;
+    FINISH(kT);
+}
+
+extern "C" void m2d_get_p_coefficients(const int *nx, const int *ny, const double *dt,
+                                       const double *rp, const double *re, const double *rn,
+                                       const double *Jp, const double *Uce, const double *Vcn,
+                                       const double *roe, const double *ron, const double *g,
+                                       const double *de, const double *dn, double *a)
+{
+    CHECK_SCALARS(nx, ny, dt);
+    Fields F = fields(nx, ny);
+    F.dt = *dt;
+    const double *in[] = {rp, re, rn, Jp, Uce, Vcn, roe, ron, g, de, dn};
+    for (int k = 0; k < 11; ++k) IN(k, in[k]);
+    F.out[0] = a;
+    FINISH(kPc);
+}
+
+extern "C" void m2d_get_p_source(const int *nx, const int *ny, const double *dt,
+                                 const double *rp, const double *re, const double *rn,
+                                 const double *Jp, const double *roe, const double *ron,
+                                 const double *ro, const double *roa, const double *Uce,
+                                 const double *Vcn, const double *de, const double *dn,
+                                 const double *g, const double *pl, double *b)
+{
+    CHECK_SCALARS(nx, ny, dt);
+    Fields F = fields(nx, ny);
+    F.dt = *dt;
+    const double *in[] = {rp, re, rn, Jp, roe, ron, ro, roa, Uce, Vcn, de, dn, g, pl};
+    for (int k = 0; k < 14; ++k) IN(k, in[k]);
+    F.out[0] = b;
+    FINISH(kPs);
+}

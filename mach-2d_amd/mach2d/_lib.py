@@ -51,7 +51,9 @@ EXPORTS = [
     "m2d_version", "m2d_enable_kernel_timing", "m2d_kernel_timing",
     "m2d_batch_create", "m2d_batch_destroy", "m2d_batch_set_stream", "m2d_batch_factor",
     "m2d_batch_solve", "m2d_batch_set_inner", "m2d_batch_set_tuning", "m2d_batch_get_tuning",
-    "m2d_batch_tune",
+    "m2d_batch_tune", "m2d_assembly_set_stream", "m2d_get_density_at_faces",
+    "m2d_get_u_coefficients", "m2d_get_u_source", "m2d_get_v_coefficients", "m2d_get_v_source",
+    "m2d_get_T_coefficients_and_source", "m2d_get_p_coefficients", "m2d_get_p_source",
 ]
 
 _lib = None
@@ -117,6 +119,11 @@ def load():
               L.m2d_batch_factor, L.m2d_batch_solve, L.m2d_batch_set_inner,
               L.m2d_batch_set_tuning, L.m2d_batch_get_tuning, L.m2d_batch_tune):
         f.restype = C.c_int
+    L.m2d_assembly_set_stream.argtypes = [_vp]
+    L.m2d_assembly_set_stream.restype = None
+    for nm in ("density_at_faces", "u_coefficients", "u_source", "v_coefficients", "v_source",
+               "T_coefficients_and_source", "p_coefficients", "p_source"):
+        getattr(L, "m2d_get_" + nm).restype = None   # pointer arguments, by reference
     L.m2d_tdma2d9_device.argtypes = [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]
     L.m2d_tdma2d9_device.restype = C.c_int
     L.m2d_enable_kernel_timing.argtypes = [_vp, C.c_int]

@@ -1,0 +1,120 @@
+"""Coefficient assembly on the GPU: the routines of module `coefficients`
+(src/equations/mach2d_coef.f90) through the C ABI (m2d_get_*).
+
+ROUTINES lists each routine's arguments in the reference's order: scalars
+(name, 'i' | 'd'), input fields, outputs (name, number of N-vectors).
+call() takes the fields as numpy arrays (copied to the GPU and back) or as
+float64 CUDA tensors (processed in place on the device)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import SolverError
+
+ROUTINES = {
+    # mach2d_coef.f90:20
+    "density_at_faces": ([("beta", "d")], ["ro", "Uce", "Vcn"], [("roe", 1), ("ron", 1)]),
+    # :63
+    "u_coefficients": ([("modvis", "i"), ("dt", "d")],
+                       ["rp", "re", "rn", "Jp", "Je", "Jn", "ye", "yk", "alphae", "betae",
+                        "betan", "gamman", "vle", "vln", "roe", "ron", "roa", "Uce", "Vcn"],
+                       [("a", 9)]),
+    # :454
+    "v_coefficients": ([("coord", "i"), ("modvis", "i"), ("dt", "d")],
+                       ["rp", "re", "rn", "Jp", "Je", "Jn", "xe", "xk", "alphae", "betae",
+                        "betan", "gamman", "vle", "vln", "vlp", "roe", "ron", "roa", "Uce",
+                        "Vcn"],
+                       [("a", 9)]),
+    # :221
+    "u_source": ([("modvis", "i"), ("beta", "d"), ("dt", "d")],
+                 ["rp", "re", "rn", "xe", "ye", "xk", "yk", "xke", "yke", "xen", "yen", "Jp",
+                  "Je", "Jn", "roe", "ron", "roa", "p", "vle", "vln", "Uce", "Vcn", "ua", "u",
+                  "v"],
+                 [("cup", 1), ("sup", 1), ("b", 1)]),
+    # :616
+    "v_source": ([("coord", "i"), ("modvis", "i"), ("beta", "d"), ("dt", "d")],
+                 ["rp", "re", "rn", "xe", "ye", "xk", "yk", "xke", "yke", "xen", "yen", "Jp",
+                  "Je", "Jn", "roe", "ron", "roa", "p", "vle", "vln", "Uce", "Vcn", "va", "u",
+                  "v"],
+                 [("cvp", 1), ("svp", 1), ("b", 1)]),
+    # :787
+    "T_coefficients_and_source": (
+        [("modvis", "i"), ("beta", "d"), ("dt", "d")],
+        ["rp", "re", "rn", "xe", "ye", "xk", "yk", "xke", "yke", "xen", "yen", "alphae",
+         "betae", "betan", "gamman", "Jp", "Je", "Jn", "roe", "ron", "ro", "roa", "p", "pa",
+         "cp", "vle", "vln", "ke", "kn", "Uce", "Vcn", "u", "v", "ue", "ve", "un", "vn", "ua",
+         "va", "T", "Ta", "fbe", "fbn"],
+        [("a", 9), ("b", 1)]),
+    # :1207
+    "p_coefficients": ([("dt", "d")],
+                       ["rp", "re", "rn", "Jp", "Uce", "Vcn", "roe", "ron", "g", "de", "dn"],
+                       [("a", 5)]),
+    # :1281
+    "p_source": ([("dt", "d")],
+                 ["rp", "re", "rn", "Jp", "roe", "ron", "ro", "roa", "Uce", "Vcn", "de", "dn",
+                  "g", "pl"],
+                 [("b", 1)]),
+}
+
+
+def _ptr(x, n, name):
+    try:
+        import torch
+        if isinstance(x, torch.Tensor):
+            if not (x.dtype == torch.float64 and x.is_contiguous() and x.numel() == n):
+                raise TypeError("%s: contiguous float64 tensor of %d values expected" % (name, n))
+            return C.c_void_p(x.data_ptr()), x
+    except ImportError:  # pragma: no cover
+        pass
+    if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous
+            and x.size == n):
+        raise TypeError("%s: contiguous float64 array of %d values expected" % (name, n))
+    return C.c_void_p(x.ctypes.data), x
+
+
+def call(name, nx, ny, scalars, fields, out=None):
+    """Run mach2d_coef.f90's get_<name>.  Outputs: `out` supplies arrays to
+    write into (host arrays, or CUDA tensors when the fields are CUDA
+    tensors); missing ones are allocated as zeros like the fields.  Returns
+    {name: array} ((k, N) coefficient arrays, (N,) vectors)."""
+    scal, names, outs = ROUTINES[name]
+    n = nx * ny
+    L = _lib.load()
+    fn = getattr(L, "m2d_get_" + name)
+    keep = []
+    args = [C.byref(C.c_int(nx)), C.byref(C.c_int(ny))]
+    for k, t in scal:
+        v = C.c_int(int(scalars[k])) if t == "i" else C.c_double(float(scalars[k]))
+        keep.append(v)
+        args.append(C.byref(v))
+    for k in names:
+        p, obj = _ptr(fields[k], n, k)
+        keep.append(obj)
+        args.append(p)
+    first = fields[names[0]]
+    res = {}
+    for k, m in outs:
+        arr = None if out is None else out.get(k)
+        if arr is None:
+            if isinstance(first, np.ndarray):
+                arr = np.zeros(m * n)
+            else:
+                import torch
+                arr = torch.zeros(m * n, dtype=torch.float64, device=first.device)
+        p, obj = _ptr(arr, m * n, k)
+        keep.append(obj)
+        args.append(p)
+        res[k] = arr
+    fn(*args)
+    st = _lib.last_status()
+    if st.status:
+        raise SolverError(st.status, "m2d_get_" + name)
+    return {k: (v.reshape(m, n) if m > 1 else v) for (k, m), v in zip(outs, res.values())}
+
+
+def set_stream(stream=None):
+    """Stream (hipStream_t as int) for device-array calls; None = default."""
+    _lib.load().m2d_assembly_set_stream(C.c_void_p(stream) if stream else None)

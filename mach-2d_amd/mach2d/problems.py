@@ -275,3 +275,67 @@ def cfg4_sizes(count: int = 256, seed: int = 4):
     rng = np.random.default_rng(seed)
     s = np.exp(rng.uniform(0.0, math.log(8.0), size=count))
     return [(int(round(64 * v)) + 2, int(round(32 * v)) + 2) for v in s]
+
+
+def flow_fields(nx: int, ny: int, seed: int = 0, q: float = 1.05, coord: int = 1,
+                cartesian: bool = False) -> dict:
+    """Seeded inputs of the coefficient assembly (mach2d_coef.f90) on the
+    ogive grid (or, with cartesian=True, a uniform orthogonal grid), as flat
+    N-vectors in the reference layout: metrics, radii (r = y when coord = 1,
+    else 1), densities, viscosities, pressure, velocities and contravariant
+    face velocities of both signs."""
+    rng = np.random.default_rng(seed)
+    if cartesian:
+        xs = np.linspace(0.0, 1.0, nx - 1)
+        ys = np.linspace(0.5, 1.5, ny - 1)
+        x = np.zeros((ny, nx))
+        y = np.zeros((ny, nx))
+        x[: ny - 1, : nx - 1] = xs[None, :]
+        y[: ny - 1, : nx - 1] = ys[:, None]
+        g = Grid(nx, ny, x, y, 1.0)
+    else:
+        g = ogive_grid(nx, ny, q)
+    m = metrics(g)
+    f = {k: np.ascontiguousarray(v.reshape(-1)) for k, v in m.items()}
+    N = nx * ny
+
+    def pos(lo, hi):
+        return rng.uniform(lo, hi, size=N)
+
+    def signed(scale):
+        return rng.uniform(-scale, scale, size=N)
+
+    # radii at centres and faces: the local y when axisymmetric
+    yp = np.abs(f["yp"]) + 0.05
+    f["rp"] = yp if coord == 1 else np.ones(N)
+    f["re"] = (yp * pos(0.95, 1.05)) if coord == 1 else np.ones(N)
+    f["rn"] = (yp * pos(0.95, 1.05)) if coord == 1 else np.ones(N)
+    for k in ("ro", "roa", "roe", "ron"):
+        f[k] = pos(0.8, 1.2)
+    for k in ("vle", "vln", "vlp"):
+        f[k] = pos(1e-3, 2e-3)
+    f["p"] = pos(0.9e5, 1.1e5)
+    f["pl"] = signed(50.0)
+    f["g"] = pos(1e-5, 1.3e-5)
+    f["de"] = pos(1e-4, 1e-3)
+    f["dn"] = pos(1e-4, 1e-3)
+    for k in ("u", "v", "ua", "va"):
+        f[k] = signed(100.0)
+    f["Uce"] = signed(1.0)
+    f["Vcn"] = signed(1.0)
+    f["T"] = pos(280.0, 320.0)
+    f["Ta"] = pos(280.0, 320.0)
+    f["cp"] = pos(1000.0, 1010.0)
+    f["pa"] = pos(0.9e5, 1.1e5)
+    f["ke"] = pos(0.02, 0.03)
+    f["kn"] = pos(0.02, 0.03)
+    for k in ("ue", "ve", "un", "vn"):
+        f[k] = signed(100.0)
+    # boundary-face flags (1 on faces of the domain boundary, else 0)
+    fbe = np.zeros((ny, nx))
+    fbn = np.zeros((ny, nx))
+    fbe[:, 0] = fbe[:, nx - 2] = 1.0
+    fbn[0, :] = fbn[ny - 2, :] = 1.0
+    f["fbe"] = fbe.reshape(-1)
+    f["fbn"] = fbn.reshape(-1)
+    return f

@@ -64,4 +64,31 @@ void orc_residual9(const double *b, int ni, int nj, const double *var,
 #ifdef __cplusplus
 }
 #endif
+
+/* coef.c: coefficient assembly of src/equations/mach2d_coef.f90 */
+void orc_density_at_faces(int nx, int ny, double beta, const double *ro, const double *Uce,
+                          const double *Vcn, double *roe, double *ron);
+void orc_u_coefficients(int nx, int ny, int modvis, double dt, const double *rp, const double *re,
+                        const double *rn, const double *Jp, const double *Je, const double *Jn,
+                        const double *ye, const double *yk, const double *alphae,
+                        const double *betae, const double *betan, const double *gamman,
+                        const double *vle, const double *vln, const double *roe,
+                        const double *ron, const double *roa, const double *Uce,
+                        const double *Vcn, double *a);
+void orc_v_coefficients(int nx, int ny, int coord, int modvis, double dt, const double *rp,
+                        const double *re, const double *rn, const double *Jp, const double *Je,
+                        const double *Jn, const double *xe, const double *xk,
+                        const double *alphae, const double *betae, const double *betan,
+                        const double *gamman, const double *vle, const double *vln,
+                        const double *vlp, const double *roe, const double *ron,
+                        const double *roa, const double *Uce, const double *Vcn, double *a);
+void orc_p_coefficients(int nx, int ny, double dt, const double *rp, const double *re,
+                        const double *rn, const double *Jp, const double *Uce, const double *Vcn,
+                        const double *roe, const double *ron, const double *g, const double *de,
+                        const double *dn, double *a);
+void orc_p_source(int nx, int ny, double dt, const double *rp, const double *re,
+                  const double *rn, const double *Jp, const double *roe, const double *ron,
+                  const double *ro, const double *roa, const double *Uce, const double *Vcn,
+                  const double *de, const double *dn, const double *g, const double *pl,
+                  double *b);
 #endif

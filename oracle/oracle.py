@@ -92,14 +92,100 @@ def ref():
     return _ref
 
 
+# --------------------------------------------------------------------------
+# Coefficient assembly (src/equations/mach2d_coef.f90): argument lists in the
+# reference's order.  name -> (scalars [(name, 'i'|'d')], input fields,
+# outputs [(name, number of N-vectors)]).  The C restatement is orc_<name>,
+# the compiled reference ref_<name>; both take (nx, ny, scalars..., fields...,
+# outputs...).
+# --------------------------------------------------------------------------
+COEF = {
+    "density_at_faces": ([("beta", "d")], ["ro", "Uce", "Vcn"], [("roe", 1), ("ron", 1)]),
+    "u_coefficients": ([("modvis", "i"), ("dt", "d")],
+                       ["rp", "re", "rn", "Jp", "Je", "Jn", "ye", "yk", "alphae", "betae",
+                        "betan", "gamman", "vle", "vln", "roe", "ron", "roa", "Uce", "Vcn"],
+                       [("a", 9)]),
+    "v_coefficients": ([("coord", "i"), ("modvis", "i"), ("dt", "d")],
+                       ["rp", "re", "rn", "Jp", "Je", "Jn", "xe", "xk", "alphae", "betae",
+                        "betan", "gamman", "vle", "vln", "vlp", "roe", "ron", "roa", "Uce",
+                        "Vcn"],
+                       [("a", 9)]),
+    "u_source": ([("modvis", "i"), ("beta", "d"), ("dt", "d")],
+                 ["rp", "re", "rn", "xe", "ye", "xk", "yk", "xke", "yke", "xen", "yen", "Jp",
+                  "Je", "Jn", "roe", "ron", "roa", "p", "vle", "vln", "Uce", "Vcn", "ua", "u",
+                  "v"],
+                 [("cup", 1), ("sup", 1), ("b", 1)]),
+    "v_source": ([("coord", "i"), ("modvis", "i"), ("beta", "d"), ("dt", "d")],
+                 ["rp", "re", "rn", "xe", "ye", "xk", "yk", "xke", "yke", "xen", "yen", "Jp",
+                  "Je", "Jn", "roe", "ron", "roa", "p", "vle", "vln", "Uce", "Vcn", "va", "u",
+                  "v"],
+                 [("cvp", 1), ("svp", 1), ("b", 1)]),
+    "T_coefficients_and_source": (
+        [("modvis", "i"), ("beta", "d"), ("dt", "d")],
+        ["rp", "re", "rn", "xe", "ye", "xk", "yk", "xke", "yke", "xen", "yen", "alphae",
+         "betae", "betan", "gamman", "Jp", "Je", "Jn", "roe", "ron", "ro", "roa", "p", "pa",
+         "cp", "vle", "vln", "ke", "kn", "Uce", "Vcn", "u", "v", "ue", "ve", "un", "vn", "ua",
+         "va", "T", "Ta", "fbe", "fbn"],
+        [("a", 9), ("b", 1)]),
+    "p_coefficients": ([("dt", "d")],
+                       ["rp", "re", "rn", "Jp", "Uce", "Vcn", "roe", "ron", "g", "de", "dn"],
+                       [("a", 5)]),
+    "p_source": ([("dt", "d")],
+                 ["rp", "re", "rn", "Jp", "roe", "ron", "ro", "roa", "Uce", "Vcn", "de", "dn",
+                  "g", "pl"],
+                 [("b", 1)]),
+}
+
+
+def _coef_argtypes(name):
+    scal, fields, outs = COEF[name]
+    t = [C.c_int, C.c_int]
+    t += [C.c_int if k == "i" else C.c_double for _, k in scal]
+    t += [_dp] * (len(fields) + len(outs))
+    return t
+
+
 def _bind_coef(L):
-    """Assembly restatements (coef.c), bound when present."""
-    if hasattr(L, "orc_coef_bind"):
-        L.orc_coef_bind()
+    for name in COEF:
+        f = getattr(L, "orc_" + name, None)
+        if f is not None:
+            f.argtypes = _coef_argtypes(name)
+            f.restype = None
 
 
 def _bind_ref_coef(R):
-    pass
+    for name in COEF:
+        f = getattr(R, "ref_" + name, None)
+        if f is not None:
+            f.argtypes = _coef_argtypes(name)
+            f.restype = None
+
+
+def _coef_call(fn, name, nx, ny, scalars, fields, out):
+    scal, names, outs = COEF[name]
+    n = nx * ny
+    args = [nx, ny] + [int(scalars[k]) if t == "i" else float(scalars[k]) for k, t in scal]
+    args += [_c(fields[k]) for k in names]
+    res = {}
+    for k, m in outs:
+        init = None if out is None else out.get(k)
+        arr = np.zeros((m, n)) if init is None else _c(init).reshape(m, n).copy()
+        res[k] = arr
+        args.append(arr)
+    fn(*args)
+    return {k: (v[0] if v.shape[0] == 1 else v) for k, v in res.items()}
+
+
+def coef(name, nx, ny, scalars, fields, out=None):
+    """C restatement of mach2d_coef.f90's `get_<name>`: returns the outputs
+    ((k, N) for coefficient arrays, (N,) for vectors).  Only interior points
+    are written; `out` gives initial output values (default zeros)."""
+    return _coef_call(getattr(lib(), "orc_" + name), name, nx, ny, scalars, fields, out)
+
+
+def ref_coef(name, nx, ny, scalars, fields, out=None):
+    """The compiled reference routine, same convention as coef()."""
+    return _coef_call(getattr(ref(), "ref_" + name), name, nx, ny, scalars, fields, out)
 
 
 def _c(a):

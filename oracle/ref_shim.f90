@@ -93,4 +93,109 @@ contains
       ref_norm_l1_b = norm_l1_b(nx, ny, b)
    end function
 
+
+   ! ---- coefficient assembly, src/equations/mach2d_coef.f90 -----------------
+
+   ! :20
+   subroutine ref_density_at_faces(nx, ny, beta, ro, Uce, Vcn, roe, ron) &
+         bind(C, name="ref_density_at_faces")
+      integer(c_int), value :: nx, ny
+      real(c_double), value :: beta
+      real(c_double), intent(in)    :: ro(nx*ny), Uce(nx*ny), Vcn(nx*ny)
+      real(c_double), intent(inout) :: roe(nx*ny), ron(nx*ny)
+      call get_density_at_faces(nx, ny, beta, ro, Uce, Vcn, roe, ron)
+   end subroutine
+
+   ! :63
+   subroutine ref_u_coefficients(nx, ny, modvis, dt, rp, re, rn, Jp, Je, Jn, ye, yk, alphae, &
+         betae, betan, gamman, vle, vln, roe, ron, roa, Uce, Vcn, a) &
+         bind(C, name="ref_u_coefficients")
+      integer(c_int), value :: nx, ny, modvis
+      real(c_double), value :: dt
+      real(c_double), dimension(nx*ny), intent(in) :: rp, re, rn, Jp, Je, Jn, ye, yk, alphae, &
+         betae, betan, gamman, vle, vln, roe, ron, roa, Uce, Vcn
+      real(c_double), intent(inout) :: a(nx*ny,9)
+      call get_u_coefficients(nx, ny, modvis, dt, rp, re, rn, Jp, Je, Jn, ye, yk, alphae, &
+         betae, betan, gamman, vle, vln, roe, ron, roa, Uce, Vcn, a)
+   end subroutine
+
+   ! :454
+   subroutine ref_v_coefficients(nx, ny, coord, modvis, dt, rp, re, rn, Jp, Je, Jn, xe, xk, &
+         alphae, betae, betan, gamman, vle, vln, vlp, roe, ron, roa, Uce, Vcn, a) &
+         bind(C, name="ref_v_coefficients")
+      integer(c_int), value :: nx, ny, coord, modvis
+      real(c_double), value :: dt
+      real(c_double), dimension(nx*ny), intent(in) :: rp, re, rn, Jp, Je, Jn, xe, xk, alphae, &
+         betae, betan, gamman, vle, vln, vlp, roe, ron, roa, Uce, Vcn
+      real(c_double), intent(inout) :: a(nx*ny,9)
+      call get_v_coefficients(nx, ny, coord, modvis, dt, rp, re, rn, Jp, Je, Jn, xe, xk, &
+         alphae, betae, betan, gamman, vle, vln, vlp, roe, ron, roa, Uce, Vcn, a)
+   end subroutine
+
+   ! :1207
+   subroutine ref_p_coefficients(nx, ny, dt, rp, re, rn, Jp, Uce, Vcn, roe, ron, g, de, dn, a) &
+         bind(C, name="ref_p_coefficients")
+      integer(c_int), value :: nx, ny
+      real(c_double), value :: dt
+      real(c_double), dimension(nx*ny), intent(in) :: rp, re, rn, Jp, Uce, Vcn, roe, ron, g, &
+         de, dn
+      real(c_double), intent(inout) :: a(nx*ny,5)
+      call get_p_coefficients(nx, ny, dt, rp, re, rn, Jp, Uce, Vcn, roe, ron, g, de, dn, a)
+   end subroutine
+
+   ! :1281
+   subroutine ref_p_source(nx, ny, dt, rp, re, rn, Jp, roe, ron, ro, roa, Uce, Vcn, de, dn, &
+         g, pl, b) bind(C, name="ref_p_source")
+      integer(c_int), value :: nx, ny
+      real(c_double), value :: dt
+      real(c_double), dimension(nx*ny), intent(in) :: rp, re, rn, Jp, roe, ron, ro, roa, Uce, &
+         Vcn, de, dn, g, pl
+      real(c_double), intent(inout) :: b(nx*ny)
+      call get_p_source(nx, ny, dt, rp, re, rn, Jp, roe, ron, ro, roa, Uce, Vcn, de, dn, g, &
+         pl, b)
+   end subroutine
+
+   ! :221
+   subroutine ref_u_source(nx, ny, modvis, beta, dt, rp, re, rn, xe, ye, xk, yk, xke, yke, &
+         xen, yen, Jp, Je, Jn, roe, ron, roa, p, vle, vln, Uce, Vcn, ua, u, v, cup, sup, b) &
+         bind(C, name="ref_u_source")
+      integer(c_int), value :: nx, ny, modvis
+      real(c_double), value :: beta, dt
+      real(c_double), dimension(nx*ny), intent(in) :: rp, re, rn, xe, ye, xk, yk, xke, yke, &
+         xen, yen, Jp, Je, Jn, roe, ron, roa, p, vle, vln, Uce, Vcn, ua, u, v
+      real(c_double), dimension(nx*ny), intent(inout) :: cup, sup, b
+      call get_u_source(nx, ny, modvis, beta, dt, rp, re, rn, xe, ye, xk, yk, xke, yke, &
+         xen, yen, Jp, Je, Jn, roe, ron, roa, p, vle, vln, Uce, Vcn, ua, u, v, cup, sup, b)
+   end subroutine
+
+   ! :616
+   subroutine ref_v_source(nx, ny, coord, modvis, beta, dt, rp, re, rn, xe, ye, xk, yk, xke, &
+         yke, xen, yen, Jp, Je, Jn, roe, ron, roa, p, vle, vln, Uce, Vcn, va, u, v, cvp, svp, &
+         b) bind(C, name="ref_v_source")
+      integer(c_int), value :: nx, ny, coord, modvis
+      real(c_double), value :: beta, dt
+      real(c_double), dimension(nx*ny), intent(in) :: rp, re, rn, xe, ye, xk, yk, xke, yke, &
+         xen, yen, Jp, Je, Jn, roe, ron, roa, p, vle, vln, Uce, Vcn, va, u, v
+      real(c_double), dimension(nx*ny), intent(inout) :: cvp, svp, b
+      call get_v_source(nx, ny, coord, modvis, beta, dt, rp, re, rn, xe, ye, xk, yk, xke, &
+         yke, xen, yen, Jp, Je, Jn, roe, ron, roa, p, vle, vln, Uce, Vcn, va, u, v, cvp, svp, b)
+   end subroutine
+
+   ! :787
+   subroutine ref_T_coefficients_and_source(nx, ny, modvis, beta, dt, rp, re, rn, xe, ye, &
+         xk, yk, xke, yke, xen, yen, alphae, betae, betan, gamman, Jp, Je, Jn, roe, ron, ro, &
+         roa, p, pa, cp, vle, vln, ke, kn, Uce, Vcn, u, v, ue, ve, un, vn, ua, va, T, Ta, fbe, &
+         fbn, a, b) bind(C, name="ref_T_coefficients_and_source")
+      integer(c_int), value :: nx, ny, modvis
+      real(c_double), value :: beta, dt
+      real(c_double), dimension(nx*ny), intent(in) :: rp, re, rn, xe, ye, xk, yk, xke, yke, &
+         xen, yen, alphae, betae, betan, gamman, Jp, Je, Jn, roe, ron, ro, roa, p, pa, cp, &
+         vle, vln, ke, kn, Uce, Vcn, u, v, ue, ve, un, vn, ua, va, T, Ta, fbe, fbn
+      real(c_double), intent(inout) :: a(nx*ny,9), b(nx*ny)
+      call get_T_coefficients_and_source(nx, ny, modvis, beta, dt, rp, re, rn, xe, ye, xk, &
+         yk, xke, yke, xen, yen, alphae, betae, betan, gamman, Jp, Je, Jn, roe, ron, ro, roa, &
+         p, pa, cp, vle, vln, ke, kn, Uce, Vcn, u, v, ue, ve, un, vn, ua, va, T, Ta, fbe, fbn, &
+         a, b)
+   end subroutine
+
 end module m2dref_shim

@@ -20,6 +20,13 @@ Fixtures (float64, diagonal-major arrays (k, nx*ny)):
   embed5.npz      lu2d5 + fb2d5 against lu2d9 + fb2d9 on the zero-corner
                   embedding of the same 5-point system
   solver_msi9d.npz  class_solver_msi9d init + solve (the plugin object)
+  coef.npz        every routine of module `coefficients` used by the hot path
+                  (face densities; u, v, T, p' coefficients and sources) on
+                  12x8 and 20x11 ogive grids (axisymmetric Navier-Stokes,
+                  planar Euler, planar Navier-Stokes), a 10x7 Cartesian grid,
+                  and a 12x8 case with faces on the axis (r = 0) for the
+                  axis branches of get_u_source; outputs start from seeded
+                  values so untouched points are checked too
 """
 from __future__ import annotations
 
@@ -143,6 +150,44 @@ def solver_msi9d():
          nitm=np.array(20), tol=np.array(1e-2))
 
 
+COEF_CASES = [  # tag, nx, ny, coord, modvis, cartesian, axis faces
+    ("ns_axi", 12, 8, 1, 1, False, False),
+    ("euler", 12, 8, 0, 0, False, False),
+    ("ns_planar", 20, 11, 0, 1, False, False),
+    ("cart", 10, 7, 1, 1, True, False),
+    ("axis", 12, 8, 1, 1, False, True),
+]
+
+
+def coef_fields(nx, ny, coord, cart, axis, seed):
+    f = P.flow_fields(nx, ny, seed=seed, coord=coord, cartesian=cart)
+    if axis:  # east/north faces of a few points on the axis
+        f["re"][nx * 3 + 4] = 0.0      # point (6, 4) west face / point (5, 4) east face
+        f["re"][nx * 5 + 8] = 0.0
+        f["rn"][nx * 2 + 6] = 0.0      # south face of point (7, 4) / north of (7, 3)
+        f["rn"][nx * 5 + 3] = 0.0
+    return f
+
+
+def coef():
+    arrs = {}
+    rng = np.random.default_rng(91)
+    for tag, nx, ny, coord, mv, cart, axis in COEF_CASES:
+        f = coef_fields(nx, ny, coord, cart, axis, seed=nx * 31 + ny + coord)
+        sc = dict(beta=0.35, modvis=mv, dt=2e-3, coord=coord)
+        for k, v in f.items():
+            arrs["%s_in_%s" % (tag, k)] = v
+        arrs["%s_scalars" % tag] = np.array([nx, ny, coord, mv, sc["beta"], sc["dt"]])
+        for name, (_, _, outs) in O.COEF.items():
+            init = {k: rng.uniform(-1, 1, size=(m, nx * ny)) for k, m in outs}
+            for k, v in init.items():
+                arrs["%s_%s_init_%s" % (tag, name, k)] = v
+            res = O.ref_coef(name, nx, ny, sc, f, out={k: v.copy() for k, v in init.items()})
+            for k, v in res.items():
+                arrs["%s_%s_%s" % (tag, name, k)] = v
+    save("coef.npz", **arrs)
+
+
 if __name__ == "__main__":
     O.build()
     if not O.ref_available():
@@ -153,3 +198,4 @@ if __name__ == "__main__":
     norms()
     embed5()
     solver_msi9d()
+    coef()

@@ -120,3 +120,30 @@ def test_oracle_matches_live_reference_when_built(orc):
     v, nit, _, _ = orc.fb2d9(s.a, dl, du, s.nx, s.ny, np.zeros(s.N), 1e-8, 300, s.rhs)
     rv = orc.ref_fb2d9(s.a, dl, du, s.nx, s.ny, np.zeros(s.N), 1e-8, 300, s.rhs)
     assert np.array_equal(v, rv)
+
+
+# ---- coefficient assembly (mach2d_coef.f90) -----------------------------------
+COEF_TAGS = ["ns_axi", "euler", "ns_planar", "cart", "axis"]
+
+
+def coef_case(g, tag):
+    nx, ny, coord, mv, beta, dt = g["%s_scalars" % tag]
+    nx, ny, coord, mv = int(nx), int(ny), int(coord), int(mv)
+    pre = "%s_in_" % tag
+    fields = {k[len(pre):]: v for k, v in g.items() if k.startswith(pre)}
+    return nx, ny, dict(coord=coord, modvis=mv, beta=beta, dt=dt), fields
+
+
+@pytest.mark.parametrize("tag", COEF_TAGS)
+def test_coef_oracle_vs_reference_fixtures(orc, tag):
+    """The C restatement of every assembly routine equals the reference's
+    output bit for bit, including the points the routine leaves untouched."""
+    g = load_golden("coef.npz")
+    nx, ny, sc, f = coef_case(g, tag)
+    for name, (_, _, outs) in orc.COEF.items():
+        init = {k: g["%s_%s_init_%s" % (tag, name, k)] for k, _ in outs}
+        got = orc.coef(name, nx, ny, sc, f, out=init)
+        for k, _ in outs:
+            want = g["%s_%s_%s" % (tag, name, k)]
+            assert np.array_equal(got[k], want), (tag, name, k)
+            assert np.all(np.isfinite(want)), (tag, name, k)

@@ -235,3 +235,135 @@ def test_tdma_converges_to_solution(orc):
     s = P.random_f3_system(nx, ny, seed=21, dominance=2.0)
     x = orc.tdma2d9(s.a, s.rhs, np.zeros(s.N), nx, ny, 60)
     assert np.allclose(x, np.linalg.solve(dense_A(s.a, nx, ny), s.rhs), rtol=1e-10, atol=1e-12)
+
+
+# ---- coefficient assembly: properties the mathematics fixes ------------------
+def _flow(nx=14, ny=9, coord=1, cart=False, seed=5):
+    from mach2d import problems as P
+    return P.flow_fields(nx, ny, seed=seed, coord=coord, cartesian=cart)
+
+
+def _interior(nx, ny):
+    m = np.zeros((ny, nx), bool)
+    m[1:-1, 1:-1] = True
+    return m.reshape(-1)
+
+
+def test_face_density_pure_upwind_at_beta0(orc):
+    nx, ny = 14, 9
+    f = _flow(nx, ny)
+    r = orc.coef("density_at_faces", nx, ny, dict(beta=0.0), f)
+    ro, U, V = f["ro"], f["Uce"], f["Vcn"]
+    for j in range(1, ny - 1):          # east faces: rows 2..ny-1, i = 1..nx-1
+        for i in range(0, nx - 1):
+            p = j * nx + i
+            assert r["roe"][p] == (ro[p] if U[p] > 0 else ro[p + 1])
+    for j in range(0, ny - 1):          # north faces
+        for i in range(1, nx - 1):
+            p = j * nx + i
+            assert r["ron"][p] == (ro[p] if V[p] > 0 else ro[p + nx])
+
+
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+@pytest.mark.parametrize("name", ["u_coefficients", "v_coefficients"])
+def test_euler_coefficients_are_upwind(orc, name, sign):
+    nx, ny = 14, 9
+    f = _flow(nx, ny)
+    f["Uce"] = sign * np.abs(f["Uce"]) + sign * 0.1
+    f["Vcn"] = sign * np.abs(f["Vcn"]) + sign * 0.1
+    a = orc.coef(name, nx, ny, dict(modvis=0, dt=1e-3, coord=1), f)["a"]
+    m = _interior(nx, ny)
+    idx = np.nonzero(m)[0]
+    fmw = f["roe"][idx - 1] * f["re"][idx - 1] * f["Uce"][idx - 1]
+    fms = f["ron"][idx - nx] * f["rn"][idx - nx] * f["Vcn"][idx - nx]
+    fme = f["roe"][idx] * f["re"][idx] * f["Uce"][idx]
+    fmn = f["ron"][idx] * f["rn"][idx] * f["Vcn"][idx]
+    for k in (0, 2, 6, 8):
+        assert np.all(a[k, idx] == 0.0)
+    if sign > 0:   # flow towards +i, +j: the upstream (W, S) neighbours carry the flux
+        assert np.all(a[5, idx] == 0.0) and np.all(a[7, idx] == 0.0)
+        assert np.array_equal(a[3, idx], -fmw) and np.array_equal(a[1, idx], -fms)
+    else:
+        assert np.all(a[3, idx] == 0.0) and np.all(a[1, idx] == 0.0)
+        assert np.array_equal(a[5, idx], fme) and np.array_equal(a[7, idx], fmn)
+
+
+@pytest.mark.parametrize("name", ["u_coefficients", "v_coefficients",
+                                  "T_coefficients_and_source"])
+def test_orthogonal_grid_has_no_corner_couplings(orc, name):
+    """betae = betan = 0 on an orthogonal grid: the cross-derivative
+    (corner) couplings of the 9-point stencil vanish exactly."""
+    nx, ny = 12, 8
+    f = _flow(nx, ny, cart=True)
+    m = _interior(nx, ny)
+    # the boundary faces' one-sided metrics carry rounding: beta = 0 exactly
+    f["betae"] = np.zeros_like(f["betae"])
+    f["betan"] = np.zeros_like(f["betan"])
+    a = orc.coef(name, nx, ny, dict(modvis=1, dt=1e-3, beta=0.2, coord=1), f)["a"]
+    for k in (0, 2, 6, 8):
+        assert np.all(a[k, m] == 0.0), k
+    assert np.any(a[1, m] != 0.0) and np.any(a[3, m] != 0.0)
+
+
+@pytest.mark.parametrize("name", ["u_coefficients", "v_coefficients"])
+def test_diffusion_is_linear_in_viscosity(orc, name):
+    """Without advection and transient terms every coefficient is linear in
+    (vle, vln, vlp): doubling the viscosities doubles them exactly."""
+    nx, ny = 14, 9
+    f = _flow(nx, ny)
+    f["Uce"] = np.zeros_like(f["Uce"])
+    f["Vcn"] = np.zeros_like(f["Vcn"])
+    f["roa"] = np.zeros_like(f["roa"])
+    sc = dict(modvis=1, dt=1e-3, coord=1)
+    a1 = orc.coef(name, nx, ny, sc, f)["a"]
+    g = dict(f)
+    for k in ("vle", "vln", "vlp"):
+        g[k] = 2.0 * f[k]
+    a2 = orc.coef(name, nx, ny, sc, g)["a"]
+    m = _interior(nx, ny)
+    assert np.array_equal(a2[:, m], 2.0 * a1[:, m])
+    assert np.any(a1[:, m] != 0.0)
+
+
+def test_p_source_vanishes_for_balanced_field(orc):
+    nx, ny = 14, 9
+    f = _flow(nx, ny)
+    f["Uce"] = np.zeros_like(f["Uce"])
+    f["Vcn"] = np.zeros_like(f["Vcn"])
+    f["roa"] = f["ro"].copy()
+    f["pl"] = np.zeros_like(f["pl"])
+    b = orc.coef("p_source", nx, ny, dict(dt=1e-3), f)["b"]
+    assert np.all(b == 0.0)
+    f["pl"] = _flow(nx, ny, seed=9)["pl"]   # a pressure correction drives a mass source
+    b = orc.coef("p_source", nx, ny, dict(dt=1e-3), f)["b"]
+    assert np.any(b[_interior(nx, ny)] != 0.0)
+
+
+def test_p_coefficients_balance(orc):
+    """Without advection: a_P = g rp / (Jp dt) - (a_S + a_W + a_E + a_N)."""
+    nx, ny = 14, 9
+    f = _flow(nx, ny)
+    f["Uce"] = np.zeros_like(f["Uce"])
+    f["Vcn"] = np.zeros_like(f["Vcn"])
+    dt = 1e-3
+    a = orc.coef("p_coefficients", nx, ny, dict(dt=dt), f)["a"]
+    m = _interior(nx, ny)
+    lhs = a[2, m]
+    rhs = f["g"][m] * f["rp"][m] / (f["Jp"][m] * dt) - (a[0, m] + a[1, m] + a[3, m] + a[4, m])
+    assert np.allclose(lhs, rhs, rtol=1e-13, atol=0)
+    assert np.all(a[[0, 1, 3, 4]][:, m] < 0.0)     # off-diagonals negative
+
+
+def test_T_source_of_uniform_state_is_transient_only(orc):
+    """Euler, fluid at rest, uniform T, no pressure change: b = mpa Ta cp / dt."""
+    nx, ny = 14, 9
+    f = _flow(nx, ny)
+    for k in ("u", "v", "ua", "va", "ue", "ve", "un", "vn"):
+        f[k] = np.zeros_like(f[k])
+    f["T"] = np.full_like(f["T"], 300.0)
+    f["pa"] = f["p"].copy()
+    dt = 1e-3
+    r = orc.coef("T_coefficients_and_source", nx, ny, dict(modvis=0, beta=0.3, dt=dt), f)
+    m = _interior(nx, ny)
+    mpa = f["roa"][m] * f["rp"][m] / f["Jp"][m]
+    assert np.array_equal(r["b"][m], mpa * f["Ta"][m] * f["cp"][m] / dt)
