@@ -261,6 +261,51 @@ void m2d_get_p_source(const int *nx, const int *ny, const double *dt, const doub
                       const double *Vcn, const double *de, const double *dn, const double *g,
                       const double *pl, double *b);
 
+/* SIMPLEC glue between the solves (mach2d_coef.f90:1350-1880); au, av are
+ * the 9-diagonal u / v coefficient arrays.  In/out arrays as in the
+ * reference (p, ro, u, v, ue, ve, un, vn, Uce, Vcn where marked inout). */
+/* get_velocities_at_internal_faces, :1350 */
+void m2d_get_velocities_at_internal_faces(
+    const int *nx, const int *ny, const double *dt, const double *rp, const double *re,
+    const double *rn, const double *xe, const double *ye, const double *xk, const double *yk,
+    const double *xen, const double *yen, const double *xke, const double *yke, const double *Jp,
+    const double *cup, const double *cvp, const double *sup, const double *svp,
+    const double *au, const double *av, const double *roa, const double *p, const double *u,
+    const double *v, const double *uea, const double *vea, const double *una, const double *vna,
+    double *ue, double *ve, double *un, double *vn, double *Uce, double *Vcn);
+/* get_internal_simplec_coefficients, :1504 */
+void m2d_get_internal_simplec_coefficients(
+    const int *nx, const int *ny, const double *re, const double *rn, const double *xe,
+    const double *ye, const double *xk, const double *yk, const double *au, const double *av,
+    double *due, double *dve, double *dun, double *dvn, double *de, double *dn);
+/* get_pressure_correction_with_pl, :1602 (p inout, every point) */
+void m2d_get_pressure_correction_with_pl(const int *nx, const int *ny, const double *pl,
+                                         double *p);
+/* get_density_correction_with_pl, :1614 (ro inout, every point) */
+void m2d_get_density_correction_with_pl(const int *nx, const int *ny, const double *pl,
+                                        const double *g, double *ro);
+/* get_u_v_at_real_nodes_with_pl, :1626 (u, v inout) */
+void m2d_get_u_v_at_real_nodes_with_pl(const int *nx, const int *ny, const double *xe,
+                                       const double *ye, const double *xk, const double *yk,
+                                       const double *rp, const double *pl, const double *au,
+                                       const double *av, double *u, double *v);
+/* get_velocity_correction_at_faces_with_pl, :1679 (last six inout) */
+void m2d_get_velocity_correction_at_faces_with_pl(
+    const int *nx, const int *ny, const double *xe, const double *ye, const double *xk,
+    const double *yk, const double *due, const double *dve, const double *dun, const double *dvn,
+    const double *de, const double *dn, const double *pl, double *ue, double *ve, double *un,
+    double *vn, double *Uce, double *Vcn);
+/* initialize_boundary_faces_flags, :1764 */
+void m2d_initialize_boundary_faces_flags(const int *nx, const int *ny, double *fbe,
+                                         double *fbn);
+/* check_mass_conservation, :1833 (res: L1 norm, summed in the reference's
+ * loop order, so bit-identical) */
+void m2d_check_mass_conservation(const int *nx, const int *ny, const double *dt,
+                                 const double *rp, const double *re, const double *rn,
+                                 const double *Jp, const double *ro, const double *roa,
+                                 const double *ron, const double *roe, const double *Uce,
+                                 const double *Vcn, double *res);
+
 /* TDMA on device pointers (tdma2d9 semantics). */
 int m2d_tdma2d9_device(int nx, int ny, int nitm, const double *d_a,
                        const double *d_b, double *d_x, void *stream);

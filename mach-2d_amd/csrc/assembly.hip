@@ -34,7 +34,7 @@ constexpr int kMaxIn = 48;
 
 struct Fields {
     const double *in[kMaxIn];
-    double *out[3];
+    double *out[6];
     int nx, ny;
     int modvis, coord;
     double dt, beta;
@@ -551,6 +551,232 @@ __global__ __launch_bounds__(kTx) void k_T_coefficients_and_source(Fields F)
     b[np] = bb;
 }
 
+// ---- SIMPLEC glue --------------------------------------------------------
+
+// sum(a(np,:)) of a 9-diagonal array, left to right (Fortran SUM)
+__device__ __forceinline__ double sum9(const double *a, long N, long np)
+{
+    double s = a[np];
+#pragma unroll
+    for (int k = 1; k < 9; ++k) s = s + a[k * N + np];
+    return s;
+}
+
+// get_velocities_at_internal_faces: mach2d_coef.f90:1350-1501.  Grid over
+// i = 2..nx-1, j = 2..ny-1: east faces for i <= nx-2, north faces for
+// j <= ny-2.
+__global__ __launch_bounds__(kTx) void k_velocities_at_internal_faces(Fields F)
+{
+    long np;
+    if (!interior(F.nx, F.ny, np)) return;
+    const double *const *q = F.in;
+    const double *rp = q[0], *re = q[1], *rn = q[2], *xe = q[3], *ye = q[4], *xk = q[5],
+                 *yk = q[6], *xen = q[7], *yen = q[8], *xke = q[9], *yke = q[10], *Jp = q[11],
+                 *cup = q[12], *cvp = q[13], *sup = q[14], *svp = q[15], *au = q[16],
+                 *av = q[17], *roa = q[18], *p = q[19], *u = q[20], *v = q[21], *uea = q[22],
+                 *vea = q[23], *una = q[24], *vna = q[25];
+    double *ue = F.out[0], *ve = F.out[1], *un = F.out[2], *vn = F.out[3], *Uce = F.out[4],
+           *Vcn = F.out[5];
+    const long nx = F.nx, N = (long)F.nx * F.ny;
+    const double dt = F.dt;
+    const int i = (int)(np % nx) + 1, j = (int)(np / nx) + 1;
+    const long nps = np - nx, npn = np + nx, npw = np - 1, npe = np + 1;
+    const long npsw = nps - 1, npse = nps + 1, npnw = npn - 1, npne = npn + 1;
+#define AU(k, r) au[(long)((k) - 1) * N + (r)]
+#define AV(k, r) av[(long)((k) - 1) * N + (r)]
+    const double mpa = roa[np] * rp[np] / Jp[np];
+    const double sumup = AU(1, np) * u[npsw] + AU(2, np) * u[nps] + AU(3, np) * u[npse] +
+                         AU(4, np) * u[npw] + AU(6, np) * u[npe] + AU(7, np) * u[npnw] +
+                         AU(8, np) * u[npn] + AU(9, np) * u[npne];
+    const double sumvp = AV(1, np) * v[npsw] + AV(2, np) * v[nps] + AV(3, np) * v[npse] +
+                         AV(4, np) * v[npw] + AV(6, np) * v[npe] + AV(7, np) * v[npnw] +
+                         AV(8, np) * v[npn] + AV(9, np) * v[npne];
+    if (i <= F.nx - 2) {
+        const long npee = npe + 1, npsee = npse + 1, npnee = npne + 1;
+        const double mea = roa[npe] * rp[npe] / Jp[npe];
+        const double sumue = AU(1, npe) * u[nps] + AU(2, npe) * u[npse] + AU(3, npe) * u[npsee] +
+                             AU(4, npe) * u[np] + AU(6, npe) * u[npee] + AU(7, npe) * u[npn] +
+                             AU(8, npe) * u[npne] + AU(9, npe) * u[npnee];
+        const double sumve = AV(1, npe) * v[nps] + AV(2, npe) * v[npse] + AV(3, npe) * v[npsee] +
+                             AV(4, npe) * v[np] + AV(6, npe) * v[npee] + AV(7, npe) * v[npn] +
+                             AV(8, npe) * v[npne] + AV(9, npe) * v[npnee];
+        const double aux = (p[npn] + p[npne] - p[nps] - p[npse]) / 4.0;
+        const double pue = re[np] * (yke[np] * aux + ye[np] * (p[np] - p[npe]));
+        const double pve = re[np] * (-xke[np] * aux + xe[np] * (p[npe] - p[np]));
+        const double a = ((mpa + mea) * uea[np] / dt + cup[np] + cup[npe] + sup[np] + sup[npe] -
+                          sumup - sumue + 2.0 * pue) /
+                         (AU(5, np) + AU(5, npe));
+        const double b = ((mpa + mea) * vea[np] / dt + cvp[np] + cvp[npe] + svp[np] + svp[npe] -
+                          sumvp - sumve + 2.0 * pve) /
+                         (AV(5, np) + AV(5, npe));
+        ue[np] = a;
+        ve[np] = b;
+        Uce[np] = a * ye[np] - b * xe[np];
+    }
+    if (j <= F.ny - 2) {
+        const long npnn = npn + nx, npnnw = npnn - 1, npnne = npnn + 1;
+        const double mna = roa[npn] * rp[npn] / Jp[npn];
+        const double sumun = AU(1, npn) * u[npw] + AU(2, npn) * u[np] + AU(3, npn) * u[npe] +
+                             AU(4, npn) * u[npnw] + AU(6, npn) * u[npne] + AU(7, npn) * u[npnnw] +
+                             AU(8, npn) * u[npnn] + AU(9, npn) * u[npnne];
+        const double sumvn = AV(1, npn) * v[npw] + AV(2, npn) * v[np] + AV(3, npn) * v[npe] +
+                             AV(4, npn) * v[npnw] + AV(6, npn) * v[npne] + AV(7, npn) * v[npnnw] +
+                             AV(8, npn) * v[npnn] + AV(9, npn) * v[npnne];
+        const double aux = (p[npne] + p[npe] - p[npnw] - p[npw]) / 4.0;
+        const double pun = rn[np] * (yk[np] * (p[npn] - p[np]) - yen[np] * aux);
+        const double pvn = rn[np] * (xk[np] * (p[np] - p[npn]) + xen[np] * aux);
+        const double a = ((mpa + mna) * una[np] / dt + cup[np] + cup[npn] + sup[np] + sup[npn] -
+                          sumup - sumun + 2.0 * pun) /
+                         (AU(5, np) + AU(5, npn));
+        const double b = ((mpa + mna) * vna[np] / dt + cvp[np] + cvp[npn] + svp[np] + svp[npn] -
+                          sumvp - sumvn + 2.0 * pvn) /
+                         (AV(5, np) + AV(5, npn));
+        un[np] = a;
+        vn[np] = b;
+        Vcn[np] = b * xk[np] - a * yk[np];
+    }
+#undef AU
+#undef AV
+}
+
+// get_internal_simplec_coefficients: mach2d_coef.f90:1504-1600
+__global__ __launch_bounds__(kTx) void k_internal_simplec_coefficients(Fields F)
+{
+    long np;
+    if (!interior(F.nx, F.ny, np)) return;
+    const double *const *q = F.in;
+    const double *re = q[0], *rn = q[1], *xe = q[2], *ye = q[3], *xk = q[4], *yk = q[5],
+                 *au = q[6], *av = q[7];
+    const long nx = F.nx, N = (long)F.nx * F.ny;
+    const int i = (int)(np % nx) + 1, j = (int)(np / nx) + 1;
+    const double sau = sum9(au, N, np), sav = sum9(av, N, np);
+    if (i <= F.nx - 2) {
+        const long npe = np + 1;
+        const double Lue = 2.0 * re[np] / (sau + sum9(au, N, npe));
+        const double Lve = 2.0 * re[np] / (sav + sum9(av, N, npe));
+        const double due = Lue * ye[np], dve = Lve * xe[np];
+        F.out[0][np] = due;
+        F.out[1][np] = dve;
+        F.out[4][np] = ye[np] * due + xe[np] * dve;
+    }
+    if (j <= F.ny - 2) {
+        const long npn = np + nx;
+        const double Lun = 2.0 * rn[np] / (sau + sum9(au, N, npn));
+        const double Lvn = 2.0 * rn[np] / (sav + sum9(av, N, npn));
+        const double dun = Lun * yk[np], dvn = Lvn * xk[np];
+        F.out[2][np] = dun;
+        F.out[3][np] = dvn;
+        F.out[5][np] = xk[np] * dvn + yk[np] * dun;
+    }
+}
+
+// Every point: get_pressure_correction_with_pl (:1602, p += pl) and
+// get_density_correction_with_pl (:1614, ro += pl*g).  Grid over all N.
+__global__ __launch_bounds__(kTx) void k_pressure_correction_with_pl(Fields F)
+{
+    const long q = (long)(blockIdx.y * gridDim.x + blockIdx.x) * kTx + threadIdx.x;
+    if (q >= (long)F.nx * F.ny) return;
+    F.out[0][q] = F.out[0][q] + F.in[0][q];
+}
+__global__ __launch_bounds__(kTx) void k_density_correction_with_pl(Fields F)
+{
+    const long q = (long)(blockIdx.y * gridDim.x + blockIdx.x) * kTx + threadIdx.x;
+    if (q >= (long)F.nx * F.ny) return;
+    F.out[0][q] = F.out[0][q] + F.in[0][q] * F.in[1][q];
+}
+
+// get_u_v_at_real_nodes_with_pl: mach2d_coef.f90:1626-1675
+__global__ __launch_bounds__(kTx) void k_u_v_at_real_nodes_with_pl(Fields F)
+{
+    long np;
+    if (!interior(F.nx, F.ny, np)) return;
+    const double *const *q = F.in;
+    const double *xe = q[0], *ye = q[1], *xk = q[2], *yk = q[3], *rp = q[4], *pl = q[5],
+                 *au = q[6], *av = q[7];
+    const long nx = F.nx, N = (long)F.nx * F.ny;
+    const long nps = np - nx, npn = np + nx, npw = np - 1, npe = np + 1;
+    const double plup = 0.5 * rp[np] *
+                        (yk[np] * (pl[npn] + pl[np]) - yk[nps] * (pl[nps] + pl[np]) -
+                         ye[np] * (pl[npe] + pl[np]) + ye[npw] * (pl[npw] + pl[np]));
+    const double plvp = 0.5 * rp[np] *
+                        (xe[np] * (pl[np] + pl[npe]) - xe[npw] * (pl[np] + pl[npw]) -
+                         xk[np] * (pl[np] + pl[npn]) + xk[nps] * (pl[np] + pl[nps]));
+    F.out[0][np] = F.out[0][np] + plup / sum9(au, N, np);
+    F.out[1][np] = F.out[1][np] + plvp / sum9(av, N, np);
+}
+
+// get_velocity_correction_at_faces_with_pl: mach2d_coef.f90:1679-1758 (east
+// faces j = 2..ny-1, i = 1..nx-1; north faces j = 1..ny-1, i = 2..nx-1).
+// Grid over i = 1..nx-1, j = 1..ny-1.
+__global__ __launch_bounds__(kTx) void k_velocity_correction_at_faces_with_pl(Fields F)
+{
+    const int nx = F.nx, ny = F.ny;
+    const int i = blockIdx.x * kTx + threadIdx.x + 1;
+    const int j = blockIdx.y + 1;
+    if (i > nx - 1 || j > ny - 1) return;
+    const double *const *q = F.in;
+    const double *xe = q[0], *ye = q[1], *xk = q[2], *yk = q[3], *due = q[4], *dve = q[5],
+                 *dun = q[6], *dvn = q[7], *de = q[8], *dn = q[9], *pl = q[10];
+    double *ue = F.out[0], *ve = F.out[1], *un = F.out[2], *vn = F.out[3], *Uce = F.out[4],
+           *Vcn = F.out[5];
+    const long np = (long)nx * (j - 1) + i - 1;
+    if (j >= 2) {
+        const long npe = np + 1;
+        Uce[np] = ue[np] * ye[np] - ve[np] * xe[np] + de[np] * (pl[np] - pl[npe]);
+        ue[np] = ue[np] + due[np] * (pl[np] - pl[npe]);
+        ve[np] = ve[np] + dve[np] * (pl[npe] - pl[np]);
+    }
+    if (i >= 2) {
+        const long npn = np + nx;
+        Vcn[np] = vn[np] * xk[np] - un[np] * yk[np] + dn[np] * (pl[np] - pl[npn]);
+        un[np] = un[np] + dun[np] * (pl[npn] - pl[np]);
+        vn[np] = vn[np] + dvn[np] * (pl[np] - pl[npn]);
+    }
+}
+
+// initialize_boundary_faces_flags: mach2d_coef.f90:1764-1828.  Grid over all N.
+__global__ __launch_bounds__(kTx) void k_boundary_faces_flags(Fields F)
+{
+    const int nx = F.nx, ny = F.ny;
+    const long q = (long)(blockIdx.y * gridDim.x + blockIdx.x) * kTx + threadIdx.x;
+    if (q >= (long)nx * ny) return;
+    const int i = (int)(q % nx) + 1, j = (int)(q / nx) + 1;
+    const bool e = (i == 1 || i == nx - 1) && j >= 2 && j <= ny - 1;
+    const bool n = (j == 1 || j == ny - 1) && i >= 2 && i <= nx - 1;
+    F.out[0][q] = e ? 1.0 : 0.0;
+    F.out[1][q] = n ? 1.0 : 0.0;
+}
+
+// check_mass_conservation: mach2d_coef.f90:1833-1880.  The per-point terms
+// in parallel (into the scratch output out[1], interior points in loop
+// order), then one lane adds them in that order, which is the reference's
+// summation order, so the L1 norm is bit-identical.
+__global__ __launch_bounds__(kTx) void k_mass_terms(Fields F)
+{
+    long np;
+    if (!interior(F.nx, F.ny, np)) return;
+    const double *const *q = F.in;
+    const double *rp = q[0], *re = q[1], *rn = q[2], *Jp = q[3], *ro = q[4], *roa = q[5],
+                 *ron = q[6], *roe = q[7], *Uce = q[8], *Vcn = q[9];
+    const long nps = np - F.nx, npw = np - 1;
+    const double fme = roe[np] * re[np] * Uce[np];
+    const double fmw = roe[npw] * re[npw] * Uce[npw];
+    const double fmn = ron[np] * rn[np] * Vcn[np];
+    const double fms = ron[nps] * rn[nps] * Vcn[nps];
+    const int i = (int)(np % F.nx) + 1, j = (int)(np / F.nx) + 1;
+    const long t = (long)(j - 2) * (F.nx - 2) + (i - 2);
+    F.out[1][t] = fabs((ro[np] - roa[np]) * rp[np] / (Jp[np] * F.dt) + fme - fmw + fmn - fms);
+}
+__global__ void k_mass_sum(Fields F)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const long n = (long)(F.nx - 2) * (F.ny - 2);
+    const double *t = F.out[1];
+    double r = 0.0;
+    for (long k = 0; k < n; ++k) r = r + t[k];
+    F.out[0][0] = r;
+}
+
 }  // namespace m2d
 
 using namespace m2d;
@@ -562,12 +788,16 @@ namespace {
 
 thread_local hipStream_t g_stream = nullptr;
 
+enum Grid { kInterior = 0, kFaces = 1, kAll = 2 };
+
 struct Routine {
     void (*kernel)(Fields);
     int nin;
+    int in9[2];   // inputs that are 9-diagonal arrays (-1: none)
     int nout;
-    int outk[3];  // N-vectors per output
-    int full;     // 1: grid over 1..nx-1 x 1..ny-1 (face densities)
+    int outk[6];  // N-vectors per output; 0 = one scalar
+    int grid;     // kInterior, kFaces (1..nx-1 x 1..ny-1) or kAll
+    void (*post)(Fields);  // one-workgroup follow-up over a scratch output
 };
 
 bool is_device_ptr(const void *p)
@@ -580,6 +810,12 @@ bool is_device_ptr(const void *p)
     return at.type == hipMemoryTypeDevice;
 }
 
+long in_len(const Routine &R, int k, long N)
+{
+    return (k == R.in9[0] || k == R.in9[1]) ? 9 * N : N;
+}
+long out_len(const Routine &R, int k, long N) { return R.outk[k] ? R.outk[k] * N : 1; }
+
 int run(const Routine &R, Fields F)
 {
     const int nx = F.nx, ny = F.ny;
@@ -589,62 +825,85 @@ int run(const Routine &R, Fields F)
     for (int k = 0; k < R.nout; ++k)
         if (!F.out[k]) return M2D_ERR_ARG;
     const long N = (long)nx * ny;
-    const bool dev = is_device_ptr(F.in[0]);
+    const bool dev = is_device_ptr(R.nin ? (const void *)F.in[0] : (const void *)F.out[0]);
     for (int k = 0; k < R.nin; ++k)
         if (is_device_ptr(F.in[k]) != dev) return M2D_ERR_ARG;
     for (int k = 0; k < R.nout; ++k)
         if (is_device_ptr(F.out[k]) != dev) return M2D_ERR_ARG;
     dim3 grid, block(kTx);
-    if (R.full)
+    if (R.grid == kFaces)
         grid = dim3((unsigned)((nx - 1 + kTx - 1) / kTx), (unsigned)(ny - 1));
-    else
+    else if (R.grid == kInterior)
         grid = dim3((unsigned)((nx - 2 + kTx - 1) / kTx), (unsigned)(ny - 2));
-    if (dev) {
-        hipLaunchKernelGGL(R.kernel, grid, block, 0, g_stream, F);
-        return hipGetLastError() == hipSuccess ? M2D_OK : M2D_ERR_HIP;
+    else
+        grid = dim3((unsigned)((N + kTx - 1) / kTx), 1);
+    // device buffer: staged host arrays and the scratch output of `post`
+    long tot = R.post ? N : 0;
+    if (!dev) {
+        for (int k = 0; k < R.nin; ++k) tot += in_len(R, k, N);
+        for (int k = 0; k < R.nout; ++k) tot += out_len(R, k, N);
     }
-    // host arrays: stage through one device buffer (outputs keep the values
-    // the routine does not write)
-    long tot = R.nin;
-    for (int k = 0; k < R.nout; ++k) tot += R.outk[k];
     double *d = nullptr;
-    if (hipMalloc((void **)&d, sizeof(double) * N * tot) != hipSuccess) return M2D_ERR_HIP;
+    if (tot && hipMalloc((void **)&d, sizeof(double) * tot) != hipSuccess) return M2D_ERR_HIP;
     Fields G = F;
-    const double *hout[3] = {F.out[0], F.out[1], F.out[2]};
+    hipStream_t s = dev ? g_stream : nullptr;
     int rc = M2D_OK;
     long off = 0;
-    for (int k = 0; k < R.nin && !rc; ++k, off += N) {
-        if (hipMemcpy(d + off, F.in[k], sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess)
-            rc = M2D_ERR_HIP;
-        G.in[k] = d + off;
+    if (!dev) {
+        for (int k = 0; k < R.nin && !rc; ++k) {
+            const long n = in_len(R, k, N);
+            if (hipMemcpy(d + off, F.in[k], sizeof(double) * n, hipMemcpyHostToDevice) !=
+                hipSuccess)
+                rc = M2D_ERR_HIP;
+            G.in[k] = d + off;
+            off += n;
+        }
+        for (int k = 0; k < R.nout && !rc; ++k) {
+            const long n = out_len(R, k, N);
+            if (hipMemcpy(d + off, F.out[k], sizeof(double) * n, hipMemcpyHostToDevice) !=
+                hipSuccess)
+                rc = M2D_ERR_HIP;
+            G.out[k] = d + off;
+            off += n;
+        }
     }
-    for (int k = 0; k < R.nout && !rc; ++k) {
-        const long n = N * R.outk[k];
-        if (hipMemcpy(d + off, hout[k], sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
-            rc = M2D_ERR_HIP;
-        G.out[k] = d + off;
-        off += n;
-    }
+    if (R.post) G.out[R.nout] = d + off;  // scratch
     if (!rc) {
-        hipLaunchKernelGGL(R.kernel, grid, block, 0, nullptr, G);
+        hipLaunchKernelGGL(R.kernel, grid, block, 0, s, G);
+        if (R.post) hipLaunchKernelGGL(R.post, dim3(1), dim3(64), 0, s, G);
         if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
     }
-    for (int k = 0; k < R.nout && !rc; ++k)
-        if (hipMemcpy(F.out[k], G.out[k], sizeof(double) * N * R.outk[k],
-                      hipMemcpyDeviceToHost) != hipSuccess)
-            rc = M2D_ERR_HIP;
-    (void)hipFree(d);
+    if (!dev)
+        for (int k = 0; k < R.nout && !rc; ++k)
+            if (hipMemcpy(F.out[k], G.out[k], sizeof(double) * out_len(R, k, N),
+                          hipMemcpyDeviceToHost) != hipSuccess)
+                rc = M2D_ERR_HIP;
+    if (d) {
+        if (dev && hipStreamSynchronize(s) != hipSuccess) rc = M2D_ERR_HIP;  // scratch in use
+        (void)hipFree(d);
+    }
     return rc;
 }
 
-const Routine kDensity = {k_density_at_faces, 3, 2, {1, 1, 0}, 1};
-const Routine kU = {k_u_coefficients, 19, 1, {9, 0, 0}, 0};
-const Routine kV = {k_v_coefficients, 20, 1, {9, 0, 0}, 0};
-const Routine kUs = {k_u_source, 25, 3, {1, 1, 1}, 0};
-const Routine kVs = {k_v_source, 25, 3, {1, 1, 1}, 0};
-const Routine kT = {k_T_coefficients_and_source, 43, 2, {9, 1, 0}, 0};
-const Routine kPc = {k_p_coefficients, 11, 1, {5, 0, 0}, 0};
-const Routine kPs = {k_p_source, 14, 1, {1, 0, 0}, 0};
+const Routine kDensity = {k_density_at_faces, 3, {-1, -1}, 2, {1, 1}, kFaces, nullptr};
+const Routine kU = {k_u_coefficients, 19, {-1, -1}, 1, {9}, kInterior, nullptr};
+const Routine kV = {k_v_coefficients, 20, {-1, -1}, 1, {9}, kInterior, nullptr};
+const Routine kUs = {k_u_source, 25, {-1, -1}, 3, {1, 1, 1}, kInterior, nullptr};
+const Routine kVs = {k_v_source, 25, {-1, -1}, 3, {1, 1, 1}, kInterior, nullptr};
+const Routine kT = {k_T_coefficients_and_source, 43, {-1, -1}, 2, {9, 1}, kInterior, nullptr};
+const Routine kPc = {k_p_coefficients, 11, {-1, -1}, 1, {5}, kInterior, nullptr};
+const Routine kPs = {k_p_source, 14, {-1, -1}, 1, {1}, kInterior, nullptr};
+const Routine kVel = {k_velocities_at_internal_faces, 26, {16, 17}, 6, {1, 1, 1, 1, 1, 1},
+                      kInterior, nullptr};
+const Routine kSimplec = {k_internal_simplec_coefficients, 8, {6, 7}, 6, {1, 1, 1, 1, 1, 1},
+                          kInterior, nullptr};
+const Routine kPcor = {k_pressure_correction_with_pl, 1, {-1, -1}, 1, {1}, kAll, nullptr};
+const Routine kRcor = {k_density_correction_with_pl, 2, {-1, -1}, 1, {1}, kAll, nullptr};
+const Routine kUVcor = {k_u_v_at_real_nodes_with_pl, 8, {6, 7}, 2, {1, 1}, kInterior, nullptr};
+const Routine kFcor = {k_velocity_correction_at_faces_with_pl, 11, {-1, -1}, 6,
+                       {1, 1, 1, 1, 1, 1}, kFaces, nullptr};
+const Routine kFlags = {k_boundary_faces_flags, 0, {-1, -1}, 2, {1, 1}, kAll, nullptr};
+const Routine kMass = {k_mass_terms, 10, {-1, -1}, 1, {0}, kInterior, k_mass_sum};
 
 Fields fields(const int *nx, const int *ny)
 {
@@ -846,4 +1105,116 @@ extern "C" void m2d_get_p_source(const int *nx, const int *ny, const double *dt,
     for (int k = 0; k < 14; ++k) IN(k, in[k]);
     F.out[0] = b;
     FINISH(kPs);
+}
+
+extern "C" void m2d_get_velocities_at_internal_faces(
+    const int *nx, const int *ny, const double *dt, const double *rp, const double *re,
+    const double *rn, const double *xe, const double *ye, const double *xk, const double *yk,
+    const double *xen, const double *yen, const double *xke, const double *yke, const double *Jp,
+    const double *cup, const double *cvp, const double *sup, const double *svp,
+    const double *au, const double *av, const double *roa, const double *p, const double *u,
+    const double *v, const double *uea, const double *vea, const double *una, const double *vna,
+    double *ue, double *ve, double *un, double *vn, double *Uce, double *Vcn)
+{
+    CHECK_SCALARS(nx, ny, dt);
+    Fields F = fields(nx, ny);
+    F.dt = *dt;
+    const double *in[] = {rp,  re,  rn,  xe, ye, xk, yk, xen, yen, xke, yke, Jp, cup,
+                          cvp, sup, svp, au, av, roa, p, u,  v,   uea, vea, una, vna};
+    for (int k = 0; k < 26; ++k) IN(k, in[k]);
+    double *out[] = {ue, ve, un, vn, Uce, Vcn};
+    for (int k = 0; k < 6; ++k) F.out[k] = out[k];
+    FINISH(kVel);
+}
+
+extern "C" void m2d_get_internal_simplec_coefficients(
+    const int *nx, const int *ny, const double *re, const double *rn, const double *xe,
+    const double *ye, const double *xk, const double *yk, const double *au, const double *av,
+    double *due, double *dve, double *dun, double *dvn, double *de, double *dn)
+{
+    CHECK_SCALARS(nx, ny);
+    Fields F = fields(nx, ny);
+    const double *in[] = {re, rn, xe, ye, xk, yk, au, av};
+    for (int k = 0; k < 8; ++k) IN(k, in[k]);
+    double *out[] = {due, dve, dun, dvn, de, dn};
+    for (int k = 0; k < 6; ++k) F.out[k] = out[k];
+    FINISH(kSimplec);
+}
+
+extern "C" void m2d_get_pressure_correction_with_pl(const int *nx, const int *ny,
+                                                    const double *pl, double *p)
+{
+    CHECK_SCALARS(nx, ny);
+    Fields F = fields(nx, ny);
+    IN(0, pl);
+    F.out[0] = p;
+    FINISH(kPcor);
+}
+
+extern "C" void m2d_get_density_correction_with_pl(const int *nx, const int *ny,
+                                                   const double *pl, const double *g, double *ro)
+{
+    CHECK_SCALARS(nx, ny);
+    Fields F = fields(nx, ny);
+    IN(0, pl);
+    IN(1, g);
+    F.out[0] = ro;
+    FINISH(kRcor);
+}
+
+extern "C" void m2d_get_u_v_at_real_nodes_with_pl(const int *nx, const int *ny,
+                                                  const double *xe, const double *ye,
+                                                  const double *xk, const double *yk,
+                                                  const double *rp, const double *pl,
+                                                  const double *au, const double *av, double *u,
+                                                  double *v)
+{
+    CHECK_SCALARS(nx, ny);
+    Fields F = fields(nx, ny);
+    const double *in[] = {xe, ye, xk, yk, rp, pl, au, av};
+    for (int k = 0; k < 8; ++k) IN(k, in[k]);
+    F.out[0] = u;
+    F.out[1] = v;
+    FINISH(kUVcor);
+}
+
+extern "C" void m2d_get_velocity_correction_at_faces_with_pl(
+    const int *nx, const int *ny, const double *xe, const double *ye, const double *xk,
+    const double *yk, const double *due, const double *dve, const double *dun, const double *dvn,
+    const double *de, const double *dn, const double *pl, double *ue, double *ve, double *un,
+    double *vn, double *Uce, double *Vcn)
+{
+    CHECK_SCALARS(nx, ny);
+    Fields F = fields(nx, ny);
+    const double *in[] = {xe, ye, xk, yk, due, dve, dun, dvn, de, dn, pl};
+    for (int k = 0; k < 11; ++k) IN(k, in[k]);
+    double *out[] = {ue, ve, un, vn, Uce, Vcn};
+    for (int k = 0; k < 6; ++k) F.out[k] = out[k];
+    FINISH(kFcor);
+}
+
+extern "C" void m2d_initialize_boundary_faces_flags(const int *nx, const int *ny, double *fbe,
+                                                    double *fbn)
+{
+    CHECK_SCALARS(nx, ny);
+    Fields F = fields(nx, ny);
+    F.out[0] = fbe;
+    F.out[1] = fbn;
+    FINISH(kFlags);
+}
+
+extern "C" void m2d_check_mass_conservation(const int *nx, const int *ny, const double *dt,
+                                            const double *rp, const double *re, const double *rn,
+                                            const double *Jp, const double *ro,
+                                            const double *roa, const double *ron,
+                                            const double *roe, const double *Uce,
+                                            const double *Vcn, double *res)
+{
+    CHECK_SCALARS(nx, ny, dt);
+    Fields F = fields(nx, ny);
+    F.dt = *dt;
+    const double *in[] = {rp, re, rn, Jp, ro, roa, ron, roe, Uce, Vcn};
+    for (int k = 0; k < 10; ++k) IN(k, in[k]);
+    F.out[0] = res;
+    FINISH(kMass);
 }

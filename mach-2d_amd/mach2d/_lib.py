@@ -54,6 +54,10 @@ EXPORTS = [
     "m2d_batch_tune", "m2d_assembly_set_stream", "m2d_get_density_at_faces",
     "m2d_get_u_coefficients", "m2d_get_u_source", "m2d_get_v_coefficients", "m2d_get_v_source",
     "m2d_get_T_coefficients_and_source", "m2d_get_p_coefficients", "m2d_get_p_source",
+    "m2d_get_velocities_at_internal_faces", "m2d_get_internal_simplec_coefficients",
+    "m2d_get_pressure_correction_with_pl", "m2d_get_density_correction_with_pl",
+    "m2d_get_u_v_at_real_nodes_with_pl", "m2d_get_velocity_correction_at_faces_with_pl",
+    "m2d_initialize_boundary_faces_flags", "m2d_check_mass_conservation",
 ]
 
 _lib = None
@@ -121,9 +125,14 @@ def load():
         f.restype = C.c_int
     L.m2d_assembly_set_stream.argtypes = [_vp]
     L.m2d_assembly_set_stream.restype = None
-    for nm in ("density_at_faces", "u_coefficients", "u_source", "v_coefficients", "v_source",
-               "T_coefficients_and_source", "p_coefficients", "p_source"):
-        getattr(L, "m2d_get_" + nm).restype = None   # pointer arguments, by reference
+    for nm in ("get_density_at_faces", "get_u_coefficients", "get_u_source",
+               "get_v_coefficients", "get_v_source", "get_T_coefficients_and_source",
+               "get_p_coefficients", "get_p_source", "get_velocities_at_internal_faces",
+               "get_internal_simplec_coefficients", "get_pressure_correction_with_pl",
+               "get_density_correction_with_pl", "get_u_v_at_real_nodes_with_pl",
+               "get_velocity_correction_at_faces_with_pl", "initialize_boundary_faces_flags",
+               "check_mass_conservation"):
+        getattr(L, "m2d_" + nm).restype = None   # pointer arguments, by reference
     L.m2d_tdma2d9_device.argtypes = [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]
     L.m2d_tdma2d9_device.restype = C.c_int
     L.m2d_enable_kernel_timing.argtypes = [_vp, C.c_int]

@@ -57,7 +57,36 @@ ROUTINES = {
                  ["rp", "re", "rn", "Jp", "roe", "ron", "ro", "roa", "Uce", "Vcn", "de", "dn",
                   "g", "pl"],
                  [("b", 1)]),
+    # SIMPLEC glue: in/out arrays are outputs whose values the caller
+    # supplies; ("au", 9) inputs are 9-diagonal arrays; size 0 = scalar output
+    # :1350
+    "velocities_at_internal_faces": (
+        [("dt", "d")],
+        ["rp", "re", "rn", "xe", "ye", "xk", "yk", "xen", "yen", "xke", "yke", "Jp", "cup", "cvp",
+         "sup", "svp", ("au", 9), ("av", 9), "roa", "p", "u", "v", "uea", "vea", "una", "vna"],
+        [("ue", 1), ("ve", 1), ("un", 1), ("vn", 1), ("Uce", 1), ("Vcn", 1)]),
+    # :1504
+    "internal_simplec_coefficients": (
+        [], ["re", "rn", "xe", "ye", "xk", "yk", ("au", 9), ("av", 9)],
+        [("due", 1), ("dve", 1), ("dun", 1), ("dvn", 1), ("de", 1), ("dn", 1)]),
+    # :1602, :1614, :1626, :1679, :1764, :1833
+    "pressure_correction_with_pl": ([], ["pl"], [("p", 1)]),
+    "density_correction_with_pl": ([], ["pl", "g"], [("ro", 1)]),
+    "u_v_at_real_nodes_with_pl": ([], ["xe", "ye", "xk", "yk", "rp", "pl", ("au", 9), ("av", 9)],
+                                  [("u", 1), ("v", 1)]),
+    "velocity_correction_at_faces_with_pl": (
+        [], ["xe", "ye", "xk", "yk", "due", "dve", "dun", "dvn", "de", "dn", "pl"],
+        [("ue", 1), ("ve", 1), ("un", 1), ("vn", 1), ("Uce", 1), ("Vcn", 1)]),
+    "boundary_faces_flags": ([], [], [("fbe", 1), ("fbn", 1)]),
+    "mass_conservation": ([("dt", "d")],
+                          ["rp", "re", "rn", "Jp", "ro", "roa", "ron", "roe", "Uce", "Vcn"],
+                          [("res", 0)]),
 }
+
+# C entry of each routine (the reference's subroutine name, prefixed)
+ENTRY = {name: "m2d_get_" + name for name in ROUTINES}
+ENTRY["boundary_faces_flags"] = "m2d_initialize_boundary_faces_flags"
+ENTRY["mass_conservation"] = "m2d_check_mass_conservation"
 
 
 def _ptr(x, n, name):
@@ -83,36 +112,46 @@ def call(name, nx, ny, scalars, fields, out=None):
     scal, names, outs = ROUTINES[name]
     n = nx * ny
     L = _lib.load()
-    fn = getattr(L, "m2d_get_" + name)
+    fn = getattr(L, ENTRY[name])
     keep = []
     args = [C.byref(C.c_int(nx)), C.byref(C.c_int(ny))]
     for k, t in scal:
         v = C.c_int(int(scalars[k])) if t == "i" else C.c_double(float(scalars[k]))
         keep.append(v)
         args.append(C.byref(v))
+    first = None
     for k in names:
-        p, obj = _ptr(fields[k], n, k)
+        key, m = k if isinstance(k, tuple) else (k, 1)
+        p, obj = _ptr(fields[key].reshape(-1) if isinstance(fields[key], np.ndarray)
+                      else fields[key].reshape(-1), m * n, key)
         keep.append(obj)
         args.append(p)
-    first = fields[names[0]]
+        first = obj if first is None else first
     res = {}
     for k, m in outs:
+        size = max(m, 1) * n if m else 1
         arr = None if out is None else out.get(k)
         if arr is None:
-            if isinstance(first, np.ndarray):
-                arr = np.zeros(m * n)
+            if first is None or isinstance(first, np.ndarray):
+                arr = np.zeros(size)
             else:
                 import torch
-                arr = torch.zeros(m * n, dtype=torch.float64, device=first.device)
-        p, obj = _ptr(arr, m * n, k)
+                arr = torch.zeros(size, dtype=torch.float64, device=first.device)
+        p, obj = _ptr(arr, size, k)
         keep.append(obj)
         args.append(p)
         res[k] = arr
     fn(*args)
     st = _lib.last_status()
     if st.status:
-        raise SolverError(st.status, "m2d_get_" + name)
-    return {k: (v.reshape(m, n) if m > 1 else v) for (k, m), v in zip(outs, res.values())}
+        raise SolverError(st.status, ENTRY[name])
+    ret = {}
+    for (k, m), v in zip(outs, res.values()):
+        if m == 0:
+            ret[k] = float(v[0]) if isinstance(v, np.ndarray) else float(v.cpu()[0])
+        else:
+            ret[k] = v.reshape(m, n) if m > 1 else v
+    return ret
 
 
 def set_stream(stream=None):

@@ -524,3 +524,216 @@ void orc_T_coefficients_and_source(
             b[np] = b[np] - raux;
         }
 }
+
+/* ---- SIMPLEC glue ---------------------------------------------------------- */
+
+/* sum(a(np,:)) of a 9-diagonal array, left to right */
+static double sum9(const double *a, long N, long np)
+{
+    double s = a[np];
+    for (int k = 1; k < 9; ++k) s = s + a[k * N + np];
+    return s;
+}
+
+/* get_velocities_at_internal_faces: mach2d_coef.f90:1350-1501 */
+void orc_velocities_at_internal_faces(
+    int nx, int ny, double dt, const double *rp, const double *re, const double *rn,
+    const double *xe, const double *ye, const double *xk, const double *yk, const double *xen,
+    const double *yen, const double *xke, const double *yke, const double *Jp,
+    const double *cup, const double *cvp, const double *sup, const double *svp,
+    const double *au, const double *av, const double *roa, const double *p, const double *u,
+    const double *v, const double *uea, const double *vea, const double *una,
+    const double *vna, double *ue, double *ve, double *un, double *vn, double *Uce,
+    double *Vcn)
+{
+    const long N = (long)nx * ny;
+#define AU(k, q) au[(long)((k) - 1) * N + (q)]
+#define AV(k, q) av[(long)((k) - 1) * N + (q)]
+    for (int j = 2; j <= ny - 1; ++j)
+        for (int i = 2; i <= nx - 2; ++i) {
+            const long np = P(i, j), nps = np - nx, npn = np + nx, npw = np - 1, npe = np + 1;
+            const long npsw = nps - 1, npse = nps + 1, npnw = npn - 1, npne = npn + 1;
+            const long npee = npe + 1, npsee = npse + 1, npnee = npne + 1;
+            const double mpa = roa[np] * rp[np] / Jp[np];
+            const double mea = roa[npe] * rp[npe] / Jp[npe];
+            const double sumup = AU(1, np) * u[npsw] + AU(2, np) * u[nps] + AU(3, np) * u[npse] +
+                                 AU(4, np) * u[npw] + AU(6, np) * u[npe] + AU(7, np) * u[npnw] +
+                                 AU(8, np) * u[npn] + AU(9, np) * u[npne];
+            const double sumue = AU(1, npe) * u[nps] + AU(2, npe) * u[npse] +
+                                 AU(3, npe) * u[npsee] + AU(4, npe) * u[np] +
+                                 AU(6, npe) * u[npee] + AU(7, npe) * u[npn] +
+                                 AU(8, npe) * u[npne] + AU(9, npe) * u[npnee];
+            const double sumvp = AV(1, np) * v[npsw] + AV(2, np) * v[nps] + AV(3, np) * v[npse] +
+                                 AV(4, np) * v[npw] + AV(6, np) * v[npe] + AV(7, np) * v[npnw] +
+                                 AV(8, np) * v[npn] + AV(9, np) * v[npne];
+            const double sumve = AV(1, npe) * v[nps] + AV(2, npe) * v[npse] +
+                                 AV(3, npe) * v[npsee] + AV(4, npe) * v[np] +
+                                 AV(6, npe) * v[npee] + AV(7, npe) * v[npn] +
+                                 AV(8, npe) * v[npne] + AV(9, npe) * v[npnee];
+            const double aux = (p[npn] + p[npne] - p[nps] - p[npse]) / 4.0;
+            const double pue = re[np] * (yke[np] * aux + ye[np] * (p[np] - p[npe]));
+            const double pve = re[np] * (-xke[np] * aux + xe[np] * (p[npe] - p[np]));
+            ue[np] = ((mpa + mea) * uea[np] / dt + cup[np] + cup[npe] + sup[np] + sup[npe] -
+                      sumup - sumue + 2.0 * pue) /
+                     (AU(5, np) + AU(5, npe));
+            ve[np] = ((mpa + mea) * vea[np] / dt + cvp[np] + cvp[npe] + svp[np] + svp[npe] -
+                      sumvp - sumve + 2.0 * pve) /
+                     (AV(5, np) + AV(5, npe));
+            Uce[np] = ue[np] * ye[np] - ve[np] * xe[np];
+        }
+    for (int j = 2; j <= ny - 2; ++j)
+        for (int i = 2; i <= nx - 1; ++i) {
+            const long np = P(i, j), nps = np - nx, npn = np + nx, npw = np - 1, npe = np + 1;
+            const long npsw = nps - 1, npse = nps + 1, npnw = npn - 1, npne = npn + 1;
+            const long npnn = npn + nx, npnnw = npnn - 1, npnne = npnn + 1;
+            const double mpa = roa[np] * rp[np] / Jp[np];
+            const double mna = roa[npn] * rp[npn] / Jp[npn];
+            const double sumup = AU(1, np) * u[npsw] + AU(2, np) * u[nps] + AU(3, np) * u[npse] +
+                                 AU(4, np) * u[npw] + AU(6, np) * u[npe] + AU(7, np) * u[npnw] +
+                                 AU(8, np) * u[npn] + AU(9, np) * u[npne];
+            const double sumun = AU(1, npn) * u[npw] + AU(2, npn) * u[np] + AU(3, npn) * u[npe] +
+                                 AU(4, npn) * u[npnw] + AU(6, npn) * u[npne] +
+                                 AU(7, npn) * u[npnnw] + AU(8, npn) * u[npnn] +
+                                 AU(9, npn) * u[npnne];
+            const double sumvp = AV(1, np) * v[npsw] + AV(2, np) * v[nps] + AV(3, np) * v[npse] +
+                                 AV(4, np) * v[npw] + AV(6, np) * v[npe] + AV(7, np) * v[npnw] +
+                                 AV(8, np) * v[npn] + AV(9, np) * v[npne];
+            const double sumvn = AV(1, npn) * v[npw] + AV(2, npn) * v[np] + AV(3, npn) * v[npe] +
+                                 AV(4, npn) * v[npnw] + AV(6, npn) * v[npne] +
+                                 AV(7, npn) * v[npnnw] + AV(8, npn) * v[npnn] +
+                                 AV(9, npn) * v[npnne];
+            const double aux = (p[npne] + p[npe] - p[npnw] - p[npw]) / 4.0;
+            const double pun = rn[np] * (yk[np] * (p[npn] - p[np]) - yen[np] * aux);
+            const double pvn = rn[np] * (xk[np] * (p[np] - p[npn]) + xen[np] * aux);
+            un[np] = ((mpa + mna) * una[np] / dt + cup[np] + cup[npn] + sup[np] + sup[npn] -
+                      sumup - sumun + 2.0 * pun) /
+                     (AU(5, np) + AU(5, npn));
+            vn[np] = ((mpa + mna) * vna[np] / dt + cvp[np] + cvp[npn] + svp[np] + svp[npn] -
+                      sumvp - sumvn + 2.0 * pvn) /
+                     (AV(5, np) + AV(5, npn));
+            Vcn[np] = vn[np] * xk[np] - un[np] * yk[np];
+        }
+#undef AU
+#undef AV
+}
+
+/* get_internal_simplec_coefficients: mach2d_coef.f90:1504-1600 */
+void orc_internal_simplec_coefficients(int nx, int ny, const double *re, const double *rn,
+                                       const double *xe, const double *ye, const double *xk,
+                                       const double *yk, const double *au, const double *av,
+                                       double *due, double *dve, double *dun, double *dvn,
+                                       double *de, double *dn)
+{
+    const long N = (long)nx * ny;
+    for (int j = 2; j <= ny - 1; ++j)
+        for (int i = 2; i <= nx - 2; ++i) {
+            const long np = P(i, j), npe = np + 1;
+            const double Lue = 2.0 * re[np] / (sum9(au, N, np) + sum9(au, N, npe));
+            const double Lve = 2.0 * re[np] / (sum9(av, N, np) + sum9(av, N, npe));
+            due[np] = Lue * ye[np];
+            dve[np] = Lve * xe[np];
+            de[np] = ye[np] * due[np] + xe[np] * dve[np];
+        }
+    for (int j = 2; j <= ny - 2; ++j)
+        for (int i = 2; i <= nx - 1; ++i) {
+            const long np = P(i, j), npn = np + nx;
+            const double Lun = 2.0 * rn[np] / (sum9(au, N, np) + sum9(au, N, npn));
+            const double Lvn = 2.0 * rn[np] / (sum9(av, N, np) + sum9(av, N, npn));
+            dun[np] = Lun * yk[np];
+            dvn[np] = Lvn * xk[np];
+            dn[np] = xk[np] * dvn[np] + yk[np] * dun[np];
+        }
+}
+
+/* get_pressure_correction_with_pl: mach2d_coef.f90:1602-1611 (every point) */
+void orc_pressure_correction_with_pl(int nx, int ny, const double *pl, double *p)
+{
+    const long N = (long)nx * ny;
+    for (long q = 0; q < N; ++q) p[q] = p[q] + pl[q];
+}
+
+/* get_density_correction_with_pl: mach2d_coef.f90:1614-1624 (every point) */
+void orc_density_correction_with_pl(int nx, int ny, const double *pl, const double *g,
+                                    double *ro)
+{
+    const long N = (long)nx * ny;
+    for (long q = 0; q < N; ++q) ro[q] = ro[q] + pl[q] * g[q];
+}
+
+/* get_u_v_at_real_nodes_with_pl: mach2d_coef.f90:1626-1675 */
+void orc_u_v_at_real_nodes_with_pl(int nx, int ny, const double *xe, const double *ye,
+                                   const double *xk, const double *yk, const double *rp,
+                                   const double *pl, const double *au, const double *av,
+                                   double *u, double *v)
+{
+    const long N = (long)nx * ny;
+    for (int j = 2; j <= ny - 1; ++j)
+        for (int i = 2; i <= nx - 1; ++i) {
+            const long np = P(i, j), nps = np - nx, npn = np + nx, npw = np - 1, npe = np + 1;
+            const double plup = 0.5 * rp[np] *
+                                (yk[np] * (pl[npn] + pl[np]) - yk[nps] * (pl[nps] + pl[np]) -
+                                 ye[np] * (pl[npe] + pl[np]) + ye[npw] * (pl[npw] + pl[np]));
+            const double plvp = 0.5 * rp[np] *
+                                (xe[np] * (pl[np] + pl[npe]) - xe[npw] * (pl[np] + pl[npw]) -
+                                 xk[np] * (pl[np] + pl[npn]) + xk[nps] * (pl[np] + pl[nps]));
+            u[np] = u[np] + plup / sum9(au, N, np);
+            v[np] = v[np] + plvp / sum9(av, N, np);
+        }
+}
+
+/* get_velocity_correction_at_faces_with_pl: mach2d_coef.f90:1679-1758 */
+void orc_velocity_correction_at_faces_with_pl(int nx, int ny, const double *xe,
+                                              const double *ye, const double *xk,
+                                              const double *yk, const double *due,
+                                              const double *dve, const double *dun,
+                                              const double *dvn, const double *de,
+                                              const double *dn, const double *pl, double *ue,
+                                              double *ve, double *un, double *vn, double *Uce,
+                                              double *Vcn)
+{
+    for (int j = 2; j <= ny - 1; ++j)
+        for (int i = 1; i <= nx - 1; ++i) {
+            const long np = P(i, j), npe = np + 1;
+            Uce[np] = ue[np] * ye[np] - ve[np] * xe[np] + de[np] * (pl[np] - pl[npe]);
+            ue[np] = ue[np] + due[np] * (pl[np] - pl[npe]);
+            ve[np] = ve[np] + dve[np] * (pl[npe] - pl[np]);
+        }
+    for (int j = 1; j <= ny - 1; ++j)
+        for (int i = 2; i <= nx - 1; ++i) {
+            const long np = P(i, j), npn = np + nx;
+            Vcn[np] = vn[np] * xk[np] - un[np] * yk[np] + dn[np] * (pl[np] - pl[npn]);
+            un[np] = un[np] + dun[np] * (pl[npn] - pl[np]);
+            vn[np] = vn[np] + dvn[np] * (pl[np] - pl[npn]);
+        }
+}
+
+/* initialize_boundary_faces_flags: mach2d_coef.f90:1764-1828 */
+void orc_boundary_faces_flags(int nx, int ny, double *fbe, double *fbn)
+{
+    const long N = (long)nx * ny;
+    for (long q = 0; q < N; ++q) fbe[q] = 0.0;
+    for (int j = 2; j <= ny - 1; ++j) fbe[P(1, j)] = 1.0;
+    for (int j = 2; j <= ny - 1; ++j) fbe[P(nx - 1, j)] = 1.0;
+    for (long q = 0; q < N; ++q) fbn[q] = 0.0;
+    for (int i = 2; i <= nx - 1; ++i) fbn[P(i, ny - 1)] = 1.0;
+    for (int i = 2; i <= nx - 1; ++i) fbn[P(i, 1)] = 1.0;
+}
+
+/* check_mass_conservation: mach2d_coef.f90:1833-1880 (L1 sum in loop order) */
+void orc_mass_conservation(int nx, int ny, double dt, const double *rp, const double *re,
+                           const double *rn, const double *Jp, const double *ro,
+                           const double *roa, const double *ron, const double *roe,
+                           const double *Uce, const double *Vcn, double *res)
+{
+    double r = 0.0;
+    for (int j = 2; j <= ny - 1; ++j)
+        for (int i = 2; i <= nx - 1; ++i) {
+            const long np = P(i, j), nps = np - nx, npw = np - 1;
+            const double fme = roe[np] * re[np] * Uce[np];
+            const double fmw = roe[npw] * re[npw] * Uce[npw];
+            const double fmn = ron[np] * rn[np] * Vcn[np];
+            const double fms = ron[nps] * rn[nps] * Vcn[nps];
+            r = r + fabs((ro[np] - roa[np]) * rp[np] / (Jp[np] * dt) + fme - fmw + fmn - fms);
+        }
+    *res = r;
+}

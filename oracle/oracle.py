@@ -130,6 +130,28 @@ COEF = {
     "p_coefficients": ([("dt", "d")],
                        ["rp", "re", "rn", "Jp", "Uce", "Vcn", "roe", "ron", "g", "de", "dn"],
                        [("a", 5)]),
+    # SIMPLEC glue (in/out arrays are listed as outputs whose initial values
+    # the caller supplies; ("name", 9) inputs are 9-diagonal arrays; an
+    # output of size 0 is a scalar)
+    "velocities_at_internal_faces": (
+        [("dt", "d")],
+        ["rp", "re", "rn", "xe", "ye", "xk", "yk", "xen", "yen", "xke", "yke", "Jp", "cup", "cvp",
+         "sup", "svp", ("au", 9), ("av", 9), "roa", "p", "u", "v", "uea", "vea", "una", "vna"],
+        [("ue", 1), ("ve", 1), ("un", 1), ("vn", 1), ("Uce", 1), ("Vcn", 1)]),
+    "internal_simplec_coefficients": (
+        [], ["re", "rn", "xe", "ye", "xk", "yk", ("au", 9), ("av", 9)],
+        [("due", 1), ("dve", 1), ("dun", 1), ("dvn", 1), ("de", 1), ("dn", 1)]),
+    "pressure_correction_with_pl": ([], ["pl"], [("p", 1)]),
+    "density_correction_with_pl": ([], ["pl", "g"], [("ro", 1)]),
+    "u_v_at_real_nodes_with_pl": ([], ["xe", "ye", "xk", "yk", "rp", "pl", ("au", 9), ("av", 9)],
+                                  [("u", 1), ("v", 1)]),
+    "velocity_correction_at_faces_with_pl": (
+        [], ["xe", "ye", "xk", "yk", "due", "dve", "dun", "dvn", "de", "dn", "pl"],
+        [("ue", 1), ("ve", 1), ("un", 1), ("vn", 1), ("Uce", 1), ("Vcn", 1)]),
+    "boundary_faces_flags": ([], [], [("fbe", 1), ("fbn", 1)]),
+    "mass_conservation": ([("dt", "d")],
+                          ["rp", "re", "rn", "Jp", "ro", "roa", "ron", "roe", "Uce", "Vcn"],
+                          [("res", 0)]),
     "p_source": ([("dt", "d")],
                  ["rp", "re", "rn", "Jp", "roe", "ron", "ro", "roa", "Uce", "Vcn", "de", "dn",
                   "g", "pl"],
@@ -161,19 +183,25 @@ def _bind_ref_coef(R):
             f.restype = None
 
 
+def _field_name(k):
+    return k[0] if isinstance(k, tuple) else k
+
+
 def _coef_call(fn, name, nx, ny, scalars, fields, out):
     scal, names, outs = COEF[name]
     n = nx * ny
     args = [nx, ny] + [int(scalars[k]) if t == "i" else float(scalars[k]) for k, t in scal]
-    args += [_c(fields[k]) for k in names]
+    args += [_c(fields[_field_name(k)]) for k in names]
     res = {}
     for k, m in outs:
         init = None if out is None else out.get(k)
-        arr = np.zeros((m, n)) if init is None else _c(init).reshape(m, n).copy()
+        shape = (1, 1) if m == 0 else (m, n)
+        arr = np.zeros(shape) if init is None else _c(init).reshape(shape).copy()
         res[k] = arr
         args.append(arr)
     fn(*args)
-    return {k: (v[0] if v.shape[0] == 1 else v) for k, v in res.items()}
+    return {k: (float(v[0, 0]) if m == 0 else (v[0] if v.shape[0] == 1 else v))
+            for (k, m), v in zip(outs, res.values())}
 
 
 def coef(name, nx, ny, scalars, fields, out=None):

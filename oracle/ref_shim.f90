@@ -198,4 +198,88 @@ contains
          a, b)
    end subroutine
 
+   ! :1350
+   subroutine ref_velocities_at_internal_faces(nx, ny, dt, rp, re, rn, xe, ye, xk, yk, xen, &
+         yen, xke, yke, Jp, cup, cvp, sup, svp, au, av, roa, p, u, v, uea, vea, una, vna, ue, &
+         ve, un, vn, Uce, Vcn) bind(C, name="ref_velocities_at_internal_faces")
+      integer(c_int), value :: nx, ny
+      real(c_double), value :: dt
+      real(c_double), dimension(nx*ny), intent(in) :: rp, re, rn, xe, ye, xk, yk, xen, yen, &
+         xke, yke, Jp, cup, cvp, sup, svp, roa, p, u, v, uea, vea, una, vna
+      real(c_double), intent(in) :: au(nx*ny,9), av(nx*ny,9)
+      real(c_double), dimension(nx*ny), intent(inout) :: ue, ve, un, vn, Uce, Vcn
+      call get_velocities_at_internal_faces(nx, ny, dt, rp, re, rn, xe, ye, xk, yk, xen, yen, &
+         xke, yke, Jp, cup, cvp, sup, svp, au, av, roa, p, u, v, uea, vea, una, vna, ue, ve, &
+         un, vn, Uce, Vcn)
+   end subroutine
+
+   ! :1504
+   subroutine ref_internal_simplec_coefficients(nx, ny, re, rn, xe, ye, xk, yk, au, av, due, &
+         dve, dun, dvn, de, dn) bind(C, name="ref_internal_simplec_coefficients")
+      integer(c_int), value :: nx, ny
+      real(c_double), dimension(nx*ny), intent(in) :: re, rn, xe, ye, xk, yk
+      real(c_double), intent(in) :: au(nx*ny,9), av(nx*ny,9)
+      real(c_double), dimension(nx*ny), intent(inout) :: due, dve, dun, dvn, de, dn
+      call get_internal_simplec_coefficients(nx, ny, re, rn, xe, ye, xk, yk, au, av, due, dve, &
+         dun, dvn, de, dn)
+   end subroutine
+
+   ! :1602
+   subroutine ref_pressure_correction_with_pl(nx, ny, pl, p) &
+         bind(C, name="ref_pressure_correction_with_pl")
+      integer(c_int), value :: nx, ny
+      real(c_double), intent(in) :: pl(nx*ny)
+      real(c_double), intent(inout) :: p(nx*ny)
+      call get_pressure_correction_with_pl(nx, ny, pl, p)
+   end subroutine
+
+   ! :1614
+   subroutine ref_density_correction_with_pl(nx, ny, pl, g, ro) &
+         bind(C, name="ref_density_correction_with_pl")
+      integer(c_int), value :: nx, ny
+      real(c_double), intent(in) :: pl(nx*ny), g(nx*ny)
+      real(c_double), intent(inout) :: ro(nx*ny)
+      call get_density_correction_with_pl(nx, ny, pl, g, ro)
+   end subroutine
+
+   ! :1626
+   subroutine ref_u_v_at_real_nodes_with_pl(nx, ny, xe, ye, xk, yk, rp, pl, au, av, u, v) &
+         bind(C, name="ref_u_v_at_real_nodes_with_pl")
+      integer(c_int), value :: nx, ny
+      real(c_double), dimension(nx*ny), intent(in) :: xe, ye, xk, yk, rp, pl
+      real(c_double), intent(in) :: au(nx*ny,9), av(nx*ny,9)
+      real(c_double), dimension(nx*ny), intent(inout) :: u, v
+      call get_u_v_at_real_nodes_with_pl(nx, ny, xe, ye, xk, yk, rp, pl, au, av, u, v)
+   end subroutine
+
+   ! :1679
+   subroutine ref_velocity_correction_at_faces_with_pl(nx, ny, xe, ye, xk, yk, due, dve, dun, &
+         dvn, de, dn, pl, ue, ve, un, vn, Uce, Vcn) &
+         bind(C, name="ref_velocity_correction_at_faces_with_pl")
+      integer(c_int), value :: nx, ny
+      real(c_double), dimension(nx*ny), intent(in) :: xe, ye, xk, yk, due, dve, dun, dvn, de, &
+         dn, pl
+      real(c_double), dimension(nx*ny), intent(inout) :: ue, ve, un, vn, Uce, Vcn
+      call get_velocity_correction_at_faces_with_pl(nx, ny, xe, ye, xk, yk, due, dve, dun, dvn, &
+         de, dn, pl, ue, ve, un, vn, Uce, Vcn)
+   end subroutine
+
+   ! :1764
+   subroutine ref_boundary_faces_flags(nx, ny, fbe, fbn) bind(C, name="ref_boundary_faces_flags")
+      integer(c_int), value :: nx, ny
+      real(c_double), dimension(nx*ny), intent(inout) :: fbe, fbn
+      call initialize_boundary_faces_flags(nx, ny, fbe, fbn)
+   end subroutine
+
+   ! :1833
+   subroutine ref_mass_conservation(nx, ny, dt, rp, re, rn, Jp, ro, roa, ron, roe, Uce, Vcn, &
+         res) bind(C, name="ref_mass_conservation")
+      integer(c_int), value :: nx, ny
+      real(c_double), value :: dt
+      real(c_double), dimension(nx*ny), intent(in) :: rp, re, rn, Jp, ro, roa, ron, roe, Uce, &
+         Vcn
+      real(c_double), intent(inout) :: res
+      call check_mass_conservation(nx, ny, dt, rp, re, rn, Jp, ro, roa, ron, roe, Uce, Vcn, res)
+   end subroutine
+
 end module m2dref_shim

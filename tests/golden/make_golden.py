@@ -21,7 +21,10 @@ Fixtures (float64, diagonal-major arrays (k, nx*ny)):
                   embedding of the same 5-point system
   solver_msi9d.npz  class_solver_msi9d init + solve (the plugin object)
   coef.npz        every routine of module `coefficients` used by the hot path
-                  (face densities; u, v, T, p' coefficients and sources) on
+                  (face densities; u, v, T, p' coefficients and sources; the
+                  SIMPLEC glue: face velocities, SIMPLEC coefficients, the
+                  p/ro/u/v/face corrections, boundary-face flags, the mass
+                  residual) on
                   12x8 and 20x11 ogive grids (axisymmetric Navier-Stokes,
                   planar Euler, planar Navier-Stokes), a 10x7 Cartesian grid,
                   and a 12x8 case with faces on the axis (r = 0) for the
@@ -166,6 +169,18 @@ def coef_fields(nx, ny, coord, cart, axis, seed):
         f["re"][nx * 5 + 8] = 0.0
         f["rn"][nx * 2 + 6] = 0.0      # south face of point (7, 4) / north of (7, 3)
         f["rn"][nx * 5 + 3] = 0.0
+    # inputs of the SIMPLEC glue: u, v systems from the assembly itself (unit
+    # diagonal on the fictitious rows), deferred-correction and viscous
+    # sources, old face velocities, SIMPLEC coefficients
+    rng = np.random.default_rng(seed + 1)
+    sc = dict(modvis=1, dt=2e-3, coord=coord)
+    for key, name in (("au", "u_coefficients"), ("av", "v_coefficients")):
+        a = O.ref_coef(name, nx, ny, sc, f)["a"]
+        a[4][a[4] == 0.0] = 1.0
+        f[key] = a
+    for k in ("cup", "cvp", "sup", "svp", "uea", "vea", "una", "vna", "due", "dve", "dun",
+              "dvn"):
+        f[k] = rng.uniform(-1.0, 1.0, size=nx * ny)
     return f
 
 
@@ -179,7 +194,7 @@ def coef():
             arrs["%s_in_%s" % (tag, k)] = v
         arrs["%s_scalars" % tag] = np.array([nx, ny, coord, mv, sc["beta"], sc["dt"]])
         for name, (_, _, outs) in O.COEF.items():
-            init = {k: rng.uniform(-1, 1, size=(m, nx * ny)) for k, m in outs}
+            init = {k: rng.uniform(-1, 1, size=(m, nx * ny) if m else (1, 1)) for k, m in outs}
             for k, v in init.items():
                 arrs["%s_%s_init_%s" % (tag, name, k)] = v
             res = O.ref_coef(name, nx, ny, sc, f, out={k: v.copy() for k, v in init.items()})

@@ -143,7 +143,9 @@ def test_coef_oracle_vs_reference_fixtures(orc, tag):
     for name, (_, _, outs) in orc.COEF.items():
         init = {k: g["%s_%s_init_%s" % (tag, name, k)] for k, _ in outs}
         got = orc.coef(name, nx, ny, sc, f, out=init)
-        for k, _ in outs:
+        for k, m in outs:
             want = g["%s_%s_%s" % (tag, name, k)]
+            if m == 0:
+                want = float(np.asarray(want).reshape(-1)[0])
             assert np.array_equal(got[k], want), (tag, name, k)
             assert np.all(np.isfinite(want)), (tag, name, k)
