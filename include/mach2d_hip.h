@@ -193,6 +193,9 @@ int m2d_batch_tune(m2d_batch *b, const double *const *d_a, const double *const *
  * points the reference writes are written.  Status: m2d_last_status. */
 void m2d_assembly_set_stream(void *stream);
 
+/* get_density_at_nodes, mach2d_coef.f90:6 (every point) */
+void m2d_get_density_at_nodes(const int *nx, const int *ny, const double *Rg, const double *p,
+                              const double *T, double *ro);
 /* get_density_at_faces, mach2d_coef.f90:20 */
 void m2d_get_density_at_faces(const int *nx, const int *ny, const double *beta,
                               const double *ro, const double *Uce, const double *Vcn,

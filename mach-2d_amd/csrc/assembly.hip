@@ -101,6 +101,15 @@ __device__ __forceinline__ bool interior(int nx, int ny, long &np)
     return true;
 }
 
+// get_density_at_nodes: mach2d_coef.f90:6-17 (ro = p / (Rg T), every
+// point; F.beta carries Rg).  Grid over all N.
+__global__ __launch_bounds__(kTx) void k_density_at_nodes(Fields F)
+{
+    const long q = (long)(blockIdx.y * gridDim.x + blockIdx.x) * kTx + threadIdx.x;
+    if (q >= (long)F.nx * F.ny) return;
+    F.out[0][q] = F.in[0][q] / (F.beta * F.in[1][q]);
+}
+
 // get_density_at_faces: mach2d_coef.f90:20-60 (east faces j = 2..ny-1,
 // i = 1..nx-1; north faces j = 1..ny-1, i = 2..nx-1).  Grid over
 // i = 1..nx-1, j = 1..ny-1.
@@ -885,6 +894,7 @@ int run(const Routine &R, Fields F)
     return rc;
 }
 
+const Routine kNodes = {k_density_at_nodes, 2, {-1, -1}, 1, {1}, kAll, nullptr};
 const Routine kDensity = {k_density_at_faces, 3, {-1, -1}, 2, {1, 1}, kFaces, nullptr};
 const Routine kU = {k_u_coefficients, 19, {-1, -1}, 1, {9}, kInterior, nullptr};
 const Routine kV = {k_v_coefficients, 20, {-1, -1}, 1, {9}, kInterior, nullptr};
@@ -935,6 +945,18 @@ void m2d_set_last_status_code(int status);
     } while (0)
 
 extern "C" void m2d_assembly_set_stream(void *stream) { g_stream = (hipStream_t)stream; }
+
+extern "C" void m2d_get_density_at_nodes(const int *nx, const int *ny, const double *Rg,
+                                         const double *p, const double *T, double *ro)
+{
+    CHECK_SCALARS(nx, ny, Rg);
+    Fields F = fields(nx, ny);
+    F.beta = *Rg;
+    IN(0, p);
+    IN(1, T);
+    F.out[0] = ro;
+    FINISH(kNodes);
+}
 
 extern "C" void m2d_get_density_at_faces(const int *nx, const int *ny, const double *beta,
                                          const double *ro, const double *Uce, const double *Vcn,

@@ -51,7 +51,8 @@ EXPORTS = [
     "m2d_version", "m2d_enable_kernel_timing", "m2d_kernel_timing",
     "m2d_batch_create", "m2d_batch_destroy", "m2d_batch_set_stream", "m2d_batch_factor",
     "m2d_batch_solve", "m2d_batch_set_inner", "m2d_batch_set_tuning", "m2d_batch_get_tuning",
-    "m2d_batch_tune", "m2d_assembly_set_stream", "m2d_get_density_at_faces",
+    "m2d_batch_tune", "m2d_assembly_set_stream", "m2d_get_density_at_nodes",
+    "m2d_get_density_at_faces",
     "m2d_get_u_coefficients", "m2d_get_u_source", "m2d_get_v_coefficients", "m2d_get_v_source",
     "m2d_get_T_coefficients_and_source", "m2d_get_p_coefficients", "m2d_get_p_source",
     "m2d_get_velocities_at_internal_faces", "m2d_get_internal_simplec_coefficients",
@@ -125,7 +126,7 @@ def load():
         f.restype = C.c_int
     L.m2d_assembly_set_stream.argtypes = [_vp]
     L.m2d_assembly_set_stream.restype = None
-    for nm in ("get_density_at_faces", "get_u_coefficients", "get_u_source",
+    for nm in ("get_density_at_nodes", "get_density_at_faces", "get_u_coefficients", "get_u_source",
                "get_v_coefficients", "get_v_source", "get_T_coefficients_and_source",
                "get_p_coefficients", "get_p_source", "get_velocities_at_internal_faces",
                "get_internal_simplec_coefficients", "get_pressure_correction_with_pl",

@@ -15,7 +15,9 @@ from . import _lib
 from ._lib import SolverError
 
 ROUTINES = {
-    # mach2d_coef.f90:20
+    # mach2d_coef.f90:6
+    "density_at_nodes": ([("Rg", "d")], ["p", "T"], [("ro", 1)]),
+    # :20
     "density_at_faces": ([("beta", "d")], ["ro", "Uce", "Vcn"], [("roe", 1), ("ron", 1)]),
     # :63
     "u_coefficients": ([("modvis", "i"), ("dt", "d")],

@@ -16,6 +16,14 @@
 #define P(i, j) ((long)nx * ((j) - 1) + (i) - 1)
 #define A(k, p) a[(long)((k) - 1) * N + (p)]
 
+/* get_density_at_nodes: mach2d_coef.f90:6-17 (every point) */
+void orc_density_at_nodes(int nx, int ny, double Rg, const double *p, const double *T,
+                          double *ro)
+{
+    const long N = (long)nx * ny;
+    for (long q = 0; q < N; ++q) ro[q] = p[q] / (Rg * T[q]);
+}
+
 /* get_density_at_faces: mach2d_coef.f90:20-60 */
 void orc_density_at_faces(int nx, int ny, double beta, const double *ro, const double *Uce,
                           const double *Vcn, double *roe, double *ron)

@@ -66,6 +66,8 @@ void orc_residual9(const double *b, int ni, int nj, const double *var,
 #endif
 
 /* coef.c: coefficient assembly of src/equations/mach2d_coef.f90 */
+void orc_density_at_nodes(int nx, int ny, double Rg, const double *p, const double *T,
+                          double *ro);
 void orc_density_at_faces(int nx, int ny, double beta, const double *ro, const double *Uce,
                           const double *Vcn, double *roe, double *ron);
 void orc_u_coefficients(int nx, int ny, int modvis, double dt, const double *rp, const double *re,

@@ -100,6 +100,7 @@ def ref():
 # outputs...).
 # --------------------------------------------------------------------------
 COEF = {
+    "density_at_nodes": ([("Rg", "d")], ["p", "T"], [("ro", 1)]),
     "density_at_faces": ([("beta", "d")], ["ro", "Uce", "Vcn"], [("roe", 1), ("ron", 1)]),
     "u_coefficients": ([("modvis", "i"), ("dt", "d")],
                        ["rp", "re", "rn", "Jp", "Je", "Jn", "ye", "yk", "alphae", "betae",

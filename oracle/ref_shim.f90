@@ -96,6 +96,15 @@ contains
 
    ! ---- coefficient assembly, src/equations/mach2d_coef.f90 -----------------
 
+   ! :6
+   subroutine ref_density_at_nodes(nx, ny, Rg, p, T, ro) bind(C, name="ref_density_at_nodes")
+      integer(c_int), value :: nx, ny
+      real(c_double), value :: Rg
+      real(c_double), intent(in)    :: p(nx*ny), T(nx*ny)
+      real(c_double), intent(inout) :: ro(nx*ny)
+      call get_density_at_nodes(nx, ny, Rg, p, T, ro)
+   end subroutine
+
    ! :20
    subroutine ref_density_at_faces(nx, ny, beta, ro, Uce, Vcn, roe, ron) &
          bind(C, name="ref_density_at_faces")

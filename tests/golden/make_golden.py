@@ -189,7 +189,7 @@ def coef():
     rng = np.random.default_rng(91)
     for tag, nx, ny, coord, mv, cart, axis in COEF_CASES:
         f = coef_fields(nx, ny, coord, cart, axis, seed=nx * 31 + ny + coord)
-        sc = dict(beta=0.35, modvis=mv, dt=2e-3, coord=coord)
+        sc = dict(beta=0.35, modvis=mv, dt=2e-3, coord=coord, Rg=287.0)
         for k, v in f.items():
             arrs["%s_in_%s" % (tag, k)] = v
         arrs["%s_scalars" % tag] = np.array([nx, ny, coord, mv, sc["beta"], sc["dt"]])

@@ -110,7 +110,59 @@ def test_fortran_dropin_module_compiles(tmp_path):
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     if not os.path.exists(fc):
         pytest.skip("no Fortran compiler")
-    src = os.path.join(ROOT, "mach-2d_amd", "fortran", "msi2d9_hip.f90")
-    r = subprocess.run([fc, "-c", "-module-dir", str(tmp_path), "-o",
-                        str(tmp_path / "m.o"), src], capture_output=True, text=True)
-    assert r.returncode == 0, r.stderr
+    for name in ("msi2d9_hip.f90", "coefficients_hip.f90"):
+        src = os.path.join(ROOT, "mach-2d_amd", "fortran", name)
+        r = subprocess.run([fc, "-c", "-module-dir", str(tmp_path), "-o",
+                            str(tmp_path / (name + ".o")), src], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+
+
+def test_fortran_coefficients_module_matches_reference_interface():
+    """Every routine of the drop-in `coefficients` module has the reference's
+    dummy-argument list (src/equations/mach2d_coef.f90), as the C table
+    mach2d.assembly.ROUTINES states it."""
+    import re
+
+    from mach2d.assembly import ROUTINES
+    text = open(os.path.join(ROOT, "mach-2d_amd", "fortran", "coefficients_hip.f90")).read()
+    body = text[text.index("contains"):]
+    refname = {k: "get_" + k for k in ROUTINES}
+    refname["boundary_faces_flags"] = "initialize_boundary_faces_flags"
+    refname["mass_conservation"] = "check_mass_conservation"
+    for k, (scal, fields, outs) in ROUTINES.items():
+        m = re.search(r"subroutine %s\((.*?)\)" % refname[k], body, re.S)
+        assert m, k
+        got = [a.strip() for a in m.group(1).replace("&", " ").split(",")]
+        want = (["nx", "ny"] + [n for n, _ in scal] +
+                [f if isinstance(f, str) else f[0] for f in fields] + [n for n, _ in outs])
+        assert got == want, k
+
+
+def test_assembly_tables_match_reference_signatures():
+    """The argument lists the C ABI, the Fortran drop-in and the oracle use
+    are the reference's: compared with the subroutine statements of
+    src/equations/mach2d_coef.f90 where the reference tree is present, and
+    the product's table with the oracle's in any case."""
+    import re
+
+    import oracle as O
+    from mach2d.assembly import ROUTINES
+    assert set(ROUTINES) == set(O.COEF)
+    for k in ROUTINES:
+        assert ROUTINES[k] == O.COEF[k], k
+    path = os.path.join(os.environ.get("MACH2D_REFERENCE", "/root/reference"), "src",
+                        "equations", "mach2d_coef.f90")
+    if not os.path.exists(path):
+        pytest.skip("reference tree not present")
+    text = open(path).read()
+    refname = {k: "get_" + k for k in ROUTINES}
+    refname["boundary_faces_flags"] = "initialize_boundary_faces_flags"
+    refname["mass_conservation"] = "check_mass_conservation"
+    for k, (scal, fields, outs) in ROUTINES.items():
+        m = re.search(r"subroutine\s+%s\s*\((.*?)\)" % refname[k], text, re.S | re.I)
+        assert m, k
+        raw = re.sub(r"!.*", "", m.group(1))
+        got = [a.strip().lower() for a in raw.replace("&", " ").split(",") if a.strip()]
+        want = (["nx", "ny"] + [n for n, _ in scal] +
+                [f if isinstance(f, str) else f[0] for f in fields] + [n for n, _ in outs])
+        assert got == [w.lower() for w in want], (k, got)

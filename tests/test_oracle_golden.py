@@ -131,7 +131,7 @@ def coef_case(g, tag):
     nx, ny, coord, mv = int(nx), int(ny), int(coord), int(mv)
     pre = "%s_in_" % tag
     fields = {k[len(pre):]: v for k, v in g.items() if k.startswith(pre)}
-    return nx, ny, dict(coord=coord, modvis=mv, beta=beta, dt=dt), fields
+    return nx, ny, dict(coord=coord, modvis=mv, beta=beta, dt=dt, Rg=287.0), fields
 
 
 @pytest.mark.parametrize("tag", COEF_TAGS)
