@@ -52,6 +52,17 @@ BYTES_PER_POINT = {
 }
 
 
+# HBM bytes per launch at 1026x514, measured with rocprofv3 (FETCH_SIZE and
+# WRITE_SIZE in separate runs of this script, scaled by 2.0 and 1.0: the
+# factors a copy of known size reads; profiles/msi_1026x514_hbm_traffic.md).
+# "residual" is the update plus the residual, as in the kernel timing.
+MEASURED_TRAFFIC = {
+    "forward": 48.05e6, "backward": 37.40e6, "residual": 63.02e6 + 16.96e6, "factor": 139.58e6,
+    "forward32": 22.75e6, "backward32": 17.41e6, "residual32": 33.72e6 + 8.53e6,
+    "factor32": 59.41e6,
+}
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -138,7 +149,7 @@ def solve_bytes(mode, N, st):
     return 4 * N * (18 + 24 * st.nit) + 8 * N * (11 * (st.outer + 1) + 3 * st.outer)
 
 
-def roofline(kt, N):
+def roofline(kt, N, traffic_known=True):
     if not kt:
         return None
     dom = max(kt, key=lambda k: kt[k][0])
@@ -147,8 +158,10 @@ def roofline(kt, N):
     per_launch = BYTES_PER_POINT[base] * size * N
     avg_ms = kt[dom][0] / kt[dom][1]
     ach = per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = MEASURED_TRAFFIC.get(dom) if traffic_known else None
     return {"bound": "hbm", "kernel": dom, "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 5),
+            "traffic": int(traffic) if traffic else None,
             "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": per_launch}
 
 
