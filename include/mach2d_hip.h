@@ -309,6 +309,31 @@ void m2d_check_mass_conservation(const int *nx, const int *ny, const double *dt,
                                  const double *ron, const double *roe, const double *Uce,
                                  const double *Vcn, double *res);
 
+/* Fictitious-row writers of the internal-flow case,
+ * src/user/internal_flow/mod_intflow.f90.  Each overwrites the coefficient
+ * rows (all diagonals) and the source entries of the boundary it owns and
+ * leaves every other row as the caller passed it.  The reference passes
+ * `type(type_intflow) :: iflow`; the fields these routines read cross the C
+ * ABI as plain arguments: plin / Tin (iflow%plin / iflow%Tin, ny values)
+ * and Twall (iflow%Twall, a scalar, listed first like the other scalars).
+ * Host or device arrays, as for the assembly entries above. */
+/* intflow_set_bcu, mod_intflow.f90:214 (Euler north rows use the slip
+ * velocity of get_u_v_north_slip, :142, which needs nx >= 4) */
+void m2d_intflow_set_bcu(const int *nx, const int *ny, const int *modvis, const double *x,
+                         const double *xp, const double *xk, const double *yk, const double *u,
+                         const double *v, double *au, double *b);
+/* intflow_set_bcv, :305 */
+void m2d_intflow_set_bcv(const int *nx, const int *ny, const int *modvis, const double *x,
+                         const double *xp, const double *xk, const double *yk, const double *u,
+                         const double *v, double *av, double *b);
+/* intflow_set_bcpl, :396 (5 diagonals S, W, P, E, N; corner rows untouched) */
+void m2d_intflow_set_bcpl(const int *nx, const int *ny, const double *x, const double *xp,
+                          const double *pl, const double *plin, double *ap, double *bp);
+/* intflow_set_bcT, :464 (Twall > 0: prescribed north wall temperature) */
+void m2d_intflow_set_bcT(const int *nx, const int *ny, const double *Twall, const double *x,
+                         const double *xp, const double *T, const double *Tin, double *at,
+                         double *bt);
+
 /* TDMA on device pointers (tdma2d9 semantics). */
 int m2d_tdma2d9_device(int nx, int ny, int nitm, const double *d_a,
                        const double *d_b, double *d_x, void *stream);

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "m2d_internal.h"
+#include "msi_stencil.h"
 
 namespace m2d {
 
@@ -38,6 +39,7 @@ struct Fields {
     int nx, ny;
     int modvis, coord;
     double dt, beta;
+    double twall;
 };
 
 __device__ __forceinline__ double dsign5(double x) { return copysign(0.5, x); }
@@ -786,6 +788,179 @@ __global__ void k_mass_sum(Fields F)
     F.out[0][0] = r;
 }
 
+// ---------------------------------------------------------------------------
+// Fictitious-row writers of the internal-flow case,
+// src/user/internal_flow/mod_intflow.f90:214-550.  Grid over the boundary:
+// thread q < nx is (q+1, 1) of the south row, nx <= q < 2nx the north row,
+// then the west column (1, 2..ny-1) and the east column (nx, 2..ny-1).
+// Each point zeroes its whole coefficient row and writes its own entries,
+// so rows the routine does not own keep the caller's values.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool boundary_point(int nx, int ny, int &i, int &j)
+{
+    const long q = (long)blockIdx.x * kTx + threadIdx.x;
+    const long r = q - 2L * nx;
+    if (q < nx) {
+        i = (int)q + 1;
+        j = 1;
+    } else if (q < 2L * nx) {
+        i = (int)(q - nx) + 1;
+        j = ny;
+    } else if (r < ny - 2) {
+        i = 1;
+        j = (int)r + 2;
+    } else if (r < 2L * (ny - 2)) {
+        i = nx;
+        j = (int)(r - (ny - 2)) + 2;
+    } else {
+        return false;
+    }
+    return true;
+}
+
+// Tangential wall velocity component C (0: u, 1: v) at np of row ny-1:
+// get_u_v_north_slip, mod_intflow.f90:158-186.
+template <int C>
+__device__ __forceinline__ double slip_at(const double *xk, const double *yk, const double *u,
+                                          const double *v, long np)
+{
+    const double Up = sqrt(u[np] * u[np] + v[np] * v[np]);
+    const double signal = copysign(1.0, xk[np] * u[np] + yk[np] * v[np]);
+    const double t = sqrt(xk[np] * xk[np] + yk[np] * yk[np]);
+    const double tc = (C == 0 ? xk[np] : yk[np]) / t;
+    return signal * Up * tc;
+}
+
+// ubn(i) / vbn(i) of get_u_v_north_slip (:142-210): the slip velocity on
+// i = 2..nx-1, linear extrapolation to the fictitious i = 1 and i = nx.
+template <int C>
+__device__ double north_slip(const double *x, const double *xp, const double *xk,
+                             const double *yk, const double *u, const double *v, int nx, int ny,
+                             int i)
+{
+    const long np = (long)nx * (ny - 2) + i - 1;
+    if (i == 1) {
+        const long npe = np + 1, npee = npe + 1;
+        const double b2 = slip_at<C>(xk, yk, u, v, npe), b3 = slip_at<C>(xk, yk, u, v, npee);
+        return b2 + (b2 - b3) * (xp[npe] - x[np]) / (xp[npee] - xp[npe]);
+    }
+    if (i == nx) {
+        const long npw = np - 1, npww = npw - 1;
+        const double b1 = slip_at<C>(xk, yk, u, v, npw), b2 = slip_at<C>(xk, yk, u, v, npww);
+        return b1 + (b1 - b2) * (x[npw] - xp[npw]) / (xp[npw] - xp[npww]);
+    }
+    return slip_at<C>(xk, yk, u, v, np);
+}
+
+// intflow_set_bcu (C = 0, :214-301) and intflow_set_bcv (C = 1, :305-392).
+// in: x, xp, xk, yk, u, v.  out: a (9 diagonals), b.
+template <int C>
+__global__ __launch_bounds__(kTx) void k_intflow_bc_uv(Fields F)
+{
+    const int nx = F.nx, ny = F.ny;
+    int i, j;
+    if (!boundary_point(nx, ny, i, j)) return;
+    const double *x = F.in[0], *xp = F.in[1], *xk = F.in[2], *yk = F.in[3], *u = F.in[4],
+                 *v = F.in[5];
+    const long N = (long)nx * ny, np = (long)nx * (j - 1) + i - 1;
+    double c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double b;
+    if (j == 1) {  // south, corners included
+        c[kN] = C == 0 ? -1.0 : 1.0;
+        c[kP] = 1.0;
+        b = 0.0;
+    } else if (j == ny) {  // north, corners included
+        c[kS] = 1.0;
+        c[kP] = 1.0;
+        b = F.modvis == 1 ? 0.0 : 2.0 * north_slip<C>(x, xp, xk, yk, u, v, nx, ny, i);
+    } else if (i == 1) {  // west
+        const long npe = np + 1, npee = npe + 1;
+        c[kE] = C == 0 ? -1.0 : 1.0;
+        c[kP] = 1.0;
+        b = C == 0 ? (2.0 * xp[npe]) / (xp[npee] - xp[npe]) * (u[npe] - u[npee]) : 0.0;
+    } else {  // east
+        const long npw = np - 1, npww = npw - 1;
+        c[kW] = -1.0;
+        c[kP] = 1.0;
+        b = C == 0 ? 2.0 * (x[npw] - xp[npw]) / (xp[npw] - xp[npww]) * (u[npw] - u[npww])
+                   : (v[npw] - v[npww]) * 2.0 * (x[npw] - xp[npw]) / (xp[npw] - xp[npww]);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F.out[0][k * N + np] = c[k];
+    F.out[1][np] = b;
+}
+
+// intflow_set_bcpl: :396-459.  5 diagonals (S, W, P, E, N); corners are not
+// written.  in: x, xp, pl, plin (ny values).  out: ap, bp.
+__global__ __launch_bounds__(kTx) void k_intflow_bc_pl(Fields F)
+{
+    const int nx = F.nx, ny = F.ny;
+    int i, j;
+    if (!boundary_point(nx, ny, i, j)) return;
+    if ((j == 1 || j == ny) && (i == 1 || i == nx)) return;
+    const double *x = F.in[0], *xp = F.in[1], *pl = F.in[2], *plin = F.in[3];
+    const long N = (long)nx * ny, np = (long)nx * (j - 1) + i - 1;
+    double c[5] = {0, 0, 0, 0, 0};
+    double b;
+    if (j == 1) {
+        c[4] = -1.0;
+        c[2] = 1.0;
+        b = 0.0;
+    } else if (j == ny) {
+        c[0] = -1.0;
+        c[2] = 1.0;
+        b = 0.0;
+    } else if (i == 1) {
+        c[3] = 1.0;
+        c[2] = 1.0;
+        b = 2.0 * plin[j - 1];
+    } else {
+        const long npw = np - 1, npww = npw - 1;
+        c[1] = -1.0;
+        c[2] = 1.0;
+        b = 2.0 * (x[npw] - xp[npw]) / (xp[npw] - xp[npww]) * (pl[npw] - pl[npww]);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) F.out[0][k * N + np] = c[k];
+    F.out[1][np] = b;
+}
+
+// intflow_set_bcT: :464-550.  F.twall > 0 prescribes the north wall
+// temperature, otherwise the north wall is adiabatic.  in: x, xp, T, Tin
+// (ny values).  out: at, bt.
+__global__ __launch_bounds__(kTx) void k_intflow_bc_T(Fields F)
+{
+    const int nx = F.nx, ny = F.ny;
+    int i, j;
+    if (!boundary_point(nx, ny, i, j)) return;
+    const double *x = F.in[0], *xp = F.in[1], *T = F.in[2], *Tin = F.in[3];
+    const long N = (long)nx * ny, np = (long)nx * (j - 1) + i - 1;
+    const bool ccTw = F.twall > 0.0;
+    double c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double b;
+    if (j == 1) {
+        c[kN] = -1.0;
+        c[kP] = 1.0;
+        b = 0.0;
+    } else if (j == ny) {
+        c[kS] = ccTw ? 1.0 : -1.0;
+        c[kP] = 1.0;
+        b = ccTw ? 2.0 * F.twall : 0.0;
+    } else if (i == 1) {
+        c[kE] = 1.0;
+        c[kP] = 1.0;
+        b = 2.0 * Tin[j - 1];
+    } else {
+        const long npw = np - 1, npww = npw - 1;
+        c[kW] = -1.0;
+        c[kP] = 1.0;
+        b = 2.0 * (x[npw] - xp[npw]) / (xp[npw] - xp[npww]) * (T[npw] - T[npww]);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F.out[0][k * N + np] = c[k];
+    F.out[1][np] = b;
+}
+
 }  // namespace m2d
 
 using namespace m2d;
@@ -797,7 +972,7 @@ namespace {
 
 thread_local hipStream_t g_stream = nullptr;
 
-enum Grid { kInterior = 0, kFaces = 1, kAll = 2 };
+enum Grid { kInterior = 0, kFaces = 1, kAll = 2, kBoundary = 3 };
 
 struct Routine {
     void (*kernel)(Fields);
@@ -805,8 +980,9 @@ struct Routine {
     int in9[2];   // inputs that are 9-diagonal arrays (-1: none)
     int nout;
     int outk[6];  // N-vectors per output; 0 = one scalar
-    int grid;     // kInterior, kFaces (1..nx-1 x 1..ny-1) or kAll
+    int grid;     // kInterior, kFaces (1..nx-1 x 1..ny-1), kAll or kBoundary
     void (*post)(Fields);  // one-workgroup follow-up over a scratch output
+    int iny1;     // 1 + index of an input of ny values (0: none)
 };
 
 bool is_device_ptr(const void *p)
@@ -819,8 +995,9 @@ bool is_device_ptr(const void *p)
     return at.type == hipMemoryTypeDevice;
 }
 
-long in_len(const Routine &R, int k, long N)
+long in_len(const Routine &R, int k, long N, int ny)
 {
+    if (k + 1 == R.iny1) return ny;
     return (k == R.in9[0] || k == R.in9[1]) ? 9 * N : N;
 }
 long out_len(const Routine &R, int k, long N) { return R.outk[k] ? R.outk[k] * N : 1; }
@@ -844,12 +1021,14 @@ int run(const Routine &R, Fields F)
         grid = dim3((unsigned)((nx - 1 + kTx - 1) / kTx), (unsigned)(ny - 1));
     else if (R.grid == kInterior)
         grid = dim3((unsigned)((nx - 2 + kTx - 1) / kTx), (unsigned)(ny - 2));
+    else if (R.grid == kBoundary)
+        grid = dim3((unsigned)((2L * nx + 2L * (ny - 2) + kTx - 1) / kTx), 1);
     else
         grid = dim3((unsigned)((N + kTx - 1) / kTx), 1);
     // device buffer: staged host arrays and the scratch output of `post`
     long tot = R.post ? N : 0;
     if (!dev) {
-        for (int k = 0; k < R.nin; ++k) tot += in_len(R, k, N);
+        for (int k = 0; k < R.nin; ++k) tot += in_len(R, k, N, ny);
         for (int k = 0; k < R.nout; ++k) tot += out_len(R, k, N);
     }
     double *d = nullptr;
@@ -860,7 +1039,7 @@ int run(const Routine &R, Fields F)
     long off = 0;
     if (!dev) {
         for (int k = 0; k < R.nin && !rc; ++k) {
-            const long n = in_len(R, k, N);
+            const long n = in_len(R, k, N, ny);
             if (hipMemcpy(d + off, F.in[k], sizeof(double) * n, hipMemcpyHostToDevice) !=
                 hipSuccess)
                 rc = M2D_ERR_HIP;
@@ -914,6 +1093,10 @@ const Routine kFcor = {k_velocity_correction_at_faces_with_pl, 11, {-1, -1}, 6,
                        {1, 1, 1, 1, 1, 1}, kFaces, nullptr};
 const Routine kFlags = {k_boundary_faces_flags, 0, {-1, -1}, 2, {1, 1}, kAll, nullptr};
 const Routine kMass = {k_mass_terms, 10, {-1, -1}, 1, {0}, kInterior, k_mass_sum};
+const Routine kBcU = {k_intflow_bc_uv<0>, 6, {-1, -1}, 2, {9, 1}, kBoundary, nullptr};
+const Routine kBcV = {k_intflow_bc_uv<1>, 6, {-1, -1}, 2, {9, 1}, kBoundary, nullptr};
+const Routine kBcPl = {k_intflow_bc_pl, 4, {-1, -1}, 2, {5, 1}, kBoundary, nullptr, 4};
+const Routine kBcT = {k_intflow_bc_T, 4, {-1, -1}, 2, {9, 1}, kBoundary, nullptr, 4};
 
 Fields fields(const int *nx, const int *ny)
 {
@@ -1239,4 +1422,69 @@ extern "C" void m2d_check_mass_conservation(const int *nx, const int *ny, const 
     for (int k = 0; k < 10; ++k) IN(k, in[k]);
     F.out[0] = res;
     FINISH(kMass);
+}
+
+// --- fictitious-row writers of the internal-flow case (mod_intflow.f90) ----
+// Rows the routine does not own keep the caller's values, as in the
+// reference; plin / Tin (iflow%plin, iflow%Tin) hold ny values, Twall is
+// iflow%Twall.  The Euler north-slip extrapolation (modvis /= 1) reads
+// columns 2 and 3 (nx-1 and nx-2), so it needs nx >= 4.
+
+extern "C" void m2d_intflow_set_bcu(const int *nx, const int *ny, const int *modvis,
+                                    const double *x, const double *xp, const double *xk,
+                                    const double *yk, const double *u, const double *v,
+                                    double *au, double *b)
+{
+    CHECK_SCALARS(nx, ny, modvis);
+    if (*nx < 4 && *modvis != 1) return m2d_set_last_status_code(M2D_ERR_ARG);
+    Fields F = fields(nx, ny);
+    F.modvis = *modvis;
+    const double *in[] = {x, xp, xk, yk, u, v};
+    for (int k = 0; k < 6; ++k) IN(k, in[k]);
+    F.out[0] = au;
+    F.out[1] = b;
+    FINISH(kBcU);
+}
+
+extern "C" void m2d_intflow_set_bcv(const int *nx, const int *ny, const int *modvis,
+                                    const double *x, const double *xp, const double *xk,
+                                    const double *yk, const double *u, const double *v,
+                                    double *av, double *b)
+{
+    CHECK_SCALARS(nx, ny, modvis);
+    if (*nx < 4 && *modvis != 1) return m2d_set_last_status_code(M2D_ERR_ARG);
+    Fields F = fields(nx, ny);
+    F.modvis = *modvis;
+    const double *in[] = {x, xp, xk, yk, u, v};
+    for (int k = 0; k < 6; ++k) IN(k, in[k]);
+    F.out[0] = av;
+    F.out[1] = b;
+    FINISH(kBcV);
+}
+
+extern "C" void m2d_intflow_set_bcpl(const int *nx, const int *ny, const double *x,
+                                     const double *xp, const double *pl, const double *plin,
+                                     double *ap, double *bp)
+{
+    CHECK_SCALARS(nx, ny);
+    Fields F = fields(nx, ny);
+    const double *in[] = {x, xp, pl, plin};
+    for (int k = 0; k < 4; ++k) IN(k, in[k]);
+    F.out[0] = ap;
+    F.out[1] = bp;
+    FINISH(kBcPl);
+}
+
+extern "C" void m2d_intflow_set_bcT(const int *nx, const int *ny, const double *Twall,
+                                    const double *x, const double *xp, const double *T,
+                                    const double *Tin, double *at, double *bt)
+{
+    CHECK_SCALARS(nx, ny, Twall);
+    Fields F = fields(nx, ny);
+    F.twall = *Twall;
+    const double *in[] = {x, xp, T, Tin};
+    for (int k = 0; k < 4; ++k) IN(k, in[k]);
+    F.out[0] = at;
+    F.out[1] = bt;
+    FINISH(kBcT);
 }

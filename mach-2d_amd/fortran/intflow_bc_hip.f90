@@ -1,0 +1,81 @@
+! intflow_bc_hip.f90 -- the fictitious-row writers of the internal-flow case
+! (src/user/internal_flow/mod_intflow.f90: intflow_set_bcu, intflow_set_bcv,
+! intflow_set_bcpl, intflow_set_bcT) on the GPU, through the C ABI of
+! libmach2d_hip.so (include/mach2d_hip.h, m2d_intflow_set_*).  Each routine
+! keeps the reference's dummy arguments, except that the derived-type
+! argument `iflow` is replaced by the fields the routine reads (plin; Tin and
+! Twall), so mod_intflow's routines become one-line calls such as
+!    call intflow_set_bcpl_hip(nx, ny, x, xp, pl, iflow%plin, ap, bp)
+! Arrays are host arrays here (the library copies them to the GPU and back);
+! status of the last call: m2d_last_status.
+module intflow_bc_hip
+   use iso_c_binding, only: c_int, c_double
+   implicit none
+   private
+   public :: intflow_set_bcu_hip, intflow_set_bcv_hip, intflow_set_bcpl_hip, &
+      intflow_set_bcT_hip
+
+   interface
+      subroutine c_intflow_set_bcu(nx, ny, modvis, x, xp, xk, yk, u, v, au, b) &
+            bind(C, name="m2d_intflow_set_bcu")
+         import :: c_int, c_double
+         integer(c_int) :: nx, ny, modvis
+         real(c_double) :: x(*), xp(*), xk(*), yk(*), u(*), v(*), au(*), b(*)
+      end subroutine
+      subroutine c_intflow_set_bcv(nx, ny, modvis, x, xp, xk, yk, u, v, av, b) &
+            bind(C, name="m2d_intflow_set_bcv")
+         import :: c_int, c_double
+         integer(c_int) :: nx, ny, modvis
+         real(c_double) :: x(*), xp(*), xk(*), yk(*), u(*), v(*), av(*), b(*)
+      end subroutine
+      subroutine c_intflow_set_bcpl(nx, ny, x, xp, pl, plin, ap, bp) &
+            bind(C, name="m2d_intflow_set_bcpl")
+         import :: c_int, c_double
+         integer(c_int) :: nx, ny
+         real(c_double) :: x(*), xp(*), pl(*), plin(*), ap(*), bp(*)
+      end subroutine
+      subroutine c_intflow_set_bcT(nx, ny, Twall, x, xp, T, Tin, at, bt) &
+            bind(C, name="m2d_intflow_set_bcT")
+         import :: c_int, c_double
+         integer(c_int) :: nx, ny
+         real(c_double) :: Twall
+         real(c_double) :: x(*), xp(*), T(*), Tin(*), at(*), bt(*)
+      end subroutine
+   end interface
+
+contains
+
+   ! mod_intflow.f90:214
+   subroutine intflow_set_bcu_hip(nx, ny, modvis, x, xp, xk, yk, u, v, au, b)
+      integer, intent(in) :: nx, ny, modvis
+      real(8), intent(in) :: x(nx*ny), xp(nx*ny), xk(nx*ny), yk(nx*ny), u(nx*ny), v(nx*ny)
+      real(8), intent(inout) :: au(nx*ny,9), b(nx*ny)
+      call c_intflow_set_bcu(nx, ny, modvis, x, xp, xk, yk, u, v, au, b)
+   end subroutine
+
+   ! mod_intflow.f90:305
+   subroutine intflow_set_bcv_hip(nx, ny, modvis, x, xp, xk, yk, u, v, av, b)
+      integer, intent(in) :: nx, ny, modvis
+      real(8), intent(in) :: x(nx*ny), xp(nx*ny), xk(nx*ny), yk(nx*ny), u(nx*ny), v(nx*ny)
+      real(8), intent(inout) :: av(nx*ny,9), b(nx*ny)
+      call c_intflow_set_bcv(nx, ny, modvis, x, xp, xk, yk, u, v, av, b)
+   end subroutine
+
+   ! mod_intflow.f90:396 (iflow%plin)
+   subroutine intflow_set_bcpl_hip(nx, ny, x, xp, pl, plin, ap, bp)
+      integer, intent(in) :: nx, ny
+      real(8), intent(in) :: x(nx*ny), xp(nx*ny), pl(nx*ny), plin(ny)
+      real(8), intent(inout) :: ap(nx*ny,5), bp(nx*ny)
+      call c_intflow_set_bcpl(nx, ny, x, xp, pl, plin, ap, bp)
+   end subroutine
+
+   ! mod_intflow.f90:464 (iflow%Twall, iflow%Tin)
+   subroutine intflow_set_bcT_hip(nx, ny, Twall, x, xp, T, Tin, at, bt)
+      integer, intent(in) :: nx, ny
+      real(8), intent(in) :: Twall
+      real(8), intent(in) :: x(nx*ny), xp(nx*ny), T(nx*ny), Tin(ny)
+      real(8), intent(inout) :: at(nx*ny,9), bt(nx*ny)
+      call c_intflow_set_bcT(nx, ny, Twall, x, xp, T, Tin, at, bt)
+   end subroutine
+
+end module intflow_bc_hip

@@ -59,6 +59,7 @@ EXPORTS = [
     "m2d_get_pressure_correction_with_pl", "m2d_get_density_correction_with_pl",
     "m2d_get_u_v_at_real_nodes_with_pl", "m2d_get_velocity_correction_at_faces_with_pl",
     "m2d_initialize_boundary_faces_flags", "m2d_check_mass_conservation",
+    "m2d_intflow_set_bcu", "m2d_intflow_set_bcv", "m2d_intflow_set_bcpl", "m2d_intflow_set_bcT",
 ]
 
 _lib = None

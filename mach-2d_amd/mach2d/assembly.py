@@ -83,12 +83,22 @@ ROUTINES = {
     "mass_conservation": ([("dt", "d")],
                           ["rp", "re", "rn", "Jp", "ro", "roa", "ron", "roe", "Uce", "Vcn"],
                           [("res", 0)]),
+    # fictitious-row writers, src/user/internal_flow/mod_intflow.f90:214, :305,
+    # :396, :464; ("plin", "ny") is iflow%plin (ny values), Twall iflow%Twall
+    "intflow_set_bcu": ([("modvis", "i")], ["x", "xp", "xk", "yk", "u", "v"],
+                        [("a", 9), ("b", 1)]),
+    "intflow_set_bcv": ([("modvis", "i")], ["x", "xp", "xk", "yk", "u", "v"],
+                        [("a", 9), ("b", 1)]),
+    "intflow_set_bcpl": ([], ["x", "xp", "pl", ("plin", "ny")], [("a", 5), ("b", 1)]),
+    "intflow_set_bcT": ([("Twall", "d")], ["x", "xp", "T", ("Tin", "ny")], [("a", 9), ("b", 1)]),
 }
 
 # C entry of each routine (the reference's subroutine name, prefixed)
 ENTRY = {name: "m2d_get_" + name for name in ROUTINES}
 ENTRY["boundary_faces_flags"] = "m2d_initialize_boundary_faces_flags"
 ENTRY["mass_conservation"] = "m2d_check_mass_conservation"
+for _k in ("intflow_set_bcu", "intflow_set_bcv", "intflow_set_bcpl", "intflow_set_bcT"):
+    ENTRY[_k] = "m2d_" + _k
 
 
 def _ptr(x, n, name):
@@ -124,8 +134,7 @@ def call(name, nx, ny, scalars, fields, out=None):
     first = None
     for k in names:
         key, m = k if isinstance(k, tuple) else (k, 1)
-        p, obj = _ptr(fields[key].reshape(-1) if isinstance(fields[key], np.ndarray)
-                      else fields[key].reshape(-1), m * n, key)
+        p, obj = _ptr(fields[key].reshape(-1), ny if m == "ny" else m * n, key)
         keep.append(obj)
         args.append(p)
         first = obj if first is None else first

@@ -338,4 +338,10 @@ def flow_fields(nx: int, ny: int, seed: int = 0, q: float = 1.05, coord: int = 1
     fbn[0, :] = fbn[ny - 2, :] = 1.0
     f["fbe"] = fbe.reshape(-1)
     f["fbn"] = fbn.reshape(-1)
+    # internal-flow boundary data (mod_intflow.f90): x of the north-east
+    # corner of each volume, inflow pressure correction and temperature per
+    # row (ny values)
+    f["x"] = np.ascontiguousarray(g.x.reshape(-1))
+    f["plin"] = rng.uniform(-50.0, 50.0, size=ny)
+    f["Tin"] = rng.uniform(280.0, 320.0, size=ny)
     return f

@@ -93,4 +93,19 @@ void orc_p_source(int nx, int ny, double dt, const double *rp, const double *re,
                   const double *ro, const double *roa, const double *Uce, const double *Vcn,
                   const double *de, const double *dn, const double *g, const double *pl,
                   double *b);
+
+/* bc.c: fictitious-row writers of src/user/internal_flow/mod_intflow.f90 */
+void orc_u_v_north_slip(int nx, int ny, const double *x, const double *xp, const double *xk,
+                        const double *yk, const double *u, const double *v, double *ubn,
+                        double *vbn);
+void orc_intflow_set_bcu(int nx, int ny, int modvis, const double *x, const double *xp,
+                         const double *xk, const double *yk, const double *u, const double *v,
+                         double *au, double *b);
+void orc_intflow_set_bcv(int nx, int ny, int modvis, const double *x, const double *xp,
+                         const double *xk, const double *yk, const double *u, const double *v,
+                         double *av, double *b);
+void orc_intflow_set_bcpl(int nx, int ny, const double *x, const double *xp, const double *pl,
+                          const double *plin, double *a, double *bp);
+void orc_intflow_set_bcT(int nx, int ny, double Twall, const double *x, const double *xp,
+                         const double *T, const double *Tin, double *a, double *bt);
 #endif

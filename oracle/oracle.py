@@ -157,6 +157,14 @@ COEF = {
                  ["rp", "re", "rn", "Jp", "roe", "ron", "ro", "roa", "Uce", "Vcn", "de", "dn",
                   "g", "pl"],
                  [("b", 1)]),
+    # fictitious-row writers of src/user/internal_flow/mod_intflow.f90 (only
+    # boundary rows are written; ("plin", "ny") is a vector of ny values)
+    "intflow_set_bcu": ([("modvis", "i")], ["x", "xp", "xk", "yk", "u", "v"],
+                        [("a", 9), ("b", 1)]),
+    "intflow_set_bcv": ([("modvis", "i")], ["x", "xp", "xk", "yk", "u", "v"],
+                        [("a", 9), ("b", 1)]),
+    "intflow_set_bcpl": ([], ["x", "xp", "pl", ("plin", "ny")], [("a", 5), ("b", 1)]),
+    "intflow_set_bcT": ([("Twall", "d")], ["x", "xp", "T", ("Tin", "ny")], [("a", 9), ("b", 1)]),
 }
 
 

@@ -14,6 +14,8 @@ module m2dref_shim
    use mod_class_solver_msi
    use mod_solvers, only: norm_l1_9d, norm_l1_9d_relative, norm_l1_b
    use coefficients
+   use mod_intflow, only: type_intflow, intflow_set_bcu, intflow_set_bcv, &
+      intflow_set_bcpl, intflow_set_bcT
    implicit none
 contains
 
@@ -289,6 +291,52 @@ contains
          Vcn
       real(c_double), intent(inout) :: res
       call check_mass_conservation(nx, ny, dt, rp, re, rn, Jp, ro, roa, ron, roe, Uce, Vcn, res)
+   end subroutine
+
+   ! src/user/internal_flow/mod_intflow.f90:214
+   subroutine ref_intflow_set_bcu(nx, ny, modvis, x, xp, xk, yk, u, v, au, b) &
+         bind(C, name="ref_intflow_set_bcu")
+      integer(c_int), value :: nx, ny, modvis
+      real(c_double), dimension(nx*ny), intent(in) :: x, xp, xk, yk, u, v
+      real(c_double), intent(inout) :: au(nx*ny,9), b(nx*ny)
+      call intflow_set_bcu(nx, ny, modvis, x, xp, xk, yk, u, v, au, b)
+   end subroutine
+
+   ! mod_intflow.f90:305
+   subroutine ref_intflow_set_bcv(nx, ny, modvis, x, xp, xk, yk, u, v, av, b) &
+         bind(C, name="ref_intflow_set_bcv")
+      integer(c_int), value :: nx, ny, modvis
+      real(c_double), dimension(nx*ny), intent(in) :: x, xp, xk, yk, u, v
+      real(c_double), intent(inout) :: av(nx*ny,9), b(nx*ny)
+      call intflow_set_bcv(nx, ny, modvis, x, xp, xk, yk, u, v, av, b)
+   end subroutine
+
+   ! mod_intflow.f90:396; iflow carries only plin here (the routine reads nothing else)
+   subroutine ref_intflow_set_bcpl(nx, ny, x, xp, pl, plin, ap, bp) &
+         bind(C, name="ref_intflow_set_bcpl")
+      integer(c_int), value :: nx, ny
+      real(c_double), dimension(nx*ny), intent(in) :: x, xp, pl
+      real(c_double), intent(in) :: plin(ny)
+      real(c_double), intent(inout) :: ap(nx*ny,5), bp(nx*ny)
+      type(type_intflow) :: iflow
+      allocate(iflow%plin(ny))
+      iflow%plin = plin
+      call intflow_set_bcpl(nx, ny, x, xp, pl, iflow, ap, bp)
+   end subroutine
+
+   ! mod_intflow.f90:464; iflow carries Tin and Twall (the fields the routine reads)
+   subroutine ref_intflow_set_bcT(nx, ny, Twall, x, xp, T, Tin, at, bt) &
+         bind(C, name="ref_intflow_set_bcT")
+      integer(c_int), value :: nx, ny
+      real(c_double), value :: Twall
+      real(c_double), dimension(nx*ny), intent(in) :: x, xp, T
+      real(c_double), intent(in) :: Tin(ny)
+      real(c_double), intent(inout) :: at(nx*ny,9), bt(nx*ny)
+      type(type_intflow) :: iflow
+      allocate(iflow%Tin(ny))
+      iflow%Tin = Tin
+      iflow%Twall = Twall
+      call intflow_set_bcT(nx, ny, x, xp, T, iflow, at, bt)
    end subroutine
 
 end module m2dref_shim

@@ -189,10 +189,14 @@ def coef():
     rng = np.random.default_rng(91)
     for tag, nx, ny, coord, mv, cart, axis in COEF_CASES:
         f = coef_fields(nx, ny, coord, cart, axis, seed=nx * 31 + ny + coord)
-        sc = dict(beta=0.35, modvis=mv, dt=2e-3, coord=coord, Rg=287.0)
+        # Twall: prescribed north wall temperature on the axisymmetric cases,
+        # adiabatic wall (Twall <= 0) on the planar ones
+        sc = dict(beta=0.35, modvis=mv, dt=2e-3, coord=coord, Rg=287.0,
+                  Twall=300.0 if coord == 1 else -1.0)
         for k, v in f.items():
             arrs["%s_in_%s" % (tag, k)] = v
         arrs["%s_scalars" % tag] = np.array([nx, ny, coord, mv, sc["beta"], sc["dt"]])
+        arrs["%s_twall" % tag] = np.array([sc["Twall"]])
         for name, (_, _, outs) in O.COEF.items():
             init = {k: rng.uniform(-1, 1, size=(m, nx * ny) if m else (1, 1)) for k, m in outs}
             for k, v in init.items():

@@ -50,7 +50,8 @@ def test_assembly_host_arrays_bitwise(gpu, orc, case):
     from mach2d import assembly as A
     nx, ny, coord, mv, cart = case
     f = _fields(nx, ny, coord, cart, seed=nx * ny + coord)
-    sc = dict(beta=0.3, modvis=mv, dt=1e-3, coord=coord, Rg=287.0)
+    sc = dict(beta=0.3, modvis=mv, dt=1e-3, coord=coord, Rg=287.0,
+              Twall=300.0 if coord else -1.0)
     rng = np.random.default_rng(7)
     for name, (_, _, outs) in A.ROUTINES.items():
         init = {k: rng.uniform(-1, 1, size=m * nx * ny if m else 1) for k, m in outs}
@@ -68,7 +69,8 @@ def test_assembly_device_arrays_bitwise(gpu, orc, case):
     nx, ny, coord, mv, cart = case
     f = _fields(nx, ny, coord, cart, seed=11 + nx)
     tf = {k: torch.tensor(v.reshape(-1), device="cuda") for k, v in f.items()}
-    sc = dict(beta=0.2, modvis=mv, dt=2e-3, coord=coord, Rg=287.0)
+    sc = dict(beta=0.2, modvis=mv, dt=2e-3, coord=coord, Rg=287.0,
+              Twall=-1.0 if coord else 310.0)
     for name, (_, _, outs) in A.ROUTINES.items():
         want = orc.coef(name, nx, ny, sc, f)
         if name == "boundary_faces_flags":   # no input fields: outputs choose the memory
@@ -102,7 +104,8 @@ def test_assembly_vs_reference_fixtures(gpu, tag):
     g = load_golden("coef.npz")
     nx, ny, coord, mv, beta, dt = g["%s_scalars" % tag]
     nx, ny = int(nx), int(ny)
-    sc = dict(coord=int(coord), modvis=int(mv), beta=beta, dt=dt, Rg=287.0)
+    sc = dict(coord=int(coord), modvis=int(mv), beta=beta, dt=dt, Rg=287.0,
+              Twall=float(g["%s_twall" % tag][0]))
     pre = "%s_in_" % tag
     f = {k[len(pre):]: v for k, v in g.items() if k.startswith(pre)}
     for name, (_, _, outs) in A.ROUTINES.items():

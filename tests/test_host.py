@@ -110,7 +110,7 @@ def test_fortran_dropin_module_compiles(tmp_path):
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     if not os.path.exists(fc):
         pytest.skip("no Fortran compiler")
-    for name in ("msi2d9_hip.f90", "coefficients_hip.f90"):
+    for name in ("msi2d9_hip.f90", "coefficients_hip.f90", "intflow_bc_hip.f90"):
         src = os.path.join(ROOT, "mach-2d_amd", "fortran", name)
         r = subprocess.run([fc, "-c", "-module-dir", str(tmp_path), "-o",
                             str(tmp_path / (name + ".o")), src], capture_output=True, text=True)
@@ -130,6 +130,8 @@ def test_fortran_coefficients_module_matches_reference_interface():
     refname["boundary_faces_flags"] = "initialize_boundary_faces_flags"
     refname["mass_conservation"] = "check_mass_conservation"
     for k, (scal, fields, outs) in ROUTINES.items():
+        if k.startswith("intflow_"):
+            continue   # module mod_intflow, covered below
         m = re.search(r"subroutine %s\((.*?)\)" % refname[k], body, re.S)
         assert m, k
         got = [a.strip() for a in m.group(1).replace("&", " ").split(",")]
@@ -158,11 +160,52 @@ def test_assembly_tables_match_reference_signatures():
     refname = {k: "get_" + k for k in ROUTINES}
     refname["boundary_faces_flags"] = "initialize_boundary_faces_flags"
     refname["mass_conservation"] = "check_mass_conservation"
+    intflow = open(os.path.join(os.path.dirname(os.path.dirname(path)), "user",
+                                "internal_flow", "mod_intflow.f90")).read()
+    # the derived-type argument iflow crosses the C ABI as the fields the
+    # routine reads; output names a / b stand for the reference's own
+    iflow = {"intflow_set_bcpl": ["plin"], "intflow_set_bcT": ["Tin"]}
+    outname = {"intflow_set_bcu": ("au", "b"), "intflow_set_bcv": ("av", "b"),
+               "intflow_set_bcpl": ("ap", "bp"), "intflow_set_bcT": ("at", "bt")}
     for k, (scal, fields, outs) in ROUTINES.items():
-        m = re.search(r"subroutine\s+%s\s*\((.*?)\)" % refname[k], text, re.S | re.I)
+        src, rn = (intflow, k) if k.startswith("intflow_") else (text, refname[k])
+        m = re.search(r"subroutine\s+%s\s*\((.*?)\)" % rn, src, re.S | re.I)
         assert m, k
         raw = re.sub(r"!.*", "", m.group(1))
         got = [a.strip().lower() for a in raw.replace("&", " ").split(",") if a.strip()]
+        if k in iflow:   # Twall (a scalar, listed with the scalars) and the arrays
+            at = got.index("iflow")
+            got = got[:at] + [w.lower() for w in iflow[k]] + got[at + 1:]
+            if k == "intflow_set_bcT":
+                got = got[:2] + ["twall"] + got[2:]
         want = (["nx", "ny"] + [n for n, _ in scal] +
                 [f if isinstance(f, str) else f[0] for f in fields] + [n for n, _ in outs])
+        if k in outname:
+            want = want[:-2] + list(outname[k])
         assert got == [w.lower() for w in want], (k, got)
+
+
+def test_fortran_intflow_module_matches_reference_interface():
+    """The boundary writers of intflow_bc_hip keep the dummy arguments of
+    src/user/internal_flow/mod_intflow.f90 (iflow replaced by the fields
+    read: plin; Twall and Tin), as the C table mach2d.assembly.ROUTINES
+    states them."""
+    import re
+
+    from mach2d.assembly import ROUTINES
+    text = open(os.path.join(ROOT, "mach-2d_amd", "fortran", "intflow_bc_hip.f90")).read()
+    body = text[text.index("contains"):]
+    names = [k for k in ROUTINES if k.startswith("intflow_")]
+    assert names == ["intflow_set_bcu", "intflow_set_bcv", "intflow_set_bcpl", "intflow_set_bcT"]
+    for k in names:
+        scal, fields, outs = ROUTINES[k]
+        m = re.search(r"subroutine %s_hip\((.*?)\)" % k, body, re.S)
+        assert m, k
+        got = [a.strip() for a in m.group(1).replace("&", " ").split(",")]
+        want = (["nx", "ny"] + [n for n, _ in scal] +
+                [f if isinstance(f, str) else f[0] for f in fields] + [n for n, _ in outs])
+        want = [{"a": {"intflow_set_bcpl": "ap", "intflow_set_bcT": "at",
+                       "intflow_set_bcv": "av"}.get(k, "au"),
+                 "b": {"intflow_set_bcpl": "bp", "intflow_set_bcT": "bt"}.get(k, "b")}.get(w, w)
+                for w in want]
+        assert got == want, k

@@ -131,7 +131,8 @@ def coef_case(g, tag):
     nx, ny, coord, mv = int(nx), int(ny), int(coord), int(mv)
     pre = "%s_in_" % tag
     fields = {k[len(pre):]: v for k, v in g.items() if k.startswith(pre)}
-    return nx, ny, dict(coord=coord, modvis=mv, beta=beta, dt=dt, Rg=287.0), fields
+    twall = float(g["%s_twall" % tag][0])
+    return nx, ny, dict(coord=coord, modvis=mv, beta=beta, dt=dt, Rg=287.0, Twall=twall), fields
 
 
 @pytest.mark.parametrize("tag", COEF_TAGS)
