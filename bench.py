@@ -15,7 +15,10 @@ stopping rule).  "best" runs both; value is the FP32-refined solves/s when
 its solution is within 1e-8 relative of the FP64 one, else the FP64 figure,
 and speedup_vs_plain_fp64 compares the two.  The batch entry times 256
 independent p' systems of uneven size (64s x 32s, s log-uniform on [1, 8])
-solved together to the same drop (autotuned placement, graph replays).
+solved together to the same drop (autotuned placement, graph replays);
+batch_1026x514 times 32 systems of the headline size (its matrix, their own
+seeded right-hand sides) solved together, with their throughput against
+the single plain-FP64 solve.
 
 Prints one JSON line.  gbps = algorithmic bytes of the solve over its time
 (FP64: 8 N (18 + 11 + 24 K); refinement: 4 N (18 + 24 K_inner) +
@@ -165,21 +168,35 @@ def roofline(kt, N, traffic_known=True):
             "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": per_launch}
 
 
-def batch_run(mode, steps, warmup, count=256):
+def batch_run(mode, steps, warmup, count=256, same=None):
     """256 independent p' systems (own grid size, stretching and RHS) solved
-    together to the 1e-10 drop."""
+    together to the 1e-10 drop; with `same` (a system), `count` systems with
+    its matrix and their own seeded Gaussian right-hand sides."""
     import torch
 
     from mach2d import problems as P
     from mach2d import solvers as S
-    sizes = P.cfg4_sizes(count)
     t = time.time()
-    systems = [P.pcorr_system(nx, ny, seed=100 + q, q=1.02 + 0.06 * ((q * 0.618) % 1.0),
-                              rhs="gauss") for q, (nx, ny) in enumerate(sizes)]
+    if same is None:
+        sizes = P.cfg4_sizes(count)
+        systems = [P.pcorr_system(nx, ny, seed=100 + q, q=1.02 + 0.06 * ((q * 0.618) % 1.0),
+                                  rhs="gauss") for q, (nx, ny) in enumerate(sizes)]
+        a = [torch.tensor(x.a.reshape(-1), device="cuda") for x in systems]
+        rhs = [torch.tensor(x.rhs, device="cuda") for x in systems]
+    else:
+        sizes = [(same.nx, same.ny)] * count
+        systems = [same] * count
+        a0 = torch.tensor(same.a.reshape(-1), device="cuda")
+        a = [a0] * count
+        interior = torch.tensor(same.rhs != 0.0, device="cuda")
+        gen = torch.Generator(device="cuda")
+        rhs = []
+        for q in range(count):
+            gen.manual_seed(7000 + q)
+            r = torch.randn(same.N, dtype=torch.float64, device="cuda", generator=gen)
+            rhs.append(torch.where(interior, r, torch.zeros_like(r)))
     log("batch: %d systems, N = %d in total, built in %.1fs" % (
         count, sum(x.N for x in systems), time.time() - t))
-    a = [torch.tensor(x.a.reshape(-1), device="cuda") for x in systems]
-    rhs = [torch.tensor(x.rhs, device="cuda") for x in systems]
     var = [torch.zeros(x.N, dtype=torch.float64, device="cuda") for x in systems]
     b = S.BatchMSI(sizes, precision=mode)
     b.factor(a)
@@ -217,6 +234,8 @@ def main():
     ap.add_argument("--mode", default="best", choices=["best", "fp64", "fp32_ir"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-cfg3", type=int, default=32,
+                    help="systems in the batch of 1026x514 p' systems")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -247,11 +266,18 @@ def main():
         nit64 = runs["fp64"]["st"].nit if "fp64" in runs else st.nit
         cpu = cpu_baseline(s, nit64)
         log("cpu: %.4g solves/s (%s)" % (cpu["value"], cpu["sample"]))
-    batch = None
+    batch = batch3 = None
     if not args.no_batch:
         batch = {m: batch_run(m, max(1, args.steps - 1), 1) for m in ("fp64", "fp32_ir")}
         for m, v in batch.items():
             log("batch %s: %s" % (m, v))
+        batch3 = {m: batch_run(m, 1, 1, count=args.batch_cfg3, same=s)
+                  for m in ("fp64", "fp32_ir")}
+        for m, v in batch3.items():
+            log("batch 1026x514 %s: %s" % (m, v))
+            if "fp64" in summary:
+                v["speedup_vs_plain_fp64"] = round(
+                    v["solves_per_s"] / summary["fp64"]["solves_per_s"], 2)
     out = {
         "metric": METRIC,
         "value": e["solves_per_s"],
@@ -277,6 +303,7 @@ def main():
         "roofline": roofline(r["kt"], s.N),
         "cpu_baseline": cpu,
         "batch": batch,
+        "batch_1026x514": batch3,
     }
     print(json.dumps(out), flush=True)
 

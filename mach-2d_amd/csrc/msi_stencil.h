@@ -90,11 +90,40 @@ M2D_HD T forward_point_y(T r, T dl1, T y, T dl2, T dl3, T dl4, T dl5, T w_w, T w
     return div_by_rcp<T>(-(r + dl2 * w_w + dl3 * w_se + dl4 * w_s + dl5 * w_sw), dl1, y);
 }
 
+// Forward substitution of the FP32 sweeps (inner solver of the FP64
+// refinement, where only the operator, not the rounding, must be the
+// reference's): the row is pre-scaled by ny = -1/dl1 (ne_k = ny * dl_k), so
+// w = ny*r + ne5*w_sw + ne4*w_s + ne2*w_w + ne3*w_se needs one multiply and
+// four fused multiply-adds, and the two values of the previous step (w_w
+// from this row, w_se from the row below) enter last.
+template <typename T>
+M2D_HD T forward_point_scaled(T r, T ny, T ne2, T ne3, T ne4, T ne5, T w_w, T w_se, T w_s,
+                              T w_sw)
+{
+    T acc = r * ny;
+    acc = fma_t(ne5, w_sw, acc);
+    acc = fma_t(ne4, w_s, acc);
+    acc = fma_t(ne2, w_w, acc);
+    return fma_t(ne3, w_se, acc);
+}
+
 // Backward substitution U z = w at one point: msi2d9-v03.f90:175.
 template <typename T>
 M2D_HD T backward_point(T w, T du1, T du2, T du3, T du4, T z_e, T z_nw, T z_n, T z_ne)
 {
     return w - du1 * z_e - du2 * z_nw - du3 * z_n - du4 * z_ne;
+}
+
+// Backward substitution of the FP32 sweeps: the same sum with fused
+// multiply-adds, the values of the previous step (z_e, then z_nw from the
+// row above) last.
+template <typename T>
+M2D_HD T backward_point_fused(T w, T du1, T du2, T du3, T du4, T z_e, T z_nw, T z_n, T z_ne)
+{
+    T acc = fma_t(-du4, z_ne, w);
+    acc = fma_t(-du3, z_n, acc);
+    acc = fma_t(-du1, z_e, acc);
+    return fma_t(-du2, z_nw, acc);
 }
 
 // Residual r = A.x - rhs at (i, j) (0-based), 2-D aware: a term whose

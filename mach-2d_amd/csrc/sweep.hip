@@ -21,8 +21,9 @@
 // next chunk.
 //
 // Padding.  Steps whose column (or row) is outside the grid compute on
-// padding.  In the sweeps the padding holds r = 0, dl1 = 1, 1/dl1 = 1 and
-// zero couplings (k_prepare_factor), so padding results are finite and no
+// padding.  In the sweeps the padding holds r = 0, dl1 = 1, 1/dl1 = 1 (in
+// the scaled FP32 rows ny = 0) and zero couplings (k_prepare_factor), so
+// padding results are finite and no
 // neighbour value needs a mask: every factor entry that couples a point to
 // a neighbour outside the grid is an exact zero (the couplings of A to
 // outside neighbours are zero, and the factorization keeps them zero), and
@@ -67,6 +68,18 @@ __device__ __forceinline__ void vstore(T *__restrict__ dst, const T (&src)[F])
         uint4 v;
         __builtin_memcpy(&v, &src[q * (int)(16 / sizeof(T))], 16);
         p[q] = v;
+    }
+}
+
+// Same, at base + ofs with a wave-uniform base and a 32-bit lane offset.
+template <typename T, int F>
+__device__ __forceinline__ void vstore(T *__restrict__ base, unsigned ofs, const T (&src)[F])
+{
+#pragma unroll
+    for (int q = 0; q < (int)(F * sizeof(T) / 16); ++q) {
+        uint4 v;
+        __builtin_memcpy(&v, &src[q * (int)(16 / sizeof(T))], 16);
+        *reinterpret_cast<uint4 *>(base + (ofs + q * (unsigned)(16 / sizeof(T)))) = v;
     }
 }
 
@@ -118,6 +131,15 @@ struct Halves {
     T v[NA][H];
 };
 
+// Row form of the sweeps.  FP64 sweeps reproduce the reference's rounding
+// (its operation order, exact division); FP32 sweeps only serve as the inner
+// solver of the FP64 refinement, so their rows are pre-scaled by -1/dl1 and
+// their sums fused and reordered to shorten the dependent chain per step.
+template <typename T>
+struct ScaledRows {
+    static constexpr bool value = sizeof(T) == 4;
+};
+
 // --------------------------------------------------------------------------
 // forward sweep  L w = -r  (msi2d9-v03.f90:139-156); r, dl (dl1..dl5, then
 // 1/dl1), w in the skewed layout.
@@ -135,7 +157,12 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
     constexpr int NW = TagWords<T>::n;
     constexpr int H = F / 2;
     constexpr int V = 16 / (int)sizeof(T);  // values per 16-byte vector
-    typedef Halves<T, 7, H> Buf;            // r, dl1..dl5, 1/dl1
+    // a lane's offset in a chunk block; chunk bases are wave-uniform, so the
+    // vector accesses take an SGPR base and this fixed VGPR offset
+    const unsigned lofs = (unsigned)lane * F;
+    constexpr bool kScaled = ScaledRows<T>::value;
+    constexpr int NA = kScaled ? 6 : 7;
+    typedef Halves<T, NA, H> Buf;  // r, dl1..dl5, 1/dl1 (scaled: r, ny, ne2..ne5)
     const u64 *bin = lk.bnd + (long)(has_up ? band - 1 : 0) * nx * NW;
     u64 *bout = lk.bnd + (long)band * nx * NW;
     u64 *scratch = lk.scratch + ((long)band * kWave + lane) * 8;
@@ -146,13 +173,13 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
     T cur = 0, se = 0, sv = 0, sw = 0;
 
     auto load_half = [&](Buf &b, int h) {  // half h = chunk h/2, elements (h%2)*H..
-        const long e = sk.chunk_base(band, min(h >> 1, nf - 1), lane) + (h & 1) * H;
+        const long e = sk.chunk_base(band, min(h >> 1, nf - 1), 0) + (h & 1) * H;
 #pragma unroll
-        for (int k = 0; k < 7; ++k) {
+        for (int k = 0; k < NA; ++k) {
             const T *p = (k == 0 ? r : dl + (long)(k - 1) * L) + e;
 #pragma unroll
             for (int q = 0; q < H / V; ++q) {
-                const uint4 v = *reinterpret_cast<const uint4 *>(p + q * V);
+                const uint4 v = *reinterpret_cast<const uint4 *>(p + (lofs + q * V));
                 __builtin_memcpy(&b.v[k][q * V], &v, 16);
             }
         }
@@ -195,7 +222,14 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
 #pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
             Buf &cb = hh == 0 ? h0 : h1;
+            // the loads of a half stay where they are: the scheduler would
+            // otherwise sink them below independent arithmetic and cut the
+            // distance the ring gives them
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_sched_barrier(0);
             load_half(hh == 0 ? h2 : h0, 2 * c + 2 + hh);
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int t = 0; t < H; ++t) {
                 const int s = hh * H + t;
@@ -203,15 +237,22 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
                 sw = sv;
                 sv = se;
                 se = up;
-                const T v = forward_point_y<T>(cb.v[0][t], cb.v[1][t], cb.v[6][t], cb.v[2][t],
-                                               cb.v[3][t], cb.v[4][t], cb.v[5][t], cur, se, sv,
-                                               sw);
+                T v;
+                if constexpr (kScaled)
+                    v = forward_point_scaled<T>(cb.v[0][t], cb.v[1][t], cb.v[2][t], cb.v[3][t],
+                                                cb.v[4][t], cb.v[5][t], cur, se, sv, sw);
+                else
+                    v = forward_point_y<T>(cb.v[0][t], cb.v[1][t], cb.v[NA - 1][t], cb.v[2][t],
+                                           cb.v[3][t], cb.v[4][t], cb.v[5][t], cur, se, sv, sw);
                 outv[s] = v;
                 cur = v;
             }
         }
-        if (c + 1 < nf && !sync(bn, c + 1)) return false;
-        vstore<T, F>(w + sk.chunk_base(band, c, lane), outv);
+        // chunk nf has no columns in the grid, so its sync returns at once;
+        // running it unconditionally keeps the boundary load of every chunk
+        // complete before the loop's back edge
+        if (!sync(bn, c + 1)) return false;
+        vstore<T, F>(w + sk.chunk_base(band, c, 0), lofs, outv);
         {  // publish the edge lane's F results (columns c*F + q - pad - 126)
             const T pv = spread_from_lane<T, F>(outv, kWave - 1, lane);
             const int col = c * F + lane - pad - 2 * (kWave - 1);
@@ -249,6 +290,7 @@ __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__res
     constexpr int NW = TagWords<T>::n;
     constexpr int H = F / 2;
     constexpr int V = 16 / (int)sizeof(T);
+    const unsigned lofs = (unsigned)lane * F;  // lane offset in a chunk block
     typedef Halves<T, 5, H> Buf;  // w, du1..du4
     const u64 *bin = lk.bnd + (long)(has_above ? band : 0) * nx * NW;
     u64 *bout = lk.bnd + (long)(has_below ? band - 1 : 0) * nx * NW;
@@ -261,13 +303,13 @@ __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__res
 
     // backward half h: stored chunk nf-1-h/2, its upper half first
     auto load_half = [&](Buf &b, int h) {
-        const long e = sk.chunk_base(band, max(nf - 1 - (h >> 1), 0), lane) + (1 - (h & 1)) * H;
+        const long e = sk.chunk_base(band, max(nf - 1 - (h >> 1), 0), 0) + (1 - (h & 1)) * H;
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             const T *p = (k == 0 ? wz : du + (long)(k - 1) * L) + e;
 #pragma unroll
             for (int q = 0; q < H / V; ++q) {
-                const uint4 v = *reinterpret_cast<const uint4 *>(p + q * V);
+                const uint4 v = *reinterpret_cast<const uint4 *>(p + (lofs + q * V));
                 __builtin_memcpy(&b.v[k][q * V], &v, 16);
             }
         }
@@ -314,14 +356,19 @@ __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__res
                 ne = nv;
                 nv = nw;
                 nw = dn;
-                const T v = backward_point<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh], cb.v[3][qh],
-                                              cb.v[4][qh], cur, nw, nv, ne);
+                T v;
+                if constexpr (ScaledRows<T>::value)
+                    v = backward_point_fused<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh],
+                                                cb.v[3][qh], cb.v[4][qh], cur, nw, nv, ne);
+                else
+                    v = backward_point<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh], cb.v[3][qh],
+                                          cb.v[4][qh], cur, nw, nv, ne);
                 outv[q] = v;
                 cur = v;
             }
         }
-        if (c + 1 < nf && !sync(bn, c + 1)) return false;
-        vstore<T, F>(wz + sk.chunk_base(band, sc, lane), outv);
+        if (!sync(bn, c + 1)) return false;
+        vstore<T, F>(wz + sk.chunk_base(band, sc, 0), lofs, outv);
         {  // publish lane 0's results: step s = lane, column sc*F + F-1-lane - pad
             T rev[F];
 #pragma unroll
@@ -623,10 +670,17 @@ __device__ __forceinline__ void prepare_entry(T *__restrict__ dl, T *__restrict_
 {
     const long L = sk.size();
     if (skew_point(sk, e).real) {
-        dl[5 * L + e] = T(1) / dl[e];
+        if constexpr (ScaledRows<T>::value) {  // ny = -1/dl1, ne_k = ny * dl_k
+            const T ny = -(T(1) / dl[e]);
+            dl[e] = ny;
+#pragma unroll
+            for (int k = 1; k < 5; ++k) dl[k * L + e] = dl[k * L + e] * ny;
+        } else {
+            dl[5 * L + e] = T(1) / dl[e];
+        }
         return;
     }
-    dl[e] = T(1);
+    dl[e] = ScaledRows<T>::value ? T(0) : T(1);
     dl[5 * L + e] = T(1);
 #pragma unroll
     for (int k = 1; k < 5; ++k) dl[k * L + e] = T(0);
