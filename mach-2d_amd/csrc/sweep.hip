@@ -140,6 +140,70 @@ struct ScaledRows {
     static constexpr bool value = sizeof(T) == 4;
 };
 
+// Results of the sweeps leave through LDS.  A sweep workgroup has two
+// waves: wave 0 computes, wave 1 stores.  After each chunk the computing
+// wave writes its F results per lane into one of two ring slots and meets
+// the storing wave at a barrier; the storing wave then writes the chunk to
+// HBM and publishes the edge row to the neighbouring band while the
+// computing wave goes on with the next chunk.  The computing wave thus
+// issues no global stores: on this architecture a store counts in the same
+// in-order counter as the loads and its registers stay busy until the store
+// completes, so stores in the computing wave made it wait, chunk after
+// chunk, for the prefetch loads behind them.  `stop` tells the storing wave
+// that the computing wave has given up (a bounded boundary wait failed);
+// both waves pass the same number of barriers on every path.
+template <typename T, int F>
+struct OutRing {
+    T v[2][kWave * F];
+    int stop;
+};
+
+// Barrier between the two waves of a sweep workgroup, after this wave's LDS
+// accesses have completed.  Only LDS is ordered: the wait leaves the vector
+// memory counter alone (vmcnt 63, expcnt 7, lgkmcnt 0), so loads in flight
+// stay in flight.  The signal fences keep the compiler from moving memory
+// accesses across the barrier.
+__device__ __forceinline__ void handoff()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+template <typename T, int F>
+__device__ __forceinline__ bool ring_stopped(OutRing<T, F> &ring)
+{
+    return __hip_atomic_load(&ring.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+}
+
+// The storing wave of a sweep: for chunk c = 0.. (after barrier c) writes
+// ring slot c&1 to dst at chunk `stored(c)` and publishes the edge lane's F
+// values to the neighbouring band: lane q < F takes element `elem(q)` of
+// lane `edge` and publishes it as column col(c, q) when `ok(col)`.
+template <typename T, int F, typename FSTORED, typename FCOL>
+__device__ void store_chunks(T *__restrict__ dst, const Skew &sk, int band, int nf,
+                             OutRing<T, F> &ring, u64 *bout, bool has_out, unsigned gen,
+                             int edge, bool reversed, FSTORED stored, FCOL col_of)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned lofs = (unsigned)lane * F;
+    constexpr int NW = TagWords<T>::n;
+    for (int c = 0; c < nf; ++c) {
+        handoff();
+        if (ring_stopped(ring)) return;
+        const T *slot = ring.v[c & 1];
+        T vals[F];
+        vload<T, F>(vals, slot + lofs);
+        vstore<T, F>(dst + sk.chunk_base(band, stored(c), 0), lofs, vals);
+        const int q = lane < F ? lane : 0;
+        const T pv = slot[edge * F + (reversed ? F - 1 - q : q)];
+        const int col = col_of(c, lane);
+        if (has_out && lane < F && col >= 0 && col < sk.nx)
+            put_tagged(bout + (long)col * NW, gen, pv);
+    }
+}
+
 // --------------------------------------------------------------------------
 // forward sweep  L w = -r  (msi2d9-v03.f90:139-156); r, dl (dl1..dl5, then
 // 1/dl1), w in the skewed layout.
@@ -149,12 +213,20 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
                                              T *__restrict__ w, const Skew &sk, const Link &lk,
                                              int band)
 {
-    const int lane = threadIdx.x;
+    __shared__ OutRing<T, F> ring;
+    const int lane = threadIdx.x & (kWave - 1);
     const int nx = sk.nx, pad = sk.pad(band);
     const int nf = sk.nst / F;  // chunks of F steps
     const long L = sk.size();
     const bool has_up = band > 0, has_down = band < sk.nb - 1;
     constexpr int NW = TagWords<T>::n;
+    if (threadIdx.x >= kWave) {  // the storing wave; edge = lane 63, columns c*F + q - pad - 126
+        store_chunks<T, F>(
+            w, sk, band, nf, ring, lk.bnd + (long)band * nx * NW, has_down, lk.gen, kWave - 1,
+            false, [](int c) { return c; },
+            [=](int c, int q) { return c * F + q - pad - 2 * (kWave - 1); });
+        return;
+    }
     constexpr int H = F / 2;
     constexpr int V = 16 / (int)sizeof(T);  // values per 16-byte vector
     // a lane's offset in a chunk block; chunk bases are wave-uniform, so the
@@ -164,8 +236,6 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
     constexpr int NA = kScaled ? 6 : 7;
     typedef Halves<T, NA, H> Buf;  // r, dl1..dl5, 1/dl1 (scaled: r, ny, ne2..ne5)
     const u64 *bin = lk.bnd + (long)(has_up ? band - 1 : 0) * nx * NW;
-    u64 *bout = lk.bnd + (long)band * nx * NW;
-    u64 *scratch = lk.scratch + ((long)band * kWave + lane) * 8;
 
     Buf X, Y, Z;
     Raw<T> bn{};
@@ -214,6 +284,12 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
         take(t, c);
         return ok;
     };
+    // Giving up: tell the storing wave, meet it at the barrier it waits at.
+    auto bail = [&]() -> bool {
+        if (lane == 0) ring.stop = 1;
+        handoff();
+        return false;
+    };
     // Chunk c: halves 2c (in h0) and 2c+1 (in h1); loads halves 2c+2 into h2
     // and 2c+3 into h0.
     auto chunk = [&](Buf &h0, Buf &h1, Buf &h2, int c) -> bool {
@@ -226,9 +302,7 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
             // otherwise sink them below independent arithmetic and cut the
             // distance the ring gives them
             __builtin_amdgcn_sched_barrier(0);
-            __builtin_amdgcn_sched_barrier(0);
             load_half(hh == 0 ? h2 : h0, 2 * c + 2 + hh);
-            __builtin_amdgcn_sched_barrier(0);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int t = 0; t < H; ++t) {
@@ -248,24 +322,22 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
                 cur = v;
             }
         }
+        vstore<T, F>(ring.v[c & 1], lofs, outv);
+        handoff();
         // chunk nf has no columns in the grid, so its sync returns at once;
         // running it unconditionally keeps the boundary load of every chunk
         // complete before the loop's back edge
-        if (!sync(bn, c + 1)) return false;
-        vstore<T, F>(w + sk.chunk_base(band, c, 0), lofs, outv);
-        {  // publish the edge lane's F results (columns c*F + q - pad - 126)
-            const T pv = spread_from_lane<T, F>(outv, kWave - 1, lane);
-            const int col = c * F + lane - pad - 2 * (kWave - 1);
-            const bool ok = has_down && lane < F && col >= 0 && col < nx;
-            put_tagged(ok ? bout + (long)col * NW : scratch, lk.gen, pv);
-        }
-        return true;
+        return sync(bn, c + 1) || bail();
     };
 
+    if (lane == 0) ring.stop = 0;
     load_half(X, 0);
     load_half(Y, 1);
     load_bnd(bn, 0);
-    if (!sync(bn, 0)) return;
+    if (!sync(bn, 0)) {
+        bail();
+        return;
+    }
     se = lane == 0 ? bv[0] : T(0);  // column -pad above (used only if pad = 0)
     for (int c = 0; c < nf; c += 3) {
         if (!chunk(X, Y, Z, c) || c + 1 >= nf) break;
@@ -282,19 +354,26 @@ template <typename T, int F>
 __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__restrict__ du,
                                               const Skew &sk, const Link &lk, int band)
 {
-    const int lane = threadIdx.x;
+    __shared__ OutRing<T, F> ring;
+    const int lane = threadIdx.x & (kWave - 1);
     const int nx = sk.nx, pad = sk.pad(band);
     const int nf = sk.nst / F;
     const long L = sk.size();
     const bool has_above = band < sk.nb - 1, has_below = band > 0;
     constexpr int NW = TagWords<T>::n;
+    if (threadIdx.x >= kWave) {  // the storing wave; edge = lane 0, step q = column
+                                 // sc*F + F-1-q - pad of stored chunk sc = nf-1-c
+        store_chunks<T, F>(
+            wz, sk, band, nf, ring, lk.bnd + (long)(has_below ? band - 1 : 0) * nx * NW,
+            has_below, lk.gen, 0, true, [=](int c) { return nf - 1 - c; },
+            [=](int c, int q) { return (nf - 1 - c) * F + F - 1 - q - pad; });
+        return;
+    }
     constexpr int H = F / 2;
     constexpr int V = 16 / (int)sizeof(T);
     const unsigned lofs = (unsigned)lane * F;  // lane offset in a chunk block
     typedef Halves<T, 5, H> Buf;  // w, du1..du4
     const u64 *bin = lk.bnd + (long)(has_above ? band : 0) * nx * NW;
-    u64 *bout = lk.bnd + (long)(has_below ? band - 1 : 0) * nx * NW;
-    u64 *scratch = lk.scratch + ((long)band * kWave + lane) * 8;
 
     Buf X, Y, Z;
     Raw<T> bn{};
@@ -338,14 +417,20 @@ __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__res
         take(t, c);
         return ok;
     };
+    auto bail = [&]() -> bool {
+        if (lane == 0) ring.stop = 1;
+        handoff();
+        return false;
+    };
     auto chunk = [&](Buf &h0, Buf &h1, Buf &h2, int c) -> bool {
         load_bnd(bn, c + 1);
-        const int sc = nf - 1 - c;
         T outv[F];
 #pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
             Buf &cb = hh == 0 ? h0 : h1;
+            __builtin_amdgcn_sched_barrier(0);  // loads stay in place, as in the forward sweep
             load_half(hh == 0 ? h2 : h0, 2 * c + 2 + hh);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int t = 0; t < H; ++t) {
                 const int s = hh * H + t;
@@ -367,24 +452,19 @@ __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__res
                 cur = v;
             }
         }
-        if (!sync(bn, c + 1)) return false;
-        vstore<T, F>(wz + sk.chunk_base(band, sc, 0), lofs, outv);
-        {  // publish lane 0's results: step s = lane, column sc*F + F-1-lane - pad
-            T rev[F];
-#pragma unroll
-            for (int s = 0; s < F; ++s) rev[s] = outv[F - 1 - s];
-            const T pv = spread_from_lane<T, F>(rev, 0, lane);
-            const int col = sc * F + F - 1 - lane - pad;
-            const bool ok = has_below && lane < F && col >= 0 && col < nx;
-            put_tagged(ok ? bout + (long)col * NW : scratch, lk.gen, pv);
-        }
-        return true;
+        vstore<T, F>(ring.v[c & 1], lofs, outv);
+        handoff();
+        return sync(bn, c + 1) || bail();
     };
 
+    if (lane == 0) ring.stop = 0;
     load_half(X, 0);
     load_half(Y, 1);
     load_bnd(bn, 0);
-    if (!sync(bn, 0)) return;
+    if (!sync(bn, 0)) {
+        bail();
+        return;
+    }
     nw = lane == kWave - 1 ? bv[0] : T(0);  // column nx-1 of the row above
     for (int c = 0; c < nf; c += 3) {
         if (!chunk(X, Y, Z, c) || c + 1 >= nf) break;
@@ -537,7 +617,7 @@ __device__ __forceinline__ void factor_band(const T *__restrict__ a, T *__restri
 // (workgroups are dealt to the XCDs round robin) and shares its L2.
 // --------------------------------------------------------------------------
 template <typename T, int F>
-__global__ __launch_bounds__(kWave) void k_forward(const T *__restrict__ r,
+__global__ __launch_bounds__(2 * kWave) void k_forward(const T *__restrict__ r,
                                                    const T *__restrict__ dl, T *__restrict__ w,
                                                    Skew sk, Link lk0, int xcd_stride)
 {
@@ -550,7 +630,7 @@ __global__ __launch_bounds__(kWave) void k_forward(const T *__restrict__ r,
 }
 
 template <typename T, int F>
-__global__ __launch_bounds__(kWave) void k_backward(T *__restrict__ wz, const T *__restrict__ du,
+__global__ __launch_bounds__(2 * kWave) void k_backward(T *__restrict__ wz, const T *__restrict__ du,
                                                     Skew sk, Link lk0, int xcd_stride)
 {
     if (lk0.st->done) return;
@@ -594,7 +674,7 @@ __device__ __forceinline__ Link batch_link(const BatchSys<T> &d, int phase)
 }
 
 template <typename T, int F>
-__global__ __launch_bounds__(kWave) void k_forward_batch(const BatchSys<T> *__restrict__ sys,
+__global__ __launch_bounds__(2 * kWave) void k_forward_batch(const BatchSys<T> *__restrict__ sys,
                                                          const int2 *__restrict__ map)
 {
     const int2 m = map[blockIdx.x];
@@ -605,7 +685,7 @@ __global__ __launch_bounds__(kWave) void k_forward_batch(const BatchSys<T> *__re
 }
 
 template <typename T, int F>
-__global__ __launch_bounds__(kWave) void k_backward_batch(const BatchSys<T> *__restrict__ sys,
+__global__ __launch_bounds__(2 * kWave) void k_backward_batch(const BatchSys<T> *__restrict__ sys,
                                                           const int2 *__restrict__ map)
 {
     const int2 m = map[blockIdx.x];
@@ -797,14 +877,14 @@ template <typename T>
 void launch_forward(const T *r, const T *dl, T *w, const Skew &sk, Link lk, int xcd_stride,
                     hipStream_t s)
 {
-    hipLaunchKernelGGL((k_forward<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride), dim3(kWave), 0,
+    hipLaunchKernelGGL((k_forward<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride), dim3(2 * kWave), 0,
                        s, r, dl, w, sk, lk, xcd_stride);
 }
 
 template <typename T>
 void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_backward<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride), dim3(kWave),
+    hipLaunchKernelGGL((k_backward<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride), dim3(2 * kWave),
                        0, s, wz, du, sk, lk, xcd_stride);
 }
 
@@ -824,13 +904,13 @@ void launch_factor_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipS
 template <typename T>
 void launch_forward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_forward_batch<T, sweep_steps<T>()>), dim3(nblk), dim3(kWave), 0, s,
+    hipLaunchKernelGGL((k_forward_batch<T, sweep_steps<T>()>), dim3(nblk), dim3(2 * kWave), 0, s,
                        sys, map);
 }
 template <typename T>
 void launch_backward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_backward_batch<T, sweep_steps<T>()>), dim3(nblk), dim3(kWave), 0, s,
+    hipLaunchKernelGGL((k_backward_batch<T, sweep_steps<T>()>), dim3(nblk), dim3(2 * kWave), 0, s,
                        sys, map);
 }
 template <typename T>
