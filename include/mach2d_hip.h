@@ -130,6 +130,12 @@ int m2d_solve(m2d_system *s, const double *d_a, const double *d_rhs,
 
 int m2d_set_inner(m2d_system *s, double inner_gamma, int inner_nitm);
 
+/* ||A.var - rhs||_2 in FP64 (device pointers), the norm fb2d9 starts from
+ * (msi2d9-v03.f90:59-133); used to judge an initial guess against the zero
+ * guess before a solve. */
+int m2d_residual_norm(m2d_system *s, const double *d_a, const double *d_rhs,
+                      const double *d_var, double *out);
+
 /* Device copies of the factor (dl 5*nx*ny, du 4*nx*ny doubles, reference
  * layout); FP64 handles only. */
 int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du);
@@ -167,6 +173,13 @@ int m2d_batch_factor(m2d_batch *b, const double *const *d_a);
  * non-zero status. */
 int m2d_batch_solve(m2d_batch *b, const double *const *d_a, const double *const *d_rhs,
                     double *const *d_var, double gamma, int nitm, m2d_status *st);
+
+/* As m2d_batch_solve with one gamma per system (host array of count
+ * values): lets every system stop at its own threshold, e.g. a fixed
+ * residual level when systems start from different initial guesses. */
+int m2d_batch_solve_each(m2d_batch *b, const double *const *d_a, const double *const *d_rhs,
+                         double *const *d_var, const double *gammas, int nitm,
+                         m2d_status *st);
 
 /* FP32_IR: the inner stopping rule (as m2d_set_inner). */
 int m2d_batch_set_inner(m2d_batch *b, double inner_gamma, int inner_nitm);
@@ -333,6 +346,50 @@ void m2d_intflow_set_bcpl(const int *nx, const int *ny, const double *x, const d
 void m2d_intflow_set_bcT(const int *nx, const int *ny, const double *Twall, const double *x,
                          const double *xp, const double *T, const double *Tin, double *at,
                          double *bt);
+
+/* ---- 5. Learned initial guess: U-Net kernels (device pointers) ---------- */
+/* No reference counterpart (the reference starts MSI from the caller's var).
+ * BF16 NHWC tensors with channel counts that are multiples of 8 and 16-byte
+ * aligned; FP32 accumulation on the matrix cores.  Weights are bf16
+ * [cout rounded up to 16][kpad], k = tap*cin + ci (tap = 3*ky + kx), kpad =
+ * 9*cin rounded up to 32, zero beyond.  mode1: 0 direct, 1 the 2x2 average
+ * of a 2h x 2w tensor, 2 the nearest 2x upsampling of an h/2 x w/2 tensor;
+ * in2 (c2 channels, may be 0) is concatenated after in1.  stream: a
+ * hipStream_t or NULL.  All return M2D_OK or an error code. */
+int m2d_unet_conv3x3(const void *in1, int c1, int mode1, const void *in2, int c2,
+                     const void *w, const float *bias, int cout, int cout_stride, int kpad,
+                     int n, int h, int w_, int relu, void *out, float *out32, int accumulate,
+                     void *stream);
+/* dw[co][k] += sum over pixels of g[p][co] * im2col(p, k); db[co] += sum g. */
+int m2d_unet_conv3x3_wgrad(const void *in1, int c1, int mode1, const void *in2, int c2,
+                           const void *g, int cout, int kpad, int n, int h, int w_, float *dw,
+                           float *db, void *stream);
+/* g *= (y > 0) */
+int m2d_unet_relu_bwd(void *g, const void *y, long count, void *stream);
+/* dst32 += gradient of channels c0..c0+c-1 of gi (cs channels, h x w) carried
+ * back through input mode `mode` to the input tensor's own resolution. */
+int m2d_unet_mode_bwd(const float *gi, int cs, int c0, int c, int mode, int n, int h, int w,
+                      float *dst32, void *stream);
+int m2d_unet_to_bf16(const float *src, void *dst, long count, void *stream);
+/* Adam (bias-corrected, step >= 1) on FP32 parameters. */
+int m2d_unet_adam(float *w, float *m, float *v, const float *g, long count, float lr, float b1,
+                  float b2, float eps, int step, void *stream);
+/* bf16 forward weights wb and input-gradient weights wr [cin_pad][kpad_r]
+ * (taps rotated by 180 degrees, channels transposed) from FP32 w. */
+int m2d_unet_pack_weights(const float *w, int cout, int cin, int kpad, void *wb, void *wr,
+                          int kpad_r, int cin_pad, void *stream);
+/* 0.5*sum(mask*(pred - target)^2)*inv_count: per-256-pixel partial sums and
+ * the bf16 gradient (gs channels, value in channel 0). */
+int m2d_unet_mse(const float *pred, int ps, const float *target, const float *mask, long npix,
+                 float inv_count, void *grad, int gs, float *partial, void *stream);
+/* Network input of nb systems (a: 9-diagonal FP64 per system, b: rhs) on an
+ * h x w >= ny x nx tensor: b/(a_P s), a_W, a_E, a_S, a_N and the corner sum
+ * over a_P; inv_s = 1/s. */
+int m2d_unet_pack_input(const double *a, const double *b, int nx, int ny, int h, int w,
+                        double inv_s, void *x, int nb, void *stream);
+/* var = s * channel 0 of the FP32 network output (os channels, width w). */
+int m2d_unet_unpack_output(const float *out, int os, int nx, int ny, int w, double s,
+                           double *var, void *stream);
 
 /* TDMA on device pointers (tdma2d9 semantics). */
 int m2d_tdma2d9_device(int nx, int ny, int nitm, const double *d_a,

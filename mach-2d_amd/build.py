@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "mach2d", "libmach2d_hip.so")
-SOURCES = ["sweep.hip", "msi9.hip", "tdma9.hip", "capi.hip", "batch.hip", "assembly.hip"]
+SOURCES = ["sweep.hip", "msi9.hip", "tdma9.hip", "capi.hip", "batch.hip", "assembly.hip",
+           "unet.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -514,7 +514,7 @@ void fill(const m2d_batch *b, int q, const SolveState &s, int nit, int outer, fl
 }
 
 int set_io(m2d_batch *b, const double *const *d_a, const double *const *d_rhs,
-           double *const *d_var, double gamma, int nitm)
+           double *const *d_var, double gamma, int nitm, const double *gammas = nullptr)
 {
     if (b->stream == b->own) {  // inputs may come from the legacy default stream
         HIP_OK(hipEventRecord(b->ev_in, nullptr));
@@ -526,7 +526,7 @@ int set_io(m2d_batch *b, const double *const *d_a, const double *const *d_rhs,
         o.ra = d_a[q];
         if (d_rhs) o.rrhs = d_rhs[q];
         if (d_var) o.rx = d_var[q];
-        o.gamma = gamma;
+        o.gamma = gammas ? gammas[q] : gamma;
         o.nitm = nitm;
     }
     return upload_desc(b);
@@ -743,13 +743,32 @@ extern "C" int m2d_batch_factor(m2d_batch *b, const double *const *d_a)
     return worst;
 }
 
+static int batch_solve(m2d_batch *b, const double *const *d_a, const double *const *d_rhs,
+                       double *const *d_var, double gamma, const double *gammas, int nitm,
+                       m2d_status *st);
+
 extern "C" int m2d_batch_solve(m2d_batch *b, const double *const *d_a,
                                const double *const *d_rhs, double *const *d_var, double gamma,
                                int nitm, m2d_status *st)
 {
+    return batch_solve(b, d_a, d_rhs, d_var, gamma, nullptr, nitm, st);
+}
+
+extern "C" int m2d_batch_solve_each(m2d_batch *b, const double *const *d_a,
+                                    const double *const *d_rhs, double *const *d_var,
+                                    const double *gammas, int nitm, m2d_status *st)
+{
+    if (!gammas) return M2D_ERR_ARG;
+    return batch_solve(b, d_a, d_rhs, d_var, 0.0, gammas, nitm, st);
+}
+
+static int batch_solve(m2d_batch *b, const double *const *d_a, const double *const *d_rhs,
+                       double *const *d_var, double gamma, const double *gammas, int nitm,
+                       m2d_status *st)
+{
     if (!b || !d_a || !d_rhs || !d_var) return M2D_ERR_ARG;
     if (!b->factored) return M2D_ERR_ARG;
-    int rc = set_io(b, d_a, d_rhs, d_var, gamma, nitm);
+    int rc = set_io(b, d_a, d_rhs, d_var, gamma, nitm, gammas);
     if (rc) return rc;
     HIP_OK(hipEventRecord(b->ev0, b->stream));
     std::vector<int> inner_total(b->n, 0);

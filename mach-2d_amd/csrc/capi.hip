@@ -211,12 +211,12 @@ extern "C" int m2d_create(int nx, int ny, int precision, m2d_system **out)
     s->nbnd = boundary_words(nx, ny) + scratch_words(ny);
     const long N = s->N;
     bool ok = dalloc(&s->partial, s->np) == hipSuccess && dalloc(&s->bnd, s->nbnd) == hipSuccess &&
-              dalloc(&s->st, 1) == hipSuccess && dalloc(&s->st_in, 1) == hipSuccess;
+              dalloc(&s->st, 1) == hipSuccess && dalloc(&s->st_in, 1) == hipSuccess &&
+              dalloc(&s->r64, N) == hipSuccess;
     if (ok && precision == M2D_FP64) ok = s->w64.alloc(nx, ny);
     if (ok && precision == M2D_FP32_IR) {
         ok = s->w32.alloc(nx, ny) && dalloc(&s->a32, 9 * N) == hipSuccess &&
-             dalloc(&s->c32, N) == hipSuccess && dalloc(&s->rr32, N) == hipSuccess &&
-             dalloc(&s->r64, N) == hipSuccess;
+             dalloc(&s->c32, N) == hipSuccess && dalloc(&s->rr32, N) == hipSuccess;
     }
     ok = ok && hipHostMalloc((void **)&s->h_st, sizeof(SolveState), hipHostMallocDefault) ==
                    hipSuccess;
@@ -481,6 +481,21 @@ static int solve_fp32_ir(m2d_system *s, const double *d_a, const double *d_rhs, 
     s->h_st->nit = total_inner;
     fill_status(s, out, ms);
     return outer.status;
+}
+
+extern "C" int m2d_residual_norm(m2d_system *s, const double *d_a, const double *d_rhs,
+                                 const double *d_var, double *out)
+{
+    if (!s || !d_a || !d_rhs || !d_var || !out) return M2D_ERR_ARG;
+    launch_reset_state(s->st, s->stream);
+    launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r64, s->nx, s->ny, s->partial,
+                                            s->st, nullptr, s->stream);
+    launch_norm_finalize(s->partial, s->np, s->st, nullptr, 0, 1, 1.0, 1, s->stream);
+    HIP_OK(hipGetLastError());
+    const int rc = read_state(s, s->st);
+    if (rc) return rc;
+    *out = s->h_st->r0;
+    return M2D_OK;
 }
 
 extern "C" int m2d_solve(m2d_system *s, const double *d_a, const double *d_rhs, double *d_var,

@@ -47,10 +47,10 @@ EXPORTS = [
     "m2d_lu2d9", "m2d_fb2d9", "m2d_tdma2d9", "m2d_solve_msi9d", "m2d_norm_l1_9d",
     "m2d_norm_l1_9d_relative", "m2d_norm_l1_b", "m2d_last_status", "m2d_create",
     "m2d_destroy", "m2d_set_stream", "m2d_factor", "m2d_solve", "m2d_set_inner",
-    "m2d_get_factor", "m2d_set_tuning", "m2d_tdma2d9_device",
+    "m2d_get_factor", "m2d_set_tuning", "m2d_tdma2d9_device", "m2d_residual_norm",
     "m2d_version", "m2d_enable_kernel_timing", "m2d_kernel_timing",
     "m2d_batch_create", "m2d_batch_destroy", "m2d_batch_set_stream", "m2d_batch_factor",
-    "m2d_batch_solve", "m2d_batch_set_inner", "m2d_batch_set_tuning", "m2d_batch_get_tuning",
+    "m2d_batch_solve", "m2d_batch_solve_each", "m2d_batch_set_inner", "m2d_batch_set_tuning", "m2d_batch_get_tuning",
     "m2d_batch_tune", "m2d_assembly_set_stream", "m2d_get_density_at_nodes",
     "m2d_get_density_at_faces",
     "m2d_get_u_coefficients", "m2d_get_u_source", "m2d_get_v_coefficients", "m2d_get_v_source",
@@ -60,6 +60,9 @@ EXPORTS = [
     "m2d_get_u_v_at_real_nodes_with_pl", "m2d_get_velocity_correction_at_faces_with_pl",
     "m2d_initialize_boundary_faces_flags", "m2d_check_mass_conservation",
     "m2d_intflow_set_bcu", "m2d_intflow_set_bcv", "m2d_intflow_set_bcpl", "m2d_intflow_set_bcT",
+    "m2d_unet_conv3x3", "m2d_unet_conv3x3_wgrad", "m2d_unet_relu_bwd", "m2d_unet_mode_bwd",
+    "m2d_unet_to_bf16", "m2d_unet_adam", "m2d_unet_pack_weights", "m2d_unet_mse",
+    "m2d_unet_pack_input", "m2d_unet_unpack_output",
 ]
 
 _lib = None
@@ -107,6 +110,8 @@ def load():
     L.m2d_solve.restype = C.c_int
     L.m2d_set_inner.argtypes = [_vp, C.c_double, C.c_int]
     L.m2d_set_inner.restype = C.c_int
+    L.m2d_residual_norm.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double)]
+    L.m2d_residual_norm.restype = C.c_int
     L.m2d_get_factor.argtypes = [_vp, _vp, _vp]
     L.m2d_get_factor.restype = C.c_int
     L.m2d_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
@@ -117,12 +122,15 @@ def load():
     L.m2d_batch_set_stream.argtypes = [_vp, _vp]
     L.m2d_batch_factor.argtypes = [_vp, _pp]
     L.m2d_batch_solve.argtypes = [_vp, _pp, _pp, _pp, C.c_double, C.c_int, C.POINTER(Status)]
+    L.m2d_batch_solve_each.argtypes = [_vp, _pp, _pp, _pp, C.POINTER(C.c_double), C.c_int,
+                                       C.POINTER(Status)]
     L.m2d_batch_set_inner.argtypes = [_vp, C.c_double, C.c_int]
     L.m2d_batch_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
     L.m2d_batch_get_tuning.argtypes = [_vp, _ip, _ip]
     L.m2d_batch_tune.argtypes = [_vp, _pp, _pp, C.c_int, _ip, _dp]
     for f in (L.m2d_batch_create, L.m2d_batch_destroy, L.m2d_batch_set_stream,
-              L.m2d_batch_factor, L.m2d_batch_solve, L.m2d_batch_set_inner,
+              L.m2d_batch_factor, L.m2d_batch_solve, L.m2d_batch_solve_each,
+              L.m2d_batch_set_inner,
               L.m2d_batch_set_tuning, L.m2d_batch_get_tuning, L.m2d_batch_tune):
         f.restype = C.c_int
     L.m2d_assembly_set_stream.argtypes = [_vp]

@@ -230,26 +230,41 @@ class System:
         return self.nx * self.ny
 
 
+def rhs_field(nx: int, ny: int, kind: str, rng) -> np.ndarray:
+    """Right-hand side on interior rows (fictitious rows 0), flat: 'uniform'
+    on [-1,1], 'gauss' N(0,1), or 'smooth' (six random sine modes plus
+    0.1 N(0,1), a mass-imbalance-like field)."""
+    b = np.zeros((ny, nx))
+    if kind == "uniform":
+        b[1:-1, 1:-1] = rng.uniform(-1.0, 1.0, size=(ny - 2, nx - 2))
+    elif kind == "gauss":
+        b[1:-1, 1:-1] = rng.standard_normal(size=(ny - 2, nx - 2))
+    elif kind == "smooth":
+        yy, xx = np.mgrid[0:ny - 2, 0:nx - 2]
+        f = np.zeros((ny - 2, nx - 2))
+        for _ in range(6):
+            kx, ky = rng.uniform(0.5, 6.0), rng.uniform(0.5, 4.0)
+            p1, p2 = rng.uniform(0.0, 2.0 * np.pi, size=2)
+            f += rng.standard_normal() * (np.sin(np.pi * kx * xx / (nx - 2) + p1) *
+                                          np.sin(np.pi * ky * yy / (ny - 2) + p2))
+        b[1:-1, 1:-1] = f + 0.1 * rng.standard_normal(size=f.shape)
+    else:
+        raise ValueError(kind)
+    return b.reshape(-1)
+
+
 def pcorr_system(nx: int, ny: int, seed: int = 0, q: float | None = None,
                  aspect: float | None = None, rhs: str = "uniform",
                  keep_metrics: bool = False) -> System:
     """A seeded 9-diagonal p' system on the ogive grid.  rhs is drawn on
-    interior rows only ('uniform' on [-1,1] or 'gauss' N(0,1)); fictitious
-    rows get 0."""
+    interior rows only (rhs_field kinds); fictitious rows get 0."""
     if q is None:
         q = ratio_for_aspect(nx, ny, aspect) if aspect else 1.05
     g = ogive_grid(nx, ny, q)
     m = metrics(g)
     a = diffusion_coefficients(nx, ny, m)
-    rng = np.random.default_rng(seed)
-    b = np.zeros((ny, nx))
-    if rhs == "uniform":
-        b[1:-1, 1:-1] = rng.uniform(-1.0, 1.0, size=(ny - 2, nx - 2))
-    elif rhs == "gauss":
-        b[1:-1, 1:-1] = rng.standard_normal(size=(ny - 2, nx - 2))
-    else:
-        raise ValueError(rhs)
-    return System(nx, ny, np.ascontiguousarray(a), b.reshape(-1).copy(), q, seed,
+    b = rhs_field(nx, ny, rhs, np.random.default_rng(seed))
+    return System(nx, ny, np.ascontiguousarray(a), b, q, seed,
                   m if keep_metrics else None)
 
 

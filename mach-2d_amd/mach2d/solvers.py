@@ -331,6 +331,17 @@ class DeviceMSI:
         if rc:
             raise SolverError(rc, "m2d_get_factor")
 
+    def residual_norm(self, a, rhs, var) -> float:
+        """||A var - rhs||_2 in FP64."""
+        out = C.c_double()
+        rc = _lib.load().m2d_residual_norm(self.h, C.c_void_p(self._chk(a, 9 * self.N, "a")),
+                                           C.c_void_p(self._chk(rhs, self.N, "rhs")),
+                                           C.c_void_p(self._chk(var, self.N, "var")),
+                                           C.byref(out))
+        if rc:
+            raise SolverError(rc, "m2d_residual_norm")
+        return out.value
+
     def solve(self, a, rhs, var, gamma, nitm, hist=None) -> Status:
         st = Status()
         hp, cap = (None, 0)
@@ -396,10 +407,18 @@ class BatchMSI:
         return rc
 
     def solve(self, a, rhs, var, gamma, nitm):
-        """Returns (first non-zero status, [Status per system])."""
+        """Returns (first non-zero status, [Status per system]); gamma is one
+        value for every system or a sequence of one per system."""
         st = (Status * self.n)()
-        rc = _lib.load().m2d_batch_solve(self.h, self._ptrs(a, 9, "a"), self._ptrs(rhs, 1, "rhs"),
-                                         self._ptrs(var, 1, "var"), float(gamma), int(nitm), st)
+        if np.ndim(gamma):
+            g = (C.c_double * self.n)(*[float(x) for x in gamma])
+            rc = _lib.load().m2d_batch_solve_each(self.h, self._ptrs(a, 9, "a"),
+                                                  self._ptrs(rhs, 1, "rhs"),
+                                                  self._ptrs(var, 1, "var"), g, int(nitm), st)
+        else:
+            rc = _lib.load().m2d_batch_solve(self.h, self._ptrs(a, 9, "a"),
+                                             self._ptrs(rhs, 1, "rhs"), self._ptrs(var, 1, "var"),
+                                             float(gamma), int(nitm), st)
         return rc, list(st)
 
     def set_inner(self, inner_gamma=1e-4, inner_nitm=200):
