@@ -114,6 +114,60 @@ def gpu_run(s, mode, steps, warmup):
     return dict(dt=dt, st=st, ok=ok, kt=kt, x=x)
 
 
+def learned_run(s, steps, warmup, x64):
+    """FP32-refined MSI from the U-Net guess (inference, residual-minimising
+    amplitude, zero-guess fallback) to the zero guess's threshold; timed
+    including the inference."""
+    import torch
+
+    from mach2d import learned as Lr
+    from mach2d import solvers as S
+    from mach2d import unet as U
+    path = Lr.default_weights()
+    if not os.path.exists(path):
+        log("learned: no weights file %s" % path)
+        return None
+    net = U.UNet()
+    net.load(path)
+    dev = S.DeviceMSI(s.nx, s.ny, precision="fp32_ir")
+    ta = torch.tensor(s.a, device="cuda")
+    tr = torch.tensor(s.rhs, device="cuda")
+    tv = torch.zeros(s.N, dtype=torch.float64, device="cuda")
+
+    def one():
+        tv.zero_()
+        dev.factor(ta)
+        return Lr.guessed_solve(dev, net, ta, tr, tv, s.nx, s.ny, GAMMA, NITM_CAP)
+
+    for _ in range(warmup):
+        one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = [one() for _ in range(steps)]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st, used, rg, rb = res[-1]
+    ok = all(r[0].status == 0 and r[0].rn < GAMMA * r[3] for r in res)
+    x = tv.cpu().numpy()
+    rel = float(np.linalg.norm(x - x64) / np.linalg.norm(x64)) if x64 is not None else None
+    # inference alone
+    t1 = time.perf_counter()
+    for _ in range(3):
+        net.guess(ta, tr, s.nx, s.ny, tv)
+    torch.cuda.synchronize()
+    t_inf = (time.perf_counter() - t1) / 3
+    dev.close()
+    e = {"solves_per_s": round(1.0 / dt, 4), "ms_per_solve": round(dt * 1e3, 3), "nit": st.nit,
+         "outer": st.outer, "guess_used": used, "r_guess_over_r_zero": rg / rb,
+         "inference_ms": round(t_inf * 1e3, 3),
+         "inference_tflops": round(43.5e-3 / t_inf, 2),
+         "correct": bool(ok and (rel is None or rel <= 1e-8))}
+    if rel is not None:
+        e["rel_error_vs_fp64"] = rel
+    log("learned fp32_ir: %s" % e)
+    return e
+
+
 def cpu_baseline(s, nit, budget_s=15.0):
     """Reference lu2d9 + fb2d9 (compiled Fortran, one core) on a bounded
     sample: the factorization plus M iterations; solves/s extrapolated to
@@ -168,7 +222,7 @@ def roofline(kt, N, traffic_known=True):
             "avg_launch_us": round(avg_ms * 1e3, 2), "bytes_per_launch": per_launch}
 
 
-def batch_run(mode, steps, warmup, count=256, same=None):
+def batch_run(mode, steps, warmup, count=256, same=None, learned=False):
     """256 independent p' systems (own grid size, stretching and RHS) solved
     together to the 1e-10 drop; with `same` (a system), `count` systems with
     its matrix and their own seeded Gaussian right-hand sides."""
@@ -204,11 +258,37 @@ def batch_run(mode, steps, warmup, count=256, same=None):
     log("batch %s: layout %d (%s), ms/iteration per layout %s" % (
         mode, layout, S.BatchMSI.LAYOUTS[layout], [round(v, 3) for v in ms]))
 
+    net = helper = None
+    if learned:
+        from mach2d import learned as Lr
+        from mach2d import unet as U
+        if not os.path.exists(Lr.default_weights()):
+            b.close()
+            return None
+        net = U.UNet()
+        net.load(Lr.default_weights())
+        helper = S.DeviceMSI(sizes[0][0], sizes[0][1], precision="fp64")
+    used = [False] * count
+
     def one():
         for v in var:
             v.zero_()
         b.factor(a)
-        return b.solve(a, rhs, var, GAMMA, NITM_CAP)
+        if net is None:
+            return b.solve(a, rhs, var, GAMMA, NITM_CAP)
+        gam = []
+        for q in range(count):   # guess, residual-minimising amplitude, fallback to zero
+            nx, ny = sizes[q]
+            rb = helper.residual_norm(a[q], rhs[q], var[q])
+            net.guess(a[q], rhs[q], nx, ny, var[q])
+            helper.guess_scale(a[q], rhs[q], var[q])
+            rg = helper.residual_norm(a[q], rhs[q], var[q])
+            used[q] = bool(np.isfinite(rg) and rg < rb)
+            if not used[q]:
+                var[q].zero_()
+                rg = rb
+            gam.append(GAMMA * rb / rg)
+        return b.solve(a, rhs, var, gam, NITM_CAP)
 
     for _ in range(warmup):
         one()
@@ -221,9 +301,32 @@ def batch_run(mode, steps, warmup, count=256, same=None):
     ok = rc == 0 and all(x.status == 0 and x.rn < x.gammarp for x in st)
     nbytes = sum(solve_bytes(mode, x.N, y) for x, y in zip(systems, st))
     b.close()
-    return {"mode": mode, "systems": count, "solves_per_s": round(count / dt, 2),
-            "ms_per_batch": round(dt * 1e3, 3), "gbps": round(nbytes / dt / 1e9, 2),
-            "iterations_max": max(y.nit for y in st), "layout": layout, "correct": bool(ok)}
+    if helper is not None:
+        helper.close()
+    out = {"mode": mode, "systems": count, "solves_per_s": round(count / dt, 2),
+           "ms_per_batch": round(dt * 1e3, 3), "gbps": round(nbytes / dt / 1e9, 2),
+           "iterations_max": max(y.nit for y in st), "layout": layout, "correct": bool(ok)}
+    if learned:
+        out["guesses_used"] = int(sum(used))
+    return out
+
+
+def target_comparison(summary, batch3):
+    """The 1026x514 solve with learned guess + FP32/FP64 refinement + batching
+    against plain FP64 MSI (one system at a time), in solves/s."""
+    if not batch3 or "fp64" not in summary:
+        return None
+    plain = summary["fp64"]["solves_per_s"]
+    best = None
+    for k in ("learned_fp32_ir", "fp32_ir"):
+        v = batch3.get(k)
+        if v and v.get("correct"):
+            best = (k, v["solves_per_s"])
+            break
+    if best is None:
+        return None
+    return {"plain_fp64_solves_per_s": plain, "path": "batch_1026x514." + best[0],
+            "solves_per_s": best[1], "speedup": round(best[1] / plain, 2), "target": 20.0}
 
 
 def main():
@@ -257,6 +360,10 @@ def main():
             e["rel_error_vs_fp64"] = float(np.linalg.norm(r["x"] - x64) / np.linalg.norm(x64))
             e["correct"] = bool(r["ok"] and e["rel_error_vs_fp64"] <= 1e-8)
         summary[m] = e
+    if args.mode == "best":
+        lr = learned_run(s, args.steps, args.warmup, x64)
+        if lr is not None:
+            summary["learned_fp32_ir"] = lr
     head = "fp32_ir" if "fp32_ir" in summary and summary["fp32_ir"]["correct"] else modes[0]
     r, e = runs[head], summary[head]
     st = r["st"]
@@ -273,6 +380,9 @@ def main():
             log("batch %s: %s" % (m, v))
         batch3 = {m: batch_run(m, 1, 1, count=args.batch_cfg3, same=s)
                   for m in ("fp64", "fp32_ir")}
+        lb = batch_run("fp32_ir", 1, 1, count=args.batch_cfg3, same=s, learned=True)
+        if lb is not None:
+            batch3["learned_fp32_ir"] = lb
         for m, v in batch3.items():
             log("batch 1026x514 %s: %s" % (m, v))
             if "fp64" in summary:
@@ -304,6 +414,7 @@ def main():
         "cpu_baseline": cpu,
         "batch": batch,
         "batch_1026x514": batch3,
+        "target_comparison": target_comparison(summary, batch3),
     }
     print(json.dumps(out), flush=True)
 

@@ -136,6 +136,13 @@ int m2d_set_inner(m2d_system *s, double inner_gamma, int inner_nitm);
 int m2d_residual_norm(m2d_system *s, const double *d_a, const double *d_rhs,
                       const double *d_var, double *out);
 
+/* Scale an initial guess g in place by the alpha that minimises
+ * ||A (alpha g) - rhs||_2, alpha = (A g . rhs) / (A g . A g) (0 when A g = 0),
+ * so the scaled guess is never worse than the zero guess.  No reference
+ * counterpart. */
+int m2d_guess_scale(m2d_system *s, const double *d_a, const double *d_rhs, double *d_g,
+                    double *alpha_out);
+
 /* Device copies of the factor (dl 5*nx*ny, du 4*nx*ny doubles, reference
  * layout); FP64 handles only. */
 int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du);
@@ -145,6 +152,11 @@ int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du);
  * xcd_stride = 1 (bands on consecutive XCDs) or 8 (bands of one system on
  * one XCD). */
 int m2d_set_tuning(m2d_system *s, int publish_every, int xcd_stride);
+
+/* Sweep operand path: 1 = the LDS-staged sweeps (a second wave loads each
+ * chunk's operands into an LDS ring ahead of use), 0 (default) = operands
+ * loaded into registers by the computing wave.  Same results either way. */
+int m2d_set_staging(m2d_system *s, int staged);
 
 /* ---- 3. Batched API (device pointers) ---------------------------------- */
 /* Many independent systems of uneven grid size solved together: one launch
@@ -186,13 +198,16 @@ int m2d_batch_set_inner(m2d_batch *b, double inner_gamma, int inner_nitm);
 
 /* Launch placement: layout 0 = systems in order, 1 = each system's bands on
  * one XCD (longest-processing-time assignment), 2 = largest systems first;
- * chunk_iters = iterations per graph replay (1..64). */
+ * layouts 3..5 = the same placements with the LDS-staged sweeps (operands
+ * loaded into an LDS ring ahead of use by a second wave; more LDS per
+ * workgroup, deeper prefetch); chunk_iters = iterations per graph replay
+ * (1..64). */
 int m2d_batch_set_tuning(m2d_batch *b, int layout, int chunk_iters);
 int m2d_batch_get_tuning(m2d_batch *b, int *layout, int *chunk_iters);
 
 /* Autotuner: time `iters` iterations of each layout on these systems (from
  * a zero iterate in scratch memory; the caller's var is not touched) and
- * keep the fastest.  ms_out (3 entries, may be NULL): time per iteration. */
+ * keep the fastest.  ms_out (6 entries, may be NULL): time per iteration. */
 int m2d_batch_tune(m2d_batch *b, const double *const *d_a, const double *const *d_rhs,
                    int iters, int *layout_out, double *ms_out);
 

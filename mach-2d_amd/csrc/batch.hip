@@ -268,7 +268,7 @@ struct m2d_batch {
     u64 *words = nullptr;
     long nwords = 0;
     Maps maps;
-    int layout = 0;
+    int layout = 0;  // placement (layout % 3), LDS-staged sweeps (layout >= 3)
     int chunk_iters = 8;
     Chunk graph;
     double inner_gamma = 1e-4;
@@ -353,7 +353,7 @@ void drop_graph(m2d_batch *b)
 int build_maps(m2d_batch *b, int layout)
 {
     std::vector<int2> fwd, bwd, pts, skw;
-    sweep_maps(b, layout, fwd, bwd);
+    sweep_maps(b, layout % 3, fwd, bwd);
     for (int q = 0; q < b->n; ++q) {
         for (long k = 0; k < blocks_of(b->N[q], kPts); ++k) pts.push_back(make_int2(q, (int)k));
         for (long k = 0; k < blocks_of(b->sk[q].size(), 256); ++k)
@@ -424,8 +424,8 @@ void enqueue_iteration(m2d_batch *b)
 {
     const BatchSys<T> *d = dsys<T>(b);
     const Maps &m = b->maps;
-    launch_forward_batch<T>(d, m.fwd, m.nf, b->stream);
-    launch_backward_batch<T>(d, m.bwd, m.nb, b->stream);
+    launch_forward_batch<T>(d, m.fwd, m.nf, b->stream, b->layout >= 3);
+    launch_backward_batch<T>(d, m.bwd, m.nb, b->stream, b->layout >= 3);
     hipLaunchKernelGGL(k_update_batch<T>, dim3(m.np), dim3(kPts), 0, b->stream, d, m.pts);
     hipLaunchKernelGGL(k_residual_batch<T>, dim3(m.np), dim3(kPts), 0, b->stream, d, m.pts, 1);
     hipLaunchKernelGGL(k_finalize_batch<T>, dim3(b->n), dim3(256), 0, b->stream, d, 0);
@@ -692,7 +692,7 @@ extern "C" int m2d_batch_set_inner(m2d_batch *b, double inner_gamma, int inner_n
 
 extern "C" int m2d_batch_set_tuning(m2d_batch *b, int layout, int chunk_iters)
 {
-    if (!b || layout < 0 || layout > 2 || chunk_iters < 1 || chunk_iters > 64) return M2D_ERR_ARG;
+    if (!b || layout < 0 || layout > 5 || chunk_iters < 1 || chunk_iters > 64) return M2D_ERR_ARG;
     b->chunk_iters = chunk_iters;
     if (layout != b->layout) return build_maps(b, layout);
     return M2D_OK;
@@ -845,7 +845,7 @@ extern "C" int m2d_batch_tune(m2d_batch *b, const double *const *d_a, const doub
     const int keep_chunk = b->chunk_iters;
     double best = 1e300;
     int best_layout = b->layout;
-    for (int layout = 0; layout < 3 && !rc; ++layout) {
+    for (int layout = 0; layout < 6 && !rc; ++layout) {
         rc = build_maps(b, layout);
         double per = 0.0;
         for (int pass = 0; pass < 2 && !rc; ++pass) {  // pass 0 warms up

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "m2d_internal.h"
+#include "msi_stencil.h"
 #include "skew.h"
 #include "wavefront.h"
 
@@ -72,6 +73,10 @@ struct m2d_system {
     int precision = M2D_FP64;
     hipStream_t stream = nullptr;
     int xcd_stride = 1;
+    // sweep operands through the LDS ring (m2d_set_staging); off by default:
+    // a single system's working set sits in the last-level cache, where the
+    // register ring's prefetch suffices and the ring's extra LDS pass costs
+    bool staged = false;
     int chunk = 8;
     double inner_gamma = 1e-4;
     int inner_nitm = 200;
@@ -245,6 +250,13 @@ extern "C" int m2d_set_tuning(m2d_system *s, int publish_every, int xcd_stride)
     return M2D_OK;
 }
 
+extern "C" int m2d_set_staging(m2d_system *s, int staged)
+{
+    if (!s || (staged != 0 && staged != 1)) return M2D_ERR_ARG;
+    s->staged = staged != 0;
+    return M2D_OK;
+}
+
 extern "C" int m2d_set_inner(m2d_system *s, double inner_gamma, int inner_nitm)
 {
     if (!s || !(inner_gamma > 0.0) || inner_nitm < 1) return M2D_ERR_ARG;
@@ -341,10 +353,12 @@ static void enqueue_iteration(m2d_system *s, Work<T> &W, SolveState *st, const T
 {
     const int off = sizeof(T) == 4 ? KT_FP32 : 0;
     timed(s, KT_FORWARD + off, k, [&] {
-        launch_forward<T>(W.r, W.dl, W.w, W.sk, make_link(s, st, 1), s->xcd_stride, s->stream);
+        launch_forward<T>(W.r, W.dl, W.w, W.sk, make_link(s, st, 1), s->xcd_stride, s->stream,
+                          s->staged);
     });
     timed(s, KT_BACKWARD + off, k, [&] {
-        launch_backward<T>(W.w, W.du, W.sk, make_link(s, st, 2), s->xcd_stride, s->stream);
+        launch_backward<T>(W.w, W.du, W.sk, make_link(s, st, 2), s->xcd_stride, s->stream,
+                           s->staged);
     });
     timed(s, KT_RESIDUAL + off, k, [&] {
         launch_update<T>(x, W.w, W.sk, st, s->stream);
@@ -495,6 +509,69 @@ extern "C" int m2d_residual_norm(m2d_system *s, const double *d_a, const double 
     const int rc = read_state(s, s->st);
     if (rc) return rc;
     *out = s->h_st->r0;
+    return M2D_OK;
+}
+
+// Per-block partial sums of (A g).b and (A g).(A g), blocks of 256 points in
+// index order (the host adds them in block order).
+__global__ __launch_bounds__(256) void k_guess_dots(const double *__restrict__ a,
+                                                    const double *__restrict__ g,
+                                                    const double *__restrict__ b, int nx, int ny,
+                                                    double *__restrict__ part)
+{
+    const long N = (long)nx * ny;
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    double ab = 0.0, aa = 0.0;
+    if (p < N) {
+        const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
+        const double v = residual_point<double, double, double>(a, N, g, 0.0, i, j, nx, ny);
+        ab = v * b[p];
+        aa = v * v;
+    }
+    __shared__ double sab[256], saa[256];
+    sab[threadIdx.x] = ab;
+    saa[threadIdx.x] = aa;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sab[threadIdx.x] += sab[threadIdx.x + w];
+            saa[threadIdx.x] += saa[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sab[0];
+        part[2 * blockIdx.x + 1] = saa[0];
+    }
+}
+
+__global__ void k_scale(double *__restrict__ g, long n, double alpha)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) g[q] = alpha * g[q];
+}
+
+extern "C" int m2d_guess_scale(m2d_system *s, const double *d_a, const double *d_rhs,
+                               double *d_g, double *alpha_out)
+{
+    if (!s || !d_a || !d_rhs || !d_g) return M2D_ERR_ARG;
+    const long N = s->N;
+    const unsigned nblk = (unsigned)((N + 255) / 256);
+    hipLaunchKernelGGL(k_guess_dots, dim3(nblk), dim3(256), 0, s->stream, d_a, d_g, d_rhs, s->nx,
+                       s->ny, s->r64);
+    std::vector<double> part(2 * (size_t)nblk);
+    HIP_OK(hipMemcpyAsync(part.data(), s->r64, sizeof(double) * part.size(),
+                          hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    double ab = 0.0, aa = 0.0;
+    for (unsigned k = 0; k < nblk; ++k) {
+        ab += part[2 * k];
+        aa += part[2 * k + 1];
+    }
+    const double alpha = aa > 0.0 && isfinite(ab) && isfinite(aa) ? ab / aa : 0.0;
+    hipLaunchKernelGGL(k_scale, dim3(nblk), dim3(256), 0, s->stream, d_g, N, alpha);
+    HIP_OK(hipGetLastError());
+    if (alpha_out) *alpha_out = alpha;
     return M2D_OK;
 }
 

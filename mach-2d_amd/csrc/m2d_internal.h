@@ -44,9 +44,10 @@ void launch_factor(const T *a_sk, T *dl, T *du, T alpha0, const Skew &sk, Link l
                    int xcd_stride, hipStream_t s);
 template <typename T>
 void launch_forward(const T *r, const T *dl, T *w, const Skew &sk, Link lk, int xcd_stride,
-                    hipStream_t s);
+                    hipStream_t s, bool staged);
 template <typename T>
-void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s);
+void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s,
+                     bool staged);
 template <typename TI, typename TO>
 void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t s);
 // dl: 6 arrays (dl1..dl5, then 1/dl1, written here); padding made finite.
@@ -63,9 +64,11 @@ struct BatchSys;
 template <typename T>
 void launch_factor_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s);
 template <typename T>
-void launch_forward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s);
+void launch_forward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s,
+                          bool staged);
 template <typename T>
-void launch_backward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s);
+void launch_backward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s,
+                           bool staged);
 template <typename T>
 void launch_to_skew_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s);
 template <typename T>

@@ -171,10 +171,12 @@ __device__ __forceinline__ void handoff()
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-template <typename T, int F>
-__device__ __forceinline__ bool ring_stopped(OutRing<T, F> &ring)
+// A plain LDS read after the barrier (handoff's fences keep it there); an
+// atomic read would make the compiler wait for every load in flight first.
+template <typename Ring>
+__device__ __forceinline__ bool ring_stopped(Ring &ring)
 {
-    return __hip_atomic_load(&ring.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+    return ring.stop != 0;
 }
 
 // The storing wave of a sweep: for chunk c = 0.. (after barrier c) writes
@@ -474,6 +476,334 @@ __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__res
 }
 
 // --------------------------------------------------------------------------
+// LDS-staged sweeps.  As above, wave 0 computes and wave 1 moves data, but
+// wave 1 now also loads: it copies every chunk's inputs from HBM straight
+// into an LDS ring of kStageSlots chunk slots (direct-to-LDS loads, no
+// registers), kStageSlots - 2 chunks ahead of the computing wave, which reads
+// its operands from LDS.  The computing wave then issues no global data
+// access at all except the boundary words.  Loads of a chunk are complete
+// before the barrier that hands the chunk over: the moving wave waits on its
+// own vector-memory counter, whose count of later operations is fixed
+// because every iteration issues the same number of stores and loads (loads
+// past the last chunk re-read the last chunk into the free slot; stores of
+// lanes that publish nothing go to a private scratch word).
+//
+// Barrier protocol (k = 0 .. nf): after barrier k the computing wave works
+// on chunk k, and chunks k and k+1 are complete in LDS; before barrier k+1 it
+// has written chunk k's results into out-slot k&1.  Between barriers k and
+// k+1 the moving wave stores chunk k-1, loads chunk k+D-1 into the slot
+// chunk k-1 used, and waits for chunk k+2.
+// --------------------------------------------------------------------------
+constexpr int kStageSlots = 4;
+
+template <typename T, int F, int NA>
+struct StagedRing {
+    static constexpr int NQ = F * (int)sizeof(T) / 16;  // 16-byte vectors per lane and array
+    uint4 in[kStageSlots][NA * NQ * kWave];            // [slot][(a * NQ + q) * 64 + lane]
+    T out[2][kWave * F];
+    int stop;
+};
+
+typedef __attribute__((address_space(1))) const void global_cvoid;
+typedef __attribute__((address_space(3))) void local_void;
+
+// s_waitcnt vmcnt(N) alone (expcnt and lgkmcnt left alone).
+template <int N>
+__device__ __forceinline__ void wait_vmem()
+{
+    static_assert(N >= 0 && N <= 63, "vmcnt is six bits");
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// The moving wave of a staged sweep.  base(a, c) is the wave-uniform start of
+// array a's block of processing chunk c; stored(c) the stored chunk index
+// of processing chunk c (for the output); the edge and publication
+// arguments are those of store_chunks.
+template <typename T, int F, int NA, typename FBASE, typename FSTORED, typename FCOL>
+__device__ void move_chunks(StagedRing<T, F, NA> &R, T *__restrict__ dst, const Skew &sk, int band,
+                            int nf, u64 *bout, bool has_out, u64 *scratch, unsigned gen, int edge,
+                            bool reversed, FBASE base, FSTORED stored, FCOL col_of)
+{
+    constexpr int D = kStageSlots, NQ = StagedRing<T, F, NA>::NQ;
+    constexpr int V = 16 / (int)sizeof(T);
+    constexpr int NW = TagWords<T>::n;
+    constexpr int LPC = NA * NQ;  // loads per chunk
+    constexpr int SPC = NQ + NW;  // stores per chunk
+    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned lofs = (unsigned)lane * F;
+    auto stage = [&](int c) {  // all NA*NQ vectors of chunk min(c, nf-1) into slot c % D
+        const int cc = min(c, nf - 1);
+        uint4 *slot = R.in[c % D];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            const T *g = base(a, cc);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+                __builtin_amdgcn_global_load_lds((global_cvoid *)(g + (lofs + q * V)),
+                                                 (local_void *)(slot + (a * NQ + q) * kWave), 16,
+                                                 0, 0);
+        }
+    };
+    auto store = [&](int c) {  // chunk c's results, from out-slot c & 1
+        const T *slot = R.out[c & 1];
+        T vals[F];
+        vload<T, F>(vals, slot + lofs);
+        vstore<T, F>(dst + sk.chunk_base(band, stored(c), 0), lofs, vals);
+        const int q = lane < F ? lane : 0;
+        const T pv = slot[edge * F + (reversed ? F - 1 - q : q)];
+        const int col = col_of(c, lane);
+        const bool ok = has_out && lane < F && col >= 0 && col < sk.nx;
+        put_tagged(ok ? bout + (long)col * NW : scratch, gen, pv);
+    };
+    for (int c = 0; c < D - 1; ++c) stage(c);
+    wait_vmem<(D - 3) * LPC>();  // chunks 0 and 1 complete
+    handoff();
+    if (ring_stopped(R)) return;
+    for (int k = 0; k < nf; ++k) {
+        if (k >= 1) store(k - 1);
+        stage(k + D - 1);
+        if (k == 0)
+            wait_vmem<(D - 3) * LPC>();  // chunk 2 (from the prologue) complete
+        else
+            wait_vmem<(D - 3) * (LPC + SPC)>();  // chunk k+2 complete
+        handoff();
+        if (ring_stopped(R)) return;
+    }
+    store(nf - 1);
+}
+
+template <typename T, int F, int NA, int H>
+__device__ __forceinline__ void read_half(Halves<T, NA, H> &b, const StagedRing<T, F, NA> &R,
+                                          int c, int qh0, int lane)
+{
+    constexpr int NQ = StagedRing<T, F, NA>::NQ, V = 16 / (int)sizeof(T);
+    const uint4 *slot = R.in[c % kStageSlots];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int q = 0; q < H / V; ++q) {
+            const uint4 v = slot[(a * NQ + qh0 + q) * kWave + lane];
+            __builtin_memcpy(&b.v[a][q * V], &v, 16);
+        }
+}
+
+// forward sweep, staged (same arithmetic and results as forward_band).
+template <typename T, int F>
+__device__ __forceinline__ void forward_band_staged(const T *__restrict__ r,
+                                                    const T *__restrict__ dl, T *__restrict__ w,
+                                                    const Skew &sk, const Link &lk, int band)
+{
+    constexpr bool kScaled = ScaledRows<T>::value;
+    constexpr int NA = kScaled ? 6 : 7;
+    constexpr int H = F / 2, NQ = StagedRing<T, F, NA>::NQ;
+    __shared__ StagedRing<T, F, NA> R;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nx = sk.nx, pad = sk.pad(band);
+    const int nf = sk.nst / F;
+    const long L = sk.size();
+    const bool has_up = band > 0, has_down = band < sk.nb - 1;
+    constexpr int NW = TagWords<T>::n;
+    if (threadIdx.x >= kWave) {
+        move_chunks<T, F, NA>(
+            R, w, sk, band, nf, lk.bnd + (long)band * nx * NW, has_down,
+            lk.scratch + ((long)band * kWave + lane) * 8, lk.gen, kWave - 1, false,
+            [=](int a, int c) {
+                return (a == 0 ? r : dl + (long)(a - 1) * L) + sk.chunk_base(band, c, 0);
+            },
+            [](int c) { return c; },
+            [=](int c, int q) { return c * F + q - pad - 2 * (kWave - 1); });
+        return;
+    }
+    typedef Halves<T, NA, H> Buf;
+    const unsigned lofs = (unsigned)lane * F;
+    const u64 *bin = lk.bnd + (long)(has_up ? band - 1 : 0) * nx * NW;
+    Buf A, B;
+    Raw<T> bn{};
+    T bv[F + 1];
+    T cur = 0, se = 0, sv = 0, sw = 0;
+    auto bcol = [&](int c) { return c * F - pad + lane; };
+    auto bneed = [&](int c) {
+        const int col = bcol(c);
+        return has_up && lane <= F && col >= 0 && col < nx;
+    };
+    auto load_bnd = [&](Raw<T> &b, int c) {
+        raw_load<T>(b, bin + (long)min(max(bcol(c), 0), nx - 1) * NW);
+    };
+    auto take = [&](const Raw<T> &b, int c) {
+        const T val = bneed(c) ? raw_value(b) : T(0);
+#pragma unroll
+        for (int q = 0; q <= F; ++q) bv[q] = read_lane(val, q);
+    };
+    auto sync = [&](const Raw<T> &b, int c) -> bool {
+        if (!has_up || __all(!bneed(c) || raw_ok<T>(b, lk.gen))) {
+            take(b, c);
+            return true;
+        }
+        Raw<T> t{};
+        const bool ok = slow_wait(lk, [&] { return !bneed(c) || raw_ok<T>(t, lk.gen); },
+                                  [&] { load_bnd(t, c); });
+        take(t, c);
+        return ok;
+    };
+    auto bail = [&]() {
+        if (lane == 0) R.stop = 1;
+        handoff();
+    };
+    auto steps = [&](const Buf &cb, T(&outv)[F], int hh) {
+#pragma unroll
+        for (int t = 0; t < H; ++t) {
+            const int s = hh * H + t;
+            const T up = lane_from_below_or(cur, bv[s + 1]);
+            sw = sv;
+            sv = se;
+            se = up;
+            T v;
+            if constexpr (kScaled)
+                v = forward_point_scaled<T>(cb.v[0][t], cb.v[1][t], cb.v[2][t], cb.v[3][t],
+                                            cb.v[4][t], cb.v[5][t], cur, se, sv, sw);
+            else
+                v = forward_point_y<T>(cb.v[0][t], cb.v[1][t], cb.v[NA - 1][t], cb.v[2][t],
+                                       cb.v[3][t], cb.v[4][t], cb.v[5][t], cur, se, sv, sw);
+            outv[s] = v;
+            cur = v;
+        }
+    };
+
+    if (lane == 0) R.stop = 0;
+    load_bnd(bn, 0);
+    if (!sync(bn, 0)) {
+        bail();
+        return;
+    }
+    se = lane == 0 ? bv[0] : T(0);
+    handoff();  // barrier 0: chunks 0 and 1 in LDS
+    read_half<T, F, NA, H>(A, R, 0, 0, lane);
+    for (int c = 0; c < nf; ++c) {
+        load_bnd(bn, c + 1);
+        T outv[F];
+        read_half<T, F, NA, H>(B, R, c, NQ / 2, lane);
+        steps(A, outv, 0);
+        read_half<T, F, NA, H>(A, R, c + 1, 0, lane);
+        steps(B, outv, 1);
+        vstore<T, F>(R.out[c & 1], lofs, outv);
+        if (!sync(bn, c + 1)) {
+            bail();
+            return;
+        }
+        handoff();  // barrier c+1
+    }
+}
+
+// backward sweep, staged (same arithmetic and results as backward_band).
+// Processing chunk c is stored chunk nf-1-c, visited from its upper half down.
+template <typename T, int F>
+__device__ __forceinline__ void backward_band_staged(T *__restrict__ wz, const T *__restrict__ du,
+                                                     const Skew &sk, const Link &lk, int band)
+{
+    constexpr int NA = 5, H = F / 2, NQ = StagedRing<T, F, NA>::NQ;
+    __shared__ StagedRing<T, F, NA> R;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nx = sk.nx, pad = sk.pad(band);
+    const int nf = sk.nst / F;
+    const long L = sk.size();
+    const bool has_above = band < sk.nb - 1, has_below = band > 0;
+    constexpr int NW = TagWords<T>::n;
+    if (threadIdx.x >= kWave) {
+        move_chunks<T, F, NA>(
+            R, wz, sk, band, nf, lk.bnd + (long)(has_below ? band - 1 : 0) * nx * NW, has_below,
+            lk.scratch + ((long)band * kWave + lane) * 8, lk.gen, 0, true,
+            [=](int a, int c) {
+                return (a == 0 ? (const T *)wz : du + (long)(a - 1) * L) +
+                       sk.chunk_base(band, nf - 1 - c, 0);
+            },
+            [=](int c) { return nf - 1 - c; },
+            [=](int c, int q) { return (nf - 1 - c) * F + F - 1 - q - pad; });
+        return;
+    }
+    typedef Halves<T, NA, H> Buf;
+    const unsigned lofs = (unsigned)lane * F;
+    const u64 *bin = lk.bnd + (long)(has_above ? band : 0) * nx * NW;
+    Buf A, B;
+    Raw<T> bn{};
+    T bv[F + 1];
+    T cur = 0, nw = 0, nv = 0, ne = 0;
+    auto bcol = [&](int c) { return nx - 1 - c * F - lane; };
+    auto bneed = [&](int c) {
+        const int col = bcol(c);
+        return has_above && lane <= F && col >= 0 && col < nx;
+    };
+    auto load_bnd = [&](Raw<T> &b, int c) {
+        raw_load<T>(b, bin + (long)min(max(bcol(c), 0), nx - 1) * NW);
+    };
+    auto take = [&](const Raw<T> &b, int c) {
+        const T val = bneed(c) ? raw_value(b) : T(0);
+#pragma unroll
+        for (int q = 0; q <= F; ++q) bv[q] = read_lane(val, q);
+    };
+    auto sync = [&](const Raw<T> &b, int c) -> bool {
+        if (!has_above || __all(!bneed(c) || raw_ok<T>(b, lk.gen))) {
+            take(b, c);
+            return true;
+        }
+        Raw<T> t{};
+        const bool ok = slow_wait(lk, [&] { return !bneed(c) || raw_ok<T>(t, lk.gen); },
+                                  [&] { load_bnd(t, c); });
+        take(t, c);
+        return ok;
+    };
+    auto bail = [&]() {
+        if (lane == 0) R.stop = 1;
+        handoff();
+    };
+    auto steps = [&](const Buf &cb, T(&outv)[F], int hh) {
+#pragma unroll
+        for (int t = 0; t < H; ++t) {
+            const int s = hh * H + t;
+            const int q = F - 1 - s, qh = H - 1 - t;
+            const T dn = lane_from_above_or(cur, bv[s + 1]);
+            ne = nv;
+            nv = nw;
+            nw = dn;
+            T v;
+            if constexpr (ScaledRows<T>::value)
+                v = backward_point_fused<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh], cb.v[3][qh],
+                                            cb.v[4][qh], cur, nw, nv, ne);
+            else
+                v = backward_point<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh], cb.v[3][qh],
+                                      cb.v[4][qh], cur, nw, nv, ne);
+            outv[q] = v;
+            cur = v;
+        }
+    };
+
+    if (lane == 0) R.stop = 0;
+    load_bnd(bn, 0);
+    if (!sync(bn, 0)) {
+        bail();
+        return;
+    }
+    nw = lane == kWave - 1 ? bv[0] : T(0);
+    handoff();
+    read_half<T, F, NA, H>(A, R, 0, NQ / 2, lane);  // upper half first
+    for (int c = 0; c < nf; ++c) {
+        load_bnd(bn, c + 1);
+        T outv[F];
+        read_half<T, F, NA, H>(B, R, c, 0, lane);
+        steps(A, outv, 0);
+        read_half<T, F, NA, H>(A, R, c + 1, NQ / 2, lane);
+        steps(B, outv, 1);
+        vstore<T, F>(R.out[c & 1], lofs, outv);
+        if (!sync(bn, c + 1)) {
+            bail();
+            return;
+        }
+        handoff();
+    }
+}
+
+// --------------------------------------------------------------------------
 // lu2d9 factorization: A (skewed copy) -> dl, du (skewed).  The edge row
 // carries the four du values of each column.
 // --------------------------------------------------------------------------
@@ -642,6 +972,33 @@ __global__ __launch_bounds__(2 * kWave) void k_backward(T *__restrict__ wz, cons
 }
 
 template <typename T, int F>
+__global__ __launch_bounds__(2 * kWave) void k_forward_staged(const T *__restrict__ r,
+                                                          const T *__restrict__ dl,
+                                                          T *__restrict__ w, Skew sk, Link lk0,
+                                                          int xcd_stride)
+{
+    if (lk0.st->done) return;
+    const int blk = blockIdx.x;
+    if (blk % xcd_stride) return;
+    Link lk = lk0;
+    lk.gen = launch_gen(lk0);
+    forward_band_staged<T, F>(r, dl, w, sk, lk, blk / xcd_stride);
+}
+
+template <typename T, int F>
+__global__ __launch_bounds__(2 * kWave) void k_backward_staged(T *__restrict__ wz,
+                                                           const T *__restrict__ du, Skew sk,
+                                                           Link lk0, int xcd_stride)
+{
+    if (lk0.st->done) return;
+    const int blk = blockIdx.x;
+    if (blk % xcd_stride) return;
+    Link lk = lk0;
+    lk.gen = launch_gen(lk0);
+    backward_band_staged<T, F>(wz, du, sk, lk, sk.nb - 1 - blk / xcd_stride);
+}
+
+template <typename T, int F>
 __global__ __launch_bounds__(kWave) void k_factor(const T *__restrict__ a, T *__restrict__ dl,
                                                   T *__restrict__ du, T alpha0, Skew sk, Link lk0,
                                                   int xcd_stride)
@@ -693,6 +1050,28 @@ __global__ __launch_bounds__(2 * kWave) void k_backward_batch(const BatchSys<T> 
     const BatchSys<T> &d = sys[m.x];
     if (d.st->done) return;
     backward_band<T, F>(d.w, d.du, d.sk, batch_link(d, 2), m.y);
+}
+
+template <typename T, int F>
+__global__ __launch_bounds__(2 * kWave) void k_forward_batch_staged(
+    const BatchSys<T> *__restrict__ sys, const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys<T> &d = sys[m.x];
+    if (d.st->done) return;
+    forward_band_staged<T, F>(d.r, d.dl, d.w, d.sk, batch_link(d, 1), m.y);
+}
+
+template <typename T, int F>
+__global__ __launch_bounds__(2 * kWave) void k_backward_batch_staged(
+    const BatchSys<T> *__restrict__ sys, const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys<T> &d = sys[m.x];
+    if (d.st->done) return;
+    backward_band_staged<T, F>(d.w, d.du, d.sk, batch_link(d, 2), m.y);
 }
 
 template <typename T, int F>
@@ -875,17 +1254,26 @@ void launch_factor(const T *a_sk, T *dl, T *du, T alpha0, const Skew &sk, Link l
 
 template <typename T>
 void launch_forward(const T *r, const T *dl, T *w, const Skew &sk, Link lk, int xcd_stride,
-                    hipStream_t s)
+                    hipStream_t s, bool staged)
 {
-    hipLaunchKernelGGL((k_forward<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride), dim3(2 * kWave), 0,
-                       s, r, dl, w, sk, lk, xcd_stride);
+    if (staged)
+        hipLaunchKernelGGL((k_forward_staged<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride),
+                           dim3(2 * kWave), 0, s, r, dl, w, sk, lk, xcd_stride);
+    else
+        hipLaunchKernelGGL((k_forward<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride),
+                           dim3(2 * kWave), 0, s, r, dl, w, sk, lk, xcd_stride);
 }
 
 template <typename T>
-void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s)
+void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s,
+                     bool staged)
 {
-    hipLaunchKernelGGL((k_backward<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride), dim3(2 * kWave),
-                       0, s, wz, du, sk, lk, xcd_stride);
+    if (staged)
+        hipLaunchKernelGGL((k_backward_staged<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride),
+                           dim3(2 * kWave), 0, s, wz, du, sk, lk, xcd_stride);
+    else
+        hipLaunchKernelGGL((k_backward<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride),
+                           dim3(2 * kWave), 0, s, wz, du, sk, lk, xcd_stride);
 }
 
 template <typename T>
@@ -902,16 +1290,26 @@ void launch_factor_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipS
                        sys, map);
 }
 template <typename T>
-void launch_forward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s)
+void launch_forward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s,
+                          bool staged)
 {
-    hipLaunchKernelGGL((k_forward_batch<T, sweep_steps<T>()>), dim3(nblk), dim3(2 * kWave), 0, s,
-                       sys, map);
+    if (staged)
+        hipLaunchKernelGGL((k_forward_batch_staged<T, sweep_steps<T>()>), dim3(nblk),
+                           dim3(2 * kWave), 0, s, sys, map);
+    else
+        hipLaunchKernelGGL((k_forward_batch<T, sweep_steps<T>()>), dim3(nblk), dim3(2 * kWave), 0,
+                           s, sys, map);
 }
 template <typename T>
-void launch_backward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s)
+void launch_backward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s,
+                           bool staged)
 {
-    hipLaunchKernelGGL((k_backward_batch<T, sweep_steps<T>()>), dim3(nblk), dim3(2 * kWave), 0, s,
-                       sys, map);
+    if (staged)
+        hipLaunchKernelGGL((k_backward_batch_staged<T, sweep_steps<T>()>), dim3(nblk),
+                           dim3(2 * kWave), 0, s, sys, map);
+    else
+        hipLaunchKernelGGL((k_backward_batch<T, sweep_steps<T>()>), dim3(nblk), dim3(2 * kWave),
+                           0, s, sys, map);
 }
 template <typename T>
 void launch_to_skew_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s)
@@ -948,13 +1346,15 @@ void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipSt
 #define M2D_INST(T)                                                                           \
     template void launch_factor<T>(const T *, T *, T *, T, const Skew &, Link, int, hipStream_t); \
     template void launch_forward<T>(const T *, const T *, T *, const Skew &, Link, int,          \
-                                    hipStream_t);                                             \
-    template void launch_backward<T>(T *, const T *, const Skew &, Link, int, hipStream_t);     \
+                                    hipStream_t, bool);                                       \
+    template void launch_backward<T>(T *, const T *, const Skew &, Link, int, hipStream_t, bool);\
     template void launch_from_skew<T>(const T *, T *, int, const Skew &, hipStream_t);          \
     template void launch_prepare_factor<T>(T *, T *, const Skew &, hipStream_t);                \
     template void launch_factor_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t);  \
-    template void launch_forward_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t); \
-    template void launch_backward_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t);\
+    template void launch_forward_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t,  \
+                                          bool);                                              \
+    template void launch_backward_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t, \
+                                           bool);                                             \
     template void launch_to_skew_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t); \
     template void launch_prepare_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t); \
     template void launch_update<T>(T *, const T *, const Skew &, const SolveState *, hipStream_t);

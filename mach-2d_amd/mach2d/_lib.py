@@ -47,7 +47,8 @@ EXPORTS = [
     "m2d_lu2d9", "m2d_fb2d9", "m2d_tdma2d9", "m2d_solve_msi9d", "m2d_norm_l1_9d",
     "m2d_norm_l1_9d_relative", "m2d_norm_l1_b", "m2d_last_status", "m2d_create",
     "m2d_destroy", "m2d_set_stream", "m2d_factor", "m2d_solve", "m2d_set_inner",
-    "m2d_get_factor", "m2d_set_tuning", "m2d_tdma2d9_device", "m2d_residual_norm",
+    "m2d_get_factor", "m2d_set_tuning", "m2d_set_staging", "m2d_tdma2d9_device",
+    "m2d_residual_norm", "m2d_guess_scale",
     "m2d_version", "m2d_enable_kernel_timing", "m2d_kernel_timing",
     "m2d_batch_create", "m2d_batch_destroy", "m2d_batch_set_stream", "m2d_batch_factor",
     "m2d_batch_solve", "m2d_batch_solve_each", "m2d_batch_set_inner", "m2d_batch_set_tuning", "m2d_batch_get_tuning",
@@ -111,6 +112,10 @@ def load():
     L.m2d_set_inner.argtypes = [_vp, C.c_double, C.c_int]
     L.m2d_set_inner.restype = C.c_int
     L.m2d_residual_norm.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double)]
+    L.m2d_guess_scale.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double)]
+    L.m2d_guess_scale.restype = C.c_int
+    L.m2d_set_staging.argtypes = [_vp, C.c_int]
+    L.m2d_set_staging.restype = C.c_int
     L.m2d_residual_norm.restype = C.c_int
     L.m2d_get_factor.argtypes = [_vp, _vp, _vp]
     L.m2d_get_factor.restype = C.c_int

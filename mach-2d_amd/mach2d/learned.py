@@ -10,7 +10,9 @@ stretching (q drawn in [1.02, 1.12]) and right-hand sides that mix smooth
 random modes with white noise; each target is the system's FP64 MSI solution
 to a 1e-10 residual drop, solved in batches by the batched solver.
 
-The guessed solve judges the guess by the FP64 residual norm: if
+The network predicts the shape of p' (targets are scaled to unit RMS); the
+guessed solve takes the amplitude that minimises the FP64 residual
+(m2d_guess_scale) and judges the result by its residual norm: if
 ||A g - b|| >= ||b|| (no better than the zero guess) it starts from zero.
 It stops when ||r|| < gamma ||b||, the zero guess's threshold, so a guess is
 never asked for more accuracy than the zero-guess solve delivers.
@@ -90,7 +92,9 @@ def make_dataset(net, mats, qs, b, x):
         s = float(np.sqrt(np.mean(qv * qv))) or 1.0
         X[i:i + 1] = net.pack_input(dm[float(qs[i])], torch.tensor(b[i], device="cuda"),
                                     TRAIN_NX, TRAIN_NY, s)
-        T[i, :TRAIN_NY, :TRAIN_NX] = torch.tensor(x[i].reshape(TRAIN_NY, TRAIN_NX) / s,
+        t = x[i] / s
+        t = t / (float(np.sqrt(np.mean(t * t))) or 1.0)   # unit RMS: the network learns the shape
+        T[i, :TRAIN_NY, :TRAIN_NX] = torch.tensor(t.reshape(TRAIN_NY, TRAIN_NX),
                                                   dtype=torch.float32, device="cuda")
     return X, T, M
 
@@ -127,6 +131,7 @@ def guessed_solve(dev, net, a, rhs, var, nx, ny, gamma=1e-10, nitm=20000):
     zero = torch.zeros_like(var)
     rb = dev.residual_norm(a, rhs, zero)
     net.guess(a, rhs, nx, ny, var)
+    dev.guess_scale(a, rhs, var)   # residual-minimising amplitude of the predicted shape
     rg = dev.residual_norm(a, rhs, var)
     used = bool(np.isfinite(rg) and rg < rb)
     if not used:

@@ -314,6 +314,11 @@ class DeviceMSI:
                 out[name] = (ms.value, n.value)
         return out
 
+    def set_staging(self, staged=True):
+        rc = _lib.load().m2d_set_staging(self.h, 1 if staged else 0)
+        if rc:
+            raise SolverError(rc, "m2d_set_staging")
+
     def set_inner(self, inner_gamma=1e-4, inner_nitm=200):
         rc = _lib.load().m2d_set_inner(self.h, inner_gamma, inner_nitm)
         if rc:
@@ -342,6 +347,16 @@ class DeviceMSI:
             raise SolverError(rc, "m2d_residual_norm")
         return out.value
 
+    def guess_scale(self, a, rhs, g) -> float:
+        """Scale g in place by the residual-minimising alpha; returns alpha."""
+        alpha = C.c_double()
+        rc = _lib.load().m2d_guess_scale(self.h, C.c_void_p(self._chk(a, 9 * self.N, "a")),
+                                         C.c_void_p(self._chk(rhs, self.N, "rhs")),
+                                         C.c_void_p(self._chk(g, self.N, "g")), C.byref(alpha))
+        if rc:
+            raise SolverError(rc, "m2d_guess_scale")
+        return alpha.value
+
     def solve(self, a, rhs, var, gamma, nitm, hist=None) -> Status:
         st = Status()
         hp, cap = (None, 0)
@@ -360,7 +375,9 @@ class BatchMSI:
     replayed from a captured hipGraph, placement chosen by an autotuner.
     Each system follows fb2d9's semantics (as DeviceMSI.solve)."""
 
-    LAYOUTS = {0: "in order", 1: "one XCD per system", 2: "largest first"}
+    LAYOUTS = {0: "in order", 1: "one XCD per system", 2: "largest first",
+               3: "in order, LDS-staged", 4: "one XCD per system, LDS-staged",
+               5: "largest first, LDS-staged"}
 
     def __init__(self, sizes, precision="fp64"):
         L = _lib.load()
@@ -439,7 +456,7 @@ class BatchMSI:
     def tune(self, a, rhs, iters=4):
         """Autotune the placement; returns (layout, [ms per iteration per layout])."""
         lay = C.c_int()
-        ms = (C.c_double * 3)()
+        ms = (C.c_double * 6)()
         rc = _lib.load().m2d_batch_tune(self.h, self._ptrs(a, 9, "a"), self._ptrs(rhs, 1, "rhs"),
                                         int(iters), C.byref(lay), ms)
         if rc:

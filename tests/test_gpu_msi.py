@@ -114,14 +114,18 @@ def test_msi5d_through_embedding(gpu):
         assert np.array_equal(var, g[c + "_x5"]), c
 
 
-@pytest.mark.parametrize("publish_every,xcd_stride", [(1, 1), (7, 1), (64, 8), (5000, 1),
-                                                      (3, 8)])
-def test_launch_shapes_do_not_change_results(gpu, orc, publish_every, xcd_stride):
+@pytest.mark.parametrize("publish_every,xcd_stride,staged",
+                         [(1, 1, True), (7, 1, False), (64, 8, True), (5000, 1, False),
+                          (3, 8, False), (3, 8, True)])
+def test_launch_shapes_do_not_change_results(gpu, orc, publish_every, xcd_stride, staged):
+    """Placement (xcd_stride) and operand path (LDS-staged or registers)
+    leave the factor and the iterates bit-identical."""
     import torch
     nx, ny = 101, 197          # 4 bands, the last one partial
     s = P.pcorr_system(nx, ny, seed=9)
     dev = gpu.DeviceMSI(nx, ny)
     dev.set_tuning(publish_every, xcd_stride)
+    dev.set_staging(staged)
     ta = torch.tensor(s.a, device="cuda")
     tr = torch.tensor(s.rhs, device="cuda")
     tv = torch.zeros(s.N, dtype=torch.float64, device="cuda")

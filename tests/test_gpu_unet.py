@@ -265,3 +265,65 @@ def test_pack_input_and_guess_scale(gpu):
     out = net.forward(net.pack_input(a, b, nx, ny, sc)).cpu()
     assert np.allclose(var.cpu().numpy().reshape(ny, nx), sc * out[0, :ny, :nx, 0].double().numpy())
     assert C.sizeof(C.c_void_p) == 8
+
+
+def test_guess_scale_minimises_residual(gpu):
+    """m2d_guess_scale: alpha = (Ag.b)/(Ag.Ag); the exact solution scales by
+    1, a doubled one by 0.5, and a scaled arbitrary guess is never worse than
+    the zero guess (first-order optimality: (A(alpha g) - b).Ag = 0)."""
+    import torch
+
+    from mach2d import problems as P
+    s = P.pcorr_system(34, 18, seed=2, q=1.05, rhs="gauss")
+    dev = gpu.DeviceMSI(s.nx, s.ny, "fp64")
+    a = torch.tensor(s.a.reshape(-1), device="cuda")
+    b = torch.tensor(s.rhs, device="cuda")
+    A = np.zeros((s.N, s.N))
+    nx = s.nx
+    off = [-nx - 1, -nx, -nx + 1, -1, 0, 1, nx - 1, nx, nx + 1]
+    for k, o in enumerate(off):
+        for p in range(s.N):
+            if 0 <= p + o < s.N and s.a[k, p] != 0.0:
+                A[p, p + o] = s.a[k, p]
+    x = np.linalg.solve(A, s.rhs)
+    for mult, want in ((1.0, 1.0), (2.0, 0.5)):
+        g = torch.tensor(mult * x, device="cuda")
+        alpha = dev.guess_scale(a, b, g)
+        assert abs(alpha - want) < 1e-10, (mult, alpha)
+    rng = np.random.default_rng(3)
+    g0 = rng.standard_normal(s.N)
+    g = torch.tensor(g0, device="cuda")
+    alpha = dev.guess_scale(a, b, g)
+    Ag = A @ g0
+    assert abs(alpha - (Ag @ s.rhs) / (Ag @ Ag)) < 1e-12 * abs(alpha) + 1e-300
+    r = A @ (alpha * g0) - s.rhs
+    assert abs(r @ Ag) < 1e-9 * np.linalg.norm(Ag) * np.linalg.norm(s.rhs)
+    assert dev.residual_norm(a, b, g) <= dev.residual_norm(a, b, torch.zeros_like(g)) * (1 + 1e-12)
+    dev.close()
+
+
+def test_guessed_solve_matches_zero_guess_solution(gpu):
+    """Guess-then-MSI with an untrained network: the guess is scaled or
+    dropped, the solve reaches gamma * ||b||, and the solution agrees with the
+    zero-guess solution to 1e-8 relative."""
+    import torch
+
+    from mach2d import learned as Lr
+    from mach2d import problems as P
+    from mach2d import unet as U
+    s = P.pcorr_system(66, 34, seed=5, q=1.05, rhs="gauss")
+    net = U.UNet(seed=11)
+    dev = gpu.DeviceMSI(s.nx, s.ny, "fp64")
+    a = torch.tensor(s.a.reshape(-1), device="cuda")
+    b = torch.tensor(s.rhs, device="cuda")
+    dev.factor(a)
+    v0 = torch.zeros(s.N, dtype=torch.float64, device="cuda")
+    st0 = dev.solve(a, b, v0, 1e-10, 5000)
+    v1 = torch.zeros_like(v0)
+    st1, used, rg, rb = Lr.guessed_solve(dev, net, a, b, v1, s.nx, s.ny)
+    assert st0.status == 0 and st1.status == 0
+    assert rg <= rb
+    assert st1.rn < 1e-10 * rb * (1 + 1e-9)
+    rel = float(torch.linalg.norm(v1 - v0) / torch.linalg.norm(v0))
+    assert rel < 1e-8, rel
+    dev.close()
