@@ -1,0 +1,121 @@
+"""Single workloads of the benchmark, one per process, for profiling runs
+(rocprofv3 traces one command):
+
+    python -m mach2d.workloads single --mode fp32_ir --solves 2
+    python -m mach2d.workloads batch --mode fp64 --count 256
+    python -m mach2d.workloads batch --mode fp32_ir --count 32 --same-1026x514 [--layout L]
+
+single: the 1026x514 p' system (wall-resolved ogive grid, aspect 1e4,
+Gaussian right-hand side) solved to a 1e-10 drop from zero, `--solves` times
+after one warm-up solve.  batch: `count` systems of uneven size (64s x 32s,
+s log-uniform on [1, 8]) or `count` copies of the 1026x514 matrix with their
+own right-hand sides, solved together (autotuned layout unless given).
+Prints one JSON line with the timing and iteration counts.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import problems as P
+
+GAMMA = 1e-10
+NITM_CAP = 20000
+
+
+def single(mode, solves, staged):
+    import torch
+
+    from .solvers import DeviceMSI
+    s = P.pcorr_system(**{k: v for k, v in P.CFG3.items() if k != "gamma"})
+    dev = DeviceMSI(s.nx, s.ny, mode)
+    dev.set_staging(staged)
+    a = torch.tensor(s.a.reshape(-1), device="cuda")
+    b = torch.tensor(s.rhs, device="cuda")
+    x = torch.zeros(s.N, dtype=torch.float64, device="cuda")
+    out = []
+    for k in range(solves + 1):
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev.factor(a)
+        st = dev.solve(a, b, x, GAMMA, NITM_CAP)
+        torch.cuda.synchronize()
+        if k:
+            out.append((time.perf_counter() - t0, st.nit, st.outer, st.status))
+        print("solve %d: %.1f ms, nit %d" % (k, (time.perf_counter() - t0) * 1e3, st.nit),
+              file=sys.stderr, flush=True)
+    dev.close()
+    return {"workload": "single 1026x514", "mode": mode, "staged": staged,
+            "ms": [round(o[0] * 1e3, 3) for o in out], "nit": out[-1][1],
+            "outer": out[-1][2], "status": out[-1][3]}
+
+
+def batch(mode, count, same, layout):
+    import torch
+
+    from .solvers import BatchMSI
+    if same:
+        s = P.pcorr_system(**{k: v for k, v in P.CFG3.items() if k != "gamma"})
+        sizes = [(s.nx, s.ny)] * count
+        a0 = torch.tensor(s.a.reshape(-1), device="cuda")
+        a = [a0] * count
+        mask = torch.tensor(s.rhs != 0.0, device="cuda")
+        g = torch.Generator(device="cuda")
+        rhs = []
+        for q in range(count):
+            g.manual_seed(7000 + q)
+            r = torch.randn(s.N, dtype=torch.float64, device="cuda", generator=g)
+            rhs.append(torch.where(mask, r, torch.zeros_like(r)))
+    else:
+        sizes = P.cfg4_sizes(count)
+        systems = [P.pcorr_system(nx, ny, seed=100 + q, q=1.02 + 0.06 * ((q * 0.618) % 1.0),
+                                  rhs="gauss") for q, (nx, ny) in enumerate(sizes)]
+        a = [torch.tensor(x.a.reshape(-1), device="cuda") for x in systems]
+        rhs = [torch.tensor(x.rhs, device="cuda") for x in systems]
+    var = [torch.zeros(nx * ny, dtype=torch.float64, device="cuda") for nx, ny in sizes]
+    b = BatchMSI(sizes, precision=mode)
+    b.factor(a)
+    if layout is None:
+        layout, ms = b.tune(a, rhs, iters=4)
+    else:
+        b.set_tuning(layout, 8)
+    print("layout %d" % layout, file=sys.stderr, flush=True)
+    for v in var:
+        v.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.factor(a)
+    rc, st = b.solve(a, rhs, var, GAMMA, NITM_CAP)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    b.close()
+    return {"workload": "batch", "mode": mode, "count": count, "same_1026x514": same,
+            "layout": layout, "ms": round(dt * 1e3, 3), "solves_per_s": round(count / dt, 2),
+            "iterations_max": max(x.nit for x in st), "status": rc}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["single", "batch"])
+    ap.add_argument("--mode", default="fp32_ir", choices=["fp64", "fp32_ir"])
+    ap.add_argument("--solves", type=int, default=1)
+    ap.add_argument("--staged", type=int, default=0)
+    ap.add_argument("--count", type=int, default=256)
+    ap.add_argument("--same-1026x514", dest="same", action="store_true")
+    ap.add_argument("--layout", type=int, default=None)
+    args = ap.parse_args(argv)
+    if args.what == "single":
+        out = single(args.mode, args.solves, bool(args.staged))
+    else:
+        out = batch(args.mode, args.count, args.same, args.layout)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
