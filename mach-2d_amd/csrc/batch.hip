@@ -271,8 +271,10 @@ struct m2d_batch {
     int layout = 0;  // placement (layout % 3), LDS-staged sweeps (layout >= 3)
     int chunk_iters = 8;
     Chunk graph;
+    // inner FP32 MSI: refreshed from the FP64 residual every 50 inner
+    // iterations (an outer step of a batch costs more than a single system's)
     double inner_gamma = 1e-4;
-    int inner_nitm = 200;
+    int inner_nitm = 50;
     bool factored = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };

@@ -78,8 +78,11 @@ struct m2d_system {
     // register ring's prefetch suffices and the ring's extra LDS pass costs
     bool staged = false;
     int chunk = 8;
+    // inner FP32 MSI: short runs between FP64 residuals; measured on the
+    // 1026x514 system, a refresh every 25 inner iterations needs 650 inner
+    // iterations in all (plain FP64: 646), every 200 needs 734
     double inner_gamma = 1e-4;
-    int inner_nitm = 200;
+    int inner_nitm = 25;
 
     Work<double> w64;
     Work<float> w32;

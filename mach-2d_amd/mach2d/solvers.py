@@ -319,7 +319,7 @@ class DeviceMSI:
         if rc:
             raise SolverError(rc, "m2d_set_staging")
 
-    def set_inner(self, inner_gamma=1e-4, inner_nitm=200):
+    def set_inner(self, inner_gamma=1e-4, inner_nitm=25):
         rc = _lib.load().m2d_set_inner(self.h, inner_gamma, inner_nitm)
         if rc:
             raise SolverError(rc, "m2d_set_inner")
@@ -438,7 +438,7 @@ class BatchMSI:
                                              float(gamma), int(nitm), st)
         return rc, list(st)
 
-    def set_inner(self, inner_gamma=1e-4, inner_nitm=200):
+    def set_inner(self, inner_gamma=1e-4, inner_nitm=50):
         rc = _lib.load().m2d_batch_set_inner(self.h, float(inner_gamma), int(inner_nitm))
         if rc:
             raise SolverError(rc, "m2d_batch_set_inner")

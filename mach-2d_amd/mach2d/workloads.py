@@ -19,21 +19,21 @@ import json
 import sys
 import time
 
-import numpy as np
-
 from . import problems as P
 
 GAMMA = 1e-10
 NITM_CAP = 20000
 
 
-def single(mode, solves, staged):
+def single(mode, solves, staged, inner_gamma=None, inner_nitm=25):
     import torch
 
     from .solvers import DeviceMSI
     s = P.pcorr_system(**{k: v for k, v in P.CFG3.items() if k != "gamma"})
     dev = DeviceMSI(s.nx, s.ny, mode)
     dev.set_staging(staged)
+    if inner_gamma is not None and mode == "fp32_ir":
+        dev.set_inner(inner_gamma, inner_nitm)
     a = torch.tensor(s.a.reshape(-1), device="cuda")
     b = torch.tensor(s.rhs, device="cuda")
     x = torch.zeros(s.N, dtype=torch.float64, device="cuda")
@@ -51,11 +51,12 @@ def single(mode, solves, staged):
               file=sys.stderr, flush=True)
     dev.close()
     return {"workload": "single 1026x514", "mode": mode, "staged": staged,
+            "inner_gamma": inner_gamma, "inner_nitm": inner_nitm,
             "ms": [round(o[0] * 1e3, 3) for o in out], "nit": out[-1][1],
             "outer": out[-1][2], "status": out[-1][3]}
 
 
-def batch(mode, count, same, layout):
+def batch(mode, count, same, layout, inner_gamma=None, inner_nitm=50):
     import torch
 
     from .solvers import BatchMSI
@@ -79,6 +80,8 @@ def batch(mode, count, same, layout):
         rhs = [torch.tensor(x.rhs, device="cuda") for x in systems]
     var = [torch.zeros(nx * ny, dtype=torch.float64, device="cuda") for nx, ny in sizes]
     b = BatchMSI(sizes, precision=mode)
+    if inner_gamma is not None and mode == "fp32_ir":
+        b.set_inner(inner_gamma, inner_nitm)
     b.factor(a)
     if layout is None:
         layout, ms = b.tune(a, rhs, iters=4)
@@ -95,6 +98,7 @@ def batch(mode, count, same, layout):
     dt = time.perf_counter() - t0
     b.close()
     return {"workload": "batch", "mode": mode, "count": count, "same_1026x514": same,
+            "inner_gamma": inner_gamma, "inner_nitm": inner_nitm,
             "layout": layout, "ms": round(dt * 1e3, 3), "solves_per_s": round(count / dt, 2),
             "iterations_max": max(x.nit for x in st), "status": rc}
 
@@ -108,11 +112,15 @@ def main(argv=None):
     ap.add_argument("--count", type=int, default=256)
     ap.add_argument("--same-1026x514", dest="same", action="store_true")
     ap.add_argument("--layout", type=int, default=None)
+    ap.add_argument("--inner-gamma", dest="inner_gamma", type=float, default=None)
+    ap.add_argument("--inner-nitm", dest="inner_nitm", type=int, default=None)
     args = ap.parse_args(argv)
     if args.what == "single":
-        out = single(args.mode, args.solves, bool(args.staged))
+        out = single(args.mode, args.solves, bool(args.staged), args.inner_gamma,
+                     args.inner_nitm or 25)
     else:
-        out = batch(args.mode, args.count, args.same, args.layout)
+        out = batch(args.mode, args.count, args.same, args.layout, args.inner_gamma,
+                    args.inner_nitm or 50)
     print(json.dumps(out), flush=True)
     return 0
 
