@@ -84,8 +84,30 @@ void m2d_tdma2d9(const int *nx, const int *ny, const int *nitm,
                  const double *a, const double *b, double *x);
 
 /* Replaces class_solver_msi9d%solve(a,b,sol) after init(nx,ny,nitm,tol),
- * src/solvers/mod_solver_msi.f90:129-166: lu2d9 then fb2d9. */
+ * src/solvers/mod_solver_msi.f90:96-133: lu2d9 then fb2d9. */
 void m2d_solve_msi9d(const int *nx, const int *ny, const int *nitm,
+                     const double *tol, const double *a, const double *b,
+                     double *sol);
+
+/* Replaces lu2d5(b,dl,du,nij,ni,nj), src/solvers/msi2d5-v03.f90:247-330:
+ * the 5-diagonal MSI factorization.  b(nij,5) holds the slots S, W, P, E, N
+ * (src/solvers/tdma2d5.f90:47-51); out: dl(nij,4) (P, W, SE, S) and
+ * du(nij,3) (E, NW, N).  Computed as the 9-diagonal factorization of the
+ * zero-corner embedding of b, whose terms reduce to lu2d5's. */
+void m2d_lu2d5(const double *b, double *dl, double *du, const int *nij,
+               const int *ni, const int *nj);
+
+/* Replaces fb2d5(b,dl,du,nj,ni,nij,var,gamma,nitm,rhs),
+ * src/solvers/msi2d5-v03.f90:42-233 (argument order (nj, ni, nij)); the
+ * stopping rule and limits of m2d_fb2d9, with the 5-diagonal residual and
+ * sweeps. */
+void m2d_fb2d5(const double *b, const double *dl, const double *du,
+               const int *nj, const int *ni, const int *nij, double *var,
+               const double *gamma, const int *nitm, const double *rhs);
+
+/* Replaces class_solver_msi5d%solve(a,b,sol) after init(nx,ny,nitm,tol),
+ * src/solvers/mod_solver_msi.f90:56-93: lu2d5 then fb2d5. */
+void m2d_solve_msi5d(const int *nx, const int *ny, const int *nitm,
                      const double *tol, const double *a, const double *b,
                      double *sol);
 
@@ -107,8 +129,15 @@ typedef struct m2d_system m2d_system;
 
 /* Create a solver for one nx*ny grid; precision is M2D_FP64 or M2D_FP32_IR.
  * Allocates the factor workspace (dl, du: 9*nx*ny values) on the current
- * device.  Mirrors class_solver_msi9d%init, mod_solver_msi.f90:129-148. */
+ * device.  Mirrors class_solver_msi9d%init, mod_solver_msi.f90:96-115. */
 int m2d_create(int nx, int ny, int precision, m2d_system **out);
+
+/* As m2d_create for A with ndiag = 9 or 5 diagonals.  ndiag = 5: A holds the
+ * slots S, W, P, E, N (5*nx*ny doubles) of msi2d5 (class_solver_msi5d,
+ * mod_solver_msi.f90:56-93); factor, solve, residual norm and guess scaling
+ * then follow lu2d5 / fb2d5, and m2d_get_factor returns dl 4*nx*ny and
+ * du 3*nx*ny values. */
+int m2d_create_stencil(int nx, int ny, int precision, int ndiag, m2d_system **out);
 int m2d_destroy(m2d_system *s);
 
 /* Run on a HIP stream (hipStream_t passed as void*; NULL = default). */

@@ -35,24 +35,44 @@ namespace m2d {
 
 constexpr int kPts = 256;  // points per workgroup of the point-wise kernels
 
+// Point (i, j) of lane threadIdx.x in point block `blk` (kPts consecutive
+// points in the reference order) of a system with nx columns.  One 32-bit
+// division per workgroup (the block index is uniform), then at most
+// kPts / nx + 1 row steps per lane; systems hold fewer than 2^27 points.
+struct BlockPoint {
+    int p, i, j;
+};
+__device__ __forceinline__ BlockPoint block_point(int blk, int nx)
+{
+    const int p0 = blk * kPts;
+    int j = p0 / nx;
+    int i = p0 - j * nx + (int)threadIdx.x;
+    while (i >= nx) {
+        i -= nx;
+        ++j;
+    }
+    return {p0 + (int)threadIdx.x, i, j};
+}
+
 // r = A.x - rhs for the 256 points of block map[g].y of system map[g].x, and
 // the block's partial sum of r.r (fixed reduction order).  skewed: write r
-// in the system's skewed layout, else in the reference layout.
+// in the system's skewed layout, else in the reference layout.  Entries with
+// map[g].x < 0 fill the map up to a multiple of the XCD count.
 template <typename T>
 __global__ __launch_bounds__(kPts) void k_residual_batch(const BatchSys<T> *__restrict__ sys,
                                                          const int2 *__restrict__ map, int skewed)
 {
     const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
     const BatchSys<T> &d = sys[m.x];
     if (d.st->done) return;
     const int nx = d.sk.nx, ny = d.sk.ny;
-    const long N = (long)nx * ny;
-    const long p = (long)m.y * kPts + threadIdx.x;
+    const int N = nx * ny;
+    const BlockPoint q = block_point(m.y, nx);
     double acc = 0.0;
-    if (p < N) {
-        const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
-        const T v = residual_point<T, T, T>(d.ra, N, d.rx, d.rrhs[p], i, j, nx, ny);
-        d.r[skewed ? d.sk.of_point(i, j) : p] = v;
+    if (q.p < N) {
+        const T v = residual_point<T, T, T>(d.ra, N, d.rx, d.rrhs[q.p], q.i, q.j, nx, ny);
+        d.r[skewed ? d.sk.template of_point_c<64 / (int)sizeof(T)>(q.i, q.j) : (long)q.p] = v;
         acc = (double)v * (double)v;
     }
     __shared__ double red[kPts];
@@ -72,13 +92,13 @@ __global__ __launch_bounds__(kPts) void k_update_batch(const BatchSys<T> *__rest
                                                        const int2 *__restrict__ map)
 {
     const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
     const BatchSys<T> &d = sys[m.x];
     if (d.st->done) return;
     const int nx = d.sk.nx;
-    const long p = (long)m.y * kPts + threadIdx.x;
-    if (p >= (long)nx * d.sk.ny) return;
-    const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
-    d.rx[p] = d.w[d.sk.of_point(i, j)] + d.rx[p];
+    const BlockPoint q = block_point(m.y, nx);
+    if (q.p >= nx * d.sk.ny) return;
+    d.rx[q.p] = d.w[d.sk.template of_point_c<64 / (int)sizeof(T)>(q.i, q.j)] + d.rx[q.p];
 }
 
 // One workgroup per system: ||r|| from the partial sums and the stopping
@@ -158,13 +178,14 @@ __global__ __launch_bounds__(kPts) void k_check_outside_batch(
     const BatchSys<double> *__restrict__ sys, const int2 *__restrict__ map)
 {
     const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
     const BatchSys<double> &d = sys[m.x];
     const int nx = d.sk.nx, ny = d.sk.ny;
-    const long N = (long)nx * ny;
-    const long p = (long)m.y * kPts + threadIdx.x;
+    const int N = nx * ny;
+    const BlockPoint q = block_point(m.y, nx);
+    const int p = q.p, i = q.i, j = q.j;
     int bad = 0;
     if (p < N) {
-        const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
         const bool w = i == 0, e = i == nx - 1, s = j == 0, n = j == ny - 1;
         const bool out[9] = {s || w, s, s || e, w, false, e, n || w, n, n || e};
 #pragma unroll
@@ -180,6 +201,7 @@ __global__ __launch_bounds__(kPts) void k_a32_batch(const BatchSys<double> *__re
                                                     const int2 *__restrict__ map)
 {
     const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
     const long N = (long)s64[m.x].sk.nx * s64[m.x].sk.ny;
     const long p = (long)m.y * kPts + threadIdx.x;
     if (p >= N) return;
@@ -195,6 +217,7 @@ __global__ __launch_bounds__(kPts) void k_ir_prepare_batch(const BatchSys<double
                                                            const int2 *__restrict__ map)
 {
     const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
     const BatchSys<double> &o = s64[m.x];
     if (o.st->done) return;
     const long N = (long)o.sk.nx * o.sk.ny;
@@ -210,6 +233,7 @@ __global__ __launch_bounds__(kPts) void k_ir_update_batch(const BatchSys<double>
                                                           const int2 *__restrict__ map)
 {
     const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
     const BatchSys<double> &o = s64[m.x];
     if (o.st->done) return;
     const long N = (long)o.sk.nx * o.sk.ny;
@@ -356,11 +380,20 @@ int build_maps(m2d_batch *b, int layout)
 {
     std::vector<int2> fwd, bwd, pts, skw;
     sweep_maps(b, layout % 3, fwd, bwd);
+    std::vector<int2> nat;
     for (int q = 0; q < b->n; ++q) {
-        for (long k = 0; k < blocks_of(b->N[q], kPts); ++k) pts.push_back(make_int2(q, (int)k));
+        for (long k = 0; k < blocks_of(b->N[q], kPts); ++k) nat.push_back(make_int2(q, (int)k));
         for (long k = 0; k < blocks_of(b->sk[q].size(), 256); ++k)
             skw.push_back(make_int2(q, (int)k));
     }
+    // Point blocks: XCD x (workgroups g with g mod 8 = x) takes the x-th
+    // contiguous eighth of the list, so the rows a block's stencil reaches
+    // (the blocks about nx / 256 before and after it) are mostly read on
+    // the same XCD and meet in its L2.  Holes of the ragged end are (-1, 0).
+    constexpr int kXcd = 8;
+    const size_t per = (nat.size() + kXcd - 1) / kXcd;
+    pts.assign(per * kXcd, make_int2(-1, 0));
+    for (size_t k = 0; k < nat.size(); ++k) pts[(k % per) * kXcd + k / per] = nat[k];
     Maps &m = b->maps;
     if (m.d) (void)hipFree(m.d);
     m.d = nullptr;

@@ -71,6 +71,10 @@ struct m2d_system {
     int nx = 0, ny = 0;
     long N = 0;
     int precision = M2D_FP64;
+    // diagonals of A: 9 (msi2d9 slots) or 5 (msi2d5 slots S, W, P, E, N; the
+    // factor is that of the zero-corner embedding, the iterations use the
+    // 5-diagonal residual and sweeps)
+    int nd = 9;
     hipStream_t stream = nullptr;
     int xcd_stride = 1;
     // sweep operands through the LDS ring (m2d_set_staging); off by default:
@@ -205,12 +209,14 @@ extern "C" int m2d_destroy(m2d_system *s)
     return M2D_OK;
 }
 
-extern "C" int m2d_create(int nx, int ny, int precision, m2d_system **out)
+extern "C" int m2d_create_stencil(int nx, int ny, int precision, int ndiag, m2d_system **out)
 {
-    if (!out || nx < 3 || ny < 3 || (precision != M2D_FP64 && precision != M2D_FP32_IR))
+    if (!out || nx < 3 || ny < 3 || (precision != M2D_FP64 && precision != M2D_FP32_IR) ||
+        (ndiag != 5 && ndiag != 9))
         return M2D_ERR_ARG;
     if ((long)nx * ny > (1L << 31) / 16) return M2D_ERR_ARG;
     m2d_system *s = new m2d_system();
+    s->nd = ndiag;
     s->nx = nx;
     s->ny = ny;
     s->N = (long)nx * ny;
@@ -223,7 +229,7 @@ extern "C" int m2d_create(int nx, int ny, int precision, m2d_system **out)
               dalloc(&s->r64, N) == hipSuccess;
     if (ok && precision == M2D_FP64) ok = s->w64.alloc(nx, ny);
     if (ok && precision == M2D_FP32_IR) {
-        ok = s->w32.alloc(nx, ny) && dalloc(&s->a32, 9 * N) == hipSuccess &&
+        ok = s->w32.alloc(nx, ny) && dalloc(&s->a32, (long)ndiag * N) == hipSuccess &&
              dalloc(&s->c32, N) == hipSuccess && dalloc(&s->rr32, N) == hipSuccess;
     }
     ok = ok && hipHostMalloc((void **)&s->h_st, sizeof(SolveState), hipHostMallocDefault) ==
@@ -235,6 +241,11 @@ extern "C" int m2d_create(int nx, int ny, int precision, m2d_system **out)
     }
     *out = s;
     return M2D_OK;
+}
+
+extern "C" int m2d_create(int nx, int ny, int precision, m2d_system **out)
+{
+    return m2d_create_stencil(nx, ny, precision, 9, out);
 }
 
 extern "C" int m2d_set_stream(m2d_system *s, void *stream)
@@ -298,10 +309,13 @@ extern "C" int m2d_factor(m2d_system *s, const double *d_a)
 {
     if (!s || !d_a) return M2D_ERR_ARG;
     begin(s, s->st);
-    launch_check_outside(d_a, s->nx, s->ny, s->st, s->stream);
+    launch_check_outside(d_a, s->nx, s->ny, s->st, s->stream, s->nd);
     if (s->precision == M2D_FP64) {
         Work<double> &W = s->w64;
-        launch_to_skew<double, double>(d_a, W.a, 9, W.sk, s->stream);
+        if (s->nd == 5)
+            launch_to_skew5<double, double>(d_a, W.a, W.sk, s->stream);
+        else
+            launch_to_skew<double, double>(d_a, W.a, 9, W.sk, s->stream);
         timed(s, KT_FACTOR, 0, [&] {
             launch_factor<double>(W.a, W.dl, W.du, 0.5, W.sk, make_link(s, s->st, 0), s->xcd_stride,
                                   s->stream);
@@ -309,8 +323,11 @@ extern "C" int m2d_factor(m2d_system *s, const double *d_a)
         launch_prepare_factor<double>(W.dl, W.du, W.sk, s->stream);
     } else {
         Work<float> &W = s->w32;
-        launch_convert<double, float>(d_a, s->a32, 9 * s->N, s->stream);
-        launch_to_skew<double, float>(d_a, W.a, 9, W.sk, s->stream);
+        launch_convert<double, float>(d_a, s->a32, (long)s->nd * s->N, s->stream);
+        if (s->nd == 5)
+            launch_to_skew5<double, float>(d_a, W.a, W.sk, s->stream);
+        else
+            launch_to_skew<double, float>(d_a, W.a, 9, W.sk, s->stream);
         timed(s, KT_FACTOR + KT_FP32, 0, [&] {
             launch_factor<float>(W.a, W.dl, W.du, 0.5f, W.sk, make_link(s, s->st, 0),
                                  s->xcd_stride, s->stream);
@@ -331,8 +348,9 @@ extern "C" int m2d_factor(m2d_system *s, const double *d_a)
 extern "C" int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du)
 {
     if (!s || s->precision != M2D_FP64 || !s->factored || !d_dl || !d_du) return M2D_ERR_ARG;
-    launch_from_skew<double>(s->w64.dl, d_dl, 5, s->w64.sk, s->stream);
-    launch_from_skew<double>(s->w64.du, d_du, 4, s->w64.sk, s->stream);
+    // 5 diagonals: dl1..dl4 and du1..du3 (msi2d5's dl(nij,4), du(nij,3))
+    launch_from_skew<double>(s->w64.dl, d_dl, s->nd == 5 ? 4 : 5, s->w64.sk, s->stream);
+    launch_from_skew<double>(s->w64.du, d_du, s->nd == 5 ? 3 : 4, s->w64.sk, s->stream);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(s->stream));
     return M2D_OK;
@@ -341,8 +359,13 @@ extern "C" int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du)
 // Install an externally computed FP64 factor (fb2d9 takes dl, du as input).
 static int put_factor(m2d_system *s, const double *d_dl, const double *d_du)
 {
-    launch_to_skew<double, double>(d_dl, s->w64.dl, 5, s->w64.sk, s->stream);
-    launch_to_skew<double, double>(d_du, s->w64.du, 4, s->w64.sk, s->stream);
+    const long L = s->w64.sk.size();
+    if (s->nd == 5) {  // dl5 and du4 of the embedding are zero (and never read)
+        HIP_OK(hipMemsetAsync(s->w64.dl + 4 * L, 0, sizeof(double) * L, s->stream));
+        HIP_OK(hipMemsetAsync(s->w64.du + 3 * L, 0, sizeof(double) * L, s->stream));
+    }
+    launch_to_skew<double, double>(d_dl, s->w64.dl, s->nd == 5 ? 4 : 5, s->w64.sk, s->stream);
+    launch_to_skew<double, double>(d_du, s->w64.du, s->nd == 5 ? 3 : 4, s->w64.sk, s->stream);
     launch_prepare_factor<double>(s->w64.dl, s->w64.du, s->w64.sk, s->stream);
     HIP_OK(hipGetLastError());
     s->factored = true;
@@ -357,15 +380,16 @@ static void enqueue_iteration(m2d_system *s, Work<T> &W, SolveState *st, const T
     const int off = sizeof(T) == 4 ? KT_FP32 : 0;
     timed(s, KT_FORWARD + off, k, [&] {
         launch_forward<T>(W.r, W.dl, W.w, W.sk, make_link(s, st, 1), s->xcd_stride, s->stream,
-                          s->staged);
+                          s->staged, s->nd == 5);
     });
     timed(s, KT_BACKWARD + off, k, [&] {
         launch_backward<T>(W.w, W.du, W.sk, make_link(s, st, 2), s->xcd_stride, s->stream,
-                           s->staged);
+                           s->staged, s->nd == 5);
     });
     timed(s, KT_RESIDUAL + off, k, [&] {
         launch_update<T>(x, W.w, W.sk, st, s->stream);
-        launch_residual<T, T, T>(a, x, rhs, W.r, s->nx, s->ny, s->partial, st, &W.sk, s->stream);
+        launch_residual<T, T, T>(a, x, rhs, W.r, s->nx, s->ny, s->partial, st, &W.sk, s->stream,
+                                 s->nd);
     });
     launch_norm_finalize(s->partial, s->np, st, hist, hist_cap, 0, gamma, nitm, s->stream);
 }
@@ -377,7 +401,7 @@ static int run_msi(m2d_system *s, Work<T> &W, SolveState *st, const T *a, const 
                    double gamma, int nitm, double *hist, int hist_cap)
 {
     begin(s, st);
-    launch_residual<T, T, T>(a, x, rhs, W.r, s->nx, s->ny, s->partial, st, &W.sk, s->stream);
+    launch_residual<T, T, T>(a, x, rhs, W.r, s->nx, s->ny, s->partial, st, &W.sk, s->stream, s->nd);
     launch_norm_finalize(s->partial, s->np, st, nullptr, 0, 1, gamma, nitm, s->stream);
     const int kmax = nitm > 0 ? nitm : kIterCap;
     int k = 1;
@@ -458,7 +482,7 @@ static int solve_fp32_ir(m2d_system *s, const double *d_a, const double *d_rhs, 
     const unsigned nblk = (unsigned)((N + 255) / 256);
     launch_reset_state(s->st, s->stream);
     launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r64, s->nx, s->ny, s->partial,
-                                            s->st, nullptr, s->stream);
+                                            s->st, nullptr, s->stream, s->nd);
     launch_norm_finalize(s->partial, s->np, s->st, nullptr, 0, 1, gamma, nitm, s->stream);
     HIP_OK(hipGetLastError());
     int rc = read_state(s, s->st);
@@ -477,7 +501,7 @@ static int solve_fp32_ir(m2d_system *s, const double *d_a, const double *d_rhs, 
         hipLaunchKernelGGL(k_ir_update, dim3(nblk), dim3(256), 0, s->stream, d_var, s->c32, N,
                            s->st);
         launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r64, s->nx, s->ny,
-                                                s->partial, s->st, nullptr, s->stream);
+                                                s->partial, s->st, nullptr, s->stream, s->nd);
         launch_norm_finalize(s->partial, s->np, s->st, d_hist, hist_cap, 0, gamma, nitm,
                              s->stream);
         HIP_OK(hipGetLastError());
@@ -506,7 +530,7 @@ extern "C" int m2d_residual_norm(m2d_system *s, const double *d_a, const double 
     if (!s || !d_a || !d_rhs || !d_var || !out) return M2D_ERR_ARG;
     launch_reset_state(s->st, s->stream);
     launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r64, s->nx, s->ny, s->partial,
-                                            s->st, nullptr, s->stream);
+                                            s->st, nullptr, s->stream, s->nd);
     launch_norm_finalize(s->partial, s->np, s->st, nullptr, 0, 1, 1.0, 1, s->stream);
     HIP_OK(hipGetLastError());
     const int rc = read_state(s, s->st);
@@ -520,14 +544,16 @@ extern "C" int m2d_residual_norm(m2d_system *s, const double *d_a, const double 
 __global__ __launch_bounds__(256) void k_guess_dots(const double *__restrict__ a,
                                                     const double *__restrict__ g,
                                                     const double *__restrict__ b, int nx, int ny,
-                                                    double *__restrict__ part)
+                                                    double *__restrict__ part, int nd)
 {
     const long N = (long)nx * ny;
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
     double ab = 0.0, aa = 0.0;
     if (p < N) {
         const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
-        const double v = residual_point<double, double, double>(a, N, g, 0.0, i, j, nx, ny);
+        const double v = nd == 5
+                             ? residual_point5<double, double, double>(a, N, g, 0.0, i, j, nx, ny)
+                             : residual_point<double, double, double>(a, N, g, 0.0, i, j, nx, ny);
         ab = v * b[p];
         aa = v * v;
     }
@@ -561,7 +587,7 @@ extern "C" int m2d_guess_scale(m2d_system *s, const double *d_a, const double *d
     const long N = s->N;
     const unsigned nblk = (unsigned)((N + 255) / 256);
     hipLaunchKernelGGL(k_guess_dots, dim3(nblk), dim3(256), 0, s->stream, d_a, d_g, d_rhs, s->nx,
-                       s->ny, s->r64);
+                       s->ny, s->r64, s->nd);
     std::vector<double> part(2 * (size_t)nblk);
     HIP_OK(hipMemcpyAsync(part.data(), s->r64, sizeof(double) * part.size(),
                           hipMemcpyDeviceToHost, s->stream));
@@ -593,7 +619,7 @@ extern "C" int m2d_solve(m2d_system *s, const double *d_a, const double *d_rhs, 
 // the device buffers and an FP64 handle for the last grid size.
 // ---------------------------------------------------------------------------
 struct HostCache {
-    int nx = 0, ny = 0;
+    int nx = 0, ny = 0, nd = 0;
     m2d_system *sys = nullptr;
     double *a = nullptr, *dl = nullptr, *du = nullptr, *rhs = nullptr, *var = nullptr;
     ~HostCache() { release(); }
@@ -605,13 +631,13 @@ struct HostCache {
         a = dl = du = rhs = var = nullptr;
         if (sys) m2d_destroy(sys);
         sys = nullptr;
-        nx = ny = 0;
+        nx = ny = nd = 0;
     }
-    int ensure(int nx_, int ny_)
+    int ensure(int nx_, int ny_, int nd_)
     {
-        if (sys && nx == nx_ && ny == ny_) return M2D_OK;
+        if (sys && nx == nx_ && ny == ny_ && nd == nd_) return M2D_OK;
         release();
-        int rc = m2d_create(nx_, ny_, M2D_FP64, &sys);
+        int rc = m2d_create_stencil(nx_, ny_, M2D_FP64, nd_, &sys);
         if (rc) return rc;
         const long N = (long)nx_ * ny_;
         if (dalloc(&a, 9 * N) || dalloc(&dl, 5 * N) || dalloc(&du, 4 * N) || dalloc(&rhs, N) ||
@@ -621,6 +647,7 @@ struct HostCache {
         }
         nx = nx_;
         ny = ny_;
+        nd = nd_;
         return M2D_OK;
     }
 };
@@ -631,21 +658,24 @@ static bool sizes_ok(int nij, int ni, int nj)
     return ni >= 3 && nj >= 3 && (long)ni * nj == (long)nij;
 }
 
-extern "C" void m2d_lu2d9(const double *b, double *dl, double *du, const int *nij,
-                          const int *ni, const int *nj)
+// lu2d9 / lu2d5 on host arrays: b has nd diagonals, dl nd == 9 ? 5 : 4 and
+// du nd == 9 ? 4 : 3.
+static void host_lu(const double *b, double *dl, double *du, const int *nij, const int *ni,
+                    const int *nj, int nd)
 {
     if (!b || !dl || !du || !nij || !ni || !nj || !sizes_ok(*nij, *ni, *nj)) {
         set_last(M2D_ERR_ARG);
         return;
     }
     const long N = *nij;
-    int rc = g_cache.ensure(*ni, *nj);
+    const int nl = nd == 9 ? 5 : 4, nu = nd == 9 ? 4 : 3;
+    int rc = g_cache.ensure(*ni, *nj, nd);
     if (rc) {
         set_last(rc);
         return;
     }
     m2d_system *s = g_cache.sys;
-    if (hipMemcpy(g_cache.a, b, sizeof(double) * 9 * N, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(g_cache.a, b, sizeof(double) * nd * N, hipMemcpyHostToDevice) != hipSuccess) {
         set_last(M2D_ERR_HIP);
         return;
     }
@@ -654,25 +684,38 @@ extern "C" void m2d_lu2d9(const double *b, double *dl, double *du, const int *ni
     if (rc == M2D_OK || rc == M2D_ERR_NONFINITE) {
         int rc2 = m2d_get_factor(s, g_cache.dl, g_cache.du);
         if (rc2 ||
-            hipMemcpy(dl, g_cache.dl, sizeof(double) * 5 * N, hipMemcpyDeviceToHost) !=
+            hipMemcpy(dl, g_cache.dl, sizeof(double) * nl * N, hipMemcpyDeviceToHost) !=
                 hipSuccess ||
-            hipMemcpy(du, g_cache.du, sizeof(double) * 4 * N, hipMemcpyDeviceToHost) != hipSuccess)
+            hipMemcpy(du, g_cache.du, sizeof(double) * nu * N, hipMemcpyDeviceToHost) != hipSuccess)
             set_last(M2D_ERR_HIP);
     }
 }
 
-static void host_fb2d9(const double *b, const double *dl, const double *du, int nj, int ni,
-                       int nij, double *var, double gamma, int nitm, const double *rhs,
-                       bool factor_here)
+extern "C" void m2d_lu2d9(const double *b, double *dl, double *du, const int *nij,
+                          const int *ni, const int *nj)
+{
+    host_lu(b, dl, du, nij, ni, nj, 9);
+}
+
+extern "C" void m2d_lu2d5(const double *b, double *dl, double *du, const int *nij,
+                          const int *ni, const int *nj)
+{
+    host_lu(b, dl, du, nij, ni, nj, 5);
+}
+
+static void host_fb(const double *b, const double *dl, const double *du, int nj, int ni, int nij,
+                    double *var, double gamma, int nitm, const double *rhs, bool factor_here,
+                    int nd)
 {
     const long N = nij;
-    int rc = g_cache.ensure(ni, nj);
+    const int nl = nd == 9 ? 5 : 4, nu = nd == 9 ? 4 : 3;
+    int rc = g_cache.ensure(ni, nj, nd);
     if (rc) {
         set_last(rc);
         return;
     }
     m2d_system *s = g_cache.sys;
-    bool ok = hipMemcpy(g_cache.a, b, sizeof(double) * 9 * N, hipMemcpyHostToDevice) ==
+    bool ok = hipMemcpy(g_cache.a, b, sizeof(double) * nd * N, hipMemcpyHostToDevice) ==
                   hipSuccess &&
               hipMemcpy(g_cache.rhs, rhs, sizeof(double) * N, hipMemcpyHostToDevice) ==
                   hipSuccess &&
@@ -689,17 +732,17 @@ static void host_fb2d9(const double *b, const double *dl, const double *du, int 
             return;
         }
     } else {
-        ok = hipMemcpy(g_cache.dl, dl, sizeof(double) * 5 * N, hipMemcpyHostToDevice) ==
+        ok = hipMemcpy(g_cache.dl, dl, sizeof(double) * nl * N, hipMemcpyHostToDevice) ==
                  hipSuccess &&
-             hipMemcpy(g_cache.du, du, sizeof(double) * 4 * N, hipMemcpyHostToDevice) ==
+             hipMemcpy(g_cache.du, du, sizeof(double) * nu * N, hipMemcpyHostToDevice) ==
                  hipSuccess;
         if (!ok) {
             set_last(M2D_ERR_HIP);
             return;
         }
-        // fb2d9 does not factor; still reject couplings across the grid edge
+        // fb2d9 / fb2d5 do not factor; still reject couplings across the grid edge
         begin(s, s->st);
-        launch_check_outside(g_cache.a, ni, nj, s->st, s->stream);
+        launch_check_outside(g_cache.a, ni, nj, s->st, s->stream, nd);
         rc = read_state(s, s->st);
         if (!rc && s->h_st->outside) rc = M2D_ERR_OUTSIDE_COUPLING;
         if (!rc) rc = put_factor(s, g_cache.dl, g_cache.du);
@@ -727,7 +770,19 @@ extern "C" void m2d_fb2d9(const double *b, const double *dl, const double *du, c
         set_last(M2D_ERR_ARG);
         return;
     }
-    host_fb2d9(b, dl, du, *nj, *ni, *nij, var, *gamma, *nitm, rhs, false);
+    host_fb(b, dl, du, *nj, *ni, *nij, var, *gamma, *nitm, rhs, false, 9);
+}
+
+extern "C" void m2d_fb2d5(const double *b, const double *dl, const double *du, const int *nj,
+                          const int *ni, const int *nij, double *var, const double *gamma,
+                          const int *nitm, const double *rhs)
+{
+    if (!b || !dl || !du || !nj || !ni || !nij || !var || !gamma || !nitm || !rhs ||
+        !sizes_ok(*nij, *ni, *nj)) {
+        set_last(M2D_ERR_ARG);
+        return;
+    }
+    host_fb(b, dl, du, *nj, *ni, *nij, var, *gamma, *nitm, rhs, false, 5);
 }
 
 extern "C" void m2d_solve_msi9d(const int *nx, const int *ny, const int *nitm, const double *tol,
@@ -737,7 +792,17 @@ extern "C" void m2d_solve_msi9d(const int *nx, const int *ny, const int *nitm, c
         set_last(M2D_ERR_ARG);
         return;
     }
-    host_fb2d9(a, nullptr, nullptr, *ny, *nx, *nx * *ny, sol, *tol, *nitm, b, true);
+    host_fb(a, nullptr, nullptr, *ny, *nx, *nx * *ny, sol, *tol, *nitm, b, true, 9);
+}
+
+extern "C" void m2d_solve_msi5d(const int *nx, const int *ny, const int *nitm, const double *tol,
+                                const double *a, const double *b, double *sol)
+{
+    if (!nx || !ny || !nitm || !tol || !a || !b || !sol || *nx < 3 || *ny < 3) {
+        set_last(M2D_ERR_ARG);
+        return;
+    }
+    host_fb(a, nullptr, nullptr, *ny, *nx, *nx * *ny, sol, *tol, *nitm, b, true, 5);
 }
 
 // Batched entry: every system keeps its own handle, factor and arrays.

@@ -42,14 +42,19 @@ void launch_clear_boundary(unsigned long long *bnd, int nx, int ny, hipStream_t 
 template <typename T>
 void launch_factor(const T *a_sk, T *dl, T *du, T alpha0, const Skew &sk, Link lk,
                    int xcd_stride, hipStream_t s);
+// five: the 5-diagonal sweeps (msi2d5) on the same arrays (dl5 and du4 unused)
 template <typename T>
 void launch_forward(const T *r, const T *dl, T *w, const Skew &sk, Link lk, int xcd_stride,
-                    hipStream_t s, bool staged);
+                    hipStream_t s, bool staged, bool five = false);
 template <typename T>
 void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s,
-                     bool staged);
+                     bool staged, bool five = false);
 template <typename TI, typename TO>
 void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t s);
+// 5 diagonals (S, W, P, E, N), reference layout -> the 9 skewed arrays of
+// their zero-corner embedding (SW, SE, NW, NE zero)
+template <typename TI, typename TO>
+void launch_to_skew5(const TI *src, TO *dst, const Skew &sk, hipStream_t s);
 // dl: 6 arrays (dl1..dl5, then 1/dl1, written here); padding made finite.
 template <typename T>
 void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s);
@@ -65,25 +70,28 @@ template <typename T>
 void launch_factor_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s);
 template <typename T>
 void launch_forward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s,
-                          bool staged);
+                          bool staged, bool five = false);
 template <typename T>
 void launch_backward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s,
-                           bool staged);
+                           bool staged, bool five = false);
 template <typename T>
 void launch_to_skew_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s);
 template <typename T>
 void launch_prepare_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s);
 
 // ---- parallel kernels (msi9.hip) ---------------------------------------------
+// nd: 9 diagonals (msi2d9 slots) or 5 (msi2d5 slots S, W, P, E, N)
 template <typename TA, typename TX, typename TR>
 void launch_residual(const TA *a, const TX *x, const TA *rhs, TR *r, int nx, int ny,
-                     double *partial, const SolveState *st, const Skew *out_skew, hipStream_t s);
+                     double *partial, const SolveState *st, const Skew *out_skew, hipStream_t s,
+                     int nd = 9);
 int residual_blocks(int nx, int ny);
 // initial = 1: the residual of the incoming var (k = 0); otherwise the
 // residual after iteration k = st->iter + 1, which becomes st->iter.
 void launch_norm_finalize(const double *partial, int np, SolveState *st, double *hist,
                           int hist_cap, int initial, double gamma, int nitm, hipStream_t s);
-void launch_check_outside(const double *a, int nx, int ny, SolveState *st, hipStream_t s);
+void launch_check_outside(const double *a, int nx, int ny, SolveState *st, hipStream_t s,
+                          int nd = 9);
 void launch_reset_state(SolveState *st, hipStream_t s);
 template <typename TI, typename TO>
 void launch_convert(const TI *in, TO *out, long n, hipStream_t s);

@@ -7,6 +7,7 @@
 // The stopping rule runs on the device, so an iteration needs no host round
 // trip: once it fires, a flag turns the later kernels of the enqueued
 // iterations into no-ops.
+#include <assert.h>
 #include <math.h>
 
 #include "m2d_internal.h"
@@ -21,23 +22,33 @@ namespace m2d {
 // --------------------------------------------------------------------------
 constexpr int kResThreads = 256;
 
-template <typename TA, typename TX, typename TR>
+// Blocks are row segments of kResThreads columns, G = nbx * ny of them in
+// the grid order (segment fastest).  Workgroup g runs on XCD g mod 8; it
+// takes segment (g mod 8) * per + g / 8 (per = ceil(G / 8)), so each XCD
+// works through a contiguous run of rows and the neighbour rows a stencil
+// reads meet in its L2.  Partial sums keep the grid order.
+template <typename TA, typename TX, typename TR, int ND>
 __global__ __launch_bounds__(kResThreads) void k_residual(const TA *__restrict__ a,
                                                           const TX *__restrict__ x,
                                                           const TA *__restrict__ rhs,
                                                           TR *__restrict__ r, int nx, int ny,
                                                           double *__restrict__ partial,
-                                                          const SolveState *st, Skew osk)
+                                                          const SolveState *st, Skew osk,
+                                                          int nbx, int per)
 {
     if (st && st->done) return;
-    const int i = blockIdx.x * kResThreads + threadIdx.x;
-    const int j = blockIdx.y;
+    const int lg = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+    if (lg >= nbx * ny) return;
+    const int j = lg / nbx;
+    const int i = (lg - j * nbx) * kResThreads + threadIdx.x;
     const long N = (long)nx * ny;
     double acc = 0.0;
     if (i < nx) {
         const long p = (long)j * nx + i;
-        const TR v = residual_point<TA, TX, TR>(a, N, x, (TR)rhs[p], i, j, nx, ny);
-        r[osk.S ? osk.of_point(i, j) : p] = v;  // skewed output when osk.S > 0
+        const TR v = ND == 5 ? residual_point5<TA, TX, TR>(a, N, x, (TR)rhs[p], i, j, nx, ny)
+                             : residual_point<TA, TX, TR>(a, N, x, (TR)rhs[p], i, j, nx, ny);
+        // skewed output when osk.S > 0 (chunk length 64 / sizeof(TR))
+        r[osk.S ? osk.template of_point_c<64 / (int)sizeof(TR)>(i, j) : p] = v;
         acc = (double)v * (double)v;
     }
     __shared__ double red[kResThreads];
@@ -48,7 +59,7 @@ __global__ __launch_bounds__(kResThreads) void k_residual(const TA *__restrict__
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+    if (threadIdx.x == 0) partial[lg] = red[0];
 }
 
 int residual_blocks(int nx, int ny)
@@ -111,9 +122,10 @@ __global__ __launch_bounds__(256) void k_norm_finalize(const double *__restrict_
     }
 }
 
-// Count couplings towards neighbours outside the grid (must all be zero).
+// Count couplings towards neighbours outside the grid (must all be zero);
+// nd = 9 (msi2d9 slots) or 5 (S, W, P, E, N).
 __global__ __launch_bounds__(256) void k_check_outside(const double *__restrict__ a, int nx,
-                                                       int ny, SolveState *st)
+                                                       int ny, SolveState *st, int nd)
 {
     const long N = (long)nx * ny;
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
@@ -121,9 +133,15 @@ __global__ __launch_bounds__(256) void k_check_outside(const double *__restrict_
     if (p < N) {
         const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
         const bool w = i == 0, e = i == nx - 1, s = j == 0, n = j == ny - 1;
-        const bool out[9] = {s || w, s, s || e, w, false, e, n || w, n, n || e};
+        if (nd == 5) {
+            const bool out[5] = {s, w, false, e, n};
 #pragma unroll
-        for (int k = 0; k < 9; ++k) bad += (out[k] && a[k * N + p] != 0.0) ? 1 : 0;
+            for (int k = 0; k < 5; ++k) bad += (out[k] && a[k * N + p] != 0.0) ? 1 : 0;
+        } else {
+            const bool out[9] = {s || w, s, s || e, w, false, e, n || w, n, n || e};
+#pragma unroll
+            for (int k = 0; k < 9; ++k) bad += (out[k] && a[k * N + p] != 0.0) ? 1 : 0;
+        }
     }
     if (__any(bad)) {
         if (bad) atomicAdd(&st->outside, bad);
@@ -158,13 +176,20 @@ __global__ void k_convert(const TI *__restrict__ in, TO *__restrict__ out, long 
 // ---- launchers -------------------------------------------------------------
 template <typename TA, typename TX, typename TR>
 void launch_residual(const TA *a, const TX *x, const TA *rhs, TR *r, int nx, int ny,
-                     double *partial, const SolveState *st, const Skew *out_skew, hipStream_t s)
+                     double *partial, const SolveState *st, const Skew *out_skew, hipStream_t s,
+                     int nd)
 {
-    dim3 grid((nx + kResThreads - 1) / kResThreads, ny);
+    const int nbx = (nx + kResThreads - 1) / kResThreads;
+    const int per = (nbx * ny + 7) / 8;
     Skew osk{};
     if (out_skew) osk = *out_skew;
-    hipLaunchKernelGGL((k_residual<TA, TX, TR>), grid, dim3(kResThreads), 0, s, a, x, rhs, r,
-                       nx, ny, partial, st, osk);
+    assert(!out_skew || osk.S == 64 / (int)sizeof(TR));
+    if (nd == 5)
+        hipLaunchKernelGGL((k_residual<TA, TX, TR, 5>), dim3(8 * per), dim3(kResThreads), 0, s, a,
+                           x, rhs, r, nx, ny, partial, st, osk, nbx, per);
+    else
+        hipLaunchKernelGGL((k_residual<TA, TX, TR, 9>), dim3(8 * per), dim3(kResThreads), 0, s, a,
+                           x, rhs, r, nx, ny, partial, st, osk, nbx, per);
 }
 
 void launch_norm_finalize(const double *partial, int np, SolveState *st, double *hist,
@@ -174,11 +199,11 @@ void launch_norm_finalize(const double *partial, int np, SolveState *st, double 
                        hist_cap, initial, gamma, nitm);
 }
 
-void launch_check_outside(const double *a, int nx, int ny, SolveState *st, hipStream_t s)
+void launch_check_outside(const double *a, int nx, int ny, SolveState *st, hipStream_t s, int nd)
 {
     const long N = (long)nx * ny;
     hipLaunchKernelGGL(k_check_outside, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a,
-                       nx, ny, st);
+                       nx, ny, st, nd);
 }
 
 void launch_reset_state(SolveState *st, hipStream_t s)
@@ -196,10 +221,11 @@ void launch_convert(const TI *in, TO *out, long n, hipStream_t s)
 template void launch_residual<double, double, double>(const double *, const double *,
                                                       const double *, double *, int, int,
                                                       double *, const SolveState *,
-                                                      const Skew *, hipStream_t);
+                                                      const Skew *, hipStream_t, int);
 template void launch_residual<float, float, float>(const float *, const float *, const float *,
                                                    float *, int, int, double *,
-                                                   const SolveState *, const Skew *, hipStream_t);
+                                                   const SolveState *, const Skew *, hipStream_t,
+                                                   int);
 template void launch_convert<double, float>(const double *, float *, long, hipStream_t);
 template void launch_convert<float, double>(const float *, double *, long, hipStream_t);
 

@@ -126,6 +126,64 @@ M2D_HD T backward_point_fused(T w, T du1, T du2, T du3, T du4, T z_e, T z_nw, T 
     return fma_t(-du2, z_nw, acc);
 }
 
+// ---- 5-diagonal MSI (src/solvers/msi2d5-v03.f90) --------------------------
+// L = dl1..dl4 (P, W, SE, S), U = I + du1..du3 (E, NW, N): the 9-diagonal
+// factors of the zero-corner embedding of A, without the SW and NE entries,
+// which are exactly zero there (dl5 = b1 = 0, du4 = b9 / dl1 = 0).  The sums
+// are the 9-diagonal ones without their last term, in the same order.
+
+// Forward substitution L w = -r at one point: msi2d5-v03.f90:139.
+template <typename T>
+M2D_HD T forward_point5_y(T r, T dl1, T y, T dl2, T dl3, T dl4, T w_w, T w_se, T w_s)
+{
+    return div_by_rcp<T>(-(r + dl2 * w_w + dl3 * w_se + dl4 * w_s), dl1, y);
+}
+
+// FP32 form on pre-scaled rows (see forward_point_scaled).
+template <typename T>
+M2D_HD T forward_point5_scaled(T r, T ny, T ne2, T ne3, T ne4, T w_w, T w_se, T w_s)
+{
+    T acc = r * ny;
+    acc = fma_t(ne4, w_s, acc);
+    acc = fma_t(ne2, w_w, acc);
+    return fma_t(ne3, w_se, acc);
+}
+
+// Backward substitution U z = w at one point: msi2d5-v03.f90:158.
+template <typename T>
+M2D_HD T backward_point5(T w, T du1, T du2, T du3, T z_e, T z_nw, T z_n)
+{
+    return w - du1 * z_e - du2 * z_nw - du3 * z_n;
+}
+
+// FP32 form with fused multiply-adds, the previous step's values last.
+template <typename T>
+M2D_HD T backward_point5_fused(T w, T du1, T du2, T du3, T z_e, T z_nw, T z_n)
+{
+    T acc = fma_t(-du3, z_n, w);
+    acc = fma_t(-du1, z_e, acc);
+    return fma_t(-du2, z_nw, acc);
+}
+
+// 5-diagonal slots, 0-based: the reference's 1=S 2=W 3=P 4=E 5=N
+// (src/solvers/tdma2d5.f90:47-51).
+enum Slot5 { k5S = 0, k5W = 1, k5P = 2, k5E = 3, k5N = 4 };
+
+// Residual of a 5-diagonal row, terms in slot order from -rhs, neighbours
+// outside the grid omitted (the row classes of msi2d5-v03.f90:59-131).
+template <typename TA, typename TX, typename TR>
+M2D_HD TR residual_point5(const TA *a, long ld, const TX *x, TR rhs, int i, int j, int nx, int ny)
+{
+    const long p = (long)j * nx + i;
+    TR acc = -rhs;
+    if (j > 0) acc += (TR)a[k5S * ld + p] * (TR)x[p - nx];
+    if (i > 0) acc += (TR)a[k5W * ld + p] * (TR)x[p - 1];
+    acc += (TR)a[k5P * ld + p] * (TR)x[p];
+    if (i < nx - 1) acc += (TR)a[k5E * ld + p] * (TR)x[p + 1];
+    if (j < ny - 1) acc += (TR)a[k5N * ld + p] * (TR)x[p + nx];
+    return acc;
+}
+
 // Residual r = A.x - rhs at (i, j) (0-based), 2-D aware: a term whose
 // neighbour lies outside the grid is omitted, as in the seven row classes of
 // msi2d9-v03.f90:59-131.  Terms are added in slot order starting from -rhs.

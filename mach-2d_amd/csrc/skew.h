@@ -53,6 +53,15 @@ struct Skew {
         const int b = j >> 6, l = j & 63;
         return index(b, l, i + 2 * l + pad(b));
     }
+    // of_point for a chunk length known at compile time (S_ == S): shifts and
+    // masks instead of two integer divisions
+    template <int S_>
+    SKEW_HD long of_point_c(int i, int j) const
+    {
+        const int b = j >> 6, l = j & 63;
+        const unsigned u = (unsigned)(i + 2 * l + pad(b));
+        return (((long)b * nch + (long)(u / S_)) * 64 + l) * S_ + (long)(u % S_);
+    }
     SKEW_HD long size() const { return (long)nb * nch * 64 * S; }
 };
 

@@ -140,6 +140,62 @@ struct ScaledRows {
     static constexpr bool value = sizeof(T) == 4;
 };
 
+// Operand arrays of the sweeps.  9 diagonals: forward r, dl1..dl5, 1/dl1
+// (FP32 scaled rows: r, ny, ne2..ne5), backward w, du1..du4.  5 diagonals
+// (msi2d5): the same arrays without dl5 (ne5) and du4, which the zero-corner
+// embedding makes exactly zero.  Forward operand k >= 1 is dl array fslot(k).
+template <typename T, bool Five>
+struct SweepOps {
+    static constexpr bool kScaled = ScaledRows<T>::value;
+    static constexpr int NF = (kScaled ? 6 : 7) - (Five ? 1 : 0);
+    static constexpr int NB = Five ? 4 : 5;
+    static constexpr int fslot(int k) { return (!kScaled && Five && k == NF - 1) ? 5 : k - 1; }
+};
+
+// One forward step from the operands of element t of a half buffer.
+template <bool Five, typename T, int NA, int H>
+__device__ __forceinline__ T forward_step(const Halves<T, NA, H> &cb, int t, T cur, T se, T sv,
+                                          T sw)
+{
+    if constexpr (ScaledRows<T>::value) {
+        if constexpr (Five)
+            return forward_point5_scaled<T>(cb.v[0][t], cb.v[1][t], cb.v[2][t], cb.v[3][t],
+                                            cb.v[4][t], cur, se, sv);
+        else
+            return forward_point_scaled<T>(cb.v[0][t], cb.v[1][t], cb.v[2][t], cb.v[3][t],
+                                           cb.v[4][t], cb.v[5][t], cur, se, sv, sw);
+    } else {
+        if constexpr (Five)
+            return forward_point5_y<T>(cb.v[0][t], cb.v[1][t], cb.v[NA - 1][t], cb.v[2][t],
+                                       cb.v[3][t], cb.v[4][t], cur, se, sv);
+        else
+            return forward_point_y<T>(cb.v[0][t], cb.v[1][t], cb.v[NA - 1][t], cb.v[2][t],
+                                      cb.v[3][t], cb.v[4][t], cb.v[5][t], cur, se, sv, sw);
+    }
+}
+
+// One backward step from the operands of element qh of a half buffer.
+template <bool Five, typename T, int NA, int H>
+__device__ __forceinline__ T backward_step(const Halves<T, NA, H> &cb, int qh, T cur, T nw, T nv,
+                                           T ne)
+{
+    if constexpr (ScaledRows<T>::value) {
+        if constexpr (Five)
+            return backward_point5_fused<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh], cb.v[3][qh],
+                                            cur, nw, nv);
+        else
+            return backward_point_fused<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh], cb.v[3][qh],
+                                           cb.v[4][qh], cur, nw, nv, ne);
+    } else {
+        if constexpr (Five)
+            return backward_point5<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh], cb.v[3][qh], cur, nw,
+                                      nv);
+        else
+            return backward_point<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh], cb.v[3][qh],
+                                     cb.v[4][qh], cur, nw, nv, ne);
+    }
+}
+
 // Results of the sweeps leave through LDS.  A sweep workgroup has two
 // waves: wave 0 computes, wave 1 stores.  After each chunk the computing
 // wave writes its F results per lane into one of two ring slots and meets
@@ -210,7 +266,7 @@ __device__ void store_chunks(T *__restrict__ dst, const Skew &sk, int band, int 
 // forward sweep  L w = -r  (msi2d9-v03.f90:139-156); r, dl (dl1..dl5, then
 // 1/dl1), w in the skewed layout.
 // --------------------------------------------------------------------------
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *__restrict__ dl,
                                              T *__restrict__ w, const Skew &sk, const Link &lk,
                                              int band)
@@ -234,8 +290,8 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
     // a lane's offset in a chunk block; chunk bases are wave-uniform, so the
     // vector accesses take an SGPR base and this fixed VGPR offset
     const unsigned lofs = (unsigned)lane * F;
-    constexpr bool kScaled = ScaledRows<T>::value;
-    constexpr int NA = kScaled ? 6 : 7;
+    typedef SweepOps<T, Five> Ops;
+    constexpr int NA = Ops::NF;
     typedef Halves<T, NA, H> Buf;  // r, dl1..dl5, 1/dl1 (scaled: r, ny, ne2..ne5)
     const u64 *bin = lk.bnd + (long)(has_up ? band - 1 : 0) * nx * NW;
 
@@ -248,7 +304,7 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
         const long e = sk.chunk_base(band, min(h >> 1, nf - 1), 0) + (h & 1) * H;
 #pragma unroll
         for (int k = 0; k < NA; ++k) {
-            const T *p = (k == 0 ? r : dl + (long)(k - 1) * L) + e;
+            const T *p = (k == 0 ? r : dl + (long)Ops::fslot(k) * L) + e;
 #pragma unroll
             for (int q = 0; q < H / V; ++q) {
                 const uint4 v = *reinterpret_cast<const uint4 *>(p + (lofs + q * V));
@@ -313,13 +369,7 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
                 sw = sv;
                 sv = se;
                 se = up;
-                T v;
-                if constexpr (kScaled)
-                    v = forward_point_scaled<T>(cb.v[0][t], cb.v[1][t], cb.v[2][t], cb.v[3][t],
-                                                cb.v[4][t], cb.v[5][t], cur, se, sv, sw);
-                else
-                    v = forward_point_y<T>(cb.v[0][t], cb.v[1][t], cb.v[NA - 1][t], cb.v[2][t],
-                                           cb.v[3][t], cb.v[4][t], cb.v[5][t], cur, se, sv, sw);
+                const T v = forward_step<Five>(cb, t, cur, se, sv, sw);
                 outv[s] = v;
                 cur = v;
             }
@@ -352,7 +402,7 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
 // backward sweep  U z = w  (msi2d9-v03.f90:158-176): chunks in reverse
 // stored order, z over w in place; bands top to bottom.
 // --------------------------------------------------------------------------
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__restrict__ du,
                                               const Skew &sk, const Link &lk, int band)
 {
@@ -374,7 +424,8 @@ __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__res
     constexpr int H = F / 2;
     constexpr int V = 16 / (int)sizeof(T);
     const unsigned lofs = (unsigned)lane * F;  // lane offset in a chunk block
-    typedef Halves<T, 5, H> Buf;  // w, du1..du4
+    constexpr int NA = SweepOps<T, Five>::NB;
+    typedef Halves<T, NA, H> Buf;  // w, du1..du4
     const u64 *bin = lk.bnd + (long)(has_above ? band : 0) * nx * NW;
 
     Buf X, Y, Z;
@@ -386,7 +437,7 @@ __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__res
     auto load_half = [&](Buf &b, int h) {
         const long e = sk.chunk_base(band, max(nf - 1 - (h >> 1), 0), 0) + (1 - (h & 1)) * H;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
+        for (int k = 0; k < NA; ++k) {
             const T *p = (k == 0 ? wz : du + (long)(k - 1) * L) + e;
 #pragma unroll
             for (int q = 0; q < H / V; ++q) {
@@ -443,13 +494,7 @@ __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__res
                 ne = nv;
                 nv = nw;
                 nw = dn;
-                T v;
-                if constexpr (ScaledRows<T>::value)
-                    v = backward_point_fused<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh],
-                                                cb.v[3][qh], cb.v[4][qh], cur, nw, nv, ne);
-                else
-                    v = backward_point<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh], cb.v[3][qh],
-                                          cb.v[4][qh], cur, nw, nv, ne);
+                const T v = backward_step<Five>(cb, qh, cur, nw, nv, ne);
                 outv[q] = v;
                 cur = v;
             }
@@ -590,13 +635,13 @@ __device__ __forceinline__ void read_half(Halves<T, NA, H> &b, const StagedRing<
 }
 
 // forward sweep, staged (same arithmetic and results as forward_band).
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __device__ __forceinline__ void forward_band_staged(const T *__restrict__ r,
                                                     const T *__restrict__ dl, T *__restrict__ w,
                                                     const Skew &sk, const Link &lk, int band)
 {
-    constexpr bool kScaled = ScaledRows<T>::value;
-    constexpr int NA = kScaled ? 6 : 7;
+    typedef SweepOps<T, Five> Ops;
+    constexpr int NA = Ops::NF;
     constexpr int H = F / 2, NQ = StagedRing<T, F, NA>::NQ;
     __shared__ StagedRing<T, F, NA> R;
     const int lane = threadIdx.x & (kWave - 1);
@@ -610,7 +655,7 @@ __device__ __forceinline__ void forward_band_staged(const T *__restrict__ r,
             R, w, sk, band, nf, lk.bnd + (long)band * nx * NW, has_down,
             lk.scratch + ((long)band * kWave + lane) * 8, lk.gen, kWave - 1, false,
             [=](int a, int c) {
-                return (a == 0 ? r : dl + (long)(a - 1) * L) + sk.chunk_base(band, c, 0);
+                return (a == 0 ? r : dl + (long)Ops::fslot(a) * L) + sk.chunk_base(band, c, 0);
             },
             [](int c) { return c; },
             [=](int c, int q) { return c * F + q - pad - 2 * (kWave - 1); });
@@ -659,13 +704,7 @@ __device__ __forceinline__ void forward_band_staged(const T *__restrict__ r,
             sw = sv;
             sv = se;
             se = up;
-            T v;
-            if constexpr (kScaled)
-                v = forward_point_scaled<T>(cb.v[0][t], cb.v[1][t], cb.v[2][t], cb.v[3][t],
-                                            cb.v[4][t], cb.v[5][t], cur, se, sv, sw);
-            else
-                v = forward_point_y<T>(cb.v[0][t], cb.v[1][t], cb.v[NA - 1][t], cb.v[2][t],
-                                       cb.v[3][t], cb.v[4][t], cb.v[5][t], cur, se, sv, sw);
+            const T v = forward_step<Five>(cb, t, cur, se, sv, sw);
             outv[s] = v;
             cur = v;
         }
@@ -698,11 +737,11 @@ __device__ __forceinline__ void forward_band_staged(const T *__restrict__ r,
 
 // backward sweep, staged (same arithmetic and results as backward_band).
 // Processing chunk c is stored chunk nf-1-c, visited from its upper half down.
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __device__ __forceinline__ void backward_band_staged(T *__restrict__ wz, const T *__restrict__ du,
                                                      const Skew &sk, const Link &lk, int band)
 {
-    constexpr int NA = 5, H = F / 2, NQ = StagedRing<T, F, NA>::NQ;
+    constexpr int NA = SweepOps<T, Five>::NB, H = F / 2, NQ = StagedRing<T, F, NA>::NQ;
     __shared__ StagedRing<T, F, NA> R;
     const int lane = threadIdx.x & (kWave - 1);
     const int nx = sk.nx, pad = sk.pad(band);
@@ -766,13 +805,7 @@ __device__ __forceinline__ void backward_band_staged(T *__restrict__ wz, const T
             ne = nv;
             nv = nw;
             nw = dn;
-            T v;
-            if constexpr (ScaledRows<T>::value)
-                v = backward_point_fused<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh], cb.v[3][qh],
-                                            cb.v[4][qh], cur, nw, nv, ne);
-            else
-                v = backward_point<T>(cb.v[0][qh], cb.v[1][qh], cb.v[2][qh], cb.v[3][qh],
-                                      cb.v[4][qh], cur, nw, nv, ne);
+            const T v = backward_step<Five>(cb, qh, cur, nw, nv, ne);
             outv[q] = v;
             cur = v;
         }
@@ -946,7 +979,7 @@ __device__ __forceinline__ void factor_band(const T *__restrict__ a, T *__restri
 // g / xcd_stride, so with xcd_stride 8 every band lands on the same XCD
 // (workgroups are dealt to the XCDs round robin) and shares its L2.
 // --------------------------------------------------------------------------
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __global__ __launch_bounds__(2 * kWave) void k_forward(const T *__restrict__ r,
                                                    const T *__restrict__ dl, T *__restrict__ w,
                                                    Skew sk, Link lk0, int xcd_stride)
@@ -956,10 +989,10 @@ __global__ __launch_bounds__(2 * kWave) void k_forward(const T *__restrict__ r,
     if (blk % xcd_stride) return;
     Link lk = lk0;
     lk.gen = launch_gen(lk0);
-    forward_band<T, F>(r, dl, w, sk, lk, blk / xcd_stride);
+    forward_band<T, F, Five>(r, dl, w, sk, lk, blk / xcd_stride);
 }
 
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __global__ __launch_bounds__(2 * kWave) void k_backward(T *__restrict__ wz, const T *__restrict__ du,
                                                     Skew sk, Link lk0, int xcd_stride)
 {
@@ -968,10 +1001,10 @@ __global__ __launch_bounds__(2 * kWave) void k_backward(T *__restrict__ wz, cons
     if (blk % xcd_stride) return;
     Link lk = lk0;
     lk.gen = launch_gen(lk0);
-    backward_band<T, F>(wz, du, sk, lk, sk.nb - 1 - blk / xcd_stride);
+    backward_band<T, F, Five>(wz, du, sk, lk, sk.nb - 1 - blk / xcd_stride);
 }
 
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __global__ __launch_bounds__(2 * kWave) void k_forward_staged(const T *__restrict__ r,
                                                           const T *__restrict__ dl,
                                                           T *__restrict__ w, Skew sk, Link lk0,
@@ -982,10 +1015,10 @@ __global__ __launch_bounds__(2 * kWave) void k_forward_staged(const T *__restric
     if (blk % xcd_stride) return;
     Link lk = lk0;
     lk.gen = launch_gen(lk0);
-    forward_band_staged<T, F>(r, dl, w, sk, lk, blk / xcd_stride);
+    forward_band_staged<T, F, Five>(r, dl, w, sk, lk, blk / xcd_stride);
 }
 
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __global__ __launch_bounds__(2 * kWave) void k_backward_staged(T *__restrict__ wz,
                                                            const T *__restrict__ du, Skew sk,
                                                            Link lk0, int xcd_stride)
@@ -995,7 +1028,7 @@ __global__ __launch_bounds__(2 * kWave) void k_backward_staged(T *__restrict__ w
     if (blk % xcd_stride) return;
     Link lk = lk0;
     lk.gen = launch_gen(lk0);
-    backward_band_staged<T, F>(wz, du, sk, lk, sk.nb - 1 - blk / xcd_stride);
+    backward_band_staged<T, F, Five>(wz, du, sk, lk, sk.nb - 1 - blk / xcd_stride);
 }
 
 template <typename T, int F>
@@ -1030,7 +1063,7 @@ __device__ __forceinline__ Link batch_link(const BatchSys<T> &d, int phase)
     return lk;
 }
 
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __global__ __launch_bounds__(2 * kWave) void k_forward_batch(const BatchSys<T> *__restrict__ sys,
                                                          const int2 *__restrict__ map)
 {
@@ -1038,10 +1071,10 @@ __global__ __launch_bounds__(2 * kWave) void k_forward_batch(const BatchSys<T> *
     if (m.x < 0) return;
     const BatchSys<T> &d = sys[m.x];
     if (d.st->done) return;
-    forward_band<T, F>(d.r, d.dl, d.w, d.sk, batch_link(d, 1), m.y);
+    forward_band<T, F, Five>(d.r, d.dl, d.w, d.sk, batch_link(d, 1), m.y);
 }
 
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __global__ __launch_bounds__(2 * kWave) void k_backward_batch(const BatchSys<T> *__restrict__ sys,
                                                           const int2 *__restrict__ map)
 {
@@ -1049,10 +1082,10 @@ __global__ __launch_bounds__(2 * kWave) void k_backward_batch(const BatchSys<T> 
     if (m.x < 0) return;
     const BatchSys<T> &d = sys[m.x];
     if (d.st->done) return;
-    backward_band<T, F>(d.w, d.du, d.sk, batch_link(d, 2), m.y);
+    backward_band<T, F, Five>(d.w, d.du, d.sk, batch_link(d, 2), m.y);
 }
 
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __global__ __launch_bounds__(2 * kWave) void k_forward_batch_staged(
     const BatchSys<T> *__restrict__ sys, const int2 *__restrict__ map)
 {
@@ -1060,10 +1093,10 @@ __global__ __launch_bounds__(2 * kWave) void k_forward_batch_staged(
     if (m.x < 0) return;
     const BatchSys<T> &d = sys[m.x];
     if (d.st->done) return;
-    forward_band_staged<T, F>(d.r, d.dl, d.w, d.sk, batch_link(d, 1), m.y);
+    forward_band_staged<T, F, Five>(d.r, d.dl, d.w, d.sk, batch_link(d, 1), m.y);
 }
 
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __global__ __launch_bounds__(2 * kWave) void k_backward_batch_staged(
     const BatchSys<T> *__restrict__ sys, const int2 *__restrict__ map)
 {
@@ -1071,7 +1104,7 @@ __global__ __launch_bounds__(2 * kWave) void k_backward_batch_staged(
     if (m.x < 0) return;
     const BatchSys<T> &d = sys[m.x];
     if (d.st->done) return;
-    backward_band_staged<T, F>(d.w, d.du, d.sk, batch_link(d, 2), m.y);
+    backward_band_staged<T, F, Five>(d.w, d.du, d.sk, batch_link(d, 2), m.y);
 }
 
 template <typename T, int F>
@@ -1155,6 +1188,24 @@ __global__ __launch_bounds__(256) void k_to_skew(const TI *__restrict__ src, TO 
     if (e < sk.size()) to_skew_entry<TI, TO>(src, dst, nk, sk, e);
 }
 
+// 5 diagonals (S, W, P, E, N), reference layout -> the 9 skewed arrays of
+// the zero-corner embedding: slot k9 takes 5-diagonal slot from5[k9] or 0.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_to_skew5(const TI *__restrict__ src, TO *__restrict__ dst,
+                                                  Skew sk)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= sk.size()) return;
+    constexpr int from5[9] = {-1, k5S, -1, k5W, k5P, k5E, -1, k5N, -1};
+    const long L = sk.size();
+    const SkewPoint q = skew_point(sk, e);
+    const long N = (long)sk.nx * sk.ny;
+    const long p = q.real ? (long)q.j * sk.nx + q.i : 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        dst[k * L + e] = (q.real && from5[k] >= 0) ? (TO)src[from5[k] * N + p] : TO(0);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_prepare_factor(T *__restrict__ dl, T *__restrict__ du,
                                                         Skew sk)
@@ -1200,16 +1251,16 @@ __global__ __launch_bounds__(256) void k_from_skew(const T *__restrict__ src, T 
 }
 
 // var = z + var (msi2d9-v03.f90:181), z skewed, var in the reference layout.
+// One workgroup per row segment of 256 columns (blockIdx.y = row).
 template <typename T>
 __global__ __launch_bounds__(256) void k_update(T *__restrict__ x, const T *__restrict__ z, Skew sk,
                                                 const SolveState *st)
 {
     if (st->done) return;
-    const long N = (long)sk.nx * sk.ny;
-    const long p = (long)blockIdx.x * 256 + threadIdx.x;
-    if (p >= N) return;
-    const int j = (int)(p / sk.nx), i = (int)(p - (long)j * sk.nx);
-    x[p] = z[sk.of_point(i, j)] + x[p];
+    const int j = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= sk.nx) return;
+    const long p = (long)j * sk.nx + i;
+    x[p] = z[sk.template of_point_c<64 / (int)sizeof(T)>(i, j)] + x[p];
 }
 
 // Clear the boundary words (generations restart with every solve).
@@ -1254,26 +1305,36 @@ void launch_factor(const T *a_sk, T *dl, T *du, T alpha0, const Skew &sk, Link l
 
 template <typename T>
 void launch_forward(const T *r, const T *dl, T *w, const Skew &sk, Link lk, int xcd_stride,
-                    hipStream_t s, bool staged)
+                    hipStream_t s, bool staged, bool five)
 {
-    if (staged)
-        hipLaunchKernelGGL((k_forward_staged<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride),
-                           dim3(2 * kWave), 0, s, r, dl, w, sk, lk, xcd_stride);
+    constexpr int F = sweep_steps<T>();
+    const dim3 g(sk.nb * xcd_stride), b(2 * kWave);
+    if (staged && five)
+        hipLaunchKernelGGL((k_forward_staged<T, F, true>), g, b, 0, s, r, dl, w, sk, lk, xcd_stride);
+    else if (staged)
+        hipLaunchKernelGGL((k_forward_staged<T, F, false>), g, b, 0, s, r, dl, w, sk, lk,
+                           xcd_stride);
+    else if (five)
+        hipLaunchKernelGGL((k_forward<T, F, true>), g, b, 0, s, r, dl, w, sk, lk, xcd_stride);
     else
-        hipLaunchKernelGGL((k_forward<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride),
-                           dim3(2 * kWave), 0, s, r, dl, w, sk, lk, xcd_stride);
+        hipLaunchKernelGGL((k_forward<T, F, false>), g, b, 0, s, r, dl, w, sk, lk, xcd_stride);
 }
 
 template <typename T>
 void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s,
-                     bool staged)
+                     bool staged, bool five)
 {
-    if (staged)
-        hipLaunchKernelGGL((k_backward_staged<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride),
-                           dim3(2 * kWave), 0, s, wz, du, sk, lk, xcd_stride);
+    constexpr int F = sweep_steps<T>();
+    const dim3 g(sk.nb * xcd_stride), b(2 * kWave);
+    if (staged && five)
+        hipLaunchKernelGGL((k_backward_staged<T, F, true>), g, b, 0, s, wz, du, sk, lk, xcd_stride);
+    else if (staged)
+        hipLaunchKernelGGL((k_backward_staged<T, F, false>), g, b, 0, s, wz, du, sk, lk,
+                           xcd_stride);
+    else if (five)
+        hipLaunchKernelGGL((k_backward<T, F, true>), g, b, 0, s, wz, du, sk, lk, xcd_stride);
     else
-        hipLaunchKernelGGL((k_backward<T, sweep_steps<T>()>), dim3(sk.nb * xcd_stride),
-                           dim3(2 * kWave), 0, s, wz, du, sk, lk, xcd_stride);
+        hipLaunchKernelGGL((k_backward<T, F, false>), g, b, 0, s, wz, du, sk, lk, xcd_stride);
 }
 
 template <typename T>
@@ -1291,25 +1352,33 @@ void launch_factor_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipS
 }
 template <typename T>
 void launch_forward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s,
-                          bool staged)
+                          bool staged, bool five)
 {
-    if (staged)
-        hipLaunchKernelGGL((k_forward_batch_staged<T, sweep_steps<T>()>), dim3(nblk),
-                           dim3(2 * kWave), 0, s, sys, map);
+    constexpr int F = sweep_steps<T>();
+    const dim3 g(nblk), b(2 * kWave);
+    if (staged && five)
+        hipLaunchKernelGGL((k_forward_batch_staged<T, F, true>), g, b, 0, s, sys, map);
+    else if (staged)
+        hipLaunchKernelGGL((k_forward_batch_staged<T, F, false>), g, b, 0, s, sys, map);
+    else if (five)
+        hipLaunchKernelGGL((k_forward_batch<T, F, true>), g, b, 0, s, sys, map);
     else
-        hipLaunchKernelGGL((k_forward_batch<T, sweep_steps<T>()>), dim3(nblk), dim3(2 * kWave), 0,
-                           s, sys, map);
+        hipLaunchKernelGGL((k_forward_batch<T, F, false>), g, b, 0, s, sys, map);
 }
 template <typename T>
 void launch_backward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s,
-                           bool staged)
+                           bool staged, bool five)
 {
-    if (staged)
-        hipLaunchKernelGGL((k_backward_batch_staged<T, sweep_steps<T>()>), dim3(nblk),
-                           dim3(2 * kWave), 0, s, sys, map);
+    constexpr int F = sweep_steps<T>();
+    const dim3 g(nblk), b(2 * kWave);
+    if (staged && five)
+        hipLaunchKernelGGL((k_backward_batch_staged<T, F, true>), g, b, 0, s, sys, map);
+    else if (staged)
+        hipLaunchKernelGGL((k_backward_batch_staged<T, F, false>), g, b, 0, s, sys, map);
+    else if (five)
+        hipLaunchKernelGGL((k_backward_batch<T, F, true>), g, b, 0, s, sys, map);
     else
-        hipLaunchKernelGGL((k_backward_batch<T, sweep_steps<T>()>), dim3(nblk), dim3(2 * kWave),
-                           0, s, sys, map);
+        hipLaunchKernelGGL((k_backward_batch<T, F, false>), g, b, 0, s, sys, map);
 }
 template <typename T>
 void launch_to_skew_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s)
@@ -1329,6 +1398,13 @@ void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t 
                        nk, sk);
 }
 
+template <typename TI, typename TO>
+void launch_to_skew5(const TI *src, TO *dst, const Skew &sk, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_to_skew5<TI, TO>), dim3(blocks_for(sk.size())), dim3(256), 0, s, src, dst,
+                       sk);
+}
+
 template <typename T>
 void launch_from_skew(const T *src, T *dst, int nk, const Skew &sk, hipStream_t s)
 {
@@ -1339,22 +1415,23 @@ void launch_from_skew(const T *src, T *dst, int nk, const Skew &sk, hipStream_t 
 template <typename T>
 void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_update<T>, dim3(blocks_for((long)sk.nx * sk.ny)), dim3(256), 0, s, x, z,
-                       sk, st);
+    hipLaunchKernelGGL(k_update<T>, dim3((sk.nx + 255) / 256, sk.ny), dim3(256), 0, s, x, z, sk,
+                       st);
 }
 
 #define M2D_INST(T)                                                                           \
     template void launch_factor<T>(const T *, T *, T *, T, const Skew &, Link, int, hipStream_t); \
     template void launch_forward<T>(const T *, const T *, T *, const Skew &, Link, int,          \
-                                    hipStream_t, bool);                                       \
-    template void launch_backward<T>(T *, const T *, const Skew &, Link, int, hipStream_t, bool);\
+                                    hipStream_t, bool, bool);                                 \
+    template void launch_backward<T>(T *, const T *, const Skew &, Link, int, hipStream_t, bool, \
+                                     bool);                                                   \
     template void launch_from_skew<T>(const T *, T *, int, const Skew &, hipStream_t);          \
     template void launch_prepare_factor<T>(T *, T *, const Skew &, hipStream_t);                \
     template void launch_factor_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t);  \
     template void launch_forward_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t,  \
-                                          bool);                                              \
+                                          bool, bool);                                        \
     template void launch_backward_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t, \
-                                           bool);                                             \
+                                           bool, bool);                                       \
     template void launch_to_skew_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t); \
     template void launch_prepare_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t); \
     template void launch_update<T>(T *, const T *, const Skew &, const SolveState *, hipStream_t);
@@ -1364,5 +1441,7 @@ template void launch_to_skew<double, double>(const double *, double *, int, cons
                                              hipStream_t);
 template void launch_to_skew<double, float>(const double *, float *, int, const Skew &,
                                             hipStream_t);
+template void launch_to_skew5<double, double>(const double *, double *, const Skew &, hipStream_t);
+template void launch_to_skew5<double, float>(const double *, float *, const Skew &, hipStream_t);
 
 }  // namespace m2d

@@ -44,7 +44,8 @@ class Status(C.Structure):
 
 # every symbol include/mach2d_hip.h declares
 EXPORTS = [
-    "m2d_lu2d9", "m2d_fb2d9", "m2d_tdma2d9", "m2d_solve_msi9d", "m2d_norm_l1_9d",
+    "m2d_lu2d9", "m2d_fb2d9", "m2d_tdma2d9", "m2d_solve_msi9d",
+    "m2d_lu2d5", "m2d_fb2d5", "m2d_solve_msi5d", "m2d_create_stencil", "m2d_norm_l1_9d",
     "m2d_norm_l1_9d_relative", "m2d_norm_l1_b", "m2d_last_status", "m2d_create",
     "m2d_destroy", "m2d_set_stream", "m2d_factor", "m2d_solve", "m2d_set_inner",
     "m2d_get_factor", "m2d_set_tuning", "m2d_set_staging", "m2d_tdma2d9_device",
@@ -83,14 +84,17 @@ def load():
             "libmach2d_hip.so is not built (%s); run `python mach-2d_amd/build.py` "
             "or __graft_entry__.build()" % LIB_PATH)
     L = C.CDLL(LIB_PATH)
-    L.m2d_lu2d9.argtypes = [_vp, _vp, _vp, _ip, _ip, _ip]
-    L.m2d_lu2d9.restype = None
-    L.m2d_fb2d9.argtypes = [_vp, _vp, _vp, _ip, _ip, _ip, _vp, _dp, _ip, _vp]
-    L.m2d_fb2d9.restype = None
+    for f in (L.m2d_lu2d9, L.m2d_lu2d5):
+        f.argtypes = [_vp, _vp, _vp, _ip, _ip, _ip]
+        f.restype = None
+    for f in (L.m2d_fb2d9, L.m2d_fb2d5):
+        f.argtypes = [_vp, _vp, _vp, _ip, _ip, _ip, _vp, _dp, _ip, _vp]
+        f.restype = None
     L.m2d_tdma2d9.argtypes = [_ip, _ip, _ip, _vp, _vp, _vp]
     L.m2d_tdma2d9.restype = None
-    L.m2d_solve_msi9d.argtypes = [_ip, _ip, _ip, _dp, _vp, _vp, _vp]
-    L.m2d_solve_msi9d.restype = None
+    for f in (L.m2d_solve_msi9d, L.m2d_solve_msi5d):
+        f.argtypes = [_ip, _ip, _ip, _dp, _vp, _vp, _vp]
+        f.restype = None
     for f in (L.m2d_norm_l1_9d, L.m2d_norm_l1_9d_relative):
         f.argtypes = [_ip, _ip, _vp, _vp, _vp, _dp]
         f.restype = None
@@ -100,6 +104,8 @@ def load():
     L.m2d_last_status.restype = C.c_int
     L.m2d_create.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]
     L.m2d_create.restype = C.c_int
+    L.m2d_create_stencil.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]
+    L.m2d_create_stencil.restype = C.c_int
     L.m2d_destroy.argtypes = [_vp]
     L.m2d_destroy.restype = C.c_int
     L.m2d_set_stream.argtypes = [_vp, _vp]

@@ -268,6 +268,49 @@ def pcorr_system(nx: int, ny: int, seed: int = 0, q: float | None = None,
                   m if keep_metrics else None)
 
 
+def diffusion_coefficients5(nx: int, ny: int, m: dict, dominance: float = 1.001) -> np.ndarray:
+    """5-point p' coefficients (5, nx*ny), slots S, W, P, E, N: the
+    orthogonal diffusion terms of the 9-point form (the cross-derivative
+    terms dropped), so A has the 5-diagonal structure of the reference's own
+    p' system (mach2d_coef.f90:1207-1278); a_P = dominance * sum |a_nb|;
+    boundary rows as in diffusion_coefficients, corners identity."""
+    a = np.zeros((5, ny, nx))
+    ge = m["Je"] * m["alphae"]
+    gn = m["Jn"] * m["gamman"]
+    J, I = slice(1, ny - 1), slice(1, nx - 1)
+    Jm, Im = slice(0, ny - 2), slice(0, nx - 2)
+    a[0, J, I] = -np.abs(gn[Jm, I])
+    a[1, J, I] = -np.abs(ge[J, Im])
+    a[3, J, I] = -np.abs(ge[J, I])
+    a[4, J, I] = -np.abs(gn[J, I])
+    a[2, J, I] = dominance * np.sum(np.abs(a[[0, 1, 3, 4], J, I]), axis=0)
+    a[:, :, 0] = 0.0
+    a[2, :, 0] = 1.0
+    a[3, 1:-1, 0] = 1.0                 # west: P + E
+    a[:, :, nx - 1] = 0.0
+    a[2, :, nx - 1] = 1.0
+    a[1, 1:-1, nx - 1] = -1.0           # east: P - W
+    a[:, 0, 1:-1] = 0.0
+    a[2, 0, 1:-1] = 1.0
+    a[4, 0, 1:-1] = -1.0                # south: P - N
+    a[:, ny - 1, 1:-1] = 0.0
+    a[2, ny - 1, 1:-1] = 1.0
+    a[0, ny - 1, 1:-1] = -1.0           # north: P - S
+    return a.reshape(5, nx * ny)
+
+
+def pcorr_system5(nx: int, ny: int, seed: int = 0, q: float | None = None,
+                  aspect: float | None = None, rhs: str = "uniform") -> System:
+    """A seeded 5-diagonal p' system (slots S, W, P, E, N) on the ogive grid;
+    `a` has shape (5, N)."""
+    if q is None:
+        q = ratio_for_aspect(nx, ny, aspect) if aspect else 1.05
+    m = metrics(ogive_grid(nx, ny, q))
+    a = diffusion_coefficients5(nx, ny, m)
+    b = rhs_field(nx, ny, rhs, np.random.default_rng(seed))
+    return System(nx, ny, np.ascontiguousarray(a), b, q, seed)
+
+
 def random_f3_system(nx: int, ny: int, seed: int = 0, dominance: float = 1.5) -> System:
     """Random diagonally dominant 9-diagonal system honouring the zero
     outside-coupling invariant (for tests on odd shapes)."""

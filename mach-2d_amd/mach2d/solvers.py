@@ -3,14 +3,15 @@ backed by the gfx950 C ABI (include/mach2d_hip.h).
 
 Reference interface (src/solvers/):
   * class_solver_abstract%solve(a, b, sol)       mod_solver_abstract.f90:12-31
-  * class_solver_msi9d%init(nx, ny, nitm, tol)   mod_solver_msi.f90:129-148
-    class_solver_msi9d%solve(a, b, sol)          mod_solver_msi.f90:152-166
-  * class_solver_msi5d%init / solve              mod_solver_msi.f90:89-126
-  * class_solver_tdma9d%init(nx, ny, nitm)       mod_solver_tdma.f90:244-268
-  * class_solver_tdma5d%init / solve             mod_solver_tdma.f90:216-240
+  * class_solver_msi9d%init(nx, ny, nitm, tol)   mod_solver_msi.f90:96-115
+    class_solver_msi9d%solve(a, b, sol)          mod_solver_msi.f90:119-133
+  * class_solver_msi5d%init / solve              mod_solver_msi.f90:56-93
+  * class_solver_tdma9d%init(nx, ny, nitm)       mod_solver_tdma.f90:76-100
+  * class_solver_tdma5d%init / solve             mod_solver_tdma.f90:48-72
   * solvers_init(nx, ny, ifile) -> (solver5d, solver9d)   mod_solvers.f90:17-87
   * norm_l1_9d / norm_l1_9d_relative / norm_l1_b          mod_solvers.f90:128-276
   * lu2d9, fb2d9 (msi2d9-v03.f90:277, :42), tdma2d9 (tdma2d9.f90:38)
+  * lu2d5, fb2d5 (msi2d5-v03.f90:247, :42)
 
 Arrays: a coefficient array for an nx*ny grid is either a C-contiguous
 (9, nx*ny) float64 array or the Fortran-ordered (nx*ny, 9) array of the
@@ -19,10 +20,12 @@ reference's intent(inout) dummies.  numpy arrays take the synchronous
 host-array entry points; torch tensors on the GPU take the device-resident
 handle API without host copies.
 
-The 5-diagonal solvers embed the 5-point system (slots S, W, P, E, N) into
-the 9-diagonal slots 2, 4, 5, 6, 8 with zero corners: lu2d9/fb2d9 and
-tdma2d9 on that embedding perform the 5-point reference's operations plus
-exact zero terms, so they reproduce lu2d5/fb2d5 and tdma2d5.
+5-diagonal systems (slots S, W, P, E, N): MSI has its own entry points
+(m2d_lu2d5 / m2d_fb2d5, and handles created with 5 diagonals), whose
+residual and sweeps read the 5 + 4 + 3 arrays of msi2d5; the factorization
+is that of the zero-corner embedding into the 9-diagonal slots 2, 4, 5, 6, 8,
+whose terms reduce to lu2d5's.  TDMA embeds the 5-point system the same way
+(tdma2d9 on it performs tdma2d5's operations plus exact zero terms).
 """
 from __future__ import annotations
 
@@ -104,6 +107,27 @@ def fb2d9(b, dl, du, nj, ni, nij, var, gamma, nitm, rhs):
     return last_status()
 
 
+def lu2d5(b, dl, du, nij, ni, nj):
+    """msi2d5-v03.f90:247 lu2d5(b, dl, du, nij, ni, nj); b (5,nij), dl (4,nij), du (3,nij) out."""
+    bb = _coef(b, nij, 5)
+    _vec(dl, 4 * nij, "dl")
+    _vec(du, 3 * nij, "du")
+    _lib.load().m2d_lu2d5(_p(bb), _p(dl), _p(du), _i(nij), _i(ni), _i(nj))
+    return last_status()
+
+
+def fb2d5(b, dl, du, nj, ni, nij, var, gamma, nitm, rhs):
+    """msi2d5-v03.f90:42 fb2d5(b, dl, du, nj, ni, nij, var, gamma, nitm, rhs); var inout."""
+    bb = _coef(b, nij, 5)
+    dl = _coef(dl, nij, 4)
+    du = _coef(du, nij, 3)
+    _vec(var, nij, "var")
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    _lib.load().m2d_fb2d5(_p(bb), _p(dl), _p(du), _i(nj), _i(ni), _i(nij), _p(var),
+                          _d(gamma), _i(nitm), _p(rhs))
+    return last_status()
+
+
 def tdma2d9(nx, ny, nitm, a, b, x):
     """tdma2d9.f90:38 tdma2d9(nx, ny, nitm, a, b, x); x inout."""
     n = nx * ny
@@ -163,7 +187,7 @@ class ClassSolverAbstract:
 
 
 class ClassSolverMSI9d(ClassSolverAbstract):
-    """MSI for 9-diagonal systems (mod_solver_msi.f90:68-166).  Every solve
+    """MSI for 9-diagonal systems (mod_solver_msi.f90:35-50,96-133).  Every solve
     factors then iterates, as the reference (lu2d9 + fb2d9)."""
 
     def init(self, nx, ny, nitm, tol):
@@ -191,21 +215,32 @@ class ClassSolverMSI9d(ClassSolverAbstract):
 
 
 class ClassSolverMSI5d(ClassSolverAbstract):
-    """MSI for 5-diagonal systems (mod_solver_msi.f90:50-126) through the
-    zero-corner embedding into the 9-diagonal solver."""
+    """MSI for 5-diagonal systems (mod_solver_msi.f90:17-31,56-93): every
+    solve factors then iterates (lu2d5 + fb2d5)."""
 
     def init(self, nx, ny, nitm, tol):
-        self.inner = ClassSolverMSI9d().init(nx, ny, nitm, tol)
-        self.nxy = nx * ny
+        self.nx, self.ny, self.nxy, self.nitm, self.tol = nx, ny, nx * ny, nitm, tol
+        self._dev = None
         return self
 
     def solve(self, a, b, sol):
-        self.status = self.inner.solve(embed5(a, self.nxy), b, sol)
+        if _is_torch(sol):
+            if self._dev is None:
+                self._dev = DeviceMSI(self.nx, self.ny, ndiag=5)
+            self._dev.factor(a)
+            self.status = self._dev.solve(a, b, sol, self.tol, self.nitm)
+            return self.status
+        aa = _coef(a, self.nxy, 5)
+        _vec(sol, self.nxy, "sol")
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        _lib.load().m2d_solve_msi5d(_i(self.nx), _i(self.ny), _i(self.nitm), _d(self.tol),
+                                    _p(aa), _p(b), _p(sol))
+        self.status = last_status()
         return self.status
 
 
 class ClassSolverTDMA9d(ClassSolverAbstract):
-    """mod_solver_tdma.f90:199-268: nitm alternating line sweeps."""
+    """mod_solver_tdma.f90:31-44,76-100: nitm alternating line sweeps."""
 
     def init(self, nx, ny, nitm):
         self.nx, self.ny, self.nitm = nx, ny, nitm
@@ -223,7 +258,7 @@ class ClassSolverTDMA9d(ClassSolverAbstract):
 
 
 class ClassSolverTDMA5d(ClassSolverAbstract):
-    """mod_solver_tdma.f90:184-240 through the zero-corner embedding."""
+    """mod_solver_tdma.f90:17-29,48-72 through the zero-corner embedding."""
 
     def init(self, nx, ny, nitm):
         self.inner = ClassSolverTDMA9d().init(nx, ny, nitm)
@@ -259,15 +294,17 @@ class DeviceMSI:
     """One solver handle for an nx*ny grid on the current GPU.
 
     precision: "fp64" (parity mode) or "fp32_ir" (FP32 factor and sweeps
-    inside FP64 iterative refinement)."""
+    inside FP64 iterative refinement).  ndiag: 9 (msi2d9) or 5 (msi2d5: A
+    holds the slots S, W, P, E, N; the factor is dl 4*N, du 3*N)."""
 
-    def __init__(self, nx, ny, precision="fp64", stream=None):
+    def __init__(self, nx, ny, precision="fp64", stream=None, ndiag=9):
         L = _lib.load()
         self.nx, self.ny, self.N = nx, ny, nx * ny
         self.precision = precision
+        self.nd = int(ndiag)
         prec = {"fp64": _lib.M2D_FP64, "fp32_ir": _lib.M2D_FP32_IR}[precision]
         h = C.c_void_p()
-        rc = L.m2d_create(nx, ny, prec, C.byref(h))
+        rc = L.m2d_create_stencil(nx, ny, prec, self.nd, C.byref(h))
         if rc:
             raise SolverError(rc, "m2d_create")
         self.h = h
@@ -325,21 +362,22 @@ class DeviceMSI:
             raise SolverError(rc, "m2d_set_inner")
 
     def factor(self, a):
-        rc = _lib.load().m2d_factor(self.h, C.c_void_p(self._chk(a, 9 * self.N, "a")))
+        rc = _lib.load().m2d_factor(self.h, C.c_void_p(self._chk(a, self.nd * self.N, "a")))
         if rc and rc != _lib.M2D_ERR_NONFINITE:
             raise SolverError(rc, "m2d_factor")
         return rc
 
     def get_factor(self, dl, du):
-        rc = _lib.load().m2d_get_factor(self.h, C.c_void_p(self._chk(dl, 5 * self.N, "dl")),
-                                        C.c_void_p(self._chk(du, 4 * self.N, "du")))
+        nl, nu = (5, 4) if self.nd == 9 else (4, 3)
+        rc = _lib.load().m2d_get_factor(self.h, C.c_void_p(self._chk(dl, nl * self.N, "dl")),
+                                        C.c_void_p(self._chk(du, nu * self.N, "du")))
         if rc:
             raise SolverError(rc, "m2d_get_factor")
 
     def residual_norm(self, a, rhs, var) -> float:
         """||A var - rhs||_2 in FP64."""
         out = C.c_double()
-        rc = _lib.load().m2d_residual_norm(self.h, C.c_void_p(self._chk(a, 9 * self.N, "a")),
+        rc = _lib.load().m2d_residual_norm(self.h, C.c_void_p(self._chk(a, self.nd * self.N, "a")),
                                            C.c_void_p(self._chk(rhs, self.N, "rhs")),
                                            C.c_void_p(self._chk(var, self.N, "var")),
                                            C.byref(out))
@@ -350,7 +388,7 @@ class DeviceMSI:
     def guess_scale(self, a, rhs, g) -> float:
         """Scale g in place by the residual-minimising alpha; returns alpha."""
         alpha = C.c_double()
-        rc = _lib.load().m2d_guess_scale(self.h, C.c_void_p(self._chk(a, 9 * self.N, "a")),
+        rc = _lib.load().m2d_guess_scale(self.h, C.c_void_p(self._chk(a, self.nd * self.N, "a")),
                                          C.c_void_p(self._chk(rhs, self.N, "rhs")),
                                          C.c_void_p(self._chk(g, self.N, "g")), C.byref(alpha))
         if rc:
@@ -362,7 +400,7 @@ class DeviceMSI:
         hp, cap = (None, 0)
         if hist is not None:
             hp, cap = C.c_void_p(self._chk(hist, hist.numel(), "hist")), hist.numel()
-        _lib.load().m2d_solve(self.h, C.c_void_p(self._chk(a, 9 * self.N, "a")),
+        _lib.load().m2d_solve(self.h, C.c_void_p(self._chk(a, self.nd * self.N, "a")),
                               C.c_void_p(self._chk(rhs, self.N, "rhs")),
                               C.c_void_p(self._chk(var, self.N, "var")), float(gamma),
                               int(nitm), hp, cap, C.byref(st))
@@ -379,7 +417,7 @@ class BatchMSI:
                3: "in order, LDS-staged", 4: "one XCD per system, LDS-staged",
                5: "largest first, LDS-staged"}
 
-    def __init__(self, sizes, precision="fp64"):
+    def __init__(self, sizes, precision="fp64", stream=None):
         L = _lib.load()
         self.sizes = [(int(nx), int(ny)) for nx, ny in sizes]
         self.n = len(self.sizes)
@@ -392,6 +430,8 @@ class BatchMSI:
         if rc:
             raise SolverError(rc, "m2d_batch_create")
         self.h = h
+        if stream is not None:  # hipStream_t as an integer (torch: stream.cuda_stream)
+            L.m2d_batch_set_stream(self.h, C.c_void_p(stream))
 
     def close(self):
         if getattr(self, "h", None):

@@ -32,8 +32,8 @@ def single(mode, solves, staged, inner_gamma=None, inner_nitm=25):
     s = P.pcorr_system(**{k: v for k, v in P.CFG3.items() if k != "gamma"})
     dev = DeviceMSI(s.nx, s.ny, mode)
     dev.set_staging(staged)
-    if inner_gamma is not None and mode == "fp32_ir":
-        dev.set_inner(inner_gamma, inner_nitm)
+    if mode == "fp32_ir":
+        dev.set_inner(1e-4 if inner_gamma is None else inner_gamma, inner_nitm)
     a = torch.tensor(s.a.reshape(-1), device="cuda")
     b = torch.tensor(s.rhs, device="cuda")
     x = torch.zeros(s.N, dtype=torch.float64, device="cuda")
@@ -80,8 +80,8 @@ def batch(mode, count, same, layout, inner_gamma=None, inner_nitm=50):
         rhs = [torch.tensor(x.rhs, device="cuda") for x in systems]
     var = [torch.zeros(nx * ny, dtype=torch.float64, device="cuda") for nx, ny in sizes]
     b = BatchMSI(sizes, precision=mode)
-    if inner_gamma is not None and mode == "fp32_ir":
-        b.set_inner(inner_gamma, inner_nitm)
+    if mode == "fp32_ir":
+        b.set_inner(1e-4 if inner_gamma is None else inner_gamma, inner_nitm)
     b.factor(a)
     if layout is None:
         layout, ms = b.tune(a, rhs, iters=4)
@@ -103,9 +103,69 @@ def batch(mode, count, same, layout, inner_gamma=None, inner_nitm=50):
             "iterations_max": max(x.nit for x in st), "status": rc}
 
 
+def batch_split(mode, count, groups, layout, inner_nitm=50):
+    """`count` copies of the 1026x514 matrix (own right-hand sides) solved as
+    `groups` batches at once, each on its own stream, driven from its own
+    host thread, so one group's sweeps overlap another group's residuals."""
+    import threading
+
+    import torch
+
+    from .solvers import BatchMSI
+    s = P.pcorr_system(**{k: v for k, v in P.CFG3.items() if k != "gamma"})
+    a0 = torch.tensor(s.a.reshape(-1), device="cuda")
+    mask = torch.tensor(s.rhs != 0.0, device="cuda")
+    g = torch.Generator(device="cuda")
+    per = [count // groups + (1 if q < count % groups else 0) for q in range(groups)]
+    jobs = []
+    for q, m in enumerate(per):
+        rhs = []
+        for k in range(m):
+            g.manual_seed(7000 + sum(per[:q]) + k)
+            r = torch.randn(s.N, dtype=torch.float64, device="cuda", generator=g)
+            rhs.append(torch.where(mask, r, torch.zeros_like(r)))
+        st = torch.cuda.Stream()
+        b = BatchMSI([(s.nx, s.ny)] * m, precision=mode, stream=st.cuda_stream)
+        if mode == "fp32_ir":
+            b.set_inner(1e-4, inner_nitm)
+        b.set_tuning(layout, 8)
+        var = [torch.zeros(s.N, dtype=torch.float64, device="cuda") for _ in range(m)]
+        jobs.append((b, [a0] * m, rhs, var))
+    torch.cuda.synchronize()
+    out = [None] * groups
+
+    def run(q):
+        b, a, rhs, var = jobs[q]
+        b.factor(a)
+        out[q] = b.solve(a, rhs, var, GAMMA, NITM_CAP)
+
+    for q in range(groups):  # warm-up (graph capture)
+        run(q)
+    for _, _, _, var in jobs:
+        for v in var:
+            v.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=run, args=(q,)) for q in range(groups)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rc = max(o[0] for o in out)
+    its = max(x.nit for o in out for x in o[1])
+    for b, *_ in jobs:
+        b.close()
+    return {"workload": "batch split", "mode": mode, "count": count, "groups": groups,
+            "layout": layout, "inner_nitm": inner_nitm, "ms": round(dt * 1e3, 3),
+            "solves_per_s": round(count / dt, 2), "iterations_max": its, "status": rc}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["single", "batch"])
+    ap.add_argument("what", choices=["single", "batch", "split"])
+    ap.add_argument("--groups", type=int, default=2)
     ap.add_argument("--mode", default="fp32_ir", choices=["fp64", "fp32_ir"])
     ap.add_argument("--solves", type=int, default=1)
     ap.add_argument("--staged", type=int, default=0)
@@ -118,6 +178,9 @@ def main(argv=None):
     if args.what == "single":
         out = single(args.mode, args.solves, bool(args.staged), args.inner_gamma,
                      args.inner_nitm or 25)
+    elif args.what == "split":
+        out = batch_split(args.mode, args.count, args.groups,
+                          0 if args.layout is None else args.layout, args.inner_nitm or 50)
     else:
         out = batch(args.mode, args.count, args.same, args.layout, args.inner_gamma,
                     args.inner_nitm or 50)

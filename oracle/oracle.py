@@ -258,7 +258,7 @@ def fb2d9(a, dl, du, nx, ny, var, gamma, nitm, rhs, hist_cap=None, max_iter_cap=
 
 
 def solve_msi9d(a, rhs, var, nx, ny, nitm, tol, max_iter_cap=0):
-    """class_solver_msi9d%solve: lu2d9 then fb2d9 (mod_solver_msi.f90:152-166)."""
+    """class_solver_msi9d%solve: lu2d9 then fb2d9 (mod_solver_msi.f90:119-133)."""
     dl, du = lu2d9(a, nx, ny)
     return fb2d9(a, dl, du, nx, ny, var, tol, nitm, rhs, max_iter_cap=max_iter_cap)
 

@@ -62,7 +62,7 @@ contains
       call tdma2d9(nx, ny, nitm, a, b, x)
    end subroutine
 
-   ! src/solvers/mod_solver_msi.f90:129-166: the plugin object, init + solve
+   ! src/solvers/mod_solver_msi.f90:96-133: the plugin object, init + solve
    subroutine ref_solver_msi9d(nx, ny, nitm, tol, a, b, sol) bind(C, name="ref_solver_msi9d")
       integer(c_int), value :: nx, ny, nitm
       real(c_double), value :: tol

@@ -79,7 +79,7 @@ def test_solvers_init_factory():
     cfg = dict(solver="MSI", nitm_u=5, nitm_p=20, tol_u=1e-2, tol_p=1e-2)
     s5, s9 = S.solvers_init(10, 6, cfg)
     assert isinstance(s5, S.ClassSolverMSI5d) and isinstance(s9, S.ClassSolverMSI9d)
-    assert (s9.nitm, s9.tol, s5.inner.nitm) == (5, 1e-2, 20)
+    assert (s9.nitm, s9.tol, s5.nitm, s5.tol) == (5, 1e-2, 20, 1e-2)
     cfg["solver"] = "TDMA"
     s5, s9 = S.solvers_init(10, 6, cfg)
     assert isinstance(s9, S.ClassSolverTDMA9d) and s9.nitm == 5
@@ -110,7 +110,8 @@ def test_fortran_dropin_module_compiles(tmp_path):
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     if not os.path.exists(fc):
         pytest.skip("no Fortran compiler")
-    for name in ("msi2d9_hip.f90", "coefficients_hip.f90", "intflow_bc_hip.f90"):
+    for name in ("msi2d9_hip.f90", "msi2d5_hip.f90", "coefficients_hip.f90",
+                 "intflow_bc_hip.f90"):
         src = os.path.join(ROOT, "mach-2d_amd", "fortran", name)
         r = subprocess.run([fc, "-c", "-module-dir", str(tmp_path), "-o",
                             str(tmp_path / (name + ".o")), src], capture_output=True, text=True)
@@ -209,3 +210,30 @@ def test_fortran_intflow_module_matches_reference_interface():
                  "b": {"intflow_set_bcpl": "bp", "intflow_set_bcT": "bt"}.get(k, "b")}.get(w, w)
                 for w in want]
         assert got == want, k
+
+
+def test_fortran_msi2d5_module_keeps_reference_interface():
+    """msi2d5_hip.f90 exports module msi2d5 with lu2d5 and fb2d5 taking the
+    dummy arguments of src/solvers/msi2d5-v03.f90:42,247 in their order."""
+    import re
+    text = open(os.path.join(ROOT, "mach-2d_amd", "fortran", "msi2d5_hip.f90")).read()
+    assert re.search(r"^module msi2d5\s*$", text, re.M)
+    body = text[text.index("contains"):]
+    sig = {m.group(1): [a.strip() for a in m.group(2).split(",")]
+           for m in re.finditer(r"subroutine (\w+)\((.*?)\)", body)}
+    assert sig["lu2d5"] == ["b", "dl", "du", "nij", "ni", "nj"]
+    assert sig["fb2d5"] == ["b", "dl", "du", "nj", "ni", "nij", "var", "gamma", "nitm", "rhs"]
+
+
+def test_generated_5point_systems():
+    """pcorr_system5: 5 diagonals (S, W, P, E, N), no coupling across the
+    grid edge, strictly dominant interior diagonal, identity corners."""
+    s = P.pcorr_system5(40, 23, seed=2)
+    a = s.a.reshape(5, 23, 40)
+    assert s.a.shape == (5, 40 * 23)
+    assert not a[0, 0].any() and not a[4, -1].any()
+    assert not a[1, :, 0].any() and not a[3, :, -1].any()
+    nb = np.abs(a[[0, 1, 3, 4], 1:-1, 1:-1]).sum(axis=0)
+    assert (a[2, 1:-1, 1:-1] > nb).all()
+    for j, i in ((0, 0), (0, -1), (-1, 0), (-1, -1)):
+        assert a[2, j, i] == 1.0 and not a[[0, 1, 3, 4], j, i].any()

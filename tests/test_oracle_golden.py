@@ -150,3 +150,22 @@ def test_coef_oracle_vs_reference_fixtures(orc, tag):
                 want = float(np.asarray(want).reshape(-1)[0])
             assert np.array_equal(got[k], want), (tag, name, k)
             assert np.all(np.isfinite(want)), (tag, name, k)
+
+
+@pytest.mark.parametrize("nx,ny", [(12, 9), (70, 33), (130, 66)])
+def test_msi5d_embedding_identity_on_generated_systems(orc, nx, ny):
+    """The identity the 5-diagonal GPU factorization relies on: on the
+    generated 5-point p' systems, lu2d9 of the zero-corner embedding gives
+    lu2d5's dl1..dl4 and du1..du3 and exact zeros in dl5 and du4, and
+    fb2d9 on the embedding iterates exactly as fb2d5."""
+    from mach2d import problems as P
+    s = P.pcorr_system5(nx, ny, seed=nx)
+    dl5, du5 = orc.lu2d5(s.a, nx, ny)
+    a9 = np.zeros((9, s.N))
+    a9[[1, 3, 4, 5, 7]] = s.a
+    dl9, du9 = orc.lu2d9(a9, nx, ny)
+    assert np.array_equal(dl9[:4], dl5) and np.array_equal(du9[:3], du5)
+    assert not dl9[4].any() and not du9[3].any()
+    x5, n5, _, h5 = orc.fb2d5(s.a, dl5, du5, nx, ny, np.zeros(s.N), 1e-8, 3000, s.rhs)
+    x9, n9, _, h9 = orc.fb2d9(a9, dl9, du9, nx, ny, np.zeros(s.N), 1e-8, 3000, s.rhs)
+    assert n5 == n9 and np.array_equal(x5, x9) and np.array_equal(h5, h9)
