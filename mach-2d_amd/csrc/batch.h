@@ -24,6 +24,7 @@ struct BatchSys {
     const T *ra;        // A, reference layout, precision T
     const T *rrhs;      // right-hand side, reference layout
     T *rx;              // iterate, reference layout
+    T *rx2;             // its alternate during an MSI run (odd iteration counts)
     double *partial;    // per-block partial sums of r.r (np of them)
     int np;
     int nitm;

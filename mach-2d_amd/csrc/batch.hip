@@ -4,8 +4,8 @@
 // Every system keeps its own arrays (pooled), its own skewed layout,
 // boundary words and solve state; a kernel finds its work item through a map
 // of (system, band) or (system, block) pairs.  An MSI iteration of the batch
-// is [forward sweep, backward sweep, update, residual, norm + stopping rule]
-// over every system that has not stopped yet; systems that stopped turn
+// is [forward sweep, backward sweep, update fused with the residual, norm +
+// stopping rule] over every system that has not stopped yet; systems that stopped turn
 // their part of each launch into a no-op.  Chunks of iterations are captured
 // once into a hipGraph and replayed, with one read of the states per replay.
 //
@@ -86,19 +86,63 @@ __global__ __launch_bounds__(kPts) void k_residual_batch(const BatchSys<T> *__re
     if (threadIdx.x == 0) d.partial[m.y] = red[0];
 }
 
-// x = z + x (msi2d9-v03.f90:181), z skewed.
+// var = z + var (msi2d9-v03.f90:181) fused with the residual after it
+// (:183-255) and its block partial sum.  During an MSI run the iterate
+// alternates between rx (an even number of iterations done) and rx2 (odd):
+// the kernel reads one and writes the other, and forms the updated value of
+// each stencil neighbour where it reads it (z + var, the sum the update
+// stores), so r is bit for bit the residual of the updated iterate.
+// k_settle_batch returns a final iterate left in rx2 to rx.
 template <typename T>
-__global__ __launch_bounds__(kPts) void k_update_batch(const BatchSys<T> *__restrict__ sys,
-                                                       const int2 *__restrict__ map)
+__global__ __launch_bounds__(kPts) void k_update_residual_batch(
+    const BatchSys<T> *__restrict__ sys, const int2 *__restrict__ map)
 {
     const int2 m = map[blockIdx.x];
     if (m.x < 0) return;
     const BatchSys<T> &d = sys[m.x];
     if (d.st->done) return;
-    const int nx = d.sk.nx;
+    const bool odd = d.st->iter & 1;
+    const T *__restrict__ xin = odd ? d.rx2 : d.rx;
+    T *__restrict__ xout = odd ? d.rx : d.rx2;
+    const T *__restrict__ z = d.w;
+    const Skew sk = d.sk;
+    const int nx = sk.nx, ny = sk.ny;
+    const int N = nx * ny;
+    constexpr int S = 64 / (int)sizeof(T);
     const BlockPoint q = block_point(m.y, nx);
-    if (q.p >= nx * d.sk.ny) return;
-    d.rx[q.p] = d.w[d.sk.template of_point_c<64 / (int)sizeof(T)>(q.i, q.j)] + d.rx[q.p];
+    double acc = 0.0;
+    if (q.p < N) {
+        auto xv = [&](int ii, int jj, long f) -> T {
+            return z[sk.template of_point_c<S>(ii, jj)] + xin[f];
+        };
+        const T v = residual_point_f<T, T>(d.ra, N, xv, d.rrhs[q.p], q.i, q.j, nx, ny);
+        xout[q.p] = xv(q.i, q.j, q.p);
+        d.r[sk.template of_point_c<S>(q.i, q.j)] = v;
+        acc = (double)v * (double)v;
+    }
+    __shared__ double red[kPts];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+#pragma unroll
+    for (int o = kPts / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) d.partial[m.y] = red[0];
+}
+
+// After an MSI run: a system that stopped after an odd number of iterations
+// holds its iterate in rx2; copy it to rx.
+template <typename T>
+__global__ __launch_bounds__(kPts) void k_settle_batch(const BatchSys<T> *__restrict__ sys,
+                                                       const int2 *__restrict__ map)
+{
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys<T> &d = sys[m.x];
+    if (!(d.st->iter & 1)) return;
+    const int p = m.y * kPts + (int)threadIdx.x;
+    if (p < d.sk.nx * d.sk.ny) d.rx[p] = d.rx2[p];
 }
 
 // One workgroup per system: ||r|| from the partial sums and the stopping
@@ -461,8 +505,8 @@ void enqueue_iteration(m2d_batch *b)
     const Maps &m = b->maps;
     launch_forward_batch<T>(d, m.fwd, m.nf, b->stream, b->layout >= 3);
     launch_backward_batch<T>(d, m.bwd, m.nb, b->stream, b->layout >= 3);
-    hipLaunchKernelGGL(k_update_batch<T>, dim3(m.np), dim3(kPts), 0, b->stream, d, m.pts);
-    hipLaunchKernelGGL(k_residual_batch<T>, dim3(m.np), dim3(kPts), 0, b->stream, d, m.pts, 1);
+    hipLaunchKernelGGL(k_update_residual_batch<T>, dim3(m.np), dim3(kPts), 0, b->stream, d,
+                       m.pts);
     hipLaunchKernelGGL(k_finalize_batch<T>, dim3(b->n), dim3(256), 0, b->stream, d, 0);
 }
 
@@ -519,6 +563,9 @@ int run_msi(m2d_batch *b, SolveState *st, SolveState *host, const SolveState *ou
         if (all_done(b, host)) break;
         if (rep * b->chunk_iters > 2L * kIterCap) return M2D_ERR_ITER_CAP;  // not reached
     }
+    hipLaunchKernelGGL(k_settle_batch<T>, dim3(b->maps.np), dim3(kPts), 0, b->stream, d,
+                       b->maps.pts);
+    HIP_OK(hipGetLastError());
     return M2D_OK;
 }
 
@@ -628,12 +675,12 @@ extern "C" int m2d_batch_create(int count, const int *nx, const int *ny, int pre
         return M2D_ERR_HIP;
     }
     // skewed pool of the sweep precision: a 9, dl 6, du 4, r, w
-    double *p64 = nullptr, *r64 = nullptr;
+    double *p64 = nullptr, *r64 = nullptr, *alt64 = nullptr;
     float *p32 = nullptr, *ref32 = nullptr;
     if (fp64) {
-        ok = balloc(b, &p64, 21 * L) == hipSuccess;
+        ok = balloc(b, &p64, 21 * L) == hipSuccess && balloc(b, &alt64, NN) == hipSuccess;
     } else {
-        ok = balloc(b, &p32, 21 * L) == hipSuccess && balloc(b, &ref32, 11 * NN) == hipSuccess &&
+        ok = balloc(b, &p32, 21 * L) == hipSuccess && balloc(b, &ref32, 12 * NN) == hipSuccess &&
              balloc(b, &r64, NN) == hipSuccess && balloc(b, &b->d32, count) == hipSuccess;
     }
     if (!ok) {
@@ -664,6 +711,7 @@ extern "C" int m2d_batch_create(int count, const int *nx, const int *ny, int pre
             o.du = base + 15 * Lq;
             o.r = base + 19 * Lq;
             o.w = base + 20 * Lq;
+            o.rx2 = alt64 + ln;
             o.bnd = b->words + lw;
             o.scratch = o.bnd + boundary_words(nx[q], ny[q]);
         } else {
@@ -680,10 +728,11 @@ extern "C" int m2d_batch_create(int count, const int *nx, const int *ny, int pre
             i.bnd = b->words + lw;
             i.scratch = i.bnd + boundary_words(nx[q], ny[q]);
             i.st = b->st_in + q;
-            float *rf = ref32 + 11 * ln;
+            float *rf = ref32 + 12 * ln;
             i.ra = rf;             // FP32 A, 9 diagonals
             i.rrhs = rf + 9 * Nq;  // -r in FP32
             i.rx = rf + 10 * Nq;   // correction c
+            i.rx2 = rf + 11 * Nq;  // its alternate
             i.partial = o.partial;
             i.np = np;
             i.gamma = b->inner_gamma;

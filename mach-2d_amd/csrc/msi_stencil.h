@@ -175,36 +175,64 @@ template <typename TA, typename TX, typename TR>
 M2D_HD TR residual_point5(const TA *a, long ld, const TX *x, TR rhs, int i, int j, int nx, int ny)
 {
     const long p = (long)j * nx + i;
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    // operands read unconditionally (clamped neighbours), absent terms skipped
+    const TR x_s = (TR)x[hs ? p - nx : p], x_w = (TR)x[hw ? p - 1 : p], x_p = (TR)x[p];
+    const TR x_e = (TR)x[he ? p + 1 : p], x_n = (TR)x[hn ? p + nx : p];
+    TR c[5];
+    for (int k = 0; k < 5; ++k) c[k] = (TR)a[k * ld + p];
     TR acc = -rhs;
-    if (j > 0) acc += (TR)a[k5S * ld + p] * (TR)x[p - nx];
-    if (i > 0) acc += (TR)a[k5W * ld + p] * (TR)x[p - 1];
-    acc += (TR)a[k5P * ld + p] * (TR)x[p];
-    if (i < nx - 1) acc += (TR)a[k5E * ld + p] * (TR)x[p + 1];
-    if (j < ny - 1) acc += (TR)a[k5N * ld + p] * (TR)x[p + nx];
+    if (hs) acc += c[k5S] * x_s;
+    if (hw) acc += c[k5W] * x_w;
+    acc += c[k5P] * x_p;
+    if (he) acc += c[k5E] * x_e;
+    if (hn) acc += c[k5N] * x_n;
     return acc;
 }
 
 // Residual r = A.x - rhs at (i, j) (0-based), 2-D aware: a term whose
 // neighbour lies outside the grid is omitted, as in the seven row classes of
 // msi2d9-v03.f90:59-131.  Terms are added in slot order starting from -rhs.
-// `a` points at diagonal 0; `ld` is the diagonal stride (nx*ny).
+// `a` points at diagonal 0; `ld` is the diagonal stride (nx*ny).  The
+// iterate is read through xv(i', j', flat index) of the neighbour.
+template <typename TA, typename TR, typename XV>
+M2D_HD TR residual_point_f(const TA *a, long ld, XV xv, TR rhs, int i, int j, int nx, int ny)
+{
+    const long p = (long)j * nx + i;
+    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
+    // Neighbour coordinates clamped into the grid: every operand is read
+    // unconditionally (so all the loads can be in flight together) and the
+    // term of an absent neighbour is skipped afterwards.
+    const int iw = hw ? i - 1 : i, ie = he ? i + 1 : i;
+    const int js = hs ? j - 1 : j, jn = hn ? j + 1 : j;
+    const long rs = (long)js * nx, rp = (long)j * nx, rn = (long)jn * nx;
+    const TR x_sw = (TR)xv(iw, js, rs + iw), x_s = (TR)xv(i, js, rs + i);
+    const TR x_se = (TR)xv(ie, js, rs + ie), x_w = (TR)xv(iw, j, rp + iw);
+    const TR x_p = (TR)xv(i, j, p), x_e = (TR)xv(ie, j, rp + ie);
+    const TR x_nw = (TR)xv(iw, jn, rn + iw), x_n = (TR)xv(i, jn, rn + i);
+    const TR x_ne = (TR)xv(ie, jn, rn + ie);
+    TR c[9];
+    for (int k = 0; k < 9; ++k) c[k] = (TR)a[k * ld + p];
+    TR acc = -rhs;
+    if (hs && hw) acc += c[kSW] * x_sw;
+    if (hs) acc += c[kS] * x_s;
+    if (hs && he) acc += c[kSE] * x_se;
+    if (hw) acc += c[kW] * x_w;
+    acc += c[kP] * x_p;
+    if (he) acc += c[kE] * x_e;
+    if (hn && hw) acc += c[kNW] * x_nw;
+    if (hn) acc += c[kN] * x_n;
+    if (hn && he) acc += c[kNE] * x_ne;
+    return acc;
+}
+
+// residual_point_f on an iterate array x.
 template <typename TA, typename TX, typename TR>
 M2D_HD TR residual_point(const TA *a, long ld, const TX *x, TR rhs, int i, int j, int nx,
                          int ny)
 {
-    const long p = (long)j * nx + i;
-    const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
-    TR acc = -rhs;
-    if (hs && hw) acc += (TR)a[kSW * ld + p] * (TR)x[p - nx - 1];
-    if (hs) acc += (TR)a[kS * ld + p] * (TR)x[p - nx];
-    if (hs && he) acc += (TR)a[kSE * ld + p] * (TR)x[p - nx + 1];
-    if (hw) acc += (TR)a[kW * ld + p] * (TR)x[p - 1];
-    acc += (TR)a[kP * ld + p] * (TR)x[p];
-    if (he) acc += (TR)a[kE * ld + p] * (TR)x[p + 1];
-    if (hn && hw) acc += (TR)a[kNW * ld + p] * (TR)x[p + nx - 1];
-    if (hn) acc += (TR)a[kN * ld + p] * (TR)x[p + nx];
-    if (hn && he) acc += (TR)a[kNE * ld + p] * (TR)x[p + nx + 1];
-    return acc;
+    return residual_point_f<TA, TR>(
+        a, ld, [x](int, int, long q) { return x[q]; }, rhs, i, j, nx, ny);
 }
 
 // alpha of msi2d9-v03.f90:287,347-351: alpha0 on real volumes, 0 on fictitious.
