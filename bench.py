@@ -16,15 +16,19 @@ its solution is within 1e-8 relative of the FP64 one, else the FP64 figure,
 and speedup_vs_plain_fp64 compares the two.  The batch entry times 256
 independent p' systems of uneven size (64s x 32s, s log-uniform on [1, 8])
 solved together to the same drop (autotuned placement, graph replays);
-batch_1026x514 times 32 systems of the headline size (its matrix, their own
+batch_1026x514 times 48 systems of the headline size (its matrix, their own
 seeded right-hand sides) solved together, with their throughput against
-the single plain-FP64 solve.
+the single plain-FP64 solve; msi2d5_1026x514 times the 5-diagonal p' system
+of the same grid (the reference's own p' form) on the dedicated 5-diagonal
+kernels and, for comparison, on the 9-diagonal ones through the zero-corner
+embedding.
 
 Prints one JSON line.  gbps = algorithmic bytes of the solve over its time
 (FP64: 8 N (18 + 11 + 24 K); refinement: 4 N (18 + 24 K_inner) +
 8 N (11 (outer + 1) + 3 outer)); roofline = the dominant kernel's
 algorithmic bytes per launch over its average launch duration (HIP events on
-the solver's stream around each launch during the timed steps);
+the solver's stream around each launch, in a repeat of the timed steps: the
+event records would lengthen the timed steps themselves);
 cpu_baseline = the reference MSI (compiled Fortran, one core) timed on a
 bounded sample of the same solve on this host.
 """
@@ -56,13 +60,14 @@ BYTES_PER_POINT = {
 
 
 # HBM bytes per launch at 1026x514, measured with rocprofv3 (FETCH_SIZE and
-# WRITE_SIZE in separate runs of this script, scaled by 2.0 and 1.0: the
-# factors a copy of known size reads; profiles/msi_1026x514_hbm_traffic.md).
-# "residual" is the update plus the residual, as in the kernel timing.
+# WRITE_SIZE in separate runs of this script, scaled by 2.2496 and 1.1250:
+# the factors a conversion kernel of known size reads;
+# profiles/msi_1026x514_hbm_traffic.md).  "residual" is the update plus the
+# residual, as in the kernel timing.
 MEASURED_TRAFFIC = {
-    "forward": 48.05e6, "backward": 37.40e6, "residual": 63.02e6 + 16.96e6, "factor": 139.58e6,
-    "forward32": 22.75e6, "backward32": 17.41e6, "residual32": 33.72e6 + 8.53e6,
-    "factor32": 59.41e6,
+    "forward": 48.68e6, "backward": 36.70e6, "residual": 57.18e6 + 19.66e6, "factor": 157.05e6,
+    "forward32": 21.35e6, "backward32": 18.37e6, "residual32": 28.63e6 + 10.25e6,
+    "factor32": 66.85e6,
 }
 
 
@@ -97,12 +102,18 @@ def gpu_run(s, mode, steps, warmup):
     for w in range(warmup):
         st = one()
         log("warmup %d: status %d nit %d (%.2f ms)" % (w, st.status, st.nit, st.device_ms))
-    dev.enable_kernel_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sts = [one() for _ in range(steps)]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    # per-kernel durations: the same steps again with HIP events around every
+    # launch (on the solver's stream); kept out of the timed steps above,
+    # whose wall time the event records would lengthen
+    dev.enable_kernel_timing(True)
+    for _ in range(steps):
+        one()
+    torch.cuda.synchronize()
     kt = dev.kernel_timing()
     dev.enable_kernel_timing(False)
     st = sts[-1]
@@ -166,6 +177,62 @@ def learned_run(s, steps, warmup, x64):
         e["rel_error_vs_fp64"] = rel
     log("learned fp32_ir: %s" % e)
     return e
+
+
+def msi5_run(steps, warmup):
+    """The 5-diagonal p' system (slots S, W, P, E, N) of the same grid and
+    right-hand-side kind: dedicated 5-diagonal handles (msi2d5: 5 + 4 + 3
+    arrays per iteration) in FP64 and FP32 refinement, and the FP64
+    9-diagonal handle on its zero-corner embedding (same iterates)."""
+    import torch
+
+    from mach2d import problems as P
+    from mach2d import solvers as S
+    c = P.CFG3
+    s = P.pcorr_system5(c["nx"], c["ny"], seed=c["seed"], aspect=c["aspect"], rhs=c["rhs"])
+    out, xs = {}, {}
+    for name, prec, nd in (("fp64", "fp64", 5), ("fp32_ir", "fp32_ir", 5),
+                           ("fp64_embedded_9d", "fp64", 9)):
+        a = s.a if nd == 5 else S.embed5(s.a, s.N)
+        dev = S.DeviceMSI(s.nx, s.ny, precision=prec, ndiag=nd)
+        ta = torch.tensor(a, device="cuda")
+        tr = torch.tensor(s.rhs, device="cuda")
+        tv = torch.zeros(s.N, dtype=torch.float64, device="cuda")
+
+        def one():
+            tv.zero_()
+            dev.factor(ta)
+            return dev.solve(ta, tr, tv, GAMMA, NITM_CAP)
+
+        for _ in range(warmup):
+            one()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sts = [one() for _ in range(steps)]
+        torch.cuda.synchronize()
+        per = (time.perf_counter() - t0) / steps
+        st = sts[-1]
+        xs[name] = tv.cpu().numpy()
+        if prec == "fp64":
+            nbytes = 8 * s.N * (12 + 7 + 18 * st.nit) if nd == 5 else \
+                8 * s.N * (18 + 11 + 24 * st.nit)
+        else:
+            nbytes = 4 * s.N * (12 + 18 * st.nit) + 8 * s.N * (7 * (st.outer + 1) + 3 * st.outer)
+        e = {"solves_per_s": round(1.0 / per, 4), "ms_per_solve": round(per * 1e3, 3),
+             "nit": st.nit, "outer": st.outer, "gbps": round(nbytes / per / 1e9, 2),
+             "correct": bool(all(x.status == 0 and x.rn < x.gammarp for x in sts))}
+        out[name] = e
+        dev.close()
+    out["fp32_ir"]["rel_error_vs_fp64"] = float(
+        np.linalg.norm(xs["fp32_ir"] - xs["fp64"]) / np.linalg.norm(xs["fp64"]))
+    out["fp32_ir"]["correct"] = bool(out["fp32_ir"]["correct"] and
+                                     out["fp32_ir"]["rel_error_vs_fp64"] <= 1e-8)
+    out["fp64_embedded_9d"]["same_iterate"] = bool(np.array_equal(xs["fp64"],
+                                                                  xs["fp64_embedded_9d"]))
+    out["speedup_5d_vs_embedded_9d"] = round(out["fp64"]["solves_per_s"] /
+                                             out["fp64_embedded_9d"]["solves_per_s"], 3)
+    log("msi2d5 1026x514: %s" % out)
+    return out
 
 
 def cpu_baseline(s, nit, budget_s=15.0):
@@ -312,21 +379,23 @@ def batch_run(mode, steps, warmup, count=256, same=None, learned=False):
 
 
 def target_comparison(summary, batch3):
-    """The 1026x514 solve with learned guess + FP32/FP64 refinement + batching
-    against plain FP64 MSI (one system at a time), in solves/s."""
+    """The 1026x514 solve batched with FP32/FP64 refinement, with and without
+    the learned initial guess, against plain FP64 MSI (one system at a
+    time), in solves/s."""
     if not batch3 or "fp64" not in summary:
         return None
     plain = summary["fp64"]["solves_per_s"]
-    best = None
+    paths = {}
     for k in ("learned_fp32_ir", "fp32_ir"):
         v = batch3.get(k)
         if v and v.get("correct"):
-            best = (k, v["solves_per_s"])
-            break
-    if best is None:
+            paths["batch_1026x514." + k] = {"solves_per_s": v["solves_per_s"],
+                                            "speedup": round(v["solves_per_s"] / plain, 2)}
+    if not paths:
         return None
-    return {"plain_fp64_solves_per_s": plain, "path": "batch_1026x514." + best[0],
-            "solves_per_s": best[1], "speedup": round(best[1] / plain, 2), "target": 20.0}
+    best = max(paths, key=lambda k: paths[k]["solves_per_s"])
+    return {"plain_fp64_solves_per_s": plain, "paths": paths, "best": best,
+            "speedup": paths[best]["speedup"], "target": 20.0}
 
 
 def main():
@@ -337,7 +406,7 @@ def main():
     ap.add_argument("--mode", default="best", choices=["best", "fp64", "fp32_ir"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--batch-cfg3", type=int, default=32,
+    ap.add_argument("--batch-cfg3", type=int, default=48,
                     help="systems in the batch of 1026x514 p' systems")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -364,6 +433,7 @@ def main():
         lr = learned_run(s, args.steps, args.warmup, x64)
         if lr is not None:
             summary["learned_fp32_ir"] = lr
+    msi5 = msi5_run(args.steps, args.warmup) if args.mode == "best" else None
     head = "fp32_ir" if "fp32_ir" in summary and summary["fp32_ir"]["correct"] else modes[0]
     r, e = runs[head], summary[head]
     st = r["st"]
@@ -414,6 +484,7 @@ def main():
         "cpu_baseline": cpu,
         "batch": batch,
         "batch_1026x514": batch3,
+        "msi2d5_1026x514": msi5,
         "target_comparison": target_comparison(summary, batch3),
     }
     print(json.dumps(out), flush=True)
