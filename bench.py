@@ -126,9 +126,9 @@ def gpu_run(s, mode, steps, warmup):
 
 
 def learned_run(s, steps, warmup, x64):
-    """FP32-refined MSI from the U-Net guess (inference, residual-minimising
-    amplitude, zero-guess fallback) to the zero guess's threshold; timed
-    including the inference."""
+    """FP32-refined MSI from the U-Net guess (inference, energy-minimising
+    amplitude) to the zero guess's threshold; timed including the
+    inference."""
     import torch
 
     from mach2d import learned as Lr
@@ -344,16 +344,9 @@ def batch_run(mode, steps, warmup, count=256, same=None, learned=False):
         if net is None:
             return b.solve(a, rhs, var, GAMMA, NITM_CAP)
         gam = []
-        for q in range(count):   # guess, residual-minimising amplitude, fallback to zero
+        for q in range(count):   # guess, energy-minimising amplitude; stop at GAMMA ||b||
             nx, ny = sizes[q]
-            rb = helper.residual_norm(a[q], rhs[q], var[q])
-            net.guess(a[q], rhs[q], nx, ny, var[q])
-            helper.guess_scale(a[q], rhs[q], var[q])
-            rg = helper.residual_norm(a[q], rhs[q], var[q])
-            used[q] = bool(np.isfinite(rg) and rg < rb)
-            if not used[q]:
-                var[q].zero_()
-                rg = rb
+            used[q], rg, rb = Lr.prepare_guess(helper, net, a[q], rhs[q], var[q], nx, ny)
             gam.append(GAMMA * rb / rg)
         return b.solve(a, rhs, var, gam, NITM_CAP)
 

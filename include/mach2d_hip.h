@@ -174,6 +174,13 @@ int m2d_residual_norm(m2d_system *s, const double *d_a, const double *d_rhs,
 int m2d_guess_scale(m2d_system *s, const double *d_a, const double *d_rhs, double *d_g,
                     double *alpha_out);
 
+/* As m2d_guess_scale with alpha = (g . rhs) / (g . A g) (0 when g.A g <= 0):
+ * the amplitude that minimises the energy functional of A (for a symmetric
+ * positive definite A, the A-norm of the error), which weighs the smooth
+ * error components MSI removes slowest.  No reference counterpart. */
+int m2d_guess_scale_energy(m2d_system *s, const double *d_a, const double *d_rhs, double *d_g,
+                           double *alpha_out);
+
 /* Device copies of the factor (dl 5*nx*ny, du 4*nx*ny doubles, reference
  * layout); FP64 handles only. */
 int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du);

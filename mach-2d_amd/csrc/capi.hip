@@ -539,8 +539,8 @@ extern "C" int m2d_residual_norm(m2d_system *s, const double *d_a, const double 
     return M2D_OK;
 }
 
-// Per-block partial sums of (A g).b and (A g).(A g), blocks of 256 points in
-// index order (the host adds them in block order).
+// Per-block partial sums of (A g).b, (A g).(A g), g.b and g.(A g), blocks of
+// 256 points in index order (the host adds them in block order).
 __global__ __launch_bounds__(256) void k_guess_dots(const double *__restrict__ a,
                                                     const double *__restrict__ g,
                                                     const double *__restrict__ b, int nx, int ny,
@@ -548,7 +548,7 @@ __global__ __launch_bounds__(256) void k_guess_dots(const double *__restrict__ a
 {
     const long N = (long)nx * ny;
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
-    double ab = 0.0, aa = 0.0;
+    double ab = 0.0, aa = 0.0, gb = 0.0, ga = 0.0;
     if (p < N) {
         const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
         const double v = nd == 5
@@ -556,22 +556,22 @@ __global__ __launch_bounds__(256) void k_guess_dots(const double *__restrict__ a
                              : residual_point<double, double, double>(a, N, g, 0.0, i, j, nx, ny);
         ab = v * b[p];
         aa = v * v;
+        gb = g[p] * b[p];
+        ga = g[p] * v;
     }
-    __shared__ double sab[256], saa[256];
-    sab[threadIdx.x] = ab;
-    saa[threadIdx.x] = aa;
+    __shared__ double red[4][256];
+    red[0][threadIdx.x] = ab;
+    red[1][threadIdx.x] = aa;
+    red[2][threadIdx.x] = gb;
+    red[3][threadIdx.x] = ga;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            sab[threadIdx.x] += sab[threadIdx.x + w];
-            saa[threadIdx.x] += saa[threadIdx.x + w];
-        }
+        if (threadIdx.x < w)
+            for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = sab[0];
-        part[2 * blockIdx.x + 1] = saa[0];
-    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) part[4 * blockIdx.x + k] = red[k][0];
 }
 
 __global__ void k_scale(double *__restrict__ g, long n, double alpha)
@@ -580,28 +580,44 @@ __global__ void k_scale(double *__restrict__ g, long n, double alpha)
     if (q < n) g[q] = alpha * g[q];
 }
 
-extern "C" int m2d_guess_scale(m2d_system *s, const double *d_a, const double *d_rhs,
-                               double *d_g, double *alpha_out)
+// Scale g in place: energy = 0 by the residual-minimising alpha
+// (A g . rhs) / (A g . A g), energy = 1 by the alpha that minimises the
+// energy functional (alpha g).A(alpha g)/2 - (alpha g).rhs, i.e.
+// (g . rhs) / (g . A g), which weighs the error's smooth components (those
+// MSI removes slowest) where the residual norm weighs the oscillatory ones.
+static int guess_scale(m2d_system *s, const double *d_a, const double *d_rhs, double *d_g,
+                       double *alpha_out, int energy)
 {
     if (!s || !d_a || !d_rhs || !d_g) return M2D_ERR_ARG;
     const long N = s->N;
     const unsigned nblk = (unsigned)((N + 255) / 256);
     hipLaunchKernelGGL(k_guess_dots, dim3(nblk), dim3(256), 0, s->stream, d_a, d_g, d_rhs, s->nx,
                        s->ny, s->r64, s->nd);
-    std::vector<double> part(2 * (size_t)nblk);
+    std::vector<double> part(4 * (size_t)nblk);
     HIP_OK(hipMemcpyAsync(part.data(), s->r64, sizeof(double) * part.size(),
                           hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
-    double ab = 0.0, aa = 0.0;
-    for (unsigned k = 0; k < nblk; ++k) {
-        ab += part[2 * k];
-        aa += part[2 * k + 1];
-    }
-    const double alpha = aa > 0.0 && isfinite(ab) && isfinite(aa) ? ab / aa : 0.0;
+    double d[4] = {0.0, 0.0, 0.0, 0.0};
+    for (unsigned k = 0; k < nblk; ++k)
+        for (int q = 0; q < 4; ++q) d[q] += part[4 * k + q];
+    const double num = energy ? d[2] : d[0], den = energy ? d[3] : d[1];
+    const double alpha = den > 0.0 && isfinite(num) && isfinite(den) ? num / den : 0.0;
     hipLaunchKernelGGL(k_scale, dim3(nblk), dim3(256), 0, s->stream, d_g, N, alpha);
     HIP_OK(hipGetLastError());
     if (alpha_out) *alpha_out = alpha;
     return M2D_OK;
+}
+
+extern "C" int m2d_guess_scale(m2d_system *s, const double *d_a, const double *d_rhs,
+                               double *d_g, double *alpha_out)
+{
+    return guess_scale(s, d_a, d_rhs, d_g, alpha_out, 0);
+}
+
+extern "C" int m2d_guess_scale_energy(m2d_system *s, const double *d_a, const double *d_rhs,
+                                      double *d_g, double *alpha_out)
+{
+    return guess_scale(s, d_a, d_rhs, d_g, alpha_out, 1);
 }
 
 extern "C" int m2d_solve(m2d_system *s, const double *d_a, const double *d_rhs, double *d_var,

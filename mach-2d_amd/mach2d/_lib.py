@@ -49,7 +49,7 @@ EXPORTS = [
     "m2d_norm_l1_9d_relative", "m2d_norm_l1_b", "m2d_last_status", "m2d_create",
     "m2d_destroy", "m2d_set_stream", "m2d_factor", "m2d_solve", "m2d_set_inner",
     "m2d_get_factor", "m2d_set_tuning", "m2d_set_staging", "m2d_tdma2d9_device",
-    "m2d_residual_norm", "m2d_guess_scale",
+    "m2d_residual_norm", "m2d_guess_scale", "m2d_guess_scale_energy",
     "m2d_version", "m2d_enable_kernel_timing", "m2d_kernel_timing",
     "m2d_batch_create", "m2d_batch_destroy", "m2d_batch_set_stream", "m2d_batch_factor",
     "m2d_batch_solve", "m2d_batch_solve_each", "m2d_batch_set_inner", "m2d_batch_set_tuning", "m2d_batch_get_tuning",
@@ -118,8 +118,9 @@ def load():
     L.m2d_set_inner.argtypes = [_vp, C.c_double, C.c_int]
     L.m2d_set_inner.restype = C.c_int
     L.m2d_residual_norm.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double)]
-    L.m2d_guess_scale.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double)]
-    L.m2d_guess_scale.restype = C.c_int
+    for f in (L.m2d_guess_scale, L.m2d_guess_scale_energy):
+        f.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double)]
+        f.restype = C.c_int
     L.m2d_set_staging.argtypes = [_vp, C.c_int]
     L.m2d_set_staging.restype = C.c_int
     L.m2d_residual_norm.restype = C.c_int

@@ -11,11 +11,14 @@ random modes with white noise; each target is the system's FP64 MSI solution
 to a 1e-10 residual drop, solved in batches by the batched solver.
 
 The network predicts the shape of p' (targets are scaled to unit RMS); the
-guessed solve takes the amplitude that minimises the FP64 residual
-(m2d_guess_scale) and judges the result by its residual norm: if
-||A g - b|| >= ||b|| (no better than the zero guess) it starts from zero.
-It stops when ||r|| < gamma ||b||, the zero guess's threshold, so a guess is
-never asked for more accuracy than the zero-guess solve delivers.
+guessed solve takes the amplitude that minimises the energy functional of A,
+alpha = (g.b)/(g.A g) (m2d_guess_scale_energy): it weighs the smooth error
+components, the ones MSI removes slowest, where the residual-minimising
+amplitude (m2d_guess_scale) is dominated by the oscillatory part of the
+residual and shrinks a smooth guess towards zero.  A guess with g.A g <= 0
+gets alpha = 0, i.e. the zero guess.  The solve stops when ||r|| <
+gamma ||b||, the zero guess's threshold, so a guess is never asked for more
+accuracy than the zero-guess solve delivers.
 """
 from __future__ import annotations
 
@@ -123,20 +126,26 @@ def train(net, X, T, M, epochs=4, batch=16, lr=1e-3, seed=0, log=print):
     return steps
 
 
-def guessed_solve(dev, net, a, rhs, var, nx, ny, gamma=1e-10, nitm=20000):
-    """MSI (handle `dev`, factored) from the network's guess, judged by the
-    FP64 residual norm against the zero guess; stops at gamma ||rhs||.
-    Returns (status, used_guess, ||r(guess)||, ||rhs||)."""
+def prepare_guess(dev, net, a, rhs, var, nx, ny):
+    """The network's guess in `var`, scaled by the energy-minimising
+    amplitude (handle `dev`, FP64 residual norms).  Returns (used,
+    ||A var - rhs||, ||rhs||); used is False when the amplitude is 0 (var is
+    then the zero guess)."""
     import torch
-    zero = torch.zeros_like(var)
-    rb = dev.residual_norm(a, rhs, zero)
+    rb = float(torch.linalg.norm(rhs))
     net.guess(a, rhs, nx, ny, var)
-    dev.guess_scale(a, rhs, var)   # residual-minimising amplitude of the predicted shape
-    rg = dev.residual_norm(a, rhs, var)
-    used = bool(np.isfinite(rg) and rg < rb)
+    alpha = dev.guess_scale(a, rhs, var, energy=True)
+    used = bool(np.isfinite(alpha) and alpha != 0.0)
     if not used:
         var.zero_()
-        rg = rb
+        return False, rb, rb
+    return True, dev.residual_norm(a, rhs, var), rb
+
+
+def guessed_solve(dev, net, a, rhs, var, nx, ny, gamma=1e-10, nitm=20000):
+    """MSI (handle `dev`, factored) from the network's guess, stopping at
+    gamma ||rhs||.  Returns (status, used_guess, ||r(guess)||, ||rhs||)."""
+    used, rg, rb = prepare_guess(dev, net, a, rhs, var, nx, ny)
     st = dev.solve(a, rhs, var, gamma * rb / rg, nitm)
     return st, used, rg, rb
 

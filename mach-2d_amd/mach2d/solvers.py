@@ -385,12 +385,14 @@ class DeviceMSI:
             raise SolverError(rc, "m2d_residual_norm")
         return out.value
 
-    def guess_scale(self, a, rhs, g) -> float:
-        """Scale g in place by the residual-minimising alpha; returns alpha."""
+    def guess_scale(self, a, rhs, g, energy=False) -> float:
+        """Scale g in place by the residual-minimising alpha (energy: the
+        energy-minimising one, (g.rhs)/(g.A g)); returns alpha."""
         alpha = C.c_double()
-        rc = _lib.load().m2d_guess_scale(self.h, C.c_void_p(self._chk(a, self.nd * self.N, "a")),
-                                         C.c_void_p(self._chk(rhs, self.N, "rhs")),
-                                         C.c_void_p(self._chk(g, self.N, "g")), C.byref(alpha))
+        fn = _lib.load().m2d_guess_scale_energy if energy else _lib.load().m2d_guess_scale
+        rc = fn(self.h, C.c_void_p(self._chk(a, self.nd * self.N, "a")),
+                C.c_void_p(self._chk(rhs, self.N, "rhs")), C.c_void_p(self._chk(g, self.N, "g")),
+                C.byref(alpha))
         if rc:
             raise SolverError(rc, "m2d_guess_scale")
         return alpha.value

@@ -302,10 +302,43 @@ def test_guess_scale_minimises_residual(gpu):
     dev.close()
 
 
+def test_guess_scale_energy(gpu):
+    """m2d_guess_scale_energy: alpha = (g.b)/(g.Ag); the exact solution
+    scales by 1, a doubled one by 0.5; for an arbitrary guess the scaled
+    one satisfies the first-order condition g.(A(alpha g) - b) = 0, and a
+    guess with g.Ag <= 0 becomes zero."""
+    import torch
+
+    from mach2d import problems as P
+    s = P.pcorr_system(34, 18, seed=2, q=1.05, rhs="gauss")
+    dev = gpu.DeviceMSI(s.nx, s.ny, "fp64")
+    a = torch.tensor(s.a.reshape(-1), device="cuda")
+    b = torch.tensor(s.rhs, device="cuda")
+    A = np.zeros((s.N, s.N))
+    nx = s.nx
+    off = [-nx - 1, -nx, -nx + 1, -1, 0, 1, nx - 1, nx, nx + 1]
+    for k, o in enumerate(off):
+        for p in range(s.N):
+            if 0 <= p + o < s.N and s.a[k, p] != 0.0:
+                A[p, p + o] = s.a[k, p]
+    x = np.linalg.solve(A, s.rhs)
+    for mult, want in ((1.0, 1.0), (2.0, 0.5)):
+        g = torch.tensor(mult * x, device="cuda")
+        alpha = dev.guess_scale(a, b, g, energy=True)
+        assert abs(alpha - want) < 1e-10, (mult, alpha)
+    g0 = np.random.default_rng(4).standard_normal(s.N)
+    alpha = dev.guess_scale(a, b, torch.tensor(g0, device="cuda"), energy=True)
+    assert abs(alpha - (g0 @ s.rhs) / (g0 @ (A @ g0))) < 1e-12 * abs(alpha)
+    assert abs(g0 @ (A @ (alpha * g0) - s.rhs)) < 1e-9 * np.linalg.norm(g0) * np.linalg.norm(s.rhs)
+    g = torch.zeros(s.N, dtype=torch.float64, device="cuda")
+    assert dev.guess_scale(a, b, g, energy=True) == 0.0
+    dev.close()
+
+
 def test_guessed_solve_matches_zero_guess_solution(gpu):
-    """Guess-then-MSI with an untrained network: the guess is scaled or
-    dropped, the solve reaches gamma * ||b||, and the solution agrees with the
-    zero-guess solution to 1e-8 relative."""
+    """Guess-then-MSI with an untrained network: the guess is scaled by the
+    energy-minimising amplitude, the solve reaches gamma * ||b||, and the
+    solution agrees with the zero-guess solution to 1e-8 relative."""
     import torch
 
     from mach2d import learned as Lr
@@ -322,7 +355,6 @@ def test_guessed_solve_matches_zero_guess_solution(gpu):
     v1 = torch.zeros_like(v0)
     st1, used, rg, rb = Lr.guessed_solve(dev, net, a, b, v1, s.nx, s.ny)
     assert st0.status == 0 and st1.status == 0
-    assert rg <= rb
     assert st1.rn < 1e-10 * rb * (1 + 1e-9)
     rel = float(torch.linalg.norm(v1 - v0) / torch.linalg.norm(v0))
     assert rel < 1e-8, rel
