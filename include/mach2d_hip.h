@@ -399,6 +399,48 @@ void m2d_intflow_set_bcpl(const int *nx, const int *ny, const double *x, const d
 void m2d_intflow_set_bcT(const int *nx, const int *ny, const double *Twall, const double *x,
                          const double *xp, const double *T, const double *Tin, double *at,
                          double *bt);
+/* intflow_velocities_at_boundary_faces, :692-787 (in/out: only the boundary
+ * faces are written; modvis = 0 gives the north face the Euler slip
+ * velocity) */
+void m2d_intflow_velocities_at_boundary_faces(const int *nx, const int *ny, const int *modvis,
+                                              const double *x, const double *xp,
+                                              const double *xe, const double *ye,
+                                              const double *xk, const double *yk,
+                                              const double *u, const double *v, double *ue,
+                                              double *ve, double *un, double *vn, double *Uce,
+                                              double *Vcn);
+
+/* Device arrays of the internal-flow pressure-correction loop: N-vectors in
+ * the reference layout (N = nx*ny), plin of ny values. */
+typedef struct m2d_pl_loop_fields {
+    /* grid and metrics */
+    const double *x, *xp, *xe, *ye, *xk, *yk, *rp, *re, *rn, *Jp;
+    /* fields frozen during the loop: SIMPLEC coefficients, g = 1/(R T),
+     * density of the previous time step, nodal velocities, inlet pl */
+    const double *due, *dve, *dun, *dvn, *de, *dn, *g, *roa, *u, *v, *plin;
+    /* updated in place */
+    double *ue, *ve, *un, *vn, *Uce, *Vcn, *roe, *ron, *ro, *p, *pl;
+    /* the p' system, persistent as in the reference: ap (5 N, slots S, W, P,
+     * E, N) and bp (N); the corner rows keep their initial values (the
+     * reference allocates ap = 0 with ap(:,3) = 1, mod_mach2d_data.f90:
+     * 401-403) */
+    double *ap, *bp;
+} m2d_pl_loop_fields;
+
+/* The pressure-correction loop of one SIMPLEC iteration of the internal-flow
+ * case, src/mach2d_main.f90:257-309: pl = 0, then for itp = 1..itpmax:
+ * get_p_coefficients, intflow_set_bcpl, get_p_source, the 5-diagonal MSI
+ * solve (lu2d5 + fb2d5 from the current pl, tol_p, nitm_p),
+ * get_velocity_correction_at_faces_with_pl,
+ * intflow_velocities_at_boundary_faces, get_pressure_correction_with_pl,
+ * get_density_correction_with_pl, get_density_at_faces, and the exit test
+ * max|pl| / p_avg < tolm.  Everything runs on the handle's stream on device
+ * arrays; the host reads one scalar and the solve state per itp (the exit
+ * test).  s5: a 5-diagonal FP64 handle (m2d_create_stencil) of the grid.
+ * itp_out: itp iterations run; normpl_out: the last max|pl| / p_avg. */
+int m2d_intflow_pl_loop(m2d_system *s5, const m2d_pl_loop_fields *f, int modvis, double dt,
+                        double beta, double p_avg, double tolm, int itpmax, int nitm_p,
+                        double tol_p, int *itp_out, double *normpl_out);
 
 /* ---- 5. Learned initial guess: U-Net kernels (device pointers) ---------- */
 /* No reference counterpart (the reference starts MSI from the caller's var).

@@ -961,6 +961,47 @@ __global__ __launch_bounds__(kTx) void k_intflow_bc_T(Fields F)
     F.out[1][np] = b;
 }
 
+// intflow_velocities_at_boundary_faces: :692-787.  One thread per boundary
+// face: west (i = 1) and east (i = nx-1) faces of rows 2..ny-1, south (j = 1)
+// and north (j = ny-1) faces of columns 2..nx-1; the north face takes the
+// Euler slip velocity (modvis = 0) or zero.  in: x, xp, xe, ye, xk, yk, u, v.
+// in/out: ue, ve, un, vn, Uce, Vcn.
+__global__ __launch_bounds__(kTx) void k_intflow_vel_bf(Fields F)
+{
+    const int nx = F.nx, ny = F.ny;
+    const long q = (long)blockIdx.x * kTx + threadIdx.x;
+    const long nr = ny - 2, nc = nx - 2;
+    const double *x = F.in[0], *xp = F.in[1], *xe = F.in[2], *ye = F.in[3], *xk = F.in[4],
+                 *yk = F.in[5], *u = F.in[6], *v = F.in[7];
+    double *ue = F.out[0], *ve = F.out[1], *un = F.out[2], *vn = F.out[3], *Uce = F.out[4],
+           *Vcn = F.out[5];
+    if (q < 2 * nr) {  // west, then east
+        const bool east = q >= nr;
+        const int j = (int)(east ? q - nr : q) + 2, i = east ? nx - 1 : 1;
+        const long np = (long)nx * (j - 1) + i - 1, npe = np + 1;
+        const double uf = (u[np] + u[npe]) / 2.0;
+        const double vf = east ? (v[np] + v[npe]) / 2.0 : 0.0;
+        ue[np] = uf;
+        ve[np] = vf;
+        Uce[np] = uf * ye[np] - vf * xe[np];
+    } else if (q < 2 * nr + 2 * nc) {  // south, then north
+        const long r = q - 2 * nr;
+        const bool north = r >= nc;
+        const int i = (int)(north ? r - nc : r) + 2, j = north ? ny - 1 : 1;
+        const long np = (long)nx * (j - 1) + i - 1;
+        if (north) {
+            un[np] = F.modvis == 0 ? slip_at<0>(xk, yk, u, v, np) : 0.0;
+            vn[np] = F.modvis == 0 ? slip_at<1>(xk, yk, u, v, np) : 0.0;
+        } else {
+            un[np] = (u[np] + u[np + nx]) / 2.0;
+            vn[np] = 0.0;
+        }
+        Vcn[np] = 0.0;
+    }
+    (void)x;
+    (void)xp;
+}
+
 }  // namespace m2d
 
 using namespace m2d;
@@ -1097,6 +1138,8 @@ const Routine kBcU = {k_intflow_bc_uv<0>, 6, {-1, -1}, 2, {9, 1}, kBoundary, nul
 const Routine kBcV = {k_intflow_bc_uv<1>, 6, {-1, -1}, 2, {9, 1}, kBoundary, nullptr};
 const Routine kBcPl = {k_intflow_bc_pl, 4, {-1, -1}, 2, {5, 1}, kBoundary, nullptr, 4};
 const Routine kBcT = {k_intflow_bc_T, 4, {-1, -1}, 2, {9, 1}, kBoundary, nullptr, 4};
+// the boundary grid (2 nx + 2 (ny-2) threads) covers its 2 (nx-2) + 2 (ny-2) faces
+const Routine kVelBf = {k_intflow_vel_bf, 8, {-1, -1}, 6, {1, 1, 1, 1, 1, 1}, kBoundary, nullptr};
 
 Fields fields(const int *nx, const int *ny)
 {
@@ -1128,6 +1171,14 @@ void m2d_set_last_status_code(int status);
     } while (0)
 
 extern "C" void m2d_assembly_set_stream(void *stream) { g_stream = (hipStream_t)stream; }
+
+// Stream of this thread's device-array assembly calls; returns the previous one.
+hipStream_t m2d::assembly_stream(hipStream_t s)
+{
+    const hipStream_t old = g_stream;
+    g_stream = s;
+    return old;
+}
 
 extern "C" void m2d_get_density_at_nodes(const int *nx, const int *ny, const double *Rg,
                                          const double *p, const double *T, double *ro)
@@ -1487,4 +1538,19 @@ extern "C" void m2d_intflow_set_bcT(const int *nx, const int *ny, const double *
     F.out[0] = at;
     F.out[1] = bt;
     FINISH(kBcT);
+}
+
+extern "C" void m2d_intflow_velocities_at_boundary_faces(
+    const int *nx, const int *ny, const int *modvis, const double *x, const double *xp,
+    const double *xe, const double *ye, const double *xk, const double *yk, const double *u,
+    const double *v, double *ue, double *ve, double *un, double *vn, double *Uce, double *Vcn)
+{
+    CHECK_SCALARS(nx, ny, modvis);
+    Fields F = fields(nx, ny);
+    F.modvis = *modvis;
+    const double *in[] = {x, xp, xe, ye, xk, yk, u, v};
+    for (int k = 0; k < 8; ++k) IN(k, in[k]);
+    double *out[] = {ue, ve, un, vn, Uce, Vcn};
+    for (int k = 0; k < 6; ++k) F.out[k] = out[k];
+    FINISH(kVelBf);
 }

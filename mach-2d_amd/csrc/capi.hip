@@ -631,6 +631,133 @@ extern "C" int m2d_solve(m2d_system *s, const double *d_a, const double *d_rhs, 
 }
 
 // ---------------------------------------------------------------------------
+// Device-resident pressure-correction loop of the internal-flow case
+// ---------------------------------------------------------------------------
+// max |x| over n values into out[0] (one workgroup, fixed reduction order).
+__global__ __launch_bounds__(1024) void k_absmax(const double *__restrict__ x, long n,
+                                                 double *__restrict__ out)
+{
+    __shared__ double red[1024];
+    double m = 0.0;
+    for (long q = threadIdx.x; q < n; q += 1024) m = fmax(m, fabs(x[q]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// lu2d5 / lu2d9 enqueued without reading the state (errors land in it).
+static void enqueue_factor64(m2d_system *s, const double *d_a)
+{
+    Work<double> &W = s->w64;
+    begin(s, s->st);
+    launch_check_outside(d_a, s->nx, s->ny, s->st, s->stream, s->nd);
+    if (s->nd == 5)
+        launch_to_skew5<double, double>(d_a, W.a, W.sk, s->stream);
+    else
+        launch_to_skew<double, double>(d_a, W.a, 9, W.sk, s->stream);
+    launch_factor<double>(W.a, W.dl, W.du, 0.5, W.sk, make_link(s, s->st, 0), s->xcd_stride,
+                          s->stream);
+    launch_prepare_factor<double>(W.dl, W.du, W.sk, s->stream);
+}
+
+// fb2d5 / fb2d9 with at most nitm iterations, enqueued without host reads:
+// the device stopping rule turns the iterations after it into no-ops.
+static void enqueue_solve64(m2d_system *s, const double *a, const double *rhs, double *x,
+                            double gamma, int nitm)
+{
+    Work<double> &W = s->w64;
+    launch_residual<double, double, double>(a, x, rhs, W.r, s->nx, s->ny, s->partial, s->st,
+                                            &W.sk, s->stream, s->nd);
+    launch_norm_finalize(s->partial, s->np, s->st, nullptr, 0, 1, gamma, nitm, s->stream);
+    for (int k = 1; k <= nitm; ++k)
+        enqueue_iteration<double>(s, W, s->st, a, rhs, x, nullptr, 0, k, gamma, nitm);
+}
+
+extern "C" int m2d_intflow_pl_loop(m2d_system *s, const m2d_pl_loop_fields *f, int modvis,
+                                   double dt, double beta, double p_avg, double tolm, int itpmax,
+                                   int nitm_p, double tol_p, int *itp_out, double *normpl_out)
+{
+    if (!s || !f || s->nd != 5 || s->precision != M2D_FP64 || itpmax < 1 || nitm_p < 1 ||
+        !(p_avg != 0.0))
+        return M2D_ERR_ARG;
+    const void *need[] = {f->x,   f->xp,  f->xe,  f->ye,  f->xk,  f->yk,  f->rp,  f->re,
+                          f->rn,  f->Jp,  f->due, f->dve, f->dun, f->dvn, f->de,  f->dn,
+                          f->g,   f->roa, f->u,   f->v,   f->plin, f->ue, f->ve,  f->un,
+                          f->vn,  f->Uce, f->Vcn, f->roe, f->ron, f->ro,  f->p,   f->pl,
+                          f->ap,  f->bp};
+    for (const void *p : need)
+        if (!p) return M2D_ERR_ARG;
+    const int nx = s->nx, ny = s->ny;
+    const long N = s->N;
+    const hipStream_t keep = assembly_stream(s->stream);
+    int rc = M2D_OK, itp = 0;
+    double normpl = 0.0;
+    // every assembly call below takes device arrays and only enqueues
+    auto ok = [&]() {
+        m2d_status st;
+        m2d_last_status(&st);
+        return st.status == M2D_OK;
+    };
+    HIP_OK(hipMemsetAsync(f->pl, 0, sizeof(double) * N, s->stream));
+    for (itp = 1; itp <= itpmax && !rc; ++itp) {
+        m2d_get_p_coefficients(&nx, &ny, &dt, f->rp, f->re, f->rn, f->Jp, f->Uce, f->Vcn,
+                               f->roe, f->ron, f->g, f->de, f->dn, f->ap);
+        if (ok()) m2d_intflow_set_bcpl(&nx, &ny, f->x, f->xp, f->pl, f->plin, f->ap, f->bp);
+        if (ok())
+            m2d_get_p_source(&nx, &ny, &dt, f->rp, f->re, f->rn, f->Jp, f->roe, f->ron, f->ro,
+                             f->roa, f->Uce, f->Vcn, f->de, f->dn, f->g, f->pl, f->bp);
+        if (!ok()) {
+            rc = M2D_ERR_ARG;
+            break;
+        }
+        enqueue_factor64(s, f->ap);
+        enqueue_solve64(s, f->ap, f->bp, f->pl, tol_p, nitm_p);
+        m2d_get_velocity_correction_at_faces_with_pl(&nx, &ny, f->xe, f->ye, f->xk, f->yk,
+                                                     f->due, f->dve, f->dun, f->dvn, f->de,
+                                                     f->dn, f->pl, f->ue, f->ve, f->un, f->vn,
+                                                     f->Uce, f->Vcn);
+        if (ok())
+            m2d_intflow_velocities_at_boundary_faces(&nx, &ny, &modvis, f->x, f->xp, f->xe,
+                                                     f->ye, f->xk, f->yk, f->u, f->v, f->ue,
+                                                     f->ve, f->un, f->vn, f->Uce, f->Vcn);
+        if (ok()) m2d_get_pressure_correction_with_pl(&nx, &ny, f->pl, f->p);
+        if (ok()) m2d_get_density_correction_with_pl(&nx, &ny, f->pl, f->g, f->ro);
+        if (ok()) m2d_get_density_at_faces(&nx, &ny, &beta, f->ro, f->Uce, f->Vcn, f->roe, f->ron);
+        if (!ok()) {
+            rc = M2D_ERR_ARG;
+            break;
+        }
+        hipLaunchKernelGGL(k_absmax, dim3(1), dim3(1024), 0, s->stream, f->pl, N, s->r64);
+        double mx = 0.0;
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&mx, s->r64, sizeof(double), hipMemcpyDeviceToHost, s->stream) !=
+                hipSuccess) {
+            rc = M2D_ERR_HIP;
+            break;
+        }
+        rc = read_state(s, s->st);  // also completes the copy of mx
+        if (rc) break;
+        const SolveState &st = *s->h_st;
+        if (st.outside) rc = M2D_ERR_OUTSIDE_COUPLING;
+        else if (st.err_pivot || st.status == M2D_ERR_NONFINITE) rc = M2D_ERR_NONFINITE;
+        else if (st.abort || st.status == M2D_ERR_SYNC_TIMEOUT) rc = M2D_ERR_SYNC_TIMEOUT;
+        if (rc) break;
+        normpl = mx / p_avg;
+        if (normpl < tolm) break;
+    }
+    assembly_stream(keep);
+    if (itp > itpmax) itp = itpmax;
+    if (itp_out) *itp_out = itp;
+    if (normpl_out) *normpl_out = normpl;
+    set_last(rc);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
 // Reference-signature entry points (host arrays).  A per-thread cache keeps
 // the device buffers and an FP64 handle for the last grid size.
 // ---------------------------------------------------------------------------

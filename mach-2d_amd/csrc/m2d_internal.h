@@ -93,6 +93,10 @@ void launch_norm_finalize(const double *partial, int np, SolveState *st, double 
 void launch_check_outside(const double *a, int nx, int ny, SolveState *st, hipStream_t s,
                           int nd = 9);
 void launch_reset_state(SolveState *st, hipStream_t s);
+
+// ---- coefficient assembly (assembly.hip) -------------------------------------
+// Stream of this thread's device-array assembly calls; returns the previous one.
+hipStream_t assembly_stream(hipStream_t s);
 template <typename TI, typename TO>
 void launch_convert(const TI *in, TO *out, long n, hipStream_t s);
 

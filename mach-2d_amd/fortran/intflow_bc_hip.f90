@@ -1,6 +1,7 @@
 ! intflow_bc_hip.f90 -- the fictitious-row writers of the internal-flow case
 ! (src/user/internal_flow/mod_intflow.f90: intflow_set_bcu, intflow_set_bcv,
-! intflow_set_bcpl, intflow_set_bcT) on the GPU, through the C ABI of
+! intflow_set_bcpl, intflow_set_bcT) and the boundary-face velocities
+! (intflow_velocities_at_boundary_faces) on the GPU, through the C ABI of
 ! libmach2d_hip.so (include/mach2d_hip.h, m2d_intflow_set_*).  Each routine
 ! keeps the reference's dummy arguments, except that the derived-type
 ! argument `iflow` is replaced by the fields the routine reads (plin; Tin and
@@ -13,7 +14,7 @@ module intflow_bc_hip
    implicit none
    private
    public :: intflow_set_bcu_hip, intflow_set_bcv_hip, intflow_set_bcpl_hip, &
-      intflow_set_bcT_hip
+      intflow_set_bcT_hip, intflow_velocities_at_boundary_faces_hip
 
    interface
       subroutine c_intflow_set_bcu(nx, ny, modvis, x, xp, xk, yk, u, v, au, b) &
@@ -40,6 +41,13 @@ module intflow_bc_hip
          integer(c_int) :: nx, ny
          real(c_double) :: Twall
          real(c_double) :: x(*), xp(*), T(*), Tin(*), at(*), bt(*)
+      end subroutine
+      subroutine c_intflow_velocities_at_boundary_faces(nx, ny, modvis, x, xp, xe, ye, xk, yk, &
+            u, v, ue, ve, un, vn, Uce, Vcn) bind(C, name="m2d_intflow_velocities_at_boundary_faces")
+         import :: c_int, c_double
+         integer(c_int) :: nx, ny, modvis
+         real(c_double) :: x(*), xp(*), xe(*), ye(*), xk(*), yk(*), u(*), v(*), ue(*), ve(*), &
+            un(*), vn(*), Uce(*), Vcn(*)
       end subroutine
    end interface
 
@@ -76,6 +84,18 @@ contains
       real(8), intent(in) :: x(nx*ny), xp(nx*ny), T(nx*ny), Tin(ny)
       real(8), intent(inout) :: at(nx*ny,9), bt(nx*ny)
       call c_intflow_set_bcT(nx, ny, Twall, x, xp, T, Tin, at, bt)
+   end subroutine
+
+   ! mod_intflow.f90:692
+   subroutine intflow_velocities_at_boundary_faces_hip(nx, ny, modvis, x, xp, xe, ye, xk, yk, &
+         u, v, ue, ve, un, vn, Uce, Vcn)
+      integer, intent(in) :: nx, ny, modvis
+      real(8), intent(in) :: x(nx*ny), xp(nx*ny), xe(nx*ny), ye(nx*ny), xk(nx*ny), yk(nx*ny), &
+         u(nx*ny), v(nx*ny)
+      real(8), intent(inout) :: ue(nx*ny), ve(nx*ny), un(nx*ny), vn(nx*ny), Uce(nx*ny), &
+         Vcn(nx*ny)
+      call c_intflow_velocities_at_boundary_faces(nx, ny, modvis, x, xp, xe, ye, xk, yk, u, v, &
+         ue, ve, un, vn, Uce, Vcn)
    end subroutine
 
 end module intflow_bc_hip

@@ -62,6 +62,7 @@ EXPORTS = [
     "m2d_get_u_v_at_real_nodes_with_pl", "m2d_get_velocity_correction_at_faces_with_pl",
     "m2d_initialize_boundary_faces_flags", "m2d_check_mass_conservation",
     "m2d_intflow_set_bcu", "m2d_intflow_set_bcv", "m2d_intflow_set_bcpl", "m2d_intflow_set_bcT",
+    "m2d_intflow_velocities_at_boundary_faces", "m2d_intflow_pl_loop",
     "m2d_unet_conv3x3", "m2d_unet_conv3x3_wgrad", "m2d_unet_relu_bwd", "m2d_unet_mode_bwd",
     "m2d_unet_to_bf16", "m2d_unet_adam", "m2d_unet_pack_weights", "m2d_unet_mse",
     "m2d_unet_pack_input", "m2d_unet_unpack_output",
@@ -161,6 +162,10 @@ def load():
     L.m2d_enable_kernel_timing.restype = C.c_int
     L.m2d_kernel_timing.argtypes = [_vp, C.c_int, _dp, C.POINTER(C.c_long)]
     L.m2d_kernel_timing.restype = C.c_int
+    L.m2d_intflow_pl_loop.argtypes = [_vp, _vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, C.c_double, _ip,
+                                      C.POINTER(C.c_double)]
+    L.m2d_intflow_pl_loop.restype = C.c_int
     L.m2d_version.argtypes = []
     L.m2d_version.restype = C.c_char_p
     _lib = L

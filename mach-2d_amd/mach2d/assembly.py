@@ -91,13 +91,18 @@ ROUTINES = {
                         [("a", 9), ("b", 1)]),
     "intflow_set_bcpl": ([], ["x", "xp", "pl", ("plin", "ny")], [("a", 5), ("b", 1)]),
     "intflow_set_bcT": ([("Twall", "d")], ["x", "xp", "T", ("Tin", "ny")], [("a", 9), ("b", 1)]),
+    # mod_intflow.f90:692 (outputs are in/out: only the boundary faces change)
+    "intflow_velocities_at_boundary_faces": (
+        [("modvis", "i")], ["x", "xp", "xe", "ye", "xk", "yk", "u", "v"],
+        [("ue", 1), ("ve", 1), ("un", 1), ("vn", 1), ("Uce", 1), ("Vcn", 1)]),
 }
 
 # C entry of each routine (the reference's subroutine name, prefixed)
 ENTRY = {name: "m2d_get_" + name for name in ROUTINES}
 ENTRY["boundary_faces_flags"] = "m2d_initialize_boundary_faces_flags"
 ENTRY["mass_conservation"] = "m2d_check_mass_conservation"
-for _k in ("intflow_set_bcu", "intflow_set_bcv", "intflow_set_bcpl", "intflow_set_bcT"):
+for _k in ("intflow_set_bcu", "intflow_set_bcv", "intflow_set_bcpl", "intflow_set_bcT",
+           "intflow_velocities_at_boundary_faces"):
     ENTRY[_k] = "m2d_" + _k
 
 

@@ -190,3 +190,45 @@ void orc_intflow_set_bcT(int nx, int ny, double Twall, const double *x, const do
         bt[np] = ccTw ? 2.0 * Twall : 0.0;
     }
 }
+
+/* intflow_velocities_at_boundary_faces: mod_intflow.f90:692-787.  Velocities
+ * on the boundary faces: west (i = 1) and east (i = nx-1) faces of rows
+ * 2..ny-1, south (j = 1) and north (j = ny-1) faces of columns 2..nx-1; the
+ * north face takes the Euler slip velocity (modvis = 0) or zero (NS). */
+void orc_intflow_velocities_at_boundary_faces(int nx, int ny, int modvis, const double *x,
+                                              const double *xp, const double *xe,
+                                              const double *ye, const double *xk,
+                                              const double *yk, const double *u, const double *v,
+                                              double *ue, double *ve, double *un, double *vn,
+                                              double *Uce, double *Vcn)
+{
+    for (int j = 2; j <= ny - 1; ++j) {  /* west */
+        const long np = P(1, j), npe = np + 1;
+        ue[np] = (u[np] + u[npe]) / 2.0;
+        ve[np] = 0.0;
+        Uce[np] = ue[np] * ye[np] - ve[np] * xe[np];
+    }
+    for (int j = 2; j <= ny - 1; ++j) {  /* east */
+        const long np = P(nx - 1, j), npe = np + 1;
+        ue[np] = (u[np] + u[npe]) / 2.0;
+        ve[np] = (v[np] + v[npe]) / 2.0;
+        Uce[np] = ue[np] * ye[np] - ve[np] * xe[np];
+    }
+    for (int i = 2; i <= nx - 1; ++i) {  /* south */
+        const long np = P(i, 1), npn = np + nx;
+        un[np] = (u[np] + u[npn]) / 2.0;
+        vn[np] = 0.0;
+        Vcn[np] = 0.0;
+    }
+    double *ubn = (double *)calloc((size_t)nx, sizeof(double));
+    double *vbn = (double *)calloc((size_t)nx, sizeof(double));
+    if (modvis == 0) orc_u_v_north_slip(nx, ny, x, xp, xk, yk, u, v, ubn, vbn);
+    for (int i = 2; i <= nx - 1; ++i) {  /* north */
+        const long np = P(i, ny - 1);
+        un[np] = ubn[i - 1];
+        vn[np] = vbn[i - 1];
+        Vcn[np] = 0.0;
+    }
+    free(ubn);
+    free(vbn);
+}

@@ -108,4 +108,10 @@ void orc_intflow_set_bcpl(int nx, int ny, const double *x, const double *xp, con
                           const double *plin, double *a, double *bp);
 void orc_intflow_set_bcT(int nx, int ny, double Twall, const double *x, const double *xp,
                          const double *T, const double *Tin, double *a, double *bt);
+void orc_intflow_velocities_at_boundary_faces(int nx, int ny, int modvis, const double *x,
+                                              const double *xp, const double *xe,
+                                              const double *ye, const double *xk,
+                                              const double *yk, const double *u, const double *v,
+                                              double *ue, double *ve, double *un, double *vn,
+                                              double *Uce, double *Vcn);
 #endif

@@ -165,6 +165,9 @@ COEF = {
                         [("a", 9), ("b", 1)]),
     "intflow_set_bcpl": ([], ["x", "xp", "pl", ("plin", "ny")], [("a", 5), ("b", 1)]),
     "intflow_set_bcT": ([("Twall", "d")], ["x", "xp", "T", ("Tin", "ny")], [("a", 9), ("b", 1)]),
+    "intflow_velocities_at_boundary_faces": (
+        [("modvis", "i")], ["x", "xp", "xe", "ye", "xk", "yk", "u", "v"],
+        [("ue", 1), ("ve", 1), ("un", 1), ("vn", 1), ("Uce", 1), ("Vcn", 1)]),
 }
 
 

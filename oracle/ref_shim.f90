@@ -15,7 +15,7 @@ module m2dref_shim
    use mod_solvers, only: norm_l1_9d, norm_l1_9d_relative, norm_l1_b
    use coefficients
    use mod_intflow, only: type_intflow, intflow_set_bcu, intflow_set_bcv, &
-      intflow_set_bcpl, intflow_set_bcT
+      intflow_set_bcpl, intflow_set_bcT, intflow_velocities_at_boundary_faces
    implicit none
 contains
 
@@ -337,6 +337,16 @@ contains
       iflow%Tin = Tin
       iflow%Twall = Twall
       call intflow_set_bcT(nx, ny, x, xp, T, iflow, at, bt)
+   end subroutine
+
+   ! mod_intflow.f90:692
+   subroutine ref_intflow_velocities_at_boundary_faces(nx, ny, modvis, x, xp, xe, ye, xk, yk, &
+         u, v, ue, ve, un, vn, Uce, Vcn) bind(C, name="ref_intflow_velocities_at_boundary_faces")
+      integer(c_int), value :: nx, ny, modvis
+      real(c_double), dimension(nx*ny), intent(in) :: x, xp, xe, ye, xk, yk, u, v
+      real(c_double), dimension(nx*ny), intent(inout) :: ue, ve, un, vn, Uce, Vcn
+      call intflow_velocities_at_boundary_faces(nx, ny, modvis, x, xp, xe, ye, xk, yk, u, v, &
+         ue, ve, un, vn, Uce, Vcn)
    end subroutine
 
 end module m2dref_shim

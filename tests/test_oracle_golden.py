@@ -169,3 +169,21 @@ def test_msi5d_embedding_identity_on_generated_systems(orc, nx, ny):
     x5, n5, _, h5 = orc.fb2d5(s.a, dl5, du5, nx, ny, np.zeros(s.N), 1e-8, 3000, s.rhs)
     x9, n9, _, h9 = orc.fb2d9(a9, dl9, du9, nx, ny, np.zeros(s.N), 1e-8, 3000, s.rhs)
     assert n5 == n9 and np.array_equal(x5, x9) and np.array_equal(h5, h9)
+
+
+@pytest.mark.parametrize("modvis", [0, 1])
+def test_pl_loop_oracle_vs_reference(orc, modvis):
+    """The pressure-correction loop composed from the oracle's routines
+    equals the same composition of the compiled reference's routines bit for
+    bit (state after three itp iterations)."""
+    import pl_loop_oracle as PL
+    if not orc.ref_available():
+        pytest.skip("reference library not built")
+    nx, ny = 26, 14
+    f = PL.loop_fields(nx, ny, seed=5)
+    p_avg = float(np.mean(f["p"]))
+    a, ia, na = PL.run(orc, f, nx, ny, modvis, 2e-3, 0.3, p_avg, 0.0, 3, 20, 1e-2)
+    b, ib, nb = PL.run(orc, f, nx, ny, modvis, 2e-3, 0.3, p_avg, 0.0, 3, 20, 1e-2, ref=True)
+    assert ia == ib == 3 and na == nb
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
