@@ -21,7 +21,9 @@ seeded right-hand sides) solved together, with their throughput against
 the single plain-FP64 solve; msi2d5_1026x514 times the 5-diagonal p' system
 of the same grid (the reference's own p' form) on the dedicated 5-diagonal
 kernels and, for comparison, on the 9-diagonal ones through the zero-corner
-embedding.
+embedding; pl_loop_258x130 times the device-resident pressure-correction
+loop of one SIMPLEC iteration of the internal-flow case (258x130, 20 itp
+iterations) against the same loop of reference routines on one core.
 
 Prints one JSON line.  gbps = algorithmic bytes of the solve over its time
 (FP64: 8 N (18 + 11 + 24 K); refinement: 4 N (18 + 24 K_inner) +
@@ -235,6 +237,35 @@ def msi5_run(steps, warmup):
     return out
 
 
+def pl_loop_run(reps, cpu_itp=3):
+    """The internal-flow pressure-correction loop of one SIMPLEC iteration
+    (20 itp iterations, p' solves of at most 20 MSI iterations) on a 258x130
+    grid, device-resident, against the same loop composed from the reference
+    routines (compiled Fortran, one core; `cpu_itp` iterations) when built."""
+    from mach2d import workloads as W
+    g = W.pl_loop(258, 130, 20, reps)
+    out = {"ms_per_itp": g["ms_per_itp"], "itp": g["itp"], "ms": g["ms"]}
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle as O
+        import pl_loop_oracle as PL
+        ref = O.ref_available()
+        f = PL.loop_fields(258, 130, seed=1)
+        p_avg = float(np.mean(f["p"]))
+        t = time.perf_counter()
+        PL.run(O, f, 258, 130, 0, 2e-3, 0.3, p_avg, 0.0, cpu_itp, 20, 1e-2, ref=ref)
+        cpu = (time.perf_counter() - t) / cpu_itp * 1e3
+        out["cpu_ms_per_itp"] = round(cpu, 3)
+        out["cpu_kind"] = "reference" if ref else "port"
+        out["speedup_vs_cpu"] = round(cpu / g["ms_per_itp"], 2)
+    except Exception as e:  # the checker is optional here
+        out["cpu_ms_per_itp"] = None
+        log("pl_loop cpu: %s" % e)
+    log("pl_loop 258x130: %s" % out)
+    return out
+
+
 def cpu_baseline(s, nit, budget_s=15.0):
     """Reference lu2d9 + fb2d9 (compiled Fortran, one core) on a bounded
     sample: the factorization plus M iterations; solves/s extrapolated to
@@ -427,6 +458,7 @@ def main():
         if lr is not None:
             summary["learned_fp32_ir"] = lr
     msi5 = msi5_run(args.steps, args.warmup) if args.mode == "best" else None
+    pl_loop = pl_loop_run(max(1, args.steps)) if args.mode == "best" else None
     head = "fp32_ir" if "fp32_ir" in summary and summary["fp32_ir"]["correct"] else modes[0]
     r, e = runs[head], summary[head]
     st = r["st"]
@@ -478,6 +510,7 @@ def main():
         "batch": batch,
         "batch_1026x514": batch3,
         "msi2d5_1026x514": msi5,
+        "pl_loop_258x130": pl_loop,
         "target_comparison": target_comparison(summary, batch3),
     }
     print(json.dumps(out), flush=True)

@@ -274,10 +274,21 @@ def solvers_init(nx, ny, ifile):
     tol_u, tol_p (a dict, or anything with get_value(name)).  Returns
     (solver5d, solver9d).  An unknown solver name stops, as the reference
     does (here: raises SystemExit)."""
-    get = ifile.get_value if hasattr(ifile, "get_value") else ifile.__getitem__
-    ksolver = str(get("solver")).strip()
-    nitm_u, nitm_p = int(get("nitm_u")), int(get("nitm_p"))
-    tol_u, tol_p = float(get("tol_u")), float(get("tol_p"))
+    if hasattr(ifile, "get_value"):   # formats.ClassIfile (or anything with get_value)
+        try:
+            ksolver = str(ifile.get_value("solver", str)).strip()
+            nitm_u, nitm_p = ifile.get_value("nitm_u", int), ifile.get_value("nitm_p", int)
+            tol_u, tol_p = ifile.get_value("tol_u", float), ifile.get_value("tol_p", float)
+        except TypeError:   # get_value(name) without a kind
+            get = ifile.get_value
+            ksolver = str(get("solver")).strip()
+            nitm_u, nitm_p = int(get("nitm_u")), int(get("nitm_p"))
+            tol_u, tol_p = float(get("tol_u")), float(get("tol_p"))
+    else:
+        get = ifile.__getitem__
+        ksolver = str(get("solver")).strip()
+        nitm_u, nitm_p = int(get("nitm_u")), int(get("nitm_p"))
+        tol_u, tol_p = float(get("tol_u")), float(get("tol_p"))
     if ksolver == "MSI":
         return (ClassSolverMSI5d().init(nx, ny, nitm_p, tol_p),
                 ClassSolverMSI9d().init(nx, ny, nitm_u, tol_u))

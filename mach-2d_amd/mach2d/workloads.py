@@ -4,13 +4,16 @@
     python -m mach2d.workloads single --mode fp32_ir --solves 2
     python -m mach2d.workloads batch --mode fp64 --count 256
     python -m mach2d.workloads batch --mode fp32_ir --count 32 --same-1026x514 [--layout L]
+    python -m mach2d.workloads pl_loop --solves 3
 
 single: the 1026x514 p' system (wall-resolved ogive grid, aspect 1e4,
 Gaussian right-hand side) solved to a 1e-10 drop from zero, `--solves` times
 after one warm-up solve.  batch: `count` systems of uneven size (64s x 32s,
 s log-uniform on [1, 8]) or `count` copies of the 1026x514 matrix with their
 own right-hand sides, solved together (autotuned layout unless given).
-Prints one JSON line with the timing and iteration counts.
+pl_loop: the internal-flow pressure-correction loop (20 itp iterations,
+p' solves of at most 20 MSI iterations) on a 258x130 grid.  Prints one JSON
+line with the timing and iteration counts.
 """
 from __future__ import annotations
 
@@ -162,9 +165,44 @@ def batch_split(mode, count, groups, layout, inner_nitm=50):
             "solves_per_s": round(count / dt, 2), "iterations_max": its, "status": rc}
 
 
+def pl_loop(nx, ny, itpmax, reps):
+    """The internal-flow pressure-correction loop (m2d_intflow_pl_loop) on an
+    nx x ny grid with seeded fields: ms per call and per itp iteration."""
+    import numpy as np
+    import torch
+
+    from . import simplec
+    from .solvers import DeviceMSI
+    f = P.flow_fields(nx, ny, seed=1)
+    rng = np.random.default_rng(18)
+    for k in ("due", "dve", "dun", "dvn"):
+        f[k] = rng.uniform(-1e-3, 1e-3, size=nx * ny)
+    ap = np.zeros((5, nx * ny))
+    ap[2] = 1.0
+    f["ap"] = ap.reshape(-1)
+    f["bp"] = np.zeros(nx * ny)
+    p_avg = float(np.mean(f["p"]))
+    dev = DeviceMSI(nx, ny, "fp64", ndiag=5)
+    base = {k: torch.tensor(np.ascontiguousarray(v), device="cuda") for k, v in f.items()}
+    out = []
+    for r in range(reps + 1):
+        t = {k: v.clone() for k, v in base.items()}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        itp, normpl = simplec.intflow_pl_loop(dev, t, modvis=0, dt=2e-3, beta=0.3, p_avg=p_avg,
+                                              tolm=0.0, itpmax=itpmax, nitm_p=20, tol_p=1e-2)
+        torch.cuda.synchronize()
+        if r:
+            out.append((time.perf_counter() - t0) * 1e3)
+    dev.close()
+    ms = sorted(out)[len(out) // 2]
+    return {"workload": "pl_loop", "nx": nx, "ny": ny, "itp": itp, "ms": round(ms, 3),
+            "ms_per_itp": round(ms / itp, 4), "normpl": normpl}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["single", "batch", "split"])
+    ap.add_argument("what", choices=["single", "batch", "split", "pl_loop"])
     ap.add_argument("--groups", type=int, default=2)
     ap.add_argument("--mode", default="fp32_ir", choices=["fp64", "fp32_ir"])
     ap.add_argument("--solves", type=int, default=1)
@@ -178,6 +216,8 @@ def main(argv=None):
     if args.what == "single":
         out = single(args.mode, args.solves, bool(args.staged), args.inner_gamma,
                      args.inner_nitm or 25)
+    elif args.what == "pl_loop":
+        out = pl_loop(258, 130, 20, max(1, args.solves))
     elif args.what == "split":
         out = batch_split(args.mode, args.count, args.groups,
                           0 if args.layout is None else args.layout, args.inner_nitm or 50)

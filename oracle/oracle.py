@@ -377,6 +377,37 @@ def ref_fb2d5(a, dl, du, nx, ny, var, gamma, nitm, rhs):
     return var
 
 
+def ref_ifile_get(fname, fs, varname, kind):
+    """class_ifile (src/util/class_ifile.f90) of the compiled reference: load
+    `fname` and read `varname` as int (kind int), real(8) (float) or string."""
+    R = ref()
+    f = R.ref_ifile_get
+    f.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_int),
+                  C.POINTER(C.c_double), C.c_char_p, C.c_int]
+    f.restype = None
+    iv, rv = C.c_int(), C.c_double()
+    buf = C.create_string_buffer(600)
+    k = {int: 0, float: 1, str: 2}[kind]
+    f(fname.encode(), fs.encode(), varname.encode(), k, C.byref(iv), C.byref(rv), buf, 500)
+    if kind is int:
+        return iv.value
+    if kind is float:
+        return rv.value
+    return buf.raw[:500].decode("latin-1").rstrip(" \x00")
+
+
+def ref_format_residual(it, dt, norm, normpl, rmass):
+    """The residual file's row and header lines as the Fortran runtime formats
+    them (src/mach2d_main.f90:495-517 edit descriptors)."""
+    f = ref().ref_format_residual
+    f.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_char_p,
+                  C.c_char_p, C.c_int]
+    f.restype = None
+    row, head = C.create_string_buffer(201), C.create_string_buffer(201)
+    f(it, dt, norm, normpl, rmass, row, head, 200)
+    return (row.raw[:200].decode().rstrip(" \x00"), head.raw[:200].decode().rstrip(" \x00"))
+
+
 def ref_tdma2d9(a, b, x, nx, ny, nitm):
     x = _c(x).copy()
     _big_stack_call(lambda: ref().ref_tdma2d9(nx, ny, nitm, _c(a), _c(b), x))

@@ -8,6 +8,7 @@
 ! Each wrapper forwards its arguments unchanged; scalars come by value.
 module m2dref_shim
    use iso_c_binding
+   use mod_class_ifile, only: class_ifile
    use msi2d9
    use msi2d5
    use mtdma2d9
@@ -347,6 +348,70 @@ contains
       real(c_double), dimension(nx*ny), intent(inout) :: ue, ve, un, vn, Uce, Vcn
       call intflow_velocities_at_boundary_faces(nx, ny, modvis, x, xp, xe, ye, xk, yk, u, v, &
          ue, ve, un, vn, Uce, Vcn)
+   end subroutine
+
+   ! src/util/class_ifile.f90: init + load a parameter file, then get_value of
+   ! one variable as an integer (kind 0), a real(8) (1) or a string (2)
+   subroutine ref_ifile_get(fname, fs, varname, kind, ival, rval, sval, slen) &
+         bind(C, name="ref_ifile_get")
+      character(kind=c_char), intent(in) :: fname(*), fs(*), varname(*)
+      integer(c_int), value :: kind, slen
+      integer(c_int), intent(out) :: ival
+      real(c_double), intent(out) :: rval
+      character(kind=c_char), intent(out) :: sval(slen)
+      type(class_ifile) :: f
+      character(len=1000) :: fn, fsp, vn, sv
+      integer :: k, iv
+      real(8) :: rv
+      call c_str(fname, fn)
+      call c_str(fs, fsp)
+      call c_str(varname, vn)
+      call f%init(trim(fn), trim(fsp))
+      call f%load()
+      ival = 0
+      rval = 0.d0
+      sval = ' '
+      if (kind == 0) then
+         call f%get_value(iv, trim(vn))
+         ival = iv
+      else if (kind == 1) then
+         call f%get_value(rv, trim(vn))
+         rval = rv
+      else
+         call f%get_value(sv, trim(vn))
+         do k = 1, min(slen, len_trim(sv))
+            sval(k) = sv(k:k)
+         end do
+      end if
+   contains
+      subroutine c_str(c, s)
+         character(kind=c_char), intent(in) :: c(*)
+         character(len=*), intent(out) :: s
+         integer :: i
+         s = ' '
+         do i = 1, len(s)
+            if (c(i) == c_null_char) exit
+            s(i:i) = c(i)
+         end do
+      end subroutine
+   end subroutine
+
+   ! the residual row and header formats of src/mach2d_main.f90:495-517
+   subroutine ref_format_residual(it, dt, norm, normpl, rmass, row, header, slen) &
+         bind(C, name="ref_format_residual")
+      integer(c_int), value :: it, slen
+      real(c_double), value :: dt, norm, normpl, rmass
+      character(kind=c_char), intent(out) :: row(slen), header(slen)
+      character(len=200) :: r, h
+      integer :: k
+      write(r, "(I10, 4(1X,ES14.7), A)") it, dt, norm, normpl, rmass, ""
+      write(h, "(A10, 4(1X, A14), A)") 'it', 'dt', 'norm', 'max(|pl|)/p_avg', 'rmass', ""
+      row = ' '
+      header = ' '
+      do k = 1, min(slen, 200)
+         row(k) = r(k:k)
+         header(k) = h(k:k)
+      end do
    end subroutine
 
 end module m2dref_shim

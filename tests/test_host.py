@@ -238,3 +238,58 @@ def test_generated_5point_systems():
     assert (a[2, 1:-1, 1:-1] > nb).all()
     for j, i in ((0, 0), (0, -1), (-1, 0), (-1, -1)):
         assert a[2, j, i] == 1.0 and not a[[0, 1, 3, 4], j, i].any()
+
+
+def _ifile(name):
+    from mach2d.formats import ClassIfile
+    path = os.path.join(ROOT, "tests", "golden", name)
+    return path, ClassIfile().init(path, "&").load()
+
+
+@pytest.mark.parametrize("name", ["parameters.txt", "parametersNozzleEulerPTC.txt"])
+def test_ifile_reads_reference_parameter_files(name):
+    """Known entries of the shipped parameter files, and solvers_init on
+    them (mod_solvers.f90:17-87 reads solver, nitm_u, nitm_p, tol_u, tol_p)."""
+    from mach2d import solvers as S
+    _, f = _ifile(name)
+    assert f.get_value("solver", str) == "MSI"
+    assert (f.get_value("nitm_u", int), f.get_value("nitm_p", int)) == (5, 20)
+    assert f.get_value("tol_p", float) == 1e-2
+    s5, s9 = S.solvers_init(10, 6, f)
+    assert (s5.nitm, s5.tol, s9.nitm, s9.tol) == (20, 1e-2, 5, 1e-2)
+    with pytest.raises(SystemExit):
+        f.get_value("no_such_variable", int)
+
+
+@pytest.mark.parametrize("name", ["parameters.txt", "parametersNozzleEulerPTC.txt"])
+def test_ifile_matches_compiled_reference(orc, name):
+    """Every variable the parser finds reads the same (as a string, and as
+    an integer or real(8) where the field is one) as class_ifile of the
+    compiled reference on the same file."""
+    if not orc.ref_available():
+        pytest.skip("reference library not built")
+    path, f = _ifile(name)
+    names = [d[1].strip() for d in f.data]
+    assert len(names) > 20
+    for n in dict.fromkeys(names):
+        assert f.get_value(n, str) == orc.ref_ifile_get(path, "&", n, str), n
+        v = f.get_value(n, str).split()[0] if f.get_value(n, str).split() else ""
+        if re.fullmatch(r"[+-]?\d+", v):
+            assert f.get_value(n, int) == orc.ref_ifile_get(path, "&", n, int), n
+        if re.fullmatch(r"[+-]?(\d+\.?\d*|\.\d+)([eEdD][+-]?\d+)?", v):
+            assert f.get_value(n, float) == orc.ref_ifile_get(path, "&", n, float), n
+
+
+def test_residual_rows_match_fortran_formatting(orc):
+    """residual_header / residual_row against the Fortran runtime's output
+    of the same edit descriptors (mach2d_main.f90:495-517)."""
+    from mach2d.formats import residual_header, residual_row
+    if not orc.ref_available():
+        pytest.skip("reference library not built")
+    cases = [(1, 1e-3, 0.5, 2.5e-7, -3.25e-12), (123456, 2.0, 0.0, 1.0, -0.0),
+             (7, 1.23456789e-120, 9.99999999e99, -5e-100, 3.0e150),
+             (42, -7.7777777e-5, 1.0e-9, 12345.678, 0.1)]
+    for c in cases:
+        row, head = orc.ref_format_residual(*c)
+        assert residual_row(*c) == row, c
+        assert residual_header() == head
