@@ -442,6 +442,27 @@ int m2d_intflow_pl_loop(m2d_system *s5, const m2d_pl_loop_fields *f, int modvis,
                         double beta, double p_avg, double tolm, int itpmax, int nitm_p,
                         double tol_p, int *itp_out, double *normpl_out);
 
+/* ---- Time-step statistics (src/tstepper/mod_tstepper_scarborough.f90) ---
+ * Host or device arrays (all of one kind; device arrays on the assembly
+ * stream); each call returns its value and synchronises.  Status of the last
+ * call: m2d_last_status. */
+
+/* get_mean_cc_9d / get_mean_cc_5d, :260-320: the mean over real volumes of
+ * (sum |a_nb|) / |a_P| (9 slots, or 5 slots S, W, P, E, N), bit for bit. */
+double m2d_get_mean_cc_9d(const int *nx, const int *ny, const double *a);
+double m2d_get_mean_cc_5d(const int *nx, const int *ny, const double *a);
+
+/* tstepper_dt0_scarborough, :40-76 */
+double m2d_tstepper_dt0_scarborough(const int *nx, const int *ny, const double *Jp,
+                                    const double *u, const double *v);
+
+/* tstepper_dt_scarborough, :80-123; h0, mincc, maxcc are the values
+ * tstepper_scarborough_init (:28-36) reads from the parameter file. */
+double m2d_tstepper_dt_scarborough(const int *nx, const int *ny, const double *dt,
+                                   const double *au, const double *av, const double *at,
+                                   const double *ap, const double *h0, const double *mincc,
+                                   const double *maxcc);
+
 /* ---- 5. Learned initial guess: U-Net kernels (device pointers) ---------- */
 /* No reference counterpart (the reference starts MSI from the caller's var).
  * BF16 NHWC tensors with channel counts that are multiples of 8 and 16-byte

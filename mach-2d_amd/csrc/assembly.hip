@@ -1180,6 +1180,10 @@ hipStream_t m2d::assembly_stream(hipStream_t s)
     return old;
 }
 
+hipStream_t m2d::assembly_stream_current() { return g_stream; }
+
+bool m2d::is_device_array(const void *p) { return is_device_ptr(p); }
+
 extern "C" void m2d_get_density_at_nodes(const int *nx, const int *ny, const double *Rg,
                                          const double *p, const double *T, double *ro)
 {

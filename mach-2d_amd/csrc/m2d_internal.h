@@ -97,6 +97,9 @@ void launch_reset_state(SolveState *st, hipStream_t s);
 // ---- coefficient assembly (assembly.hip) -------------------------------------
 // Stream of this thread's device-array assembly calls; returns the previous one.
 hipStream_t assembly_stream(hipStream_t s);
+hipStream_t assembly_stream_current();
+// true for device memory (false for host memory and unknown pointers)
+bool is_device_array(const void *p);
 template <typename TI, typename TO>
 void launch_convert(const TI *in, TO *out, long n, hipStream_t s);
 

@@ -1,0 +1,249 @@
+// tstep.hip -- the Scarborough time-step statistics of
+// src/tstepper/mod_tstepper_scarborough.f90 on the GPU: the mean convergence
+// coefficient of a 9- or 5-diagonal system (get_mean_cc_9d / _5d,
+// :260-320), the first time step (tstepper_dt0_scarborough, :40-76) and the
+// next one (tstepper_dt_scarborough, :80-123).
+//
+// The per-point coefficients (sum of |a_nb| over |a_P|, the sums left to
+// right as the reference's SUM intrinsic adds them) are computed in parallel
+// into a scratch array in the reference's loop order (i outer, j inner); one
+// lane then adds them in that order, so the mean is the reference's bit for
+// bit.  The minimum of dt0 does not depend on the order and is a parallel
+// reduction.  Entries take host or device arrays (all of one kind); they
+// return the value, so they synchronise.
+#include <math.h>
+
+#include <vector>
+
+#include "m2d_internal.h"
+
+using namespace m2d;
+
+namespace {
+
+constexpr int kT = 256;
+
+// term of interior point (i, j) (1-based) -> t[(i-2)*(ny-2) + (j-2)]
+template <int ND>
+__global__ __launch_bounds__(kT) void k_cc_terms(const double *__restrict__ a, int nx, int ny,
+                                                double *__restrict__ t)
+{
+    const int i = blockIdx.x * kT + threadIdx.x + 2, j = blockIdx.y + 2;
+    if (i > nx - 1) return;
+    const long N = (long)nx * ny, np = (long)nx * (j - 1) + i - 1;
+    double s1, s2, ap;
+    if constexpr (ND == 9) {
+        s1 = fabs(a[np]) + fabs(a[N + np]) + fabs(a[2 * N + np]) + fabs(a[3 * N + np]);
+        s2 = fabs(a[5 * N + np]) + fabs(a[6 * N + np]) + fabs(a[7 * N + np]) +
+             fabs(a[8 * N + np]);
+        ap = fabs(a[4 * N + np]);
+    } else {
+        s1 = fabs(a[np]) + fabs(a[N + np]);
+        s2 = fabs(a[3 * N + np]) + fabs(a[4 * N + np]);
+        ap = fabs(a[2 * N + np]);
+    }
+    t[(long)(i - 2) * (ny - 2) + (j - 2)] = (s1 + s2) / ap;
+}
+
+// m = t(q) + m over the terms in order, then m / count (one lane)
+__global__ void k_seq_mean(const double *__restrict__ t, long n, double *__restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    double m = 0.0;
+    long q = 0;
+    for (; q + 8 <= n; q += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = t[q + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) m = v[k] + m;
+    }
+    for (; q < n; ++q) m = t[q] + m;
+    out[0] = m / (double)n;
+}
+
+// per-block minimum of 1/sqrt(Jp (u^2 + v^2)) over interior points
+__global__ __launch_bounds__(kT) void k_dt0_partial(const double *__restrict__ Jp,
+                                                   const double *__restrict__ u,
+                                                   const double *__restrict__ v, int nx, int ny,
+                                                   double *__restrict__ part)
+{
+    const int i = blockIdx.x * kT + threadIdx.x + 2, j = blockIdx.y + 2;
+    double m = 1.0e10;
+    if (i <= nx - 1) {
+        const long np = (long)nx * (j - 1) + i - 1;
+        const double aux = 1.0 / sqrt(Jp[np] * (u[np] * u[np] + v[np] * v[np]));
+        if (m > aux) m = aux;
+    }
+    __shared__ double red[kT];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = kT / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o && red[threadIdx.x] > red[threadIdx.x + o])
+            red[threadIdx.x] = red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kT) void k_min(const double *__restrict__ part, long n,
+                                           double *__restrict__ out)
+{
+    __shared__ double red[kT];
+    double m = 1.0e10;
+    for (long q = threadIdx.x; q < n; q += kT)
+        if (m > part[q]) m = part[q];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = kT / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o && red[threadIdx.x] > red[threadIdx.x + o])
+            red[threadIdx.x] = red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] * 0.25;
+}
+
+// device copies of host inputs (or the inputs themselves), plus scratch
+struct Staged {
+    std::vector<const double *> in;
+    double *buf = nullptr;
+    ~Staged()
+    {
+        if (buf) (void)hipFree(buf);
+    }
+};
+
+int stage(Staged &S, const std::vector<const double *> &ptrs, const std::vector<long> &lens,
+          long scratch, double **scratch_out, hipStream_t s)
+{
+    bool dev = is_device_array(ptrs[0]);
+    for (const double *p : ptrs)
+        if (!p || is_device_array(p) != dev) return M2D_ERR_ARG;
+    long tot = scratch;
+    if (!dev)
+        for (long n : lens) tot += n;
+    if (hipMalloc((void **)&S.buf, sizeof(double) * (size_t)(tot > 0 ? tot : 1)) != hipSuccess)
+        return M2D_ERR_HIP;
+    long off = 0;
+    for (size_t k = 0; k < ptrs.size(); ++k) {
+        if (dev) {
+            S.in.push_back(ptrs[k]);
+            continue;
+        }
+        if (hipMemcpyAsync(S.buf + off, ptrs[k], sizeof(double) * lens[k], hipMemcpyHostToDevice,
+                           s) != hipSuccess)
+            return M2D_ERR_HIP;
+        S.in.push_back(S.buf + off);
+        off += lens[k];
+    }
+    *scratch_out = S.buf + off;
+    return M2D_OK;
+}
+
+int mean_cc(int nx, int ny, const double *a, int nd, double *out)
+{
+    if (nx < 3 || ny < 3 || !a || !out) return M2D_ERR_ARG;
+    const hipStream_t s = assembly_stream_current();
+    const long N = (long)nx * ny, n = (long)(nx - 2) * (ny - 2);
+    Staged S;
+    double *t = nullptr;
+    int rc = stage(S, {a}, {nd * N}, n + 1, &t, s);
+    if (rc) return rc;
+    const dim3 grid((unsigned)((nx - 2 + kT - 1) / kT), (unsigned)(ny - 2));
+    if (nd == 9)
+        hipLaunchKernelGGL(k_cc_terms<9>, grid, dim3(kT), 0, s, S.in[0], nx, ny, t);
+    else
+        hipLaunchKernelGGL(k_cc_terms<5>, grid, dim3(kT), 0, s, S.in[0], nx, ny, t);
+    hipLaunchKernelGGL(k_seq_mean, dim3(1), dim3(64), 0, s, t, n, t + n);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(out, t + n, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return M2D_ERR_HIP;
+    return M2D_OK;
+}
+
+}  // namespace
+
+void m2d_set_last_status_code(int status);
+
+// get_mean_cc_9d (mod_tstepper_scarborough.f90:260-288)
+extern "C" double m2d_get_mean_cc_9d(const int *nx, const int *ny, const double *a)
+{
+    double v = 0.0;
+    const int rc = (!nx || !ny) ? M2D_ERR_ARG : mean_cc(*nx, *ny, a, 9, &v);
+    m2d_set_last_status_code(rc);
+    return v;
+}
+
+// get_mean_cc_5d (:292-320)
+extern "C" double m2d_get_mean_cc_5d(const int *nx, const int *ny, const double *a)
+{
+    double v = 0.0;
+    const int rc = (!nx || !ny) ? M2D_ERR_ARG : mean_cc(*nx, *ny, a, 5, &v);
+    m2d_set_last_status_code(rc);
+    return v;
+}
+
+// tstepper_dt0_scarborough (:40-76)
+extern "C" double m2d_tstepper_dt0_scarborough(const int *nx_, const int *ny_, const double *Jp,
+                                               const double *u, const double *v)
+{
+    double dt = 0.0;
+    int rc = M2D_OK;
+    if (!nx_ || !ny_ || *nx_ < 3 || *ny_ < 3 || !Jp || !u || !v) rc = M2D_ERR_ARG;
+    if (!rc) {
+        const int nx = *nx_, ny = *ny_;
+        const long N = (long)nx * ny;
+        const hipStream_t s = assembly_stream_current();
+        const dim3 grid((unsigned)((nx - 2 + kT - 1) / kT), (unsigned)(ny - 2));
+        const long np = (long)grid.x * grid.y;
+        Staged S;
+        double *part = nullptr;
+        rc = stage(S, {Jp, u, v}, {N, N, N}, np + 1, &part, s);
+        if (!rc) {
+            hipLaunchKernelGGL(k_dt0_partial, grid, dim3(kT), 0, s, S.in[0], S.in[1], S.in[2],
+                               nx, ny, part);
+            hipLaunchKernelGGL(k_min, dim3(1), dim3(kT), 0, s, part, np, part + np);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(&dt, part + np, sizeof(double), hipMemcpyDeviceToHost, s) !=
+                    hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                rc = M2D_ERR_HIP;
+        }
+    }
+    m2d_set_last_status_code(rc);
+    return dt;
+}
+
+// tstepper_dt_scarborough (:80-123); h0, mincc and maxcc are the module
+// variables tstepper_scarborough_init reads from the parameter file.
+extern "C" double m2d_tstepper_dt_scarborough(const int *nx, const int *ny, const double *dt,
+                                              const double *au, const double *av,
+                                              const double *at, const double *ap,
+                                              const double *h0, const double *mincc,
+                                              const double *maxcc)
+{
+    if (!nx || !ny || !dt || !h0 || !mincc || !maxcc) {
+        m2d_set_last_status_code(M2D_ERR_ARG);
+        return 0.0;
+    }
+    double cu = 0.0, cv = 0.0, ct = 0.0, cp = 0.0;
+    int rc = mean_cc(*nx, *ny, au, 9, &cu);
+    if (!rc) rc = mean_cc(*nx, *ny, av, 9, &cv);
+    if (!rc) rc = mean_cc(*nx, *ny, at, 9, &ct);
+    if (!rc) rc = mean_cc(*nx, *ny, ap, 5, &cp);
+    m2d_set_last_status_code(rc);
+    if (rc) return 0.0;
+    double c = cu;
+    if (cv > c) c = cv;
+    if (ct > c) c = ct;
+    if (cp > c) c = cp;
+    double h;
+    if (c < *mincc)
+        h = *h0 * sqrt(1.0 - c / *mincc);
+    else if (*maxcc < c)
+        h = -*h0 * sqrt(c / *maxcc - 1.0);
+    else
+        h = 0.0;
+    return *dt * (1.0 + h);
+}

@@ -63,6 +63,8 @@ EXPORTS = [
     "m2d_initialize_boundary_faces_flags", "m2d_check_mass_conservation",
     "m2d_intflow_set_bcu", "m2d_intflow_set_bcv", "m2d_intflow_set_bcpl", "m2d_intflow_set_bcT",
     "m2d_intflow_velocities_at_boundary_faces", "m2d_intflow_pl_loop",
+    "m2d_get_mean_cc_9d", "m2d_get_mean_cc_5d", "m2d_tstepper_dt0_scarborough",
+    "m2d_tstepper_dt_scarborough",
     "m2d_unet_conv3x3", "m2d_unet_conv3x3_wgrad", "m2d_unet_relu_bwd", "m2d_unet_mode_bwd",
     "m2d_unet_to_bf16", "m2d_unet_adam", "m2d_unet_pack_weights", "m2d_unet_mse",
     "m2d_unet_pack_input", "m2d_unet_unpack_output",
@@ -166,6 +168,13 @@ def load():
                                       C.c_double, C.c_int, C.c_int, C.c_double, _ip,
                                       C.POINTER(C.c_double)]
     L.m2d_intflow_pl_loop.restype = C.c_int
+    for f in (L.m2d_get_mean_cc_9d, L.m2d_get_mean_cc_5d):
+        f.argtypes = [_ip, _ip, _vp]
+        f.restype = C.c_double
+    L.m2d_tstepper_dt0_scarborough.argtypes = [_ip, _ip, _vp, _vp, _vp]
+    L.m2d_tstepper_dt0_scarborough.restype = C.c_double
+    L.m2d_tstepper_dt_scarborough.argtypes = [_ip, _ip, _dp, _vp, _vp, _vp, _vp, _dp, _dp, _dp]
+    L.m2d_tstepper_dt_scarborough.restype = C.c_double
     L.m2d_version.argtypes = []
     L.m2d_version.restype = C.c_char_p
     _lib = L

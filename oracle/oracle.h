@@ -114,4 +114,14 @@ void orc_intflow_velocities_at_boundary_faces(int nx, int ny, int modvis, const 
                                               const double *yk, const double *u, const double *v,
                                               double *ue, double *ve, double *un, double *vn,
                                               double *Uce, double *Vcn);
+
+/* tstep.c: Scarborough time-step statistics, src/tstepper/mod_tstepper_scarborough.f90 */
+double orc_get_mean_cc_9d(int nx, int ny, const double *a);
+double orc_get_mean_cc_5d(int nx, int ny, const double *a);
+double orc_tstepper_dt0_scarborough(int nx, int ny, const double *Jp, const double *u,
+                                    const double *v);
+double orc_tstepper_dt_scarborough(int nx, int ny, double dt, const double *au,
+                                   const double *av, const double *at, const double *ap,
+                                   double h0, double mincc, double maxcc);
+
 #endif

@@ -408,6 +408,59 @@ def ref_format_residual(it, dt, norm, normpl, rmass):
     return (row.raw[:200].decode().rstrip(" \x00"), head.raw[:200].decode().rstrip(" \x00"))
 
 
+def mean_cc(a, nx, ny):
+    """get_mean_cc_9d / get_mean_cc_5d (mod_tstepper_scarborough.f90:260-320)
+    of a (9, N) or (5, N) coefficient array."""
+    a = _c(a)
+    f = lib().orc_get_mean_cc_9d if a.shape[0] == 9 else lib().orc_get_mean_cc_5d
+    f.argtypes = [C.c_int, C.c_int, _dp]
+    f.restype = C.c_double
+    return f(nx, ny, a)
+
+
+def dt0_scarborough(Jp, u, v, nx, ny):
+    f = lib().orc_tstepper_dt0_scarborough
+    f.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+    f.restype = C.c_double
+    return f(nx, ny, _c(Jp), _c(u), _c(v))
+
+
+def dt_scarborough(dt, au, av, at, ap, nx, ny, h0, mincc, maxcc):
+    f = lib().orc_tstepper_dt_scarborough
+    f.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                  C.c_double]
+    f.restype = C.c_double
+    return f(nx, ny, dt, _c(au), _c(av), _c(at), _c(ap), h0, mincc, maxcc)
+
+
+def ref_mean_cc(a, nx, ny):
+    """The reference's own (module-private) get_mean_cc_9d / get_mean_cc_5d,
+    called through its symbol with Fortran by-reference arguments."""
+    a = _c(a)
+    sym = ("_QMmod_tstepper_scarboroughPget_mean_cc_9d" if a.shape[0] == 9
+           else "_QMmod_tstepper_scarboroughPget_mean_cc_5d")
+    f = getattr(ref(), sym)
+    f.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
+    f.restype = C.c_double
+    return f(C.byref(C.c_int(nx)), C.byref(C.c_int(ny)), a)
+
+
+def ref_dt0_scarborough(Jp, u, v, nx, ny):
+    f = ref().ref_tstepper_dt0_scarborough
+    f.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+    f.restype = C.c_double
+    return f(nx, ny, _c(Jp), _c(u), _c(v))
+
+
+def ref_dt_scarborough(paramfile, dt, au, av, at, ap, nx, ny):
+    """tstepper_scarborough_init from `paramfile` (h0, mincc, maxcc), then
+    tstepper_dt_scarborough."""
+    f = ref().ref_tstepper_dt_scarborough
+    f.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+    f.restype = C.c_double
+    return f(paramfile.encode(), nx, ny, dt, _c(au), _c(av), _c(at), _c(ap))
+
+
 def ref_tdma2d9(a, b, x, nx, ny, nitm):
     x = _c(x).copy()
     _big_stack_call(lambda: ref().ref_tdma2d9(nx, ny, nitm, _c(a), _c(b), x))

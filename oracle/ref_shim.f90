@@ -9,6 +9,8 @@
 module m2dref_shim
    use iso_c_binding
    use mod_class_ifile, only: class_ifile
+   use mod_tstepper_scarborough, only: tstepper_scarborough_init, tstepper_dt_scarborough, &
+      tstepper_dt0_scarborough
    use msi2d9
    use msi2d5
    use mtdma2d9
@@ -413,5 +415,34 @@ contains
          header(k) = h(k:k)
       end do
    end subroutine
+
+   ! src/tstepper/mod_tstepper_scarborough.f90: init from a parameter file
+   ! (h0, mincc, maxcc), then tstepper_dt_scarborough (:80) / dt0 (:40)
+   real(c_double) function ref_tstepper_dt_scarborough(fname, nx, ny, dt, au, av, at, ap) &
+         bind(C, name="ref_tstepper_dt_scarborough")
+      character(kind=c_char), intent(in) :: fname(*)
+      integer(c_int), value :: nx, ny
+      real(c_double), value :: dt
+      real(c_double), intent(in) :: au(nx*ny,9), av(nx*ny,9), at(nx*ny,9), ap(nx*ny,5)
+      type(class_ifile) :: f
+      character(len=1000) :: fn
+      integer :: i
+      fn = ' '
+      do i = 1, len(fn)
+         if (fname(i) == c_null_char) exit
+         fn(i:i) = fname(i)
+      end do
+      call f%init(trim(fn), "&")
+      call f%load()
+      call tstepper_scarborough_init(f)
+      ref_tstepper_dt_scarborough = tstepper_dt_scarborough(nx, ny, dt, au, av, at, ap)
+   end function
+
+   real(c_double) function ref_tstepper_dt0_scarborough(nx, ny, Jp, u, v) &
+         bind(C, name="ref_tstepper_dt0_scarborough")
+      integer(c_int), value :: nx, ny
+      real(c_double), dimension(nx*ny), intent(in) :: Jp, u, v
+      ref_tstepper_dt0_scarborough = tstepper_dt0_scarborough(nx, ny, Jp, u, v)
+   end function
 
 end module m2dref_shim

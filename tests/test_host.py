@@ -293,3 +293,18 @@ def test_residual_rows_match_fortran_formatting(orc):
         row, head = orc.ref_format_residual(*c)
         assert residual_row(*c) == row, c
         assert residual_header() == head
+
+
+def test_fortran_tstepper_module_compiles(orc, tmp_path):
+    """The drop-in mod_tstepper_scarborough compiles against the reference's
+    mod_class_ifile (module files of the reference build)."""
+    import shutil
+    import subprocess
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    moddir = os.path.join(ROOT, "oracle", "_ref", "mod")
+    if not os.path.exists(fc) or not os.path.isdir(moddir):
+        pytest.skip("no Fortran compiler or reference module files")
+    src = os.path.join(ROOT, "mach-2d_amd", "fortran", "tstepper_scarborough_hip.f90")
+    r = subprocess.run([fc, "-c", "-I", moddir, "-module-dir", str(tmp_path), "-o",
+                        str(tmp_path / "t.o"), src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

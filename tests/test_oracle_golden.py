@@ -187,3 +187,37 @@ def test_pl_loop_oracle_vs_reference(orc, modvis):
     assert ia == ib == 3 and na == nb
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+def _scarb_inputs(nx, ny, seed):
+    from mach2d import problems as P
+    rng = np.random.default_rng(seed)
+    au = P.pcorr_system(nx, ny, seed=seed).a
+    av = rng.uniform(-2.0, 2.0, size=(9, nx * ny))
+    at = rng.uniform(-1.0, 1.0, size=(9, nx * ny))
+    ap = P.pcorr_system5(nx, ny, seed=seed + 1).a
+    return au, av, at, ap
+
+
+@pytest.mark.parametrize("nx,ny", [(3, 3), (12, 9), (70, 33)])
+def test_scarborough_oracle_vs_reference(orc, nx, ny, tmp_path):
+    """get_mean_cc_9d / _5d (the reference's own module procedures, called
+    through their symbols), tstepper_dt0_scarborough and
+    tstepper_dt_scarborough on every branch of h (c < mincc, between,
+    c > maxcc), bit for bit."""
+    if not orc.ref_available():
+        pytest.skip("reference library not built")
+    au, av, at, ap = _scarb_inputs(nx, ny, nx + ny)
+    for a in (au, av, at, ap):
+        assert orc.mean_cc(a, nx, ny) == orc.ref_mean_cc(a, nx, ny)
+    rng = np.random.default_rng(1)
+    J, u, v = (rng.uniform(0.5, 2.0, nx * ny), rng.uniform(-5, 5, nx * ny),
+               rng.uniform(-5, 5, nx * ny))
+    assert orc.dt0_scarborough(J, u, v, nx, ny) == orc.ref_dt0_scarborough(J, u, v, nx, ny)
+    c = max(orc.mean_cc(a, nx, ny) for a in (au, av, at, ap))
+    for mincc, maxcc in ((2 * c, 4 * c), (0.5 * c, 2 * c), (0.1 * c, 0.5 * c)):
+        pf = tmp_path / "scarb.txt"
+        pf.write_text("   0.25 & h0 & amplitude\n %r & mincc & min\n %r & maxcc & max\n"
+                      % (mincc, maxcc))
+        want = orc.ref_dt_scarborough(str(pf), 1.5e-3, au, av, at, ap, nx, ny)
+        assert orc.dt_scarborough(1.5e-3, au, av, at, ap, nx, ny, 0.25, mincc, maxcc) == want
