@@ -266,6 +266,69 @@ def pl_loop_run(reps, cpu_itp=3):
     return out
 
 
+def grid_tstep_run(reps=20, nx=1026, ny=514):
+    """Once-per-run and once-per-step work either side of the solves on the
+    1026x514 grid: the tail of grid_create (centroids, metrics, radii of
+    mod_grid.f90) and the Scarborough time step (four mean convergence
+    coefficients of mod_tstepper_scarborough.f90), device arrays, HIP events
+    on the stream the kernels run on; the oracle (one core) beside it."""
+    import ctypes as C
+
+    import torch
+
+    from mach2d import _lib, grid, tstep
+    from mach2d import problems as P
+    L = _lib.load()
+    g = P.ogive_grid(nx, ny, 1.05)
+    N = nx * ny
+    x = torch.tensor(g.x.reshape(-1), device="cuda")
+    y = torch.tensor(g.y.reshape(-1), device="cuda")
+    xp, yp = grid.centroids(x, y, nx, ny, 2)
+    outs = [torch.empty(N, dtype=torch.float64, device="cuda") for _ in grid.METRICS]
+    pin = [C.c_void_p(t.data_ptr()) for t in (x, y, xp, yp)]
+    pout = [C.c_void_p(t.data_ptr()) for t in outs]
+    i_nx, i_ny = C.byref(C.c_int(nx)), C.byref(C.c_int(ny))
+    metrics = lambda: L.m2d_get_metrics(i_nx, i_ny, *pin, *pout)
+    full = lambda: grid.grid_create(x, y, nx, ny, kcm=2, coord=1)
+    rng = np.random.default_rng(7)
+    coefs = [torch.tensor(rng.uniform(-1, 1, k * N), device="cuda")
+             for k in (9, 9, 9, 5)]
+    dtf = lambda: tstep.dt(1e-3, *coefs, nx, ny, 0.1, 1.0, 10.0)
+    res = {}
+    for name, f, nbytes in (("metrics", metrics, 19 * 8 * N), ("grid_create", full, None),
+                            ("tstepper_dt", dtf, None)):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {"ms": round(ms, 4)}
+        if nbytes:
+            res[name]["gbps"] = round(nbytes / (ms * 1e-3) / 1e9, 1)
+            res[name]["frac_hbm_peak"] = round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        hx, hy = g.x.reshape(-1), g.y.reshape(-1)
+        t = time.perf_counter()
+        hxp, hyp = O.centroids(hx, hy, nx, ny, 2)
+        O.grid_metrics(hx, hy, hxp, hyp, nx, ny)
+        O.radius(1, hy, hyp, nx, ny)
+        res["grid_create"]["cpu_ms"] = round((time.perf_counter() - t) * 1e3, 2)
+        hc = [c.cpu().numpy() for c in coefs]
+        t = time.perf_counter()
+        O.dt_scarborough(1e-3, *hc, nx, ny, 0.1, 1.0, 10.0)
+        res["tstepper_dt"]["cpu_ms"] = round((time.perf_counter() - t) * 1e3, 2)
+    except Exception as e:  # the checker is optional here
+        log("grid/tstep cpu: %s" % e)
+    log("grid/tstep 1026x514: %s" % res)
+    return res
+
+
 def cpu_baseline(s, nit, budget_s=15.0):
     """Reference lu2d9 + fb2d9 (compiled Fortran, one core) on a bounded
     sample: the factorization plus M iterations; solves/s extrapolated to
@@ -459,6 +522,7 @@ def main():
             summary["learned_fp32_ir"] = lr
     msi5 = msi5_run(args.steps, args.warmup) if args.mode == "best" else None
     pl_loop = pl_loop_run(max(1, args.steps)) if args.mode == "best" else None
+    grid_ts = grid_tstep_run() if args.mode == "best" else None
     head = "fp32_ir" if "fp32_ir" in summary and summary["fp32_ir"]["correct"] else modes[0]
     r, e = runs[head], summary[head]
     st = r["st"]
@@ -511,6 +575,7 @@ def main():
         "batch_1026x514": batch3,
         "msi2d5_1026x514": msi5,
         "pl_loop_258x130": pl_loop,
+        "grid_tstep_1026x514": grid_ts,
         "target_comparison": target_comparison(summary, batch3),
     }
     print(json.dumps(out), flush=True)

@@ -463,6 +463,28 @@ double m2d_tstepper_dt_scarborough(const int *nx, const int *ny, const double *d
                                    const double *ap, const double *h0, const double *mincc,
                                    const double *maxcc);
 
+/* ---- Grid centroids, metrics and radii (src/grid/mod_grid.f90) ---------
+ * The steps of grid_create after the corner coordinates (:174-187).  Every
+ * point of every output is written, bit for bit the reference's values.
+ * Host or device arrays (all of one kind): device arrays are processed on
+ * the assembly stream (enqueue only), host arrays are copied in and out.
+ * Status: m2d_last_status. */
+
+/* get_real_centroids_xy, :237-300 (opt 1 simple mean, else weighted mean) */
+void m2d_get_real_centroids_xy(const int *opt, const int *nx, const int *ny, const double *x,
+                               const double *y, double *xp, double *yp);
+
+/* get_metrics, :304-558 */
+void m2d_get_metrics(const int *nx, const int *ny, const double *x, const double *y,
+                     const double *xp, const double *yp, double *xe, double *ye, double *xen,
+                     double *yen, double *xk, double *yk, double *xke, double *yke, double *Jp,
+                     double *Je, double *Jn, double *alphae, double *gamman, double *betae,
+                     double *betan);
+
+/* get_radius, :562-723 (coord 1 axisymmetric, else every radius 1) */
+void m2d_get_radius(const int *coord, const int *nx, const int *ny, const double *y,
+                    const double *yp, double *radius, double *re, double *rn, double *rp);
+
 /* ---- 5. Learned initial guess: U-Net kernels (device pointers) ---------- */
 /* No reference counterpart (the reference starts MSI from the caller's var).
  * BF16 NHWC tensors with channel counts that are multiples of 8 and 16-byte

@@ -65,6 +65,7 @@ EXPORTS = [
     "m2d_intflow_velocities_at_boundary_faces", "m2d_intflow_pl_loop",
     "m2d_get_mean_cc_9d", "m2d_get_mean_cc_5d", "m2d_tstepper_dt0_scarborough",
     "m2d_tstepper_dt_scarborough",
+    "m2d_get_real_centroids_xy", "m2d_get_metrics", "m2d_get_radius",
     "m2d_unet_conv3x3", "m2d_unet_conv3x3_wgrad", "m2d_unet_relu_bwd", "m2d_unet_mode_bwd",
     "m2d_unet_to_bf16", "m2d_unet_adam", "m2d_unet_pack_weights", "m2d_unet_mse",
     "m2d_unet_pack_input", "m2d_unet_unpack_output",
@@ -175,6 +176,11 @@ def load():
     L.m2d_tstepper_dt0_scarborough.restype = C.c_double
     L.m2d_tstepper_dt_scarborough.argtypes = [_ip, _ip, _dp, _vp, _vp, _vp, _vp, _dp, _dp, _dp]
     L.m2d_tstepper_dt_scarborough.restype = C.c_double
+    L.m2d_get_real_centroids_xy.argtypes = [_ip, _ip, _ip] + [_vp] * 4
+    L.m2d_get_metrics.argtypes = [_ip, _ip] + [_vp] * 19
+    L.m2d_get_radius.argtypes = [_ip, _ip, _ip] + [_vp] * 6
+    for f in (L.m2d_get_real_centroids_xy, L.m2d_get_metrics, L.m2d_get_radius):
+        f.restype = None
     L.m2d_version.argtypes = []
     L.m2d_version.restype = C.c_char_p
     _lib = L

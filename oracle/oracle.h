@@ -124,4 +124,15 @@ double orc_tstepper_dt_scarborough(int nx, int ny, double dt, const double *au,
                                    const double *av, const double *at, const double *ap,
                                    double h0, double mincc, double maxcc);
 
+
+/* grid.c: centroids, metrics and radii, src/grid/mod_grid.f90:237-723 */
+void orc_get_real_centroids_xy(int opt, int nx, int ny, const double *x, const double *y,
+                               double *xp, double *yp);
+void orc_get_metrics(int nx, int ny, const double *x, const double *y, const double *xp,
+                     const double *yp, double *xe, double *ye, double *xen, double *yen,
+                     double *xk, double *yk, double *xke, double *yke, double *Jp, double *Je,
+                     double *Jn, double *alphae, double *gamman, double *betae, double *betan);
+void orc_get_radius(int coord, int nx, int ny, const double *y, const double *yp,
+                    double *radius, double *re, double *rn, double *rp);
+
 #endif

@@ -408,6 +408,43 @@ def ref_format_residual(it, dt, norm, normpl, rmass):
     return (row.raw[:200].decode().rstrip(" \x00"), head.raw[:200].decode().rstrip(" \x00"))
 
 
+METRICS = ("xe", "ye", "xen", "yen", "xk", "yk", "xke", "yke", "Jp", "Je", "Jn", "alphae",
+           "gamman", "betae", "betan")
+
+
+def centroids(x, y, nx, ny, opt=1):
+    """get_real_centroids_xy (src/grid/mod_grid.f90:237-300) -> (xp, yp)."""
+    f = lib().orc_get_real_centroids_xy
+    f.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+    f.restype = None
+    n = nx * ny
+    xp, yp = np.zeros(n), np.zeros(n)
+    f(opt, nx, ny, _c(x), _c(y), xp, yp)
+    return xp, yp
+
+
+def grid_metrics(x, y, xp, yp, nx, ny):
+    """get_metrics (mod_grid.f90:304-558) -> {name: N-vector} (METRICS)."""
+    f = lib().orc_get_metrics
+    f.argtypes = [C.c_int, C.c_int] + [_dp] * 19
+    f.restype = None
+    n = nx * ny
+    out = {k: np.zeros(n) for k in METRICS}
+    f(nx, ny, _c(x), _c(y), _c(xp), _c(yp), *[out[k] for k in METRICS])
+    return out
+
+
+def radius(coord, y, yp, nx, ny):
+    """get_radius (mod_grid.f90:562-723) -> (radius, re, rn, rp)."""
+    f = lib().orc_get_radius
+    f.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+    f.restype = None
+    n = nx * ny
+    out = [np.zeros(n) for _ in range(4)]
+    f(coord, nx, ny, _c(y), _c(yp), *out)
+    return tuple(out)
+
+
 def mean_cc(a, nx, ny):
     """get_mean_cc_9d / get_mean_cc_5d (mod_tstepper_scarborough.f90:260-320)
     of a (9, N) or (5, N) coefficient array."""

@@ -111,7 +111,7 @@ def test_fortran_dropin_module_compiles(tmp_path):
     if not os.path.exists(fc):
         pytest.skip("no Fortran compiler")
     for name in ("msi2d9_hip.f90", "msi2d5_hip.f90", "coefficients_hip.f90",
-                 "intflow_bc_hip.f90"):
+                 "intflow_bc_hip.f90", "grid_metrics_hip.f90"):
         src = os.path.join(ROOT, "mach-2d_amd", "fortran", name)
         r = subprocess.run([fc, "-c", "-module-dir", str(tmp_path), "-o",
                             str(tmp_path / (name + ".o")), src], capture_output=True, text=True)

@@ -221,3 +221,157 @@ def test_scarborough_oracle_vs_reference(orc, nx, ny, tmp_path):
                       % (mincc, maxcc))
         want = orc.ref_dt_scarborough(str(pf), 1.5e-3, au, av, at, ap, nx, ny)
         assert orc.dt_scarborough(1.5e-3, au, av, at, ap, nx, ny, 0.25, mincc, maxcc) == want
+
+
+# ---------------------------------------------------------------------------
+# Grid centroids, metrics and radii (oracle/grid.c).  mod_grid.f90 does not
+# build here (its module uses mod_extflow, whose geometry module needs the
+# absent depp_interface), so these checks pin the restatement by the
+# mathematics: affine maps give constant metrics at every point the
+# reference writes (one-sided boundary differences included), tensor grids
+# give the spacing formulas, 1/Jp is the area of each quadrilateral, the
+# weighted centroid is the polygon centroid, and alpha, gamma, beta, J are
+# invariant under rotation.
+# ---------------------------------------------------------------------------
+
+def _affine_grid(nx, ny, A, b):
+    """corner (i, j), 1-based, at A (i, j) + b; flat index nx*(j-1) + i - 1"""
+    i, j = np.meshgrid(np.arange(1, nx + 1, dtype=float), np.arange(1, ny + 1, dtype=float))
+    x = A[0][0] * i + A[0][1] * j + b[0]
+    y = A[1][0] * i + A[1][1] * j + b[1]
+    return x.reshape(-1), y.reshape(-1)
+
+
+def _masks(nx, ny):
+    I, J = np.meshgrid(np.arange(1, nx + 1), np.arange(1, ny + 1))
+    east = (J >= 2) & (J <= ny - 1) & (I <= nx - 1)
+    north = (J <= ny - 1) & (I >= 2) & (I <= nx - 1)
+    inner = (I >= 2) & (I <= nx - 1) & (J >= 2) & (J <= ny - 1)
+    return east.reshape(-1), north.reshape(-1), inner.reshape(-1)
+
+
+@pytest.mark.parametrize("nx,ny", [(3, 3), (4, 7), (19, 11)])
+def test_grid_metrics_affine_closed_forms(orc, nx, ny):
+    A = [[0.7, -0.4], [0.3, 1.1]]
+    x, y = _affine_grid(nx, ny, A, (2.0, -1.0))
+    for opt in (1, 2):
+        xp, yp = orc.centroids(x, y, nx, ny, opt)
+        east, north, inner = _masks(nx, ny)
+        m = orc.grid_metrics(x, y, xp, yp, nx, ny)
+        det = A[0][0] * A[1][1] - A[0][1] * A[1][0]
+        want_e = {"xe": A[0][1], "ye": A[1][1], "xke": A[0][0], "yke": A[1][0],
+                  "alphae": A[0][1] ** 2 + A[1][1] ** 2,
+                  "betae": A[0][0] * A[0][1] + A[1][0] * A[1][1], "Je": 1 / abs(det)}
+        want_n = {"xk": A[0][0], "yk": A[1][0], "xen": A[0][1], "yen": A[1][1],
+                  "gamman": A[0][0] ** 2 + A[1][0] ** 2,
+                  "betan": A[0][0] * A[0][1] + A[1][0] * A[1][1], "Jn": 1 / abs(det)}
+        for k, v in want_e.items():
+            np.testing.assert_allclose(m[k][east], v, rtol=1e-12, atol=1e-12, err_msg=k)
+            assert not m[k][~east].any(), k
+        for k, v in want_n.items():
+            np.testing.assert_allclose(m[k][north], v, rtol=1e-12, atol=1e-12, err_msg=k)
+            assert not m[k][~north].any(), k
+        np.testing.assert_allclose(m["Jp"][inner], 1 / abs(det), rtol=1e-12)
+        assert not m["Jp"][~inner].any()
+        # centroids: the parallelogram's centre, zero outside real volumes
+        cx, cy = _affine_grid(nx, ny, A, (2.0 - 0.5 * (A[0][0] + A[0][1]),
+                                          -1.0 - 0.5 * (A[1][0] + A[1][1])))
+        np.testing.assert_allclose(xp[inner], cx[inner], rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(yp[inner], cy[inner], rtol=1e-12, atol=1e-12)
+        assert not xp[~inner].any() and not yp[~inner].any()
+
+
+def test_grid_metrics_stretched_tensor_grid(orc):
+    """Cartesian grid with geometric spacing: xke is the centroid spacing
+    inside and the adjacent cell width at i = 1 and nx-1 (likewise yen);
+    Jp = 1 / (cell area); beta = 0."""
+    nx, ny = 13, 9
+    X = np.cumsum(np.r_[0.0, 0.3 * 1.2 ** np.arange(nx - 1)])    # corners i = 1..nx
+    Y = np.cumsum(np.r_[0.0, 0.1 * 1.15 ** np.arange(ny - 1)])
+    x = np.tile(X, ny)
+    y = np.repeat(Y, nx)
+    xp, yp = orc.centroids(x, y, nx, ny)
+    m = orc.grid_metrics(x, y, xp, yp, nx, ny)
+    g = lambda a, i, j: a[nx * (j - 1) + i - 1]
+    for j in range(2, ny):
+        assert g(m["xke"], 1, j) == pytest.approx(X[1] - X[0], rel=1e-12)
+        assert g(m["xke"], nx - 1, j) == pytest.approx(X[nx - 2] - X[nx - 3], rel=1e-12)
+        for i in range(2, nx - 1):
+            assert g(m["xke"], i, j) == pytest.approx((X[i] - X[i - 2]) / 2, rel=1e-12)
+    for i in range(2, nx):
+        assert g(m["yen"], i, 1) == pytest.approx(Y[1] - Y[0], rel=1e-12)
+        assert g(m["yen"], i, ny - 1) == pytest.approx(Y[ny - 2] - Y[ny - 3], rel=1e-12)
+        for j in range(2, ny - 1):
+            assert g(m["yen"], i, j) == pytest.approx((Y[j] - Y[j - 2]) / 2, rel=1e-12)
+        for j in range(2, ny):
+            area = (X[i - 1] - X[i - 2]) * (Y[j - 1] - Y[j - 2])
+            assert g(m["Jp"], i, j) == pytest.approx(1 / area, rel=1e-12)
+    for k in ("betae", "betan", "yke", "xen"):    # zero up to rounding
+        assert np.abs(m[k]).max() < 1e-14, k
+
+
+def _perturbed_grid(nx, ny, seed, amp=0.2):
+    rng = np.random.default_rng(seed)
+    x, y = _affine_grid(nx, ny, [[1.0, 0.15], [-0.1, 0.8]], (0.0, 0.0))
+    return x + amp * rng.uniform(-1, 1, x.size), y + amp * rng.uniform(-1, 1, y.size)
+
+
+def test_grid_jp_area_and_weighted_centroid(orc):
+    """1/Jp is the shoelace area of each quadrilateral; opt 2 gives its
+    polygon centroid."""
+    nx, ny = 12, 10
+    x, y = _perturbed_grid(nx, ny, 3)
+    xp, yp = orc.centroids(x, y, nx, ny, opt=2)
+    m = orc.grid_metrics(x, y, xp, yp, nx, ny)
+    for j in range(2, ny):
+        for i in range(2, nx):
+            q = [nx * (j - 2) + i - 2, nx * (j - 2) + i - 1, nx * (j - 1) + i - 1,
+                 nx * (j - 1) + i - 2]                          # SW, SE, NE, NW
+            px, py = x[q], y[q]
+            cr = px * np.roll(py, -1) - np.roll(px, -1) * py
+            area = 0.5 * cr.sum()
+            cx = ((px + np.roll(px, -1)) * cr).sum() / (6 * area)
+            cy = ((py + np.roll(py, -1)) * cr).sum() / (6 * area)
+            p = nx * (j - 1) + i - 1
+            assert m["Jp"][p] == pytest.approx(1 / abs(area), rel=1e-12)
+            assert xp[p] == pytest.approx(cx, rel=1e-12, abs=1e-12)
+            assert yp[p] == pytest.approx(cy, rel=1e-12, abs=1e-12)
+            assert xp[p] != pytest.approx(px.mean(), rel=1e-9, abs=1e-9) or \
+                yp[p] != pytest.approx(py.mean(), rel=1e-9, abs=1e-9)
+
+
+def test_grid_metrics_rotation_invariance(orc):
+    nx, ny = 15, 8
+    x, y = _perturbed_grid(nx, ny, 5)
+    th = 0.7
+    xr, yr = np.cos(th) * x - np.sin(th) * y, np.sin(th) * x + np.cos(th) * y
+    m = orc.grid_metrics(x, y, *orc.centroids(x, y, nx, ny), nx, ny)
+    mr = orc.grid_metrics(xr, yr, *orc.centroids(xr, yr, nx, ny), nx, ny)
+    for k in ("alphae", "gamman", "betae", "betan", "Je", "Jn", "Jp"):
+        np.testing.assert_allclose(mr[k], m[k], rtol=1e-10, atol=1e-12, err_msg=k)
+    # face tangents rotate as vectors
+    np.testing.assert_allclose(mr["xke"], np.cos(th) * m["xke"] - np.sin(th) * m["yke"],
+                               atol=1e-12)
+
+
+@pytest.mark.parametrize("nx,ny", [(3, 3), (6, 5), (17, 12)])
+def test_grid_radius_affine_reflection(orc, nx, ny):
+    """coord 1 on an affine grid: every rp (fictitious volumes and corners
+    included) is y at the volume's centre, re / rn the face centres; coord 0
+    gives ones."""
+    A = [[0.4, 0.2], [0.1, 0.9]]
+    x, y = _affine_grid(nx, ny, A, (0.5, 3.0))
+    xp, yp = orc.centroids(x, y, nx, ny)
+    rad, re, rn, rp = orc.radius(1, y, yp, nx, ny)
+    _, cy = _affine_grid(nx, ny, A, (0.5 - 0.5 * (A[0][0] + A[0][1]),
+                                     3.0 - 0.5 * (A[1][0] + A[1][1])))
+    np.testing.assert_allclose(rp, cy, rtol=1e-12, atol=1e-12)
+    assert np.array_equal(rad, y)
+    east, north, _ = _masks(nx, ny)
+    _, ey = _affine_grid(nx, ny, A, (0.5, 3.0 - 0.5 * A[1][1]))
+    _, ny_ = _affine_grid(nx, ny, A, (0.5, 3.0 - 0.5 * A[1][0]))
+    np.testing.assert_allclose(re[east], ey[east], rtol=1e-12)
+    np.testing.assert_allclose(rn[north], ny_[north], rtol=1e-12)
+    assert (re[~east] == 1).all() and (rn[~north] == 1).all()
+    for a in orc.radius(0, y, yp, nx, ny):
+        assert (a == 1).all()
