@@ -8,7 +8,8 @@
 // right as the reference's SUM intrinsic adds them) are computed in parallel
 // into a scratch array in the reference's loop order (i outer, j inner); one
 // lane then adds them in that order, so the mean is the reference's bit for
-// bit.  The minimum of dt0 does not depend on the order and is a parallel
+// bit.  That chain of dependent additions bounds the call (about 10 cycles
+// per point); tstepper_dt runs its four means concurrently.  The minimum of dt0 does not depend on the order and is a parallel
 // reduction.  Entries take host or device arrays (all of one kind); they
 // return the value, so they synchronise.
 #include <math.h>
@@ -45,21 +46,56 @@ __global__ __launch_bounds__(kT) void k_cc_terms(const double *__restrict__ a, i
     t[(long)(i - 2) * (ny - 2) + (j - 2)] = (s1 + s2) / ap;
 }
 
-// m = t(q) + m over the terms in order, then m / count (one lane)
-__global__ void k_seq_mean(const double *__restrict__ t, long n, double *__restrict__ out)
+// m = t(q) + m over the terms in order, then m / n; one workgroup per
+// array of n terms (t + blockIdx.x * n).  The dependent chain of additions
+// is the whole cost, so lane 0 of wave 0 only adds, from LDS, while the
+// other waves stage the next chunk of terms.
+constexpr int kSeqT = 256, kChunk = 2048;
+
+__global__ __launch_bounds__(kSeqT) void k_seq_mean(const double *__restrict__ t, long n,
+                                                   double *__restrict__ out)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ double buf[2][kChunk];
+    const double *__restrict__ tt = t + (long)blockIdx.x * n;
+    const int tid = threadIdx.x;
+    const long nch = (n + kChunk - 1) / kChunk;
+    for (long q = tid; q < kChunk && q < n; q += kSeqT) buf[0][q] = tt[q];
+    __syncthreads();
     double m = 0.0;
-    long q = 0;
-    for (; q + 8 <= n; q += 8) {
-        double v[8];
+    for (long c = 0; c < nch; ++c) {
+        const int cur = (int)(c & 1);
+        const long base = c * kChunk;
+        const int len = (int)min((long)kChunk, n - base);
+        if (tid >= 64) {
+            const long nb = base + kChunk;
+            if (nb < n) {
+                const int nlen = (int)min((long)kChunk, n - nb);
+                for (int q = tid - 64; q < nlen; q += kSeqT - 64) buf[cur ^ 1][q] = tt[nb + q];
+            }
+        } else if (tid == 0) {
+            const double *b = buf[cur];
+            int q = 0;
+            if (len >= 8) {
+                double v[8], w[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = t[q + k];
+                for (int k = 0; k < 8; ++k) v[k] = b[k];
+                for (; q + 16 <= len; q += 8) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) m = v[k] + m;
+                    for (int k = 0; k < 8; ++k) w[k] = b[q + 8 + k];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) m = v[k] + m;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) v[k] = w[k];
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) m = v[k] + m;
+                q += 8;
+            }
+            for (; q < len; ++q) m = b[q] + m;
+        }
+        __syncthreads();
     }
-    for (; q < n; ++q) m = t[q] + m;
-    out[0] = m / (double)n;
+    if (tid == 0) out[blockIdx.x] = m / (double)n;
 }
 
 // per-block minimum of 1/sqrt(Jp (u^2 + v^2)) over interior points
@@ -140,23 +176,31 @@ int stage(Staged &S, const std::vector<const double *> &ptrs, const std::vector<
     return M2D_OK;
 }
 
-int mean_cc(int nx, int ny, const double *a, int nd, double *out)
+// means of K coefficient arrays (nd[k] diagonals each): the terms of all
+// K into one scratch, then K concurrent in-order sums
+int mean_cc(int nx, int ny, int K, const double *const *a, const int *nd, double *out)
 {
-    if (nx < 3 || ny < 3 || !a || !out) return M2D_ERR_ARG;
+    if (nx < 3 || ny < 3 || !out) return M2D_ERR_ARG;
     const hipStream_t s = assembly_stream_current();
     const long N = (long)nx * ny, n = (long)(nx - 2) * (ny - 2);
+    std::vector<const double *> ptrs(a, a + K);
+    std::vector<long> lens;
+    for (int k = 0; k < K; ++k) lens.push_back(nd[k] * N);
     Staged S;
     double *t = nullptr;
-    int rc = stage(S, {a}, {nd * N}, n + 1, &t, s);
+    int rc = stage(S, ptrs, lens, K * n + K, &t, s);
     if (rc) return rc;
     const dim3 grid((unsigned)((nx - 2 + kT - 1) / kT), (unsigned)(ny - 2));
-    if (nd == 9)
-        hipLaunchKernelGGL(k_cc_terms<9>, grid, dim3(kT), 0, s, S.in[0], nx, ny, t);
-    else
-        hipLaunchKernelGGL(k_cc_terms<5>, grid, dim3(kT), 0, s, S.in[0], nx, ny, t);
-    hipLaunchKernelGGL(k_seq_mean, dim3(1), dim3(64), 0, s, t, n, t + n);
+    for (int k = 0; k < K; ++k) {
+        if (nd[k] == 9)
+            hipLaunchKernelGGL(k_cc_terms<9>, grid, dim3(kT), 0, s, S.in[k], nx, ny, t + k * n);
+        else
+            hipLaunchKernelGGL(k_cc_terms<5>, grid, dim3(kT), 0, s, S.in[k], nx, ny, t + k * n);
+    }
+    hipLaunchKernelGGL(k_seq_mean, dim3(K), dim3(kSeqT), 0, s, t, n, t + K * n);
     if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(out, t + n, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(out, t + K * n, sizeof(double) * K, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return M2D_ERR_HIP;
     return M2D_OK;
@@ -170,7 +214,8 @@ void m2d_set_last_status_code(int status);
 extern "C" double m2d_get_mean_cc_9d(const int *nx, const int *ny, const double *a)
 {
     double v = 0.0;
-    const int rc = (!nx || !ny) ? M2D_ERR_ARG : mean_cc(*nx, *ny, a, 9, &v);
+    const int nd = 9;
+    const int rc = (!nx || !ny) ? M2D_ERR_ARG : mean_cc(*nx, *ny, 1, &a, &nd, &v);
     m2d_set_last_status_code(rc);
     return v;
 }
@@ -179,7 +224,8 @@ extern "C" double m2d_get_mean_cc_9d(const int *nx, const int *ny, const double 
 extern "C" double m2d_get_mean_cc_5d(const int *nx, const int *ny, const double *a)
 {
     double v = 0.0;
-    const int rc = (!nx || !ny) ? M2D_ERR_ARG : mean_cc(*nx, *ny, a, 5, &v);
+    const int nd = 5;
+    const int rc = (!nx || !ny) ? M2D_ERR_ARG : mean_cc(*nx, *ny, 1, &a, &nd, &v);
     m2d_set_last_status_code(rc);
     return v;
 }
@@ -227,13 +273,13 @@ extern "C" double m2d_tstepper_dt_scarborough(const int *nx, const int *ny, cons
         m2d_set_last_status_code(M2D_ERR_ARG);
         return 0.0;
     }
-    double cu = 0.0, cv = 0.0, ct = 0.0, cp = 0.0;
-    int rc = mean_cc(*nx, *ny, au, 9, &cu);
-    if (!rc) rc = mean_cc(*nx, *ny, av, 9, &cv);
-    if (!rc) rc = mean_cc(*nx, *ny, at, 9, &ct);
-    if (!rc) rc = mean_cc(*nx, *ny, ap, 5, &cp);
+    const double *a[4] = {au, av, at, ap};
+    const int nd[4] = {9, 9, 9, 5};
+    double cc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int rc = mean_cc(*nx, *ny, 4, a, nd, cc);
     m2d_set_last_status_code(rc);
     if (rc) return 0.0;
+    const double cu = cc[0], cv = cc[1], ct = cc[2], cp = cc[3];
     double c = cu;
     if (cv > c) c = cv;
     if (ct > c) c = ct;
