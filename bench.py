@@ -266,6 +266,66 @@ def pl_loop_run(reps, cpu_itp=3):
     return out
 
 
+def time_step_run(steps=20, nx=258, ny=130, cpu_steps=3):
+    """The full SIMPLEC time step of the internal-flow case (Euler, constant
+    properties, the shipped nozzle parameters: itmmax 1, itpmax 20, nitm_u 5,
+    nitm_p 20, tolerances 1e-2, Scarborough time step) on a 258x130 grid,
+    device-resident (m2d_intflow_time_step), u and v by MSI and by TDMA;
+    the same step composed from the reference's routines (compiled Fortran,
+    one core) beside it when built."""
+    import torch
+
+    from mach2d import simplec, solvers
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import time_step_oracle as TS
+    out = {}
+    f0 = TS.step_fields(nx, ny, seed=5)
+    for name, solver9d in (("msi", 0), ("tdma", 1)):
+        prm = dict(TS.PARAMS, solver9d=solver9d)
+        d = {k: torch.tensor(np.ascontiguousarray(v).reshape(-1), device="cuda")
+             for k, v in f0.items()}
+        dev9 = solvers.DeviceMSI(nx, ny) if solver9d == 0 else None
+        dev5 = solvers.DeviceMSI(nx, ny, ndiag=5)
+        torch.cuda.synchronize()   # the copies above run on torch's stream
+        dt = 2e-5
+        dt, r = simplec.intflow_time_step(dev9, dev5, d, prm, dt)   # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        itps = 0
+        for _ in range(steps):
+            dt, r = simplec.intflow_time_step(dev9, dev5, d, prm, dt)
+            itps += r["itp"]
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) / steps * 1e3
+        out[name] = {"ms_per_step": round(ms, 3), "itp_per_step": round(itps / steps, 2),
+                     "ms_200_steps": round(200 * ms, 1)}
+        for h in (dev9, dev5):
+            if h is not None:
+                h.close()
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        ref = O.ref_available()
+        pf = os.path.join(ROOT, "tests", "golden", "parametersNozzleEulerPTC.txt")
+        # the first step (20 pressure-correction iterations from the initial
+        # field) is the GPU's untimed warm-up: leave it out here too
+        s, dt, _r = TS.step(O, f0, nx, ny, dict(TS.PARAMS), 2e-5, ref=ref, paramfile=pf)
+        itps = 0
+        t = time.perf_counter()
+        for _ in range(cpu_steps):
+            s, dt, _r = TS.step(O, s, nx, ny, dict(TS.PARAMS), dt, ref=ref, paramfile=pf)
+            itps += _r["itp"]
+        cpu = (time.perf_counter() - t) / cpu_steps * 1e3
+        out["cpu_ms_per_step"] = round(cpu, 2)
+        out["cpu_itp_per_step"] = round(itps / cpu_steps, 2)
+        out["cpu_kind"] = "reference" if ref else "port"
+        out["speedup_msi_vs_cpu"] = round(cpu / out["msi"]["ms_per_step"], 2)
+    except Exception as e:  # the checker is optional here
+        log("time step cpu: %s" % e)
+    log("time step 258x130: %s" % out)
+    return out
+
+
 def grid_tstep_run(reps=20, nx=1026, ny=514):
     """Once-per-run and once-per-step work either side of the solves on the
     1026x514 grid: the tail of grid_create (centroids, metrics, radii of
@@ -523,6 +583,7 @@ def main():
     msi5 = msi5_run(args.steps, args.warmup) if args.mode == "best" else None
     pl_loop = pl_loop_run(max(1, args.steps)) if args.mode == "best" else None
     grid_ts = grid_tstep_run() if args.mode == "best" else None
+    tstep_full = time_step_run() if args.mode == "best" else None
     head = "fp32_ir" if "fp32_ir" in summary and summary["fp32_ir"]["correct"] else modes[0]
     r, e = runs[head], summary[head]
     st = r["st"]
@@ -576,6 +637,7 @@ def main():
         "msi2d5_1026x514": msi5,
         "pl_loop_258x130": pl_loop,
         "grid_tstep_1026x514": grid_ts,
+        "simplec_time_step_258x130": tstep_full,
         "target_comparison": target_comparison(summary, batch3),
     }
     print(json.dumps(out), flush=True)

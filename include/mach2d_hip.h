@@ -118,6 +118,14 @@ void m2d_norm_l1_9d(const int *nx, const int *ny, const double *var,
 void m2d_norm_l1_9d_relative(const int *nx, const int *ny, const double *var,
                              const double *b, const double *a, double *norm);
 double m2d_norm_l1_b(const int *nx, const int *ny, const double *b);
+/* norm_l1_5d, mod_solvers.f90:91-125 (a: 5 diagonals S, W, P, E, N; host
+ * or device arrays, norm a host scalar) */
+void m2d_norm_l1_5d(const int *nx, const int *ny, const double *var, const double *b,
+                    const double *a, double *norm);
+/* get_average, src/util/mod_field_operations.f90:115-142: the mean over
+ * real volumes, added in the reference's order (bit for bit); host or
+ * device array */
+double m2d_get_average(const int *nx, const int *ny, const double *F);
 
 /* Status of the last call made by this host thread (no reference
  * counterpart: the reference routines return nothing and fail silently). */
@@ -410,6 +418,37 @@ void m2d_intflow_velocities_at_boundary_faces(const int *nx, const int *ny, cons
                                               double *ve, double *un, double *vn, double *Uce,
                                               double *Vcn);
 
+/* intflow_boundary_simplec, :791-877 (in/out; modvis unused, kept for the
+ * reference's argument list) */
+void m2d_intflow_boundary_simplec(const int *nx, const int *ny, const int *modvis, double *due,
+                                  double *dve, double *dun, double *dvn, double *de,
+                                  double *dn);
+/* intflow_extrapolate_u_v_to_fictitious, :881-964 (u, v in/out; nx >= 4,
+ * the north slip extrapolation reads the third column) */
+void m2d_intflow_extrapolate_u_v_to_fictitious(const int *nx, const int *ny, const int *modvis,
+                                               const double *x, const double *xp,
+                                               const double *xk, const double *yk, double *u,
+                                               double *v);
+/* intflow_boundary_T_from_H_conservation, :557-688 (iflow contributes Tin,
+ * ny values; T in/out; Tbe, Tbw: ny values, Tbn, Tbs: nx values) */
+void m2d_intflow_boundary_T_from_H_conservation(const int *nx, const int *ny,
+                                                const double *Cpref, const double *Href,
+                                                const double *Tin, const double *u,
+                                                const double *ue, const double *un,
+                                                const double *v, const double *ve,
+                                                const double *vn, double *T, double *Tbe,
+                                                double *Tbw, double *Tbn, double *Tbs);
+/* intflow_time_cycle_update, :1106-1134: gamma = thermomodel%gm(T0),
+ * Rg = thermomodel%Rg, P0 and T0 of iflow; the inflow vectors (ny values)
+ * are the iflow fields of the same names; p in/out.  The inflow pressure is
+ * a real power: the device pow may differ from the host libm in the last
+ * bit, so pin, plin and the fictitious pressures agree to 2 ulp. */
+void m2d_intflow_time_cycle_update(const int *nx, const int *ny, const double *gamma,
+                                   const double *Rg, const double *P0, const double *T0,
+                                   const double *x, const double *xp, const double *u,
+                                   double *uin, double *vin, double *pin, double *Tin,
+                                   double *Mw, double *plin, double *pina, double *p);
+
 /* Device arrays of the internal-flow pressure-correction loop: N-vectors in
  * the reference layout (N = nx*ny), plin of ny values. */
 typedef struct m2d_pl_loop_fields {
@@ -441,6 +480,68 @@ typedef struct m2d_pl_loop_fields {
 int m2d_intflow_pl_loop(m2d_system *s5, const m2d_pl_loop_fields *f, int modvis, double dt,
                         double beta, double p_avg, double tolm, int itpmax, int nitm_p,
                         double tol_p, int *itp_out, double *normpl_out);
+
+/* Device arrays of one internal-flow time step (src/mach2d_main.f90:94-
+ * 404): N-vectors in the reference layout unless stated. */
+typedef struct m2d_time_step_fields {
+    /* grid and metrics */
+    const double *x, *xp, *xe, *ye, *xk, *yk, *xke, *yke, *xen, *yen, *rp, *re, *rn, *Jp, *Je,
+        *Jn, *alphae, *betae, *betan, *gamman;
+    /* properties: laminar viscosity at faces and nodes (zeros for Euler);
+     * for the T equation (modvis = 1) cp, ke, kn and the boundary-face
+     * flags fbe, fbn (may be NULL for Euler) */
+    const double *vle, *vln, *vlp, *cp, *ke, *kn, *fbe, *fbn;
+    /* state, updated in place */
+    double *u, *v, *ue, *ve, *un, *vn, *Uce, *Vcn, *p, *T, *ro, *roe, *ron, *pl;
+    /* previous time level (written by the step) */
+    double *ua, *va, *uea, *vea, *una, *vna, *pa, *Ta, *roa;
+    /* linear systems (au, av, at: 9 N; ap: 5 N, its corner rows keep their
+     * initial values as in the reference) and work arrays */
+    double *au, *av, *at, *ap, *bu, *bv, *bt, *bp, *cup, *sup, *cvp, *svp, *due, *dve, *dun,
+        *dvn, *de, *dn, *g;
+    /* iflow: inflow vectors of ny values; boundary temperatures Tbe, Tbw (ny
+     * values), Tbn, Tbs (nx values) */
+    double *uin, *vin, *pin, *Tin, *Mw, *plin, *pina, *Tbe, *Tbw, *Tbn, *Tbs;
+} m2d_time_step_fields;
+
+typedef struct m2d_time_step_params {
+    int modvis;      /* 0 Euler (T from total enthalpy), 1 Navier-Stokes (T equation) */
+    int coord;       /* 1 axisymmetric, 0 planar */
+    int solver9d;    /* u, v, T solver: 0 MSI (lu2d9 + fb2d9), 1 TDMA (tdma2d9) */
+    int itmmax, itpmax, nitm_u, nitm_p;
+    double tol_u, tol_p, tolm, beta;
+    double Rg, gamma, P0, T0, Cpref, Href, Twall;  /* gamma = thermomodel%gm(T0) */
+    double h0, mincc, maxcc;                        /* Scarborough time stepper */
+} m2d_time_step_params;
+
+typedef struct m2d_time_step_result {
+    double norm;     /* sum of the relative L1 residual norms of the last mass cycle */
+    double rmass;    /* check_mass_conservation */
+    double normpl;   /* max|pl| / p_avg of the last pressure-correction iteration */
+    int itp;         /* pressure-correction iterations of the last mass cycle */
+    int stage;       /* 0, or the part of the step that failed: 1 time levels,
+                        2 inflow update, 3 p average and g, 4 u, v assembly,
+                        5 u solve, 6 v solve, 7 face velocities, 8 p' loop,
+                        9 node velocities, 10 mass check, 11 temperature,
+                        12 densities, 13 next dt */
+} m2d_time_step_result;
+
+/* One time step of the internal-flow case with constant thermophysical
+ * properties, src/mach2d_main.f90:94-404: time levels,
+ * intflow_time_cycle_update, get_average(p), g; itmmax mass cycles (u, v
+ * coefficients, sources and fictitious rows, SIMPLEC coefficients, the u
+ * and v solves, face velocities, the pressure-correction loop
+ * (m2d_intflow_pl_loop), norm_l1_5d, the node velocity correction and the
+ * extrapolation to fictitious volumes); check_mass_conservation; T from
+ * total enthalpy (Euler) or the T equation (NS); the state equation and
+ * face densities; the Scarborough time step (*dt in: this step's, out: the
+ * next).  All on the handles' stream (s5) with device arrays; the host reads
+ * a few scalars (the exit tests, p_avg, the norms, rmass, dt).  s9: a
+ * 9-diagonal FP64 handle (solver9d = 0; may be NULL for TDMA); s5: a
+ * 5-diagonal FP64 handle of the same grid. */
+int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5, const m2d_time_step_fields *f,
+                          const m2d_time_step_params *p, double *dt,
+                          m2d_time_step_result *res);
 
 /* ---- Time-step statistics (src/tstepper/mod_tstepper_scarborough.f90) ---
  * Host or device arrays (all of one kind; device arrays on the assembly

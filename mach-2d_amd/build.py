@@ -17,7 +17,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "mach2d", "libmach2d_hip.so")
 SOURCES = ["sweep.hip", "msi9.hip", "tdma9.hip", "capi.hip", "batch.hip", "assembly.hip",
-           "unet.hip", "tstep.hip", "grid.hip"]
+           "unet.hip", "tstep.hip", "grid.hip",
+           "intflow.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

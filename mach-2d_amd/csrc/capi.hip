@@ -969,3 +969,261 @@ extern "C" int m2d_kernel_timing(m2d_system *s, int kernel, double *total_ms, lo
     *launches = s->kt_n[kernel];
     return M2D_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Device-resident time step of the internal-flow case
+// ---------------------------------------------------------------------------
+namespace m2d {
+int norm_l1_device(const double *a, const double *b, const double *var, int nx, int ny,
+                   double *d_work, double *h_out, hipStream_t s);
+int norm_l1_work_size(int nx, int ny);
+}  // namespace m2d
+
+// g = 1 / (Rg T) (mach2d_main.f90:153)
+__global__ __launch_bounds__(256) void k_g_nodes(const double *__restrict__ T, double Rg, long n,
+                                                 double *__restrict__ g)
+{
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q < n) g[q] = 1.0 / (Rg * T[q]);
+}
+
+// T from the conservation of total enthalpy (mach2d_main.f90:352)
+__global__ __launch_bounds__(256) void k_T_nodes(const double *__restrict__ u,
+                                                 const double *__restrict__ v, double Href,
+                                                 double Cpref, long n, double *__restrict__ T)
+{
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q < n) T[q] = (Href - (u[q] * u[q] + v[q] * v[q]) / 2.0) / Cpref;
+}
+
+// solver9d%solve (mod_solver_msi.f90:96-133 or mod_solver_tdma.f90) on the
+// 9-diagonal system a x = b from the current x
+static int solve9(m2d_system *s9, const m2d_time_step_params *P, const double *a,
+                  const double *b, double *x, int nx, int ny, hipStream_t stream)
+{
+    if (P->solver9d == 1) return m2d_tdma2d9_device(nx, ny, P->nitm_u, a, b, x, stream);
+    // the 9-diagonal handle has its own stream: the system assembled on
+    // `stream` must be complete before its factorization starts (read_state
+    // below completes the solve before `stream` uses x)
+    if (hipStreamSynchronize(stream) != hipSuccess) return M2D_ERR_HIP;
+    enqueue_factor64(s9, a);
+    enqueue_solve64(s9, a, b, x, P->tol_u, P->nitm_u);
+    int rc = read_state(s9, s9->st);
+    if (rc) return rc;
+    const SolveState &st = *s9->h_st;
+    if (st.outside) return M2D_ERR_OUTSIDE_COUPLING;
+    if (st.err_pivot || st.status == M2D_ERR_NONFINITE) return M2D_ERR_NONFINITE;
+    if (st.abort || st.status == M2D_ERR_SYNC_TIMEOUT) return M2D_ERR_SYNC_TIMEOUT;
+    return M2D_OK;
+}
+
+extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
+                                     const m2d_time_step_fields *f,
+                                     const m2d_time_step_params *P, double *dt_io,
+                                     m2d_time_step_result *res)
+{
+    if (!s5 || !f || !P || !dt_io || s5->nd != 5 || s5->precision != M2D_FP64 ||
+        P->itmmax < 1 || P->nitm_u < 1 || (P->solver9d != 0 && P->solver9d != 1))
+        return M2D_ERR_ARG;
+    if (P->solver9d == 0 && (!s9 || s9->nd != 9 || s9->precision != M2D_FP64 ||
+                             s9->nx != s5->nx || s9->ny != s5->ny))
+        return M2D_ERR_ARG;
+    const bool euler = P->modvis == 0;
+    const void *need[] = {f->x,    f->xp,   f->xe,   f->ye,   f->xk,   f->yk,  f->xke, f->yke,
+                          f->xen,  f->yen,  f->rp,   f->re,   f->rn,   f->Jp,  f->Je,  f->Jn,
+                          f->alphae, f->betae, f->betan, f->gamman, f->vle, f->vln, f->vlp,
+                          f->u,    f->v,    f->ue,   f->ve,   f->un,   f->vn,  f->Uce, f->Vcn,
+                          f->p,    f->T,    f->ro,   f->roe,  f->ron,  f->pl,  f->ua,  f->va,
+                          f->uea,  f->vea,  f->una,  f->vna,  f->pa,   f->Ta,  f->roa, f->au,
+                          f->av,   f->at,   f->ap,   f->bu,   f->bv,   f->bt,  f->bp,  f->cup,
+                          f->sup,  f->cvp,  f->svp,  f->due,  f->dve,  f->dun, f->dvn, f->de,
+                          f->dn,   f->g,    f->uin,  f->vin,  f->pin,  f->Tin, f->Mw,  f->plin,
+                          f->pina, f->Tbe,  f->Tbw,  f->Tbn,  f->Tbs};
+    for (const void *q : need)
+        if (!q) return M2D_ERR_ARG;
+    if (!euler && (!f->cp || !f->ke || !f->kn || !f->fbe || !f->fbn)) return M2D_ERR_ARG;
+    const int nx = s5->nx, ny = s5->ny;
+    const long N = s5->N;
+    const hipStream_t st = s5->stream;
+    const hipStream_t keep = assembly_stream(st);
+    const int modvis = P->modvis, coord = P->coord;
+    double dt = *dt_io, beta = P->beta;
+    int rc = M2D_OK;
+    auto ok = [&]() {
+        m2d_status q;
+        m2d_last_status(&q);
+        if (q.status && !rc) rc = q.status;
+        return rc == M2D_OK;
+    };
+    auto copy = [&](double *dst, const double *src, long n) {
+        if (!rc && hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, st) !=
+                       hipSuccess)
+            rc = M2D_ERR_HIP;
+    };
+    const unsigned nb = (unsigned)((N + 255) / 256);
+    double *work = nullptr;
+    const int nw = norm_l1_work_size(nx, ny);
+    if (hipMalloc((void **)&work, sizeof(double) * (nw + 1)) != hipSuccess) {
+        assembly_stream(keep);
+        return M2D_ERR_HIP;
+    }
+    double norm = 0.0, rmass = 0.0, normpl = 0.0;
+    int itp = 0, stage = 1;
+    auto norm9_relative = [&](const double *a, const double *b, const double *x) {
+        double o[2] = {0.0, 0.0};
+        if (rc) return;
+        rc = norm_l1_device(a, b, x, nx, ny, work, o, st);
+        const double eps = 10.0 * 2.220446049250313080847e-16;  // 10*epsilon(1d0)
+        if (!rc) norm = (o[1] > eps) ? norm + o[0] / o[1] : norm + o[0];
+    };
+    // time levels (mach2d_main.f90:96-104)
+    copy(f->roa, f->ro, N);
+    copy(f->Ta, f->T, N);
+    copy(f->pa, f->p, N);
+    copy(f->ua, f->u, N);
+    copy(f->va, f->v, N);
+    copy(f->uea, f->ue, N);
+    copy(f->vea, f->ve, N);
+    copy(f->una, f->un, N);
+    copy(f->vna, f->vn, N);
+    // inflow update and fictitious pressures (:107)
+    stage = 2;
+    if (!rc)
+        m2d_intflow_time_cycle_update(&nx, &ny, &P->gamma, &P->Rg, &P->P0, &P->T0, f->x, f->xp,
+                                      f->u, f->uin, f->vin, f->pin, f->Tin, f->Mw, f->plin,
+                                      f->pina, f->p);
+    double p_avg = 0.0;
+    if (ok()) stage = 3;
+    if (!rc) p_avg = m2d_get_average(&nx, &ny, f->p);  // (:151)
+    if (ok()) {
+        hipLaunchKernelGGL(k_g_nodes, dim3(nb), dim3(256), 0, st, f->T, P->Rg, N, f->g);
+        if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
+    }
+    for (int itm = 1; itm <= P->itmmax && !rc; ++itm) {  // mass cycle (:155-344)
+        norm = 0.0;
+        stage = 4;
+        m2d_get_u_coefficients(&nx, &ny, &modvis, &dt, f->rp, f->re, f->rn, f->Jp, f->Je, f->Jn,
+                               f->ye, f->yk, f->alphae, f->betae, f->betan, f->gamman, f->vle,
+                               f->vln, f->roe, f->ron, f->roa, f->Uce, f->Vcn, f->au);
+        if (ok())
+            m2d_get_u_source(&nx, &ny, &modvis, &beta, &dt, f->rp, f->re, f->rn, f->xe, f->ye,
+                             f->xk, f->yk, f->xke, f->yke, f->xen, f->yen, f->Jp, f->Je, f->Jn,
+                             f->roe, f->ron, f->roa, f->p, f->vle, f->vln, f->Uce, f->Vcn,
+                             f->ua, f->u, f->v, f->cup, f->sup, f->bu);
+        if (ok())
+            m2d_intflow_set_bcu(&nx, &ny, &modvis, f->x, f->xp, f->xk, f->yk, f->u, f->v, f->au,
+                                f->bu);
+        if (ok())
+            m2d_get_v_coefficients(&nx, &ny, &coord, &modvis, &dt, f->rp, f->re, f->rn, f->Jp,
+                                   f->Je, f->Jn, f->xe, f->xk, f->alphae, f->betae, f->betan,
+                                   f->gamman, f->vle, f->vln, f->vlp, f->roe, f->ron, f->roa,
+                                   f->Uce, f->Vcn, f->av);
+        if (ok())
+            m2d_get_v_source(&nx, &ny, &coord, &modvis, &beta, &dt, f->rp, f->re, f->rn, f->xe,
+                             f->ye, f->xk, f->yk, f->xke, f->yke, f->xen, f->yen, f->Jp, f->Je,
+                             f->Jn, f->roe, f->ron, f->roa, f->p, f->vle, f->vln, f->Uce,
+                             f->Vcn, f->va, f->u, f->v, f->cvp, f->svp, f->bv);
+        if (ok())
+            m2d_intflow_set_bcv(&nx, &ny, &modvis, f->x, f->xp, f->xk, f->yk, f->u, f->v, f->av,
+                                f->bv);
+        if (ok())
+            m2d_get_internal_simplec_coefficients(&nx, &ny, f->re, f->rn, f->xe, f->ye, f->xk,
+                                                  f->yk, f->au, f->av, f->due, f->dve, f->dun,
+                                                  f->dvn, f->de, f->dn);
+        if (ok())
+            m2d_intflow_boundary_simplec(&nx, &ny, &modvis, f->due, f->dve, f->dun, f->dvn,
+                                         f->de, f->dn);
+        if (!ok()) break;
+        stage = 5;
+        rc = solve9(s9, P, f->au, f->bu, f->u, nx, ny, st);
+        norm9_relative(f->au, f->bu, f->u);
+        if (rc) break;
+        stage = 6;
+        rc = solve9(s9, P, f->av, f->bv, f->v, nx, ny, st);
+        norm9_relative(f->av, f->bv, f->v);
+        if (rc) break;
+        stage = 7;
+        m2d_get_velocities_at_internal_faces(&nx, &ny, &dt, f->rp, f->re, f->rn, f->xe, f->ye,
+                                             f->xk, f->yk, f->xen, f->yen, f->xke, f->yke,
+                                             f->Jp, f->cup, f->cvp, f->sup, f->svp, f->au, f->av,
+                                             f->roa, f->p, f->u, f->v, f->uea, f->vea, f->una,
+                                             f->vna, f->ue, f->ve, f->un, f->vn, f->Uce, f->Vcn);
+        if (ok())
+            m2d_intflow_velocities_at_boundary_faces(&nx, &ny, &modvis, f->x, f->xp, f->xe,
+                                                     f->ye, f->xk, f->yk, f->u, f->v, f->ue,
+                                                     f->ve, f->un, f->vn, f->Uce, f->Vcn);
+        if (!ok()) break;
+        stage = 8;
+        m2d_pl_loop_fields L = {f->x,   f->xp,  f->xe,  f->ye,  f->xk,  f->yk,  f->rp,
+                                f->re,  f->rn,  f->Jp,  f->due, f->dve, f->dun, f->dvn,
+                                f->de,  f->dn,  f->g,   f->roa, f->u,   f->v,   f->plin,
+                                f->ue,  f->ve,  f->un,  f->vn,  f->Uce, f->Vcn, f->roe,
+                                f->ron, f->ro,  f->p,   f->pl,  f->ap,  f->bp};
+        rc = m2d_intflow_pl_loop(s5, &L, modvis, dt, beta, p_avg, P->tolm, P->itpmax,
+                                 P->nitm_p, P->tol_p, &itp, &normpl);
+        if (rc) break;
+        stage = 9;
+        m2d_norm_l1_5d(&nx, &ny, f->pl, f->bp, f->ap, &norm);
+        if (ok())
+            m2d_get_u_v_at_real_nodes_with_pl(&nx, &ny, f->xe, f->ye, f->xk, f->yk, f->rp, f->pl,
+                                              f->au, f->av, f->u, f->v);
+        if (ok())
+            m2d_intflow_extrapolate_u_v_to_fictitious(&nx, &ny, &modvis, f->x, f->xp, f->xk,
+                                                      f->yk, f->u, f->v);
+        ok();
+    }
+    if (!rc) stage = 10;
+    if (!rc) {  // mass conservation (:346-348)
+        m2d_check_mass_conservation(&nx, &ny, &dt, f->rp, f->re, f->rn, f->Jp, f->ro, f->roa,
+                                    f->ron, f->roe, f->Uce, f->Vcn, work + nw);
+        if (ok() && (hipMemcpyAsync(&rmass, work + nw, sizeof(double), hipMemcpyDeviceToHost,
+                                    st) != hipSuccess ||
+                     hipStreamSynchronize(st) != hipSuccess))
+            rc = M2D_ERR_HIP;
+    }
+    if (!rc) stage = 11;
+    if (!rc && euler) {  // Euler, constant properties: T from H (:349-362)
+        hipLaunchKernelGGL(k_T_nodes, dim3(nb), dim3(256), 0, st, f->u, f->v, P->Href, P->Cpref,
+                           N, f->T);
+        if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
+        if (!rc)
+            m2d_intflow_boundary_T_from_H_conservation(&nx, &ny, &P->Cpref, &P->Href, f->Tin,
+                                                       f->u, f->ue, f->un, f->v, f->ve, f->vn,
+                                                       f->T, f->Tbe, f->Tbw, f->Tbn, f->Tbs);
+        ok();
+    } else if (!rc) {  // T equation (:363-380)
+        m2d_get_T_coefficients_and_source(
+            &nx, &ny, &modvis, &beta, &dt, f->rp, f->re, f->rn, f->xe, f->ye, f->xk, f->yk,
+            f->xke, f->yke, f->xen, f->yen, f->alphae, f->betae, f->betan, f->gamman, f->Jp,
+            f->Je, f->Jn, f->roe, f->ron, f->ro, f->roa, f->p, f->pa, f->cp, f->vle, f->vln,
+            f->ke, f->kn, f->Uce, f->Vcn, f->u, f->v, f->ue, f->ve, f->un, f->vn, f->ua, f->va,
+            f->T, f->Ta, f->fbe, f->fbn, f->at, f->bt);
+        if (ok()) m2d_intflow_set_bcT(&nx, &ny, &P->Twall, f->x, f->xp, f->T, f->Tin, f->at, f->bt);
+        if (ok()) rc = solve9(s9, P, f->at, f->bt, f->T, nx, ny, st);
+        norm9_relative(f->at, f->bt, f->T);
+    }
+    if (!rc) stage = 12;
+    if (!rc) {  // state equation and face densities (:386-391)
+        m2d_get_density_at_nodes(&nx, &ny, &P->Rg, f->p, f->T, f->ro);
+        if (ok()) m2d_get_density_at_faces(&nx, &ny, &beta, f->ro, f->Uce, f->Vcn, f->roe, f->ron);
+        ok();
+    }
+    if (!rc) stage = 13;
+    if (!rc) {  // next time step (:394)
+        const double nd = m2d_tstepper_dt_scarborough(&nx, &ny, &dt, f->au, f->av, f->at, f->ap,
+                                                      &P->h0, &P->mincc, &P->maxcc);
+        if (ok()) *dt_io = nd;
+    }
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = M2D_ERR_HIP;
+    (void)hipFree(work);
+    assembly_stream(keep);
+    if (res) {
+        res->norm = norm;
+        res->rmass = rmass;
+        res->normpl = normpl;
+        res->itp = itp;
+        res->stage = rc ? stage : 0;
+    }
+    set_last(rc);
+    return rc;
+}

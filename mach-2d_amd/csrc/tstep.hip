@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "m2d_internal.h"
+#include "seqsum.h"
 
 using namespace m2d;
 
@@ -47,55 +48,13 @@ __global__ __launch_bounds__(kT) void k_cc_terms(const double *__restrict__ a, i
 }
 
 // m = t(q) + m over the terms in order, then m / n; one workgroup per
-// array of n terms (t + blockIdx.x * n).  The dependent chain of additions
-// is the whole cost, so lane 0 of wave 0 only adds, from LDS, while the
-// other waves stage the next chunk of terms.
-constexpr int kSeqT = 256, kChunk = 2048;
-
-__global__ __launch_bounds__(kSeqT) void k_seq_mean(const double *__restrict__ t, long n,
-                                                   double *__restrict__ out)
+// array of n terms (t + blockIdx.x * n)
+__global__ __launch_bounds__(kSeqThreads) void k_seq_mean(const double *__restrict__ t, long n,
+                                                         double *__restrict__ out)
 {
-    __shared__ double buf[2][kChunk];
     const double *__restrict__ tt = t + (long)blockIdx.x * n;
-    const int tid = threadIdx.x;
-    const long nch = (n + kChunk - 1) / kChunk;
-    for (long q = tid; q < kChunk && q < n; q += kSeqT) buf[0][q] = tt[q];
-    __syncthreads();
-    double m = 0.0;
-    for (long c = 0; c < nch; ++c) {
-        const int cur = (int)(c & 1);
-        const long base = c * kChunk;
-        const int len = (int)min((long)kChunk, n - base);
-        if (tid >= 64) {
-            const long nb = base + kChunk;
-            if (nb < n) {
-                const int nlen = (int)min((long)kChunk, n - nb);
-                for (int q = tid - 64; q < nlen; q += kSeqT - 64) buf[cur ^ 1][q] = tt[nb + q];
-            }
-        } else if (tid == 0) {
-            const double *b = buf[cur];
-            int q = 0;
-            if (len >= 8) {
-                double v[8], w[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) v[k] = b[k];
-                for (; q + 16 <= len; q += 8) {
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) w[k] = b[q + 8 + k];
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) m = v[k] + m;
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) v[k] = w[k];
-                }
-#pragma unroll
-                for (int k = 0; k < 8; ++k) m = v[k] + m;
-                q += 8;
-            }
-            for (; q < len; ++q) m = b[q] + m;
-        }
-        __syncthreads();
-    }
-    if (tid == 0) out[blockIdx.x] = m / (double)n;
+    const double m = seq_sum([&](long q) { return tt[q]; }, n);
+    if (threadIdx.x == 0) out[blockIdx.x] = m / (double)n;
 }
 
 // per-block minimum of 1/sqrt(Jp (u^2 + v^2)) over interior points
@@ -197,7 +156,7 @@ int mean_cc(int nx, int ny, int K, const double *const *a, const int *nd, double
         else
             hipLaunchKernelGGL(k_cc_terms<5>, grid, dim3(kT), 0, s, S.in[k], nx, ny, t + k * n);
     }
-    hipLaunchKernelGGL(k_seq_mean, dim3(K), dim3(kSeqT), 0, s, t, n, t + K * n);
+    hipLaunchKernelGGL(k_seq_mean, dim3(K), dim3(kSeqThreads), 0, s, t, n, t + K * n);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(out, t + K * n, sizeof(double) * K, hipMemcpyDeviceToHost, s) !=
             hipSuccess ||

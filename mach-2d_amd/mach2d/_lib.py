@@ -66,6 +66,9 @@ EXPORTS = [
     "m2d_get_mean_cc_9d", "m2d_get_mean_cc_5d", "m2d_tstepper_dt0_scarborough",
     "m2d_tstepper_dt_scarborough",
     "m2d_get_real_centroids_xy", "m2d_get_metrics", "m2d_get_radius",
+    "m2d_intflow_boundary_simplec", "m2d_intflow_extrapolate_u_v_to_fictitious",
+    "m2d_intflow_boundary_T_from_H_conservation", "m2d_intflow_time_cycle_update",
+    "m2d_norm_l1_5d", "m2d_get_average", "m2d_intflow_time_step",
     "m2d_unet_conv3x3", "m2d_unet_conv3x3_wgrad", "m2d_unet_relu_bwd", "m2d_unet_mode_bwd",
     "m2d_unet_to_bf16", "m2d_unet_adam", "m2d_unet_pack_weights", "m2d_unet_mse",
     "m2d_unet_pack_input", "m2d_unet_unpack_output",
@@ -181,6 +184,12 @@ def load():
     L.m2d_get_radius.argtypes = [_ip, _ip, _ip] + [_vp] * 6
     for f in (L.m2d_get_real_centroids_xy, L.m2d_get_metrics, L.m2d_get_radius):
         f.restype = None
+    L.m2d_norm_l1_5d.argtypes = [_ip, _ip, _vp, _vp, _vp, _dp]
+    L.m2d_norm_l1_5d.restype = None
+    L.m2d_intflow_time_step.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double), _vp]
+    L.m2d_intflow_time_step.restype = C.c_int
+    L.m2d_get_average.argtypes = [_ip, _ip, _vp]
+    L.m2d_get_average.restype = C.c_double
     L.m2d_version.argtypes = []
     L.m2d_version.restype = C.c_char_p
     _lib = L

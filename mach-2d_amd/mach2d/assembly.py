@@ -95,6 +95,20 @@ ROUTINES = {
     "intflow_velocities_at_boundary_faces": (
         [("modvis", "i")], ["x", "xp", "xe", "ye", "xk", "yk", "u", "v"],
         [("ue", 1), ("ve", 1), ("un", 1), ("vn", 1), ("Uce", 1), ("Vcn", 1)]),
+    # mod_intflow.f90:791, :881, :557, :1106 (in/out arrays as outputs whose
+    # initial values `out` supplies; ("Tbe", "ny") an output of ny values)
+    "intflow_boundary_simplec": ([("modvis", "i")], [],
+                                 [("due", 1), ("dve", 1), ("dun", 1), ("dvn", 1), ("de", 1),
+                                  ("dn", 1)]),
+    "intflow_extrapolate_u_v_to_fictitious": ([("modvis", "i")], ["x", "xp", "xk", "yk"],
+                                              [("u", 1), ("v", 1)]),
+    "intflow_boundary_T_from_H_conservation": (
+        [("Cpref", "d"), ("Href", "d")], [("Tin", "ny"), "u", "ue", "un", "v", "ve", "vn"],
+        [("T", 1), ("Tbe", "ny"), ("Tbw", "ny"), ("Tbn", "nx"), ("Tbs", "nx")]),
+    "intflow_time_cycle_update": (
+        [("gamma", "d"), ("Rg", "d"), ("P0", "d"), ("T0", "d")], ["x", "xp", "u"],
+        [("uin", "ny"), ("vin", "ny"), ("pin", "ny"), ("Tin", "ny"), ("Mw", "ny"),
+         ("plin", "ny"), ("pina", "ny"), ("p", 1)]),
 }
 
 # C entry of each routine (the reference's subroutine name, prefixed)
@@ -102,7 +116,9 @@ ENTRY = {name: "m2d_get_" + name for name in ROUTINES}
 ENTRY["boundary_faces_flags"] = "m2d_initialize_boundary_faces_flags"
 ENTRY["mass_conservation"] = "m2d_check_mass_conservation"
 for _k in ("intflow_set_bcu", "intflow_set_bcv", "intflow_set_bcpl", "intflow_set_bcT",
-           "intflow_velocities_at_boundary_faces"):
+           "intflow_velocities_at_boundary_faces", "intflow_boundary_simplec",
+           "intflow_extrapolate_u_v_to_fictitious", "intflow_boundary_T_from_H_conservation",
+           "intflow_time_cycle_update"):
     ENTRY[_k] = "m2d_" + _k
 
 
@@ -145,7 +161,7 @@ def call(name, nx, ny, scalars, fields, out=None):
         first = obj if first is None else first
     res = {}
     for k, m in outs:
-        size = max(m, 1) * n if m else 1
+        size = ny if m == "ny" else nx if m == "nx" else max(m, 1) * n if m else 1
         arr = None if out is None else out.get(k)
         if arr is None:
             if first is None or isinstance(first, np.ndarray):
@@ -166,8 +182,18 @@ def call(name, nx, ny, scalars, fields, out=None):
         if m == 0:
             ret[k] = float(v[0]) if isinstance(v, np.ndarray) else float(v.cpu()[0])
         else:
-            ret[k] = v.reshape(m, n) if m > 1 else v
+            ret[k] = v.reshape(m, n) if isinstance(m, int) and m > 1 else v
     return ret
+
+
+def get_average(nx, ny, F):
+    """get_average (src/util/mod_field_operations.f90:115-142), bit for bit"""
+    p, obj = _ptr(F.reshape(-1), nx * ny, "F")
+    v = _lib.load().m2d_get_average(C.byref(C.c_int(nx)), C.byref(C.c_int(ny)), p)
+    st = _lib.last_status()
+    if st.status:
+        raise SolverError(st.status, "m2d_get_average")
+    return v
 
 
 def set_stream(stream=None):

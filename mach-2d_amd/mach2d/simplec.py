@@ -54,3 +54,77 @@ def intflow_pl_loop(dev, f, modvis, dt, beta, p_avg, tolm, itpmax=20, nitm_p=20,
     if rc:
         raise SolverError(rc, "m2d_intflow_pl_loop")
     return itp.value, normpl.value
+
+
+# ---------------------------------------------------------------------------
+# One internal-flow time step (src/mach2d_main.f90:94-404), m2d_intflow_time_step
+# ---------------------------------------------------------------------------
+TS_CONST = ("x", "xp", "xe", "ye", "xk", "yk", "xke", "yke", "xen", "yen", "rp", "re", "rn",
+            "Jp", "Je", "Jn", "alphae", "betae", "betan", "gamman",
+            "vle", "vln", "vlp", "cp", "ke", "kn", "fbe", "fbn")
+TS_STATE = ("u", "v", "ue", "ve", "un", "vn", "Uce", "Vcn", "p", "T", "ro", "roe", "ron", "pl")
+TS_LEVEL = ("ua", "va", "uea", "vea", "una", "vna", "pa", "Ta", "roa")
+TS_WORK = ("au", "av", "at", "ap", "bu", "bv", "bt", "bp", "cup", "sup", "cvp", "svp", "due",
+           "dve", "dun", "dvn", "de", "dn", "g")
+TS_INFLOW = ("uin", "vin", "pin", "Tin", "Mw", "plin", "pina", "Tbe", "Tbw", "Tbn", "Tbs")
+TS_ALL = TS_CONST + TS_STATE + TS_LEVEL + TS_WORK + TS_INFLOW
+TS_OPTIONAL = ("cp", "ke", "kn", "fbe", "fbn")   # used by the T equation only
+
+
+class TimeStepFields(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in TS_ALL]
+
+
+class TimeStepParams(C.Structure):
+    _fields_ = ([(k, C.c_int) for k in ("modvis", "coord", "solver9d", "itmmax", "itpmax",
+                                          "nitm_u", "nitm_p")] +
+                [(k, C.c_double) for k in ("tol_u", "tol_p", "tolm", "beta", "Rg", "gamma", "P0",
+                                             "T0", "Cpref", "Href", "Twall", "h0", "mincc",
+                                             "maxcc")])
+
+
+class TimeStepResult(C.Structure):
+    _fields_ = [("norm", C.c_double), ("rmass", C.c_double), ("normpl", C.c_double),
+                ("itp", C.c_int), ("stage", C.c_int)]
+
+STAGES = {1: "time levels", 2: "inflow update", 3: "p average and g", 4: "u, v assembly",
+          5: "u solve", 6: "v solve", 7: "face velocities", 8: "p' loop",
+          9: "node velocities", 10: "mass check", 11: "temperature", 12: "densities",
+          13: "next dt"}
+
+
+def ts_size(k, nx, ny):
+    N = nx * ny
+    if k in ("uin", "vin", "pin", "Tin", "Mw", "plin", "pina", "Tbe", "Tbw"):
+        return ny
+    if k in ("Tbn", "Tbs"):
+        return nx
+    return 9 * N if k in ("au", "av", "at") else 5 * N if k == "ap" else N
+
+
+def intflow_time_step(dev9, dev5, f, params, dt):
+    """One time step on device tensors `f` (names of TS_ALL; TS_OPTIONAL may
+    be missing for Euler) with the FP64 handles dev9 (9 diagonals; None for
+    solver9d = 1) and dev5 (5 diagonals).  Returns (next dt, result dict)."""
+    import torch
+    nx, ny = dev5.nx, dev5.ny
+    flds = TimeStepFields()
+    for k in TS_ALL:
+        t = f.get(k)
+        if t is None and k in TS_OPTIONAL:
+            continue
+        n = ts_size(k, nx, ny)
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
+                and t.is_contiguous() and t.numel() == n):
+            raise TypeError("%s must be a contiguous float64 CUDA tensor of %d values" % (k, n))
+        setattr(flds, k, t.data_ptr())
+    prm = TimeStepParams(**params)
+    d = C.c_double(dt)
+    res = TimeStepResult()
+    rc = _lib.load().m2d_intflow_time_step(dev9.h if dev9 is not None else None, dev5.h,
+                                           C.byref(flds), C.byref(prm), C.byref(d),
+                                           C.byref(res))
+    if rc:
+        raise SolverError(rc, "m2d_intflow_time_step (%s)" % STAGES.get(res.stage, "?"))
+    return d.value, {"norm": res.norm, "rmass": res.rmass, "normpl": res.normpl,
+                     "itp": res.itp}

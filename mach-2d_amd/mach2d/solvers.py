@@ -149,6 +149,18 @@ def norm_l1_9d(nx, ny, var, b, a, norm=0.0):
     return v.value
 
 
+def norm_l1_5d(nx, ny, var, b, a, norm=0.0):
+    """mod_solvers.f90:91; returns norm + sum over real volumes |A.var - b|
+    (a: 5 diagonals S, W, P, E, N)."""
+    v = C.c_double(norm)
+    n = nx * ny
+    aa = _coef(a, n, 5)
+    _lib.load().m2d_norm_l1_5d(_i(nx), _i(ny), _p(np.ascontiguousarray(var, np.float64)),
+                               _p(np.ascontiguousarray(b, np.float64)), _p(aa), C.byref(v))
+    _check_last("norm_l1_5d")
+    return v.value
+
+
 def norm_l1_9d_relative(nx, ny, var, b, a, norm=0.0):
     """mod_solvers.f90:176; relative to ||b||_1 when ||b||_1 > 10 eps."""
     v = C.c_double(norm)

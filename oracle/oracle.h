@@ -135,4 +135,33 @@ void orc_get_metrics(int nx, int ny, const double *x, const double *y, const dou
 void orc_get_radius(int coord, int nx, int ny, const double *y, const double *yp,
                     double *radius, double *re, double *rn, double *rp);
 
+
+/* intflow.c: the rest of one internal-flow time step, mod_intflow.f90:557-1134,
+ * mod_solvers.f90:91-125, mod_field_operations.f90:115-142 */
+void orc_intflow_boundary_T_from_H_conservation(int nx, int ny, double Cpref, double Href,
+                                                const double *Tin, const double *u,
+                                                const double *ue, const double *un,
+                                                const double *v, const double *ve,
+                                                const double *vn, double *T, double *Tbe,
+                                                double *Tbw, double *Tbn, double *Tbs);
+void orc_intflow_boundary_simplec(int nx, int ny, int modvis, double *due, double *dve,
+                                  double *dun, double *dvn, double *de, double *dn);
+void orc_intflow_extrapolate_u_v_to_fictitious(int nx, int ny, int modvis, const double *x,
+                                               const double *xp, const double *xk,
+                                               const double *yk, double *u, double *v);
+void orc_get_uin_vin_pin_Tin_Mw(int nx, int ny, double gamma, double Rg, double po, double T0,
+                                const double *u, double *uin, double *vin, double *pin,
+                                double *Tin, double *Mw);
+void orc_get_plin_and_p_fictitious(int nx, int ny, const double *x, const double *xp,
+                                   const double *pina, const double *pin, double *plin,
+                                   double *p);
+void orc_intflow_time_cycle_update(int nx, int ny, double gamma, double Rg, double P0,
+                                   double T0, const double *x, const double *xp,
+                                   const double *u, double *uin, double *vin, double *pin,
+                                   double *Tin, double *Mw, double *plin, double *pina,
+                                   double *p);
+void orc_norm_l1_5d(int nx, int ny, const double *var, const double *b, const double *a,
+                    double *norm);
+double orc_get_average(int nx, int ny, const double *F);
+
 #endif

@@ -168,6 +168,20 @@ COEF = {
     "intflow_velocities_at_boundary_faces": (
         [("modvis", "i")], ["x", "xp", "xe", "ye", "xk", "yk", "u", "v"],
         [("ue", 1), ("ve", 1), ("un", 1), ("vn", 1), ("Uce", 1), ("Vcn", 1)]),
+    # the rest of one internal-flow time step (in/out arrays as outputs;
+    # ("Tbe", "ny") an output of ny values)
+    "intflow_boundary_simplec": ([("modvis", "i")], [],
+                                 [("due", 1), ("dve", 1), ("dun", 1), ("dvn", 1), ("de", 1),
+                                  ("dn", 1)]),
+    "intflow_extrapolate_u_v_to_fictitious": ([("modvis", "i")], ["x", "xp", "xk", "yk"],
+                                              [("u", 1), ("v", 1)]),
+    "intflow_boundary_T_from_H_conservation": (
+        [("Cpref", "d"), ("Href", "d")], [("Tin", "ny"), "u", "ue", "un", "v", "ve", "vn"],
+        [("T", 1), ("Tbe", "ny"), ("Tbw", "ny"), ("Tbn", "nx"), ("Tbs", "nx")]),
+    "intflow_time_cycle_update": (
+        [("gamma", "d"), ("Rg", "d"), ("P0", "d"), ("T0", "d")], ["x", "xp", "u"],
+        [("uin", "ny"), ("vin", "ny"), ("pin", "ny"), ("Tin", "ny"), ("Mw", "ny"),
+         ("plin", "ny"), ("pina", "ny"), ("p", 1)]),
 }
 
 
@@ -207,7 +221,8 @@ def _coef_call(fn, name, nx, ny, scalars, fields, out):
     res = {}
     for k, m in outs:
         init = None if out is None else out.get(k)
-        shape = (1, 1) if m == 0 else (m, n)
+        shape = ((1, ny) if m == "ny" else (1, nx) if m == "nx" else (1, 1) if m == 0
+                 else (m, n))
         arr = np.zeros(shape) if init is None else _c(init).reshape(shape).copy()
         res[k] = arr
         args.append(arr)
@@ -443,6 +458,51 @@ def radius(coord, y, yp, nx, ny):
     out = [np.zeros(n) for _ in range(4)]
     f(coord, nx, ny, _c(y), _c(yp), *out)
     return tuple(out)
+
+
+def norm_l1_5d(nx, ny, var, b, a, norm=0.0, ref_=False):
+    """norm_l1_5d (mod_solvers.f90:91-125): norm + sum |A var - b| over real
+    volumes, a (5, N)."""
+    L = ref() if ref_ else lib()
+    f = L.ref_norm_l1_5d if ref_ else L.orc_norm_l1_5d
+    f.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dptr]
+    f.restype = None
+    v = C.c_double(norm)
+    f(nx, ny, _c(var), _c(b), _c(a), C.byref(v))
+    return v.value
+
+
+def get_average(nx, ny, F, ref_=False):
+    """get_average (mod_field_operations.f90:115-142)"""
+    L = ref() if ref_ else lib()
+    f = L.ref_get_average if ref_ else L.orc_get_average
+    f.argtypes = [C.c_int, C.c_int, _dp]
+    f.restype = C.c_double
+    return f(nx, ny, _c(F))
+
+
+def ref_time_cycle_update(nx, ny, scalars, fields, out):
+    """intflow_time_cycle_update (mod_intflow.f90:1106-1134) composed from the
+    reference's own module procedures get_uin_vin_pin_Tin_Mw and
+    get_plin_and_p_fictitious (private: called by their module symbols),
+    with gamma = thermomodel%gm(T0) and Rg given."""
+    R = ref()
+    d = C.c_double
+    f1 = getattr(R, "_QMmod_intflowPget_uin_vin_pin_tin_mw")
+    f1.argtypes = [_ip, _ip, _dptr, _dptr, _dptr, _dptr, _dp, _dp, _dp, _dp, _dp, _dp]
+    f1.restype = None
+    f2 = getattr(R, "_QMmod_intflowPget_plin_and_p_fictitious")
+    f2.argtypes = [_ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
+    f2.restype = None
+    res = {k: _c(out[k]).copy() for k in ("uin", "vin", "pin", "Tin", "Mw", "plin", "pina", "p")}
+    res["pina"] = res["pin"].copy()
+    inx, iny = C.c_int(nx), C.c_int(ny)
+    f1(C.byref(inx), C.byref(iny), C.byref(d(scalars["gamma"])), C.byref(d(scalars["Rg"])),
+       C.byref(d(scalars["P0"])), C.byref(d(scalars["T0"])), _c(fields["u"]), res["uin"],
+       res["vin"], res["pin"], res["Tin"], res["Mw"])
+    f2(C.byref(inx), C.byref(iny), _c(fields["x"]), _c(fields["xp"]), res["pina"], res["pin"],
+       res["plin"], res["p"])
+    return res
 
 
 def mean_cc(a, nx, ny):

@@ -15,10 +15,13 @@ module m2dref_shim
    use msi2d5
    use mtdma2d9
    use mod_class_solver_msi
-   use mod_solvers, only: norm_l1_9d, norm_l1_9d_relative, norm_l1_b
+   use mod_solvers, only: norm_l1_9d, norm_l1_9d_relative, norm_l1_b, norm_l1_5d
+   use mod_field_operations, only: get_average
    use coefficients
    use mod_intflow, only: type_intflow, intflow_set_bcu, intflow_set_bcv, &
-      intflow_set_bcpl, intflow_set_bcT, intflow_velocities_at_boundary_faces
+      intflow_set_bcpl, intflow_set_bcT, intflow_velocities_at_boundary_faces, &
+      intflow_boundary_simplec, intflow_extrapolate_u_v_to_fictitious, &
+      intflow_boundary_T_from_H_conservation
    implicit none
 contains
 
@@ -444,5 +447,54 @@ contains
       real(c_double), dimension(nx*ny), intent(in) :: Jp, u, v
       ref_tstepper_dt0_scarborough = tstepper_dt0_scarborough(nx, ny, Jp, u, v)
    end function
+
+   ! mod_solvers.f90:91
+   subroutine ref_norm_l1_5d(nx, ny, var, b, a, norm) bind(C, name="ref_norm_l1_5d")
+      integer(c_int), value :: nx, ny
+      real(c_double), intent(in) :: var(nx*ny), b(nx*ny), a(nx*ny,5)
+      real(c_double), intent(inout) :: norm
+      call norm_l1_5d(nx, ny, var, b, a, norm)
+   end subroutine
+
+   ! mod_field_operations.f90:115
+   real(c_double) function ref_get_average(nx, ny, F) bind(C, name="ref_get_average")
+      integer(c_int), value :: nx, ny
+      real(c_double), intent(in) :: F(nx*ny)
+      real(8) :: avg
+      call get_average(nx, ny, F, avg)
+      ref_get_average = avg
+   end function
+
+   ! mod_intflow.f90:791
+   subroutine ref_intflow_boundary_simplec(nx, ny, modvis, due, dve, dun, dvn, de, dn) &
+         bind(C, name="ref_intflow_boundary_simplec")
+      integer(c_int), value :: nx, ny, modvis
+      real(c_double), dimension(nx*ny), intent(inout) :: due, dve, dun, dvn, de, dn
+      call intflow_boundary_simplec(nx, ny, modvis, due, dve, dun, dvn, de, dn)
+   end subroutine
+
+   ! mod_intflow.f90:881
+   subroutine ref_intflow_extrapolate_u_v_to_fictitious(nx, ny, modvis, x, xp, xk, yk, u, v) &
+         bind(C, name="ref_intflow_extrapolate_u_v_to_fictitious")
+      integer(c_int), value :: nx, ny, modvis
+      real(c_double), dimension(nx*ny), intent(in) :: x, xp, xk, yk
+      real(c_double), dimension(nx*ny), intent(inout) :: u, v
+      call intflow_extrapolate_u_v_to_fictitious(nx, ny, modvis, x, xp, xk, yk, u, v)
+   end subroutine
+
+   ! mod_intflow.f90:557; iflow carries Tin (the field the routine reads)
+   subroutine ref_intflow_boundary_T_from_H_conservation(nx, ny, Cpref, Href, Tin, u, ue, un, &
+         v, ve, vn, T, Tbe, Tbw, Tbn, Tbs) bind(C, name="ref_intflow_boundary_T_from_H_conservation")
+      integer(c_int), value :: nx, ny
+      real(c_double), value :: Cpref, Href
+      real(c_double), intent(in) :: Tin(ny)
+      real(c_double), dimension(nx*ny), intent(in) :: u, ue, un, v, ve, vn
+      real(c_double), intent(inout) :: T(nx*ny), Tbe(ny), Tbw(ny), Tbn(nx), Tbs(nx)
+      type(type_intflow) :: iflow
+      allocate(iflow%Tin(ny))
+      iflow%Tin = Tin
+      call intflow_boundary_T_from_H_conservation(nx, ny, Cpref, Href, iflow, u, ue, un, v, ve, &
+         vn, T, Tbe, Tbw, Tbn, Tbs)
+   end subroutine
 
 end module m2dref_shim

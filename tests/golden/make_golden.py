@@ -153,6 +153,10 @@ def solver_msi9d():
          nitm=np.array(20), tol=np.array(1e-2))
 
 
+# scalars of the time-step routines: constant thermophysical model and
+# chamber conditions of the shipped nozzle parameter file
+STEP_SCALARS = dict(Cpref=1004.5, Href=837083.3, gamma=1.4, P0=1038350.4, T0=833.33333333)
+
 COEF_CASES = [  # tag, nx, ny, coord, modvis, cartesian, axis faces
     ("ns_axi", 12, 8, 1, 1, False, False),
     ("euler", 12, 8, 0, 0, False, False),
@@ -192,16 +196,21 @@ def coef():
         # Twall: prescribed north wall temperature on the axisymmetric cases,
         # adiabatic wall (Twall <= 0) on the planar ones
         sc = dict(beta=0.35, modvis=mv, dt=2e-3, coord=coord, Rg=287.0,
-                  Twall=300.0 if coord == 1 else -1.0)
+                  Twall=300.0 if coord == 1 else -1.0, **STEP_SCALARS)
         for k, v in f.items():
             arrs["%s_in_%s" % (tag, k)] = v
         arrs["%s_scalars" % tag] = np.array([nx, ny, coord, mv, sc["beta"], sc["dt"]])
         arrs["%s_twall" % tag] = np.array([sc["Twall"]])
         for name, (_, _, outs) in O.COEF.items():
-            init = {k: rng.uniform(-1, 1, size=(m, nx * ny) if m else (1, 1)) for k, m in outs}
+            init = {k: rng.uniform(-1, 1, size=(1, ny) if m == "ny" else (1, nx) if m == "nx"
+                                   else (m, nx * ny) if m else (1, 1)) for k, m in outs}
             for k, v in init.items():
                 arrs["%s_%s_init_%s" % (tag, name, k)] = v
-            res = O.ref_coef(name, nx, ny, sc, f, out={k: v.copy() for k, v in init.items()})
+            if name == "intflow_time_cycle_update":   # private module procedures
+                res = O.ref_time_cycle_update(nx, ny, sc, f, {k: v.copy() for k, v in init.items()})
+                res = {k: v.reshape(-1) for k, v in res.items()}
+            else:
+                res = O.ref_coef(name, nx, ny, sc, f, out={k: v.copy() for k, v in init.items()})
             for k, v in res.items():
                 arrs["%s_%s_%s" % (tag, name, k)] = v
     save("coef.npz", **arrs)

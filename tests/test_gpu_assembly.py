@@ -2,7 +2,11 @@
 against the oracle, through the C ABI, with host and with device arrays.
 
 Bar: bit for bit (the kernels keep the reference's operation order without
-contraction); points the reference does not write keep their values."""
+contraction); points the reference does not write keep their values.  The
+one exception is the inflow pressure of intflow_time_cycle_update, a real
+power x**y: the device pow and the host libm pow may differ in the last bit,
+so pin, plin (= pin - pina) and the pressures set from pin are held to 4 ulp
+of the inflow pressure."""
 import os
 import sys
 
@@ -39,10 +43,26 @@ def _fields(nx, ny, coord, cart, seed):
     return f
 
 
-def _same(got, want):
+STEP = dict(Cpref=1004.5, Href=837083.3, gamma=1.4, P0=1038350.4, T0=833.33333333)
+POW_OUTPUTS = {("intflow_time_cycle_update", k) for k in ("pin", "plin", "p")}
+
+
+def _size(m, nx, ny):
+    return ny if m == "ny" else nx if m == "nx" else m * nx * ny if m else 1
+
+
+def _same(got, want, name=None, k=None, ref_scale=None):
     if isinstance(want, float):
         return float(got) == want
-    return np.array_equal(np.asarray(got), want)
+    got, want = np.asarray(got).reshape(-1), np.asarray(want).reshape(-1)
+    if (name, k) in POW_OUTPUTS:
+        return np.abs(got - want).max() <= 4 * np.finfo(float).eps * ref_scale
+    return np.array_equal(got, want)
+
+
+def _skip(name, nx):
+    # the north slip extrapolation reads the third column (nx >= 4)
+    return name == "intflow_extrapolate_u_v_to_fictitious" and nx < 4
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -51,14 +71,16 @@ def test_assembly_host_arrays_bitwise(gpu, orc, case):
     nx, ny, coord, mv, cart = case
     f = _fields(nx, ny, coord, cart, seed=nx * ny + coord)
     sc = dict(beta=0.3, modvis=mv, dt=1e-3, coord=coord, Rg=287.0,
-              Twall=300.0 if coord else -1.0)
+              Twall=300.0 if coord else -1.0, **STEP)
     rng = np.random.default_rng(7)
     for name, (_, _, outs) in A.ROUTINES.items():
-        init = {k: rng.uniform(-1, 1, size=m * nx * ny if m else 1) for k, m in outs}
+        if _skip(name, nx):
+            continue
+        init = {k: rng.uniform(-1, 1, size=_size(m, nx, ny)) for k, m in outs}
         want = orc.coef(name, nx, ny, sc, f, out={k: v.copy() for k, v in init.items()})
         got = A.call(name, nx, ny, sc, f, out={k: v.copy() for k, v in init.items()})
         for k in want:
-            assert _same(got[k], want[k]), (name, k, case)
+            assert _same(got[k], want[k], name, k, STEP["P0"]), (name, k, case)
 
 
 @pytest.mark.parametrize("case", CASES[:4])
@@ -70,10 +92,13 @@ def test_assembly_device_arrays_bitwise(gpu, orc, case):
     f = _fields(nx, ny, coord, cart, seed=11 + nx)
     tf = {k: torch.tensor(v.reshape(-1), device="cuda") for k, v in f.items()}
     sc = dict(beta=0.2, modvis=mv, dt=2e-3, coord=coord, Rg=287.0,
-              Twall=-1.0 if coord else 310.0)
+              Twall=-1.0 if coord else 310.0, **STEP)
     for name, (_, _, outs) in A.ROUTINES.items():
+        if _skip(name, nx):
+            continue
         want = orc.coef(name, nx, ny, sc, f)
-        if name == "boundary_faces_flags":   # no input fields: outputs choose the memory
+        if name in ("boundary_faces_flags", "intflow_boundary_simplec"):
+            # no input fields: the outputs choose the memory
             out = {k: torch.zeros(nx * ny, dtype=torch.float64, device="cuda") for k, _ in outs}
             got = A.call(name, nx, ny, sc, tf, out=out)
         else:
@@ -81,7 +106,7 @@ def test_assembly_device_arrays_bitwise(gpu, orc, case):
         torch.cuda.synchronize()
         for k in want:
             g = got[k] if isinstance(got[k], float) else got[k].cpu().numpy()
-            assert _same(g, want[k]), (name, k, case)
+            assert _same(g, want[k], name, k, STEP["P0"]), (name, k, case)
 
 
 def test_assembly_rejects_mixed_memory(gpu):
@@ -105,7 +130,7 @@ def test_assembly_vs_reference_fixtures(gpu, tag):
     nx, ny, coord, mv, beta, dt = g["%s_scalars" % tag]
     nx, ny = int(nx), int(ny)
     sc = dict(coord=int(coord), modvis=int(mv), beta=beta, dt=dt, Rg=287.0,
-              Twall=float(g["%s_twall" % tag][0]))
+              Twall=float(g["%s_twall" % tag][0]), **STEP)
     pre = "%s_in_" % tag
     f = {k[len(pre):]: v for k, v in g.items() if k.startswith(pre)}
     for name, (_, _, outs) in A.ROUTINES.items():
@@ -115,5 +140,5 @@ def test_assembly_vs_reference_fixtures(gpu, tag):
             want = g["%s_%s_%s" % (tag, name, k)]
             if m == 0:
                 want = float(np.asarray(want).reshape(-1)[0])
-            assert _same(got[k], want if m == 0 else want.reshape(np.asarray(got[k]).shape)), \
-                (name, k)
+            assert _same(got[k], want if m == 0 else want.reshape(np.asarray(got[k]).shape),
+                         name, k, STEP["P0"]), (name, k)

@@ -165,7 +165,8 @@ def test_assembly_tables_match_reference_signatures():
                                 "internal_flow", "mod_intflow.f90")).read()
     # the derived-type argument iflow crosses the C ABI as the fields the
     # routine reads; output names a / b stand for the reference's own
-    iflow = {"intflow_set_bcpl": ["plin"], "intflow_set_bcT": ["Tin"]}
+    iflow = {"intflow_set_bcpl": ["plin"], "intflow_set_bcT": ["Tin"],
+             "intflow_boundary_T_from_H_conservation": ["Tin"]}
     outname = {"intflow_set_bcu": ("au", "b"), "intflow_set_bcv": ("av", "b"),
                "intflow_set_bcpl": ("ap", "bp"), "intflow_set_bcT": ("at", "bt")}
     for k, (scal, fields, outs) in ROUTINES.items():
@@ -179,6 +180,12 @@ def test_assembly_tables_match_reference_signatures():
             got = got[:at] + [w.lower() for w in iflow[k]] + got[at + 1:]
             if k == "intflow_set_bcT":
                 got = got[:2] + ["twall"] + got[2:]
+        if k == "intflow_time_cycle_update":
+            # thermomodel and iflow cross as gamma = gm(T0), Rg, P0, T0 and
+            # the inflow vectors of iflow (in/out)
+            assert got == ["nx", "ny", "x", "xp", "u", "thermomodel", "iflow", "p"], got
+            got = (["nx", "ny", "gamma", "rg", "p0", "t0", "x", "xp", "u"] +
+                   ["uin", "vin", "pin", "tin", "mw", "plin", "pina", "p"])
         want = (["nx", "ny"] + [n for n, _ in scal] +
                 [f if isinstance(f, str) else f[0] for f in fields] + [n for n, _ in outs])
         if k in outname:
@@ -198,7 +205,9 @@ def test_fortran_intflow_module_matches_reference_interface():
     body = text[text.index("contains"):]
     names = [k for k in ROUTINES if k.startswith("intflow_")]
     assert names == ["intflow_set_bcu", "intflow_set_bcv", "intflow_set_bcpl", "intflow_set_bcT",
-                     "intflow_velocities_at_boundary_faces"]
+                     "intflow_velocities_at_boundary_faces", "intflow_boundary_simplec",
+                     "intflow_extrapolate_u_v_to_fictitious",
+                     "intflow_boundary_T_from_H_conservation", "intflow_time_cycle_update"]
     for k in names:
         scal, fields, outs = ROUTINES[k]
         m = re.search(r"subroutine %s_hip\((.*?)\)" % k, body, re.S)

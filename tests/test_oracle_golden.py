@@ -132,7 +132,9 @@ def coef_case(g, tag):
     pre = "%s_in_" % tag
     fields = {k[len(pre):]: v for k, v in g.items() if k.startswith(pre)}
     twall = float(g["%s_twall" % tag][0])
-    return nx, ny, dict(coord=coord, modvis=mv, beta=beta, dt=dt, Rg=287.0, Twall=twall), fields
+    return nx, ny, dict(coord=coord, modvis=mv, beta=beta, dt=dt, Rg=287.0, Twall=twall,
+                        Cpref=1004.5, Href=837083.3, gamma=1.4, P0=1038350.4,
+                        T0=833.33333333), fields
 
 
 @pytest.mark.parametrize("tag", COEF_TAGS)
@@ -146,6 +148,9 @@ def test_coef_oracle_vs_reference_fixtures(orc, tag):
         got = orc.coef(name, nx, ny, sc, f, out=init)
         for k, m in outs:
             want = g["%s_%s_%s" % (tag, name, k)]
+            if m in ("nx", "ny"):
+                want = want.reshape(-1)
+                got[k] = got[k].reshape(-1)
             if m == 0:
                 want = float(np.asarray(want).reshape(-1)[0])
             assert np.array_equal(got[k], want), (tag, name, k)
@@ -375,3 +380,131 @@ def test_grid_radius_affine_reflection(orc, nx, ny):
     assert (re[~east] == 1).all() and (rn[~north] == 1).all()
     for a in orc.radius(0, y, yp, nx, ny):
         assert (a == 1).all()
+
+
+# ---------------------------------------------------------------------------
+# The rest of one internal-flow time step (oracle/intflow.c) against the
+# compiled reference: bit for bit.
+# ---------------------------------------------------------------------------
+
+def _step_inputs(nx, ny, seed):
+    from mach2d import problems as P
+    g = P.ogive_grid(nx, ny, 1.05)
+    m = P.metrics(g)
+    rng = np.random.default_rng(seed)
+    f = {k: np.ascontiguousarray(m[k].reshape(-1)) for k in ("xp", "xk", "yk")}
+    f["x"] = np.ascontiguousarray(g.x.reshape(-1))
+    n = nx * ny
+    for k in ("u", "ue", "un", "v", "ve", "vn"):
+        f[k] = rng.uniform(20.0, 120.0, n) * rng.choice([-1.0, 1.0], n)
+    f["u"] = rng.uniform(30.0, 90.0, n)
+    f["Tin"] = rng.uniform(780.0, 830.0, ny)
+    return f, rng
+
+
+@pytest.mark.parametrize("nx,ny", [(4, 3), (5, 6), (23, 14), (130, 66)])
+def test_intflow_step_routines_vs_reference(orc, nx, ny):
+    if not orc.ref_available():
+        pytest.skip("reference library not built")
+    f, rng = _step_inputs(nx, ny, nx * ny)
+    n = nx * ny
+    for modvis in (0, 1):
+        init = {k: rng.uniform(-1, 1, n) for k in ("due", "dve", "dun", "dvn", "de", "dn")}
+        a = orc.coef("intflow_boundary_simplec", nx, ny, {"modvis": modvis}, {}, init)
+        b = orc.ref_coef("intflow_boundary_simplec", nx, ny, {"modvis": modvis}, {}, init)
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (modvis, k)
+        init = {"u": rng.uniform(10, 90, n), "v": rng.uniform(-9, 9, n)}
+        a = orc.coef("intflow_extrapolate_u_v_to_fictitious", nx, ny, {"modvis": modvis}, f, init)
+        b = orc.ref_coef("intflow_extrapolate_u_v_to_fictitious", nx, ny, {"modvis": modvis}, f,
+                         init)
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (modvis, k)
+            assert not np.array_equal(a[k], init[k])
+    sc = {"Cpref": 1004.5, "Href": 837000.0}
+    init = {"T": rng.uniform(700, 830, n), "Tbe": np.full(ny, -1.0), "Tbw": np.full(ny, -1.0),
+            "Tbn": np.full(nx, -1.0), "Tbs": np.full(nx, -1.0)}
+    a = orc.coef("intflow_boundary_T_from_H_conservation", nx, ny, sc, f, init)
+    b = orc.ref_coef("intflow_boundary_T_from_H_conservation", nx, ny, sc, f, init)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    sc = {"gamma": 1.4, "Rg": 287.0, "P0": 1038350.4, "T0": 833.33333333}
+    init = {k: rng.uniform(1, 2, ny) for k in ("uin", "vin", "Tin", "Mw", "plin", "pina")}
+    init["pin"] = rng.uniform(9.9e5, 1.03e6, ny)
+    init["p"] = rng.uniform(9.0e5, 1.0e6, n)
+    a = orc.coef("intflow_time_cycle_update", nx, ny, sc, f, init)
+    b = orc.ref_time_cycle_update(nx, ny, sc, f, init)
+    for k in a:
+        assert np.array_equal(a[k].reshape(-1), b[k].reshape(-1)), k
+    assert np.array_equal(a["pina"].reshape(-1)[1:-1], init["pin"][1:-1])
+    # norms and the average
+    a5 = rng.uniform(-1, 1, (5, n))
+    var, rhs = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    assert orc.norm_l1_5d(nx, ny, var, rhs, a5, 0.25) == \
+        orc.norm_l1_5d(nx, ny, var, rhs, a5, 0.25, ref_=True)
+    F = rng.uniform(9e5, 1e6, n)
+    assert orc.get_average(nx, ny, F) == orc.get_average(nx, ny, F, ref_=True)
+
+
+def test_intflow_step_routines_invariants(orc):
+    """Closed forms the reference's formulas fix: at rest the inflow keeps
+    T0 and P0 (Mw = 0), plin is pin - pina, and the west fictitious pressure
+    averages to pin; the T writer puts the face temperature midway between
+    the fictitious and real values; the extrapolation is exact for u linear
+    in xp along rows."""
+    nx, ny = 9, 7
+    f, rng = _step_inputs(nx, ny, 3)
+    n = nx * ny
+    f0 = dict(f, u=np.zeros(n))
+    sc = {"gamma": 1.4, "Rg": 287.0, "P0": 1.0e6, "T0": 800.0}
+    init = {k: np.zeros(ny) for k in ("uin", "vin", "Tin", "Mw", "plin", "pina")}
+    init["pin"] = np.full(ny, 0.9e6)
+    init["p"] = rng.uniform(9.0e5, 1.0e6, n)
+    a = orc.coef("intflow_time_cycle_update", nx, ny, sc, f0, init)
+    J = slice(1, ny - 1)
+    assert (a["Tin"].reshape(-1)[J] == 800.0).all() and (a["pin"].reshape(-1)[J] == 1.0e6).all()
+    assert np.allclose(a["plin"].reshape(-1)[J], 0.1e6)
+    p = a["p"].reshape(ny, nx)
+    assert np.allclose((p[J, 0] + p[J, 1]) / 2, 1.0e6)
+    assert np.array_equal(p[0, 1:-1], p[1, 1:-1]) and np.array_equal(p[-1, 1:-1], p[-2, 1:-1])
+    sc = {"Cpref": 1000.0, "Href": 9.0e5}
+    init = {"T": rng.uniform(700, 830, n), "Tbe": np.zeros(ny), "Tbw": np.zeros(ny),
+            "Tbn": np.zeros(nx), "Tbs": np.zeros(nx)}
+    a = orc.coef("intflow_boundary_T_from_H_conservation", nx, ny, sc, f, init)
+    T = a["T"].reshape(ny, nx)
+    np.testing.assert_allclose((T[J, 0] + T[J, 1]) / 2, f["Tin"][J], rtol=1e-14)
+    ue, ve = f["ue"].reshape(ny, nx), f["ve"].reshape(ny, nx)
+    np.testing.assert_allclose((T[J, -1] + T[J, -2]) / 2,
+                               (9.0e5 - (ue[J, -2] ** 2 + ve[J, -2] ** 2) / 2) / 1000.0,
+                               rtol=1e-13)
+    xp = f["xp"].reshape(ny, nx)
+    u = 3.0 + 0.5 * xp
+    a = orc.coef("intflow_extrapolate_u_v_to_fictitious", nx, ny, {"modvis": 1}, f,
+                 {"u": u.reshape(-1), "v": np.zeros(n)})
+    uw = a["u"].reshape(ny, nx)[J, 0]
+    np.testing.assert_allclose(uw, 3.0 + 0.5 * (-xp[J, 1]), rtol=1e-12)
+
+
+@pytest.mark.parametrize("modvis,solver9d,nx,ny", [(0, 0, 22, 12), (0, 1, 22, 12),
+                                                   (1, 0, 18, 10)])
+def test_time_step_oracle_vs_reference_composition(orc, modvis, solver9d, nx, ny):
+    """One internal-flow time step composed from the oracle's routines equals
+    the same step composed from the compiled reference's own routines, bit
+    for bit: every field, the next dt, the norms, rmass and itp."""
+    import os
+
+    import time_step_oracle as TS
+    if not orc.ref_available():
+        pytest.skip("reference library not built")
+    f = TS.step_fields(nx, ny, seed=nx + modvis)
+    prm = dict(TS.PARAMS, modvis=modvis, solver9d=solver9d)
+    pf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                      "parametersNozzleEulerPTC.txt")
+    a, dta, ra = TS.step(orc, f, nx, ny, prm, 2e-5)
+    b, dtb, rb = TS.step(orc, f, nx, ny, prm, 2e-5, ref=True, paramfile=pf)
+    assert dta == dtb and ra == rb, (dta, dtb, ra, rb)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    assert ra["itp"] >= 1 and np.isfinite(ra["norm"]) and dta > 0
+    for k in ("u", "v", "p", "T", "ro", "pl", "pin"):
+        assert not np.array_equal(a[k], f[k]), k
