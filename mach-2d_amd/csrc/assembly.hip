@@ -27,6 +27,7 @@
 
 #include "m2d_internal.h"
 #include "msi_stencil.h"
+#include "seqsum.h"
 
 namespace m2d {
 
@@ -778,14 +779,14 @@ __global__ __launch_bounds__(kTx) void k_mass_terms(Fields F)
     const long t = (long)(j - 2) * (F.nx - 2) + (i - 2);
     F.out[1][t] = fabs((ro[np] - roa[np]) * rp[np] / (Jp[np] * F.dt) + fme - fmw + fmn - fms);
 }
-__global__ void k_mass_sum(Fields F)
+// the terms added in the reference's order on one lane, staged through LDS
+// by the other waves (one workgroup of kSeqThreads)
+__global__ __launch_bounds__(kSeqThreads) void k_mass_sum(Fields F)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const long n = (long)(F.nx - 2) * (F.ny - 2);
     const double *t = F.out[1];
-    double r = 0.0;
-    for (long k = 0; k < n; ++k) r = r + t[k];
-    F.out[0][0] = r;
+    const double r = seq_sum([&](long k) { return t[k]; }, n);
+    if (threadIdx.x == 0) F.out[0][0] = r;
 }
 
 // ---------------------------------------------------------------------------
@@ -1099,7 +1100,7 @@ int run(const Routine &R, Fields F)
     if (R.post) G.out[R.nout] = d + off;  // scratch
     if (!rc) {
         hipLaunchKernelGGL(R.kernel, grid, block, 0, s, G);
-        if (R.post) hipLaunchKernelGGL(R.post, dim3(1), dim3(64), 0, s, G);
+        if (R.post) hipLaunchKernelGGL(R.post, dim3(1), dim3(kSeqThreads), 0, s, G);
         if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
     }
     if (!dev)
