@@ -266,6 +266,15 @@ def pl_loop_run(reps, cpu_itp=3):
     return out
 
 
+# the shipped nozzle settings (parametersNozzleEulerPTC.txt: itmmax, itpmax,
+# nitm_u, nitm_p, tolerances, beta, Scarborough h0/mincc/maxcc; constant
+# properties Rg, gamma) with chamber conditions of a slow test flow
+NOZZLE = dict(modvis=0, coord=1, solver9d=0, itmmax=1, itpmax=20, nitm_u=5, nitm_p=20,
+              tol_u=1e-2, tol_p=1e-2, tolm=1e-2, beta=0.0, Rg=287.0, gamma=1.4, P0=1.2e5,
+              T0=310.0, Cpref=1004.5, Href=1004.5 * 310.0, Twall=-1.0, h0=0.01, mincc=0.99,
+              maxcc=0.9999)
+
+
 def time_step_run(steps=20, nx=258, ny=130, cpu_steps=3):
     """The full SIMPLEC time step of the internal-flow case (Euler, constant
     properties, the shipped nozzle parameters: itmmax 1, itpmax 20, nitm_u 5,
@@ -275,13 +284,11 @@ def time_step_run(steps=20, nx=258, ny=130, cpu_steps=3):
     one core) beside it when built."""
     import torch
 
-    from mach2d import simplec, solvers
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import time_step_oracle as TS
+    from mach2d import problems, simplec, solvers
     out = {}
-    f0 = TS.step_fields(nx, ny, seed=5)
+    f0 = problems.time_step_fields(nx, ny, 5, NOZZLE)
     for name, solver9d in (("msi", 0), ("tdma", 1)):
-        prm = dict(TS.PARAMS, solver9d=solver9d)
+        prm = dict(NOZZLE, solver9d=solver9d)
         d = {k: torch.tensor(np.ascontiguousarray(v).reshape(-1), device="cuda")
              for k, v in f0.items()}
         dev9 = solvers.DeviceMSI(nx, ny) if solver9d == 0 else None
@@ -303,17 +310,21 @@ def time_step_run(steps=20, nx=258, ny=130, cpu_steps=3):
             if h is not None:
                 h.close()
     try:
+        if cpu_steps < 1:
+            raise RuntimeError("CPU leg not asked for")
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle as O
+        import time_step_oracle as TS
         ref = O.ref_available()
         pf = os.path.join(ROOT, "tests", "golden", "parametersNozzleEulerPTC.txt")
         # the first step (20 pressure-correction iterations from the initial
         # field) is the GPU's untimed warm-up: leave it out here too
-        s, dt, _r = TS.step(O, f0, nx, ny, dict(TS.PARAMS), 2e-5, ref=ref, paramfile=pf)
+        s, dt, _r = TS.step(O, f0, nx, ny, dict(NOZZLE), 2e-5, ref=ref, paramfile=pf)
         itps = 0
         t = time.perf_counter()
         for _ in range(cpu_steps):
-            s, dt, _r = TS.step(O, s, nx, ny, dict(TS.PARAMS), dt, ref=ref, paramfile=pf)
+            s, dt, _r = TS.step(O, s, nx, ny, dict(NOZZLE), dt, ref=ref, paramfile=pf)
             itps += _r["itp"]
         cpu = (time.perf_counter() - t) / cpu_steps * 1e3
         out["cpu_ms_per_step"] = round(cpu, 2)

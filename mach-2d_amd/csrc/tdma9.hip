@@ -71,7 +71,6 @@ __global__ __launch_bounds__(kTile) void k_tdma_x(const T *__restrict__ a, const
         }
         __syncthreads();
         if (row_ok) {
-            const long rb = (long)j * nx;
             for (int ci = 0; ci < cols; ++ci) {
                 const int i = c0 + ci;
                 const T a1 = ta[kSW][lane][ci], a2 = ta[kS][lane][ci], a3 = ta[kSE][lane][ci];
@@ -115,18 +114,22 @@ __global__ __launch_bounds__(kTile) void k_tdma_x(const T *__restrict__ a, const
                     Pm = Pn;
                     Qm = Qn;
                 }
-                Pw[rb + i] = Pm;
-                Qw[rb + i] = Qm;
+                // column-major P, Q: the lanes (rows) of one column are
+                // adjacent, so these stores and the loads below coalesce
+                Pw[(long)i * ny + j] = Pm;
+                Qw[(long)i * ny + j] = Qm;
             }
         }
     }
-    // back substitution: tdma2d9.f90:175-193
+    // back substitution: tdma2d9.f90:175-193 (the loads do not depend on the
+    // recurrence; unrolling lets them run ahead of it)
     if (row_ok) {
         const long rb = (long)j * nx;
-        T xv = Qw[rb + nx - 1];
+        T xv = Qw[(long)(nx - 1) * ny + j];
         xout[rb + nx - 1] = xv;
+#pragma unroll 8
         for (int i = nx - 2; i >= 0; --i) {
-            xv = Pw[rb + i] * xv + Qw[rb + i];
+            xv = Pw[(long)i * ny + j] * xv + Qw[(long)i * ny + j];
             xout[rb + i] = xv;
         }
     }

@@ -508,3 +508,18 @@ def test_time_step_oracle_vs_reference_composition(orc, modvis, solver9d, nx, ny
     assert ra["itp"] >= 1 and np.isfinite(ra["norm"]) and dta > 0
     for k in ("u", "v", "p", "T", "ro", "pl", "pin"):
         assert not np.array_equal(a[k], f[k]), k
+
+
+def test_problem_radii_match_oracle_radius(orc):
+    """The synthetic-problem radii (mach2d.problems.radii) equal get_radius's
+    restatement bit for bit."""
+    import time_step_oracle as TS
+    from mach2d import problems as P
+    nx, ny = 21, 9
+    g = TS.nozzle_grid(nx, ny)
+    m = P.metrics(g)
+    r = P.radii(g, m["yp"])
+    _, re, rn, rp = orc.radius(1, g.y.reshape(-1), m["yp"].reshape(-1), nx, ny)
+    assert np.array_equal(r["re"].reshape(-1), re)
+    assert np.array_equal(r["rn"].reshape(-1), rn)
+    assert np.array_equal(r["rp"].reshape(-1), rp)

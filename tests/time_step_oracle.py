@@ -13,81 +13,11 @@ PARAMS = dict(modvis=0, coord=1, solver9d=0, itmmax=1, itpmax=20, nitm_u=5, nitm
               maxcc=0.9999)
 
 
-def nozzle_grid(nx, ny):
-    """Converging-diverging nozzle, x in [0, 3], wall radius sqrt(A(x)/pi)
-    with A(x) = 1 + 2.2 (x - 1.5)^2, the axis at y = 0; corners as the
-    reference stores them (north-east corner of each volume)."""
-    xs = np.linspace(0.0, 3.0, nx - 1)
-    rw = np.sqrt((1.0 + 2.2 * (xs - 1.5) ** 2) / np.pi)
-    x = np.zeros((ny, nx))
-    y = np.zeros((ny, nx))
-    fr = np.linspace(0.0, 1.0, ny - 1)
-    x[: ny - 1, : nx - 1] = xs[None, :]
-    y[: ny - 1, : nx - 1] = fr[:, None] * rw[None, :]
-    return P.Grid(nx, ny, x, y, 1.0)
+nozzle_grid = P.nozzle_grid
 
 
 def step_fields(nx, ny, seed, coord=1, u_in=10.0):
-    """Seeded inputs of one step on the nozzle: a slow flow at rest
-    temperature and pressure (velocity u_in A_in / A(x) plus 1 % noise, the
-    matching face velocities and densities), the radii of get_radius, the
-    9-diagonal systems as the reference allocates them (a = 0, unit
-    diagonal), the 5-diagonal one likewise, and the inflow vectors."""
-    import oracle as O
-    g = nozzle_grid(nx, ny)
-    m = P.metrics(g)
-    f = {k: np.ascontiguousarray(v.reshape(-1)) for k, v in m.items()}
-    N = nx * ny
-    rng = np.random.default_rng(seed)
-    xf, yf = g.x.reshape(-1), g.y.reshape(-1)
-    f["x"] = np.ascontiguousarray(xf)
-    rad = O.radius(coord, yf, f["yp"], nx, ny)
-    f["re"], f["rn"], f["rp"] = rad[1], rad[2], rad[3]
-    area = 1.0 + 2.2 * (f["xp"] - 1.5) ** 2
-    T0, P0, Rg = PARAMS["T0"], PARAMS["P0"], PARAMS["Rg"]
-    f["u"] = u_in * 5.95 / area * (1.0 + 0.01 * rng.uniform(-1, 1, N))
-    f["v"] = np.zeros(N)
-    f["T"] = np.full(N, T0 - 2.0)
-    f["p"] = np.full(N, 0.99 * P0) * (1.0 + 1e-4 * rng.uniform(-1, 1, N))
-    f["ro"] = f["p"] / (Rg * f["T"])
-    f["roe"], f["ron"] = f["ro"].copy(), f["ro"].copy()
-    ug = f["u"].reshape(ny, nx)
-    ue = np.zeros((ny, nx))
-    ue[:, :-1] = 0.5 * (ug[:, :-1] + ug[:, 1:])
-    f["ue"], f["un"] = ue.reshape(-1), f["u"].copy()
-    f["ve"], f["vn"] = np.zeros(N), np.zeros(N)
-    f["Uce"] = f["ue"] * f["ye"] - f["ve"] * f["xe"]
-    f["Vcn"] = f["vn"] * f["xk"] - f["un"] * f["yk"]
-    f["pl"] = np.zeros(N)
-    f["g"] = 1.0 / (Rg * f["T"])
-    ap = np.zeros((5, N))
-    ap[2] = 1.0
-    f["ap"], f["bp"] = ap, np.zeros(N)
-    for k in ("vle", "vln", "vlp"):
-        f[k] = np.zeros(N)
-    f["cp"] = np.full(N, 1004.5)
-    for k in ("ke", "kn"):
-        f[k] = np.full(N, 0.026)
-    for k in ("au", "av", "at"):
-        a = np.zeros((9, N))
-        a[4] = 1.0
-        f[k] = a
-    for k in ("due", "dve", "dun", "dvn"):
-        f[k] = np.zeros(N)
-    for k in ("bu", "bv", "bt", "cup", "sup", "cvp", "svp", "de", "dn", "ua", "va", "uea",
-              "vea", "una", "vna", "pa", "Ta", "roa"):
-        f[k] = np.zeros(N)
-    f["pin"] = np.full(ny, 0.99 * P0)
-    for k in ("uin", "vin", "Tin", "Mw", "plin", "pina"):
-        f[k] = np.zeros(ny)
-    f["Tin"] = np.full(ny, T0 - 2.0)
-    for k in ("Tbe", "Tbw"):
-        f[k] = np.zeros(ny)
-    for k in ("Tbn", "Tbs"):
-        f[k] = np.zeros(nx)
-    f["fbe"] = np.zeros(N)
-    f["fbn"] = np.zeros(N)
-    return {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in f.items()}
+    return P.time_step_fields(nx, ny, seed, PARAMS, coord=coord, u_in=u_in)
 
 
 def step(O, f, nx, ny, prm, dt, ref=False, paramfile=None):
