@@ -400,6 +400,54 @@ def grid_tstep_run(reps=20, nx=1026, ny=514):
     return res
 
 
+def batch_cpu_run(count=256, budget_s=12.0):
+    """The 256-system batch on the host: the reference lu2d9 + fb2d9 (compiled
+    Fortran) with one system per thread over all host cores, on a bounded
+    sample (systems taken in order until the budget is spent, each solved to
+    the 1e-10 drop); solves/s of the sample, the batch's total scaled by
+    the sample's share of its points."""
+    import concurrent.futures as cf
+    import threading
+
+    from mach2d import problems as P
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    ref = O.ref_available()
+    sizes = P.cfg4_sizes(count)
+    threads = min(16, os.cpu_count() or 1)
+
+    def solve(q):
+        nx, ny = sizes[q]
+        s = P.pcorr_system(nx, ny, seed=100 + q, q=1.02 + 0.06 * ((q * 0.618) % 1.0),
+                           rhs="gauss")
+        if ref:   # this worker thread has the stack fb2d9 needs
+            O.ref().ref_solver_msi9d(nx, ny, 20000, GAMMA, np.ascontiguousarray(s.a),
+                                     np.ascontiguousarray(s.rhs), np.zeros(s.N))
+        else:
+            O.solve_msi9d(s.a, s.rhs, np.zeros(s.N), nx, ny, 20000, GAMMA)
+        return nx * ny
+
+    old = threading.stack_size()
+    threading.stack_size(256 * 1024 * 1024)   # fb2d9's automatic arrays
+    try:
+        t0 = time.perf_counter()
+        done, pts, q = 0, 0, 0
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            while q < count and time.perf_counter() - t0 < budget_s:
+                futs = [ex.submit(solve, k) for k in range(q, min(count, q + threads))]
+                for fu in futs:
+                    pts += fu.result()
+                    done += 1
+                q += len(futs)
+        dt = time.perf_counter() - t0
+    finally:
+        threading.stack_size(old)
+    total = sum(nx * ny for nx, ny in sizes)
+    return {"solves_per_s": round(done / dt, 3), "threads": threads, "systems": done,
+            "seconds": round(dt, 2), "kind": "reference" if ref else "port",
+            "batch_seconds_est": round(dt * total / pts, 1)}
+
+
 def cpu_baseline(s, nit, budget_s=15.0):
     """Reference lu2d9 + fb2d9 (compiled Fortran, one core) on a bounded
     sample: the factorization plus M iterations; solves/s extrapolated to
@@ -614,6 +662,15 @@ def main():
         lb = batch_run("fp32_ir", 1, 1, count=args.batch_cfg3, same=s, learned=True)
         if lb is not None:
             batch3["learned_fp32_ir"] = lb
+        if not args.no_cpu:
+            try:
+                bc = batch_cpu_run()
+                log("batch cpu: %s" % bc)
+                for m, v in batch.items():
+                    v["cpu_threads_solves_per_s"] = bc["solves_per_s"]
+                batch["cpu"] = bc
+            except Exception as e:  # the checker is optional here
+                log("batch cpu: %s" % e)
         for m, v in batch3.items():
             log("batch 1026x514 %s: %s" % (m, v))
             if "fp64" in summary:
