@@ -142,6 +142,23 @@ def prepare_guess(dev, net, a, rhs, var, nx, ny):
     return True, dev.residual_norm(a, rhs, var), rb
 
 
+def prepare_guesses(dev, net, a_list, rhs_list, var_list, nx, ny):
+    """prepare_guess for several systems of one grid size, with one network
+    pass over all of them.  Returns [(used, ||A var - rhs||, ||rhs||)]."""
+    import torch
+    net.guess_many(a_list, rhs_list, nx, ny, var_list)
+    rbs = [float(v) for v in torch.stack([torch.linalg.norm(r) for r in rhs_list]).cpu()]
+    res = []
+    for a, rhs, var, rb in zip(a_list, rhs_list, var_list, rbs):
+        alpha = dev.guess_scale(a, rhs, var, energy=True)
+        if not (np.isfinite(alpha) and alpha != 0.0):
+            var.zero_()
+            res.append((False, rb, rb))
+        else:
+            res.append((True, dev.residual_norm(a, rhs, var), rb))
+    return res
+
+
 def guessed_solve(dev, net, a, rhs, var, nx, ny, gamma=1e-10, nitm=20000):
     """MSI (handle `dev`, factored) from the network's guess, stopping at
     gamma ||rhs||.  Returns (status, used_guess, ||r(guess)||, ||rhs||)."""

@@ -257,6 +257,32 @@ class UNet:
                                              _ptr(var), stream), "unpack_output")
         return s
 
+    def guess_many(self, a_list, b_list, nx, ny, var_list, stream=None):
+        """guess() for several systems of one grid size with one forward pass
+        over all of them (the images of a batch are independent, so each
+        guess equals guess() of its system alone).  Returns the scales."""
+        t = self.torch
+        h, w = self.padded(nx, ny)
+        n = len(b_list)
+        x = t.empty(n, h, w, IN_CH, dtype=t.bfloat16, device=self.dev)
+        # the scales of all systems (guess()'s expression, system by system,
+        # so the values are the same) with one read back
+        sq = []
+        for a, b in zip(a_list, b_list):
+            ap = a.view(9, -1)[4]
+            r = t.where(ap != 0, b / t.where(ap != 0, ap, t.ones_like(ap)), t.zeros_like(b))
+            sq.append(t.sqrt((r * r).mean()))
+        scales = [float(v) or 1.0 for v in t.stack(sq).cpu()]
+        for q, (a, b) in enumerate(zip(a_list, b_list)):
+            _check(self.L.m2d_unet_pack_input(_ptr(a), _ptr(b), nx, ny, h, w, 1.0 / scales[q],
+                                              _ptr(x[q]), 1, stream), "pack_input")
+        out = self.forward(x, stream)
+        for q, var in enumerate(var_list):
+            _check(self.L.m2d_unet_unpack_output(_ptr(out[q]), out.shape[-1], nx, ny,
+                                                 out.shape[2], scales[q], _ptr(var), stream),
+                   "unpack_output")
+        return scales
+
     # -- weights file ----------------------------------------------------------
     def save(self, path):
         """Versioned binary weights file: b'M2DW', version, layer count, then

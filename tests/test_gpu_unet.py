@@ -359,3 +359,28 @@ def test_guessed_solve_matches_zero_guess_solution(gpu):
     rel = float(torch.linalg.norm(v1 - v0) / torch.linalg.norm(v0))
     assert rel < 1e-8, rel
     dev.close()
+
+
+def test_guess_many_equals_single_guesses(gpu):
+    """One network pass over several systems of one grid gives each system
+    the guess it gets alone, bit for bit (the images are independent)."""
+    import torch
+
+    from mach2d import learned as Lr
+    from mach2d import problems as P
+    from mach2d import unet as U
+    net = U.UNet()
+    net.load(Lr.default_weights())
+    nx, ny = 130, 66
+    sys_ = [P.pcorr_system(nx, ny, seed=40 + q, rhs="gauss") for q in range(3)]
+    a = [torch.tensor(s.a.reshape(-1), device="cuda") for s in sys_]
+    b = [torch.tensor(s.rhs, device="cuda") for s in sys_]
+    one = [torch.zeros(s.N, dtype=torch.float64, device="cuda") for s in sys_]
+    many = [torch.zeros(s.N, dtype=torch.float64, device="cuda") for s in sys_]
+    s1 = [net.guess(a[q], b[q], nx, ny, one[q]) for q in range(3)]
+    s2 = net.guess_many(a, b, nx, ny, many)
+    torch.cuda.synchronize()
+    assert s1 == s2
+    for q in range(3):
+        assert torch.equal(one[q], many[q]), q
+        assert float(one[q].abs().max()) > 0
