@@ -189,6 +189,14 @@ int m2d_guess_scale(m2d_system *s, const double *d_a, const double *d_rhs, doubl
 int m2d_guess_scale_energy(m2d_system *s, const double *d_a, const double *d_rhs, double *d_g,
                            double *alpha_out);
 
+/* m2d_guess_scale(_energy) (energy = 0 / 1) for `count` systems of the
+ * handle's grid (device arrays d_a[q], d_rhs[q], d_g[q]) with two host reads
+ * in all: alpha_out[q] and rnorm_out[q] = ||rhs_q - A_q g_q|| of the scaled
+ * guess.  No reference counterpart (batched learned guesses). */
+int m2d_guess_prepare_many(m2d_system *s, int count, const double *const *d_a,
+                           const double *const *d_rhs, double *const *d_g, int energy,
+                           double *alpha_out, double *rnorm_out);
+
 /* Device copies of the factor (dl 5*nx*ny, du 4*nx*ny doubles, reference
  * layout); FP64 handles only. */
 int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du);

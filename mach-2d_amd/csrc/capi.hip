@@ -608,6 +608,84 @@ static int guess_scale(m2d_system *s, const double *d_a, const double *d_rhs, do
     return M2D_OK;
 }
 
+// per-block partial sums of |b - A g|^2 (blocks of 256 points in index order)
+__global__ __launch_bounds__(256) void k_resid_sq(const double *__restrict__ a,
+                                                  const double *__restrict__ g,
+                                                  const double *__restrict__ b, int nx, int ny,
+                                                  double *__restrict__ part, int nd)
+{
+    const long N = (long)nx * ny;
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    double rr = 0.0;
+    if (p < N) {
+        const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
+        const double v = nd == 5
+                             ? residual_point5<double, double, double>(a, N, g, b[p], i, j, nx, ny)
+                             : residual_point<double, double, double>(a, N, g, b[p], i, j, nx, ny);
+        rr = v * v;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = rr;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// guess_scale for `count` systems of the handle's grid with two host reads
+// in all (the dots of every system, then the residual norms of the scaled
+// guesses): alpha_out[q], and rnorm_out[q] = ||rhs_q - A_q g_q|| after the
+// scaling (block partials added in block order).
+extern "C" int m2d_guess_prepare_many(m2d_system *s, int count, const double *const *d_a,
+                                      const double *const *d_rhs, double *const *d_g,
+                                      int energy, double *alpha_out, double *rnorm_out)
+{
+    if (!s || count < 1 || !d_a || !d_rhs || !d_g || !alpha_out || !rnorm_out)
+        return M2D_ERR_ARG;
+    for (int q = 0; q < count; ++q)
+        if (!d_a[q] || !d_rhs[q] || !d_g[q]) return M2D_ERR_ARG;
+    const long N = s->N;
+    const unsigned nblk = (unsigned)((N + 255) / 256);
+    double *part = nullptr;
+    HIP_OK(hipMalloc((void **)&part, sizeof(double) * 4 * (size_t)nblk * count));
+    std::vector<double> h(4 * (size_t)nblk * count);
+    int rc = M2D_OK;
+    for (int q = 0; q < count; ++q)
+        hipLaunchKernelGGL(k_guess_dots, dim3(nblk), dim3(256), 0, s->stream, d_a[q], d_g[q],
+                           d_rhs[q], s->nx, s->ny, part + 4 * (size_t)nblk * q, s->nd);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        rc = M2D_ERR_HIP;
+    for (int q = 0; q < count && !rc; ++q) {
+        double d[4] = {0.0, 0.0, 0.0, 0.0};
+        const double *pq = h.data() + 4 * (size_t)nblk * q;
+        for (unsigned k = 0; k < nblk; ++k)
+            for (int c = 0; c < 4; ++c) d[c] += pq[4 * k + c];
+        const double num = energy ? d[2] : d[0], den = energy ? d[3] : d[1];
+        const double alpha = den > 0.0 && isfinite(num) && isfinite(den) ? num / den : 0.0;
+        alpha_out[q] = alpha;
+        hipLaunchKernelGGL(k_scale, dim3(nblk), dim3(256), 0, s->stream, d_g[q], N, alpha);
+        hipLaunchKernelGGL(k_resid_sq, dim3(nblk), dim3(256), 0, s->stream, d_a[q], d_g[q],
+                           d_rhs[q], s->nx, s->ny, part + (size_t)nblk * q, s->nd);
+    }
+    if (!rc && (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(h.data(), part, sizeof(double) * (size_t)nblk * count,
+                               hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+                hipStreamSynchronize(s->stream) != hipSuccess))
+        rc = M2D_ERR_HIP;
+    for (int q = 0; q < count && !rc; ++q) {
+        double r2 = 0.0;
+        for (unsigned k = 0; k < nblk; ++k) r2 += h[(size_t)nblk * q + k];
+        rnorm_out[q] = sqrt(r2);
+    }
+    (void)hipFree(part);
+    return rc;
+}
+
 extern "C" int m2d_guess_scale(m2d_system *s, const double *d_a, const double *d_rhs,
                                double *d_g, double *alpha_out)
 {

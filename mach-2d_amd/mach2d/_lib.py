@@ -68,7 +68,7 @@ EXPORTS = [
     "m2d_get_real_centroids_xy", "m2d_get_metrics", "m2d_get_radius",
     "m2d_intflow_boundary_simplec", "m2d_intflow_extrapolate_u_v_to_fictitious",
     "m2d_intflow_boundary_T_from_H_conservation", "m2d_intflow_time_cycle_update",
-    "m2d_norm_l1_5d", "m2d_get_average", "m2d_intflow_time_step",
+    "m2d_norm_l1_5d", "m2d_get_average", "m2d_intflow_time_step", "m2d_guess_prepare_many",
     "m2d_unet_conv3x3", "m2d_unet_conv3x3_wgrad", "m2d_unet_relu_bwd", "m2d_unet_mode_bwd",
     "m2d_unet_to_bf16", "m2d_unet_adam", "m2d_unet_pack_weights", "m2d_unet_mse",
     "m2d_unet_pack_input", "m2d_unet_unpack_output",
@@ -188,6 +188,9 @@ def load():
     L.m2d_norm_l1_5d.restype = None
     L.m2d_intflow_time_step.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double), _vp]
     L.m2d_intflow_time_step.restype = C.c_int
+    L.m2d_guess_prepare_many.argtypes = [_vp, C.c_int, _vp, _vp, _vp, C.c_int,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.m2d_guess_prepare_many.restype = C.c_int
     L.m2d_get_average.argtypes = [_ip, _ip, _vp]
     L.m2d_get_average.restype = C.c_double
     L.m2d_version.argtypes = []

@@ -148,14 +148,14 @@ def prepare_guesses(dev, net, a_list, rhs_list, var_list, nx, ny):
     import torch
     net.guess_many(a_list, rhs_list, nx, ny, var_list)
     rbs = [float(v) for v in torch.stack([torch.linalg.norm(r) for r in rhs_list]).cpu()]
+    alphas, rnorms = dev.guess_prepare_many(a_list, rhs_list, var_list, energy=True)
     res = []
-    for a, rhs, var, rb in zip(a_list, rhs_list, var_list, rbs):
-        alpha = dev.guess_scale(a, rhs, var, energy=True)
+    for var, rb, alpha, rg in zip(var_list, rbs, alphas, rnorms):
         if not (np.isfinite(alpha) and alpha != 0.0):
             var.zero_()
             res.append((False, rb, rb))
         else:
-            res.append((True, dev.residual_norm(a, rhs, var), rb))
+            res.append((True, rg, rb))
     return res
 
 

@@ -420,6 +420,22 @@ class DeviceMSI:
             raise SolverError(rc, "m2d_guess_scale")
         return alpha.value
 
+    def guess_prepare_many(self, a_list, rhs_list, g_list, energy=True):
+        """guess_scale for several systems of this grid with two host reads in
+        all; returns (alphas, ||rhs - A g|| of each scaled guess)."""
+        n = len(g_list)
+        arr = lambda xs, k, name: (C.c_void_p * n)(
+            *[C.c_void_p(self._chk(x, k, name)) for x in xs])
+        pa = arr(a_list, self.nd * self.N, "a")
+        pr = arr(rhs_list, self.N, "rhs")
+        pg = arr(g_list, self.N, "g")
+        alphas, rn = (C.c_double * n)(), (C.c_double * n)()
+        rc = _lib.load().m2d_guess_prepare_many(self.h, n, pa, pr, pg, 1 if energy else 0,
+                                                alphas, rn)
+        if rc:
+            raise SolverError(rc, "m2d_guess_prepare_many")
+        return list(alphas), list(rn)
+
     def solve(self, a, rhs, var, gamma, nitm, hist=None) -> Status:
         st = Status()
         hp, cap = (None, 0)
