@@ -1,8 +1,8 @@
 // seqsum.h -- in-order sum of n terms by one lane, for results that must
 // equal a sequential loop bit for bit (the reference's left-to-right sums).
 //
-// The chain of dependent additions is the whole cost (about 10 cycles per
-// term), so lane 0 of wave 0 only adds, reading the terms from LDS, while
+// The chain of dependent additions is the whole cost, so lane 0 of wave 0
+// only adds, reading the terms from LDS, while
 // the other waves of the workgroup stage the next chunk from HBM.  Call from
 // every thread of a 256-thread workgroup; returns the sum on thread 0.
 #pragma once
@@ -35,23 +35,27 @@ __device__ double seq_sum(Term term, long n)
                     buf[cur ^ 1][q] = term(nb + q);
             }
         } else if (tid == 0) {
+            // groups of G terms: the reads of the next group are in flight
+            // while the G additions of this one run (G additions outlast
+            // the LDS latency)
+            constexpr int G = 16;
             const double *b = buf[cur];
             int q = 0;
-            if (len >= 8) {
-                double v[8], w[8];
+            if (len >= G) {
+                double v[G], w[G];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) v[k] = b[k];
-                for (; q + 16 <= len; q += 8) {
+                for (int k = 0; k < G; ++k) v[k] = b[k];
+                for (; q + 2 * G <= len; q += G) {
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) w[k] = b[q + 8 + k];
+                    for (int k = 0; k < G; ++k) w[k] = b[q + G + k];
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) m = v[k] + m;
+                    for (int k = 0; k < G; ++k) m = v[k] + m;
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) v[k] = w[k];
+                    for (int k = 0; k < G; ++k) v[k] = w[k];
                 }
 #pragma unroll
-                for (int k = 0; k < 8; ++k) m = v[k] + m;
-                q += 8;
+                for (int k = 0; k < G; ++k) m = v[k] + m;
+                q += G;
             }
             for (; q < len; ++q) m = b[q] + m;
         }
