@@ -29,6 +29,9 @@
 namespace m2d {
 
 constexpr int kTile = 64;
+// threads of an x-pass workgroup: all stage the tiles, the first kTile run
+// the rows
+constexpr int kLoad = 4 * kTile;
 
 // x-pass: rows j = blockIdx.x*64 + lane.  xin is the pass input, xout its
 // output (P, Q scratch of N each).  Tiles of 64 rows x 16 columns are staged
@@ -37,7 +40,7 @@ constexpr int kTile = 64;
 constexpr int kTX = 16;
 
 template <typename T>
-__global__ __launch_bounds__(kTile) void k_tdma_x(const T *__restrict__ a, const T *__restrict__ b,
+__global__ __launch_bounds__(kLoad) void k_tdma_x(const T *__restrict__ a, const T *__restrict__ b,
                                                   const T *__restrict__ xin, T *__restrict__ xout,
                                                   T *__restrict__ Pw, T *__restrict__ Qw, int nx,
                                                   int ny)
@@ -56,7 +59,7 @@ __global__ __launch_bounds__(kTile) void k_tdma_x(const T *__restrict__ a, const
     for (int c0 = 0; c0 < nx; c0 += kTX) {
         const int cols = min(kTX, nx - c0);
         __syncthreads();
-        for (int idx = lane; idx < rows * cols; idx += kTile) {
+        for (int idx = lane; idx < rows * cols; idx += kLoad) {
             const int rr = idx / cols, cc = idx - rr * cols;
             const long p = (long)(j0 + rr) * nx + c0 + cc;
 #pragma unroll
@@ -64,7 +67,7 @@ __global__ __launch_bounds__(kTile) void k_tdma_x(const T *__restrict__ a, const
             tb[rr][cc] = b[p];
         }
         const int xr = rows + 2, xc = cols + 2;
-        for (int idx = lane; idx < xr * xc; idx += kTile) {
+        for (int idx = lane; idx < xr * xc; idx += kLoad) {
             const int rr = idx / xc, cc = idx - rr * xc;
             const int jj = j0 + rr - 1, col = c0 + cc - 1;
             if (jj >= 0 && jj < ny && col >= 0 && col < nx) tx[rr][cc] = xin[(long)jj * nx + col];
@@ -147,6 +150,7 @@ __global__ __launch_bounds__(kTile) void k_tdma_y(const T *__restrict__ a, const
     if (i >= nx) return;
     const bool west = i == 0, east = i == nx - 1;
     T Pm = 0, Qm = 0;
+#pragma unroll 4
     for (int j = 0; j < ny; ++j) {
         const long p = (long)j * nx + i;
         const T a1 = a[kSW * nn + p], a2 = a[kS * nn + p], a3 = a[kSE * nn + p];
@@ -209,7 +213,7 @@ template <typename T>
 void launch_tdma_iteration(const T *a, const T *b, T *x, T *tmp, T *Pw, T *Qw, int nx, int ny,
                            hipStream_t s)
 {
-    hipLaunchKernelGGL(k_tdma_x<T>, dim3((ny + kTile - 1) / kTile), dim3(kTile), 0, s, a, b, x,
+    hipLaunchKernelGGL(k_tdma_x<T>, dim3((ny + kTile - 1) / kTile), dim3(kLoad), 0, s, a, b, x,
                        tmp, Pw, Qw, nx, ny);
     hipLaunchKernelGGL(k_tdma_y<T>, dim3((nx + kTile - 1) / kTile), dim3(kTile), 0, s, a, b, tmp,
                        x, Pw, Qw, nx, ny);
