@@ -138,70 +138,103 @@ __global__ __launch_bounds__(kLoad) void k_tdma_x(const T *__restrict__ a, const
     }
 }
 
-// y-pass: column i = blockIdx.x*blockDim.x + threadIdx.x (tdma2d9.f90:200-316).
+// y-pass: column i = blockIdx.x*kTile + lane (tdma2d9.f90:200-316).  Chunks
+// of kRY rows of the 64 columns (coefficients, b and the x halo) are staged
+// through LDS by all four waves, so the recurrence of each column reads LDS
+// instead of waiting on HBM row by row.
+constexpr int kRY = 16;
+
 template <typename T>
-__global__ __launch_bounds__(kTile) void k_tdma_y(const T *__restrict__ a, const T *__restrict__ b,
+__global__ __launch_bounds__(kLoad) void k_tdma_y(const T *__restrict__ a, const T *__restrict__ b,
                                                   const T *__restrict__ xin, T *__restrict__ xout,
                                                   T *__restrict__ Pw, T *__restrict__ Qw, int nx,
                                                   int ny)
 {
     const long nn = (long)nx * ny;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nx) return;
+    const int lane = threadIdx.x;
+    const int i0 = blockIdx.x * kTile;
+    const int cols = min(kTile, nx - i0);
+    const int i = i0 + lane;
+    const bool col_ok = lane < cols;
     const bool west = i == 0, east = i == nx - 1;
+    __shared__ T ta[9][kRY][kTile];
+    __shared__ T tb[kRY][kTile];
+    __shared__ T tx[kRY + 2][kTile + 2];  // rows r0-1 .. r0+kRY, columns i0-1 .. i0+64
     T Pm = 0, Qm = 0;
-#pragma unroll 4
-    for (int j = 0; j < ny; ++j) {
-        const long p = (long)j * nx + i;
-        const T a1 = a[kSW * nn + p], a2 = a[kS * nn + p], a3 = a[kSE * nn + p];
-        const T a4 = a[kW * nn + p], a5 = a[kP * nn + p], a6 = a[kE * nn + p];
-        const T a7 = a[kNW * nn + p], a8 = a[kN * nn + p], a9 = a[kNE * nn + p];
-        const T bp = b[p];
-        // neighbour x (the pass input); out-of-grid reads are never used
-        const T xw = west ? T(0) : xin[p - 1];
-        const T xe = east ? T(0) : xin[p + 1];
-        const T xnw = (west || j == ny - 1) ? T(0) : xin[p + nx - 1];
-        const T xne = (east || j == ny - 1) ? T(0) : xin[p + nx + 1];
-        const T xsw = (west || j == 0) ? T(0) : xin[p - nx - 1];
-        const T xse = (east || j == 0) ? T(0) : xin[p - nx + 1];
-        T D;
-        if (j == 0) {
-            if (west)
-                D = bp - a6 * xe - a9 * xne;
-            else if (east)
-                D = bp - a4 * xw - a7 * xnw;
-            else
-                D = bp - a4 * xw - a7 * xnw - a6 * xe - a9 * xne;
-            Pm = -a8 / a5;
-            Qm = D / a5;
-        } else {
-            if (j < ny - 1) {
-                if (west)
-                    D = bp - a3 * xse - a6 * xe - a9 * xne;
-                else if (east)
-                    D = bp - a1 * xsw - a4 * xw - a7 * xnw;
-                else
-                    D = bp - a1 * xsw - a4 * xw - a7 * xnw - a3 * xse - a6 * xe - a9 * xne;
-            } else {
-                if (west)
-                    D = bp - a3 * xse - a6 * xe;
-                else if (east)
-                    D = bp - a1 * xsw - a4 * xw;
-                else
-                    D = bp - a1 * xsw - a4 * xw - a3 * xse - a6 * xe;
-            }
-            const T Pn = -a8 / (a5 + a2 * Pm);
-            const T Qn = (D - a2 * Qm) / (a5 + a2 * Pm);
-            Pm = Pn;
-            Qm = Qn;
+    for (int r0 = 0; r0 < ny; r0 += kRY) {
+        const int rows = min(kRY, ny - r0);
+        __syncthreads();
+        for (int idx = lane; idx < rows * cols; idx += kLoad) {
+            const int rr = idx / cols, cc = idx - rr * cols;
+            const long p = (long)(r0 + rr) * nx + i0 + cc;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) ta[k][rr][cc] = a[k * nn + p];
+            tb[rr][cc] = b[p];
         }
-        Pw[p] = Pm;
-        Qw[p] = Qm;
+        const int xr = rows + 2, xc = cols + 2;
+        for (int idx = lane; idx < xr * xc; idx += kLoad) {
+            const int rr = idx / xc, cc = idx - rr * xc;
+            const int jj = r0 + rr - 1, col = i0 + cc - 1;
+            if (jj >= 0 && jj < ny && col >= 0 && col < nx) tx[rr][cc] = xin[(long)jj * nx + col];
+        }
+        __syncthreads();
+        if (!col_ok) continue;
+        for (int rj = 0; rj < rows; ++rj) {
+            const int j = r0 + rj;
+            const long p = (long)j * nx + i;
+            const T a1 = ta[kSW][rj][lane], a2 = ta[kS][rj][lane], a3 = ta[kSE][rj][lane];
+            const T a4 = ta[kW][rj][lane], a5 = ta[kP][rj][lane], a6 = ta[kE][rj][lane];
+            const T a7 = ta[kNW][rj][lane], a8 = ta[kN][rj][lane], a9 = ta[kNE][rj][lane];
+            const T bp = tb[rj][lane];
+            // neighbour x: tx[row - r0 + 1][col - i0 + 1]; entries outside the
+            // grid are never staged and never used by the branch taken
+            const T xw = west ? T(0) : tx[rj + 1][lane];
+            const T xe = east ? T(0) : tx[rj + 1][lane + 2];
+            const T xnw = (west || j == ny - 1) ? T(0) : tx[rj + 2][lane];
+            const T xne = (east || j == ny - 1) ? T(0) : tx[rj + 2][lane + 2];
+            const T xsw = (west || j == 0) ? T(0) : tx[rj][lane];
+            const T xse = (east || j == 0) ? T(0) : tx[rj][lane + 2];
+            T D;
+            if (j == 0) {
+                if (west)
+                    D = bp - a6 * xe - a9 * xne;
+                else if (east)
+                    D = bp - a4 * xw - a7 * xnw;
+                else
+                    D = bp - a4 * xw - a7 * xnw - a6 * xe - a9 * xne;
+                Pm = -a8 / a5;
+                Qm = D / a5;
+            } else {
+                if (j < ny - 1) {
+                    if (west)
+                        D = bp - a3 * xse - a6 * xe - a9 * xne;
+                    else if (east)
+                        D = bp - a1 * xsw - a4 * xw - a7 * xnw;
+                    else
+                        D = bp - a1 * xsw - a4 * xw - a7 * xnw - a3 * xse - a6 * xe - a9 * xne;
+                } else {
+                    if (west)
+                        D = bp - a3 * xse - a6 * xe;
+                    else if (east)
+                        D = bp - a1 * xsw - a4 * xw;
+                    else
+                        D = bp - a1 * xsw - a4 * xw - a3 * xse - a6 * xe;
+                }
+                const T Pn = -a8 / (a5 + a2 * Pm);
+                const T Qn = (D - a2 * Qm) / (a5 + a2 * Pm);
+                Pm = Pn;
+                Qm = Qn;
+            }
+            Pw[p] = Pm;
+            Qw[p] = Qm;
+        }
     }
+    if (!col_ok) return;
     // back substitution: tdma2d9.f90:298-316
     long p = (long)(ny - 1) * nx + i;
     T xv = Qw[p];
     xout[p] = xv;
+#pragma unroll 8
     for (int j = ny - 2; j >= 0; --j) {
         p = (long)j * nx + i;
         xv = xv * Pw[p] + Qw[p];
@@ -215,7 +248,7 @@ void launch_tdma_iteration(const T *a, const T *b, T *x, T *tmp, T *Pw, T *Qw, i
 {
     hipLaunchKernelGGL(k_tdma_x<T>, dim3((ny + kTile - 1) / kTile), dim3(kLoad), 0, s, a, b, x,
                        tmp, Pw, Qw, nx, ny);
-    hipLaunchKernelGGL(k_tdma_y<T>, dim3((nx + kTile - 1) / kTile), dim3(kTile), 0, s, a, b, tmp,
+    hipLaunchKernelGGL(k_tdma_y<T>, dim3((nx + kTile - 1) / kTile), dim3(kLoad), 0, s, a, b, tmp,
                        x, Pw, Qw, nx, ny);
 }
 template void launch_tdma_iteration<double>(const double *, const double *, double *, double *,
