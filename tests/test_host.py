@@ -317,3 +317,20 @@ def test_fortran_tstepper_module_compiles(orc, tmp_path):
     r = subprocess.run([fc, "-c", "-I", moddir, "-module-dir", str(tmp_path), "-o",
                         str(tmp_path / "t.o"), src], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_time_step_entry_rejects_bad_arguments():
+    """m2d_intflow_time_step validates its arguments before touching the
+    GPU: missing handles, fields or parameters are M2D_ERR_ARG."""
+    import ctypes as C
+
+    from mach2d import _lib, simplec
+    L = _lib.load()
+    f = simplec.TimeStepFields()
+    p = simplec.TimeStepParams(itmmax=1, nitm_u=5)
+    dt = C.c_double(1e-5)
+    r = simplec.TimeStepResult()
+    assert L.m2d_intflow_time_step(None, None, C.byref(f), C.byref(p), C.byref(dt),
+                                   C.byref(r)) == _lib.M2D_ERR_ARG
+    assert L.m2d_intflow_time_step(None, None, None, None, None, None) == _lib.M2D_ERR_ARG
+    assert dt.value == 1e-5
