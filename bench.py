@@ -418,8 +418,7 @@ def batch_cpu_run(count=256, budget_s=12.0):
 
     def solve(q):
         nx, ny = sizes[q]
-        s = P.pcorr_system(nx, ny, seed=100 + q, q=1.02 + 0.06 * ((q * 0.618) % 1.0),
-                           rhs="gauss")
+        s = P.cfg4_system(q, nx, ny)
         if ref:   # this worker thread has the stack fb2d9 needs
             O.ref().ref_solver_msi9d(nx, ny, 20000, GAMMA, np.ascontiguousarray(s.a),
                                      np.ascontiguousarray(s.rhs), np.zeros(s.N))
@@ -513,8 +512,7 @@ def batch_run(mode, steps, warmup, count=256, same=None, learned=False):
     t = time.time()
     if same is None:
         sizes = P.cfg4_sizes(count)
-        systems = [P.pcorr_system(nx, ny, seed=100 + q, q=1.02 + 0.06 * ((q * 0.618) % 1.0),
-                                  rhs="gauss") for q, (nx, ny) in enumerate(sizes)]
+        systems = P.cfg4_systems(count)
         a = [torch.tensor(x.a.reshape(-1), device="cuda") for x in systems]
         rhs = [torch.tensor(x.rhs, device="cuda") for x in systems]
     else:

@@ -16,9 +16,11 @@ alpha = (g.b)/(g.A g) (m2d_guess_scale_energy): it weighs the smooth error
 components, the ones MSI removes slowest, where the residual-minimising
 amplitude (m2d_guess_scale) is dominated by the oscillatory part of the
 residual and shrinks a smooth guess towards zero.  A guess with g.A g <= 0
-gets alpha = 0, i.e. the zero guess.  The solve stops when ||r|| <
-gamma ||b||, the zero guess's threshold, so a guess is never asked for more
-accuracy than the zero-guess solve delivers.
+gets alpha = 0, i.e. the zero guess.  Plain MSI from the zero guess is the
+fallback and the judge: a scaled guess whose residual norm is not below
+||b|| (the zero guess's residual) is replaced by the zero guess.  The solve
+stops when ||r|| < gamma ||b||, the zero guess's threshold, so a guess is
+never asked for more accuracy than the zero-guess solve delivers.
 """
 from __future__ import annotations
 
@@ -126,32 +128,48 @@ def train(net, X, T, M, epochs=4, batch=16, lr=1e-3, seed=0, log=print):
     return steps
 
 
-def prepare_guess(dev, net, a, rhs, var, nx, ny):
-    """The network's guess in `var`, scaled by the energy-minimising
-    amplitude (handle `dev`, FP64 residual norms).  Returns (used,
-    ||A var - rhs||, ||rhs||); used is False when the amplitude is 0 (var is
-    then the zero guess)."""
+def judge_guess(dev, a, rhs, var, rg=None):
+    """The residual-norm judge of a guess held in `var`: plain MSI from the
+    zero guess is the fallback, so the guess is kept only when its residual
+    ||A var - rhs|| is below the zero guess's, ||rhs||; otherwise `var` is
+    zeroed.  Returns (used, ||A var - rhs||, ||rhs||) for what `var` holds
+    afterwards.  `rg` passes a residual norm already computed for `var`."""
     import torch
     rb = float(torch.linalg.norm(rhs))
-    net.guess(a, rhs, nx, ny, var)
-    alpha = dev.guess_scale(a, rhs, var, energy=True)
-    used = bool(np.isfinite(alpha) and alpha != 0.0)
-    if not used:
+    if rg is None:
+        rg = dev.residual_norm(a, rhs, var)
+    if not (np.isfinite(rg) and rg < rb):
         var.zero_()
         return False, rb, rb
-    return True, dev.residual_norm(a, rhs, var), rb
+    return True, rg, rb
+
+
+def prepare_guess(dev, net, a, rhs, var, nx, ny):
+    """The network's guess in `var`, scaled by the energy-minimising
+    amplitude (handle `dev`, FP64 residual norms), then judged by
+    judge_guess: kept only if its residual is below ||rhs||, else the zero
+    guess.  Returns (used, ||A var - rhs||, ||rhs||)."""
+    import torch
+    net.guess(a, rhs, nx, ny, var)
+    alpha = dev.guess_scale(a, rhs, var, energy=True)
+    if not (np.isfinite(alpha) and alpha != 0.0):
+        var.zero_()
+        rb = float(torch.linalg.norm(rhs))
+        return False, rb, rb
+    return judge_guess(dev, a, rhs, var)
 
 
 def prepare_guesses(dev, net, a_list, rhs_list, var_list, nx, ny):
     """prepare_guess for several systems of one grid size, with one network
-    pass over all of them.  Returns [(used, ||A var - rhs||, ||rhs||)]."""
+    pass over all of them and the same residual-norm judge per system.
+    Returns [(used, ||A var - rhs||, ||rhs||)]."""
     import torch
     net.guess_many(a_list, rhs_list, nx, ny, var_list)
     rbs = [float(v) for v in torch.stack([torch.linalg.norm(r) for r in rhs_list]).cpu()]
     alphas, rnorms = dev.guess_prepare_many(a_list, rhs_list, var_list, energy=True)
     res = []
     for var, rb, alpha, rg in zip(var_list, rbs, alphas, rnorms):
-        if not (np.isfinite(alpha) and alpha != 0.0):
+        if not (np.isfinite(alpha) and alpha != 0.0 and np.isfinite(rg) and rg < rb):
             var.zero_()
             res.append((False, rb, rb))
         else:

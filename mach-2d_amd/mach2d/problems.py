@@ -359,6 +359,17 @@ def cfg4_sizes(count: int = 256, seed: int = 4):
     return [(int(round(64 * v)) + 2, int(round(32 * v)) + 2) for v in s]
 
 
+def cfg4_system(q: int, nx: int, ny: int) -> System:
+    """System q of the uneven batch: its own seeded wall stretching (ratio in
+    [1.02, 1.08)) and seeded Gaussian right-hand side."""
+    return pcorr_system(nx, ny, seed=100 + q, q=1.02 + 0.06 * ((q * 0.618) % 1.0), rhs="gauss")
+
+
+def cfg4_systems(count: int = 256):
+    """The uneven batch: cfg4_sizes(count) with cfg4_system per member."""
+    return [cfg4_system(q, nx, ny) for q, (nx, ny) in enumerate(cfg4_sizes(count))]
+
+
 def flow_fields(nx: int, ny: int, seed: int = 0, q: float = 1.05, coord: int = 1,
                 cartesian: bool = False) -> dict:
     """Seeded inputs of the coefficient assembly (mach2d_coef.f90) on the

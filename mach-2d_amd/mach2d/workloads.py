@@ -77,8 +77,7 @@ def batch(mode, count, same, layout, inner_gamma=None, inner_nitm=50):
             rhs.append(torch.where(mask, r, torch.zeros_like(r)))
     else:
         sizes = P.cfg4_sizes(count)
-        systems = [P.pcorr_system(nx, ny, seed=100 + q, q=1.02 + 0.06 * ((q * 0.618) % 1.0),
-                                  rhs="gauss") for q, (nx, ny) in enumerate(sizes)]
+        systems = P.cfg4_systems(count)
         a = [torch.tensor(x.a.reshape(-1), device="cuda") for x in systems]
         rhs = [torch.tensor(x.rhs, device="cuda") for x in systems]
     var = [torch.zeros(nx * ny, dtype=torch.float64, device="cuda") for nx, ny in sizes]
