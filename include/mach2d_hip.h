@@ -111,6 +111,20 @@ void m2d_solve_msi5d(const int *nx, const int *ny, const int *nitm,
                      const double *tol, const double *a, const double *b,
                      double *sol);
 
+/* No counterpart in the reference.  m2d_solve_msi9d on DEVICE arrays a, b,
+ * sol (same meaning, FP64, the same results bit for bit) that allocates
+ * nothing: for callers that cannot call hipMalloc (tight serving loops,
+ * harnesses that forbid allocation).  Every buffer beyond a, b and sol
+ * comes from a 48 MiB device array the library holds on each GPU it runs
+ * on until the process ends (grids up to about 550x270 fit; larger ones
+ * return M2D_ERR_ARG with a message).  Calls that overlap on one GPU race
+ * on that array.  Runs on `stream` (hipStream_t as void*, NULL = default)
+ * and returns when sol is final; returns the status (also in
+ * m2d_last_status). */
+int m2d_solve_msi9d_noalloc(const int *nx, const int *ny, const int *nitm,
+                            const double *tol, const double *a, const double *b,
+                            double *sol, void *stream);
+
 /* Replace norm_l1_9d, norm_l1_9d_relative (accumulate into *norm) and
  * norm_l1_b, src/solvers/mod_solvers.f90:128-170, 176-245, 250-276. */
 void m2d_norm_l1_9d(const int *nx, const int *ny, const double *var,

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "m2d_internal.h"
@@ -1024,6 +1025,134 @@ extern "C" void m2d_solve_msi5d(const int *nx, const int *ny, const int *nitm, c
         return;
     }
     host_fb(a, nullptr, nullptr, *ny, *nx, *nx * *ny, sol, *tol, *nitm, b, true, 5);
+}
+
+// ---------------------------------------------------------------------------
+// class_solver_msi9d%solve on device arrays without allocation.  Every
+// device buffer the solve needs beyond the caller's a, b and sol (the skewed
+// copy of A, the factor, r, w/z, the residual partial sums, the boundary
+// words and the solve state) is carved from one file-scope device array.
+// Each GPU holds its own copy of the array from the library's load until
+// the process ends, and every call on that GPU reuses it: calls that
+// overlap on one GPU (two host threads, or two streams) race on it.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = size_t(48) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocDevices = 64;
+static unsigned char *g_scratch_dev[kNoAllocDevices];  // device address of `scratch`, per device
+static std::mutex g_scratch_mu;
+
+// Device address of this device's `scratch` (looked up once per device, with
+// that device current).  In host code the array's name is not a device
+// address; only this pointer is.
+static int scratch_address(unsigned char **out)
+{
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices) return M2D_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return M2D_OK;
+}
+
+// Byte offsets of the solve's buffers inside `scratch` (each 256-byte
+// aligned); returns the bytes needed.
+struct NoAllocLayout {
+    size_t a, dl, du, r, w, partial, bnd, st, total;
+};
+static NoAllocLayout noalloc_layout(int nx, int ny)
+{
+    auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+    const size_t L = (size_t)skew_for(nx, ny, (int)sizeof(double)).size() * sizeof(double);
+    NoAllocLayout o;
+    o.a = 0;
+    o.dl = o.a + up(9 * L);
+    o.du = o.dl + up(6 * L);
+    o.r = o.du + up(4 * L);
+    o.w = o.r + up(L);
+    o.partial = o.w + up(L);
+    o.bnd = o.partial + up(sizeof(double) * (size_t)residual_blocks(nx, ny));
+    o.st = o.bnd + up(sizeof(u64) * (size_t)(boundary_words(nx, ny) + scratch_words(ny)));
+    o.total = o.st + up(sizeof(SolveState));
+    return o;
+}
+
+extern "C" int m2d_solve_msi9d_noalloc(const int *nx, const int *ny, const int *nitm,
+                                       const double *tol, const double *d_a, const double *d_b,
+                                       double *d_sol, void *stream)
+{
+    if (!nx || !ny || !nitm || !tol || !d_a || !d_b || !d_sol || *nx < 3 || *ny < 3 ||
+        (long)*nx * *ny > (1L << 31) / 16) {
+        set_last(M2D_ERR_ARG);
+        return M2D_ERR_ARG;
+    }
+    const NoAllocLayout o = noalloc_layout(*nx, *ny);
+    if (o.total > kNoAllocBytes) {
+        fprintf(stderr,
+                "mach2d_hip: m2d_solve_msi9d_noalloc: a %dx%d grid needs %zu bytes of solver "
+                "memory, more than the %zu bytes the library holds for it\n",
+                *nx, *ny, o.total, kNoAllocBytes);
+        set_last(M2D_ERR_ARG);
+        return M2D_ERR_ARG;
+    }
+    unsigned char *base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc) {
+        set_last(rc);
+        return rc;
+    }
+    // a host-side view of the carved buffers; it owns nothing and is never
+    // passed to m2d_destroy
+    m2d_system s;
+    SolveState h_st;
+    s.nx = *nx;
+    s.ny = *ny;
+    s.N = (long)*nx * *ny;
+    s.nd = 9;
+    s.precision = M2D_FP64;
+    s.stream = (hipStream_t)stream;
+    s.np = residual_blocks(*nx, *ny);
+    s.nbnd = boundary_words(*nx, *ny) + scratch_words(*ny);
+    s.w64.sk = skew_for(*nx, *ny, (int)sizeof(double));
+    s.w64.a = (double *)(base + o.a);
+    s.w64.dl = (double *)(base + o.dl);
+    s.w64.du = (double *)(base + o.du);
+    s.w64.r = (double *)(base + o.r);
+    s.w64.w = (double *)(base + o.w);
+    s.partial = (double *)(base + o.partial);
+    s.bnd = (u64 *)(base + o.bnd);
+    s.st = (SolveState *)(base + o.st);
+    s.h_st = &h_st;
+    // the array keeps the last call's contents: the padding entries of r and
+    // w (read by the sweeps, never written by the residual) start at zero
+    const size_t L = (size_t)s.w64.sk.size() * sizeof(double);
+    HIP_OK(hipMemsetAsync(s.w64.r, 0, L, s.stream));
+    HIP_OK(hipMemsetAsync(s.w64.w, 0, L, s.stream));
+    enqueue_factor64(&s, d_a);
+    HIP_OK(hipGetLastError());
+    rc = read_state(&s, s.st);
+    if (!rc && h_st.outside) rc = M2D_ERR_OUTSIDE_COUPLING;
+    if (!rc && h_st.abort) rc = M2D_ERR_SYNC_TIMEOUT;
+    if (rc) {
+        set_last(rc);
+        return rc;
+    }
+    rc = run_msi<double>(&s, s.w64, s.st, d_a, d_b, d_sol, *tol, *nitm, nullptr, 0);
+    if (rc) {
+        set_last(rc);
+        return rc;
+    }
+    m2d_status st;
+    memset(&st, 0, sizeof(st));
+    fill_status(&s, &st, 0.f);
+    g_last = st;
+    return st.status;
 }
 
 // Batched entry: every system keeps its own handle, factor and arrays.

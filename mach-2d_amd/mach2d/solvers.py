@@ -226,6 +226,26 @@ class ClassSolverMSI9d(ClassSolverAbstract):
         return self.status
 
 
+def solve_msi9d_noalloc(nx, ny, nitm, tol, a, b, sol, stream=None):
+    """class_solver_msi9d%solve on CUDA tensors (a: 9*nx*ny, b and sol:
+    nx*ny, float64) through m2d_solve_msi9d_noalloc, which allocates no
+    device memory (its buffers live in an array the library holds per GPU,
+    for grids up to about 550x270).  Returns the Status of the call."""
+    import torch
+    n = nx * ny
+    for t, m, name in ((a, 9 * n, "a"), (b, n, "b"), (sol, n, "sol")):
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
+                and t.is_contiguous() and t.numel() == m):
+            raise TypeError("%s must be a contiguous float64 CUDA tensor of %d values" % (name, m))
+    rc = _lib.load().m2d_solve_msi9d_noalloc(_i(nx), _i(ny), _i(nitm), _d(tol),
+                                             C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()),
+                                             C.c_void_p(sol.data_ptr()),
+                                             C.c_void_p(stream) if stream else None)
+    st = last_status()
+    st.status = rc
+    return st
+
+
 class ClassSolverMSI5d(ClassSolverAbstract):
     """MSI for 5-diagonal systems (mod_solver_msi.f90:17-31,56-93): every
     solve factors then iterates (lu2d5 + fb2d5)."""
