@@ -1229,8 +1229,10 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
                                      const m2d_time_step_params *P, double *dt_io,
                                      m2d_time_step_result *res)
 {
+    // every parameter is checked here, before the step changes any field
     if (!s5 || !f || !P || !dt_io || s5->nd != 5 || s5->precision != M2D_FP64 ||
-        P->itmmax < 1 || P->nitm_u < 1 || (P->solver9d != 0 && P->solver9d != 1))
+        P->itmmax < 1 || P->nitm_u < 1 || P->itpmax < 1 || P->nitm_p < 1 ||
+        (P->modvis != 0 && P->modvis != 1) || (P->solver9d != 0 && P->solver9d != 1))
         return M2D_ERR_ARG;
     if (P->solver9d == 0 && (!s9 || s9->nd != 9 || s9->precision != M2D_FP64 ||
                              s9->nx != s5->nx || s9->ny != s5->ny))

@@ -10,9 +10,11 @@
 // does row j = t+2 (west and east ends, then the corners next to it when j
 // is 2 or ny-1) and column i = t+2 (south and north ends), reading before
 // writing, so every value is the reference's (operation order kept, no
-// contraction) and no thread reads what another writes.  The exception is
-// the inflow pressure of intflow_time_cycle_update, a real power (x**y):
-// the device pow and the host libm pow may differ in the last bit.
+// contraction) and no thread reads what another writes.  The inflow
+// pressure of intflow_time_cycle_update, a real power (x**y), is evaluated
+// on the host with the C library's pow (the function the reference calls;
+// the device pow may differ in the last bit), from u values read back, so
+// it is the reference's too.
 // get_average adds the real volumes in the reference's order on one lane
 // (bit for bit); norm_l1_5d is a fixed-tree sum (as the 9-diagonal norms).
 //
@@ -22,6 +24,7 @@
 // (Fortran bind(C) callers pass them unchanged).  Status: m2d_last_status.
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "m2d_internal.h"
@@ -208,15 +211,43 @@ __global__ __launch_bounds__(kT) void k_T_from_H(int nx, int ny, double Cpref, d
     }
 }
 
+// The inflow pressure of get_uin_vin_pin_Tin_Mw (mod_intflow.f90:968-1006)
+// for row j: a real power, evaluated on the host with the C library's pow,
+// the function the reference's x**y calls, so the value is the reference's
+// bit for bit (the device pow may differ in the last bit).  uw, ue: u of the
+// west fictitious volume and its east neighbour.
+static double inflow_pressure(double gamma, double Rg, double po, double T0, double uw,
+                              double ue)
+{
+    const double ui = (uw + ue) / 2.0, vi = 0.0;
+    const double Ti = T0 - (gamma - 1.0) / (2.0 * gamma * Rg) * (ui * ui + vi * vi);
+    const double M = sqrt((ui * ui + vi * vi) / (gamma * Rg * Ti));
+    return po * pow(1.0 + 0.5 * (gamma - 1.0) * (M * M), -gamma / (gamma - 1.0));
+}
+
+// u of the two west-most volumes of rows j = 2..ny-1 into uw[2(j-2)], [+1]
+__global__ __launch_bounds__(kT) void k_inflow_u(int nx, int ny, const double *u,
+                                                double *__restrict__ uw)
+{
+    const int j = blockIdx.x * kT + threadIdx.x + 2;
+    if (j <= ny - 1) {
+        const long w = at(1, j, nx);
+        uw[2 * (j - 2)] = u[w];
+        uw[2 * (j - 2) + 1] = u[w + 1];
+    }
+}
+
 // intflow_time_cycle_update, mod_intflow.f90:1106-1134: pina = pin, then
 // get_uin_vin_pin_Tin_Mw (:968-1006) and get_plin_and_p_fictitious
 // (:1010-1101).  Thread t: pina of j = t+1; row j = t+2; column i = t+2.
+// pnew[j-2]: the inflow pressure of row j (inflow_pressure).
 __global__ __launch_bounds__(kT) void k_time_cycle(int nx, int ny, double gamma, double Rg,
-                                                  double po, double T0, const double *x,
+                                                  double T0, const double *x,
                                                   const double *xp, const double *u,
-                                                  double *uin, double *vin, double *pin,
-                                                  double *Tin, double *Mw, double *plin,
-                                                  double *pina, double *p)
+                                                  const double *pnew, double *uin,
+                                                  double *vin, double *pin, double *Tin,
+                                                  double *Mw, double *plin, double *pina,
+                                                  double *p)
 {
     const int t = blockIdx.x * kT + threadIdx.x;
     if (t + 1 == 1 || t + 1 == ny) pina[t] = pin[t];  // rows the loops below leave
@@ -228,7 +259,7 @@ __global__ __launch_bounds__(kT) void k_time_cycle(int nx, int ny, double gamma,
         const double ui = (u[w] + u[w + 1]) / 2.0, vi = 0.0;
         const double Ti = T0 - (gamma - 1.0) / (2.0 * gamma * Rg) * (ui * ui + vi * vi);
         const double M = sqrt((ui * ui + vi * vi) / (gamma * Rg * Ti));
-        const double pi = po * pow(1.0 + 0.5 * (gamma - 1.0) * (M * M), -gamma / (gamma - 1.0));
+        const double pi = pnew[j - 2];
         pina[k] = pa;
         uin[k] = ui;
         vin[k] = vi;
@@ -468,16 +499,41 @@ extern "C" void m2d_intflow_time_cycle_update(const int *nx, const int *ny, cons
     const int NX = *nx, NY = *ny;
     const double g = *gamma, rg = *Rg, po = *P0, t0 = *T0;
     const long N = (long)NX * NY;
+    const int nr = NY - 2;  // inflow rows j = 2..ny-1
+    // host copies of the west u values and the inflow pressures; they stay
+    // alive until run() has synchronised the stream
+    std::vector<double> hu(2 * (size_t)std::max(nr, 1)), hp((size_t)std::max(nr, 1));
+    int inner = M2D_OK;
     const int rc = run(NX, NY,
                        {{x, N, false}, {xp, N, false}, {u, N, false}, {uin, NY, true},
                         {vin, NY, true}, {pin, NY, true}, {Tin, NY, true}, {Mw, NY, true},
                         {plin, NY, true}, {pina, NY, true}, {p, N, true}},
-                       0, false, [&](std::vector<double *> &d, hipStream_t s, double *, dim3 gr) {
+                       3 * (long)std::max(nr, 1), false,
+                       [&](std::vector<double *> &d, hipStream_t s, double *scr, dim3 gr) {
+                           // the pressures need the C library's pow: one read of 2(ny-2)
+                           // values, the pressures computed here, then written back
+                           double *duw = scr, *dp = scr + 2 * std::max(nr, 1);
+                           hipLaunchKernelGGL(k_inflow_u, dim3((unsigned)((nr + kT - 1) / kT)),
+                                              dim3(kT), 0, s, NX, NY, d[2], duw);
+                           if (hipGetLastError() != hipSuccess ||
+                               hipMemcpyAsync(hu.data(), duw, sizeof(double) * 2 * nr,
+                                              hipMemcpyDeviceToHost, s) != hipSuccess ||
+                               hipStreamSynchronize(s) != hipSuccess) {
+                               inner = M2D_ERR_HIP;
+                               return;
+                           }
+                           for (int q = 0; q < nr; ++q)
+                               hp[q] = inflow_pressure(g, rg, po, t0, hu[2 * q], hu[2 * q + 1]);
+                           if (hipMemcpyAsync(dp, hp.data(), sizeof(double) * nr,
+                                              hipMemcpyHostToDevice, s) != hipSuccess) {
+                               inner = M2D_ERR_HIP;
+                               return;
+                           }
                            hipLaunchKernelGGL(k_time_cycle, gr, dim3(kT), 0, s, NX, NY, g, rg,
-                                              po, t0, d[0], d[1], d[2], d[3], d[4], d[5], d[6],
+                                              t0, d[0], d[1], d[2], dp, d[3], d[4], d[5], d[6],
                                               d[7], d[8], d[9], d[10]);
                        });
-    m2d_set_last_status_code(rc);
+    m2d_set_last_status_code(rc ? rc : inner);
 }
 
 // norm_l1_5d, mod_solvers.f90:91-125 (norm += sum |A var - b|, a: 5 N)

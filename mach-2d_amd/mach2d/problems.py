@@ -454,13 +454,30 @@ def nozzle_grid(nx, ny):
     return Grid(nx, ny, x, y, 1.0)
 
 
-def time_step_fields(nx, ny, seed, params, coord=1, u_in=10.0):
+# The internal-flow time-step workload: the 258x130 nozzle, Euler, inlet
+# Mach 0.2 (the initial velocity u_in A_in / A(x) is then supersonic past the
+# throat, M 1.2), first time step 5e-6 s (the reference's own step diverges
+# from this state within ten steps at 2e-5 s; at 5e-6 s it stays finite for
+# 200 steps with MSI and with TDMA for u and v), 200 time steps.
+CFG2 = dict(nx=258, ny=130, mach_in=0.2, dt0=5e-6, steps=200)
+
+
+def inlet_speed(mach, params):
+    """Velocity of inlet Mach `mach` at the initial temperature T0 - 2 of
+    time_step_fields."""
+    return mach * math.sqrt(params["gamma"] * params["Rg"] * (params["T0"] - 2.0))
+
+
+def time_step_fields(nx, ny, seed, params, coord=1, u_in=10.0, mach_in=None):
     """Seeded inputs of one internal-flow time step (m2d_intflow_time_step)
-    on the nozzle, `params` giving T0, P0 and Rg: a slow flow at rest
+    on the nozzle, `params` giving T0, P0, Rg and gamma: a flow at rest
     temperature and pressure (velocity u_in A_in / A(x) plus 1 % noise, the
-    matching face velocities and densities), the radii of get_radius, the
-    9-diagonal systems as the reference allocates them (a = 0, unit
-    diagonal), the 5-diagonal one likewise, and the inflow vectors."""
+    matching face velocities and densities; mach_in, when given, sets u_in
+    to that inlet Mach number), the radii of get_radius, the 9-diagonal
+    systems as the reference allocates them (a = 0, unit diagonal), the
+    5-diagonal one likewise, and the inflow vectors."""
+    if mach_in is not None:
+        u_in = inlet_speed(mach_in, params)
     g = nozzle_grid(nx, ny)
     m = metrics(g)
     f = {k: np.ascontiguousarray(v.reshape(-1)) for k, v in m.items()}

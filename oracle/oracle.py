@@ -460,6 +460,44 @@ def radius(coord, y, yp, nx, ny):
     return tuple(out)
 
 
+def _ref_grid(name):
+    f = getattr(ref(), "_QMmod_gridP" + name)
+    f.restype = None
+    return f
+
+
+def ref_centroids(x, y, nx, ny, opt=1):
+    """The reference's own (module-private) get_real_centroids_xy
+    (mod_grid.f90:237-300), called through its module symbol."""
+    f = _ref_grid("get_real_centroids_xy")
+    f.argtypes = [_ip, _ip, _ip, _dp, _dp, _dp, _dp]
+    n = nx * ny
+    xp, yp = np.zeros(n), np.zeros(n)
+    f(C.byref(C.c_int(opt)), C.byref(C.c_int(nx)), C.byref(C.c_int(ny)), _c(x), _c(y), xp, yp)
+    return xp, yp
+
+
+def ref_grid_metrics(x, y, xp, yp, nx, ny):
+    """The reference's own get_metrics (mod_grid.f90:304-558) -> {name: N-vector}."""
+    f = _ref_grid("get_metrics")
+    f.argtypes = [_ip, _ip] + [_dp] * 19
+    n = nx * ny
+    out = {k: np.zeros(n) for k in METRICS}
+    f(C.byref(C.c_int(nx)), C.byref(C.c_int(ny)), _c(x), _c(y), _c(xp), _c(yp),
+      *[out[k] for k in METRICS])
+    return out
+
+
+def ref_radius(coord, y, yp, nx, ny):
+    """The reference's own get_radius (mod_grid.f90:562-723) -> (radius, re, rn, rp)."""
+    f = _ref_grid("get_radius")
+    f.argtypes = [_ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
+    n = nx * ny
+    out = [np.zeros(n) for _ in range(4)]
+    f(C.byref(C.c_int(coord)), C.byref(C.c_int(nx)), C.byref(C.c_int(ny)), _c(y), _c(yp), *out)
+    return tuple(out)
+
+
 def norm_l1_5d(nx, ny, var, b, a, norm=0.0, ref_=False):
     """norm_l1_5d (mod_solvers.f90:91-125): norm + sum |A var - b| over real
     volumes, a (5, N)."""

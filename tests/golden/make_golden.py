@@ -30,6 +30,10 @@ Fixtures (float64, diagonal-major arrays (k, nx*ny)):
                   and a 12x8 case with faces on the axis (r = 0) for the
                   axis branches of get_u_source; outputs start from seeded
                   values so untouched points are checked too
+  grid.npz        get_real_centroids_xy (opt 1, 2), get_metrics and
+                  get_radius (coord 0, 1) of mod_grid.f90 (module-private,
+                  called through their module symbols) on ogive, nozzle,
+                  perturbed skewed and strongly stretched grids
 """
 from __future__ import annotations
 
@@ -216,6 +220,51 @@ def coef():
     save("coef.npz", **arrs)
 
 
+GRID_CASES = [("ogive", 34, 18), ("ogive", 3, 3), ("ogive", 4, 9), ("nozzle", 22, 12),
+              ("perturbed", 17, 12), ("stretched", 41, 23)]
+
+
+def grid_xy(kind, nx, ny):
+    """Corner coordinates (north-east corner of each volume, the reference's
+    x, y) of the fixture grids: the g01 ogive, the nozzle, a seeded randomly
+    perturbed skewed grid, a strongly wall-stretched ogive."""
+    if kind in ("ogive", "stretched"):
+        g = P.ogive_grid(nx, ny, 1.05 if kind == "ogive" else 1.3)
+    elif kind == "nozzle":
+        g = P.nozzle_grid(nx, ny)
+    else:
+        rng = np.random.default_rng(nx * ny)
+        x = np.zeros((ny, nx))
+        y = np.zeros((ny, nx))
+        ii, jj = np.meshgrid(np.arange(nx - 1, dtype=float), np.arange(ny - 1, dtype=float))
+        x[: ny - 1, : nx - 1] = 1.0 * ii + 0.15 * jj + 0.2 * rng.uniform(-1, 1, ii.shape)
+        y[: ny - 1, : nx - 1] = -0.1 * ii + 0.8 * jj + 0.2 * rng.uniform(-1, 1, ii.shape)
+        return x.reshape(-1), y.reshape(-1)
+    return g.x.reshape(-1).copy(), g.y.reshape(-1).copy()
+
+
+def grid():
+    """get_real_centroids_xy (both options), get_metrics (from the area-
+    weighted centroids) and get_radius (both coordinate systems) of
+    mod_grid.f90, the reference's own module procedures."""
+    arrs = {}
+    for kind, nx, ny in GRID_CASES:
+        tag = "%s_%dx%d" % (kind, nx, ny)
+        x, y = grid_xy(kind, nx, ny)
+        arrs[tag + "_x"], arrs[tag + "_y"] = x, y
+        for opt in (1, 2):
+            xp, yp = O.ref_centroids(x, y, nx, ny, opt)
+            arrs["%s_xp%d" % (tag, opt)], arrs["%s_yp%d" % (tag, opt)] = xp, yp
+        m = O.ref_grid_metrics(x, y, xp, yp, nx, ny)
+        for k, v in m.items():
+            arrs["%s_m_%s" % (tag, k)] = v
+        for coord in (0, 1):
+            for k, v in zip(("radius", "re", "rn", "rp"), O.ref_radius(coord, y, yp, nx, ny)):
+                arrs["%s_r%d_%s" % (tag, coord, k)] = v
+    arrs["cases"] = np.array(["%s_%dx%d" % c for c in GRID_CASES])
+    save("grid.npz", **arrs)
+
+
 if __name__ == "__main__":
     O.build()
     if not O.ref_available():
@@ -227,3 +276,4 @@ if __name__ == "__main__":
     embed5()
     solver_msi9d()
     coef()
+    grid()

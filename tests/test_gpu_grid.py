@@ -1,10 +1,12 @@
 """GPU parity of the grid centroids, metrics and radii
-(src/grid/mod_grid.f90:237-723) against the oracle, bit for bit, with host
-and device arrays.  The oracle is pinned by the mathematical checks of
-tests/test_oracle_golden.py (the reference module does not build here)."""
+(src/grid/mod_grid.f90:237-723), bit for bit, with host and device arrays:
+against the fixture the reference's own module procedures wrote
+(tests/golden/grid.npz) and against the oracle, which equals the compiled
+reference bit for bit (tests/test_oracle_golden.py)."""
 import numpy as np
 import pytest
 
+from conftest import load_golden
 from mach2d import problems as P
 from test_oracle_golden import _perturbed_grid
 
@@ -69,3 +71,32 @@ def test_grid_rejects_small_and_mixed(gpu):
     _lib.load().m2d_get_radius(C.byref(C.c_int(1)), i4, i4, C.c_void_p(t.data_ptr()),
                                C.c_void_p(y.ctypes.data), *[C.c_void_p(o.data_ptr()) for o in outs])
     assert _lib.last_status().status == M2D_ERR_ARG
+
+
+def test_grid_kernels_vs_reference_fixtures(gpu):
+    """k_centroids, k_metrics and k_radius against the outputs of the
+    reference's own get_real_centroids_xy, get_metrics and get_radius."""
+    import torch
+
+    from mach2d import grid
+    g = load_golden("grid.npz")
+    for tag in [str(c) for c in g["cases"]]:
+        nx, ny = (int(v) for v in tag.rsplit("_", 1)[1].split("x"))
+        x, y = g[tag + "_x"], g[tag + "_y"]
+        for opt in (1, 2):
+            xp, yp = grid.centroids(x, y, nx, ny, opt)
+            assert np.array_equal(xp, g["%s_xp%d" % (tag, opt)]), (tag, opt)
+            assert np.array_equal(yp, g["%s_yp%d" % (tag, opt)]), (tag, opt)
+        m = grid.metrics(x, y, xp, yp, nx, ny)
+        for k in grid.METRICS:
+            assert np.array_equal(m[k], g["%s_m_%s" % (tag, k)]), (tag, k)
+        for coord in (0, 1):
+            for k, v in zip(("radius", "re", "rn", "rp"), grid.radius(coord, y, yp, nx, ny)):
+                assert np.array_equal(v, g["%s_r%d_%s" % (tag, coord, k)]), (tag, coord, k)
+        dx, dy = (torch.tensor(v, device="cuda") for v in (x, y))
+        d = grid.grid_create(dx, dy, nx, ny, kcm=2, coord=1)
+        torch.cuda.synchronize()
+        for k in grid.METRICS:
+            assert np.array_equal(d[k].cpu().numpy(), g["%s_m_%s" % (tag, k)]), (tag, k)
+        for k in ("radius", "re", "rn", "rp"):
+            assert np.array_equal(d[k].cpu().numpy(), g["%s_r1_%s" % (tag, k)]), (tag, k)

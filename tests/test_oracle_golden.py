@@ -485,23 +485,26 @@ def test_intflow_step_routines_invariants(orc):
     np.testing.assert_allclose(uw, 3.0 + 0.5 * (-xp[J, 1]), rtol=1e-12)
 
 
-@pytest.mark.parametrize("modvis,solver9d,nx,ny", [(0, 0, 22, 12), (0, 1, 22, 12),
-                                                   (1, 0, 18, 10)])
-def test_time_step_oracle_vs_reference_composition(orc, modvis, solver9d, nx, ny):
+@pytest.mark.parametrize("modvis,solver9d,nx,ny,mach,dt0",
+                         [(0, 0, 22, 12, None, 2e-5), (0, 1, 22, 12, None, 2e-5),
+                          (1, 0, 18, 10, None, 2e-5), (0, 0, 66, 34, 0.2, 5e-6),
+                          (0, 1, 66, 34, 0.2, 5e-6)])
+def test_time_step_oracle_vs_reference_composition(orc, modvis, solver9d, nx, ny, mach, dt0):
     """One internal-flow time step composed from the oracle's routines equals
     the same step composed from the compiled reference's own routines, bit
-    for bit: every field, the next dt, the norms, rmass and itp."""
+    for bit: every field, the next dt, the norms, rmass and itp (mach: the
+    nozzle state at that inlet Mach number)."""
     import os
 
     import time_step_oracle as TS
     if not orc.ref_available():
         pytest.skip("reference library not built")
-    f = TS.step_fields(nx, ny, seed=nx + modvis)
+    f = TS.step_fields(nx, ny, seed=nx + modvis, mach_in=mach)
     prm = dict(TS.PARAMS, modvis=modvis, solver9d=solver9d)
     pf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
                       "parametersNozzleEulerPTC.txt")
-    a, dta, ra = TS.step(orc, f, nx, ny, prm, 2e-5)
-    b, dtb, rb = TS.step(orc, f, nx, ny, prm, 2e-5, ref=True, paramfile=pf)
+    a, dta, ra = TS.step(orc, f, nx, ny, prm, dt0)
+    b, dtb, rb = TS.step(orc, f, nx, ny, prm, dt0, ref=True, paramfile=pf)
     assert dta == dtb and ra == rb, (dta, dtb, ra, rb)
     for k in a:
         assert np.array_equal(a[k], b[k]), k
@@ -523,3 +526,50 @@ def test_problem_radii_match_oracle_radius(orc):
     assert np.array_equal(r["re"].reshape(-1), re)
     assert np.array_equal(r["rn"].reshape(-1), rn)
     assert np.array_equal(r["rp"].reshape(-1), rp)
+
+
+def _grid_cases():
+    g = load_golden("grid.npz")
+    for tag in [str(c) for c in g["cases"]]:
+        nx, ny = (int(v) for v in tag.rsplit("_", 1)[1].split("x"))
+        yield g, tag, nx, ny
+
+
+def test_grid_oracle_vs_reference_fixtures(orc):
+    """get_real_centroids_xy, get_metrics and get_radius restated in
+    oracle/grid.c equal the reference's own module procedures bit for bit
+    (fixture written by the compiled mod_grid.f90) on ogive, nozzle,
+    perturbed skewed and strongly stretched grids."""
+    for g, tag, nx, ny in _grid_cases():
+        x, y = g[tag + "_x"], g[tag + "_y"]
+        for opt in (1, 2):
+            xp, yp = orc.centroids(x, y, nx, ny, opt)
+            assert np.array_equal(xp, g["%s_xp%d" % (tag, opt)]), (tag, opt)
+            assert np.array_equal(yp, g["%s_yp%d" % (tag, opt)]), (tag, opt)
+        m = orc.grid_metrics(x, y, xp, yp, nx, ny)
+        for k in orc.METRICS:
+            assert np.array_equal(m[k], g["%s_m_%s" % (tag, k)]), (tag, k)
+        for coord in (0, 1):
+            for k, v in zip(("radius", "re", "rn", "rp"), orc.radius(coord, y, yp, nx, ny)):
+                assert np.array_equal(v, g["%s_r%d_%s" % (tag, coord, k)]), (tag, coord, k)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_grid_oracle_vs_reference_live(orc, seed):
+    """The same three routines on seeded random grids (sizes and
+    perturbations drawn per seed), against the compiled reference."""
+    if not orc.ref_available():
+        pytest.skip("reference library not built")
+    rng = np.random.default_rng(seed)
+    nx, ny = int(rng.integers(3, 60)), int(rng.integers(3, 40))
+    x, y = _perturbed_grid(nx, ny, seed, amp=float(rng.uniform(0.0, 0.3)))
+    for opt in (1, 2):
+        a, b = orc.centroids(x, y, nx, ny, opt), orc.ref_centroids(x, y, nx, ny, opt)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), opt
+    xp, yp = b
+    a, b = orc.grid_metrics(x, y, xp, yp, nx, ny), orc.ref_grid_metrics(x, y, xp, yp, nx, ny)
+    for k in orc.METRICS:
+        assert np.array_equal(a[k], b[k]), k
+    for coord in (0, 1):
+        for u, v in zip(orc.radius(coord, y, yp, nx, ny), orc.ref_radius(coord, y, yp, nx, ny)):
+            assert np.array_equal(u, v), coord

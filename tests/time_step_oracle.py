@@ -16,8 +16,8 @@ PARAMS = dict(modvis=0, coord=1, solver9d=0, itmmax=1, itpmax=20, nitm_u=5, nitm
 nozzle_grid = P.nozzle_grid
 
 
-def step_fields(nx, ny, seed, coord=1, u_in=10.0):
-    return P.time_step_fields(nx, ny, seed, PARAMS, coord=coord, u_in=u_in)
+def step_fields(nx, ny, seed, coord=1, u_in=10.0, mach_in=None):
+    return P.time_step_fields(nx, ny, seed, PARAMS, coord=coord, u_in=u_in, mach_in=mach_in)
 
 
 def step(O, f, nx, ny, prm, dt, ref=False, paramfile=None):
