@@ -652,9 +652,18 @@ int m2d_unet_pack_input(const double *a, const double *b, int nx, int ny, int h,
 int m2d_unet_unpack_output(const float *out, int os, int nx, int ny, int w, double s,
                            double *var, void *stream);
 
-/* TDMA on device pointers (tdma2d9 semantics). */
+/* tdma2d9(nx,ny,nitm,a,b,x), src/solvers/tdma2d9.f90:38-322, on device
+ * pointers (same results bit for bit; nx, ny <= 3072).  Runs on `stream`
+ * and returns when x is final. */
 int m2d_tdma2d9_device(int nx, int ny, int nitm, const double *d_a,
                        const double *d_b, double *d_x, void *stream);
+/* tdma2d9 on `count` independent systems of one grid size (d_a[q], d_b[q],
+ * d_x[q] device pointers) in the same launches: the u and v systems of a
+ * SIMPLEC iteration, which the reference solves one after the other with
+ * the same solver9d (src/mach2d_main.f90:226,233).  Each x[q] is what
+ * m2d_tdma2d9_device gives for that system alone. */
+int m2d_tdma2d9_device_many(int count, int nx, int ny, int nitm, const double *const *d_a,
+                            const double *const *d_b, double *const *d_x, void *stream);
 
 /* Per-kernel device time of this handle's launches (HIP events on the
  * handle's stream around each launch; launches after the stopping rule

@@ -1184,6 +1184,8 @@ namespace m2d {
 int norm_l1_device(const double *a, const double *b, const double *var, int nx, int ny,
                    double *d_work, double *h_out, hipStream_t s);
 int norm_l1_work_size(int nx, int ny);
+int tdma_device_many(int count, int nx, int ny, int nitm, const double *const *a,
+                     const double *const *b, double *const *x, hipStream_t s);
 }  // namespace m2d
 
 // g = 1 / (Rg T) (mach2d_main.f90:153)
@@ -1344,13 +1346,25 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
                                          f->de, f->dn);
         if (!ok()) break;
         stage = 5;
-        rc = solve9(s9, P, f->au, f->bu, f->u, nx, ny, st);
-        norm9_relative(f->au, f->bu, f->u);
-        if (rc) break;
-        stage = 6;
-        rc = solve9(s9, P, f->av, f->bv, f->v, nx, ny, st);
-        norm9_relative(f->av, f->bv, f->v);
-        if (rc) break;
+        if (P->solver9d == 1) {
+            // TDMA: u and v are independent systems (both assembled above),
+            // solved in the same launches; each result is its solo one
+            const double *as[2] = {f->au, f->av}, *bs[2] = {f->bu, f->bv};
+            double *xs[2] = {f->u, f->v};
+            rc = tdma_device_many(2, nx, ny, P->nitm_u, as, bs, xs, st);
+            if (rc) break;
+            norm9_relative(f->au, f->bu, f->u);
+            norm9_relative(f->av, f->bv, f->v);
+            if (rc) break;
+        } else {
+            rc = solve9(s9, P, f->au, f->bu, f->u, nx, ny, st);
+            norm9_relative(f->au, f->bu, f->u);
+            if (rc) break;
+            stage = 6;
+            rc = solve9(s9, P, f->av, f->bv, f->v, nx, ny, st);
+            norm9_relative(f->av, f->bv, f->v);
+            if (rc) break;
+        }
         stage = 7;
         m2d_get_velocities_at_internal_faces(&nx, &ny, &dt, f->rp, f->re, f->rn, f->xe, f->ye,
                                              f->xk, f->yk, f->xen, f->yen, f->xke, f->yke,

@@ -5,254 +5,628 @@
 // One TDMA iteration is two line-Jacobi passes.  x-pass: every grid row is a
 // tridiagonal system in its W/E couplings with the other six neighbours
 // taken from x as it was before the pass; y-pass: every column, S/N
-// implicit, explicit neighbours from the x-pass result.  All forward
-// eliminations of a pass read only the pass's input, so each pass reads one
-// buffer and writes the other (x -> tmp -> x) and needs no grid-wide barrier
-// between elimination and back substitution.
+// implicit, explicit neighbours from the x-pass result.  The Thomas
+// recurrences of a line,
 //
-// One lane per line runs the Thomas recurrences in the reference's order
-// (bit-identical results).  y-lines: lane i walks column i, so a wave reads
-// 64 consecutive points per step (coalesced).  x-lines: a workgroup owns 64
-// rows and stages 64x64 tiles of the nine diagonals, b and the three input
-// rows through LDS with coalesced loads, then each lane walks its row in LDS.
-// P and Q spill to global memory (they are re-read once by the back
-// substitution, from L2).
+//     den(t) = a_P + a_lo P(t-1),  P(t) = -a_up / den(t)      (t >= 1)
+//     Q(t)   = (D(t) - a_lo Q(t-1)) / den(t),  x(t) = P(t) x(t+1) + Q(t)
+//
+// (den(0) = a_P, Q(0) = D(0) / a_P), keep the reference's operation order
+// (bit-identical results), but the work is split by what it depends on:
+//
+//   * P and den depend on A only: one chain per line, once per call
+//     (k_tdma_pre), for all nitm iterations, with 1/den, so that the
+//     quotient of Q is taken exactly through the reciprocal (two Markstein
+//     corrections, div_by_rcp) instead of a full division on the chain;
+//   * D, the right-hand side with the explicit neighbours, depends on x but
+//     not on the recurrence: one thread per point (k_tdma_d);
+//   * what is left per iteration is the chain of Q and the back
+//     substitution (k_tdma_line): one lane per line, a subtraction, a
+//     multiplication and a division per step forward, a multiplication and
+//     an addition back.
+//
+// A chain runs on lane l < R of the first wave of its workgroup, one line
+// per lane.  Its operands stream through an LDS ring: the wave itself
+// issues direct-to-LDS loads (no registers) three chunks ahead and waits on
+// its vector-memory counter for the chunk it needs.  Its results (Q, then x
+// over it; den and P in k_tdma_pre) stay in an LDS tile that every thread
+// of the workgroup stores once the chain has ended, so the chain issues no
+// global store (a store ahead of a load in the in-order vector-memory
+// counter would hold up the wait for that load).  The operands are read in
+// a "pass layout" where the lanes' lines are adjacent in memory: element
+// (t, l) of line l at step t sits at t*LP + l (LP = the pass's lines padded
+// to a multiple of 64), written once per call by k_tdma_setup (the
+// coefficients) and per iteration by k_tdma_d (D).  Measured on one
+// MI355X at 258x130 (u and v together): an x-pass chain step (forward and
+// back) costs ~110 ns and a y-pass step ~150 ns, against ~25 ns of
+// dependent FP64 arithmetic (6 cycles per dependent operation): the LDS
+// operand traffic and the ring's load issue take the rest.  Several
+// systems of one grid size (u and v of a SIMPLEC iteration) run in the
+// same launches.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "m2d_internal.h"
 #include "msi_stencil.h"
 
 namespace m2d {
 
-constexpr int kTile = 64;
-// threads of an x-pass workgroup: all stage the tiles, the first kTile run
-// the rows
-constexpr int kLoad = 4 * kTile;
+// LDS of a chain workgroup, in doubles (144 KB of the 160 KB of a gfx950 CU)
+constexpr int kTdmaLds = 18432;
+// systems per launch (kernel-argument struct)
+constexpr int kTdmaMaxSys = 8;
+// Operand ring of a chain: kTdmaSlots chunks (kTdmaSlots - 1 in flight), a
+// chunk being kTdmaChunk doubles of each streamed array, i.e. 512/R steps
+// of the workgroup's R lines, loaded by kTdmaNI 16-byte direct-to-LDS loads
+// per array (64 lanes x 2 doubles each).
+constexpr int kTdmaSlots = 4;
+constexpr int kTdmaNI = 4;
+constexpr int kTdmaChunk = 128 * kTdmaNI;
+// longest line (grid dimension) the chain kernels take
+constexpr int kTdmaMaxLen = 2048;
 
-// x-pass: rows j = blockIdx.x*64 + lane.  xin is the pass input, xout its
-// output (P, Q scratch of N each).  Tiles of 64 rows x 16 columns are staged
-// through LDS (97 KB) with lanes mapped to (row, column) pairs so that every
-// wave-wide load reads runs of 16 consecutive points.
-constexpr int kTX = 16;
+// Lines of a pass padded to a multiple of 64 (the pass layout's row pitch).
+__host__ __device__ inline long tdma_lp(int lines) { return ((long)lines + 63) / 64 * 64; }
 
-template <typename T>
-__global__ __launch_bounds__(kLoad) void k_tdma_x(const T *__restrict__ a, const T *__restrict__ b,
-                                                  const T *__restrict__ xin, T *__restrict__ xout,
-                                                  T *__restrict__ Pw, T *__restrict__ Qw, int nx,
-                                                  int ny)
+// Lanes (lines) of a chain workgroup: the largest R in 64, 32, ..., 2 whose
+// `tiles` LDS tiles fit beside a ring of `na` arrays; 0 if none does.  A
+// tile holds R+1 columns of tdma_rows(len, R) rows.  With R = 32 it has
+// 64-R extra rows, and idle lane q >= R writes the padding column of row
+// t + q - R, its own address (the idle lanes' writes do not collide, and
+// need no branch); with R < 32 the idle lanes' writes are masked off.
+__host__ __device__ constexpr int tdma_rows(int len, int R) { return R >= 32 ? len + 64 - R : len; }
+// the tile writes of lanes beyond R are masked off (see tdma_rows)
+template <int R>
+__device__ __forceinline__ bool tdma_writes(int lane)
 {
-    const long nn = (long)nx * ny;
-    const int lane = threadIdx.x;
-    const int j0 = blockIdx.x * kTile;
-    const int rows = min(kTile, ny - j0);
-    __shared__ T ta[9][kTile][kTX + 1];
-    __shared__ T tb[kTile][kTX + 1];
-    __shared__ T tx[kTile + 2][kTX + 3];  // rows j0-1 .. j0+64, columns c0-1 .. c0+16
+    return R >= 32 || lane < R;
+}
+__host__ __device__ inline int tdma_lanes(int len, int tiles, int na)
+{
+    const int budget = kTdmaLds - kTdmaSlots * na * kTdmaChunk;
+    int R = 64;
+    while (R > 2 && (R + 1) * tiles * tdma_rows(len, R) > budget) R >>= 1;
+    return (R + 1) * tiles * tdma_rows(len, R) <= budget ? R : 0;
+}
+constexpr int kTdmaPreArrays = 3, kTdmaPreTiles = 2;    // k_tdma_pre
+constexpr int kTdmaLineArrays = 4, kTdmaLineTiles = 1;  // k_tdma_line
 
-    const int j = j0 + lane;
-    const bool row_ok = lane < rows;
-    T Pm = 0, Qm = 0;  // P, Q of the previous column in this row
-    for (int c0 = 0; c0 < nx; c0 += kTX) {
-        const int cols = min(kTX, nx - c0);
-        __syncthreads();
-        for (int idx = lane; idx < rows * cols; idx += kLoad) {
-            const int rr = idx / cols, cc = idx - rr * cols;
-            const long p = (long)(j0 + rr) * nx + c0 + cc;
+struct TdmaSys {
+    const double *a, *b;
+    double *x;
+    double *w;  // tdma_scratch(nx, ny) doubles
+};
+struct TdmaBatch {
+    TdmaSys s[kTdmaMaxSys];
+    int count, nx, ny;
+};
+
+// Per-system scratch: for each pass, seven arrays in the pass layout
+// (element (t, l) of line l at step t at t*LP + l; x-pass lines are rows,
+// LP = tdma_lp(ny), steps the nx columns; y-pass lines are columns, LP =
+// tdma_lp(nx), steps the ny rows): the coefficients a_lo, a_P, a_up (x:
+// a_W, a_P, a_E; y: a_S, a_P, a_N), den, 1/den, P and D; padding lines hold
+// a_P = 1, a_lo = a_up = D = 0.  Then the x-pass result (reference layout).
+enum TdmaArr { kTLo = 0, kTMp, kTUp, kTDen, kTRd, kTP, kTD, kTArrays };
+
+__host__ __device__ inline long tdma_pass_size(int nx, int ny, bool x)
+{
+    return x ? tdma_lp(ny) * nx : tdma_lp(nx) * ny;
+}
+__host__ __device__ inline long tdma_scratch(int nx, int ny)
+{
+    const long N = (long)nx * ny;
+    return kTArrays * (tdma_pass_size(nx, ny, true) + tdma_pass_size(nx, ny, false)) +
+           (N + 1) / 2 * 2;
+}
+__device__ __forceinline__ double *tdma_arr(const TdmaBatch &B, const TdmaSys &S, bool x, int k)
+{
+    const long Sx = tdma_pass_size(B.nx, B.ny, true), Sy = tdma_pass_size(B.nx, B.ny, false);
+    return S.w + (x ? (long)k * Sx : kTArrays * Sx + (long)k * Sy);
+}
+__device__ __forceinline__ double *tdma_tmp(const TdmaBatch &B, const TdmaSys &S)
+{
+    return S.w + kTArrays * (tdma_pass_size(B.nx, B.ny, true) + tdma_pass_size(B.nx, B.ny, false));
+}
+
+// Coefficients of both passes in their pass layouts (padding lines
+// included, with their D), once per call.
+__global__ __launch_bounds__(256) void k_tdma_setup(TdmaBatch B)
+{
+    const int nx = B.nx, ny = B.ny;
+    const long N = (long)nx * ny;
+    const long Sx = tdma_pass_size(nx, ny, true), Sy = tdma_pass_size(nx, ny, false);
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const int q = (int)(e / (Sx + Sy));
+    if (q >= B.count) return;
+    const TdmaSys S = B.s[q];
+    long r = e - (long)q * (Sx + Sy);
+    const bool x = r < Sx;
+    if (!x) r -= Sx;
+    const long LP = x ? tdma_lp(ny) : tdma_lp(nx);
+    const int t = (int)(r / LP), l = (int)(r - (long)t * LP);
+    double lo = 0.0, mp = 1.0, up = 0.0;
+    if (l < (x ? ny : nx)) {
+        const long p = x ? (long)l * nx + t : (long)t * nx + l;
+        lo = S.a[(x ? kW : kS) * N + p];
+        mp = S.a[kP * N + p];
+        up = S.a[(x ? kE : kN) * N + p];
+    } else {
+        tdma_arr(B, S, x, kTD)[r] = 0.0;
+    }
+    tdma_arr(B, S, x, kTLo)[r] = lo;
+    tdma_arr(B, S, x, kTMp)[r] = mp;
+    tdma_arr(B, S, x, kTUp)[r] = up;
+}
+
+// D of the x-pass (tdma2d9.f90:79-173) at point (i, j), from the pass input x
+__device__ __forceinline__ double d_xpass(const double *__restrict__ a,
+                                          const double *__restrict__ b,
+                                          const double *__restrict__ x, long N, int nx, int ny,
+                                          int i, int j)
+{
+    const long p = (long)j * nx + i;
+    const double bp = b[p];
+    if (i == 0) {
+        if (j == 0) return bp - a[kN * N + p] * x[p + nx] - a[kNE * N + p] * x[p + nx + 1];
+        if (j == ny - 1) return bp - a[kS * N + p] * x[p - nx] - a[kSE * N + p] * x[p - nx + 1];
+        return bp - a[kS * N + p] * x[p - nx] - a[kSE * N + p] * x[p - nx + 1] -
+               a[kN * N + p] * x[p + nx] - a[kNE * N + p] * x[p + nx + 1];
+    }
+    if (i < nx - 1) {
+        if (j == 0)
+            return bp - a[kNW * N + p] * x[p + nx - 1] - a[kN * N + p] * x[p + nx] -
+                   a[kNE * N + p] * x[p + nx + 1];
+        if (j == ny - 1)
+            return bp - a[kSW * N + p] * x[p - nx - 1] - a[kS * N + p] * x[p - nx] -
+                   a[kSE * N + p] * x[p - nx + 1];
+        return bp - a[kSW * N + p] * x[p - nx - 1] - a[kS * N + p] * x[p - nx] -
+               a[kSE * N + p] * x[p - nx + 1] - a[kNW * N + p] * x[p + nx - 1] -
+               a[kN * N + p] * x[p + nx] - a[kNE * N + p] * x[p + nx + 1];
+    }
+    if (j == 0) return bp - a[kNW * N + p] * x[p + nx - 1] - a[kN * N + p] * x[p + nx];
+    if (j == ny - 1) return bp - a[kSW * N + p] * x[p - nx - 1] - a[kS * N + p] * x[p - nx];
+    return bp - a[kSW * N + p] * x[p - nx - 1] - a[kS * N + p] * x[p - nx] -
+           a[kNW * N + p] * x[p + nx - 1] - a[kN * N + p] * x[p + nx];
+}
+
+// D of the y-pass (tdma2d9.f90:200-296) at point (i, j), from the x-pass result x
+__device__ __forceinline__ double d_ypass(const double *__restrict__ a,
+                                          const double *__restrict__ b,
+                                          const double *__restrict__ x, long N, int nx, int ny,
+                                          int i, int j)
+{
+    const long p = (long)j * nx + i;
+    const double bp = b[p];
+    if (j == 0) {
+        if (i == 0) return bp - a[kE * N + p] * x[p + 1] - a[kNE * N + p] * x[p + nx + 1];
+        if (i == nx - 1) return bp - a[kW * N + p] * x[p - 1] - a[kNW * N + p] * x[p + nx - 1];
+        return bp - a[kW * N + p] * x[p - 1] - a[kNW * N + p] * x[p + nx - 1] -
+               a[kE * N + p] * x[p + 1] - a[kNE * N + p] * x[p + nx + 1];
+    }
+    if (j < ny - 1) {
+        if (i == 0)
+            return bp - a[kSE * N + p] * x[p - nx + 1] - a[kE * N + p] * x[p + 1] -
+                   a[kNE * N + p] * x[p + nx + 1];
+        if (i == nx - 1)
+            return bp - a[kSW * N + p] * x[p - nx - 1] - a[kW * N + p] * x[p - 1] -
+                   a[kNW * N + p] * x[p + nx - 1];
+        return bp - a[kSW * N + p] * x[p - nx - 1] - a[kW * N + p] * x[p - 1] -
+               a[kNW * N + p] * x[p + nx - 1] - a[kSE * N + p] * x[p - nx + 1] -
+               a[kE * N + p] * x[p + 1] - a[kNE * N + p] * x[p + nx + 1];
+    }
+    if (i == 0) return bp - a[kSE * N + p] * x[p - nx + 1] - a[kE * N + p] * x[p + 1];
+    if (i == nx - 1) return bp - a[kSW * N + p] * x[p - nx - 1] - a[kW * N + p] * x[p - 1];
+    return bp - a[kSW * N + p] * x[p - nx - 1] - a[kW * N + p] * x[p - 1] -
+           a[kSE * N + p] * x[p - nx + 1] - a[kE * N + p] * x[p + 1];
+}
+
+// D of one pass for every real point of every system, in the pass layout
+// (padding lines keep the zero of k_tdma_setup).  x-pass: input x (the
+// iteration's); y-pass: input the x-pass result.
+template <bool XPASS>
+__global__ __launch_bounds__(256) void k_tdma_d(TdmaBatch B)
+{
+    const int nx = B.nx, ny = B.ny;
+    const long N = (long)nx * ny;
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const int q = (int)(e / N);
+    if (q >= B.count) return;
+    const long p = e - (long)q * N;
+    const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
+    const TdmaSys S = B.s[q];
+    double *D = tdma_arr(B, S, XPASS, kTD);
+    if (XPASS)
+        D[(long)i * tdma_lp(ny) + j] = d_xpass(S.a, S.b, S.x, N, nx, ny, i, j);
+    else
+        D[(long)j * tdma_lp(nx) + i] = d_ypass(S.a, S.b, tdma_tmp(B, S), N, nx, ny, i, j);
+}
+
+typedef __attribute__((address_space(1))) const void tdma_gvoid;
+typedef __attribute__((address_space(3))) void tdma_lvoid;
+
+// s_waitcnt vmcnt(N) alone (expcnt and lgkmcnt left alone)
+template <int N>
+__device__ __forceinline__ void tdma_wait_vmem()
+{
+    static_assert(N >= 0 && N <= 63, "vmcnt is six bits");
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// One chunk of NA arrays into its ring slot, issued by the 64 lanes of the
+// chain wave: load ii of array a brings 128 doubles, lane q the two at flat
+// position f = 128 ii + 2q of the chunk, i.e. step s = f / R, lines
+// l0 + f % R and the next one (adjacent in the pass layout: 16 bytes,
+// aligned since LP, l0 and f % R are even).  step(s) maps the chunk's step
+// index to the line step (clamped into the line by the caller's mapping).
+template <int NA, int R, typename STEP>
+__device__ __forceinline__ void tdma_issue(double *slot, const double *const (&arr)[NA], long LP,
+                                           int l0, int lane, STEP step)
+{
 #pragma unroll
-            for (int k = 0; k < 9; ++k) ta[k][rr][cc] = a[k * nn + p];
-            tb[rr][cc] = b[p];
+    for (int ii = 0; ii < kTdmaNI; ++ii) {
+        const int f = 128 * ii + 2 * lane;
+        const long g = (long)step(f / R) * LP + l0 + (f % R);
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+            __builtin_amdgcn_global_load_lds((tdma_gvoid *)(arr[a] + g),
+                                             (tdma_lvoid *)(slot + a * kTdmaChunk + 128 * ii), 16,
+                                             0, 0);
+    }
+}
+
+// Block b of a chain launch over one pass -> (system, first line)
+struct TdmaBlock {
+    int q, l0, len;
+    long LP;
+};
+template <int R>
+__device__ __forceinline__ TdmaBlock tdma_block(const TdmaBatch &B, bool x, int b)
+{
+    TdmaBlock k;
+    k.len = x ? B.nx : B.ny;
+    k.LP = tdma_lp(x ? B.ny : B.nx);
+    const int per = (int)(k.LP / R);
+    k.q = b / per;
+    k.l0 = (b - k.q * per) * R;
+    return k;
+}
+
+// P, den and 1/den of every line of one pass (once per call).  The chain
+// den(t) = a_P + a_lo P(t-1) (den(0) = a_P), P(t) = -a_up / den(t) runs on
+// lane l < R of wave 0 with its operands streamed through the LDS ring; den
+// and P go to LDS tiles, and after the chain every thread of the
+// workgroup stores den, 1/den (computed there, off the chain) and P.
+// 1/den (a correctly rounded quotient) lets the iterations take Q's
+// quotient exactly through the reciprocal (div_by_rcp).
+template <bool XPASS, int R>
+__device__ __forceinline__ void tdma_pre_body(const TdmaBatch &B, int blk, double *lds)
+{
+    constexpr int NA = kTdmaPreArrays, RS = R + 1, K = kTdmaChunk / R;
+    const TdmaBlock k = tdma_block<R>(B, XPASS, blk);
+    if (k.q >= B.count) return;
+    const TdmaSys S = B.s[k.q];
+    const int len = k.len;
+    const long T = (long)RS * tdma_rows(len, R);
+    double *ring = lds, *tden = lds + kTdmaSlots * NA * kTdmaChunk, *tP = tden + T;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const int wl = lane < R ? lane : R + (lane - R) * RS;  // idle lanes (R >= 32)
+        const int rl = lane < R ? lane : 0;
+        const double *const arr[NA] = {tdma_arr(B, S, XPASS, kTLo), tdma_arr(B, S, XPASS, kTMp),
+                                       tdma_arr(B, S, XPASS, kTUp)};
+        const int nch = (len + K - 1) / K;
+        auto issue = [&](int c) {
+            tdma_issue<NA, R>(ring + (c % kTdmaSlots) * NA * kTdmaChunk, arr, k.LP, k.l0, lane,
+                              [&](int s) { return min(c * K + s, len - 1); });
+        };
+        for (int c = 0; c < kTdmaSlots - 1; ++c) issue(c);
+        double Pm = 0.0;
+        for (int c = 0; c < nch; ++c) {
+            issue(c + kTdmaSlots - 1);
+            tdma_wait_vmem<(kTdmaSlots - 1) * NA * kTdmaNI>();
+            const double *sl = ring + (c % kTdmaSlots) * NA * kTdmaChunk;
+            const int t0 = c * K, ns = min(K, len - t0);
+            for (int s = 0; s < ns; ++s) {
+                const double lo = sl[s * R + rl], mp = sl[kTdmaChunk + s * R + rl],
+                             up = sl[2 * kTdmaChunk + s * R + rl];
+                const int t = t0 + s;
+                const double den = t == 0 ? mp : mp + lo * Pm;
+                Pm = -up / den;
+                if (tdma_writes<R>(lane)) {
+                    tden[t * RS + wl] = den;
+                    tP[t * RS + wl] = Pm;
+                }
+            }
         }
-        const int xr = rows + 2, xc = cols + 2;
-        for (int idx = lane; idx < xr * xc; idx += kLoad) {
-            const int rr = idx / xc, cc = idx - rr * xc;
-            const int jj = j0 + rr - 1, col = c0 + cc - 1;
-            if (jj >= 0 && jj < ny && col >= 0 && col < nx) tx[rr][cc] = xin[(long)jj * nx + col];
-        }
-        __syncthreads();
-        if (row_ok) {
-            for (int ci = 0; ci < cols; ++ci) {
-                const int i = c0 + ci;
-                const T a1 = ta[kSW][lane][ci], a2 = ta[kS][lane][ci], a3 = ta[kSE][lane][ci];
-                const T a4 = ta[kW][lane][ci], a5 = ta[kP][lane][ci], a6 = ta[kE][lane][ci];
-                const T a7 = ta[kNW][lane][ci], a8 = ta[kN][lane][ci], a9 = ta[kNE][lane][ci];
-                const T bp = tb[lane][ci];
-                // neighbour x: tx[row - j0 + 1][col - c0 + 1]; entries outside the
-                // grid are never staged and never used by the branch taken
-                const T xs = tx[lane][ci + 1], xse = tx[lane][ci + 2], xsw = tx[lane][ci];
-                const T xn = tx[lane + 2][ci + 1], xne = tx[lane + 2][ci + 2],
-                        xnw = tx[lane + 2][ci];
-                T D;
-                if (i == 0) {
-                    if (j == 0)
-                        D = bp - a8 * xn - a9 * xne;
-                    else if (j == ny - 1)
-                        D = bp - a2 * xs - a3 * xse;
-                    else
-                        D = bp - a2 * xs - a3 * xse - a8 * xn - a9 * xne;
-                    Pm = -a6 / a5;
-                    Qm = D / a5;
-                } else {
-                    if (i < nx - 1) {
-                        if (j == 0)
-                            D = bp - a7 * xnw - a8 * xn - a9 * xne;
-                        else if (j == ny - 1)
-                            D = bp - a1 * xsw - a2 * xs - a3 * xse;
-                        else
-                            D = bp - a1 * xsw - a2 * xs - a3 * xse - a7 * xnw - a8 * xn -
-                                a9 * xne;
-                    } else {
-                        if (j == 0)
-                            D = bp - a7 * xnw - a8 * xn;
-                        else if (j == ny - 1)
-                            D = bp - a1 * xsw - a2 * xs;
-                        else
-                            D = bp - a1 * xsw - a2 * xs - a7 * xnw - a8 * xn;
+        tdma_wait_vmem<0>();
+    }
+    __syncthreads();
+    double *gden = tdma_arr(B, S, XPASS, kTDen), *grd = tdma_arr(B, S, XPASS, kTRd),
+           *gP = tdma_arr(B, S, XPASS, kTP);
+    for (int e = threadIdx.x; e < R * len; e += 256) {
+        const int t = e / R, lane = e % R;
+        const long g = (long)t * k.LP + k.l0 + lane;
+        const double den = tden[t * RS + lane];
+        gden[g] = den;
+        grd[g] = 1.0 / den;
+        gP[g] = tP[t * RS + lane];
+    }
+}
+
+// Both passes' P chains in one launch: blocks [0, nbx) the x-pass lines (RX
+// per block), the rest the y-pass lines (RY per block).
+template <int RX, int RY>
+__global__ __launch_bounds__(256) void k_tdma_pre(TdmaBatch B, int nbx)
+{
+    __shared__ double lds[kTdmaLds];
+    if ((int)blockIdx.x < nbx)
+        tdma_pre_body<true, RX>(B, blockIdx.x, lds);
+    else
+        tdma_pre_body<false, RY>(B, blockIdx.x - nbx, lds);
+}
+
+// Q chain and back substitution of every line of one pass (each
+// iteration); the result goes to the reference layout: x-pass into the
+// scratch's x-pass result, y-pass into the system's x.  The operands a_lo,
+// den, 1/den, D (forward) and P (backward) stream through the LDS ring; Q
+// and then x over it stay in an LDS tile, stored by every thread once the
+// chain has ended.
+template <bool XPASS, int R>
+__global__ __launch_bounds__(256) void k_tdma_line(TdmaBatch B)
+{
+    __shared__ double lds[kTdmaLds];
+    constexpr int NA = kTdmaLineArrays, RS = R + 1, K = kTdmaChunk / R;
+    const TdmaBlock k = tdma_block<R>(B, XPASS, blockIdx.x);
+    if (k.q >= B.count) return;
+    const TdmaSys S = B.s[k.q];
+    const int len = k.len;
+    double *ring = lds, *tile = lds + kTdmaSlots * NA * kTdmaChunk;
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const int wl = lane < R ? lane : R + (lane - R) * RS;  // idle lanes (R >= 32)
+        const int rl = lane < R ? lane : 0;
+        // forward: Q(0) = D(0) / den(0) (den(0) = a_P), then
+        // Q(t) = (D(t) - a_lo Q(t-1)) / den(t), each quotient exact through 1/den
+        {
+            const double *const arr[NA] = {tdma_arr(B, S, XPASS, kTLo),
+                                           tdma_arr(B, S, XPASS, kTDen),
+                                           tdma_arr(B, S, XPASS, kTRd), tdma_arr(B, S, XPASS, kTD)};
+            const int nch = (len + K - 1) / K;
+            auto issue = [&](int c) {
+                tdma_issue<NA, R>(ring + (c % kTdmaSlots) * NA * kTdmaChunk, arr, k.LP, k.l0,
+                                  lane, [&](int s) { return min(c * K + s, len - 1); });
+            };
+            for (int c = 0; c < kTdmaSlots - 1; ++c) issue(c);
+            double Qm = 0.0;
+            for (int c = 0; c < nch; ++c) {
+                issue(c + kTdmaSlots - 1);
+                tdma_wait_vmem<(kTdmaSlots - 1) * NA * kTdmaNI>();
+                const double *sl = ring + (c % kTdmaSlots) * NA * kTdmaChunk;
+                const double *slo = sl, *sden = sl + kTdmaChunk, *srd = sl + 2 * kTdmaChunk,
+                             *sD = sl + 3 * kTdmaChunk;
+                const int t0 = c * K;
+                if (c > 0 && t0 + K <= len) {  // full chunk past step 0: 8 steps at a time
+                    for (int s0 = 0; s0 < K; s0 += 8) {
+                        double lo[8], dn[8], rd[8], dv[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const int o = (s0 + u) * R + rl;
+                            lo[u] = slo[o];
+                            dn[u] = sden[o];
+                            rd[u] = srd[o];
+                            dv[u] = sD[o];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            Qm = div_by_rcp<double>(dv[u] - lo[u] * Qm, dn[u], rd[u]);
+                            if (tdma_writes<R>(lane)) tile[(t0 + s0 + u) * RS + wl] = Qm;
+                        }
                     }
-                    const T Pn = -a6 / (a5 + a4 * Pm);
-                    const T Qn = (D - a4 * Qm) / (a5 + a4 * Pm);
-                    Pm = Pn;
-                    Qm = Qn;
-                }
-                // column-major P, Q: the lanes (rows) of one column are
-                // adjacent, so these stores and the loads below coalesce
-                Pw[(long)i * ny + j] = Pm;
-                Qw[(long)i * ny + j] = Qm;
-            }
-        }
-    }
-    // back substitution: tdma2d9.f90:175-193 (the loads do not depend on the
-    // recurrence; unrolling lets them run ahead of it)
-    if (row_ok) {
-        const long rb = (long)j * nx;
-        T xv = Qw[(long)(nx - 1) * ny + j];
-        xout[rb + nx - 1] = xv;
-#pragma unroll 8
-        for (int i = nx - 2; i >= 0; --i) {
-            xv = Pw[(long)i * ny + j] * xv + Qw[(long)i * ny + j];
-            xout[rb + i] = xv;
-        }
-    }
-}
-
-// y-pass: column i = blockIdx.x*kTile + lane (tdma2d9.f90:200-316).  Chunks
-// of kRY rows of the 64 columns (coefficients, b and the x halo) are staged
-// through LDS by all four waves, so the recurrence of each column reads LDS
-// instead of waiting on HBM row by row.
-constexpr int kRY = 16;
-
-template <typename T>
-__global__ __launch_bounds__(kLoad) void k_tdma_y(const T *__restrict__ a, const T *__restrict__ b,
-                                                  const T *__restrict__ xin, T *__restrict__ xout,
-                                                  T *__restrict__ Pw, T *__restrict__ Qw, int nx,
-                                                  int ny)
-{
-    const long nn = (long)nx * ny;
-    const int lane = threadIdx.x;
-    const int i0 = blockIdx.x * kTile;
-    const int cols = min(kTile, nx - i0);
-    const int i = i0 + lane;
-    const bool col_ok = lane < cols;
-    const bool west = i == 0, east = i == nx - 1;
-    __shared__ T ta[9][kRY][kTile];
-    __shared__ T tb[kRY][kTile];
-    __shared__ T tx[kRY + 2][kTile + 2];  // rows r0-1 .. r0+kRY, columns i0-1 .. i0+64
-    T Pm = 0, Qm = 0;
-    for (int r0 = 0; r0 < ny; r0 += kRY) {
-        const int rows = min(kRY, ny - r0);
-        __syncthreads();
-        for (int idx = lane; idx < rows * cols; idx += kLoad) {
-            const int rr = idx / cols, cc = idx - rr * cols;
-            const long p = (long)(r0 + rr) * nx + i0 + cc;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) ta[k][rr][cc] = a[k * nn + p];
-            tb[rr][cc] = b[p];
-        }
-        const int xr = rows + 2, xc = cols + 2;
-        for (int idx = lane; idx < xr * xc; idx += kLoad) {
-            const int rr = idx / xc, cc = idx - rr * xc;
-            const int jj = r0 + rr - 1, col = i0 + cc - 1;
-            if (jj >= 0 && jj < ny && col >= 0 && col < nx) tx[rr][cc] = xin[(long)jj * nx + col];
-        }
-        __syncthreads();
-        if (!col_ok) continue;
-        for (int rj = 0; rj < rows; ++rj) {
-            const int j = r0 + rj;
-            const long p = (long)j * nx + i;
-            const T a1 = ta[kSW][rj][lane], a2 = ta[kS][rj][lane], a3 = ta[kSE][rj][lane];
-            const T a4 = ta[kW][rj][lane], a5 = ta[kP][rj][lane], a6 = ta[kE][rj][lane];
-            const T a7 = ta[kNW][rj][lane], a8 = ta[kN][rj][lane], a9 = ta[kNE][rj][lane];
-            const T bp = tb[rj][lane];
-            // neighbour x: tx[row - r0 + 1][col - i0 + 1]; entries outside the
-            // grid are never staged and never used by the branch taken
-            const T xw = west ? T(0) : tx[rj + 1][lane];
-            const T xe = east ? T(0) : tx[rj + 1][lane + 2];
-            const T xnw = (west || j == ny - 1) ? T(0) : tx[rj + 2][lane];
-            const T xne = (east || j == ny - 1) ? T(0) : tx[rj + 2][lane + 2];
-            const T xsw = (west || j == 0) ? T(0) : tx[rj][lane];
-            const T xse = (east || j == 0) ? T(0) : tx[rj][lane + 2];
-            T D;
-            if (j == 0) {
-                if (west)
-                    D = bp - a6 * xe - a9 * xne;
-                else if (east)
-                    D = bp - a4 * xw - a7 * xnw;
-                else
-                    D = bp - a4 * xw - a7 * xnw - a6 * xe - a9 * xne;
-                Pm = -a8 / a5;
-                Qm = D / a5;
-            } else {
-                if (j < ny - 1) {
-                    if (west)
-                        D = bp - a3 * xse - a6 * xe - a9 * xne;
-                    else if (east)
-                        D = bp - a1 * xsw - a4 * xw - a7 * xnw;
-                    else
-                        D = bp - a1 * xsw - a4 * xw - a7 * xnw - a3 * xse - a6 * xe - a9 * xne;
                 } else {
-                    if (west)
-                        D = bp - a3 * xse - a6 * xe;
-                    else if (east)
-                        D = bp - a1 * xsw - a4 * xw;
-                    else
-                        D = bp - a1 * xsw - a4 * xw - a3 * xse - a6 * xe;
+                    const int ns = min(K, len - t0);
+                    for (int s = 0; s < ns; ++s) {
+                        const int o = s * R + rl, t = t0 + s;
+                        Qm = t == 0 ? div_by_rcp<double>(sD[o], sden[o], srd[o])
+                                    : div_by_rcp<double>(sD[o] - slo[o] * Qm, sden[o], srd[o]);
+                        if (tdma_writes<R>(lane)) tile[t * RS + wl] = Qm;
+                    }
                 }
-                const T Pn = -a8 / (a5 + a2 * Pm);
-                const T Qn = (D - a2 * Qm) / (a5 + a2 * Pm);
-                Pm = Pn;
-                Qm = Qn;
             }
-            Pw[p] = Pm;
-            Qw[p] = Qm;
+            tdma_wait_vmem<0>();  // the ring is reused below
+        }
+        // back substitution: x(len-1) = Q(len-1) (already in the tile), then
+        // x(t) = P(t) x(t+1) + Q(t) (x-pass, tdma2d9.f90:175-193) or
+        // x(t) = x(t+1) P(t) + Q(t) (y-pass, :298-316), t = len-2 .. 0,
+        // P streamed in descending steps
+        if (len > 1) {
+            const double *const arr[1] = {tdma_arr(B, S, XPASS, kTP)};
+            const int nst = len - 1, nch = (nst + K - 1) / K;
+            auto issue = [&](int c) {
+                tdma_issue<1, R>(ring + (c % kTdmaSlots) * kTdmaChunk, arr, k.LP, k.l0, lane,
+                                 [&](int s) { return max(len - 2 - c * K - s, 0); });
+            };
+            for (int c = 0; c < kTdmaSlots - 1; ++c) issue(c);
+            double xv = tile[(len - 1) * RS + rl];
+            for (int c = 0; c < nch; ++c) {
+                issue(c + kTdmaSlots - 1);
+                tdma_wait_vmem<(kTdmaSlots - 1) * kTdmaNI>();
+                const double *sP = ring + (c % kTdmaSlots) * kTdmaChunk;
+                const int ns = min(K, nst - c * K);
+                int t = len - 2 - c * K;
+                if (ns == K) {
+                    for (int s0 = 0; s0 < K; s0 += 8) {
+                        double pv[8], qv[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            pv[u] = sP[(s0 + u) * R + rl];
+                            qv[u] = tile[(t - s0 - u) * RS + rl];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            xv = XPASS ? pv[u] * xv + qv[u] : xv * pv[u] + qv[u];
+                            if (tdma_writes<R>(lane)) tile[(t - s0 - u) * RS + wl] = xv;
+                        }
+                    }
+                } else {
+                    for (int s = 0; s < ns; ++s) {
+                        const double pv = sP[s * R + rl], qv = tile[(t - s) * RS + rl];
+                        xv = XPASS ? pv * xv + qv : xv * pv + qv;
+                        if (tdma_writes<R>(lane)) tile[(t - s) * RS + wl] = xv;
+                    }
+                }
+            }
+            tdma_wait_vmem<0>();
         }
     }
-    if (!col_ok) return;
-    // back substitution: tdma2d9.f90:298-316
-    long p = (long)(ny - 1) * nx + i;
-    T xv = Qw[p];
-    xout[p] = xv;
-#pragma unroll 8
-    for (int j = ny - 2; j >= 0; --j) {
-        p = (long)j * nx + i;
-        xv = xv * Pw[p] + Qw[p];
-        xout[p] = xv;
+    __syncthreads();
+    const int LS = XPASS ? B.ny : B.nx;
+    const int lanes = min(R, LS - k.l0);  // lines of the grid (the rest is padding)
+    if (lanes <= 0) return;
+    if (XPASS) {  // rows l0.. into the x-pass result, row by row
+        double *out = tdma_tmp(B, S);
+        for (int e = threadIdx.x; e < lanes * len; e += 256) {
+            const int lane = e / len, t = e - lane * len;
+            out[(long)(k.l0 + lane) * B.nx + t] = tile[t * RS + lane];
+        }
+    } else {  // columns l0.. into x
+        for (int e = threadIdx.x; e < lanes * len; e += 256) {
+            const int t = e / lanes, lane = e - t * lanes;
+            S.x[(long)t * B.nx + k.l0 + lane] = tile[t * RS + lane];
+        }
     }
 }
 
-template <typename T>
-void launch_tdma_iteration(const T *a, const T *b, T *x, T *tmp, T *Pw, T *Qw, int nx, int ny,
-                           hipStream_t s)
+static unsigned tdma_setup_blocks(const TdmaBatch &B)
 {
-    hipLaunchKernelGGL(k_tdma_x<T>, dim3((ny + kTile - 1) / kTile), dim3(kLoad), 0, s, a, b, x,
-                       tmp, Pw, Qw, nx, ny);
-    hipLaunchKernelGGL(k_tdma_y<T>, dim3((nx + kTile - 1) / kTile), dim3(kLoad), 0, s, a, b, tmp,
-                       x, Pw, Qw, nx, ny);
+    const long per = tdma_pass_size(B.nx, B.ny, true) + tdma_pass_size(B.nx, B.ny, false);
+    return (unsigned)(((long)B.count * per + 255) / 256);
 }
-template void launch_tdma_iteration<double>(const double *, const double *, double *, double *,
-                                            double *, double *, int, int, hipStream_t);
+static unsigned tdma_point_blocks(const TdmaBatch &B)
+{
+    return (unsigned)(((long)B.count * B.nx * B.ny + 255) / 256);
+}
+static unsigned tdma_chain_blocks(const TdmaBatch &B, bool x, int R)
+{
+    return (unsigned)(B.count * (tdma_lp(x ? B.ny : B.nx) / R));
+}
+
+// Launch KERNEL<R...> for the runtime lane count r (a power of two 2..64).
+#define TDMA_R_SWITCH(r, CALL) \
+    switch (r) {               \
+    case 64: CALL(64); break;  \
+    case 32: CALL(32); break;  \
+    case 16: CALL(16); break;  \
+    case 8: CALL(8); break;    \
+    case 4: CALL(4); break;    \
+    default: CALL(2); break;   \
+    }
+
+template <int RX>
+static void tdma_launch_pre(const TdmaBatch &B, int ry, hipStream_t s)
+{
+    const unsigned nbx = tdma_chain_blocks(B, true, RX);
+#define TDMA_PRE(RY)                                                                            \
+    hipLaunchKernelGGL((k_tdma_pre<RX, RY>), dim3(nbx + tdma_chain_blocks(B, false, RY)),       \
+                       dim3(256), 0, s, B, (int)nbx)
+    TDMA_R_SWITCH(ry, TDMA_PRE)
+#undef TDMA_PRE
+}
+
+// tdma2d9 on up to kTdmaMaxSys systems of one grid size: the pass layouts
+// and the P chains once, then nitm iterations of [D, x lines, D, y lines].
+static void tdma_enqueue(const TdmaBatch &B, int nitm, hipStream_t s)
+{
+    if (nitm < 1) return;
+    const dim3 T(256);
+    hipLaunchKernelGGL(k_tdma_setup, dim3(tdma_setup_blocks(B)), T, 0, s, B);
+    const int prx = tdma_lanes(B.nx, kTdmaPreTiles, kTdmaPreArrays),
+              pry = tdma_lanes(B.ny, kTdmaPreTiles, kTdmaPreArrays);
+#define TDMA_PRE_X(RX) tdma_launch_pre<RX>(B, pry, s)
+    TDMA_R_SWITCH(prx, TDMA_PRE_X)
+#undef TDMA_PRE_X
+    const int rx = tdma_lanes(B.nx, kTdmaLineTiles, kTdmaLineArrays),
+              ry = tdma_lanes(B.ny, kTdmaLineTiles, kTdmaLineArrays);
+    for (int it = 0; it < nitm; ++it) {
+        hipLaunchKernelGGL(k_tdma_d<true>, dim3(tdma_point_blocks(B)), T, 0, s, B);
+#define TDMA_LX(R) \
+    hipLaunchKernelGGL((k_tdma_line<true, R>), dim3(tdma_chain_blocks(B, true, R)), T, 0, s, B)
+        TDMA_R_SWITCH(rx, TDMA_LX)
+#undef TDMA_LX
+        hipLaunchKernelGGL(k_tdma_d<false>, dim3(tdma_point_blocks(B)), T, 0, s, B);
+#define TDMA_LY(R) \
+    hipLaunchKernelGGL((k_tdma_line<false, R>), dim3(tdma_chain_blocks(B, false, R)), T, 0, s, B)
+        TDMA_R_SWITCH(ry, TDMA_LY)
+#undef TDMA_LY
+    }
+}
+
+// Scratch of the device entry points, kept per host thread and grown on
+// demand (one allocation serves every later call of that size or smaller).
+struct TdmaScratchCache {
+    double *p = nullptr;
+    size_t n = 0;
+    ~TdmaScratchCache()
+    {
+        if (p) (void)hipFree(p);
+    }
+    double *get(size_t need)
+    {
+        if (need > n) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            n = 0;
+            if (hipMalloc((void **)&p, sizeof(double) * need) != hipSuccess) return nullptr;
+            n = need;
+        }
+        return p;
+    }
+};
+static thread_local TdmaScratchCache g_tdma_scratch;
+
+// Grid dimensions the chain kernels take: 3 .. kTdmaMaxLen (each pass's
+// line fits the LDS tiles beside the operand ring).
+bool tdma_size_ok(int nx, int ny)
+{
+    return nx >= 3 && ny >= 3 && nx <= kTdmaMaxLen && ny <= kTdmaMaxLen &&
+           tdma_lanes(nx, kTdmaPreTiles, kTdmaPreArrays) >= 2 &&
+           tdma_lanes(ny, kTdmaPreTiles, kTdmaPreArrays) >= 2 &&
+           tdma_lanes(nx, kTdmaLineTiles, kTdmaLineArrays) >= 2 &&
+           tdma_lanes(ny, kTdmaLineTiles, kTdmaLineArrays) >= 2;
+}
+
+int tdma_device_many(int count, int nx, int ny, int nitm, const double *const *a,
+                     const double *const *b, double *const *x, hipStream_t s)
+{
+    if (count < 1 || !tdma_size_ok(nx, ny) || !a || !b || !x) return M2D_ERR_ARG;
+    for (int q = 0; q < count; ++q)
+        if (!a[q] || !b[q] || !x[q]) return M2D_ERR_ARG;
+    if (nitm < 1) return M2D_OK;
+    const long W = tdma_scratch(nx, ny);
+    const int per = std::min(count, kTdmaMaxSys);
+    double *w = g_tdma_scratch.get((size_t)W * per);
+    if (!w) return M2D_ERR_HIP;
+    for (int q0 = 0; q0 < count; q0 += kTdmaMaxSys) {
+        TdmaBatch B;
+        memset(&B, 0, sizeof(B));
+        B.count = std::min(kTdmaMaxSys, count - q0);
+        B.nx = nx;
+        B.ny = ny;
+        for (int q = 0; q < B.count; ++q)
+            B.s[q] = {a[q0 + q], b[q0 + q], x[q0 + q], w + W * q};
+        tdma_enqueue(B, nitm, s);
+        // the next group reuses the scratch: wait for this one
+        if (q0 + kTdmaMaxSys < count && hipStreamSynchronize(s) != hipSuccess) return M2D_ERR_HIP;
+    }
+    if (hipGetLastError() != hipSuccess) return M2D_ERR_HIP;
+    return hipStreamSynchronize(s) == hipSuccess ? M2D_OK : M2D_ERR_HIP;
+}
 
 // ---------------------------------------------------------------------------
 // L1 norms over real volumes (mod_solvers.f90:128-276): per-block partial
@@ -357,17 +731,14 @@ void m2d_set_last_status_code(int status);
 extern "C" int m2d_tdma2d9_device(int nx, int ny, int nitm, const double *d_a, const double *d_b,
                                   double *d_x, void *stream)
 {
-    if (nx < 3 || ny < 3 || !d_a || !d_b || !d_x) return M2D_ERR_ARG;
-    const long nn = (long)nx * ny;
-    double *work = nullptr;
-    if (hipMalloc((void **)&work, sizeof(double) * 3 * nn) != hipSuccess) return M2D_ERR_HIP;
-    hipStream_t s = (hipStream_t)stream;
-    for (int it = 0; it < nitm; ++it)
-        launch_tdma_iteration<double>(d_a, d_b, d_x, work, work + nn, work + 2 * nn, nx, ny, s);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(work);
-    return e == hipSuccess ? M2D_OK : M2D_ERR_HIP;
+    return tdma_device_many(1, nx, ny, nitm, &d_a, &d_b, &d_x, (hipStream_t)stream);
+}
+
+extern "C" int m2d_tdma2d9_device_many(int count, int nx, int ny, int nitm,
+                                       const double *const *d_a, const double *const *d_b,
+                                       double *const *d_x, void *stream)
+{
+    return tdma_device_many(count, nx, ny, nitm, d_a, d_b, d_x, (hipStream_t)stream);
 }
 
 extern "C" void m2d_tdma2d9(const int *nx, const int *ny, const int *nitm, const double *a,

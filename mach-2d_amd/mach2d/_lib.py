@@ -72,6 +72,7 @@ EXPORTS = [
     "m2d_unet_conv3x3", "m2d_unet_conv3x3_wgrad", "m2d_unet_relu_bwd", "m2d_unet_mode_bwd",
     "m2d_unet_to_bf16", "m2d_unet_adam", "m2d_unet_pack_weights", "m2d_unet_mse",
     "m2d_unet_pack_input", "m2d_unet_unpack_output", "m2d_solve_msi9d_noalloc",
+    "m2d_tdma2d9_device_many",
 ]
 
 _lib = None
@@ -164,6 +165,9 @@ def load():
         getattr(L, "m2d_" + nm).restype = None   # pointer arguments, by reference
     L.m2d_tdma2d9_device.argtypes = [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]
     L.m2d_tdma2d9_device.restype = C.c_int
+    L.m2d_tdma2d9_device_many.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
+                                          _vp]
+    L.m2d_tdma2d9_device_many.restype = C.c_int
     L.m2d_solve_msi9d_noalloc.argtypes = [_ip, _ip, _ip, _dp, _vp, _vp, _vp, _vp]
     L.m2d_solve_msi9d_noalloc.restype = C.c_int
     L.m2d_enable_kernel_timing.argtypes = [_vp, C.c_int]

@@ -289,6 +289,28 @@ class ClassSolverTDMA9d(ClassSolverAbstract):
         return self.status
 
 
+def tdma2d9_device_many(nx, ny, nitm, a_list, b_list, x_list, stream=None):
+    """tdma2d9 on several systems of one grid size (CUDA float64 tensors:
+    a 9*nx*ny, b and x nx*ny each) in the same launches
+    (m2d_tdma2d9_device_many); x updated in place."""
+    import torch
+    n = nx * ny
+    if not (len(a_list) == len(b_list) == len(x_list) >= 1):
+        raise ValueError("a, b and x lists must have the same nonzero length")
+    for t, m, name in [(t, 9 * n, "a") for t in a_list] + [(t, n, "b") for t in b_list] + \
+            [(t, n, "x") for t in x_list]:
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
+                and t.is_contiguous() and t.numel() == m):
+            raise TypeError("%s must be a contiguous float64 CUDA tensor of %d values" % (name, m))
+    cnt = len(a_list)
+    arr = lambda ts: (C.c_void_p * cnt)(*[t.data_ptr() for t in ts])
+    rc = _lib.load().m2d_tdma2d9_device_many(cnt, nx, ny, nitm, arr(a_list), arr(b_list),
+                                             arr(x_list), C.c_void_p(stream) if stream else None)
+    if rc:
+        raise SolverError(rc, "m2d_tdma2d9_device_many")
+    return rc
+
+
 class ClassSolverTDMA5d(ClassSolverAbstract):
     """mod_solver_tdma.f90:17-29,48-72 through the zero-corner embedding."""
 

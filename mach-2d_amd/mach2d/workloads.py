@@ -5,6 +5,8 @@
     python -m mach2d.workloads batch --mode fp64 --count 256
     python -m mach2d.workloads batch --mode fp32_ir --count 32 --same-1026x514 [--layout L]
     python -m mach2d.workloads pl_loop --solves 3
+    python -m mach2d.workloads tdma --nx 258 --ny 130 --nitm 5 --count 2 --solves 20
+    python -m mach2d.workloads time_step --solver9d 1 --solves 20
 
 single: the 1026x514 p' system (wall-resolved ogive grid, aspect 1e4,
 Gaussian right-hand side) solved to a 1e-10 drop from zero, `--solves` times
@@ -199,9 +201,70 @@ def pl_loop(nx, ny, itpmax, reps):
             "ms_per_itp": round(ms / itp, 4), "normpl": normpl}
 
 
+def tdma(nx, ny, nitm, count, reps):
+    """tdma2d9 on `count` seeded systems of an nx x ny grid in the same
+    launches (the u and v solves of a SIMPLEC iteration: count 2), `reps`
+    calls after one warm-up: ms per call and per TDMA iteration."""
+    import numpy as np
+    import torch
+
+    from .solvers import tdma2d9_device_many
+    systems = [P.random_f3_system(nx, ny, seed=q) for q in range(count)]
+    a = [torch.tensor(s.a.reshape(-1), device="cuda") for s in systems]
+    b = [torch.tensor(s.rhs, device="cuda") for s in systems]
+    x = [torch.zeros(nx * ny, dtype=torch.float64, device="cuda") for _ in systems]
+    out = []
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tdma2d9_device_many(nx, ny, nitm, a, b, x)
+        if r:
+            out.append((time.perf_counter() - t0) * 1e3)
+    ms = sorted(out)[len(out) // 2]
+    return {"workload": "tdma", "nx": nx, "ny": ny, "nitm": nitm, "count": count,
+            "ms": round(ms, 4), "ms_per_iteration": round(ms / nitm, 4)}
+
+
+def time_step(solver9d, steps):
+    """The internal-flow time step (m2d_intflow_time_step) on the 258x130
+    nozzle from inlet Mach 0.2, `steps` steps after one warm-up step."""
+    import numpy as np
+    import torch
+
+    from . import simplec
+    from .solvers import DeviceMSI
+    c = P.CFG2
+    nx, ny = c["nx"], c["ny"]
+    prm = dict(modvis=0, coord=1, solver9d=solver9d, itmmax=1, itpmax=20, nitm_u=5, nitm_p=20,
+               tol_u=1e-2, tol_p=1e-2, tolm=1e-2, beta=0.0, Rg=287.0, gamma=1.4, P0=1.2e5,
+               T0=310.0, Cpref=1004.5, Href=1004.5 * 310.0, Twall=-1.0, h0=0.01, mincc=0.99,
+               maxcc=0.9999)
+    f0 = P.time_step_fields(nx, ny, 5, prm, mach_in=c["mach_in"])
+    d = {k: torch.tensor(np.ascontiguousarray(v).reshape(-1), device="cuda")
+         for k, v in f0.items()}
+    dev9 = DeviceMSI(nx, ny) if solver9d == 0 else None
+    dev5 = DeviceMSI(nx, ny, ndiag=5)
+    torch.cuda.synchronize()
+    dt = c["dt0"]
+    dt, r = simplec.intflow_time_step(dev9, dev5, d, prm, dt)
+    t0 = time.perf_counter()
+    itps = 0
+    for _ in range(steps):
+        dt, r = simplec.intflow_time_step(dev9, dev5, d, prm, dt)
+        itps += r["itp"]
+    ms = (time.perf_counter() - t0) * 1e3 / max(steps, 1)
+    return {"workload": "time_step", "solver9d": solver9d, "steps": steps,
+            "ms_per_step": round(ms, 3), "itp_per_step": round(itps / max(steps, 1), 2)}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["single", "batch", "split", "pl_loop"])
+    ap.add_argument("what", choices=["single", "batch", "split", "pl_loop", "tdma",
+                                     "time_step"])
+    ap.add_argument("--nx", type=int, default=258)
+    ap.add_argument("--ny", type=int, default=130)
+    ap.add_argument("--nitm", type=int, default=5)
+    ap.add_argument("--solver9d", type=int, default=0)
     ap.add_argument("--groups", type=int, default=2)
     ap.add_argument("--mode", default="fp32_ir", choices=["fp64", "fp32_ir"])
     ap.add_argument("--solves", type=int, default=1)
@@ -217,6 +280,10 @@ def main(argv=None):
                      args.inner_nitm or 25)
     elif args.what == "pl_loop":
         out = pl_loop(258, 130, 20, max(1, args.solves))
+    elif args.what == "tdma":
+        out = tdma(args.nx, args.ny, args.nitm, args.count, max(1, args.solves))
+    elif args.what == "time_step":
+        out = time_step(args.solver9d, max(1, args.solves))
     elif args.what == "split":
         out = batch_split(args.mode, args.count, args.groups,
                           0 if args.layout is None else args.layout, args.inner_nitm or 50)

@@ -1,8 +1,11 @@
 """GPU parity of the line TDMA (tdma2d9) and the L1 norms through the C ABI.
 
-TDMA: bit-exact against the reference's golden vectors (the Thomas
-recurrences run in the reference's order, one lane per line).  Norms:
-within 1e-13 relative (the GPU sums in a different order)."""
+TDMA: bit-exact against the reference's golden vectors and the oracle (the
+Thomas recurrences run in the reference's order, one lane per line; P and
+its denominators once per call, the right-hand sides D in parallel), on
+short and long lines (lines longer than 64 KB take fewer lanes per LDS
+tile), and for batches of systems in the same launches.  Norms: within
+1e-13 relative (the GPU sums in a different order)."""
 import numpy as np
 import pytest
 
@@ -23,7 +26,8 @@ def test_tdma2d9_bitwise_vs_fixtures(gpu, k):
         assert np.array_equal(x, g[c + "_x%d" % k]), c
 
 
-@pytest.mark.parametrize("nx,ny", [(258, 130), (130, 66), (17, 300), (1026, 9)])
+@pytest.mark.parametrize("nx,ny", [(258, 130), (130, 66), (17, 300), (1026, 9), (3, 3),
+                                   (1100, 37), (45, 2000), (1026, 514)])
 def test_tdma2d9_bitwise_vs_oracle(gpu, orc, nx, ny):
     rng = np.random.default_rng(nx * ny)
     a = -rng.uniform(0.05, 1.0, size=(9, nx * ny))
@@ -33,6 +37,30 @@ def test_tdma2d9_bitwise_vs_oracle(gpu, orc, nx, ny):
     x = x0.copy()
     gpu.tdma2d9(nx, ny, 3, a, b, x)
     assert np.array_equal(x, orc.tdma2d9(a, b, x0, nx, ny, 3))
+
+
+def test_tdma_device_many_equals_single(gpu, orc):
+    """Three systems of one grid size in the same launches: each result is
+    its single-system result and the oracle's, bit for bit; the x-pass
+    result of one system never leaks into another."""
+    import torch
+    nx, ny = 258, 130
+    systems = [P.random_f3_system(nx, ny, seed=30 + q) for q in range(3)]
+    x0 = [np.random.default_rng(q).uniform(-1, 1, nx * ny) for q in range(3)]
+    ta = [torch.tensor(s.a.reshape(-1), device="cuda") for s in systems]
+    tb = [torch.tensor(s.rhs, device="cuda") for s in systems]
+    tx = [torch.tensor(v, device="cuda") for v in x0]
+    gpu.tdma2d9_device_many(nx, ny, 5, ta, tb, tx)
+    for q, s in enumerate(systems):
+        want = orc.tdma2d9(s.a, s.rhs, x0[q], nx, ny, 5)
+        assert np.array_equal(tx[q].cpu().numpy(), want), q
+        one = torch.tensor(x0[q], device="cuda")
+        gpu.ClassSolverTDMA9d().init(nx, ny, 5).solve(ta[q], tb[q], one)
+        assert torch.equal(one, tx[q]), q
+    # nitm = 0 leaves x untouched (the reference's loop runs zero times)
+    keep = [t.clone() for t in tx]
+    gpu.tdma2d9_device_many(nx, ny, 0, ta, tb, tx)
+    assert all(torch.equal(u, v) for u, v in zip(keep, tx))
 
 
 def test_tdma_solver_objects(gpu, orc):
