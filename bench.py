@@ -421,14 +421,15 @@ def batch_cpu_run(count=256, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     ref = O.ref_available()
+    R, path = O.ref_timing() if ref else (None, None)
     sizes = P.cfg4_sizes(count)
-    threads = min(16, os.cpu_count() or 1)
+    threads = min(16, host_cpu()["usable_cores"])
 
     def solve(q):
         nx, ny = sizes[q]
         s = P.cfg4_system(q, nx, ny)
         if ref:   # this worker thread has the stack fb2d9 needs
-            O.ref().ref_solver_msi9d(nx, ny, 20000, GAMMA, np.ascontiguousarray(s.a),
+            R.ref_solver_msi9d(nx, ny, 20000, GAMMA, np.ascontiguousarray(s.a),
                                      np.ascontiguousarray(s.rhs), np.zeros(s.N))
         else:
             O.solve_msi9d(s.a, s.rhs, np.zeros(s.N), nx, ny, 20000, GAMMA)
@@ -452,21 +453,44 @@ def batch_cpu_run(count=256, budget_s=12.0):
     total = sum(nx * ny for nx, ny in sizes)
     return {"solves_per_s": round(done / dt, 3), "threads": threads, "systems": done,
             "seconds": round(dt, 2), "kind": "reference" if ref else "port",
+            "build": O.REF_FLAGS[path] if ref else "gcc -O2 (the oracle)",
             "batch_seconds_est": round(dt * total / pts, 1)}
 
 
+def host_cpu():
+    """Model name of the host CPU and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "usable_cores": usable}
+
+
 def cpu_baseline(s, nit, budget_s=15.0):
-    """Reference lu2d9 + fb2d9 (compiled Fortran, one core) on a bounded
-    sample: the factorization plus M iterations; solves/s extrapolated to
-    the nit iterations the same algorithm needs for the 1e-10 drop."""
+    """Reference lu2d9 + fb2d9 (compiled Fortran, the -O3 build, one core)
+    on a bounded sample: the factorization plus M iterations; solves/s
+    extrapolated to the nit iterations the same algorithm needs for the
+    1e-10 drop."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     kind = "reference" if O.ref_available() else "port"
     if kind == "reference":
-        lu = lambda: O.ref_lu2d9(s.a, s.nx, s.ny)
+        R, path = O.ref_timing()
+        build = O.REF_FLAGS[path] + " (" + os.path.basename(path) + ")"
+        lu = lambda: O.ref_lu2d9(s.a, s.nx, s.ny, lib=R)
         fb = lambda dl, du, m: O.ref_fb2d9(s.a, dl, du, s.nx, s.ny, np.zeros(s.N), 0.0, m,
-                                           s.rhs)
+                                           s.rhs, lib=R)
     else:
+        build = "gcc -O2 -ffp-contract=off (the oracle's C restatement)"
         lu = lambda: O.lu2d9(s.a, s.nx, s.ny)
         fb = lambda dl, du, m: O.fb2d9(s.a, dl, du, s.nx, s.ny, np.zeros(s.N), 0.0, m, s.rhs)
     t = time.perf_counter()
@@ -482,6 +506,7 @@ def cpu_baseline(s, nit, budget_s=15.0):
     t_iter = t_m / m
     t_solve = t_lu + nit * t_iter
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": kind,
+            "build": build, "cpu": host_cpu(),
             "sample": "lu2d9 + %d fb2d9 iterations of the 1026x514 system (%.1fs); "
                       "solve time = t_lu %.3fs + %d iterations x %.2f ms" % (
                           m, t_lu + t_m, t_lu, nit, t_iter * 1e3)}
@@ -524,17 +549,25 @@ def batch_run(mode, steps, warmup, count=256, same=None, learned=False):
         a = [torch.tensor(x.a.reshape(-1), device="cuda") for x in systems]
         rhs = [torch.tensor(x.rhs, device="cuda") for x in systems]
     else:
+        # `count` distinct matrices (each system its own copy in HBM, so no
+        # two systems share cached coefficients): the matrix of `same` with
+        # the diagonal of its interior rows scaled by a seeded factor in
+        # [1, 1.001], and a seeded Gaussian right-hand side per system
         sizes = [(same.nx, same.ny)] * count
         systems = [same] * count
         a0 = torch.tensor(same.a.reshape(-1), device="cuda")
-        a = [a0] * count
+        N = same.N
         interior = torch.tensor(same.rhs != 0.0, device="cuda")
         gen = torch.Generator(device="cuda")
-        rhs = []
+        a, rhs = [], []
         for q in range(count):
             gen.manual_seed(7000 + q)
-            r = torch.randn(same.N, dtype=torch.float64, device="cuda", generator=gen)
+            r = torch.randn(N, dtype=torch.float64, device="cuda", generator=gen)
             rhs.append(torch.where(interior, r, torch.zeros_like(r)))
+            f = 1.0 + 1e-3 * float(np.random.default_rng(8000 + q).uniform())
+            aq = a0.clone()
+            aq[4 * N:5 * N] = torch.where(interior, aq[4 * N:5 * N] * f, aq[4 * N:5 * N])
+            a.append(aq)
     log("batch: %d systems, N = %d in total, built in %.1fs" % (
         count, sum(x.N for x in systems), time.time() - t))
     var = [torch.zeros(x.N, dtype=torch.float64, device="cuda") for x in systems]
@@ -598,23 +631,37 @@ def batch_run(mode, steps, warmup, count=256, same=None, learned=False):
 
 
 def target_comparison(summary, batch3):
-    """The 1026x514 solve batched with FP32/FP64 refinement, with and without
-    the learned initial guess, against plain FP64 MSI (one system at a
-    time), in solves/s."""
-    if not batch3 or "fp64" not in summary:
+    """The 1026x514 solve to the 1e-10 drop: the refined (FP32 sweeps in
+    FP64 refinement) and learned-guess paths against plain FP64 MSI, in
+    solves/s, three ways: the same batch of distinct systems in both
+    precisions (like for like), one system at a time, and the batched paths
+    against one plain FP64 solve at a time."""
+    if "fp64" not in summary:
         return None
     plain = summary["fp64"]["solves_per_s"]
-    paths = {}
-    for k in ("learned_fp32_ir", "fp32_ir"):
-        v = batch3.get(k)
+    out = {"plain_fp64_single_solves_per_s": plain, "target": 20.0}
+    single = {}
+    for k in ("fp32_ir", "learned_fp32_ir"):
+        v = summary.get(k)
         if v and v.get("correct"):
-            paths["batch_1026x514." + k] = {"solves_per_s": v["solves_per_s"],
-                                            "speedup": round(v["solves_per_s"] / plain, 2)}
-    if not paths:
-        return None
-    best = max(paths, key=lambda k: paths[k]["solves_per_s"])
-    return {"plain_fp64_solves_per_s": plain, "paths": paths, "best": best,
-            "speedup": paths[best]["speedup"], "target": 20.0}
+            single[k] = round(v["solves_per_s"] / plain, 3)
+    out["single_vs_single_fp64"] = single
+    if batch3 and batch3.get("fp64", {}).get("correct"):
+        b64 = batch3["fp64"]["solves_per_s"]
+        out["plain_fp64_batched_solves_per_s"] = b64
+        same, cross = {}, {}
+        for k in ("fp32_ir", "learned_fp32_ir"):
+            v = batch3.get(k)
+            if v and v.get("correct"):
+                same[k] = round(v["solves_per_s"] / b64, 3)
+                cross[k] = round(v["solves_per_s"] / plain, 2)
+        out["batched_vs_batched_fp64"] = same
+        out["batched_vs_single_fp64"] = cross
+        if cross:
+            best = max(cross, key=lambda k: cross[k])
+            out["best_batched_vs_single_fp64"] = {"path": "batch_1026x514." + best,
+                                                  "speedup": cross[best]}
+    return out
 
 
 def main():

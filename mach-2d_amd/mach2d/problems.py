@@ -440,12 +440,13 @@ def flow_fields(nx: int, ny: int, seed: int = 0, q: float = 1.05, coord: int = 1
     return f
 
 
-def nozzle_grid(nx, ny):
+def nozzle_grid(nx, ny, area_c=2.2):
     """Converging-diverging nozzle, x in [0, 3], wall radius sqrt(A(x)/pi)
-    with A(x) = 1 + 2.2 (x - 1.5)^2, the axis at y = 0; corners as the
-    reference stores them (north-east corner of each volume)."""
+    with A(x) = 1 + area_c (x - 1.5)^2 (area_c = 2.2: the workload's
+    nozzle), the axis at y = 0; corners as the reference stores them
+    (north-east corner of each volume)."""
     xs = np.linspace(0.0, 3.0, nx - 1)
-    rw = np.sqrt((1.0 + 2.2 * (xs - 1.5) ** 2) / np.pi)
+    rw = np.sqrt((1.0 + area_c * (xs - 1.5) ** 2) / np.pi)
     x = np.zeros((ny, nx))
     y = np.zeros((ny, nx))
     fr = np.linspace(0.0, 1.0, ny - 1)
@@ -468,7 +469,7 @@ def inlet_speed(mach, params):
     return mach * math.sqrt(params["gamma"] * params["Rg"] * (params["T0"] - 2.0))
 
 
-def time_step_fields(nx, ny, seed, params, coord=1, u_in=10.0, mach_in=None):
+def time_step_fields(nx, ny, seed, params, coord=1, u_in=10.0, mach_in=None, area_c=2.2):
     """Seeded inputs of one internal-flow time step (m2d_intflow_time_step)
     on the nozzle, `params` giving T0, P0, Rg and gamma: a flow at rest
     temperature and pressure (velocity u_in A_in / A(x) plus 1 % noise, the
@@ -478,7 +479,7 @@ def time_step_fields(nx, ny, seed, params, coord=1, u_in=10.0, mach_in=None):
     5-diagonal one likewise, and the inflow vectors."""
     if mach_in is not None:
         u_in = inlet_speed(mach_in, params)
-    g = nozzle_grid(nx, ny)
+    g = nozzle_grid(nx, ny, area_c)
     m = metrics(g)
     f = {k: np.ascontiguousarray(v.reshape(-1)) for k, v in m.items()}
     N = nx * ny
@@ -491,9 +492,9 @@ def time_step_fields(nx, ny, seed, params, coord=1, u_in=10.0, mach_in=None):
                                      for k in ("re", "rn", "rp"))
     else:
         f["re"], f["rn"], f["rp"] = np.ones(N), np.ones(N), np.ones(N)
-    area = 1.0 + 2.2 * (f["xp"] - 1.5) ** 2
+    area = 1.0 + area_c * (f["xp"] - 1.5) ** 2
     T0, P0, Rg = params["T0"], params["P0"], params["Rg"]
-    f["u"] = u_in * 5.95 / area * (1.0 + 0.01 * rng.uniform(-1, 1, N))
+    f["u"] = u_in * (1.0 + area_c * 2.25) / area * (1.0 + 0.01 * rng.uniform(-1, 1, N))
     f["v"] = np.zeros(N)
     f["T"] = np.full(N, T0 - 2.0)
     f["p"] = np.full(N, 0.99 * P0) * (1.0 + 1e-4 * rng.uniform(-1, 1, N))

@@ -27,6 +27,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libm2dref.so")
+# the same reference sources built with -O3 (amdflang), for CPU timings only
+REF_O3_SO = os.path.join(HERE, "_ref", "libm2dref_o3.so")
+REF_FLAGS = {REF_SO: "amdflang -O2 -ffp-contract=off", REF_O3_SO: "amdflang -O3"}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = C.POINTER(C.c_int)
@@ -34,6 +37,7 @@ _dptr = C.POINTER(C.c_double)
 
 _lib = None
 _ref = None
+_ref_o3 = None
 
 
 def build():
@@ -71,25 +75,39 @@ def ref_available() -> bool:
     return os.path.exists(REF_SO)
 
 
+def ref_timing():
+    """The -O3 build of the reference (CPU-baseline timings), else the
+    fixture build; returns (library, path)."""
+    global _ref_o3
+    if _ref_o3 is None and os.path.exists(REF_O3_SO):
+        _ref_o3 = _bind_ref(C.CDLL(REF_O3_SO))
+    if _ref_o3 is not None:
+        return _ref_o3, REF_O3_SO
+    return ref(), REF_SO
+
+
 def ref():
     global _ref
     if _ref is None:
         if not ref_available():
             raise FileNotFoundError(REF_SO + " (run `make -C oracle ref` where the reference exists)")
-        R = C.CDLL(REF_SO)
-        R.ref_lu2d9.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
-        R.ref_lu2d5.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
-        for f in (R.ref_fb2d9, R.ref_fb2d5):
-            f.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_double, C.c_int, _dp]
-        R.ref_tdma2d9.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
-        R.ref_solver_msi9d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
-        R.ref_norm_l1_9d.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dptr]
-        R.ref_norm_l1_9d_relative.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dptr]
-        R.ref_norm_l1_b.argtypes = [C.c_int, C.c_int, _dp]
-        R.ref_norm_l1_b.restype = C.c_double
-        _bind_ref_coef(R)
-        _ref = R
+        _ref = _bind_ref(C.CDLL(REF_SO))
     return _ref
+
+
+def _bind_ref(R):
+    R.ref_lu2d9.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
+    R.ref_lu2d5.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
+    for f in (R.ref_fb2d9, R.ref_fb2d5):
+        f.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, C.c_double, C.c_int, _dp]
+    R.ref_tdma2d9.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+    R.ref_solver_msi9d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp]
+    R.ref_norm_l1_9d.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dptr]
+    R.ref_norm_l1_9d_relative.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dptr]
+    R.ref_norm_l1_b.argtypes = [C.c_int, C.c_int, _dp]
+    R.ref_norm_l1_b.restype = C.c_double
+    _bind_ref_coef(R)
+    return R
 
 
 # --------------------------------------------------------------------------
@@ -358,20 +376,21 @@ def _big_stack_call(fn, stack_mb=512):
     return out.get("v")
 
 
-def ref_lu2d9(a, nx, ny):
+def ref_lu2d9(a, nx, ny, lib=None):
     a = _c(a)
     n = nx * ny
     dl = np.zeros((5, n))
     du = np.zeros((4, n))
-    ref().ref_lu2d9(a, dl, du, n, nx, ny)
+    (lib or ref()).ref_lu2d9(a, dl, du, n, nx, ny)
     return dl, du
 
 
-def ref_fb2d9(a, dl, du, nx, ny, var, gamma, nitm, rhs):
+def ref_fb2d9(a, dl, du, nx, ny, var, gamma, nitm, rhs, lib=None):
     var = _c(var).copy()
     n = nx * ny
-    _big_stack_call(lambda: ref().ref_fb2d9(_c(a), _c(dl), _c(du), ny, nx, n, var,
-                                            gamma, nitm, _c(rhs)))
+    R = lib or ref()
+    _big_stack_call(lambda: R.ref_fb2d9(_c(a), _c(dl), _c(du), ny, nx, n, var, gamma, nitm,
+                                        _c(rhs)))
     return var
 
 

@@ -30,6 +30,9 @@ Fixtures (float64, diagonal-major arrays (k, nx*ny)):
                   and a 12x8 case with faces on the axis (r = 0) for the
                   axis branches of get_u_source; outputs start from seeded
                   values so untouched points are checked too
+  simplec_pairs_34x18.m2dt  learned-guess training pairs (M2DT format) from
+                  12 SIMPLEC time steps of a 34x18 nozzle composed from the
+                  reference's routines, targets by its class_solver_msi9d
   grid.npz        get_real_centroids_xy (opt 1, 2), get_metrics and
                   get_radius (coord 0, 1) of mod_grid.f90 (module-private,
                   called through their module symbols) on ogive, nozzle,
@@ -265,6 +268,37 @@ def grid():
     save("grid.npz", **arrs)
 
 
+def simplec_pairs():
+    """Training pairs of the learned guess as the SIMPLEC time step
+    generates them (mach2d.learned.simplec_pairs), on a 34x18 nozzle: the
+    time step composed from the compiled reference's routines, the p'
+    system of every step's last pressure-correction iteration, and its
+    solution by the reference's class_solver_msi9d (zero-corner embedding)
+    to a 1e-10 drop, written in the M2DT format."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import time_step_oracle as TS
+    from mach2d import dataset as D
+    from mach2d import learned as Lr
+    nx, ny, steps = 34, 18, 12
+    pf = os.path.join(HERE, "parametersNozzleEulerPTC.txt")
+    fseed, area_c, mach = Lr.variant_params(0, 0)
+    prm = dict(Lr.NOZZLE)
+    s = P.time_step_fields(nx, ny, fseed, prm, mach_in=mach, area_c=area_c)
+    dt, n = 5e-6, nx * ny
+    path = os.path.join(HERE, "simplec_pairs_34x18.m2dt")
+    with D.Writer(path, nx, ny, 5, 1e-10, D.SOURCE_SIMPLEC) as W:
+        for k in range(steps):
+            s, dt, r = TS.step(O, s, nx, ny, prm, dt, ref=True, paramfile=pf)
+            a5 = np.asarray(s["ap"]).reshape(5, n)
+            b = np.asarray(s["bp"]).reshape(n)
+            a9 = np.zeros((9, n))
+            a9[[1, 3, 4, 5, 7]] = a5
+            x = O.ref_solver_msi9d(a9, b, np.zeros(n), nx, ny, 20000, 1e-10)
+            rel = np.linalg.norm(O.residual9(a9, x, b, nx, ny)) / np.linalg.norm(b)
+            W.add(a5, b, x, fseed, k, r["itp"], rel)
+    print("%-18s %8.1f KB" % (os.path.basename(path), os.path.getsize(path) / 1024))
+
+
 if __name__ == "__main__":
     O.build()
     if not O.ref_available():
@@ -277,3 +311,4 @@ if __name__ == "__main__":
     solver_msi9d()
     coef()
     grid()
+    simplec_pairs()

@@ -196,3 +196,18 @@ def test_worse_than_zero_guess_falls_back_to_zero(gpu):
     used, rg, rb = Lr.judge_guess(dev, ta, tr, good)
     assert used and rg < rb and float(good.abs().max()) > 0.0
     dev.close()
+
+
+def test_cfg5_shipped_weights_on_heldout_simplec_systems(gpu):
+    """The trained network on p' systems of a SIMPLEC run it was not trained
+    on (258x130 nozzle, variant seed 99): the guessed solve stops at
+    gamma ||b||, agrees with the zero-guess solution to 1e-8 relative, and
+    keeps the guess only where its residual is below ||b||."""
+    from mach2d import learned as Lr
+    from mach2d import unet as U
+    net = U.UNet()
+    net.load(Lr.default_weights())
+    systems = Lr.simplec_systems(258, 130, 99, 60, 4)
+    for z, g, used, rr, rel in Lr.compare_guess(net, systems, 258, 130):
+        assert rel < 1e-8, rel
+        assert (rr < 1.0) if used else (rr == 1.0 and g == z)

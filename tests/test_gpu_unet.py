@@ -408,3 +408,30 @@ def test_guess_prepare_many_matches_single_calls(gpu):
         assert torch.equal(g1[q], g2[q])
         assert rn[q] == pytest.approx(dev.residual_norm(a[q], b[q], g1[q]), rel=1e-12)
     dev.close()
+
+
+def test_simplec_pairs_match_reference_fixture(gpu, tmp_path):
+    """Training pairs generated on the GPU (device SIMPLEC time step, batched
+    FP64 MSI targets) equal the pairs of the same time steps composed from
+    the compiled reference's routines and solved by its class_solver_msi9d
+    (tests/golden/simplec_pairs_34x18.m2dt): systems, right-hand sides and
+    targets bit for bit (as stored, float32), the same step and itp
+    metadata, the residual drop to 1e-3 relative (a residual 1e-13 of ||b||
+    summed in another order and, on the host, from float64 values
+    recomputed by the oracle)."""
+    import os
+
+    from mach2d import dataset as D
+    from mach2d import learned as Lr
+    path = str(tmp_path / "p.m2dt")
+    n = Lr.simplec_pairs(path, variants=1, steps=12, nx=34, ny=18, seed=0)
+    assert n == 12
+    got = D.Dataset(path)
+    want = D.Dataset(os.path.join(os.path.dirname(__file__), "golden",
+                                  "simplec_pairs_34x18.m2dt"))
+    assert (got.nx, got.ny, got.ndiag, len(got)) == (want.nx, want.ny, want.ndiag, len(want))
+    for q in range(len(want)):
+        for u, v in zip(got[q], want[q]):
+            assert np.array_equal(u, v), q
+        assert list(got.meta[q][:3]) == list(want.meta[q][:3])
+        assert got.meta[q][3] == pytest.approx(want.meta[q][3], rel=1e-3)

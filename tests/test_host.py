@@ -334,3 +334,59 @@ def test_time_step_entry_rejects_bad_arguments():
                                    C.byref(r)) == _lib.M2D_ERR_ARG
     assert L.m2d_intflow_time_step(None, None, None, None, None, None) == _lib.M2D_ERR_ARG
     assert dt.value == 1e-5
+
+
+def test_m2dt_round_trip_and_checks(tmp_path):
+    """The training-set format: records and metadata come back as written
+    (float32), the header carries grid, diagonals, count, gamma and source,
+    and a wrong magic, version or size is refused."""
+    import struct
+
+    from mach2d import dataset as D
+    nx, ny, nd = 7, 5, 5
+    N = nx * ny
+    rng = np.random.default_rng(3)
+    recs = [(rng.standard_normal((nd, N)), rng.standard_normal(N), rng.standard_normal(N))
+            for _ in range(4)]
+    path = str(tmp_path / "t.m2dt")
+    with D.Writer(path, nx, ny, nd, 1e-10) as W:
+        for q, (a, b, x) in enumerate(recs):
+            W.add(a, b, x, seed=q, step=10 + q, itp=2, rel_residual=1e-11 * q)
+    ds = D.Dataset(path)
+    assert (len(ds), ds.nx, ds.ny, ds.ndiag, ds.gamma, ds.source) == (4, nx, ny, nd, 1e-10,
+                                                                       D.SOURCE_SIMPLEC)
+    for q, (a, b, x) in enumerate(recs):
+        ga, gb, gx = ds[q]
+        assert np.array_equal(ga, a.astype(np.float32)) and np.array_equal(gb, b.astype(np.float32))
+        assert np.array_equal(gx, x.astype(np.float32))
+        assert list(ds.meta[q]) == [q, 10 + q, 2, 1e-11 * q]
+    raw = bytearray(open(path, "rb").read())
+    for patch, msg in ((lambda r: r.__setitem__(slice(0, 4), b"XXXX"), "not an M2DT"),
+                       (lambda r: r.__setitem__(slice(4, 8), struct.pack("<I", 9)), "version"),
+                       (lambda r: r.extend(b"\0" * 8), "size")):
+        bad = bytearray(raw)
+        patch(bad)
+        p2 = str(tmp_path / "bad.m2dt")
+        open(p2, "wb").write(bytes(bad))
+        with pytest.raises(ValueError, match=msg):
+            D.Dataset(p2)
+
+
+def test_simplec_pairs_fixture_is_solved(orc):
+    """The committed training pairs (SIMPLEC p' systems of the reference's
+    time step, solved by its class_solver_msi9d): each target's residual on
+    its own system is below 1e-10 of ||b|| (float32 storage aside: checked
+    in FP64 from the stored values against the metadata's drop)."""
+    from mach2d import dataset as D
+    path = os.path.join(os.path.dirname(__file__), "golden", "simplec_pairs_34x18.m2dt")
+    ds = D.Dataset(path)
+    assert len(ds) == 12 and ds.ndiag == 5 and ds.gamma == 1e-10
+    for q in range(len(ds)):
+        a5, b, x = (np.asarray(v, dtype=np.float64) for v in ds[q])
+        a9 = np.zeros((9, ds.N))
+        a9[[1, 3, 4, 5, 7]] = a5
+        assert ds.meta[q][3] < 1e-10
+        # the float32-rounded target still solves the float32-rounded system
+        # far better than the zero guess
+        r = np.linalg.norm(orc.residual9(a9, x, b, ds.nx, ds.ny)) / np.linalg.norm(b)
+        assert r < 1e-5, (q, r)
