@@ -779,13 +779,13 @@ __global__ __launch_bounds__(kTx) void k_mass_terms(Fields F)
     const long t = (long)(j - 2) * (F.nx - 2) + (i - 2);
     F.out[1][t] = fabs((ro[np] - roa[np]) * rp[np] / (Jp[np] * F.dt) + fme - fmw + fmn - fms);
 }
-// the terms added in the reference's order on one lane, staged through LDS
-// by the other waves (one workgroup of kSeqThreads)
+// the terms added in the reference's order by one wave (seq_sum)
 __global__ __launch_bounds__(kSeqThreads) void k_mass_sum(Fields F)
 {
     const long n = (long)(F.nx - 2) * (F.ny - 2);
     const double *t = F.out[1];
-    const double r = seq_sum([&](long k) { return t[k]; }, n);
+    __shared__ double lds[kSeqSlots * kSeqChunk];
+    const double r = seq_sum(t, n, lds);
     if (threadIdx.x == 0) F.out[0][0] = r;
 }
 

@@ -330,14 +330,20 @@ __global__ __launch_bounds__(kT) void k_sum(const double *__restrict__ part, lon
 }
 
 // get_average: F over real volumes in the reference's order (j outer, i
-// inner), added by one lane, divided by (nx-2)(ny-2)
-__global__ __launch_bounds__(kSeqThreads) void k_average(const double *__restrict__ F, int nx,
+// inner) gathered into t (contiguous, in that order), then added in order
+// by one wave and divided by (nx-2)(ny-2)
+__global__ __launch_bounds__(kT) void k_average_gather(const double *__restrict__ F, int nx,
+                                                      int ny, double *__restrict__ t)
+{
+    const int i = blockIdx.x * kT + threadIdx.x + 1, j = blockIdx.y + 1;
+    if (i <= nx - 2) t[(long)(j - 1) * (nx - 2) + (i - 1)] = F[(long)nx * j + i];
+}
+__global__ __launch_bounds__(kSeqThreads) void k_average(const double *__restrict__ t, int nx,
                                                         int ny, double *__restrict__ out)
 {
-    const int w = nx - 2;
-    const long n = (long)w * (ny - 2);
-    const double s = seq_sum(
-        [&](long q) { return F[(long)nx * (q / w + 1) + q % w + 1]; }, n);
+    const long n = (long)(nx - 2) * (ny - 2);
+    __shared__ double lds[kSeqSlots * kSeqChunk];
+    const double s = seq_sum(t, n, lds);
     if (threadIdx.x == 0) out[0] = s / ((double)(nx - 2) * (double)(ny - 2));
 }
 
@@ -566,11 +572,17 @@ extern "C" double m2d_get_average(const int *nx, const int *ny, const double *F)
         return 0.0;
     }
     const int NX = *nx, NY = *ny;
+    const long n = (long)(NX - 2) * (NY - 2);
     double out = 0.0;
-    const int rc = run(NX, NY, {{F, (long)NX * NY, false}}, 1, true,
+    // scratch: the mean, the n gathered terms, one more (seq_sum reads whole
+    // 16-byte pairs)
+    const int rc = run(NX, NY, {{F, (long)NX * NY, false}}, n + 2, true,
                        [&](std::vector<double *> &d, hipStream_t s, double *scr, dim3) {
-                           hipLaunchKernelGGL(k_average, dim3(1), dim3(kSeqThreads), 0, s, d[0],
-                                              NX, NY, scr);
+                           const dim3 g((unsigned)((NX - 2 + kT - 1) / kT), (unsigned)(NY - 2));
+                           hipLaunchKernelGGL(k_average_gather, g, dim3(kT), 0, s, d[0], NX, NY,
+                                              scr + 1);
+                           hipLaunchKernelGGL(k_average, dim3(1), dim3(kSeqThreads), 0, s,
+                                              scr + 1, NX, NY, scr);
                            (void)hipMemcpyAsync(&out, scr, sizeof(double),
                                                 hipMemcpyDeviceToHost, s);
                        });

@@ -52,8 +52,9 @@ __global__ __launch_bounds__(kT) void k_cc_terms(const double *__restrict__ a, i
 __global__ __launch_bounds__(kSeqThreads) void k_seq_mean(const double *__restrict__ t, long n,
                                                          double *__restrict__ out)
 {
+    __shared__ double lds[kSeqSlots * kSeqChunk];
     const double *__restrict__ tt = t + (long)blockIdx.x * n;
-    const double m = seq_sum([&](long q) { return tt[q]; }, n);
+    const double m = seq_sum(tt, n, lds);
     if (threadIdx.x == 0) out[blockIdx.x] = m / (double)n;
 }
 
