@@ -496,12 +496,26 @@ typedef struct m2d_pl_loop_fields {
  * intflow_velocities_at_boundary_faces, get_pressure_correction_with_pl,
  * get_density_correction_with_pl, get_density_at_faces, and the exit test
  * max|pl| / p_avg < tolm.  Everything runs on the handle's stream on device
- * arrays; the host reads one scalar and the solve state per itp (the exit
- * test).  s5: a 5-diagonal FP64 handle (m2d_create_stencil) of the grid.
- * itp_out: itp iterations run; normpl_out: the last max|pl| / p_avg. */
+ * arrays.  The exit test runs on the device: once it fires (or the solve
+ * reports an error) every kernel of the iterations enqueued after it
+ * returns at once.  The iterations go out in batches, replayed from HIP
+ * graphs captured on first use (m2d_set_pl_graph), each batch followed by
+ * one host read; the first batch is as long as the previous call's loop,
+ * later ones one iteration each, so a steady run reads the host once per
+ * call.  s5: a 5-diagonal FP64 handle (m2d_create_stencil) of the grid,
+ * not shared with another thread while the call runs.  itp_out: itp
+ * iterations run; normpl_out: the last max|pl| / p_avg. */
 int m2d_intflow_pl_loop(m2d_system *s5, const m2d_pl_loop_fields *f, int modvis, double dt,
                         double beta, double p_avg, double tolm, int itpmax, int nitm_p,
                         double tol_p, int *itp_out, double *normpl_out);
+
+/* No reference counterpart (tuning and introspection of the loop above).
+ * m2d_set_pl_graph: 1 (default) replays the batches from captured HIP
+ * graphs, 0 enqueues the same kernels directly; the results are the same
+ * bit for bit.  m2d_pl_loop_info: the host reads and batches of the last
+ * m2d_intflow_pl_loop call on this handle. */
+int m2d_set_pl_graph(m2d_system *s5, int use_graph);
+int m2d_pl_loop_info(const m2d_system *s5, int *host_reads, int *batches);
 
 /* Device arrays of one internal-flow time step (src/mach2d_main.f90:94-
  * 404): N-vectors in the reference layout unless stated. */

@@ -41,7 +41,17 @@ struct Fields {
     int modvis, coord;
     double dt, beta;
     double twall;
+    // set by a loop that is enqueued ahead of its exit test (the pressure-
+    // correction loop): when gate is non-null and *gate != 0 the kernel
+    // returns at once; when sc is non-null, dt = sc[0] and beta = sc[1] are
+    // read from device memory, so one captured loop serves every dt
+    const int *gate;
+    const double *sc;
 };
+
+__device__ __forceinline__ bool gated(const Fields &F) { return F.gate && *F.gate; }
+__device__ __forceinline__ double field_dt(const Fields &F) { return F.sc ? F.sc[0] : F.dt; }
+__device__ __forceinline__ double field_beta(const Fields &F) { return F.sc ? F.sc[1] : F.beta; }
 
 __device__ __forceinline__ double dsign5(double x) { return copysign(0.5, x); }
 
@@ -118,12 +128,13 @@ __global__ __launch_bounds__(kTx) void k_density_at_nodes(Fields F)
 // i = 1..nx-1, j = 1..ny-1.
 __global__ __launch_bounds__(kTx) void k_density_at_faces(Fields F)
 {
+    if (gated(F)) return;
     const int nx = F.nx, ny = F.ny;
     const int i = blockIdx.x * kTx + threadIdx.x + 1;
     const int j = blockIdx.y + 1;
     if (i > nx - 1 || j > ny - 1) return;
     const double *ro = F.in[0], *Uce = F.in[1], *Vcn = F.in[2];
-    const double beta = F.beta;
+    const double beta = field_beta(F);
     const long np = (long)nx * (j - 1) + i - 1;
     if (j >= 2) {
         const long npe = np + 1;
@@ -185,6 +196,7 @@ __global__ __launch_bounds__(kTx) void k_v_coefficients(Fields F)
 // get_p_coefficients: mach2d_coef.f90:1207-1278 (5 diagonals S W P E N)
 __global__ __launch_bounds__(kTx) void k_p_coefficients(Fields F)
 {
+    if (gated(F)) return;
     long np;
     if (!interior(F.nx, F.ny, np)) return;
     const double *const *q = F.in;
@@ -192,7 +204,7 @@ __global__ __launch_bounds__(kTx) void k_p_coefficients(Fields F)
                  *roe = q[6], *ron = q[7], *g = q[8], *de = q[9], *dn = q[10];
     const long N = (long)F.nx * F.ny;
     const long nps = np - F.nx, npn = np + F.nx, npw = np - 1, npe = np + 1;
-    const double dt = F.dt;
+    const double dt = field_dt(F);
     const double as = dsign5(Vcn[nps]), an = dsign5(Vcn[np]);
     const double aw = dsign5(Uce[npw]), ae = dsign5(Uce[np]);
     const double roem = roe[np], rowm = roe[npw], ronm = ron[np], rosm = ron[nps];
@@ -211,6 +223,7 @@ __global__ __launch_bounds__(kTx) void k_p_coefficients(Fields F)
 // get_p_source: mach2d_coef.f90:1281-1348
 __global__ __launch_bounds__(kTx) void k_p_source(Fields F)
 {
+    if (gated(F)) return;
     long np;
     if (!interior(F.nx, F.ny, np)) return;
     const double *const *q = F.in;
@@ -218,7 +231,7 @@ __global__ __launch_bounds__(kTx) void k_p_source(Fields F)
                  *ro = q[6], *roa = q[7], *Uce = q[8], *Vcn = q[9], *de = q[10], *dn = q[11],
                  *g = q[12], *pl = q[13];
     const long nps = np - F.nx, npn = np + F.nx, npw = np - 1, npe = np + 1;
-    const double dt = F.dt;
+    const double dt = field_dt(F);
     const double as = dsign5(Vcn[nps]), an = dsign5(Vcn[np]);
     const double aw = dsign5(Uce[npw]), ae = dsign5(Uce[np]);
     const double roel = g[np] * pl[np] * (0.5 + ae) + g[npe] * pl[npe] * (0.5 - ae);
@@ -686,12 +699,14 @@ __global__ __launch_bounds__(kTx) void k_internal_simplec_coefficients(Fields F)
 // get_density_correction_with_pl (:1614, ro += pl*g).  Grid over all N.
 __global__ __launch_bounds__(kTx) void k_pressure_correction_with_pl(Fields F)
 {
+    if (gated(F)) return;
     const long q = (long)(blockIdx.y * gridDim.x + blockIdx.x) * kTx + threadIdx.x;
     if (q >= (long)F.nx * F.ny) return;
     F.out[0][q] = F.out[0][q] + F.in[0][q];
 }
 __global__ __launch_bounds__(kTx) void k_density_correction_with_pl(Fields F)
 {
+    if (gated(F)) return;
     const long q = (long)(blockIdx.y * gridDim.x + blockIdx.x) * kTx + threadIdx.x;
     if (q >= (long)F.nx * F.ny) return;
     F.out[0][q] = F.out[0][q] + F.in[0][q] * F.in[1][q];
@@ -722,6 +737,7 @@ __global__ __launch_bounds__(kTx) void k_u_v_at_real_nodes_with_pl(Fields F)
 // Grid over i = 1..nx-1, j = 1..ny-1.
 __global__ __launch_bounds__(kTx) void k_velocity_correction_at_faces_with_pl(Fields F)
 {
+    if (gated(F)) return;
     const int nx = F.nx, ny = F.ny;
     const int i = blockIdx.x * kTx + threadIdx.x + 1;
     const int j = blockIdx.y + 1;
@@ -895,6 +911,7 @@ __global__ __launch_bounds__(kTx) void k_intflow_bc_uv(Fields F)
 // written.  in: x, xp, pl, plin (ny values).  out: ap, bp.
 __global__ __launch_bounds__(kTx) void k_intflow_bc_pl(Fields F)
 {
+    if (gated(F)) return;
     const int nx = F.nx, ny = F.ny;
     int i, j;
     if (!boundary_point(nx, ny, i, j)) return;
@@ -969,6 +986,7 @@ __global__ __launch_bounds__(kTx) void k_intflow_bc_T(Fields F)
 // in/out: ue, ve, un, vn, Uce, Vcn.
 __global__ __launch_bounds__(kTx) void k_intflow_vel_bf(Fields F)
 {
+    if (gated(F)) return;
     const int nx = F.nx, ny = F.ny;
     const long q = (long)blockIdx.x * kTx + threadIdx.x;
     const long nr = ny - 2, nc = nx - 2;
@@ -1013,6 +1031,8 @@ using namespace m2d;
 namespace {
 
 thread_local hipStream_t g_stream = nullptr;
+thread_local const int *g_gate = nullptr;
+thread_local const double *g_sc = nullptr;
 
 enum Grid { kInterior = 0, kFaces = 1, kAll = 2, kBoundary = 3 };
 
@@ -1077,6 +1097,10 @@ int run(const Routine &R, Fields F)
     if (tot && hipMalloc((void **)&d, sizeof(double) * tot) != hipSuccess) return M2D_ERR_HIP;
     Fields G = F;
     hipStream_t s = dev ? g_stream : nullptr;
+    if (dev) {
+        G.gate = g_gate;
+        G.sc = g_sc;
+    }
     int rc = M2D_OK;
     long off = 0;
     if (!dev) {
@@ -1182,6 +1206,12 @@ hipStream_t m2d::assembly_stream(hipStream_t s)
 }
 
 hipStream_t m2d::assembly_stream_current() { return g_stream; }
+
+void m2d::assembly_gate(const int *gate, const double *sc)
+{
+    g_gate = gate;
+    g_sc = sc;
+}
 
 bool m2d::is_device_array(const void *p) { return is_device_ptr(p); }
 

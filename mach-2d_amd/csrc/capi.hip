@@ -68,6 +68,27 @@ struct Work {
     }
 };
 
+// Device control block of the pressure-correction loop
+// (m2d_intflow_pl_loop): the per-call scalars the captured iterations read
+// and the exit test's results.
+struct PlCtl {
+    double dt, beta;     // read by the assembly kernels (sc[0], sc[1])
+    double p_avg, tolm;  // exit test: max |pl| / p_avg < tolm
+    double mx;           // max |pl| of the latest iteration
+    double normpl;       // max |pl| / p_avg of the latest iteration that ended normally
+    int itp;             // iterations run
+    int pad;
+};
+
+// One captured batch of pressure-correction iterations and what it was
+// captured for (pointers, loop parameters and the solver's launch settings).
+struct PlGraph {
+    m2d_pl_loop_fields f;
+    int modvis, nitm, iters, xcd_stride, staged;
+    double tol;
+    hipGraphExec_t exec;
+};
+
 struct m2d_system {
     int nx = 0, ny = 0;
     long N = 0;
@@ -117,6 +138,16 @@ struct m2d_system {
     std::vector<Rec> recs;
     double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     long kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    // pressure-correction loop: control block (device) and its pinned
+    // mirror, the stream batches are captured on, the captured batches,
+    // the previous call's iteration count (the next call's first batch)
+    // and the last call's host reads and batches
+    PlCtl *pl = nullptr, *h_pl = nullptr;
+    hipStream_t cap = nullptr;
+    std::vector<PlGraph> plg;
+    bool pl_graph = true;
+    int pl_last = 0, pl_reads = 0, pl_batches = 0;
 };
 
 static hipEvent_t pool_event(m2d_system *s)
@@ -203,6 +234,10 @@ extern "C" int m2d_destroy(m2d_system *s)
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : s->pool) (void)hipEventDestroy(e);
+    for (PlGraph &g : s->plg) (void)hipGraphExecDestroy(g.exec);
+    if (s->cap) (void)hipStreamDestroy(s->cap);
+    if (s->pl) (void)hipFree(s->pl);
+    if (s->h_pl) (void)hipHostFree(s->h_pl);
     if (s->h_st) (void)hipHostFree(s->h_st);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -300,9 +335,9 @@ static int read_state(m2d_system *s, SolveState *dev)
 
 // Reset the solve state and clear the boundary words: generations restart
 // with every solve (they follow the device iteration counter).
-static void begin(m2d_system *s, SolveState *st)
+static void begin(m2d_system *s, SolveState *st, bool gated = false)
 {
-    launch_reset_state(st, s->stream);
+    launch_reset_state(st, s->stream, gated);
     launch_clear_boundary(s->bnd, s->nx, s->ny, s->stream);
 }
 
@@ -712,10 +747,12 @@ extern "C" int m2d_solve(m2d_system *s, const double *d_a, const double *d_rhs, 
 // ---------------------------------------------------------------------------
 // Device-resident pressure-correction loop of the internal-flow case
 // ---------------------------------------------------------------------------
-// max |x| over n values into out[0] (one workgroup, fixed reduction order).
+// max |x| over n values into out[0] (one workgroup, fixed reduction order);
+// no-op while *gate is set.
 __global__ __launch_bounds__(1024) void k_absmax(const double *__restrict__ x, long n,
-                                                 double *__restrict__ out)
+                                                 double *__restrict__ out, const int *gate)
 {
+    if (*gate) return;
     __shared__ double red[1024];
     double m = 0.0;
     for (long q = threadIdx.x; q < n; q += 1024) m = fmax(m, fabs(x[q]));
@@ -728,11 +765,39 @@ __global__ __launch_bounds__(1024) void k_absmax(const double *__restrict__ x, l
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// lu2d5 / lu2d9 enqueued without reading the state (errors land in it).
-static void enqueue_factor64(m2d_system *s, const double *d_a)
+// Start of a pressure-correction loop: iterations armed, none run.
+__global__ void k_pl_start(SolveState *st, PlCtl *c)
+{
+    if (threadIdx.x != 0) return;
+    st->skip = 0;
+    c->itp = 0;
+    c->mx = 0.0;
+    c->normpl = 0.0;
+}
+
+// Exit test at the end of one iteration (mach2d_main.f90:306-308): count
+// it; a solver error, or normpl = max|pl| / p_avg below tolm, ends the loop
+// (st->skip: the iterations enqueued after this one are no-ops and the
+// solver state stays that of this iteration's solve).
+__global__ void k_pl_check(SolveState *st, PlCtl *c)
+{
+    if (threadIdx.x != 0 || st->skip) return;
+    c->itp += 1;
+    const bool err = st->outside || st->err_pivot || st->abort ||
+                     st->status == M2D_ERR_NONFINITE || st->status == M2D_ERR_SYNC_TIMEOUT;
+    if (!err) c->normpl = c->mx / c->p_avg;
+    if (err || c->normpl < c->tolm) {
+        st->skip = 1;
+        st->done = 1;
+    }
+}
+
+// lu2d5 / lu2d9 enqueued without reading the state (errors land in it);
+// gated: a no-op once a loop around it has ended (st->skip).
+static void enqueue_factor64(m2d_system *s, const double *d_a, bool gated = false)
 {
     Work<double> &W = s->w64;
-    begin(s, s->st);
+    begin(s, s->st, gated);
     launch_check_outside(d_a, s->nx, s->ny, s->st, s->stream, s->nd);
     if (s->nd == 5)
         launch_to_skew5<double, double>(d_a, W.a, W.sk, s->stream);
@@ -740,7 +805,7 @@ static void enqueue_factor64(m2d_system *s, const double *d_a)
         launch_to_skew<double, double>(d_a, W.a, 9, W.sk, s->stream);
     launch_factor<double>(W.a, W.dl, W.du, 0.5, W.sk, make_link(s, s->st, 0), s->xcd_stride,
                           s->stream);
-    launch_prepare_factor<double>(W.dl, W.du, W.sk, s->stream);
+    launch_prepare_factor<double>(W.dl, W.du, W.sk, s->stream, s->st);
 }
 
 // fb2d5 / fb2d9 with at most nitm iterations, enqueued without host reads:
@@ -756,33 +821,24 @@ static void enqueue_solve64(m2d_system *s, const double *a, const double *rhs, d
         enqueue_iteration<double>(s, W, s->st, a, rhs, x, nullptr, 0, k, gamma, nitm);
 }
 
-extern "C" int m2d_intflow_pl_loop(m2d_system *s, const m2d_pl_loop_fields *f, int modvis,
-                                   double dt, double beta, double p_avg, double tolm, int itpmax,
-                                   int nitm_p, double tol_p, int *itp_out, double *normpl_out)
+// `count` iterations of the pressure-correction loop (mach2d_main.f90:261-309)
+// enqueued on s->stream with no host read: every kernel is gated by
+// s->st->skip, which the exit test of the iteration that ends the loop sets.
+// The assembly calls read dt and beta from the control block.
+static int enqueue_pl_iterations(m2d_system *s, const m2d_pl_loop_fields *f, int modvis,
+                                 int nitm_p, double tol_p, int count)
 {
-    if (!s || !f || s->nd != 5 || s->precision != M2D_FP64 || itpmax < 1 || nitm_p < 1 ||
-        !(p_avg != 0.0))
-        return M2D_ERR_ARG;
-    const void *need[] = {f->x,   f->xp,  f->xe,  f->ye,  f->xk,  f->yk,  f->rp,  f->re,
-                          f->rn,  f->Jp,  f->due, f->dve, f->dun, f->dvn, f->de,  f->dn,
-                          f->g,   f->roa, f->u,   f->v,   f->plin, f->ue, f->ve,  f->un,
-                          f->vn,  f->Uce, f->Vcn, f->roe, f->ron, f->ro,  f->p,   f->pl,
-                          f->ap,  f->bp};
-    for (const void *p : need)
-        if (!p) return M2D_ERR_ARG;
     const int nx = s->nx, ny = s->ny;
-    const long N = s->N;
-    const hipStream_t keep = assembly_stream(s->stream);
-    int rc = M2D_OK, itp = 0;
-    double normpl = 0.0;
-    // every assembly call below takes device arrays and only enqueues
+    const double dt = 0.0, beta = 0.0;  // placeholders: the kernels read s->pl
     auto ok = [&]() {
-        m2d_status st;
-        m2d_last_status(&st);
-        return st.status == M2D_OK;
+        m2d_status q;
+        m2d_last_status(&q);
+        return q.status == M2D_OK;
     };
-    HIP_OK(hipMemsetAsync(f->pl, 0, sizeof(double) * N, s->stream));
-    for (itp = 1; itp <= itpmax && !rc; ++itp) {
+    const hipStream_t keep = assembly_stream(s->stream);
+    assembly_gate(&s->st->skip, &s->pl->dt);
+    int rc = M2D_OK;
+    for (int k = 0; k < count && !rc; ++k) {
         m2d_get_p_coefficients(&nx, &ny, &dt, f->rp, f->re, f->rn, f->Jp, f->Uce, f->Vcn,
                                f->roe, f->ron, f->g, f->de, f->dn, f->ap);
         if (ok()) m2d_intflow_set_bcpl(&nx, &ny, f->x, f->xp, f->pl, f->plin, f->ap, f->bp);
@@ -793,7 +849,7 @@ extern "C" int m2d_intflow_pl_loop(m2d_system *s, const m2d_pl_loop_fields *f, i
             rc = M2D_ERR_ARG;
             break;
         }
-        enqueue_factor64(s, f->ap);
+        enqueue_factor64(s, f->ap, true);
         enqueue_solve64(s, f->ap, f->bp, f->pl, tol_p, nitm_p);
         m2d_get_velocity_correction_at_faces_with_pl(&nx, &ny, f->xe, f->ye, f->xk, f->yk,
                                                      f->due, f->dve, f->dun, f->dvn, f->de,
@@ -810,28 +866,147 @@ extern "C" int m2d_intflow_pl_loop(m2d_system *s, const m2d_pl_loop_fields *f, i
             rc = M2D_ERR_ARG;
             break;
         }
-        hipLaunchKernelGGL(k_absmax, dim3(1), dim3(1024), 0, s->stream, f->pl, N, s->r64);
-        double mx = 0.0;
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(&mx, s->r64, sizeof(double), hipMemcpyDeviceToHost, s->stream) !=
-                hipSuccess) {
+        hipLaunchKernelGGL(k_absmax, dim3(1), dim3(1024), 0, s->stream, f->pl, s->N, &s->pl->mx,
+                           &s->st->skip);
+        hipLaunchKernelGGL(k_pl_check, dim3(1), dim3(64), 0, s->stream, s->st, s->pl);
+        if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
+    }
+    assembly_gate(nullptr, nullptr);
+    assembly_stream(keep);
+    return rc;
+}
+
+// The captured batch of `count` iterations for these arguments: captured on
+// s->cap and instantiated the first time, replayed after that (the per-call
+// scalars live in the control block, so a batch serves every call with the
+// same arrays and loop parameters).
+static int pl_graph_for(m2d_system *s, const m2d_pl_loop_fields *f, int modvis, int nitm_p,
+                        double tol_p, int count, hipGraphExec_t *out)
+{
+    for (const PlGraph &g : s->plg)
+        if (!memcmp(&g.f, f, sizeof(*f)) && g.modvis == modvis && g.nitm == nitm_p &&
+            g.tol == tol_p && g.iters == count && g.xcd_stride == s->xcd_stride &&
+            g.staged == (int)s->staged) {
+            *out = g.exec;
+            return M2D_OK;
+        }
+    if (!s->cap) HIP_OK(hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking));
+    const hipStream_t user = s->stream;
+    const bool timing = s->timing;
+    s->stream = s->cap;
+    s->timing = false;
+    HIP_OK(hipStreamBeginCapture(s->cap, hipStreamCaptureModeRelaxed));
+    int rc = enqueue_pl_iterations(s, f, modvis, nitm_p, tol_p, count);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s->cap, &g);
+    s->stream = user;
+    s->timing = timing;
+    if (!rc && e != hipSuccess) rc = M2D_ERR_HIP;
+    PlGraph pg;
+    pg.f = *f;
+    pg.modvis = modvis;
+    pg.nitm = nitm_p;
+    pg.tol = tol_p;
+    pg.iters = count;
+    pg.xcd_stride = s->xcd_stride;
+    pg.staged = (int)s->staged;
+    pg.exec = nullptr;
+    if (!rc && hipGraphInstantiate(&pg.exec, g, nullptr, nullptr, 0) != hipSuccess)
+        rc = M2D_ERR_HIP;
+    if (g) (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    if (s->plg.size() >= 4) {  // a few shapes of batch at most; drop the oldest
+        (void)hipGraphExecDestroy(s->plg.front().exec);
+        s->plg.erase(s->plg.begin());
+    }
+    s->plg.push_back(pg);
+    *out = pg.exec;
+    return M2D_OK;
+}
+
+extern "C" int m2d_set_pl_graph(m2d_system *s, int use_graph)
+{
+    if (!s || (use_graph != 0 && use_graph != 1)) return M2D_ERR_ARG;
+    s->pl_graph = use_graph != 0;
+    return M2D_OK;
+}
+
+extern "C" int m2d_pl_loop_info(const m2d_system *s, int *host_reads, int *batches)
+{
+    if (!s || !host_reads || !batches) return M2D_ERR_ARG;
+    *host_reads = s->pl_reads;
+    *batches = s->pl_batches;
+    return M2D_OK;
+}
+
+extern "C" int m2d_intflow_pl_loop(m2d_system *s, const m2d_pl_loop_fields *f, int modvis,
+                                   double dt, double beta, double p_avg, double tolm, int itpmax,
+                                   int nitm_p, double tol_p, int *itp_out, double *normpl_out)
+{
+    if (!s || !f || s->nd != 5 || s->precision != M2D_FP64 || itpmax < 1 || nitm_p < 1 ||
+        !(p_avg != 0.0))
+        return M2D_ERR_ARG;
+    const void *need[] = {f->x,   f->xp,  f->xe,  f->ye,  f->xk,  f->yk,  f->rp,  f->re,
+                          f->rn,  f->Jp,  f->due, f->dve, f->dun, f->dvn, f->de,  f->dn,
+                          f->g,   f->roa, f->u,   f->v,   f->plin, f->ue, f->ve,  f->un,
+                          f->vn,  f->Uce, f->Vcn, f->roe, f->ron, f->ro,  f->p,   f->pl,
+                          f->ap,  f->bp};
+    for (const void *p : need)
+        if (!p) return M2D_ERR_ARG;
+    if (!s->pl) {
+        HIP_OK(hipMalloc((void **)&s->pl, sizeof(PlCtl)));
+        HIP_OK(hipHostMalloc((void **)&s->h_pl, sizeof(PlCtl), hipHostMallocDefault));
+    }
+    // pl = 0 (:257) and the per-call scalars, ahead of the iterations
+    PlCtl *h = s->h_pl;
+    memset(h, 0, sizeof(*h));
+    h->dt = dt;
+    h->beta = beta;
+    h->p_avg = p_avg;
+    h->tolm = tolm;
+    HIP_OK(hipMemsetAsync(f->pl, 0, sizeof(double) * s->N, s->stream));
+    HIP_OK(hipMemcpyAsync(s->pl, h, sizeof(PlCtl), hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_pl_start, dim3(1), dim3(64), 0, s->stream, s->st, s->pl);
+    HIP_OK(hipGetLastError());
+    // batches of iterations, each followed by one read of the control block
+    // and the solver state.  The first batch is as long as the previous
+    // call's loop, so a steady run reads the host once per call; the ones
+    // after it hold one iteration each (an iteration past the exit costs
+    // its launches, about 18 + 5 nitm_p of them, as no-ops).
+    int rc = M2D_OK, run = 0;
+    s->pl_reads = s->pl_batches = 0;
+    const int first = std::min(itpmax, std::max(1, s->pl_last));
+    bool ended = false;
+    while (!rc && !ended && run < itpmax) {
+        const int count = run == 0 ? first : 1;
+        if (s->pl_graph) {
+            hipGraphExec_t exec = nullptr;
+            rc = pl_graph_for(s, f, modvis, nitm_p, tol_p, count, &exec);
+            if (!rc && hipGraphLaunch(exec, s->stream) != hipSuccess) rc = M2D_ERR_HIP;
+        } else {
+            rc = enqueue_pl_iterations(s, f, modvis, nitm_p, tol_p, count);
+        }
+        if (rc) break;
+        run += count;
+        ++s->pl_batches;
+        if (hipMemcpyAsync(h, s->pl, sizeof(PlCtl), hipMemcpyDeviceToHost, s->stream) !=
+            hipSuccess) {
             rc = M2D_ERR_HIP;
             break;
         }
-        rc = read_state(s, s->st);  // also completes the copy of mx
+        rc = read_state(s, s->st);  // also completes the copy of the control block
+        ++s->pl_reads;
         if (rc) break;
         const SolveState &st = *s->h_st;
         if (st.outside) rc = M2D_ERR_OUTSIDE_COUPLING;
         else if (st.err_pivot || st.status == M2D_ERR_NONFINITE) rc = M2D_ERR_NONFINITE;
         else if (st.abort || st.status == M2D_ERR_SYNC_TIMEOUT) rc = M2D_ERR_SYNC_TIMEOUT;
-        if (rc) break;
-        normpl = mx / p_avg;
-        if (normpl < tolm) break;
+        ended = st.skip != 0;
     }
-    assembly_stream(keep);
-    if (itp > itpmax) itp = itpmax;
+    const int itp = std::min(std::max(h->itp, 1), itpmax);
+    if (!rc) s->pl_last = h->itp;
     if (itp_out) *itp_out = itp;
-    if (normpl_out) *normpl_out = normpl;
+    if (normpl_out) *normpl_out = h->normpl;
     set_last(rc);
     return rc;
 }

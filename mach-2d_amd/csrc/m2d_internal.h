@@ -24,7 +24,9 @@ struct SolveState {
     int outside;     // number of non-zero couplings to outside neighbours
     int outer;       // refinement steps (mixed precision)
     int iter;        // iterations completed (drives the launch generations)
-    int pad0;
+    int skip;        // set when a loop enqueued ahead of its exit test has
+                     // ended: the gated reset keeps the state, the
+                     // factorization and every kernel of the loop no-op
     double r0;       // ||r0||
     double gammarp;  // gamma * ||r0||
     double rn;       // latest ||r||
@@ -57,7 +59,9 @@ template <typename TI, typename TO>
 void launch_to_skew5(const TI *src, TO *dst, const Skew &sk, hipStream_t s);
 // dl: 6 arrays (dl1..dl5, then 1/dl1, written here); padding made finite.
 template <typename T>
-void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s);
+// st (optional): no-op while st->skip is set
+void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s,
+                           const SolveState *st = nullptr);
 template <typename T>
 void launch_from_skew(const T *src, T *dst, int nk, const Skew &sk, hipStream_t s);
 template <typename T>
@@ -92,12 +96,18 @@ void launch_norm_finalize(const double *partial, int np, SolveState *st, double 
                           int hist_cap, int initial, double gamma, int nitm, hipStream_t s);
 void launch_check_outside(const double *a, int nx, int ny, SolveState *st, hipStream_t s,
                           int nd = 9);
-void launch_reset_state(SolveState *st, hipStream_t s);
+// gated: leave the state as it is when st->skip is set (a plain reset
+// clears skip too)
+void launch_reset_state(SolveState *st, hipStream_t s, bool gated = false);
 
 // ---- coefficient assembly (assembly.hip) -------------------------------------
 // Stream of this thread's device-array assembly calls; returns the previous one.
 hipStream_t assembly_stream(hipStream_t s);
 hipStream_t assembly_stream_current();
+// This thread's device-array assembly calls pass gate and sc to their
+// kernels (nullptr, nullptr: off): a kernel returns at once while *gate is
+// non-zero, and reads dt = sc[0], beta = sc[1] from device memory.
+void assembly_gate(const int *gate, const double *sc);
 // true for device memory (false for host memory and unknown pointers)
 bool is_device_array(const void *p);
 template <typename TI, typename TO>

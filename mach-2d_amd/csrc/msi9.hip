@@ -127,6 +127,7 @@ __global__ __launch_bounds__(256) void k_norm_finalize(const double *__restrict_
 __global__ __launch_bounds__(256) void k_check_outside(const double *__restrict__ a, int nx,
                                                        int ny, SolveState *st, int nd)
 {
+    if (st->skip) return;
     const long N = (long)nx * ny;
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
     int bad = 0;
@@ -148,9 +149,11 @@ __global__ __launch_bounds__(256) void k_check_outside(const double *__restrict_
     }
 }
 
-__global__ void k_reset_state(SolveState *st)
+__global__ void k_reset_state(SolveState *st, int gated)
 {
     if (threadIdx.x == 0) {
+        if (gated && st->skip) return;
+        st->skip = 0;
         st->done = 0;
         st->nit = 0;
         st->status = 0;
@@ -206,9 +209,9 @@ void launch_check_outside(const double *a, int nx, int ny, SolveState *st, hipSt
                        nx, ny, st, nd);
 }
 
-void launch_reset_state(SolveState *st, hipStream_t s)
+void launch_reset_state(SolveState *st, hipStream_t s, bool gated)
 {
-    hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(64), 0, s, st);
+    hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(64), 0, s, st, gated ? 1 : 0);
 }
 
 template <typename TI, typename TO>

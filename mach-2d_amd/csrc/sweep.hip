@@ -1037,7 +1037,7 @@ __global__ __launch_bounds__(kWave) void k_factor(const T *__restrict__ a, T *__
                                                   int xcd_stride)
 {
     const int blk = blockIdx.x;
-    if (blk % xcd_stride) return;
+    if (blk % xcd_stride || lk0.st->skip) return;
     Link lk = lk0;
     lk.gen = launch_gen(lk0);
     factor_band<T, F>(a, dl, du, alpha0, sk, lk, blk / xcd_stride);
@@ -1208,8 +1208,9 @@ __global__ __launch_bounds__(256) void k_to_skew5(const TI *__restrict__ src, TO
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_prepare_factor(T *__restrict__ dl, T *__restrict__ du,
-                                                        Skew sk)
+                                                        Skew sk, const SolveState *st)
 {
+    if (st && st->skip) return;
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     if (e < sk.size()) prepare_entry<T>(dl, du, sk, e);
 }
@@ -1338,10 +1339,10 @@ void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride
 }
 
 template <typename T>
-void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s)
+void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s, const SolveState *st)
 {
     hipLaunchKernelGGL(k_prepare_factor<T>, dim3(blocks_for(sk.size())), dim3(256), 0, s, dl, du,
-                       sk);
+                       sk, st);
 }
 
 template <typename T>
@@ -1426,7 +1427,8 @@ void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipSt
     template void launch_backward<T>(T *, const T *, const Skew &, Link, int, hipStream_t, bool, \
                                      bool);                                                   \
     template void launch_from_skew<T>(const T *, T *, int, const Skew &, hipStream_t);          \
-    template void launch_prepare_factor<T>(T *, T *, const Skew &, hipStream_t);                \
+    template void launch_prepare_factor<T>(T *, T *, const Skew &, hipStream_t,                 \
+                                           const SolveState *);                               \
     template void launch_factor_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t);  \
     template void launch_forward_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t,  \
                                           bool, bool);                                        \

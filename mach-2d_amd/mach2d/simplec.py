@@ -56,6 +56,23 @@ def intflow_pl_loop(dev, f, modvis, dt, beta, p_avg, tolm, itpmax=20, nitm_p=20,
     return itp.value, normpl.value
 
 
+def set_pl_graph(dev, use_graph):
+    """1: replay the loop's batches from captured HIP graphs (the default);
+    0: enqueue the same kernels directly (m2d_set_pl_graph)."""
+    rc = _lib.load().m2d_set_pl_graph(dev.h, int(use_graph))
+    if rc:
+        raise SolverError(rc, "m2d_set_pl_graph")
+
+
+def pl_loop_info(dev):
+    """(host reads, batches) of the last intflow_pl_loop call on `dev`."""
+    reads, batches = C.c_int(), C.c_int()
+    rc = _lib.load().m2d_pl_loop_info(dev.h, C.byref(reads), C.byref(batches))
+    if rc:
+        raise SolverError(rc, "m2d_pl_loop_info")
+    return reads.value, batches.value
+
+
 # ---------------------------------------------------------------------------
 # One internal-flow time step (src/mach2d_main.f90:94-404), m2d_intflow_time_step
 # ---------------------------------------------------------------------------

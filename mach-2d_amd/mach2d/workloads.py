@@ -166,9 +166,10 @@ def batch_split(mode, count, groups, layout, inner_nitm=50):
             "solves_per_s": round(count / dt, 2), "iterations_max": its, "status": rc}
 
 
-def pl_loop(nx, ny, itpmax, reps):
+def pl_loop(nx, ny, itpmax, reps, pl_graph=1):
     """The internal-flow pressure-correction loop (m2d_intflow_pl_loop) on an
-    nx x ny grid with seeded fields: ms per call and per itp iteration."""
+    nx x ny grid with seeded fields: ms per call and per itp iteration (the
+    same arrays every call, reset to the seeded fields in between)."""
     import numpy as np
     import torch
 
@@ -184,10 +185,13 @@ def pl_loop(nx, ny, itpmax, reps):
     f["bp"] = np.zeros(nx * ny)
     p_avg = float(np.mean(f["p"]))
     dev = DeviceMSI(nx, ny, "fp64", ndiag=5)
+    simplec.set_pl_graph(dev, pl_graph)
     base = {k: torch.tensor(np.ascontiguousarray(v), device="cuda") for k, v in f.items()}
+    t = {k: v.clone() for k, v in base.items()}
     out = []
     for r in range(reps + 1):
-        t = {k: v.clone() for k, v in base.items()}
+        for k, v in base.items():
+            t[k].copy_(v)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         itp, normpl = simplec.intflow_pl_loop(dev, t, modvis=0, dt=2e-3, beta=0.3, p_avg=p_avg,
@@ -195,10 +199,12 @@ def pl_loop(nx, ny, itpmax, reps):
         torch.cuda.synchronize()
         if r:
             out.append((time.perf_counter() - t0) * 1e3)
+    reads = simplec.pl_loop_info(dev)[0]
     dev.close()
     ms = sorted(out)[len(out) // 2]
-    return {"workload": "pl_loop", "nx": nx, "ny": ny, "itp": itp, "ms": round(ms, 3),
-            "ms_per_itp": round(ms / itp, 4), "normpl": normpl}
+    return {"workload": "pl_loop", "nx": nx, "ny": ny, "pl_graph": pl_graph, "itp": itp,
+            "ms": round(ms, 3), "ms_per_itp": round(ms / itp, 4), "normpl": normpl,
+            "host_reads": reads}
 
 
 def tdma(nx, ny, nitm, count, reps):
@@ -225,9 +231,11 @@ def tdma(nx, ny, nitm, count, reps):
             "ms": round(ms, 4), "ms_per_iteration": round(ms / nitm, 4)}
 
 
-def time_step(solver9d, steps):
+def time_step(solver9d, steps, pl_graph=1):
     """The internal-flow time step (m2d_intflow_time_step) on the 258x130
-    nozzle from inlet Mach 0.2, `steps` steps after one warm-up step."""
+    nozzle from inlet Mach 0.2, `steps` steps after one warm-up step;
+    pl_graph 0 enqueues the p' loop's kernels directly instead of replaying
+    captured graphs."""
     import numpy as np
     import torch
 
@@ -244,6 +252,7 @@ def time_step(solver9d, steps):
          for k, v in f0.items()}
     dev9 = DeviceMSI(nx, ny) if solver9d == 0 else None
     dev5 = DeviceMSI(nx, ny, ndiag=5)
+    simplec.set_pl_graph(dev5, pl_graph)
     torch.cuda.synchronize()
     dt = c["dt0"]
     dt, r = simplec.intflow_time_step(dev9, dev5, d, prm, dt)
@@ -253,8 +262,9 @@ def time_step(solver9d, steps):
         dt, r = simplec.intflow_time_step(dev9, dev5, d, prm, dt)
         itps += r["itp"]
     ms = (time.perf_counter() - t0) * 1e3 / max(steps, 1)
-    return {"workload": "time_step", "solver9d": solver9d, "steps": steps,
-            "ms_per_step": round(ms, 3), "itp_per_step": round(itps / max(steps, 1), 2)}
+    return {"workload": "time_step", "solver9d": solver9d, "pl_graph": pl_graph, "steps": steps,
+            "ms_per_step": round(ms, 3), "itp_per_step": round(itps / max(steps, 1), 2),
+            "pl_host_reads": simplec.pl_loop_info(dev5)[0]}
 
 
 def main(argv=None):
@@ -269,6 +279,7 @@ def main(argv=None):
     ap.add_argument("--mode", default="fp32_ir", choices=["fp64", "fp32_ir"])
     ap.add_argument("--solves", type=int, default=1)
     ap.add_argument("--staged", type=int, default=0)
+    ap.add_argument("--pl-graph", dest="pl_graph", type=int, default=1)
     ap.add_argument("--count", type=int, default=256)
     ap.add_argument("--same-1026x514", dest="same", action="store_true")
     ap.add_argument("--layout", type=int, default=None)
@@ -279,11 +290,11 @@ def main(argv=None):
         out = single(args.mode, args.solves, bool(args.staged), args.inner_gamma,
                      args.inner_nitm or 25)
     elif args.what == "pl_loop":
-        out = pl_loop(258, 130, 20, max(1, args.solves))
+        out = pl_loop(258, 130, 20, max(1, args.solves), args.pl_graph)
     elif args.what == "tdma":
         out = tdma(args.nx, args.ny, args.nitm, args.count, max(1, args.solves))
     elif args.what == "time_step":
-        out = time_step(args.solver9d, max(1, args.solves))
+        out = time_step(args.solver9d, max(1, args.solves), args.pl_graph)
     elif args.what == "split":
         out = batch_split(args.mode, args.count, args.groups,
                           0 if args.layout is None else args.layout, args.inner_nitm or 50)

@@ -2,7 +2,11 @@
 internal-flow case (m2d_intflow_pl_loop, src/mach2d_main.f90:257-309)
 against the same loop composed from the oracle's routines
 (tests/pl_loop_oracle.py): every updated field bit for bit, the same number
-of itp iterations and the same max|pl| / p_avg."""
+of itp iterations and the same max|pl| / p_avg.  The loop's batches are
+replayed from captured HIP graphs: replays with eager work between them
+equal the directly enqueued kernels bit for bit, an exit inside a batch
+turns the rest of it into no-ops, and a steady run reads the host once per
+call."""
 import numpy as np
 import pytest
 
@@ -55,3 +59,69 @@ def test_pl_loop_exit_test(gpu, orc):
     assert itp == itp0 == k
     for key in want:
         assert np.array_equal(got[key], want[key]), key
+
+
+def _tensors(f):
+    import torch
+    return {k: torch.tensor(v.reshape(-1), device="cuda") for k, v in f.items()}
+
+
+def test_pl_loop_graph_replays_equal_direct_launches(gpu):
+    """Three calls in a row on one handle replaying captured batches, and on
+    another enqueueing the same kernels directly, with eager torch work on
+    the fields and a new dt between the calls: every field equal bit for bit
+    after every call.  The first call (no history) reads the host after each
+    of its itpmax one-iteration batches; the later ones replay one batch of
+    itpmax iterations and read once."""
+    import torch
+
+    from mach2d import simplec
+    nx, ny = 70, 38
+    f = PL.loop_fields(nx, ny, seed=5)
+    p_avg = float(np.mean(f["p"]))
+    kw = dict(modvis=0, beta=0.3, p_avg=p_avg, tolm=0.0, itpmax=3, nitm_p=20, tol_p=1e-2)
+    devs = [gpu.DeviceMSI(nx, ny, "fp64", ndiag=5) for _ in range(2)]
+    simplec.set_pl_graph(devs[1], 0)
+    ts = [_tensors(f), _tensors(f)]
+    for call, dt in enumerate((2e-3, 1.7e-3, 2.3e-3)):
+        res = []
+        for dev, t in zip(devs, ts):
+            res.append(simplec.intflow_pl_loop(dev, t, dt=dt, **kw))
+            torch.cuda.synchronize()
+        assert res[0] == res[1] and res[0][0] == 3, res
+        for k in PL.INOUT + ("ap", "bp"):
+            assert torch.equal(ts[0][k], ts[1][k]), (call, k)
+        reads, batches = simplec.pl_loop_info(devs[0])
+        assert (reads, batches) == ((3, 3) if call == 0 else (1, 1)), (call, reads, batches)
+        for t in ts:  # eager work between the replays
+            t["p"].mul_(1.0 + 1e-3)
+            t["ro"].mul_(1.0 - 1e-3)
+        torch.cuda.synchronize()
+    for d in devs:
+        d.close()
+
+
+def test_pl_loop_exit_inside_a_batch(gpu, orc):
+    """After a call that ran 6 iterations, the next call replays a batch of 6;
+    its exit test fires at iteration k < 6 and the rest of the batch must
+    leave every field as the oracle's loop leaves it at k."""
+    from mach2d import simplec
+    nx, ny = 40, 22
+    f = PL.loop_fields(nx, ny, seed=3)
+    p_avg = float(np.mean(f["p"]))
+    args = (0, 2e-3, 0.3, p_avg)
+    seq = [PL.run(orc, f, nx, ny, *args, 0.0, k, 20, 1e-2)[2] for k in range(1, 6)]
+    k = next(k for k in range(2, 6) if seq[k - 1] < min(seq[:k - 1]))
+    tolm = 0.5 * (seq[k - 1] + min(seq[:k - 1]))
+    want, itp0, n0 = PL.run(orc, f, nx, ny, *args, tolm, 10, 20, 1e-2)
+    dev = gpu.DeviceMSI(nx, ny, "fp64", ndiag=5)
+    kw = dict(modvis=0, dt=2e-3, beta=0.3, p_avg=p_avg, itpmax=10, nitm_p=20, tol_p=1e-2)
+    assert simplec.intflow_pl_loop(dev, _tensors(f), tolm=0.0, **dict(kw, itpmax=6))[0] == 6
+    t = _tensors(f)
+    itp, normpl = simplec.intflow_pl_loop(dev, t, tolm=tolm, **kw)
+    assert simplec.pl_loop_info(dev) == (1, 1)
+    assert itp == itp0 == k and normpl == n0
+    for key in want:
+        got = t[key].cpu().numpy()
+        assert np.array_equal(got.reshape(want[key].shape), want[key]), key
+    dev.close()
