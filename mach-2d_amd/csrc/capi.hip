@@ -62,8 +62,12 @@ struct Work {
                   hipMalloc((void **)&dl, 6 * z) == hipSuccess &&
                   hipMalloc((void **)&du, 4 * z) == hipSuccess &&
                   hipMalloc((void **)&r, z) == hipSuccess && hipMalloc((void **)&w, z) == hipSuccess;
-        if (ok)  // padding entries are never consumed; start them finite
-            ok = hipMemset(r, 0, z) == hipSuccess && hipMemset(w, 0, z) == hipSuccess;
+        // padding entries are never consumed; start them finite.  dl5 and
+        // du4 stay zero for a 5-diagonal system (its factorization writes
+        // neither)
+        if (ok)
+            ok = hipMemset(r, 0, z) == hipSuccess && hipMemset(w, 0, z) == hipSuccess &&
+                 hipMemset(dl, 0, 6 * z) == hipSuccess && hipMemset(du, 0, 4 * z) == hipSuccess;
         return ok;
     }
 };
@@ -351,24 +355,24 @@ extern "C" int m2d_factor(m2d_system *s, const double *d_a)
         if (s->nd == 5)
             launch_to_skew5<double, double>(d_a, W.a, W.sk, s->stream);
         else
-            launch_to_skew<double, double>(d_a, W.a, 9, W.sk, s->stream);
+            launch_to_skew<double, double>(d_a, W.a, 9, W.sk, s->stream, kP);
         timed(s, KT_FACTOR, 0, [&] {
             launch_factor<double>(W.a, W.dl, W.du, 0.5, W.sk, make_link(s, s->st, 0), s->xcd_stride,
-                                  s->stream);
+                                  s->stream, s->nd == 5);
         });
-        launch_prepare_factor<double>(W.dl, W.du, W.sk, s->stream);
+        launch_prepare_factor<double>(W.dl, W.du, W.sk, s->stream, s->st);
     } else {
         Work<float> &W = s->w32;
         launch_convert<double, float>(d_a, s->a32, (long)s->nd * s->N, s->stream);
         if (s->nd == 5)
             launch_to_skew5<double, float>(d_a, W.a, W.sk, s->stream);
         else
-            launch_to_skew<double, float>(d_a, W.a, 9, W.sk, s->stream);
+            launch_to_skew<double, float>(d_a, W.a, 9, W.sk, s->stream, kP);
         timed(s, KT_FACTOR + KT_FP32, 0, [&] {
             launch_factor<float>(W.a, W.dl, W.du, 0.5f, W.sk, make_link(s, s->st, 0),
-                                 s->xcd_stride, s->stream);
+                                 s->xcd_stride, s->stream, s->nd == 5);
         });
-        launch_prepare_factor<float>(W.dl, W.du, W.sk, s->stream);
+        launch_prepare_factor<float>(W.dl, W.du, W.sk, s->stream, s->st);
     }
     HIP_OK(hipGetLastError());
     int rc = read_state(s, s->st);
@@ -802,9 +806,9 @@ static void enqueue_factor64(m2d_system *s, const double *d_a, bool gated = fals
     if (s->nd == 5)
         launch_to_skew5<double, double>(d_a, W.a, W.sk, s->stream);
     else
-        launch_to_skew<double, double>(d_a, W.a, 9, W.sk, s->stream);
+        launch_to_skew<double, double>(d_a, W.a, 9, W.sk, s->stream, kP);
     launch_factor<double>(W.a, W.dl, W.du, 0.5, W.sk, make_link(s, s->st, 0), s->xcd_stride,
-                          s->stream);
+                          s->stream, s->nd == 5);
     launch_prepare_factor<double>(W.dl, W.du, W.sk, s->stream, s->st);
 }
 

@@ -42,8 +42,9 @@ long scratch_words(int ny);
 Skew skew_for(int nx, int ny, int value_bytes);
 void launch_clear_boundary(unsigned long long *bnd, int nx, int ny, hipStream_t s);
 template <typename T>
+// five: lu2d5 on the embedding (A's S, W, P, E, N arrays; dl1..dl4, du1..du3)
 void launch_factor(const T *a_sk, T *dl, T *du, T alpha0, const Skew &sk, Link lk,
-                   int xcd_stride, hipStream_t s);
+                   int xcd_stride, hipStream_t s, bool five = false);
 // five: the 5-diagonal sweeps (msi2d5) on the same arrays (dl5 and du4 unused)
 template <typename T>
 void launch_forward(const T *r, const T *dl, T *w, const Skew &sk, Link lk, int xcd_stride,
@@ -52,16 +53,19 @@ template <typename T>
 void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s,
                      bool staged, bool five = false);
 template <typename TI, typename TO>
-void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t s);
+// unit: array whose padding entries get one (the diagonal of A), or -1
+void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t s,
+                    int unit = -1);
 // 5 diagonals (S, W, P, E, N), reference layout -> the 9 skewed arrays of
 // their zero-corner embedding (SW, SE, NW, NE zero)
 template <typename TI, typename TO>
 void launch_to_skew5(const TI *src, TO *dst, const Skew &sk, hipStream_t s);
 // dl: 6 arrays (dl1..dl5, then 1/dl1, written here); padding made finite.
 template <typename T>
-// st (optional): no-op while st->skip is set
+// st: a zero or non-finite pivot sets st->err_pivot; no-op while st->skip
+// is set (nullptr: neither)
 void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s,
-                           const SolveState *st = nullptr);
+                           SolveState *st = nullptr);
 template <typename T>
 void launch_from_skew(const T *src, T *dst, int nk, const Skew &sk, hipStream_t s);
 template <typename T>

@@ -90,6 +90,64 @@ M2D_HD T forward_point_y(T r, T dl1, T y, T dl2, T dl3, T dl4, T dl5, T w_w, T w
     return div_by_rcp<T>(-(r + dl2 * w_w + dl3 * w_se + dl4 * w_s + dl5 * w_sw), dl1, y);
 }
 
+// factor_point with the four quotients by dl1 taken through y = 1/dl1
+// (div_by_rcp: each equals the division, up to the sign of a zero quotient):
+// one division where there were four.
+template <typename T>
+M2D_HD void factor_point_y(const T *b, T alpha, const T *du_w, const T *du_s,
+                           const T *du_se, const T *du_sw, T *dl, T *du)
+{
+    const T one = T(1), two = T(2);
+    T dl5 = b[kSW];
+    T dl4 = (b[kS] - dl5 * du_sw[0] - alpha * b[kSE] * du_se[0]) /
+            (one - alpha * du_s[0] * du_se[0]);
+    T dl3 = b[kSE] - dl4 * du_s[0];
+    T dl2 = (b[kW] - dl4 * du_s[1] - dl5 * du_sw[2] - two * alpha * dl5 * du_sw[1]) /
+            (one + two * alpha * du_w[1]);
+    T phi1 = dl3 * du_se[0];
+    T phi2 = dl5 * du_sw[1];
+    T phi3 = dl3 * du_se[3];
+    T phi4 = dl2 * du_w[1];
+    T dl1 = b[kP] - dl2 * du_w[0] - dl3 * du_se[1] - dl4 * du_s[2] - dl5 * du_sw[3] +
+            alpha * (two * phi1 + phi2 + phi3 + two * phi4);
+    const T y = one / dl1;
+    du[0] = div_by_rcp<T>(b[kE] - dl3 * du_se[2] - dl4 * du_s[3] - two * alpha * (phi1 + phi3),
+                          dl1, y);
+    du[1] = div_by_rcp<T>(b[kNW] - dl2 * du_w[2], dl1, y);
+    du[2] = div_by_rcp<T>(b[kN] - dl2 * du_w[3] - alpha * phi4, dl1, y);
+    du[3] = div_by_rcp<T>(b[kNE], dl1, y);
+    dl[0] = dl1;
+    dl[1] = dl2;
+    dl[2] = dl3;
+    dl[3] = dl4;
+    dl[4] = dl5;
+}
+
+// lu2d5's point (msi2d5-v03.f90:287-295) with the three quotients by dl1
+// taken through y = 1/dl1.  b[5]: S, W, P, E, N; du_w/du_s/du_se: the U
+// entries (E, NW, N) of the W, S and SE neighbours (zeros outside the grid).
+// Out: dl[4] (P, W, SE, S) and du[3] (E, NW, N).
+template <typename T>
+M2D_HD void factor_point5_y(const T *b, T alpha, const T *du_w, const T *du_s,
+                            const T *du_se, T *dl, T *du)
+{
+    const T one = T(1), two = T(2);
+    T dl4 = b[0] / (one - alpha * du_s[0] * du_se[0]);
+    T dl3 = -dl4 * du_s[0];
+    T dl2 = (b[1] - dl4 * du_s[1]) / (one + two * alpha * du_w[1]);
+    T phi1 = dl3 * du_se[0];
+    T phi4 = dl2 * du_w[1];
+    T dl1 = b[2] - dl2 * du_w[0] - dl3 * du_se[1] - dl4 * du_s[2] + two * alpha * (phi1 + phi4);
+    const T y = one / dl1;
+    du[0] = div_by_rcp<T>(b[3] - dl3 * du_se[2] - two * alpha * phi1, dl1, y);
+    du[1] = -div_by_rcp<T>(dl2 * du_w[2], dl1, y);
+    du[2] = div_by_rcp<T>(b[4] - alpha * phi4, dl1, y);
+    dl[0] = dl1;
+    dl[1] = dl2;
+    dl[2] = dl3;
+    dl[3] = dl4;
+}
+
 // Forward substitution of the FP32 sweeps (inner solver of the FP64
 // refinement, where only the operator, not the rounding, must be the
 // reference's): the row is pre-scaled by ny = -1/dl1 (ne_k = ny * dl_k), so

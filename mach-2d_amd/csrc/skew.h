@@ -47,6 +47,12 @@ struct Skew {
     {
         return (((long)b * nch + u / S) * 64 + l) * S + u % S;
     }
+    // index for a chunk length known at compile time (S_ == S)
+    template <int S_>
+    SKEW_HD long index_c(int b, int l, int u) const
+    {
+        return (((long)b * nch + (long)((unsigned)u / S_)) * 64 + l) * S_ + (long)((unsigned)u % S_);
+    }
     SKEW_HD long chunk_base(int b, int c, int l) const { return (((long)b * nch + c) * 64 + l) * S; }
     SKEW_HD long of_point(int i, int j) const
     {

@@ -39,6 +39,8 @@
 // under in-order workgroup dispatch, and every wait is bounded.
 #include <math.h>
 
+#include <type_traits>
+
 #include "batch.h"
 #include "m2d_internal.h"
 #include "msi_stencil.h"
@@ -839,34 +841,70 @@ __device__ __forceinline__ void backward_band_staged(T *__restrict__ wz, const T
 // --------------------------------------------------------------------------
 // lu2d9 factorization: A (skewed copy) -> dl, du (skewed).  The edge row
 // carries the four du values of each column.
+//
+// The skewed copy of A holds a unit diagonal at padding entries (columns
+// outside the grid, rows beyond the last one), where alpha is 0 as well, so
+// a padding point's factor is dl1 = 1 and every other entry a zero: its du
+// values are the zeros the reference's edge formulas assume for a missing
+// neighbour, and the step needs no masks.  Pivots are checked afterwards,
+// by prepare_entry.  Per step the wave issues the point's arithmetic, three
+// divisions (dl4, dl2, 1/dl1; the four du quotients through 1/dl1) and two
+// lane shifts per du value; the band below's edge values reach lane 0
+// through LDS once per chunk, and the edge row goes out the same way.
 // --------------------------------------------------------------------------
+// this band's last row over the F columns of a chunk, on its way from lane
+// 63 to lanes 0..F-1 (which publish one column each)
 template <typename T, int F>
+struct FactorEdge {
+    T out[4][F];
+};
+
+// fn(integral_constant<int, I>) for I = 0 .. N-1: the step index as a
+// compile-time constant (DPP controls are immediates)
+template <int I, int N, typename Fn>
+__device__ __forceinline__ void static_for(Fn &&fn)
+{
+    if constexpr (I < N) {
+        fn(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(fn);
+    }
+}
+
+template <typename T, int F, bool Five>
 __device__ __forceinline__ void factor_band(const T *__restrict__ a, T *__restrict__ dl,
                                             T *__restrict__ du, T alpha0, const Skew &sk,
                                             const Link &lk, int band)
 {
+    // Five (lu2d5): the S, W, P, E, N arrays of the skewed 9-array copy in,
+    // dl1..dl4 and du1..du3 out (dl5 and du4 of the embedding are zero and
+    // are neither read nor written)
+    constexpr int NA = Five ? 5 : 9, NL = Five ? 4 : 5, NU = Five ? 3 : 4;
+    constexpr int S = 2 * F;  // skew chunk: 64 bytes per lane and array
+    __shared__ FactorEdge<T, F> E;
     const int lane = threadIdx.x;
-    const int nx = sk.nx, ny = sk.ny, pad = sk.pad(band), rows = sk.rows(band);
+    const int nx = sk.nx, ny = sk.ny, pad = sk.pad(band);
     const int nf = sk.nst / F;
     const long L = sk.size();
     const int j = band * kWave + lane;
-    const bool hs = j > 0, has_up = band > 0, has_down = band < sk.nb - 1;
+    const bool has_up = band > 0, has_down = band < sk.nb - 1;
     constexpr int NW = TagWords<T>::n;
     constexpr int CW = 4 * NW;
     const u64 *bin = lk.bnd + (long)(has_up ? band - 1 : 0) * nx * CW;
     u64 *bout = lk.bnd + (long)band * nx * CW;
     u64 *scratch = lk.scratch + ((long)band * kWave + lane) * 8;
+    // alpha of this row's real volumes (alpha_at: 1 <= i <= nx-2 as well)
+    const T arow = (j >= 1 && j <= ny - 2) ? alpha0 : T(0);
+    auto aslot = [](int k) { return Five ? (k == 0 ? kS : k == 1 ? kW : k == 2 ? kP : k == 3 ? kE : kN) : k; };
 
-    T A[9][F], B[9][F];
+    T A[NA][F], B[NA][F];
     Raw<T> bn[4] = {};
-    T bv[4][F + 1];
-    T cur[4] = {0, 0, 0, 0}, se[4] = {0, 0, 0, 0}, sv[4] = {0, 0, 0, 0}, sw[4] = {0, 0, 0, 0};
-    int bad = 0;
+    T val[NU];
+    T cur[NU] = {}, se[NU] = {}, sv[NU] = {}, sw[NU] = {};
 
-    auto load_data = [&](T(&buf)[9][F], int c) {
-        const long e = sk.index(band, lane, min(c, nf - 1) * F);
+    auto load_data = [&](T(&buf)[NA][F], int c) {
+        const long e = sk.template index_c<S>(band, lane, min(c, nf - 1) * F);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) vload<T, F>(buf[k], a + k * L + e);
+        for (int k = 0; k < NA; ++k) vload<T, F>(buf[k], a + aslot(k) * L + e);
     };
     auto bcol = [&](int c) { return c * F - pad + lane; };
     auto bneed = [&](int c) {
@@ -876,102 +914,102 @@ __device__ __forceinline__ void factor_band(const T *__restrict__ a, T *__restri
     auto load_bnd = [&](Raw<T>(&b)[4], int c) {
         const u64 *p = bin + (long)min(max(bcol(c), 0), nx - 1) * CW;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) raw_load<T>(b[k], p + k * NW);
+        for (int k = 0; k < NU; ++k) raw_load<T>(b[k], p + k * NW);
     };
-    auto ok4 = [&](const Raw<T>(&b)[4]) {
+    auto okall = [&](const Raw<T>(&b)[4]) {
         bool ok = true;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) ok = ok && raw_ok<T>(b[k], lk.gen);
+        for (int k = 0; k < NU; ++k) ok = ok && raw_ok<T>(b[k], lk.gen);
         return ok;
     };
     auto ready = [&](Raw<T>(&b)[4], int c) -> bool {
-        if (!has_up || __all(!bneed(c) || ok4(b))) return true;
+        if (!has_up || __all(!bneed(c) || okall(b))) return true;
         Raw<T> t[4] = {};
-        const bool ok = slow_wait(lk, [&] { return !bneed(c) || ok4(t); }, [&] { load_bnd(t, c); });
+        const bool ok =
+            slow_wait(lk, [&] { return !bneed(c) || okall(t); }, [&] { load_bnd(t, c); });
 #pragma unroll
-        for (int k = 0; k < 4; ++k) b[k] = t[k];
+        for (int k = 0; k < NU; ++k) b[k] = t[k];
         return ok;
     };
-    auto take = [&](const Raw<T>(&b)[4]) {
+    // lane q <= F holds column c*F - pad + q of the band below (zero where
+    // there is none); at step s lane 0 takes lane s+1's value
+    auto take = [&](const Raw<T>(&b)[4], int c) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const T val = raw_value(b[k]);
-#pragma unroll
-            for (int q = 0; q <= F; ++q) bv[k][q] = read_lane(val, q);
-        }
+        for (int k = 0; k < NU; ++k) val[k] = bneed(c) ? raw_value(b[k]) : T(0);
     };
-    auto chunk = [&](T(&cb)[9][F], T(&nbuf)[9][F], int c) -> bool {
+    // the values of steps h*H .. h*H+H-1 (H = F/2: 16-byte halves) go out as
+    // soon as they exist, so only half a chunk of results is held
+    auto put_half = [&](const T(&oL)[NL][F], const T(&oU)[NU][F], long e, int h) {
+        constexpr int H = F / 2;
+#pragma unroll
+        for (int k = 0; k < NL; ++k)
+            vstore<T, H>(dl + k * L + e + h * H, reinterpret_cast<const T(&)[H]>(oL[k][h * H]));
+#pragma unroll
+        for (int k = 0; k < NU; ++k)
+            vstore<T, H>(du + k * L + e + h * H, reinterpret_cast<const T(&)[H]>(oU[k][h * H]));
+    };
+    auto chunk = [&](T(&cb)[NA][F], T(&nbuf)[NA][F], int c) -> bool {
         load_data(nbuf, c + 1);
         load_bnd(bn, c + 1);
-        T oL[5][F], oU[4][F];
+        T oL[NL][F], oU[NU][F];
+        const int i0 = c * F - pad - 2 * lane;
+        const long e = sk.template index_c<S>(band, lane, c * F);
+        static_for<0, F>([&](auto St) {
+            constexpr int s = decltype(St)::value;
+            static_assert(s + 1 <= 15, "lane 0's edge value comes by row_shl");
 #pragma unroll
-        for (int s = 0; s < F; ++s) {
-            const int i = c * F + s - pad - 2 * lane;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const T up = lane_from_below(cur[k]);
+            for (int k = 0; k < NU; ++k) {
+                const T up = lane_from_below_or(cur[k], lane_plus<s + 1>(val[k]));
                 sw[k] = sv[k];
                 sv[k] = se[k];
-                se[k] = lane == 0 ? (has_up ? bv[k][s + 1] : T(0)) : up;
+                se[k] = up;
             }
-            const bool hw = i > 0, he = i + 1 < nx;
-            T b9[9], dw[4], ds[4], dse[4], dsw[4], Lv[5], Uv[4];
+            const T al = (unsigned)(i0 + s - 1) < (unsigned)(nx - 2) ? arow : T(0);
+            T bp[NA], Lv[NL], Uv[NU];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) b9[k] = cb[k][s];
+            for (int k = 0; k < NA; ++k) bp[k] = cb[k][s];
+            if constexpr (Five)
+                factor_point5_y<T>(bp, al, cur, sv, se, Lv, Uv);
+            else
+                factor_point_y<T>(bp, al, cur, sv, se, sw, Lv, Uv);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                dw[k] = hw ? cur[k] : T(0);
-                ds[k] = hs ? sv[k] : T(0);
-                dse[k] = (hs && he) ? se[k] : T(0);
-                dsw[k] = (hs && hw) ? sw[k] : T(0);
-            }
-            factor_point<T>(b9, alpha_at<T>(i, j, nx, ny, alpha0), dw, ds, dse, dsw, Lv, Uv);
+            for (int k = 0; k < NL; ++k) oL[k][s] = Lv[k];
 #pragma unroll
-            for (int k = 0; k < 5; ++k) oL[k][s] = Lv[k];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < NU; ++k) {
                 oU[k][s] = Uv[k];
                 cur[k] = Uv[k];
             }
-            bad |= (lane < rows && i >= 0 && i < nx && bad_pivot(Lv[0])) ? 1 : 0;
-        }
+            if constexpr (s % (F / 2) == F / 2 - 1) put_half(oL, oU, e, s / (F / 2));
+        });
         if (c + 1 < nf) {
             if (!ready(bn, c + 1)) return false;
-            take(bn);
+            take(bn, c + 1);
         }
-        const long e = sk.index(band, lane, c * F);
+        if (has_down) {
+            if (lane == kWave - 1) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) vstore<T, F>(dl + k * L + e, oL[k]);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) vstore<T, F>(du + k * L + e, oU[k]);
-        {
+                for (int k = 0; k < NU; ++k) vstore<T, F>(E.out[k], oU[k]);
+            }
             const int col = c * F + lane - pad - 2 * (kWave - 1);
-            const bool ok = has_down && lane < F && col >= 0 && col < nx;
+            const bool ok = lane < F && col >= 0 && col < nx;
             u64 *p = ok ? bout + (long)col * CW : scratch;
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                put_tagged(p + (ok ? k * NW : 0), lk.gen,
-                           spread_from_lane<T, F>(oU[k], kWave - 1, lane));
+            for (int k = 0; k < NU; ++k)
+                put_tagged(p + (ok ? k * NW : 0), lk.gen, E.out[k][lane < F ? lane : 0]);
         }
         return true;
     };
 
     load_data(A, 0);
     load_bnd(bn, 0);
-    bool ok = ready(bn, 0);
-    if (ok) {
-        take(bn);
+    if (!ready(bn, 0)) return;
+    take(bn, 0);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) se[k] = (lane == 0 && has_up) ? bv[k][0] : T(0);
-        for (int c = 0; c < nf; c += 2) {
-            ok = chunk(A, B, c);
-            if (!ok || c + 1 >= nf) break;
-            ok = chunk(B, A, c + 1);
-            if (!ok) break;
-        }
+    for (int k = 0; k < NU; ++k) se[k] = lane == 0 ? val[k] : T(0);
+    for (int c = 0; c < nf; c += 2) {
+        if (!chunk(A, B, c) || c + 1 >= nf) break;
+        if (!chunk(B, A, c + 1)) break;
     }
-    if (__any(bad))
-        if (bad) __hip_atomic_store(&lk.st->err_pivot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // --------------------------------------------------------------------------
@@ -1031,7 +1069,7 @@ __global__ __launch_bounds__(2 * kWave) void k_backward_staged(T *__restrict__ w
     backward_band_staged<T, F, Five>(wz, du, sk, lk, sk.nb - 1 - blk / xcd_stride);
 }
 
-template <typename T, int F>
+template <typename T, int F, bool Five>
 __global__ __launch_bounds__(kWave) void k_factor(const T *__restrict__ a, T *__restrict__ dl,
                                                   T *__restrict__ du, T alpha0, Skew sk, Link lk0,
                                                   int xcd_stride)
@@ -1040,7 +1078,7 @@ __global__ __launch_bounds__(kWave) void k_factor(const T *__restrict__ a, T *__
     if (blk % xcd_stride || lk0.st->skip) return;
     Link lk = lk0;
     lk.gen = launch_gen(lk0);
-    factor_band<T, F>(a, dl, du, alpha0, sk, lk, blk / xcd_stride);
+    factor_band<T, F, Five>(a, dl, du, alpha0, sk, lk, blk / xcd_stride);
 }
 
 // --------------------------------------------------------------------------
@@ -1114,7 +1152,7 @@ __global__ __launch_bounds__(kWave) void k_factor_batch(const BatchSys<T> *__res
     const int2 m = map[blockIdx.x];
     if (m.x < 0) return;
     const BatchSys<T> &d = sys[m.x];
-    factor_band<T, F>(d.a, d.dl, d.du, T(0.5), d.sk, batch_link(d, 0), m.y);
+    factor_band<T, F, false>(d.a, d.dl, d.du, T(0.5), d.sk, batch_link(d, 0), m.y);
 }
 
 // --------------------------------------------------------------------------
@@ -1141,27 +1179,31 @@ __device__ __forceinline__ SkewPoint skew_point(const Skew &sk, long e)
 }
 
 // nk arrays, reference layout (stride nx*ny) -> skewed entry e (stride
-// sk.size()); padding entries get zero.
+// sk.size()); padding entries get zero, and one in array `unit` (the
+// diagonal of A: the factorization's unit padding; -1: none).
 template <typename TI, typename TO>
 __device__ __forceinline__ void to_skew_entry(const TI *__restrict__ src, TO *__restrict__ dst,
-                                              int nk, const Skew &sk, long e)
+                                              int nk, const Skew &sk, long e, int unit)
 {
     const long L = sk.size();
     const SkewPoint q = skew_point(sk, e);
     const long N = (long)sk.nx * sk.ny;
     const long p = q.real ? (long)q.j * sk.nx + q.i : 0;
-    for (int k = 0; k < nk; ++k) dst[k * L + e] = q.real ? (TO)src[k * N + p] : TO(0);
+    for (int k = 0; k < nk; ++k)
+        dst[k * L + e] = q.real ? (TO)src[k * N + p] : (k == unit ? TO(1) : TO(0));
 }
 
 // Sweep-ready factor at skewed entry e: dl[5] = 1/dl1 at grid points; at
 // padding entries (outside the grid or beyond the last row) dl1 = 1/dl1 = 1
 // and every coupling zero, so the sweeps compute finite padding values.
+// Returns true for a grid point whose pivot dl1 is zero or not finite.
 template <typename T>
-__device__ __forceinline__ void prepare_entry(T *__restrict__ dl, T *__restrict__ du,
+__device__ __forceinline__ bool prepare_entry(T *__restrict__ dl, T *__restrict__ du,
                                               const Skew &sk, long e)
 {
     const long L = sk.size();
     if (skew_point(sk, e).real) {
+        const bool bad = bad_pivot(dl[e]);
         if constexpr (ScaledRows<T>::value) {  // ny = -1/dl1, ne_k = ny * dl_k
             const T ny = -(T(1) / dl[e]);
             dl[e] = ny;
@@ -1170,7 +1212,7 @@ __device__ __forceinline__ void prepare_entry(T *__restrict__ dl, T *__restrict_
         } else {
             dl[5 * L + e] = T(1) / dl[e];
         }
-        return;
+        return bad;
     }
     dl[e] = ScaledRows<T>::value ? T(0) : T(1);
     dl[5 * L + e] = T(1);
@@ -1178,14 +1220,22 @@ __device__ __forceinline__ void prepare_entry(T *__restrict__ dl, T *__restrict_
     for (int k = 1; k < 5; ++k) dl[k * L + e] = T(0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) du[k * L + e] = T(0);
+    return false;
+}
+
+// a zero or non-finite pivot of the factorization: st->err_pivot
+__device__ __forceinline__ void flag_pivot(bool bad, SolveState *st)
+{
+    if (__any(bad) && bad && st)
+        __hip_atomic_store(&st->err_pivot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename TI, typename TO>
 __global__ __launch_bounds__(256) void k_to_skew(const TI *__restrict__ src, TO *__restrict__ dst,
-                                                 int nk, Skew sk)
+                                                 int nk, Skew sk, int unit)
 {
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
-    if (e < sk.size()) to_skew_entry<TI, TO>(src, dst, nk, sk, e);
+    if (e < sk.size()) to_skew_entry<TI, TO>(src, dst, nk, sk, e, unit);
 }
 
 // 5 diagonals (S, W, P, E, N), reference layout -> the 9 skewed arrays of
@@ -1203,16 +1253,17 @@ __global__ __launch_bounds__(256) void k_to_skew5(const TI *__restrict__ src, TO
     const long p = q.real ? (long)q.j * sk.nx + q.i : 0;
 #pragma unroll
     for (int k = 0; k < 9; ++k)
-        dst[k * L + e] = (q.real && from5[k] >= 0) ? (TO)src[from5[k] * N + p] : TO(0);
+        dst[k * L + e] = q.real ? (from5[k] >= 0 ? (TO)src[from5[k] * N + p] : TO(0))
+                                : (k == kP ? TO(1) : TO(0));
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_prepare_factor(T *__restrict__ dl, T *__restrict__ du,
-                                                        Skew sk, const SolveState *st)
+                                                        Skew sk, SolveState *st)
 {
     if (st && st->skip) return;
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
-    if (e < sk.size()) prepare_entry<T>(dl, du, sk, e);
+    flag_pivot(e < sk.size() && prepare_entry<T>(dl, du, sk, e), st);
 }
 
 // Batched: workgroup g covers skewed entries map[g].y*256 .. of system
@@ -1224,7 +1275,7 @@ __global__ __launch_bounds__(256) void k_to_skew_batch(const BatchSys<T> *__rest
     const int2 m = map[blockIdx.x];
     const BatchSys<T> &d = sys[m.x];
     const long e = (long)m.y * 256 + threadIdx.x;
-    if (e < d.sk.size()) to_skew_entry<T, T>(d.ra, d.a, 9, d.sk, e);
+    if (e < d.sk.size()) to_skew_entry<T, T>(d.ra, d.a, 9, d.sk, e, kP);
 }
 
 template <typename T>
@@ -1234,7 +1285,7 @@ __global__ __launch_bounds__(256) void k_prepare_batch(const BatchSys<T> *__rest
     const int2 m = map[blockIdx.x];
     const BatchSys<T> &d = sys[m.x];
     const long e = (long)m.y * 256 + threadIdx.x;
-    if (e < d.sk.size()) prepare_entry<T>(d.dl, d.du, d.sk, e);
+    flag_pivot(e < d.sk.size() && prepare_entry<T>(d.dl, d.du, d.sk, e), d.st);
 }
 
 // skewed -> reference layout, one thread per point.
@@ -1298,10 +1349,16 @@ Skew skew_for(int nx, int ny, int value_bytes) { return Skew::make(nx, ny, 64 / 
 
 template <typename T>
 void launch_factor(const T *a_sk, T *dl, T *du, T alpha0, const Skew &sk, Link lk, int xcd_stride,
-                   hipStream_t s)
+                   hipStream_t s, bool five)
 {
-    hipLaunchKernelGGL((k_factor<T, factor_steps<T>()>), dim3(sk.nb * xcd_stride), dim3(kWave), 0,
-                       s, a_sk, dl, du, alpha0, sk, lk, xcd_stride);
+    constexpr int F = factor_steps<T>();
+    const dim3 g(sk.nb * xcd_stride), b(kWave);
+    if (five)
+        hipLaunchKernelGGL((k_factor<T, F, true>), g, b, 0, s, a_sk, dl, du, alpha0, sk, lk,
+                           xcd_stride);
+    else
+        hipLaunchKernelGGL((k_factor<T, F, false>), g, b, 0, s, a_sk, dl, du, alpha0, sk, lk,
+                           xcd_stride);
 }
 
 template <typename T>
@@ -1339,7 +1396,7 @@ void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride
 }
 
 template <typename T>
-void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s, const SolveState *st)
+void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s, SolveState *st)
 {
     hipLaunchKernelGGL(k_prepare_factor<T>, dim3(blocks_for(sk.size())), dim3(256), 0, s, dl, du,
                        sk, st);
@@ -1393,10 +1450,10 @@ void launch_prepare_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hip
 }
 
 template <typename TI, typename TO>
-void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t s)
+void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t s, int unit)
 {
     hipLaunchKernelGGL((k_to_skew<TI, TO>), dim3(blocks_for(sk.size())), dim3(256), 0, s, src, dst,
-                       nk, sk);
+                       nk, sk, unit);
 }
 
 template <typename TI, typename TO>
@@ -1421,14 +1478,14 @@ void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipSt
 }
 
 #define M2D_INST(T)                                                                           \
-    template void launch_factor<T>(const T *, T *, T *, T, const Skew &, Link, int, hipStream_t); \
+    template void launch_factor<T>(const T *, T *, T *, T, const Skew &, Link, int, hipStream_t, \
+                                   bool);                                                     \
     template void launch_forward<T>(const T *, const T *, T *, const Skew &, Link, int,          \
                                     hipStream_t, bool, bool);                                 \
     template void launch_backward<T>(T *, const T *, const Skew &, Link, int, hipStream_t, bool, \
                                      bool);                                                   \
     template void launch_from_skew<T>(const T *, T *, int, const Skew &, hipStream_t);          \
-    template void launch_prepare_factor<T>(T *, T *, const Skew &, hipStream_t,                 \
-                                           const SolveState *);                               \
+    template void launch_prepare_factor<T>(T *, T *, const Skew &, hipStream_t, SolveState *);  \
     template void launch_factor_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t);  \
     template void launch_forward_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t,  \
                                           bool, bool);                                        \
@@ -1440,9 +1497,9 @@ void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipSt
 M2D_INST(double)
 M2D_INST(float)
 template void launch_to_skew<double, double>(const double *, double *, int, const Skew &,
-                                             hipStream_t);
+                                             hipStream_t, int);
 template void launch_to_skew<double, float>(const double *, float *, int, const Skew &,
-                                            hipStream_t);
+                                            hipStream_t, int);
 template void launch_to_skew5<double, double>(const double *, double *, const Skew &, hipStream_t);
 template void launch_to_skew5<double, float>(const double *, float *, const Skew &, hipStream_t);
 

@@ -84,6 +84,25 @@ __device__ __forceinline__ float lane_from_above_or(float v, float edge)
     return __int_as_float(dpp_shl1_or(__float_as_int(v), __float_as_int(edge)));
 }
 
+// row_shl:N -- lane l receives lane l+N's value within its row of 16 lanes
+// (lanes whose source lies past the row end receive 0); for lane 0 this is
+// lane N's value, N <= 15.
+template <int N>
+__device__ __forceinline__ double lane_plus(double v)
+{
+    static_assert(N >= 1 && N <= 15, "row_shl takes 1..15");
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), 0x100 + N, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x100 + N, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int N>
+__device__ __forceinline__ float lane_plus(float v)
+{
+    static_assert(N >= 1 && N <= 15, "row_shl takes 1..15");
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + N, 0xf, 0xf, true));
+}
+
 // Value of lane `src` (wave-uniform constant) broadcast to every lane.
 __device__ __forceinline__ double read_lane(double v, int src)
 {
