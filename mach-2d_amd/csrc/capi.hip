@@ -152,6 +152,13 @@ struct m2d_system {
     std::vector<PlGraph> plg;
     bool pl_graph = true;
     int pl_last = 0, pl_reads = 0, pl_batches = 0;
+
+    // time step (m2d_intflow_time_step): a second stream for the in-order
+    // statistics that run beside the step, its events, the device scratch
+    // of those sums and the pinned copy of their results (p_avg, 4 means)
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    double *ts_scr = nullptr, *h_ts = nullptr;
 };
 
 static hipEvent_t pool_event(m2d_system *s)
@@ -240,6 +247,11 @@ extern "C" int m2d_destroy(m2d_system *s)
     for (hipEvent_t e : s->pool) (void)hipEventDestroy(e);
     for (PlGraph &g : s->plg) (void)hipGraphExecDestroy(g.exec);
     if (s->cap) (void)hipStreamDestroy(s->cap);
+    if (s->aux) (void)hipStreamDestroy(s->aux);
+    if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+    if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+    if (s->ts_scr) (void)hipFree(s->ts_scr);
+    if (s->h_ts) (void)hipHostFree(s->h_ts);
     if (s->pl) (void)hipFree(s->pl);
     if (s->h_pl) (void)hipHostFree(s->h_pl);
     if (s->h_st) (void)hipHostFree(s->h_st);
@@ -1405,6 +1417,22 @@ static int solve9(m2d_system *s9, const m2d_time_step_params *P, const double *a
     return M2D_OK;
 }
 
+// The time step's side stream and the scratch of its in-order sums: the
+// average of p (n + 2 doubles) and the four mean coefficients (4 n + 4),
+// n = (nx-2)(ny-2); created with the first step on the handle.
+static int ts_side_ready(m2d_system *s)
+{
+    if (s->aux) return M2D_OK;
+    const long n = (long)(s->nx - 2) * (s->ny - 2);
+    HIP_OK(hipMalloc((void **)&s->ts_scr,
+                     sizeof(double) * (size_t)(average_scratch(s->nx, s->ny) + 4 * n + 4)));
+    HIP_OK(hipHostMalloc((void **)&s->h_ts, sizeof(double) * 5, hipHostMallocDefault));
+    HIP_OK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
+    HIP_OK(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
+    return M2D_OK;
+}
+
 extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
                                      const m2d_time_step_fields *f,
                                      const m2d_time_step_params *P, double *dt_io,
@@ -1482,9 +1510,25 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
         m2d_intflow_time_cycle_update(&nx, &ny, &P->gamma, &P->Rg, &P->P0, &P->T0, f->x, f->xp,
                                       f->u, f->uin, f->vin, f->pin, f->Tin, f->Mw, f->plin,
                                       f->pina, f->p);
+    // get_average(p) (:151) on the side stream, beside the u and v stages:
+    // only the p' loop changes p, and it waits for the value
     double p_avg = 0.0;
+    bool avg_pending = false;
     if (ok()) stage = 3;
-    if (!rc) p_avg = m2d_get_average(&nx, &ny, f->p);  // (:151)
+    if (!rc) rc = ts_side_ready(s5);
+    if (!rc) {
+        double *scr = s5->ts_scr;
+        if (hipEventRecord(s5->ev_fork, st) != hipSuccess ||
+            hipStreamWaitEvent(s5->aux, s5->ev_fork, 0) != hipSuccess)
+            rc = M2D_ERR_HIP;
+        if (!rc) average_enqueue(nx, ny, f->p, scr, s5->aux);
+        if (!rc && (hipGetLastError() != hipSuccess ||
+                    hipMemcpyAsync(s5->h_ts, scr, sizeof(double), hipMemcpyDeviceToHost,
+                                   s5->aux) != hipSuccess ||
+                    hipEventRecord(s5->ev_join, s5->aux) != hipSuccess))
+            rc = M2D_ERR_HIP;
+        avg_pending = !rc;
+    }
     if (ok()) {
         hipLaunchKernelGGL(k_g_nodes, dim3(nb), dim3(256), 0, st, f->T, P->Rg, N, f->g);
         if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
@@ -1561,6 +1605,14 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
                                 f->de,  f->dn,  f->g,   f->roa, f->u,   f->v,   f->plin,
                                 f->ue,  f->ve,  f->un,  f->vn,  f->Uce, f->Vcn, f->roe,
                                 f->ron, f->ro,  f->p,   f->pl,  f->ap,  f->bp};
+        if (avg_pending) {
+            if (hipEventSynchronize(s5->ev_join) != hipSuccess) {
+                rc = M2D_ERR_HIP;
+                break;
+            }
+            p_avg = s5->h_ts[0];
+            avg_pending = false;
+        }
         rc = m2d_intflow_pl_loop(s5, &L, modvis, dt, beta, p_avg, P->tolm, P->itpmax,
                                  P->nitm_p, P->tol_p, &itp, &normpl);
         if (rc) break;
@@ -1573,6 +1625,26 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
             m2d_intflow_extrapolate_u_v_to_fictitious(&nx, &ny, &modvis, f->x, f->xp, f->xk,
                                                       f->yk, f->u, f->v);
         ok();
+    }
+    // Euler: au, av, at, ap are final once the mass cycles end, so the four
+    // mean coefficients of the next time step (:394) run on the side stream
+    // beside the mass check, T and the densities
+    bool cc_pending = false;
+    if (!rc && euler) {
+        const double *a4[4] = {f->au, f->av, f->at, f->ap};
+        const int nd4[4] = {9, 9, 9, 5};
+        double *scr = s5->ts_scr + average_scratch(nx, ny);
+        const long n = (long)(nx - 2) * (ny - 2);
+        if (hipEventRecord(s5->ev_fork, st) != hipSuccess ||
+            hipStreamWaitEvent(s5->aux, s5->ev_fork, 0) != hipSuccess)
+            rc = M2D_ERR_HIP;
+        if (!rc) mean_cc_enqueue(nx, ny, 4, a4, nd4, scr, s5->aux);
+        if (!rc && (hipGetLastError() != hipSuccess ||
+                    hipMemcpyAsync(s5->h_ts + 1, scr + 4 * n, sizeof(double) * 4,
+                                   hipMemcpyDeviceToHost, s5->aux) != hipSuccess ||
+                    hipEventRecord(s5->ev_join, s5->aux) != hipSuccess))
+            rc = M2D_ERR_HIP;
+        cc_pending = !rc;
     }
     if (!rc) stage = 10;
     if (!rc) {  // mass conservation (:346-348)
@@ -1611,12 +1683,19 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
         ok();
     }
     if (!rc) stage = 13;
-    if (!rc) {  // next time step (:394)
+    if (!rc && cc_pending) {  // next time step (:394), means from the side stream
+        if (hipEventSynchronize(s5->ev_join) != hipSuccess)
+            rc = M2D_ERR_HIP;
+        else
+            *dt_io = scarborough_dt(dt, s5->h_ts + 1, P->h0, P->mincc, P->maxcc);
+    } else if (!rc) {  // next time step (:394)
         const double nd = m2d_tstepper_dt_scarborough(&nx, &ny, &dt, f->au, f->av, f->at, f->ap,
                                                       &P->h0, &P->mincc, &P->maxcc);
         if (ok()) *dt_io = nd;
     }
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = M2D_ERR_HIP;
+    // nothing of the step stays in flight on the side stream, on any path
+    if (s5->aux && hipStreamSynchronize(s5->aux) != hipSuccess && !rc) rc = M2D_ERR_HIP;
     (void)hipFree(work);
     assembly_stream(keep);
     if (res) {

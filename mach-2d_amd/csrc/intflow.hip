@@ -413,6 +413,17 @@ int run(int nx, int ny, std::vector<Arr> arrs, long scratch, bool sync, Launch l
 
 }  // namespace
 
+// scratch: the mean, the n gathered terms, one more (seq_sum reads whole
+// 16-byte pairs)
+long m2d::average_scratch(int nx, int ny) { return (long)(nx - 2) * (ny - 2) + 2; }
+
+void m2d::average_enqueue(int nx, int ny, const double *F, double *scr, hipStream_t s)
+{
+    const dim3 g((unsigned)((nx - 2 + kT - 1) / kT), (unsigned)(ny - 2));
+    hipLaunchKernelGGL(k_average_gather, g, dim3(kT), 0, s, F, nx, ny, scr + 1);
+    hipLaunchKernelGGL(k_average, dim3(1), dim3(kSeqThreads), 0, s, scr + 1, nx, ny, scr);
+}
+
 void m2d_set_last_status_code(int status);
 
 #define NEED(...)                                               \
@@ -574,15 +585,9 @@ extern "C" double m2d_get_average(const int *nx, const int *ny, const double *F)
     const int NX = *nx, NY = *ny;
     const long n = (long)(NX - 2) * (NY - 2);
     double out = 0.0;
-    // scratch: the mean, the n gathered terms, one more (seq_sum reads whole
-    // 16-byte pairs)
-    const int rc = run(NX, NY, {{F, (long)NX * NY, false}}, n + 2, true,
+    const int rc = run(NX, NY, {{F, (long)NX * NY, false}}, average_scratch(NX, NY), true,
                        [&](std::vector<double *> &d, hipStream_t s, double *scr, dim3) {
-                           const dim3 g((unsigned)((NX - 2 + kT - 1) / kT), (unsigned)(NY - 2));
-                           hipLaunchKernelGGL(k_average_gather, g, dim3(kT), 0, s, d[0], NX, NY,
-                                              scr + 1);
-                           hipLaunchKernelGGL(k_average, dim3(1), dim3(kSeqThreads), 0, s,
-                                              scr + 1, NX, NY, scr);
+                           average_enqueue(NX, NY, d[0], scr, s);
                            (void)hipMemcpyAsync(&out, scr, sizeof(double),
                                                 hipMemcpyDeviceToHost, s);
                        });

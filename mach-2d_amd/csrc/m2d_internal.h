@@ -112,6 +112,19 @@ hipStream_t assembly_stream_current();
 // kernels (nullptr, nullptr: off): a kernel returns at once while *gate is
 // non-zero, and reads dt = sc[0], beta = sc[1] from device memory.
 void assembly_gate(const int *gate, const double *sc);
+
+// ---- in-order statistics of the time step (tstep.hip, intflow.hip) -----------
+// The K mean convergence coefficients of device arrays a[k] (nd[k] = 9 or 5
+// diagonals) enqueued on s: scratch t of K*(nx-2)*(ny-2) + K doubles, the
+// means at its end.
+void mean_cc_enqueue(int nx, int ny, int K, const double *const *a, const int *nd, double *t,
+                     hipStream_t s);
+// tstepper_dt_scarborough's step rule on the four means cc (u, v, T, p')
+double scarborough_dt(double dt, const double *cc, double h0, double mincc, double maxcc);
+// get_average of the device field F enqueued on s: the mean lands in scr[0]
+// (scratch of average_scratch(nx, ny) doubles)
+long average_scratch(int nx, int ny);
+void average_enqueue(int nx, int ny, const double *F, double *scr, hipStream_t s);
 // true for device memory (false for host memory and unknown pointers)
 bool is_device_array(const void *p);
 template <typename TI, typename TO>

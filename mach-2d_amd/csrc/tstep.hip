@@ -150,14 +150,7 @@ int mean_cc(int nx, int ny, int K, const double *const *a, const int *nd, double
     double *t = nullptr;
     int rc = stage(S, ptrs, lens, K * n + K, &t, s);
     if (rc) return rc;
-    const dim3 grid((unsigned)((nx - 2 + kT - 1) / kT), (unsigned)(ny - 2));
-    for (int k = 0; k < K; ++k) {
-        if (nd[k] == 9)
-            hipLaunchKernelGGL(k_cc_terms<9>, grid, dim3(kT), 0, s, S.in[k], nx, ny, t + k * n);
-        else
-            hipLaunchKernelGGL(k_cc_terms<5>, grid, dim3(kT), 0, s, S.in[k], nx, ny, t + k * n);
-    }
-    hipLaunchKernelGGL(k_seq_mean, dim3(K), dim3(kSeqThreads), 0, s, t, n, t + K * n);
+    mean_cc_enqueue(nx, ny, K, S.in.data(), nd, t, s);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(out, t + K * n, sizeof(double) * K, hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
@@ -167,6 +160,37 @@ int mean_cc(int nx, int ny, int K, const double *const *a, const int *nd, double
 }
 
 }  // namespace
+
+void m2d::mean_cc_enqueue(int nx, int ny, int K, const double *const *a, const int *nd,
+                          double *t, hipStream_t s)
+{
+    const long n = (long)(nx - 2) * (ny - 2);
+    const dim3 grid((unsigned)((nx - 2 + kT - 1) / kT), (unsigned)(ny - 2));
+    for (int k = 0; k < K; ++k) {
+        if (nd[k] == 9)
+            hipLaunchKernelGGL(k_cc_terms<9>, grid, dim3(kT), 0, s, a[k], nx, ny, t + k * n);
+        else
+            hipLaunchKernelGGL(k_cc_terms<5>, grid, dim3(kT), 0, s, a[k], nx, ny, t + k * n);
+    }
+    hipLaunchKernelGGL(k_seq_mean, dim3(K), dim3(kSeqThreads), 0, s, t, n, t + K * n);
+}
+
+// tstepper_dt_scarborough's rule (mod_tstepper_scarborough.f90:80-123) on
+// the four means: the largest against mincc and maxcc
+double m2d::scarborough_dt(double dt, const double *cc, double h0, double mincc, double maxcc)
+{
+    double c = cc[0];
+    for (int k = 1; k < 4; ++k)
+        if (cc[k] > c) c = cc[k];
+    double h;
+    if (c < mincc)
+        h = h0 * sqrt(1.0 - c / mincc);
+    else if (maxcc < c)
+        h = -h0 * sqrt(c / maxcc - 1.0);
+    else
+        h = 0.0;
+    return dt * (1.0 + h);
+}
 
 void m2d_set_last_status_code(int status);
 
@@ -239,17 +263,5 @@ extern "C" double m2d_tstepper_dt_scarborough(const int *nx, const int *ny, cons
     const int rc = mean_cc(*nx, *ny, 4, a, nd, cc);
     m2d_set_last_status_code(rc);
     if (rc) return 0.0;
-    const double cu = cc[0], cv = cc[1], ct = cc[2], cp = cc[3];
-    double c = cu;
-    if (cv > c) c = cv;
-    if (ct > c) c = ct;
-    if (cp > c) c = cp;
-    double h;
-    if (c < *mincc)
-        h = *h0 * sqrt(1.0 - c / *mincc);
-    else if (*maxcc < c)
-        h = -*h0 * sqrt(c / *maxcc - 1.0);
-    else
-        h = 0.0;
-    return *dt * (1.0 + h);
+    return scarborough_dt(*dt, cc, *h0, *mincc, *maxcc);
 }
