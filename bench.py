@@ -55,8 +55,8 @@ NITM_CAP = 20000
 # algorithmic bytes per point of one launch, in units of the value size
 BYTES_PER_POINT = {
     "factor": 18,    # read a (9), write dl (5) and du (4)
-    "forward": 7,    # read r, dl (5); write w
-    "backward": 6,   # read w, du (4); write z
+    "sweeps": 13,    # forward: read r, dl (5), write w; backward: read w, du (4), write z
+    "backward": 6,   # staged sweeps only: the backward sweep alone
     "residual": 15,  # update: read z, var, write var; residual: read a (9), rhs, var, write r
 }
 

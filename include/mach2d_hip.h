@@ -667,7 +667,8 @@ int m2d_unet_unpack_output(const float *out, int os, int nx, int ny, int w, doub
                            double *var, void *stream);
 
 /* tdma2d9(nx,ny,nitm,a,b,x), src/solvers/tdma2d9.f90:38-322, on device
- * pointers (same results bit for bit; nx, ny <= 3072).  Runs on `stream`
+ * pointers (same results bit for bit; 3 <= nx, ny <= 2048, else
+ * M2D_ERR_ARG).  Runs on `stream`
  * and returns when x is final. */
 int m2d_tdma2d9_device(int nx, int ny, int nitm, const double *d_a,
                        const double *d_b, double *d_x, void *stream);
@@ -681,9 +682,10 @@ int m2d_tdma2d9_device_many(int count, int nx, int ny, int nitm, const double *c
 
 /* Per-kernel device time of this handle's launches (HIP events on the
  * handle's stream around each launch; launches after the stopping rule
- * fired are not counted).  Kernel ids: 0 factor, 1 forward sweep,
- * 2 backward sweep, 3 update + residual; +4 for the FP32 kernels of
- * M2D_FP32_IR.  Enabling resets the totals.  No reference counterpart (the
+ * fired are not counted).  Kernel ids: 0 factor, 1 both sweeps of an
+ * iteration (one launch; with staged sweeps the forward sweep alone),
+ * 2 backward sweep (staged sweeps only), 3 update + residual and its norm;
+ * +4 for the FP32 kernels of M2D_FP32_IR.  Enabling resets the totals.  No reference counterpart (the
  * reference times only the whole run with cpu_time). */
 int m2d_enable_kernel_timing(m2d_system *s, int on);
 int m2d_kernel_timing(m2d_system *s, int kernel, double *total_ms, long *launches);

@@ -1197,6 +1197,20 @@ void m2d_set_last_status_code(int status);
 
 extern "C" void m2d_assembly_set_stream(void *stream) { g_stream = (hipStream_t)stream; }
 
+void m2d::mass_terms_enqueue(int nx, int ny, double dt, const double *const *in, double *t,
+                             hipStream_t s)
+{
+    Fields F;
+    memset(&F, 0, sizeof(F));
+    F.nx = nx;
+    F.ny = ny;
+    F.dt = dt;
+    for (int k = 0; k < 10; ++k) F.in[k] = in[k];
+    F.out[1] = t;
+    const dim3 grid((unsigned)((nx - 2 + kTx - 1) / kTx), (unsigned)(ny - 2));
+    hipLaunchKernelGGL(k_mass_terms, grid, dim3(kTx), 0, s, F);
+}
+
 // Stream of this thread's device-array assembly calls; returns the previous one.
 hipStream_t m2d::assembly_stream(hipStream_t s)
 {

@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kPts) void k_settle_batch(const BatchSys<T> *__rest
 }
 
 // One workgroup per system: ||r|| from the partial sums and the stopping
-// rule of msi2d9-v03.f90:133,258-259 (same rule as k_norm_finalize).
+// rule of msi2d9-v03.f90:133,258-259 (same rule as the single-system residual's norm_rule_block).
 template <typename T>
 __global__ __launch_bounds__(256) void k_finalize_batch(const BatchSys<T> *__restrict__ sys,
                                                         int initial)

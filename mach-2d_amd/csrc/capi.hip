@@ -121,7 +121,7 @@ struct m2d_system {
     float *a32 = nullptr, *c32 = nullptr, *rr32 = nullptr;
     double *r64 = nullptr;
 
-    double *partial = nullptr;
+    u64 *partial = nullptr;  // residual partial sums (tagged words)
     int np = 0;
     u64 *bnd = nullptr;  // boundary words, then per-lane scratch words
     long nbnd = 0;
@@ -276,7 +276,10 @@ extern "C" int m2d_create_stencil(int nx, int ny, int precision, int ndiag, m2d_
     s->np = residual_blocks(nx, ny);
     s->nbnd = boundary_words(nx, ny) + scratch_words(ny);
     const long N = s->N;
-    bool ok = dalloc(&s->partial, s->np) == hipSuccess && dalloc(&s->bnd, s->nbnd) == hipSuccess &&
+    const long npw = residual_partial_words(nx, ny);
+    bool ok = dalloc(&s->partial, npw) == hipSuccess &&
+              hipMemset(s->partial, 0, sizeof(u64) * (size_t)npw) == hipSuccess &&
+              dalloc(&s->bnd, s->nbnd) == hipSuccess &&
               dalloc(&s->st, 1) == hipSuccess && dalloc(&s->st_in, 1) == hipSuccess &&
               dalloc(&s->r64, N) == hipSuccess;
     if (ok && precision == M2D_FP64) ok = s->w64.alloc(nx, ny);
@@ -430,20 +433,26 @@ static void enqueue_iteration(m2d_system *s, Work<T> &W, SolveState *st, const T
                               T *x, double *hist, int hist_cap, int k, double gamma, int nitm)
 {
     const int off = sizeof(T) == 4 ? KT_FP32 : 0;
-    timed(s, KT_FORWARD + off, k, [&] {
-        launch_forward<T>(W.r, W.dl, W.w, W.sk, make_link(s, st, 1), s->xcd_stride, s->stream,
-                          s->staged, s->nd == 5);
-    });
-    timed(s, KT_BACKWARD + off, k, [&] {
-        launch_backward<T>(W.w, W.du, W.sk, make_link(s, st, 2), s->xcd_stride, s->stream,
-                           s->staged, s->nd == 5);
-    });
+    if (!s->staged) {  // both sweeps in one launch, timed as the forward sweep
+        timed(s, KT_FORWARD + off, k, [&] {
+            launch_sweeps<T>(W.r, W.dl, W.w, W.du, W.sk, make_link(s, st, 1), s->xcd_stride,
+                             s->stream, s->nd == 5);
+        });
+    } else {  // the staged rings of both sweeps do not fit one workgroup's LDS
+        timed(s, KT_FORWARD + off, k, [&] {
+            launch_forward_staged<T>(W.r, W.dl, W.w, W.sk, make_link(s, st, 1), s->xcd_stride,
+                                     s->stream, s->nd == 5);
+        });
+        timed(s, KT_BACKWARD + off, k, [&] {
+            launch_backward_staged<T>(W.w, W.du, W.sk, make_link(s, st, 2), s->xcd_stride,
+                                      s->stream, s->nd == 5);
+        });
+    }
     timed(s, KT_RESIDUAL + off, k, [&] {
         launch_update<T>(x, W.w, W.sk, st, s->stream);
-        launch_residual<T, T, T>(a, x, rhs, W.r, s->nx, s->ny, s->partial, st, &W.sk, s->stream,
-                                 s->nd);
+        launch_residual_norm<T, T, T>(a, x, rhs, W.r, s->nx, s->ny, s->partial, st, &W.sk,
+                                      NormRule{hist, hist_cap, 0, gamma, nitm}, s->stream, s->nd);
     });
-    launch_norm_finalize(s->partial, s->np, st, hist, hist_cap, 0, gamma, nitm, s->stream);
 }
 
 // fb2d9 on the device, in precision T, from the current x (reference
@@ -453,8 +462,8 @@ static int run_msi(m2d_system *s, Work<T> &W, SolveState *st, const T *a, const 
                    double gamma, int nitm, double *hist, int hist_cap)
 {
     begin(s, st);
-    launch_residual<T, T, T>(a, x, rhs, W.r, s->nx, s->ny, s->partial, st, &W.sk, s->stream, s->nd);
-    launch_norm_finalize(s->partial, s->np, st, nullptr, 0, 1, gamma, nitm, s->stream);
+    launch_residual_norm<T, T, T>(a, x, rhs, W.r, s->nx, s->ny, s->partial, st, &W.sk,
+                                  NormRule{nullptr, 0, 1, gamma, nitm}, s->stream, s->nd);
     const int kmax = nitm > 0 ? nitm : kIterCap;
     int k = 1;
     int chunk = 2;  // grows geometrically up to s->chunk
@@ -533,9 +542,10 @@ static int solve_fp32_ir(m2d_system *s, const double *d_a, const double *d_rhs, 
     const long N = s->N;
     const unsigned nblk = (unsigned)((N + 255) / 256);
     launch_reset_state(s->st, s->stream);
-    launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r64, s->nx, s->ny, s->partial,
-                                            s->st, nullptr, s->stream, s->nd);
-    launch_norm_finalize(s->partial, s->np, s->st, nullptr, 0, 1, gamma, nitm, s->stream);
+    launch_residual_norm<double, double, double>(d_a, d_var, d_rhs, s->r64, s->nx, s->ny,
+                                                 s->partial, s->st, nullptr,
+                                                 NormRule{nullptr, 0, 1, gamma, nitm}, s->stream,
+                                                 s->nd);
     HIP_OK(hipGetLastError());
     int rc = read_state(s, s->st);
     if (rc) return rc;
@@ -552,10 +562,10 @@ static int solve_fp32_ir(m2d_system *s, const double *d_a, const double *d_rhs, 
         const int inner_status = s->h_st->status;
         hipLaunchKernelGGL(k_ir_update, dim3(nblk), dim3(256), 0, s->stream, d_var, s->c32, N,
                            s->st);
-        launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r64, s->nx, s->ny,
-                                                s->partial, s->st, nullptr, s->stream, s->nd);
-        launch_norm_finalize(s->partial, s->np, s->st, d_hist, hist_cap, 0, gamma, nitm,
-                             s->stream);
+        launch_residual_norm<double, double, double>(d_a, d_var, d_rhs, s->r64, s->nx, s->ny,
+                                                     s->partial, s->st, nullptr,
+                                                     NormRule{d_hist, hist_cap, 0, gamma, nitm},
+                                                     s->stream, s->nd);
         HIP_OK(hipGetLastError());
         rc = read_state(s, s->st);
         if (rc) return rc;
@@ -581,9 +591,10 @@ extern "C" int m2d_residual_norm(m2d_system *s, const double *d_a, const double 
 {
     if (!s || !d_a || !d_rhs || !d_var || !out) return M2D_ERR_ARG;
     launch_reset_state(s->st, s->stream);
-    launch_residual<double, double, double>(d_a, d_var, d_rhs, s->r64, s->nx, s->ny, s->partial,
-                                            s->st, nullptr, s->stream, s->nd);
-    launch_norm_finalize(s->partial, s->np, s->st, nullptr, 0, 1, 1.0, 1, s->stream);
+    launch_residual_norm<double, double, double>(d_a, d_var, d_rhs, s->r64, s->nx, s->ny,
+                                                 s->partial, s->st, nullptr,
+                                                 NormRule{nullptr, 0, 1, 1.0, 1}, s->stream,
+                                                 s->nd);
     HIP_OK(hipGetLastError());
     const int rc = read_state(s, s->st);
     if (rc) return rc;
@@ -830,9 +841,9 @@ static void enqueue_solve64(m2d_system *s, const double *a, const double *rhs, d
                             double gamma, int nitm)
 {
     Work<double> &W = s->w64;
-    launch_residual<double, double, double>(a, x, rhs, W.r, s->nx, s->ny, s->partial, s->st,
-                                            &W.sk, s->stream, s->nd);
-    launch_norm_finalize(s->partial, s->np, s->st, nullptr, 0, 1, gamma, nitm, s->stream);
+    launch_residual_norm<double, double, double>(a, x, rhs, W.r, s->nx, s->ny, s->partial,
+                                                 s->st, &W.sk, NormRule{nullptr, 0, 1, gamma, nitm},
+                                                 s->stream, s->nd);
     for (int k = 1; k <= nitm; ++k)
         enqueue_iteration<double>(s, W, s->st, a, rhs, x, nullptr, 0, k, gamma, nitm);
 }
@@ -1268,7 +1279,7 @@ static NoAllocLayout noalloc_layout(int nx, int ny)
     o.r = o.du + up(4 * L);
     o.w = o.r + up(L);
     o.partial = o.w + up(L);
-    o.bnd = o.partial + up(sizeof(double) * (size_t)residual_blocks(nx, ny));
+    o.bnd = o.partial + up(sizeof(u64) * (size_t)residual_partial_words(nx, ny));
     o.st = o.bnd + up(sizeof(u64) * (size_t)(boundary_words(nx, ny) + scratch_words(ny)));
     o.total = o.st + up(sizeof(SolveState));
     return o;
@@ -1316,7 +1327,7 @@ extern "C" int m2d_solve_msi9d_noalloc(const int *nx, const int *ny, const int *
     s.w64.du = (double *)(base + o.du);
     s.w64.r = (double *)(base + o.r);
     s.w64.w = (double *)(base + o.w);
-    s.partial = (double *)(base + o.partial);
+    s.partial = (u64 *)(base + o.partial);
     s.bnd = (u64 *)(base + o.bnd);
     s.st = (SolveState *)(base + o.st);
     s.h_st = &h_st;
@@ -1325,6 +1336,9 @@ extern "C" int m2d_solve_msi9d_noalloc(const int *nx, const int *ny, const int *
     const size_t L = (size_t)s.w64.sk.size() * sizeof(double);
     HIP_OK(hipMemsetAsync(s.w64.r, 0, L, s.stream));
     HIP_OK(hipMemsetAsync(s.w64.w, 0, L, s.stream));
+    // the residual's tagged partial sums start from generation zero
+    HIP_OK(hipMemsetAsync(s.partial, 0, sizeof(u64) * (size_t)residual_partial_words(*nx, *ny),
+                          s.stream));
     enqueue_factor64(&s, d_a);
     HIP_OK(hipGetLastError());
     rc = read_state(&s, s.st);
@@ -1417,16 +1431,19 @@ static int solve9(m2d_system *s9, const m2d_time_step_params *P, const double *a
     return M2D_OK;
 }
 
-// The time step's side stream and the scratch of its in-order sums: the
-// average of p (n + 2 doubles) and the four mean coefficients (4 n + 4),
-// n = (nx-2)(ny-2); created with the first step on the handle.
+// The time step's side stream and its scratch: the residual norms' work
+// (norm_l1_work_size), then the in-order sums': the average of p (n + 2
+// doubles), the terms of the four mean coefficients and of the mass check
+// (5 n) and their five sums, n = (nx-2)(ny-2); h_ts receives p's average and
+// the five sums.  Created with the first step on the handle.
 static int ts_side_ready(m2d_system *s)
 {
     if (s->aux) return M2D_OK;
     const long n = (long)(s->nx - 2) * (s->ny - 2);
     HIP_OK(hipMalloc((void **)&s->ts_scr,
-                     sizeof(double) * (size_t)(average_scratch(s->nx, s->ny) + 4 * n + 4)));
-    HIP_OK(hipHostMalloc((void **)&s->h_ts, sizeof(double) * 5, hipHostMallocDefault));
+                     sizeof(double) * (size_t)(norm_l1_work_size(s->nx, s->ny) +
+                                               average_scratch(s->nx, s->ny) + 5 * n + 5)));
+    HIP_OK(hipHostMalloc((void **)&s->h_ts, sizeof(double) * 6, hipHostMallocDefault));
     HIP_OK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
     HIP_OK(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
@@ -1479,12 +1496,12 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
             rc = M2D_ERR_HIP;
     };
     const unsigned nb = (unsigned)((N + 255) / 256);
-    double *work = nullptr;
-    const int nw = norm_l1_work_size(nx, ny);
-    if (hipMalloc((void **)&work, sizeof(double) * (nw + 1)) != hipSuccess) {
+    if (ts_side_ready(s5) != M2D_OK) {
         assembly_stream(keep);
         return M2D_ERR_HIP;
     }
+    double *const work = s5->ts_scr;
+    double *const side = work + norm_l1_work_size(nx, ny);
     double norm = 0.0, rmass = 0.0, normpl = 0.0;
     int itp = 0, stage = 1;
     auto norm9_relative = [&](const double *a, const double *b, const double *x) {
@@ -1515,9 +1532,8 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
     double p_avg = 0.0;
     bool avg_pending = false;
     if (ok()) stage = 3;
-    if (!rc) rc = ts_side_ready(s5);
     if (!rc) {
-        double *scr = s5->ts_scr;
+        double *scr = side;
         if (hipEventRecord(s5->ev_fork, st) != hipSuccess ||
             hipStreamWaitEvent(s5->aux, s5->ev_fork, 0) != hipSuccess)
             rc = M2D_ERR_HIP;
@@ -1626,34 +1642,41 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
                                                       f->yk, f->u, f->v);
         ok();
     }
-    // Euler: au, av, at, ap are final once the mass cycles end, so the four
-    // mean coefficients of the next time step (:394) run on the side stream
-    // beside the mass check, T and the densities
-    bool cc_pending = false;
+    // The in-order sums that end the step run on the side stream, beside T
+    // and the densities: the mass check's (:346-348), whose terms are taken
+    // on st before the densities change, and, for Euler steps, where au, av,
+    // at, ap are final once the mass cycles end, the four mean coefficients
+    // of the next time step (:394).  Scratch: terms of the four means at
+    // t[0, 4n), of the mass check at t[4n, 5n), the sums at t[5n, 5n + 5).
+    const long nin = (long)(nx - 2) * (ny - 2);
+    double *tsum = side + average_scratch(nx, ny);
+    bool side_pending = false;
     if (!rc && euler) {
         const double *a4[4] = {f->au, f->av, f->at, f->ap};
         const int nd4[4] = {9, 9, 9, 5};
-        double *scr = s5->ts_scr + average_scratch(nx, ny);
-        const long n = (long)(nx - 2) * (ny - 2);
         if (hipEventRecord(s5->ev_fork, st) != hipSuccess ||
             hipStreamWaitEvent(s5->aux, s5->ev_fork, 0) != hipSuccess)
             rc = M2D_ERR_HIP;
-        if (!rc) mean_cc_enqueue(nx, ny, 4, a4, nd4, scr, s5->aux);
-        if (!rc && (hipGetLastError() != hipSuccess ||
-                    hipMemcpyAsync(s5->h_ts + 1, scr + 4 * n, sizeof(double) * 4,
-                                   hipMemcpyDeviceToHost, s5->aux) != hipSuccess ||
-                    hipEventRecord(s5->ev_join, s5->aux) != hipSuccess))
-            rc = M2D_ERR_HIP;
-        cc_pending = !rc;
+        if (!rc) cc_terms_enqueue(nx, ny, 4, a4, nd4, tsum, s5->aux);
+        if (!rc && hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
     }
     if (!rc) stage = 10;
     if (!rc) {  // mass conservation (:346-348)
-        m2d_check_mass_conservation(&nx, &ny, &dt, f->rp, f->re, f->rn, f->Jp, f->ro, f->roa,
-                                    f->ron, f->roe, f->Uce, f->Vcn, work + nw);
-        if (ok() && (hipMemcpyAsync(&rmass, work + nw, sizeof(double), hipMemcpyDeviceToHost,
-                                    st) != hipSuccess ||
-                     hipStreamSynchronize(st) != hipSuccess))
+        const double *mi[10] = {f->rp, f->re, f->rn,  f->Jp,  f->ro,
+                                f->roa, f->ron, f->roe, f->Uce, f->Vcn};
+        mass_terms_enqueue(nx, ny, dt, mi, tsum + 4 * nin, st);
+        const int first = euler ? 0 : 4, K = 5 - first;
+        if (hipGetLastError() != hipSuccess || hipEventRecord(s5->ev_fork, st) != hipSuccess ||
+            hipStreamWaitEvent(s5->aux, s5->ev_fork, 0) != hipSuccess)
             rc = M2D_ERR_HIP;
+        if (!rc) seq_sums_enqueue(tsum + first * nin, nin, K, tsum + 5 * nin + first, s5->aux);
+        if (!rc && (hipGetLastError() != hipSuccess ||
+                    hipMemcpyAsync(s5->h_ts + 1 + first, tsum + 5 * nin + first,
+                                   sizeof(double) * K, hipMemcpyDeviceToHost,
+                                   s5->aux) != hipSuccess ||
+                    hipEventRecord(s5->ev_join, s5->aux) != hipSuccess))
+            rc = M2D_ERR_HIP;
+        side_pending = !rc;
     }
     if (!rc) stage = 11;
     if (!rc && euler) {  // Euler, constant properties: T from H (:349-362)
@@ -1682,12 +1705,18 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
         if (ok()) m2d_get_density_at_faces(&nx, &ny, &beta, f->ro, f->Uce, f->Vcn, f->roe, f->ron);
         ok();
     }
-    if (!rc) stage = 13;
-    if (!rc && cc_pending) {  // next time step (:394), means from the side stream
+    if (!rc && side_pending) {
         if (hipEventSynchronize(s5->ev_join) != hipSuccess)
             rc = M2D_ERR_HIP;
         else
-            *dt_io = scarborough_dt(dt, s5->h_ts + 1, P->h0, P->mincc, P->maxcc);
+            rmass = s5->h_ts[5];
+    }
+    if (!rc) stage = 13;
+    if (!rc && euler) {  // next time step (:394), means from the side stream
+        // mean = sum / n, the division get_mean_cc makes, here on the host
+        double cc[4];
+        for (int k = 0; k < 4; ++k) cc[k] = s5->h_ts[1 + k] / (double)nin;
+        *dt_io = scarborough_dt(dt, cc, P->h0, P->mincc, P->maxcc);
     } else if (!rc) {  // next time step (:394)
         const double nd = m2d_tstepper_dt_scarborough(&nx, &ny, &dt, f->au, f->av, f->at, f->ap,
                                                       &P->h0, &P->mincc, &P->maxcc);
@@ -1696,7 +1725,6 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = M2D_ERR_HIP;
     // nothing of the step stays in flight on the side stream, on any path
     if (s5->aux && hipStreamSynchronize(s5->aux) != hipSuccess && !rc) rc = M2D_ERR_HIP;
-    (void)hipFree(work);
     assembly_stream(keep);
     if (res) {
         res->norm = norm;

@@ -45,13 +45,18 @@ template <typename T>
 // five: lu2d5 on the embedding (A's S, W, P, E, N arrays; dl1..dl4, du1..du3)
 void launch_factor(const T *a_sk, T *dl, T *du, T alpha0, const Skew &sk, Link lk,
                    int xcd_stride, hipStream_t s, bool five = false);
-// five: the 5-diagonal sweeps (msi2d5) on the same arrays (dl5 and du4 unused)
+// the LDS-staged sweeps, one launch each; five: the 5-diagonal sweeps
+// (msi2d5) on the same arrays (dl5 and du4 unused)
 template <typename T>
-void launch_forward(const T *r, const T *dl, T *w, const Skew &sk, Link lk, int xcd_stride,
-                    hipStream_t s, bool staged, bool five = false);
+void launch_forward_staged(const T *r, const T *dl, T *w, const Skew &sk, Link lk,
+                           int xcd_stride, hipStream_t s, bool five = false);
 template <typename T>
-void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s,
-                     bool staged, bool five = false);
+void launch_backward_staged(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride,
+                            hipStream_t s, bool five = false);
+// both sweeps of one iteration in one launch (w from r, then z over w)
+template <typename T>
+void launch_sweeps(const T *r, const T *dl, T *w, const T *du, const Skew &sk, Link lk,
+                   int xcd_stride, hipStream_t s, bool five = false);
 template <typename TI, typename TO>
 // unit: array whose padding entries get one (the diagonal of A), or -1
 void launch_to_skew(const TI *src, TO *dst, int nk, const Skew &sk, hipStream_t s,
@@ -89,15 +94,25 @@ void launch_prepare_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hip
 
 // ---- parallel kernels (msi9.hip) ---------------------------------------------
 // nd: 9 diagonals (msi2d9 slots) or 5 (msi2d5 slots S, W, P, E, N)
+// The norm and stopping rule that follow a residual: initial = 1 for the
+// residual of the incoming var (k = 0), otherwise the residual after
+// iteration k = st->iter + 1, which becomes st->iter; ||r|| of iteration k
+// goes to hist[k-1] for k <= hist_cap.
+struct NormRule {
+    double *hist;
+    int hist_cap;
+    int initial;
+    double gamma;
+    int nitm;
+};
+// r = A.x - rhs, its norm and the stopping rule, in one launch; partial
+// holds residual_partial_words(nx, ny) words, zero when first used
 template <typename TA, typename TX, typename TR>
-void launch_residual(const TA *a, const TX *x, const TA *rhs, TR *r, int nx, int ny,
-                     double *partial, const SolveState *st, const Skew *out_skew, hipStream_t s,
-                     int nd = 9);
+void launch_residual_norm(const TA *a, const TX *x, const TA *rhs, TR *r, int nx, int ny,
+                          unsigned long long *partial, SolveState *st, const Skew *out_skew,
+                          const NormRule &rule, hipStream_t s, int nd = 9);
 int residual_blocks(int nx, int ny);
-// initial = 1: the residual of the incoming var (k = 0); otherwise the
-// residual after iteration k = st->iter + 1, which becomes st->iter.
-void launch_norm_finalize(const double *partial, int np, SolveState *st, double *hist,
-                          int hist_cap, int initial, double gamma, int nitm, hipStream_t s);
+long residual_partial_words(int nx, int ny);
 void launch_check_outside(const double *a, int nx, int ny, SolveState *st, hipStream_t s,
                           int nd = 9);
 // gated: leave the state as it is when st->skip is set (a plain reset
@@ -119,6 +134,17 @@ void assembly_gate(const int *gate, const double *sc);
 // means at its end.
 void mean_cc_enqueue(int nx, int ny, int K, const double *const *a, const int *nd, double *t,
                      hipStream_t s);
+// the terms of those means only (t + k*n for array k), and K in-order sums of
+// consecutive arrays of n terms (t + k*n) into out[k], each on its own
+// workgroup of one launch
+void cc_terms_enqueue(int nx, int ny, int K, const double *const *a, const int *nd, double *t,
+                      hipStream_t s);
+void seq_sums_enqueue(const double *t, long n, int K, double *out, hipStream_t s);
+// check_mass_conservation's terms (in = rp, re, rn, Jp, ro, roa, ron, roe,
+// Uce, Vcn; device arrays) into t[0 .. (nx-2)(ny-2)), in the reference's
+// order, on s; their in-order sum is rmass
+void mass_terms_enqueue(int nx, int ny, double dt, const double *const *in, double *t,
+                        hipStream_t s);
 // tstepper_dt_scarborough's step rule on the four means cc (u, v, T, p')
 double scarborough_dt(double dt, const double *cc, double h0, double mincc, double maxcc);
 // get_average of the device field F enqueued on s: the mean lands in scr[0]

@@ -1017,29 +1017,42 @@ __device__ __forceinline__ void factor_band(const T *__restrict__ a, T *__restri
 // g / xcd_stride, so with xcd_stride 8 every band lands on the same XCD
 // (workgroups are dealt to the XCDs round robin) and shares its L2.
 // --------------------------------------------------------------------------
+// Both sweeps of one iteration in one launch: each band's workgroup runs its
+// forward sweep, then its backward sweep, which waits only for the band
+// above, whose forward sweep ended after this band's.  Boundary slot s
+// carries band s's forward edge to band s+1 and band s+1's backward edge to
+// band s, each under its own sweep's generation; band s+1 writes its
+// backward words only after its forward sweep has read every forward word of
+// the slot, so no forward word lands on a backward one.
 template <typename T, int F, bool Five>
-__global__ __launch_bounds__(2 * kWave) void k_forward(const T *__restrict__ r,
-                                                   const T *__restrict__ dl, T *__restrict__ w,
-                                                   Skew sk, Link lk0, int xcd_stride)
+__global__ __launch_bounds__(2 * kWave) void k_sweeps(const T *__restrict__ r,
+                                                  const T *__restrict__ dl, T *__restrict__ w,
+                                                  const T *__restrict__ du, Skew sk, Link lk0,
+                                                  int xcd_stride)
 {
     if (lk0.st->done) return;
     const int blk = blockIdx.x;
     if (blk % xcd_stride) return;
+    const int band = blk / xcd_stride;
     Link lk = lk0;
-    lk.gen = launch_gen(lk0);
-    forward_band<T, F, Five>(r, dl, w, sk, lk, blk / xcd_stride);
-}
-
-template <typename T, int F, bool Five>
-__global__ __launch_bounds__(2 * kWave) void k_backward(T *__restrict__ wz, const T *__restrict__ du,
-                                                    Skew sk, Link lk0, int xcd_stride)
-{
-    if (lk0.st->done) return;
-    const int blk = blockIdx.x;
-    if (blk % xcd_stride) return;
-    Link lk = lk0;
-    lk.gen = launch_gen(lk0);
-    backward_band<T, F, Five>(wz, du, sk, lk, sk.nb - 1 - blk / xcd_stride);
+    lk.phase = 1;
+    lk.gen = launch_gen(lk);
+    forward_band<T, F, Five>(r, dl, w, sk, lk, band);
+    // the band's w is stored (the storing wave's last chunk included) before
+    // the backward sweep loads it: the two waves share the CU, so waiting for
+    // the stores to complete is enough (no cache write-back); a band that
+    // gave up skips the backward sweep, on both waves alike
+    __shared__ int stop;
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (threadIdx.x == 0)
+        stop = __hip_atomic_load(&lk0.st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (stop) return;
+    lk.phase = 2;
+    lk.gen = launch_gen(lk);
+    backward_band<T, F, Five>(w, du, sk, lk, band);
 }
 
 template <typename T, int F, bool Five>
@@ -1362,37 +1375,41 @@ void launch_factor(const T *a_sk, T *dl, T *du, T alpha0, const Skew &sk, Link l
 }
 
 template <typename T>
-void launch_forward(const T *r, const T *dl, T *w, const Skew &sk, Link lk, int xcd_stride,
-                    hipStream_t s, bool staged, bool five)
+void launch_forward_staged(const T *r, const T *dl, T *w, const Skew &sk, Link lk,
+                           int xcd_stride, hipStream_t s, bool five)
 {
     constexpr int F = sweep_steps<T>();
     const dim3 g(sk.nb * xcd_stride), b(2 * kWave);
-    if (staged && five)
+    if (five)
         hipLaunchKernelGGL((k_forward_staged<T, F, true>), g, b, 0, s, r, dl, w, sk, lk, xcd_stride);
-    else if (staged)
+    else
         hipLaunchKernelGGL((k_forward_staged<T, F, false>), g, b, 0, s, r, dl, w, sk, lk,
                            xcd_stride);
-    else if (five)
-        hipLaunchKernelGGL((k_forward<T, F, true>), g, b, 0, s, r, dl, w, sk, lk, xcd_stride);
-    else
-        hipLaunchKernelGGL((k_forward<T, F, false>), g, b, 0, s, r, dl, w, sk, lk, xcd_stride);
 }
 
 template <typename T>
-void launch_backward(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride, hipStream_t s,
-                     bool staged, bool five)
+void launch_backward_staged(T *wz, const T *du, const Skew &sk, Link lk, int xcd_stride,
+                            hipStream_t s, bool five)
 {
     constexpr int F = sweep_steps<T>();
     const dim3 g(sk.nb * xcd_stride), b(2 * kWave);
-    if (staged && five)
+    if (five)
         hipLaunchKernelGGL((k_backward_staged<T, F, true>), g, b, 0, s, wz, du, sk, lk, xcd_stride);
-    else if (staged)
+    else
         hipLaunchKernelGGL((k_backward_staged<T, F, false>), g, b, 0, s, wz, du, sk, lk,
                            xcd_stride);
-    else if (five)
-        hipLaunchKernelGGL((k_backward<T, F, true>), g, b, 0, s, wz, du, sk, lk, xcd_stride);
+}
+
+template <typename T>
+void launch_sweeps(const T *r, const T *dl, T *w, const T *du, const Skew &sk, Link lk,
+                   int xcd_stride, hipStream_t s, bool five)
+{
+    constexpr int F = sweep_steps<T>();
+    const dim3 g(sk.nb * xcd_stride), b(2 * kWave);
+    if (five)
+        hipLaunchKernelGGL((k_sweeps<T, F, true>), g, b, 0, s, r, dl, w, du, sk, lk, xcd_stride);
     else
-        hipLaunchKernelGGL((k_backward<T, F, false>), g, b, 0, s, wz, du, sk, lk, xcd_stride);
+        hipLaunchKernelGGL((k_sweeps<T, F, false>), g, b, 0, s, r, dl, w, du, sk, lk, xcd_stride);
 }
 
 template <typename T>
@@ -1480,10 +1497,12 @@ void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipSt
 #define M2D_INST(T)                                                                           \
     template void launch_factor<T>(const T *, T *, T *, T, const Skew &, Link, int, hipStream_t, \
                                    bool);                                                     \
-    template void launch_forward<T>(const T *, const T *, T *, const Skew &, Link, int,          \
-                                    hipStream_t, bool, bool);                                 \
-    template void launch_backward<T>(T *, const T *, const Skew &, Link, int, hipStream_t, bool, \
-                                     bool);                                                   \
+    template void launch_forward_staged<T>(const T *, const T *, T *, const Skew &, Link, int,   \
+                                           hipStream_t, bool);                                 \
+    template void launch_backward_staged<T>(T *, const T *, const Skew &, Link, int,            \
+                                            hipStream_t, bool);                                \
+    template void launch_sweeps<T>(const T *, const T *, T *, const T *, const Skew &, Link,    \
+                                   int, hipStream_t, bool);                                    \
     template void launch_from_skew<T>(const T *, T *, int, const Skew &, hipStream_t);          \
     template void launch_prepare_factor<T>(T *, T *, const Skew &, hipStream_t, SolveState *);  \
     template void launch_factor_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t);  \

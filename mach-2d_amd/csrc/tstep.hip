@@ -47,15 +47,15 @@ __global__ __launch_bounds__(kT) void k_cc_terms(const double *__restrict__ a, i
     t[(long)(i - 2) * (ny - 2) + (j - 2)] = (s1 + s2) / ap;
 }
 
-// m = t(q) + m over the terms in order, then m / n; one workgroup per
-// array of n terms (t + blockIdx.x * n)
+// m = t(q) + m over the terms in order, then m / n (mean) or m; one
+// workgroup per array of n terms (t + blockIdx.x * n)
 __global__ __launch_bounds__(kSeqThreads) void k_seq_mean(const double *__restrict__ t, long n,
-                                                         double *__restrict__ out)
+                                                         double *__restrict__ out, int mean)
 {
     __shared__ double lds[kSeqSlots * kSeqChunk];
     const double *__restrict__ tt = t + (long)blockIdx.x * n;
     const double m = seq_sum(tt, n, lds);
-    if (threadIdx.x == 0) out[blockIdx.x] = m / (double)n;
+    if (threadIdx.x == 0) out[blockIdx.x] = mean ? m / (double)n : m;
 }
 
 // per-block minimum of 1/sqrt(Jp (u^2 + v^2)) over interior points
@@ -161,8 +161,8 @@ int mean_cc(int nx, int ny, int K, const double *const *a, const int *nd, double
 
 }  // namespace
 
-void m2d::mean_cc_enqueue(int nx, int ny, int K, const double *const *a, const int *nd,
-                          double *t, hipStream_t s)
+void m2d::cc_terms_enqueue(int nx, int ny, int K, const double *const *a, const int *nd,
+                           double *t, hipStream_t s)
 {
     const long n = (long)(nx - 2) * (ny - 2);
     const dim3 grid((unsigned)((nx - 2 + kT - 1) / kT), (unsigned)(ny - 2));
@@ -172,7 +172,19 @@ void m2d::mean_cc_enqueue(int nx, int ny, int K, const double *const *a, const i
         else
             hipLaunchKernelGGL(k_cc_terms<5>, grid, dim3(kT), 0, s, a[k], nx, ny, t + k * n);
     }
-    hipLaunchKernelGGL(k_seq_mean, dim3(K), dim3(kSeqThreads), 0, s, t, n, t + K * n);
+}
+
+void m2d::seq_sums_enqueue(const double *t, long n, int K, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_seq_mean, dim3(K), dim3(kSeqThreads), 0, s, t, n, out, 0);
+}
+
+void m2d::mean_cc_enqueue(int nx, int ny, int K, const double *const *a, const int *nd,
+                          double *t, hipStream_t s)
+{
+    const long n = (long)(nx - 2) * (ny - 2);
+    cc_terms_enqueue(nx, ny, K, a, nd, t, s);
+    hipLaunchKernelGGL(k_seq_mean, dim3(K), dim3(kSeqThreads), 0, s, t, n, t + K * n, 1);
 }
 
 // tstepper_dt_scarborough's rule (mod_tstepper_scarborough.f90:80-123) on

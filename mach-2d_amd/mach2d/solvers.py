@@ -400,8 +400,10 @@ class DeviceMSI:
         if rc:
             raise SolverError(rc, "m2d_set_tuning")
 
-    KERNELS = {"factor": 0, "forward": 1, "backward": 2, "residual": 3,
-               "factor32": 4, "forward32": 5, "backward32": 6, "residual32": 7}
+    # "sweeps": both sweeps of an iteration (one launch), or the forward
+    # sweep alone when the sweeps are staged (then "backward" times the other)
+    KERNELS = {"factor": 0, "sweeps": 1, "backward": 2, "residual": 3,
+               "factor32": 4, "sweeps32": 5, "backward32": 6, "residual32": 7}
 
     def enable_kernel_timing(self, on=True):
         _lib.load().m2d_enable_kernel_timing(self.h, 1 if on else 0)
