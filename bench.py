@@ -67,9 +67,8 @@ BYTES_PER_POINT = {
 # profiles/msi_1026x514_hbm_traffic.md).  "residual" is the update plus the
 # residual, as in the kernel timing.
 MEASURED_TRAFFIC = {
-    "forward": 48.68e6, "backward": 36.70e6, "residual": 57.18e6 + 19.66e6, "factor": 157.05e6,
-    "forward32": 21.35e6, "backward32": 18.37e6, "residual32": 28.63e6 + 10.25e6,
-    "factor32": 66.85e6,
+    "sweeps": 83.71e6, "residual": 57.40e6 + 19.66e6, "factor": 151.64e6,
+    "sweeps32": 38.12e6, "residual32": 28.84e6 + 10.25e6, "factor32": 65.35e6,
 }
 
 
