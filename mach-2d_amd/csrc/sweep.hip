@@ -264,11 +264,39 @@ __device__ void store_chunks(T *__restrict__ dst, const Skew &sk, int band, int 
     }
 }
 
+// Loads of a half chunk with padding masked off (the batch's sweeps, whose
+// small systems are mostly padding): buffer loads, whose out-of-range
+// offsets read zeros without a memory request.  A lane whose half lies
+// wholly outside its row's columns (or below the last row) gets that offset
+// and so zeros where the arrays hold the padding values; the zero padding
+// values that follow meet only zero couplings, as the stored ones do.
+constexpr int kNoOffset = int(0x80000000u);
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t array_rsrc(const void *base, long elems,
+                                                             int size)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0,
+                                             (int)(elems * size), 0x00020000);
+}
+template <typename T, int NA, int H>
+__device__ __forceinline__ void masked_half(Halves<T, NA, H> &b,
+                                            const __amdgpu_buffer_rsrc_t (&rs)[NA], int ofs)
+{
+    constexpr int V = 16 / (int)sizeof(T);
+#pragma unroll
+    for (int k = 0; k < NA; ++k)
+#pragma unroll
+        for (int q = 0; q < H / V; ++q) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs[k], ofs + q * 16, 0, 0);
+            __builtin_memcpy(&b.v[k][q * V], &v, 16);
+        }
+}
+
 // --------------------------------------------------------------------------
 // forward sweep  L w = -r  (msi2d9-v03.f90:139-156); r, dl (dl1..dl5, then
 // 1/dl1), w in the skewed layout.
 // --------------------------------------------------------------------------
-template <typename T, int F, bool Five>
+// Mask: padding loads masked off (masked_half).
+template <typename T, int F, bool Five, bool Mask = false>
 __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *__restrict__ dl,
                                              T *__restrict__ w, const Skew &sk, const Link &lk,
                                              int band)
@@ -302,8 +330,22 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
     T bv[F + 1];
     T cur = 0, se = 0, sv = 0, sw = 0;
 
+    __amdgpu_buffer_rsrc_t rs[NA];
+    const bool row_real = band * kWave + lane < sk.ny;
+    if constexpr (Mask) {
+#pragma unroll
+        for (int k = 0; k < NA; ++k)
+            rs[k] = array_rsrc(k == 0 ? r : dl + (long)Ops::fslot(k) * L, L, (int)sizeof(T));
+    }
     auto load_half = [&](Buf &b, int h) {  // half h = chunk h/2, elements (h%2)*H..
-        const long e = sk.chunk_base(band, min(h >> 1, nf - 1), 0) + (h & 1) * H;
+        const int c = min(h >> 1, nf - 1);
+        const long e = sk.chunk_base(band, c, 0) + (h & 1) * H;
+        if constexpr (Mask) {
+            const int i0 = c * F + (h & 1) * H - 2 * lane - pad;  // column of the first step
+            const bool on = row_real && i0 < nx && i0 + H > 0;
+            masked_half<T, NA, H>(b, rs, on ? (int)((e + lofs) * sizeof(T)) : kNoOffset);
+            return;
+        }
 #pragma unroll
         for (int k = 0; k < NA; ++k) {
             const T *p = (k == 0 ? r : dl + (long)Ops::fslot(k) * L) + e;
@@ -404,7 +446,8 @@ __device__ __forceinline__ void forward_band(const T *__restrict__ r, const T *_
 // backward sweep  U z = w  (msi2d9-v03.f90:158-176): chunks in reverse
 // stored order, z over w in place; bands top to bottom.
 // --------------------------------------------------------------------------
-template <typename T, int F, bool Five>
+// Mask: padding loads masked off (masked_half).
+template <typename T, int F, bool Five, bool Mask = false>
 __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__restrict__ du,
                                               const Skew &sk, const Link &lk, int band)
 {
@@ -435,9 +478,23 @@ __device__ __forceinline__ void backward_band(T *__restrict__ wz, const T *__res
     T bv[F + 1];
     T cur = 0, nw = 0, nv = 0, ne = 0;
 
+    __amdgpu_buffer_rsrc_t rs[NA];
+    const bool row_real = band * kWave + lane < sk.ny;
+    if constexpr (Mask) {
+#pragma unroll
+        for (int k = 0; k < NA; ++k)
+            rs[k] = array_rsrc(k == 0 ? (const T *)wz : du + (long)(k - 1) * L, L, (int)sizeof(T));
+    }
     // backward half h: stored chunk nf-1-h/2, its upper half first
     auto load_half = [&](Buf &b, int h) {
-        const long e = sk.chunk_base(band, max(nf - 1 - (h >> 1), 0), 0) + (1 - (h & 1)) * H;
+        const int sc = max(nf - 1 - (h >> 1), 0);
+        const long e = sk.chunk_base(band, sc, 0) + (1 - (h & 1)) * H;
+        if constexpr (Mask) {
+            const int i0 = sc * F + (1 - (h & 1)) * H - 2 * lane - pad;
+            const bool on = row_real && i0 < nx && i0 + H > 0;
+            masked_half<T, NA, H>(b, rs, on ? (int)((e + lofs) * sizeof(T)) : kNoOffset);
+            return;
+        }
 #pragma unroll
         for (int k = 0; k < NA; ++k) {
             const T *p = (k == 0 ? wz : du + (long)(k - 1) * L) + e;
@@ -1122,7 +1179,7 @@ __global__ __launch_bounds__(2 * kWave) void k_forward_batch(const BatchSys<T> *
     if (m.x < 0) return;
     const BatchSys<T> &d = sys[m.x];
     if (d.st->done) return;
-    forward_band<T, F, Five>(d.r, d.dl, d.w, d.sk, batch_link(d, 1), m.y);
+    forward_band<T, F, Five, true>(d.r, d.dl, d.w, d.sk, batch_link(d, 1), m.y);
 }
 
 template <typename T, int F, bool Five>
@@ -1133,7 +1190,7 @@ __global__ __launch_bounds__(2 * kWave) void k_backward_batch(const BatchSys<T> 
     if (m.x < 0) return;
     const BatchSys<T> &d = sys[m.x];
     if (d.st->done) return;
-    backward_band<T, F, Five>(d.w, d.du, d.sk, batch_link(d, 2), m.y);
+    backward_band<T, F, Five, true>(d.w, d.du, d.sk, batch_link(d, 2), m.y);
 }
 
 template <typename T, int F, bool Five>
