@@ -46,12 +46,13 @@ struct Work {
     T *du = nullptr;  // 4 arrays
     T *r = nullptr;   // residual
     T *w = nullptr;   // forward result, then z
+    T *x2 = nullptr;  // the iterate after an odd number of iterations (reference layout)
     void release()
     {
-        T *b[] = {a, dl, du, r, w};
+        T *b[] = {a, dl, du, r, w, x2};
         for (T *p : b)
             if (p) (void)hipFree(p);
-        a = dl = du = r = w = nullptr;
+        a = dl = du = r = w = x2 = nullptr;
     }
     bool alloc(int nx, int ny)
     {
@@ -61,7 +62,8 @@ struct Work {
         bool ok = hipMalloc((void **)&a, 9 * z) == hipSuccess &&
                   hipMalloc((void **)&dl, 6 * z) == hipSuccess &&
                   hipMalloc((void **)&du, 4 * z) == hipSuccess &&
-                  hipMalloc((void **)&r, z) == hipSuccess && hipMalloc((void **)&w, z) == hipSuccess;
+                  hipMalloc((void **)&r, z) == hipSuccess && hipMalloc((void **)&w, z) == hipSuccess &&
+                  hipMalloc((void **)&x2, sizeof(T) * (size_t)nx * ny) == hipSuccess;
         // padding entries are never consumed; start them finite.  dl5 and
         // du4 stay zero for a 5-diagonal system (its factorization writes
         // neither)
@@ -449,9 +451,9 @@ static void enqueue_iteration(m2d_system *s, Work<T> &W, SolveState *st, const T
         });
     }
     timed(s, KT_RESIDUAL + off, k, [&] {
-        launch_update<T>(x, W.w, W.sk, st, s->stream);
-        launch_residual_norm<T, T, T>(a, x, rhs, W.r, s->nx, s->ny, s->partial, st, &W.sk,
-                                      NormRule{hist, hist_cap, 0, gamma, nitm}, s->stream, s->nd);
+        launch_update_residual_norm<T>(a, x, W.x2, W.w, rhs, W.r, W.sk, s->partial, st,
+                                       NormRule{hist, hist_cap, 0, gamma, nitm}, s->stream,
+                                       s->nd);
     });
 }
 
@@ -477,6 +479,8 @@ static int run_msi(m2d_system *s, Work<T> &W, SolveState *st, const T *a, const 
         if (s->h_st->done) break;
         chunk = std::min(chunk * 2, s->chunk);
     }
+    launch_settle<T>(x, W.x2, s->N, st, s->stream);
+    HIP_OK(hipGetLastError());
     harvest(s, s->h_st->nit);
     return M2D_OK;
 }
@@ -846,6 +850,7 @@ static void enqueue_solve64(m2d_system *s, const double *a, const double *rhs, d
                                                  s->stream, s->nd);
     for (int k = 1; k <= nitm; ++k)
         enqueue_iteration<double>(s, W, s->st, a, rhs, x, nullptr, 0, k, gamma, nitm);
+    launch_settle<double>(x, W.x2, s->N, s->st, s->stream);
 }
 
 // `count` iterations of the pressure-correction loop (mach2d_main.f90:261-309)
@@ -1232,7 +1237,7 @@ extern "C" void m2d_solve_msi5d(const int *nx, const int *ny, const int *nitm, c
 // ---------------------------------------------------------------------------
 // class_solver_msi9d%solve on device arrays without allocation.  Every
 // device buffer the solve needs beyond the caller's a, b and sol (the skewed
-// copy of A, the factor, r, w/z, the residual partial sums, the boundary
+// copy of A, the factor, r, w/z, the second iterate, the residual partial sums, the boundary
 // words and the solve state) is carved from one file-scope device array.
 // Each GPU holds its own copy of the array from the library's load until
 // the process ends, and every call on that GPU reuses it: calls that
@@ -1266,7 +1271,7 @@ static int scratch_address(unsigned char **out)
 // Byte offsets of the solve's buffers inside `scratch` (each 256-byte
 // aligned); returns the bytes needed.
 struct NoAllocLayout {
-    size_t a, dl, du, r, w, partial, bnd, st, total;
+    size_t a, dl, du, r, w, x2, partial, bnd, st, total;
 };
 static NoAllocLayout noalloc_layout(int nx, int ny)
 {
@@ -1278,7 +1283,8 @@ static NoAllocLayout noalloc_layout(int nx, int ny)
     o.du = o.dl + up(6 * L);
     o.r = o.du + up(4 * L);
     o.w = o.r + up(L);
-    o.partial = o.w + up(L);
+    o.x2 = o.w + up(L);
+    o.partial = o.x2 + up(sizeof(double) * (size_t)nx * ny);
     o.bnd = o.partial + up(sizeof(u64) * (size_t)residual_partial_words(nx, ny));
     o.st = o.bnd + up(sizeof(u64) * (size_t)(boundary_words(nx, ny) + scratch_words(ny)));
     o.total = o.st + up(sizeof(SolveState));
@@ -1327,6 +1333,7 @@ extern "C" int m2d_solve_msi9d_noalloc(const int *nx, const int *ny, const int *
     s.w64.du = (double *)(base + o.du);
     s.w64.r = (double *)(base + o.r);
     s.w64.w = (double *)(base + o.w);
+    s.w64.x2 = (double *)(base + o.x2);
     s.partial = (u64 *)(base + o.partial);
     s.bnd = (u64 *)(base + o.bnd);
     s.st = (SolveState *)(base + o.st);

@@ -73,8 +73,6 @@ void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s,
                            SolveState *st = nullptr);
 template <typename T>
 void launch_from_skew(const T *src, T *dst, int nk, const Skew &sk, hipStream_t s);
-template <typename T>
-void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipStream_t s);
 
 // ---- batched wavefront and layout kernels (sweep.hip) -------------------------
 template <typename T>
@@ -111,6 +109,15 @@ template <typename TA, typename TX, typename TR>
 void launch_residual_norm(const TA *a, const TX *x, const TA *rhs, TR *r, int nx, int ny,
                           unsigned long long *partial, SolveState *st, const Skew *out_skew,
                           const NormRule &rule, hipStream_t s, int nd = 9);
+// the update var = z + var (z skewed by sk) and then the residual, its norm
+// and the stopping rule, in one launch; the iterate alternates between x
+// (even number of iterations done) and x2 (odd), launch_settle returns it
+template <typename T>
+void launch_update_residual_norm(const T *a, T *x, T *x2, const T *z, const T *rhs, T *r,
+                                 const Skew &sk, unsigned long long *partial, SolveState *st,
+                                 const NormRule &rule, hipStream_t s, int nd = 9);
+template <typename T>
+void launch_settle(T *x, const T *x2, long n, const SolveState *st, hipStream_t s);
 int residual_blocks(int nx, int ny);
 long residual_partial_words(int nx, int ny);
 void launch_check_outside(const double *a, int nx, int ny, SolveState *st, hipStream_t s,

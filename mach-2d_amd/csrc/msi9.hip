@@ -131,9 +131,17 @@ __device__ void norm_rule_block(const u64 *__restrict__ pw, int np, unsigned tag
 // grid's last workgroup (dispatched last; it holds one slot while it waits,
 // so the others run whatever the dispatch order) then waits for every sum,
 // adds them, applies the stopping rule and records the generation.
-template <typename TA, typename TX, typename TR, int ND>
+//
+// Upd: the update var = z + var (msi2d9-v03.f90:181) comes first, in the
+// same launch.  The iterate alternates between x (an even number of
+// iterations done) and x2 (odd): the kernel reads one and writes the other,
+// forming each stencil neighbour's updated value z + var where it reads it
+// (z skewed by osk), the sum the update stores, so r is bit for bit the
+// residual of the updated iterate.  settle returns an iterate left in x2.
+template <typename TA, typename TX, typename TR, int ND, bool Upd>
 __global__ __launch_bounds__(kResThreads) void k_residual(const TA *__restrict__ a,
-                                                          const TX *__restrict__ x,
+                                                          TX *__restrict__ x, TX *__restrict__ x2,
+                                                          const TR *__restrict__ z,
                                                           const TA *__restrict__ rhs,
                                                           TR *__restrict__ r, int nx, int ny,
                                                           u64 *__restrict__ pw, SolveState *st,
@@ -153,9 +161,21 @@ __global__ __launch_bounds__(kResThreads) void k_residual(const TA *__restrict__
         double acc = 0.0;
         if (i < nx) {
             const long p = (long)j * nx + i;
-            const TR v = ND == 5
-                             ? residual_point5<TA, TX, TR>(a, N, x, (TR)rhs[p], i, j, nx, ny)
-                             : residual_point<TA, TX, TR>(a, N, x, (TR)rhs[p], i, j, nx, ny);
+            TR v;
+            if constexpr (Upd) {
+                const bool odd = st->iter & 1;
+                const TX *__restrict__ xin = odd ? x2 : x;
+                TX *__restrict__ xout = odd ? x : x2;
+                auto xv = [&](int ii, int jj, long f) -> TR {
+                    return z[osk.template of_point_c<64 / (int)sizeof(TR)>(ii, jj)] + (TR)xin[f];
+                };
+                v = ND == 5 ? residual_point5_f<TA, TR>(a, N, xv, (TR)rhs[p], i, j, nx, ny)
+                            : residual_point_f<TA, TR>(a, N, xv, (TR)rhs[p], i, j, nx, ny);
+                xout[p] = (TX)xv(i, j, p);
+            } else {
+                v = ND == 5 ? residual_point5<TA, TX, TR>(a, N, x, (TR)rhs[p], i, j, nx, ny)
+                            : residual_point<TA, TX, TR>(a, N, x, (TR)rhs[p], i, j, nx, ny);
+            }
             // skewed output when osk.S > 0 (chunk length 64 / sizeof(TR))
             r[osk.S ? osk.template of_point_c<64 / (int)sizeof(TR)>(i, j) : p] = v;
             acc = (double)v * (double)v;
@@ -174,6 +194,17 @@ __global__ __launch_bounds__(kResThreads) void k_residual(const TA *__restrict__
     norm_rule_block(pw, np, tag, st, rule);
     if (threadIdx.x == 0)
         __hip_atomic_store(pw + 2L * np, (u64)tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// After an MSI run: an iterate left in x2 (an odd number of iterations)
+// goes back to x.
+template <typename T>
+__global__ __launch_bounds__(256) void k_settle(T *__restrict__ x, const T *__restrict__ x2,
+                                                long n, const SolveState *st)
+{
+    if (!(st->iter & 1)) return;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) x[q] = x2[q];
 }
 
 int residual_blocks(int nx, int ny)
@@ -246,12 +277,39 @@ void launch_residual_norm(const TA *a, const TX *x, const TA *rhs, TR *r, int nx
     Skew osk{};
     if (out_skew) osk = *out_skew;
     assert(!out_skew || osk.S == 64 / (int)sizeof(TR));
+    TX *xm = const_cast<TX *>(x);  // read only without the update
     if (nd == 5)
-        hipLaunchKernelGGL((k_residual<TA, TX, TR, 5>), dim3(8 * per), dim3(kResThreads), 0, s, a,
-                           x, rhs, r, nx, ny, partial, st, osk, nbx, per, rule);
+        hipLaunchKernelGGL((k_residual<TA, TX, TR, 5, false>), dim3(8 * per), dim3(kResThreads), 0,
+                           s, a, xm, xm, (const TR *)nullptr, rhs, r, nx, ny, partial, st, osk,
+                           nbx, per, rule);
     else
-        hipLaunchKernelGGL((k_residual<TA, TX, TR, 9>), dim3(8 * per), dim3(kResThreads), 0, s, a,
-                           x, rhs, r, nx, ny, partial, st, osk, nbx, per, rule);
+        hipLaunchKernelGGL((k_residual<TA, TX, TR, 9, false>), dim3(8 * per), dim3(kResThreads), 0,
+                           s, a, xm, xm, (const TR *)nullptr, rhs, r, nx, ny, partial, st, osk,
+                           nbx, per, rule);
+}
+
+template <typename T>
+void launch_update_residual_norm(const T *a, T *x, T *x2, const T *z, const T *rhs, T *r,
+                                 const Skew &sk, u64 *partial, SolveState *st,
+                                 const NormRule &rule, hipStream_t s, int nd)
+{
+    const int nx = sk.nx, ny = sk.ny;
+    const int nbx = (nx + kResThreads - 1) / kResThreads;
+    const int per = (nbx * ny + 7) / 8;
+    assert(sk.S == 64 / (int)sizeof(T));
+    if (nd == 5)
+        hipLaunchKernelGGL((k_residual<T, T, T, 5, true>), dim3(8 * per), dim3(kResThreads), 0, s,
+                           a, x, x2, z, rhs, r, nx, ny, partial, st, sk, nbx, per, rule);
+    else
+        hipLaunchKernelGGL((k_residual<T, T, T, 9, true>), dim3(8 * per), dim3(kResThreads), 0, s,
+                           a, x, x2, z, rhs, r, nx, ny, partial, st, sk, nbx, per, rule);
+}
+
+template <typename T>
+void launch_settle(T *x, const T *x2, long n, const SolveState *st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_settle<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, x2, n,
+                       st);
 }
 
 void launch_check_outside(const double *a, int nx, int ny, SolveState *st, hipStream_t s, int nd)
@@ -277,6 +335,17 @@ template void launch_residual_norm<double, double, double>(const double *, const
                                                            const double *, double *, int, int,
                                                            u64 *, SolveState *, const Skew *,
                                                            const NormRule &, hipStream_t, int);
+template void launch_update_residual_norm<double>(const double *, double *, double *,
+                                                  const double *, const double *, double *,
+                                                  const Skew &, u64 *, SolveState *,
+                                                  const NormRule &, hipStream_t, int);
+template void launch_update_residual_norm<float>(const float *, float *, float *, const float *,
+                                                 const float *, float *, const Skew &, u64 *,
+                                                 SolveState *, const NormRule &, hipStream_t,
+                                                 int);
+template void launch_settle<double>(double *, const double *, long, const SolveState *,
+                                    hipStream_t);
+template void launch_settle<float>(float *, const float *, long, const SolveState *, hipStream_t);
 template void launch_residual_norm<float, float, float>(const float *, const float *,
                                                         const float *, float *, int, int,
                                                         u64 *, SolveState *, const Skew *,

@@ -228,15 +228,18 @@ M2D_HD T backward_point5_fused(T w, T du1, T du2, T du3, T z_e, T z_nw, T z_n)
 enum Slot5 { k5S = 0, k5W = 1, k5P = 2, k5E = 3, k5N = 4 };
 
 // Residual of a 5-diagonal row, terms in slot order from -rhs, neighbours
-// outside the grid omitted (the row classes of msi2d5-v03.f90:59-131).
-template <typename TA, typename TX, typename TR>
-M2D_HD TR residual_point5(const TA *a, long ld, const TX *x, TR rhs, int i, int j, int nx, int ny)
+// outside the grid omitted (the row classes of msi2d5-v03.f90:59-131).  The
+// iterate is read through xv(i', j', flat index) of the neighbour.
+template <typename TA, typename TR, typename XV>
+M2D_HD TR residual_point5_f(const TA *a, long ld, XV xv, TR rhs, int i, int j, int nx, int ny)
 {
     const long p = (long)j * nx + i;
     const bool hw = i > 0, he = i < nx - 1, hs = j > 0, hn = j < ny - 1;
     // operands read unconditionally (clamped neighbours), absent terms skipped
-    const TR x_s = (TR)x[hs ? p - nx : p], x_w = (TR)x[hw ? p - 1 : p], x_p = (TR)x[p];
-    const TR x_e = (TR)x[he ? p + 1 : p], x_n = (TR)x[hn ? p + nx : p];
+    const int iw = hw ? i - 1 : i, ie = he ? i + 1 : i, js = hs ? j - 1 : j, jn = hn ? j + 1 : j;
+    const TR x_s = (TR)xv(i, js, hs ? p - nx : p), x_w = (TR)xv(iw, j, hw ? p - 1 : p);
+    const TR x_p = (TR)xv(i, j, p);
+    const TR x_e = (TR)xv(ie, j, he ? p + 1 : p), x_n = (TR)xv(i, jn, hn ? p + nx : p);
     TR c[5];
     for (int k = 0; k < 5; ++k) c[k] = (TR)a[k * ld + p];
     TR acc = -rhs;
@@ -246,6 +249,14 @@ M2D_HD TR residual_point5(const TA *a, long ld, const TX *x, TR rhs, int i, int 
     if (he) acc += c[k5E] * x_e;
     if (hn) acc += c[k5N] * x_n;
     return acc;
+}
+
+// residual_point5_f on an iterate array x.
+template <typename TA, typename TX, typename TR>
+M2D_HD TR residual_point5(const TA *a, long ld, const TX *x, TR rhs, int i, int j, int nx, int ny)
+{
+    return residual_point5_f<TA, TR>(
+        a, ld, [x](int, int, long q) { return x[q]; }, rhs, i, j, nx, ny);
 }
 
 // Residual r = A.x - rhs at (i, j) (0-based), 2-D aware: a term whose

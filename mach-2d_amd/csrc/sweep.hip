@@ -1372,19 +1372,6 @@ __global__ __launch_bounds__(256) void k_from_skew(const T *__restrict__ src, T 
     for (int k = 0; k < nk; ++k) dst[k * N + p] = src[k * L + e];
 }
 
-// var = z + var (msi2d9-v03.f90:181), z skewed, var in the reference layout.
-// One workgroup per row segment of 256 columns (blockIdx.y = row).
-template <typename T>
-__global__ __launch_bounds__(256) void k_update(T *__restrict__ x, const T *__restrict__ z, Skew sk,
-                                                const SolveState *st)
-{
-    if (st->done) return;
-    const int j = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= sk.nx) return;
-    const long p = (long)j * sk.nx + i;
-    x[p] = z[sk.template of_point_c<64 / (int)sizeof(T)>(i, j)] + x[p];
-}
-
 // Clear the boundary words (generations restart with every solve).
 __global__ void k_clear_words(u64 *p, long n)
 {
@@ -1544,12 +1531,6 @@ void launch_from_skew(const T *src, T *dst, int nk, const Skew &sk, hipStream_t 
                        dst, nk, sk);
 }
 
-template <typename T>
-void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_update<T>, dim3((sk.nx + 255) / 256, sk.ny), dim3(256), 0, s, x, z, sk,
-                       st);
-}
 
 #define M2D_INST(T)                                                                           \
     template void launch_factor<T>(const T *, T *, T *, T, const Skew &, Link, int, hipStream_t, \
@@ -1569,7 +1550,7 @@ void launch_update(T *x, const T *z, const Skew &sk, const SolveState *st, hipSt
                                            bool, bool);                                       \
     template void launch_to_skew_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t); \
     template void launch_prepare_batch<T>(const BatchSys<T> *, const int2 *, int, hipStream_t); \
-    template void launch_update<T>(T *, const T *, const Skew &, const SolveState *, hipStream_t);
+
 M2D_INST(double)
 M2D_INST(float)
 template void launch_to_skew<double, double>(const double *, double *, int, const Skew &,
