@@ -57,18 +57,18 @@ BYTES_PER_POINT = {
     "factor": 18,    # read a (9), write dl (5) and du (4)
     "sweeps": 13,    # forward: read r, dl (5), write w; backward: read w, du (4), write z
     "backward": 6,   # staged sweeps only: the backward sweep alone
-    "residual": 15,  # update: read z, var, write var; residual: read a (9), rhs, var, write r
+    "residual": 14,  # one launch: read z, var, a (9), rhs; write var, r
 }
 
 
 # HBM bytes per launch at 1026x514, measured with rocprofv3 (FETCH_SIZE and
-# WRITE_SIZE in separate runs of this script, scaled by 2.2496 and 1.1250:
+# WRITE_SIZE in separate runs of this script, scaled by 2.2488 and 1.1250:
 # the factors a conversion kernel of known size reads;
-# profiles/msi_1026x514_hbm_traffic.md).  "residual" is the update plus the
-# residual, as in the kernel timing.
+# profiles/msi_1026x514_hbm_traffic.md).  "residual" is the update and the
+# residual of one launch.
 MEASURED_TRAFFIC = {
-    "sweeps": 83.71e6, "residual": 57.40e6 + 19.66e6, "factor": 151.64e6,
-    "sweeps32": 38.12e6, "residual32": 28.84e6 + 10.25e6, "factor32": 65.35e6,
+    "sweeps": 83.84e6, "residual": 67.14e6, "factor": 151.67e6,
+    "sweeps32": 38.02e6, "residual32": 33.75e6, "factor32": 65.34e6,
 }
 
 
