@@ -218,7 +218,8 @@ int m2d_get_factor(m2d_system *s, double *d_dl, double *d_du);
 /* Launch-shape tuning of the wavefront kernels (autotuner hook):
  * publish_every = columns between progress publications of a band;
  * xcd_stride = 1 (bands on consecutive XCDs) or 8 (bands of one system on
- * one XCD). */
+ * one XCD).  Default: 8 for systems of five or more 64-row bands (ny > 256),
+ * else 1. */
 int m2d_set_tuning(m2d_system *s, int publish_every, int xcd_stride);
 
 /* Sweep operand path: 1 = the LDS-staged sweeps (a second wave loads each

@@ -104,6 +104,11 @@ struct m2d_system {
     // 5-diagonal residual and sweeps)
     int nd = 9;
     hipStream_t stream = nullptr;
+    // 8: the bands of the system on one XCD (workgroups are dealt to the
+    // XCDs round robin, every eighth runs a band); 1: on consecutive XCDs.
+    // The default (m2d_create_stencil) follows measurement: 8 from five
+    // bands up (1026x514 solves 5-6 % faster), 1 below (258x130 time step
+    // 1-2 % faster)
     int xcd_stride = 1;
     // sweep operands through the LDS ring (m2d_set_staging); off by default:
     // a single system's working set sits in the last-level cache, where the
@@ -274,6 +279,7 @@ extern "C" int m2d_create_stencil(int nx, int ny, int precision, int ndiag, m2d_
     s->nx = nx;
     s->ny = ny;
     s->N = (long)nx * ny;
+    s->xcd_stride = (ny + 63) / 64 >= 5 ? 8 : 1;
     s->precision = precision;
     s->np = residual_blocks(nx, ny);
     s->nbnd = boundary_words(nx, ny) + scratch_words(ny);
@@ -1321,6 +1327,7 @@ extern "C" int m2d_solve_msi9d_noalloc(const int *nx, const int *ny, const int *
     SolveState h_st;
     s.nx = *nx;
     s.ny = *ny;
+    s.xcd_stride = (*ny + 63) / 64 >= 5 ? 8 : 1;  // as m2d_create_stencil
     s.N = (long)*nx * *ny;
     s.nd = 9;
     s.precision = M2D_FP64;
