@@ -135,6 +135,10 @@ struct m2d_system {
     SolveState *st = nullptr;     // device
     SolveState *st_in = nullptr;  // device, inner solve (mixed precision)
     SolveState *h_st = nullptr;   // pinned host mirror
+    // run_msi's pipelined state reads: two pinned states after consecutive
+    // chunks of iterations and their events (null: one chunk at a time)
+    SolveState *h_chunk = nullptr;
+    hipEvent_t ev_chunk[2] = {nullptr, nullptr};
     bool factored = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -262,6 +266,9 @@ extern "C" int m2d_destroy(m2d_system *s)
     if (s->pl) (void)hipFree(s->pl);
     if (s->h_pl) (void)hipHostFree(s->h_pl);
     if (s->h_st) (void)hipHostFree(s->h_st);
+    if (s->h_chunk) (void)hipHostFree(s->h_chunk);
+    for (hipEvent_t e : s->ev_chunk)
+        if (e) (void)hipEventDestroy(e);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     delete s;
@@ -297,6 +304,10 @@ extern "C" int m2d_create_stencil(int nx, int ny, int precision, int ndiag, m2d_
     }
     ok = ok && hipHostMalloc((void **)&s->h_st, sizeof(SolveState), hipHostMallocDefault) ==
                    hipSuccess;
+    ok = ok && hipHostMalloc((void **)&s->h_chunk, 2 * sizeof(SolveState),
+                             hipHostMallocDefault) == hipSuccess;
+    for (hipEvent_t &e : s->ev_chunk)
+        ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipEventCreate(&s->ev0) == hipSuccess && hipEventCreate(&s->ev1) == hipSuccess;
     if (!ok) {
         m2d_destroy(s);
@@ -465,6 +476,10 @@ static void enqueue_iteration(m2d_system *s, Work<T> &W, SolveState *st, const T
 
 // fb2d9 on the device, in precision T, from the current x (reference
 // layout); returns after the stopping rule fired (state in s->h_st).
+// Iterations go out in chunks (2, 4, ... up to s->chunk), each followed by
+// a copy of the state; the next chunk is enqueued before the host waits for
+// the previous one's state, so the device never waits for the host, and a
+// chunk enqueued after the stopping rule fired turns into no-ops.
 template <typename T>
 static int run_msi(m2d_system *s, Work<T> &W, SolveState *st, const T *a, const T *rhs, T *x,
                    double gamma, int nitm, double *hist, int hist_cap)
@@ -473,17 +488,47 @@ static int run_msi(m2d_system *s, Work<T> &W, SolveState *st, const T *a, const 
     launch_residual_norm<T, T, T>(a, x, rhs, W.r, s->nx, s->ny, s->partial, st, &W.sk,
                                   NormRule{nullptr, 0, 1, gamma, nitm}, s->stream, s->nd);
     const int kmax = nitm > 0 ? nitm : kIterCap;
+    const bool piped = s->h_chunk && s->ev_chunk[0] && s->ev_chunk[1];
     int k = 1;
     int chunk = 2;  // grows geometrically up to s->chunk
-    while (k <= kmax) {
+    auto enqueue_chunk = [&](int slot) -> int {
         const int kend = std::min(kmax, k + chunk - 1);
         for (; k <= kend; ++k)
             enqueue_iteration<T>(s, W, st, a, rhs, x, hist, hist_cap, k, gamma, nitm);
-        HIP_OK(hipGetLastError());
-        int rc = read_state(s, st);
-        if (rc) return rc;
-        if (s->h_st->done) break;
         chunk = std::min(chunk * 2, s->chunk);
+        HIP_OK(hipMemcpyAsync(s->h_chunk + slot, st, sizeof(SolveState), hipMemcpyDeviceToHost,
+                              s->stream));
+        HIP_OK(hipEventRecord(s->ev_chunk[slot], s->stream));
+        return M2D_OK;
+    };
+    if (!piped) {
+        while (k <= kmax) {
+            const int kend = std::min(kmax, k + chunk - 1);
+            for (; k <= kend; ++k)
+                enqueue_iteration<T>(s, W, st, a, rhs, x, hist, hist_cap, k, gamma, nitm);
+            HIP_OK(hipGetLastError());
+            int rc = read_state(s, st);
+            if (rc) return rc;
+            if (s->h_st->done) break;
+            chunk = std::min(chunk * 2, s->chunk);
+        }
+    } else {
+        // chunks in flight, oldest first in slot head, the next in head ^ 1
+        int head = 0, inflight = 0;
+        for (;;) {
+            while (k <= kmax && inflight < 2) {  // one chunk queued behind the oldest
+                const int rc = enqueue_chunk(head ^ inflight);
+                if (rc) return rc;
+                ++inflight;
+            }
+            if (inflight == 0) break;
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipEventSynchronize(s->ev_chunk[head]));
+            *s->h_st = s->h_chunk[head];
+            head ^= 1;
+            --inflight;
+            if (s->h_st->done) break;
+        }
     }
     launch_settle<T>(x, W.x2, s->N, st, s->stream);
     HIP_OK(hipGetLastError());
