@@ -281,6 +281,35 @@ __device__ __forceinline__ void tdma_issue(double *slot, const double *const (&a
     }
 }
 
+// tdma_issue for a chunk that lies wholly inside the line, where step(s) is
+// t0 + s (forward) or t0 - s (backward): the address of load ii of every
+// array is the lane's fixed offset goff[ii] = (+-(f / R)) * LP + l0 + f % R
+// plus the wave-uniform t0 * LP, so no lane recomputes a 64-bit product.
+template <int NA>
+__device__ __forceinline__ void tdma_issue_at(double *slot, const double *const (&arr)[NA],
+                                              long base, const long (&goff)[kTdmaNI])
+{
+#pragma unroll
+    for (int ii = 0; ii < kTdmaNI; ++ii) {
+        const long g = base + goff[ii];
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+            __builtin_amdgcn_global_load_lds((tdma_gvoid *)(arr[a] + g),
+                                             (tdma_lvoid *)(slot + a * kTdmaChunk + 128 * ii), 16,
+                                             0, 0);
+    }
+}
+template <int R>
+__device__ __forceinline__ void tdma_lane_offsets(long (&goff)[kTdmaNI], long LP, int l0,
+                                                  int lane, int dir)
+{
+#pragma unroll
+    for (int ii = 0; ii < kTdmaNI; ++ii) {
+        const int f = 128 * ii + 2 * lane;
+        goff[ii] = (long)(dir * (f / R)) * LP + l0 + (f % R);
+    }
+}
+
 // Block b of a chain launch over one pass -> (system, first line)
 struct TdmaBlock {
     int q, l0, len;
@@ -399,9 +428,15 @@ __global__ __launch_bounds__(256) void k_tdma_line(TdmaBatch B)
                                            tdma_arr(B, S, XPASS, kTDen),
                                            tdma_arr(B, S, XPASS, kTRd), tdma_arr(B, S, XPASS, kTD)};
             const int nch = (len + K - 1) / K;
+            long goff[kTdmaNI];
+            tdma_lane_offsets<R>(goff, k.LP, k.l0, lane, 1);
             auto issue = [&](int c) {
-                tdma_issue<NA, R>(ring + (c % kTdmaSlots) * NA * kTdmaChunk, arr, k.LP, k.l0,
-                                  lane, [&](int s) { return min(c * K + s, len - 1); });
+                double *slot = ring + (c % kTdmaSlots) * NA * kTdmaChunk;
+                if (c * K + K <= len)
+                    tdma_issue_at<NA>(slot, arr, (long)(c * K) * k.LP, goff);
+                else
+                    tdma_issue<NA, R>(slot, arr, k.LP, k.l0, lane,
+                                      [&](int s) { return min(c * K + s, len - 1); });
             };
             for (int c = 0; c < kTdmaSlots - 1; ++c) issue(c);
             double Qm = 0.0;
@@ -448,9 +483,15 @@ __global__ __launch_bounds__(256) void k_tdma_line(TdmaBatch B)
         if (len > 1) {
             const double *const arr[1] = {tdma_arr(B, S, XPASS, kTP)};
             const int nst = len - 1, nch = (nst + K - 1) / K;
+            long goff[kTdmaNI];
+            tdma_lane_offsets<R>(goff, k.LP, k.l0, lane, -1);
             auto issue = [&](int c) {
-                tdma_issue<1, R>(ring + (c % kTdmaSlots) * kTdmaChunk, arr, k.LP, k.l0, lane,
-                                 [&](int s) { return max(len - 2 - c * K - s, 0); });
+                double *slot = ring + (c % kTdmaSlots) * kTdmaChunk;
+                if (len - 2 - c * K - (K - 1) >= 0)
+                    tdma_issue_at<1>(slot, arr, (long)(len - 2 - c * K) * k.LP, goff);
+                else
+                    tdma_issue<1, R>(slot, arr, k.LP, k.l0, lane,
+                                     [&](int s) { return max(len - 2 - c * K - s, 0); });
             };
             for (int c = 0; c < kTdmaSlots - 1; ++c) issue(c);
             double xv = tile[(len - 1) * RS + rl];
