@@ -447,21 +447,30 @@ __global__ __launch_bounds__(256) void k_tdma_line(TdmaBatch B)
                 const double *slo = sl, *sden = sl + kTdmaChunk, *srd = sl + 2 * kTdmaChunk,
                              *sD = sl + 3 * kTdmaChunk;
                 const int t0 = c * K;
-                if (c > 0 && t0 + K <= len) {  // full chunk past step 0: 8 steps at a time
-                    for (int s0 = 0; s0 < K; s0 += 8) {
-                        double lo[8], dn[8], rd[8], dv[8];
+                if (c > 0 && t0 + K <= len) {
+                    // full chunk past step 0, G steps at a time: the operands
+                    // of group g+1 are read from LDS while group g computes
+                    constexpr int G = 4, NG = K / G;
+                    double lo[2][G], dn[2][G], rd[2][G], dv[2][G];
+                    auto fetch = [&](int b, int s0) {
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) {
+                        for (int u = 0; u < G; ++u) {
                             const int o = (s0 + u) * R + rl;
-                            lo[u] = slo[o];
-                            dn[u] = sden[o];
-                            rd[u] = srd[o];
-                            dv[u] = sD[o];
+                            lo[b][u] = slo[o];
+                            dn[b][u] = sden[o];
+                            rd[b][u] = srd[o];
+                            dv[b][u] = sD[o];
                         }
+                    };
+                    fetch(0, 0);
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) {
-                            Qm = div_by_rcp<double>(dv[u] - lo[u] * Qm, dn[u], rd[u]);
-                            if (tdma_writes<R>(lane)) tile[(t0 + s0 + u) * RS + wl] = Qm;
+                    for (int g = 0; g < NG; ++g) {
+                        if (g + 1 < NG) fetch((g + 1) & 1, (g + 1) * G);
+                        const int b = g & 1;
+#pragma unroll
+                        for (int u = 0; u < G; ++u) {
+                            Qm = div_by_rcp<double>(dv[b][u] - lo[b][u] * Qm, dn[b][u], rd[b][u]);
+                            if (tdma_writes<R>(lane)) tile[(t0 + g * G + u) * RS + wl] = Qm;
                         }
                     }
                 } else {
@@ -501,18 +510,25 @@ __global__ __launch_bounds__(256) void k_tdma_line(TdmaBatch B)
                 const double *sP = ring + (c % kTdmaSlots) * kTdmaChunk;
                 const int ns = min(K, nst - c * K);
                 int t = len - 2 - c * K;
-                if (ns == K) {
-                    for (int s0 = 0; s0 < K; s0 += 8) {
-                        double pv[8], qv[8];
+                if (ns == K) {  // G steps at a time, group g+1's reads beside group g
+                    constexpr int G = 4, NG = K / G;
+                    double pv[2][G], qv[2][G];
+                    auto fetch = [&](int b, int s0) {
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) {
-                            pv[u] = sP[(s0 + u) * R + rl];
-                            qv[u] = tile[(t - s0 - u) * RS + rl];
+                        for (int u = 0; u < G; ++u) {
+                            pv[b][u] = sP[(s0 + u) * R + rl];
+                            qv[b][u] = tile[(t - s0 - u) * RS + rl];
                         }
+                    };
+                    fetch(0, 0);
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) {
-                            xv = XPASS ? pv[u] * xv + qv[u] : xv * pv[u] + qv[u];
-                            if (tdma_writes<R>(lane)) tile[(t - s0 - u) * RS + wl] = xv;
+                    for (int g = 0; g < NG; ++g) {
+                        if (g + 1 < NG) fetch((g + 1) & 1, (g + 1) * G);
+                        const int b = g & 1;
+#pragma unroll
+                        for (int u = 0; u < G; ++u) {
+                            xv = XPASS ? pv[b][u] * xv + qv[b][u] : xv * pv[b][u] + qv[b][u];
+                            if (tdma_writes<R>(lane)) tile[(t - g * G - u) * RS + wl] = xv;
                         }
                     }
                 } else {
