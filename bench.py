@@ -67,8 +67,8 @@ BYTES_PER_POINT = {
 # profiles/msi_1026x514_hbm_traffic.md).  "residual" is the update and the
 # residual of one launch.
 MEASURED_TRAFFIC = {
-    "sweeps": 83.84e6, "residual": 67.14e6, "factor": 151.67e6,
-    "sweeps32": 38.02e6, "residual32": 33.75e6, "factor32": 65.34e6,
+    "sweeps": 84.73e6, "residual": 67.14e6, "factor": 152.32e6,
+    "sweeps32": 39.27e6, "residual32": 33.75e6, "factor32": 65.42e6,
 }
 
 
