@@ -23,7 +23,9 @@ of the same grid (the reference's own p' form) on the dedicated 5-diagonal
 kernels and, for comparison, on the 9-diagonal ones through the zero-corner
 embedding; pl_loop_258x130 times the device-resident pressure-correction
 loop of one SIMPLEC iteration of the internal-flow case (258x130, 20 itp
-iterations) against the same loop of reference routines on one core.
+iterations) against the same loop of reference routines on one core;
+tdma_258x130 times the device TDMA of the u and v systems with the lines
+solved by the Thomas recurrence and by parallel cyclic reduction.
 
 Prints one JSON line.  gbps = algorithmic bytes of the solve over its time
 (FP64: 8 N (18 + 11 + 24 K); refinement: 4 N (18 + 24 K_inner) +
@@ -265,6 +267,42 @@ def pl_loop_run(reps, cpu_itp=3):
     return out
 
 
+def tdma_run(nx=258, ny=130, nitm=5, reps=20):
+    """The device TDMA (tdma2d9.f90:38) on seeded nx x ny systems, nitm
+    iterations per call, with the lines solved by the Thomas recurrence
+    (mode 0, the reference's arithmetic bit for bit) and by parallel cyclic
+    reduction (mode 1): ms per call and per iteration for the u and v pair
+    (count 2) and a single system, and the largest difference between the
+    two modes' solutions relative to the solution's largest magnitude."""
+    import torch
+
+    from mach2d import problems as P, solvers, workloads as W
+    out = {"nx": nx, "ny": ny, "nitm": nitm}
+    try:
+        for mode, name in ((0, "thomas"), (1, "pcr")):
+            solvers.set_tdma_mode(mode)
+            rows = {}
+            for count in (2, 1):
+                g = W.tdma(nx, ny, nitm, count, reps)
+                rows["count%d" % count] = {"ms": g["ms"], "ms_per_iteration": g["ms_per_iteration"]}
+            out[name] = rows
+        s = P.random_f3_system(nx, ny, seed=0)
+        a = torch.tensor(s.a.reshape(-1), device="cuda")
+        b = torch.tensor(s.rhs, device="cuda")
+        xs = []
+        for mode in (0, 1):
+            solvers.set_tdma_mode(mode)
+            x = torch.zeros(nx * ny, dtype=torch.float64, device="cuda")
+            solvers.tdma2d9_device_many(nx, ny, nitm, [a], [b], [x])
+            torch.cuda.synchronize()
+            xs.append(x)
+        out["pcr_max_rel_diff"] = float((xs[1] - xs[0]).abs().max() / xs[0].abs().max())
+    finally:
+        solvers.set_tdma_mode(0)
+    log("tdma %dx%d: %s" % (nx, ny, out))
+    return out
+
+
 # the shipped nozzle settings (parametersNozzleEulerPTC.txt: itmmax, itpmax,
 # nitm_u, nitm_p, tolerances, beta, Scarborough h0/mincc/maxcc; constant
 # properties Rg, gamma) with chamber conditions of a slow test flow
@@ -279,8 +317,10 @@ def time_step_run(cpu_steps=5):
     properties, the shipped nozzle parameters: itmmax 1, itpmax 20, nitm_u 5,
     nitm_p 20, tolerances 1e-2, Scarborough time step) on the 258x130 nozzle
     from inlet Mach 0.2, 200 time steps from the initial state, each step
-    timed, device-resident (m2d_intflow_time_step), u and v by MSI and by
-    TDMA; the first `cpu_steps` steps of the same trajectory composed from
+    timed, device-resident (m2d_intflow_time_step), u and v by MSI, by
+    TDMA and by TDMA with the lines solved by parallel cyclic reduction
+    (largest difference of the final fields from the Thomas trajectory
+    reported beside it); the first `cpu_steps` steps of the same trajectory composed from
     the reference's routines (compiled Fortran, one core) beside them when
     built."""
     import torch
@@ -290,7 +330,9 @@ def time_step_run(cpu_steps=5):
     nx, ny, steps = c["nx"], c["ny"], c["steps"]
     out = {"steps": steps, "mach_in": c["mach_in"], "dt0": c["dt0"]}
     f0 = problems.time_step_fields(nx, ny, 5, NOZZLE, mach_in=c["mach_in"])
-    for name, solver9d in (("msi", 0), ("tdma", 1)):
+    final = {}
+    for name, solver9d, tdma_mode in (("msi", 0, 0), ("tdma", 1, 0), ("tdma_pcr", 1, 1)):
+        solvers.set_tdma_mode(tdma_mode)
         prm = dict(NOZZLE, solver9d=solver9d)
         dev9 = solvers.DeviceMSI(nx, ny) if solver9d == 0 else None
         dev5 = solvers.DeviceMSI(nx, ny, ndiag=5)
@@ -314,9 +356,16 @@ def time_step_run(cpu_steps=5):
                      "ms_200_steps": round(sum(per) * 1e3, 1),
                      "ms_first_steps": round(sum(per[:cpu_steps]) * 1e3, 2),
                      "finite": ok}
+        final[name] = d
         for h in (dev9, dev5):
             if h is not None:
                 h.close()
+    solvers.set_tdma_mode(0)
+    # the PCR line solver changes the u, v solutions by rounding only: the
+    # trajectories' largest difference after the timed steps, field by field
+    out["tdma_pcr"]["max_rel_diff_vs_tdma"] = {
+        k: float((final["tdma_pcr"][k] - final["tdma"][k]).abs().max()
+                 / final["tdma"][k].abs().max()) for k in ("u", "v", "p", "T")}
     try:
         if cpu_steps < 1:
             raise RuntimeError("CPU leg not asked for")
@@ -702,6 +751,7 @@ def main():
     pl_loop = pl_loop_run(max(1, args.steps)) if args.mode == "best" else None
     grid_ts = grid_tstep_run() if args.mode == "best" else None
     tstep_full = time_step_run() if args.mode == "best" else None
+    tdma = tdma_run() if args.mode == "best" else None
     head = "fp32_ir" if "fp32_ir" in summary and summary["fp32_ir"]["correct"] else modes[0]
     r, e = runs[head], summary[head]
     st = r["st"]
@@ -765,6 +815,7 @@ def main():
         "pl_loop_258x130": pl_loop,
         "grid_tstep_1026x514": grid_ts,
         "simplec_time_step_258x130": tstep_full,
+        "tdma_258x130": tdma,
         "target_comparison": target_comparison(summary, batch3),
     }
     print(json.dumps(out), flush=True)

@@ -673,6 +673,15 @@ int m2d_unet_unpack_output(const float *out, int os, int nx, int ny, int w, doub
  * and returns when x is final. */
 int m2d_tdma2d9_device(int nx, int ny, int nitm, const double *d_a,
                        const double *d_b, double *d_x, void *stream);
+/* Line solver of the device TDMA (this host thread's later calls of
+ * m2d_tdma2d9, m2d_tdma2d9_device, m2d_tdma2d9_device_many and the time
+ * step):
+ * 0 (default) the reference's Thomas recurrence, bit for bit; 1 parallel
+ * cyclic reduction of each line (grids up to 400 x 400, larger ones keep the
+ * recurrence), the same line-Jacobi iteration up to rounding (not the
+ * reference's rounding).  No reference counterpart (an alternative solver
+ * of the line systems of tdma2d9.f90:79-316). */
+int m2d_set_tdma_mode(int mode);
 /* tdma2d9 on `count` independent systems of one grid size (d_a[q], d_b[q],
  * d_x[q] device pointers) in the same launches: the u and v systems of a
  * SIMPLEC iteration, which the reference solves one after the other with

@@ -133,6 +133,60 @@ __device__ __forceinline__ double *tdma_tmp(const TdmaBatch &B, const TdmaSys &S
     return S.w + kTArrays * (tdma_pass_size(B.nx, B.ny, true) + tdma_pass_size(B.nx, B.ny, false));
 }
 
+// ---------------------------------------------------------------------------
+// Parallel cyclic reduction of the line systems (m2d_set_tdma_mode(1)): the
+// same line-Jacobi passes, each line's tridiagonal system
+//
+//     lo(t) x(t-1) + d(t) x(t) + up(t) x(t+1) = D(t)   (lo(0), up(n-1) unused,
+//                                                       as in the recurrence)
+//
+// solved by reduction instead of the Thomas recurrence: step s = 1, 2, 4, ...
+// combines row t with rows t-s and t+s (k1 = lo(t)/d(t-s), k2 = up(t)/d(t+s))
+// until every row stands alone, x(t) = D'(t)/d'(t).  What depends on A only
+// (k1, k2 of every step, 1/d') is computed once per call (k_tdma_pcr_setup);
+// an iteration applies the steps to D (k_tdma_pcr_line): ceil(log2 n) rounds
+// of two FMAs per point with the line in LDS, instead of a recurrence of n
+// dependent steps.  The result is the same solution up to rounding (not the
+// reference's rounding).  Lines of up to kPcrMaxLen points.
+// ---------------------------------------------------------------------------
+constexpr int kPcrMaxLen = 400;  // the line's D and factors fit 64 KB of LDS
+__host__ __device__ inline int pcr_steps(int n)
+{
+    int k = 0;
+    while ((1 << k) < n) ++k;
+    return k;
+}
+// per system, after the recurrence's scratch: for the x-pass (lines = ny,
+// n = nx) then the y-pass (lines = nx, n = ny), line-major:
+// K [line][step][2][n] (k1, k2), R [line][n] (1/d'), D [line][n]
+__host__ __device__ inline long pcr_pass_size(int nx, int ny, bool x)
+{
+    const int n = x ? nx : ny;
+    return (long)nx * ny * (2 * pcr_steps(n) + 2);
+}
+__host__ __device__ inline long pcr_scratch(int nx, int ny)
+{
+    return pcr_pass_size(nx, ny, true) + pcr_pass_size(nx, ny, false);
+}
+__device__ __forceinline__ double *pcr_base(const TdmaBatch &B, const TdmaSys &S, bool x)
+{
+    return S.w + tdma_scratch(B.nx, B.ny) + (x ? 0 : pcr_pass_size(B.nx, B.ny, true));
+}
+__device__ __forceinline__ double *pcr_k(const TdmaBatch &B, const TdmaSys &S, bool x, int line)
+{
+    const int n = x ? B.nx : B.ny;
+    return pcr_base(B, S, x) + (long)line * 2 * pcr_steps(n) * n;
+}
+__device__ __forceinline__ double *pcr_r(const TdmaBatch &B, const TdmaSys &S, bool x)
+{
+    const int n = x ? B.nx : B.ny;
+    return pcr_base(B, S, x) + (long)B.nx * B.ny * 2 * pcr_steps(n);
+}
+__device__ __forceinline__ double *pcr_d(const TdmaBatch &B, const TdmaSys &S, bool x)
+{
+    return pcr_r(B, S, x) + (long)B.nx * B.ny;
+}
+
 // Coefficients of both passes in their pass layouts (padding lines
 // included, with their D), once per call.
 __global__ __launch_bounds__(256) void k_tdma_setup(TdmaBatch B)
@@ -228,7 +282,8 @@ __device__ __forceinline__ double d_ypass(const double *__restrict__ a,
 // D of one pass for every real point of every system, in the pass layout
 // (padding lines keep the zero of k_tdma_setup).  x-pass: input x (the
 // iteration's); y-pass: input the x-pass result.
-template <bool XPASS>
+// LM: D in the reduction's line-major layout (pcr_d) instead.
+template <bool XPASS, bool LM = false>
 __global__ __launch_bounds__(256) void k_tdma_d(TdmaBatch B)
 {
     const int nx = B.nx, ny = B.ny;
@@ -239,6 +294,14 @@ __global__ __launch_bounds__(256) void k_tdma_d(TdmaBatch B)
     const long p = e - (long)q * N;
     const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
     const TdmaSys S = B.s[q];
+    if (LM) {  // x-pass line j, step i; y-pass line i, step j
+        double *D = pcr_d(B, S, XPASS);
+        if (XPASS)
+            D[p] = d_xpass(S.a, S.b, S.x, N, nx, ny, i, j);
+        else
+            D[(long)i * ny + j] = d_ypass(S.a, S.b, tdma_tmp(B, S), N, nx, ny, i, j);
+        return;
+    }
     double *D = tdma_arr(B, S, XPASS, kTD);
     if (XPASS)
         D[(long)i * tdma_lp(ny) + j] = d_xpass(S.a, S.b, S.x, N, nx, ny, i, j);
@@ -560,6 +623,94 @@ __global__ __launch_bounds__(256) void k_tdma_line(TdmaBatch B)
     }
 }
 
+// One workgroup (one wave) per (system, line) of a pass: the reduction's
+// factors from A, in LDS, double-buffered per step.
+template <bool XPASS>
+__global__ __launch_bounds__(64) void k_tdma_pcr_setup(TdmaBatch B)
+{
+    __shared__ double buf[2][3][kPcrMaxLen];
+    const int nx = B.nx, ny = B.ny, n = XPASS ? nx : ny, lines = XPASS ? ny : nx;
+    const long N = (long)nx * ny;
+    const int q = (int)blockIdx.x / lines, line = (int)blockIdx.x - q * lines;
+    if (q >= B.count) return;
+    const TdmaSys S = B.s[q];
+    for (int t = threadIdx.x; t < n; t += 64) {
+        const long p = XPASS ? (long)line * nx + t : (long)t * nx + line;
+        buf[0][0][t] = t > 0 ? S.a[(XPASS ? kW : kS) * N + p] : 0.0;
+        buf[0][1][t] = S.a[kP * N + p];
+        buf[0][2][t] = t < n - 1 ? S.a[(XPASS ? kE : kN) * N + p] : 0.0;
+    }
+    __syncthreads();
+    double *K = pcr_k(B, S, XPASS, line);
+    const int ns = pcr_steps(n);
+    int cur = 0;
+    for (int k = 0, s = 1; k < ns; ++k, s <<= 1, cur ^= 1) {
+        const double *lo = buf[cur][0], *d = buf[cur][1], *up = buf[cur][2];
+        for (int t = threadIdx.x; t < n; t += 64) {
+            const bool hl = t - s >= 0, hu = t + s < n;
+            const double k1 = hl ? lo[t] / d[t - s] : 0.0;
+            const double k2 = hu ? up[t] / d[t + s] : 0.0;
+            double dn = d[t];
+            if (hl) dn = fma(-k1, up[t - s], dn);
+            if (hu) dn = fma(-k2, lo[t + s], dn);
+            buf[cur ^ 1][0][t] = hl ? -k1 * lo[t - s] : 0.0;
+            buf[cur ^ 1][1][t] = dn;
+            buf[cur ^ 1][2][t] = hu ? -k2 * up[t + s] : 0.0;
+            K[(2L * k) * n + t] = k1;
+            K[(2L * k + 1) * n + t] = k2;
+        }
+        __syncthreads();
+    }
+    double *R = pcr_r(B, S, XPASS) + (long)line * n;
+    for (int t = threadIdx.x; t < n; t += 64) R[t] = 1.0 / buf[cur][1][t];
+}
+
+// One wave per (system, line) of a pass, each iteration: D (line-major, from
+// k_tdma_d<., true>) and every step's factors into LDS, the steps applied,
+// x = D' / d'.  x-pass into the scratch's x-pass result, y-pass into x.
+template <bool XPASS>
+__global__ __launch_bounds__(64) void k_tdma_pcr_line(TdmaBatch B)
+{
+    extern __shared__ double lds[];
+    const int nx = B.nx, ny = B.ny, n = XPASS ? nx : ny, lines = XPASS ? ny : nx;
+    const int q = (int)blockIdx.x / lines, line = (int)blockIdx.x - q * lines;
+    if (q >= B.count) return;
+    const TdmaSys S = B.s[q];
+    const int ns = pcr_steps(n);
+    double *rhs[2] = {lds, lds + n}, *kl = lds + 2 * n;
+    const double *D = pcr_d(B, S, XPASS) + (long)line * n;
+    const double *K = pcr_k(B, S, XPASS, line);
+    // the factors (ns * n 16-byte vectors, 16-byte aligned: 2 ns n doubles a
+    // line, even) straight into LDS, every load in flight at once; lanes
+    // past the end re-read the last vector into the slack after it
+    const int lane = threadIdx.x, nv = ns * n;
+    for (int v0 = 0; v0 < nv; v0 += 64)
+        __builtin_amdgcn_global_load_lds((tdma_gvoid *)(K + 2L * min(v0 + lane, nv - 1)),
+                                         (tdma_lvoid *)(kl + 2L * v0), 16, 0, 0);
+    for (int t = lane; t < n; t += 64) rhs[0][t] = D[t];
+    tdma_wait_vmem<0>();
+    __syncthreads();
+    int cur = 0;
+    for (int k = 0, s = 1; k < ns; ++k, s <<= 1, cur ^= 1) {
+        const double *r = rhs[cur], *k1 = kl + 2L * k * n, *k2 = k1 + n;
+        for (int t = threadIdx.x; t < n; t += 64) {
+            double v = r[t];
+            if (t - s >= 0) v = fma(-k1[t], r[t - s], v);
+            if (t + s < n) v = fma(-k2[t], r[t + s], v);
+            rhs[cur ^ 1][t] = v;
+        }
+        __syncthreads();
+    }
+    const double *R = pcr_r(B, S, XPASS) + (long)line * n;
+    for (int t = threadIdx.x; t < n; t += 64) {
+        const double x = rhs[cur][t] * R[t];
+        if (XPASS)
+            tdma_tmp(B, S)[(long)line * nx + t] = x;
+        else
+            S.x[(long)t * nx + line] = x;
+    }
+}
+
 static unsigned tdma_setup_blocks(const TdmaBatch &B)
 {
     const long per = tdma_pass_size(B.nx, B.ny, true) + tdma_pass_size(B.nx, B.ny, false);
@@ -598,6 +749,23 @@ static void tdma_launch_pre(const TdmaBatch &B, int ry, hipStream_t s)
 
 // tdma2d9 on up to kTdmaMaxSys systems of one grid size: the pass layouts
 // and the P chains once, then nitm iterations of [D, x lines, D, y lines].
+static void tdma_enqueue_pcr(const TdmaBatch &B, int nitm, hipStream_t s)
+{
+    const dim3 T(256), W(64);
+    const unsigned bx = (unsigned)(B.count * B.ny), by = (unsigned)(B.count * B.nx);
+    hipLaunchKernelGGL(k_tdma_pcr_setup<true>, dim3(bx), W, 0, s, B);
+    hipLaunchKernelGGL(k_tdma_pcr_setup<false>, dim3(by), W, 0, s, B);
+    // D double-buffered, then the factors with a wave's slack of 128 doubles
+    const size_t lx = sizeof(double) * ((size_t)(2 + 2 * pcr_steps(B.nx)) * B.nx + 128),
+                 ly = sizeof(double) * ((size_t)(2 + 2 * pcr_steps(B.ny)) * B.ny + 128);
+    for (int it = 0; it < nitm; ++it) {
+        hipLaunchKernelGGL((k_tdma_d<true, true>), dim3(tdma_point_blocks(B)), T, 0, s, B);
+        hipLaunchKernelGGL(k_tdma_pcr_line<true>, dim3(bx), W, lx, s, B);
+        hipLaunchKernelGGL((k_tdma_d<false, true>), dim3(tdma_point_blocks(B)), T, 0, s, B);
+        hipLaunchKernelGGL(k_tdma_pcr_line<false>, dim3(by), W, ly, s, B);
+    }
+}
+
 static void tdma_enqueue(const TdmaBatch &B, int nitm, hipStream_t s)
 {
     if (nitm < 1) return;
@@ -646,6 +814,8 @@ struct TdmaScratchCache {
     }
 };
 static thread_local TdmaScratchCache g_tdma_scratch;
+// 0: the reference's recurrence (bit for bit); 1: parallel cyclic reduction
+static thread_local int g_tdma_mode = 0;
 
 // Grid dimensions the chain kernels take: 3 .. kTdmaMaxLen (each pass's
 // line fits the LDS tiles beside the operand ring).
@@ -665,7 +835,8 @@ int tdma_device_many(int count, int nx, int ny, int nitm, const double *const *a
     for (int q = 0; q < count; ++q)
         if (!a[q] || !b[q] || !x[q]) return M2D_ERR_ARG;
     if (nitm < 1) return M2D_OK;
-    const long W = tdma_scratch(nx, ny);
+    const bool pcr = g_tdma_mode == 1 && nx <= kPcrMaxLen && ny <= kPcrMaxLen;
+    const long W = tdma_scratch(nx, ny) + (pcr ? pcr_scratch(nx, ny) : 0);
     const int per = std::min(count, kTdmaMaxSys);
     double *w = g_tdma_scratch.get((size_t)W * per);
     if (!w) return M2D_ERR_HIP;
@@ -677,7 +848,10 @@ int tdma_device_many(int count, int nx, int ny, int nitm, const double *const *a
         B.ny = ny;
         for (int q = 0; q < B.count; ++q)
             B.s[q] = {a[q0 + q], b[q0 + q], x[q0 + q], w + W * q};
-        tdma_enqueue(B, nitm, s);
+        if (pcr)
+            tdma_enqueue_pcr(B, nitm, s);
+        else
+            tdma_enqueue(B, nitm, s);
         // the next group reuses the scratch: wait for this one
         if (q0 + kTdmaMaxSys < count && hipStreamSynchronize(s) != hipSuccess) return M2D_ERR_HIP;
     }
@@ -784,6 +958,13 @@ using namespace m2d;
 // C ABI: TDMA and norms
 // ---------------------------------------------------------------------------
 void m2d_set_last_status_code(int status);
+
+extern "C" int m2d_set_tdma_mode(int mode)
+{
+    if (mode != 0 && mode != 1) return M2D_ERR_ARG;
+    g_tdma_mode = mode;
+    return M2D_OK;
+}
 
 extern "C" int m2d_tdma2d9_device(int nx, int ny, int nitm, const double *d_a, const double *d_b,
                                   double *d_x, void *stream)

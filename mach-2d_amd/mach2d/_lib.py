@@ -73,7 +73,7 @@ EXPORTS = [
     "m2d_unet_conv3x3", "m2d_unet_conv3x3_wgrad", "m2d_unet_relu_bwd", "m2d_unet_mode_bwd",
     "m2d_unet_to_bf16", "m2d_unet_adam", "m2d_unet_pack_weights", "m2d_unet_mse",
     "m2d_unet_pack_input", "m2d_unet_unpack_output", "m2d_solve_msi9d_noalloc",
-    "m2d_tdma2d9_device_many",
+    "m2d_tdma2d9_device_many", "m2d_set_tdma_mode",
 ]
 
 _lib = None
@@ -92,6 +92,15 @@ def load():
         raise RuntimeError(
             "libmach2d_hip.so is not built (%s); run `python mach-2d_amd/build.py` "
             "or __graft_entry__.build()" % LIB_PATH)
+    # PyTorch-ROCm carries its own HIP runtime: with it loaded first, this
+    # library's HIP calls bind to that runtime, so device memory and streams
+    # from torch and this library's kernels share one runtime.  Loaded the
+    # other way round, a second runtime comes up and launches can fail
+    # ("invalid device function").
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     for f in (L.m2d_lu2d9, L.m2d_lu2d5):
         f.argtypes = [_vp, _vp, _vp, _ip, _ip, _ip]
@@ -169,6 +178,8 @@ def load():
     L.m2d_tdma2d9_device_many.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
                                           _vp]
     L.m2d_tdma2d9_device_many.restype = C.c_int
+    L.m2d_set_tdma_mode.argtypes = [C.c_int]
+    L.m2d_set_tdma_mode.restype = C.c_int
     L.m2d_solve_msi9d_noalloc.argtypes = [_ip, _ip, _ip, _dp, _vp, _vp, _vp, _vp]
     L.m2d_solve_msi9d_noalloc.restype = C.c_int
     L.m2d_enable_kernel_timing.argtypes = [_vp, C.c_int]

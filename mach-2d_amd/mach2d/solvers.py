@@ -311,6 +311,16 @@ def tdma2d9_device_many(nx, ny, nitm, a_list, b_list, x_list, stream=None):
     return rc
 
 
+def set_tdma_mode(mode):
+    """Line solver of the device TDMA for this thread's later calls
+    (m2d_set_tdma_mode): 0 the reference's Thomas recurrence (bit for bit,
+    the default), 1 parallel cyclic reduction (grids up to 400 x 400; the
+    same iteration up to rounding)."""
+    rc = _lib.load().m2d_set_tdma_mode(int(mode))
+    if rc:
+        raise SolverError(rc, "m2d_set_tdma_mode")
+
+
 class ClassSolverTDMA5d(ClassSolverAbstract):
     """mod_solver_tdma.f90:17-29,48-72 through the zero-corner embedding."""
 

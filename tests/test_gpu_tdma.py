@@ -39,6 +39,42 @@ def test_tdma2d9_bitwise_vs_oracle(gpu, orc, nx, ny):
     assert np.array_equal(x, orc.tdma2d9(a, b, x0, nx, ny, 3))
 
 
+@pytest.mark.parametrize("nx,ny", [(258, 130), (130, 66), (17, 300), (3, 3), (400, 37),
+                                   (1026, 9)])
+def test_tdma_pcr_mode_within_1e12_of_oracle(gpu, orc, nx, ny):
+    """m2d_set_tdma_mode(1): the lines solved by parallel cyclic reduction.
+    Same line-Jacobi iteration, other rounding: 3 iterations within 1e-12
+    (max norm, relative) of the reference's recurrence; grids past 400
+    points a line keep the recurrence (bit for bit)."""
+    rng = np.random.default_rng(nx * ny + 7)
+    a = -rng.uniform(0.05, 1.0, size=(9, nx * ny))
+    a[4] = 1.5 * np.sum(np.abs(a[[0, 1, 2, 3, 5, 6, 7, 8]]), axis=0) + 0.1
+    b = rng.uniform(-1, 1, nx * ny)
+    x0 = rng.uniform(-1, 1, nx * ny)
+    want = orc.tdma2d9(a, b, x0, nx, ny, 3)
+    x = x0.copy()
+    gpu.set_tdma_mode(1)
+    try:
+        st = gpu.tdma2d9(nx, ny, 3, a, b, x)
+    finally:
+        gpu.set_tdma_mode(0)
+    assert st.status == 0
+    err = np.max(np.abs(x - want)) / np.max(np.abs(want))
+    if max(nx, ny) > 400:
+        assert np.array_equal(x, want)
+    else:
+        assert err <= 1e-12, err
+    # the default mode is the recurrence again
+    x = x0.copy()
+    gpu.tdma2d9(nx, ny, 3, a, b, x)
+    assert np.array_equal(x, want)
+
+
+def test_tdma_pcr_mode_rejects_other_modes(gpu):
+    with pytest.raises(gpu.SolverError):
+        gpu.set_tdma_mode(2)
+
+
 def test_tdma_device_many_equals_single(gpu, orc):
     """Three systems of one grid size in the same launches: each result is
     its single-system result and the oracle's, bit for bit; the x-pass

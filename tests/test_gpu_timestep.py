@@ -70,6 +70,41 @@ def test_time_step_matches_oracle(gpu, orc, modvis, solver9d, nx, ny, mach, dt0)
             h.close()
 
 
+def test_time_step_pcr_tdma_within_1e9_of_oracle(gpu, orc):
+    """u and v by TDMA with the lines solved by parallel cyclic reduction
+    (m2d_set_tdma_mode(1)) on the 258x130 nozzle at inlet Mach 0.2: three
+    steps, the same p' iteration counts, every field, dt and the norms
+    within 1e-9 of the reference's trajectory (largest difference relative
+    to the field's largest magnitude); the line solutions differ from the
+    recurrence's by rounding only."""
+    import torch
+
+    from mach2d import simplec
+    nx, ny, dt0 = 258, 130, 5e-6
+    f = TS.step_fields(nx, ny, seed=nx, mach_in=0.2)
+    prm = dict(TS.PARAMS, modvis=0, solver9d=1)
+    d = {k: torch.tensor(np.ascontiguousarray(v).reshape(-1), device="cuda")
+         for k, v in f.items()}
+    dev5 = gpu.DeviceMSI(nx, ny, ndiag=5)
+    torch.cuda.synchronize()
+    s, dt_o, dt_g = f, dt0, dt0
+    gpu.set_tdma_mode(1)
+    try:
+        for it in range(3):
+            s, dt_o, ro = TS.step(orc, s, nx, ny, prm, dt_o)
+            dt_g, rg = simplec.intflow_time_step(None, dev5, d, prm, dt_g)
+            assert rg["itp"] == ro["itp"], (it, rg, ro)
+            assert dt_g == pytest.approx(dt_o, rel=1e-9), (it, dt_g, dt_o)
+            assert rg["normpl"] == pytest.approx(ro["normpl"], rel=1e-9), it
+            for k in simplec.TS_STATE + ("pin", "plin", "Tin"):
+                got = d[k].cpu().numpy().reshape(np.asarray(s[k]).shape)
+                err = np.abs(got - s[k]).max() / max(np.abs(s[k]).max(), 1e-300)
+                assert err <= 1e-9, (it, k, err)
+    finally:
+        gpu.set_tdma_mode(0)
+        dev5.close()
+
+
 @pytest.mark.parametrize("bad", [dict(itpmax=0), dict(nitm_p=0), dict(modvis=2),
                                  dict(itmmax=0), dict(solver9d=3)])
 def test_time_step_rejects_bad_parameters_before_changing_fields(gpu, bad):
