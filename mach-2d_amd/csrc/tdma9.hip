@@ -144,12 +144,15 @@ __device__ __forceinline__ double *tdma_tmp(const TdmaBatch &B, const TdmaSys &S
 // combines row t with rows t-s and t+s (k1 = lo(t)/d(t-s), k2 = up(t)/d(t+s))
 // until every row stands alone, x(t) = D'(t)/d'(t).  What depends on A only
 // (k1, k2 of every step, 1/d') is computed once per call (k_tdma_pcr_setup);
-// an iteration applies the steps to D (k_tdma_pcr_line): ceil(log2 n) rounds
-// of two FMAs per point with the line in LDS, instead of a recurrence of n
-// dependent steps.  The result is the same solution up to rounding (not the
+// an iteration forms each line's D and applies the steps to it in one
+// workgroup per line (k_tdma_pcr_line): ceil(log2 n) rounds of two FMAs per
+// point with the line in LDS, instead of a recurrence of n dependent steps.  The result is the same solution up to rounding (not the
 // reference's rounding).  Lines of up to kPcrMaxLen points.
 // ---------------------------------------------------------------------------
 constexpr int kPcrMaxLen = 400;  // the line's D and factors fit 64 KB of LDS
+// a line's workgroup: one lane per point, ceil(n / 64) waves
+constexpr int kPcrMaxThreads = (kPcrMaxLen + 63) / 64 * 64;
+inline unsigned pcr_threads(int n) { return (unsigned)((n + 63) / 64 * 64); }
 __host__ __device__ inline int pcr_steps(int n)
 {
     int k = 0;
@@ -158,11 +161,11 @@ __host__ __device__ inline int pcr_steps(int n)
 }
 // per system, after the recurrence's scratch: for the x-pass (lines = ny,
 // n = nx) then the y-pass (lines = nx, n = ny), line-major:
-// K [line][step][2][n] (k1, k2), R [line][n] (1/d'), D [line][n]
+// K [line][step][2][n] (k1, k2), R [line][n] (1/d')
 __host__ __device__ inline long pcr_pass_size(int nx, int ny, bool x)
 {
     const int n = x ? nx : ny;
-    return (long)nx * ny * (2 * pcr_steps(n) + 2);
+    return (long)nx * ny * (2 * pcr_steps(n) + 1);
 }
 __host__ __device__ inline long pcr_scratch(int nx, int ny)
 {
@@ -181,10 +184,6 @@ __device__ __forceinline__ double *pcr_r(const TdmaBatch &B, const TdmaSys &S, b
 {
     const int n = x ? B.nx : B.ny;
     return pcr_base(B, S, x) + (long)B.nx * B.ny * 2 * pcr_steps(n);
-}
-__device__ __forceinline__ double *pcr_d(const TdmaBatch &B, const TdmaSys &S, bool x)
-{
-    return pcr_r(B, S, x) + (long)B.nx * B.ny;
 }
 
 // Coefficients of both passes in their pass layouts (padding lines
@@ -282,8 +281,7 @@ __device__ __forceinline__ double d_ypass(const double *__restrict__ a,
 // D of one pass for every real point of every system, in the pass layout
 // (padding lines keep the zero of k_tdma_setup).  x-pass: input x (the
 // iteration's); y-pass: input the x-pass result.
-// LM: D in the reduction's line-major layout (pcr_d) instead.
-template <bool XPASS, bool LM = false>
+template <bool XPASS>
 __global__ __launch_bounds__(256) void k_tdma_d(TdmaBatch B)
 {
     const int nx = B.nx, ny = B.ny;
@@ -294,14 +292,6 @@ __global__ __launch_bounds__(256) void k_tdma_d(TdmaBatch B)
     const long p = e - (long)q * N;
     const int j = (int)(p / nx), i = (int)(p - (long)j * nx);
     const TdmaSys S = B.s[q];
-    if (LM) {  // x-pass line j, step i; y-pass line i, step j
-        double *D = pcr_d(B, S, XPASS);
-        if (XPASS)
-            D[p] = d_xpass(S.a, S.b, S.x, N, nx, ny, i, j);
-        else
-            D[(long)i * ny + j] = d_ypass(S.a, S.b, tdma_tmp(B, S), N, nx, ny, i, j);
-        return;
-    }
     double *D = tdma_arr(B, S, XPASS, kTD);
     if (XPASS)
         D[(long)i * tdma_lp(ny) + j] = d_xpass(S.a, S.b, S.x, N, nx, ny, i, j);
@@ -623,10 +613,10 @@ __global__ __launch_bounds__(256) void k_tdma_line(TdmaBatch B)
     }
 }
 
-// One workgroup (one wave) per (system, line) of a pass: the reduction's
-// factors from A, in LDS, double-buffered per step.
+// One workgroup per (system, line) of a pass: the reduction's factors from
+// A, in LDS, double-buffered per step.
 template <bool XPASS>
-__global__ __launch_bounds__(64) void k_tdma_pcr_setup(TdmaBatch B)
+__global__ __launch_bounds__(kPcrMaxThreads) void k_tdma_pcr_setup(TdmaBatch B)
 {
     __shared__ double buf[2][3][kPcrMaxLen];
     const int nx = B.nx, ny = B.ny, n = XPASS ? nx : ny, lines = XPASS ? ny : nx;
@@ -634,7 +624,7 @@ __global__ __launch_bounds__(64) void k_tdma_pcr_setup(TdmaBatch B)
     const int q = (int)blockIdx.x / lines, line = (int)blockIdx.x - q * lines;
     if (q >= B.count) return;
     const TdmaSys S = B.s[q];
-    for (int t = threadIdx.x; t < n; t += 64) {
+    for (int t = threadIdx.x; t < n; t += (int)blockDim.x) {
         const long p = XPASS ? (long)line * nx + t : (long)t * nx + line;
         buf[0][0][t] = t > 0 ? S.a[(XPASS ? kW : kS) * N + p] : 0.0;
         buf[0][1][t] = S.a[kP * N + p];
@@ -646,7 +636,7 @@ __global__ __launch_bounds__(64) void k_tdma_pcr_setup(TdmaBatch B)
     int cur = 0;
     for (int k = 0, s = 1; k < ns; ++k, s <<= 1, cur ^= 1) {
         const double *lo = buf[cur][0], *d = buf[cur][1], *up = buf[cur][2];
-        for (int t = threadIdx.x; t < n; t += 64) {
+        for (int t = threadIdx.x; t < n; t += (int)blockDim.x) {
             const bool hl = t - s >= 0, hu = t + s < n;
             const double k1 = hl ? lo[t] / d[t - s] : 0.0;
             const double k2 = hu ? up[t] / d[t + s] : 0.0;
@@ -662,14 +652,16 @@ __global__ __launch_bounds__(64) void k_tdma_pcr_setup(TdmaBatch B)
         __syncthreads();
     }
     double *R = pcr_r(B, S, XPASS) + (long)line * n;
-    for (int t = threadIdx.x; t < n; t += 64) R[t] = 1.0 / buf[cur][1][t];
+    for (int t = threadIdx.x; t < n; t += (int)blockDim.x) R[t] = 1.0 / buf[cur][1][t];
 }
 
-// One wave per (system, line) of a pass, each iteration: D (line-major, from
-// k_tdma_d<., true>) and every step's factors into LDS, the steps applied,
-// x = D' / d'.  x-pass into the scratch's x-pass result, y-pass into x.
+// One workgroup per (system, line) of a pass, each iteration: every step's
+// factors into LDS while the line's D is formed (the x-pass from the
+// iteration's x, along a row; the y-pass from the x-pass result, down a
+// column), the steps applied, x = D' / d'.  x-pass into the scratch's x-pass
+// result, y-pass into x.
 template <bool XPASS>
-__global__ __launch_bounds__(64) void k_tdma_pcr_line(TdmaBatch B)
+__global__ __launch_bounds__(kPcrMaxThreads) void k_tdma_pcr_line(TdmaBatch B)
 {
     extern __shared__ double lds[];
     const int nx = B.nx, ny = B.ny, n = XPASS ? nx : ny, lines = XPASS ? ny : nx;
@@ -678,22 +670,27 @@ __global__ __launch_bounds__(64) void k_tdma_pcr_line(TdmaBatch B)
     const TdmaSys S = B.s[q];
     const int ns = pcr_steps(n);
     double *rhs[2] = {lds, lds + n}, *kl = lds + 2 * n;
-    const double *D = pcr_d(B, S, XPASS) + (long)line * n;
     const double *K = pcr_k(B, S, XPASS, line);
     // the factors (ns * n 16-byte vectors, 16-byte aligned: 2 ns n doubles a
-    // line, even) straight into LDS, every load in flight at once; lanes
-    // past the end re-read the last vector into the slack after it
-    const int lane = threadIdx.x, nv = ns * n;
-    for (int v0 = 0; v0 < nv; v0 += 64)
-        __builtin_amdgcn_global_load_lds((tdma_gvoid *)(K + 2L * min(v0 + lane, nv - 1)),
+    // line, even) straight into LDS, every load in flight at once: each wave
+    // brings 64 vectors per round (lane q the wave's q-th), a wave wholly
+    // past the end none; lanes past the end re-read the last vector into the
+    // slack of 64 vectors after it
+    const int tid = threadIdx.x, nv = ns * n;
+    const int w0 = tid & ~63;  // the wave's first thread
+    for (int v0 = w0; v0 < nv; v0 += (int)blockDim.x)
+        __builtin_amdgcn_global_load_lds((tdma_gvoid *)(K + 2L * min(v0 + (tid & 63), nv - 1)),
                                          (tdma_lvoid *)(kl + 2L * v0), 16, 0, 0);
-    for (int t = lane; t < n; t += 64) rhs[0][t] = D[t];
+    const long N = (long)nx * ny;
+    for (int t = tid; t < n; t += (int)blockDim.x)
+        rhs[0][t] = XPASS ? d_xpass(S.a, S.b, S.x, N, nx, ny, t, line)
+                          : d_ypass(S.a, S.b, tdma_tmp(B, S), N, nx, ny, line, t);
     tdma_wait_vmem<0>();
     __syncthreads();
     int cur = 0;
     for (int k = 0, s = 1; k < ns; ++k, s <<= 1, cur ^= 1) {
         const double *r = rhs[cur], *k1 = kl + 2L * k * n, *k2 = k1 + n;
-        for (int t = threadIdx.x; t < n; t += 64) {
+        for (int t = threadIdx.x; t < n; t += (int)blockDim.x) {
             double v = r[t];
             if (t - s >= 0) v = fma(-k1[t], r[t - s], v);
             if (t + s < n) v = fma(-k2[t], r[t + s], v);
@@ -702,7 +699,7 @@ __global__ __launch_bounds__(64) void k_tdma_pcr_line(TdmaBatch B)
         __syncthreads();
     }
     const double *R = pcr_r(B, S, XPASS) + (long)line * n;
-    for (int t = threadIdx.x; t < n; t += 64) {
+    for (int t = threadIdx.x; t < n; t += (int)blockDim.x) {
         const double x = rhs[cur][t] * R[t];
         if (XPASS)
             tdma_tmp(B, S)[(long)line * nx + t] = x;
@@ -747,22 +744,20 @@ static void tdma_launch_pre(const TdmaBatch &B, int ry, hipStream_t s)
 #undef TDMA_PRE
 }
 
-// tdma2d9 on up to kTdmaMaxSys systems of one grid size: the pass layouts
-// and the P chains once, then nitm iterations of [D, x lines, D, y lines].
+// tdma2d9 on up to kTdmaMaxSys systems of one grid size by reduction: the
+// factors of both passes once, then nitm iterations of [x lines, y lines].
 static void tdma_enqueue_pcr(const TdmaBatch &B, int nitm, hipStream_t s)
 {
-    const dim3 T(256), W(64);
+    const dim3 Wx(pcr_threads(B.nx)), Wy(pcr_threads(B.ny));
     const unsigned bx = (unsigned)(B.count * B.ny), by = (unsigned)(B.count * B.nx);
-    hipLaunchKernelGGL(k_tdma_pcr_setup<true>, dim3(bx), W, 0, s, B);
-    hipLaunchKernelGGL(k_tdma_pcr_setup<false>, dim3(by), W, 0, s, B);
+    hipLaunchKernelGGL(k_tdma_pcr_setup<true>, dim3(bx), Wx, 0, s, B);
+    hipLaunchKernelGGL(k_tdma_pcr_setup<false>, dim3(by), Wy, 0, s, B);
     // D double-buffered, then the factors with a wave's slack of 128 doubles
     const size_t lx = sizeof(double) * ((size_t)(2 + 2 * pcr_steps(B.nx)) * B.nx + 128),
                  ly = sizeof(double) * ((size_t)(2 + 2 * pcr_steps(B.ny)) * B.ny + 128);
     for (int it = 0; it < nitm; ++it) {
-        hipLaunchKernelGGL((k_tdma_d<true, true>), dim3(tdma_point_blocks(B)), T, 0, s, B);
-        hipLaunchKernelGGL(k_tdma_pcr_line<true>, dim3(bx), W, lx, s, B);
-        hipLaunchKernelGGL((k_tdma_d<false, true>), dim3(tdma_point_blocks(B)), T, 0, s, B);
-        hipLaunchKernelGGL(k_tdma_pcr_line<false>, dim3(by), W, ly, s, B);
+        hipLaunchKernelGGL(k_tdma_pcr_line<true>, dim3(bx), Wx, lx, s, B);
+        hipLaunchKernelGGL(k_tdma_pcr_line<false>, dim3(by), Wy, ly, s, B);
     }
 }
 

@@ -285,7 +285,12 @@ def main(argv=None):
     ap.add_argument("--layout", type=int, default=None)
     ap.add_argument("--inner-gamma", dest="inner_gamma", type=float, default=None)
     ap.add_argument("--inner-nitm", dest="inner_nitm", type=int, default=None)
+    ap.add_argument("--tdma-mode", dest="tdma_mode", type=int, default=0,
+                    help="line solver of the device TDMA: 0 recurrence, 1 cyclic reduction")
     args = ap.parse_args(argv)
+    if args.tdma_mode:
+        from .solvers import set_tdma_mode
+        set_tdma_mode(args.tdma_mode)
     if args.what == "single":
         out = single(args.mode, args.solves, bool(args.staged), args.inner_gamma,
                      args.inner_nitm or 25)
