@@ -344,17 +344,22 @@ def time_step_run(cpu_steps=5):
             return d
         simplec.intflow_time_step(dev9, dev5, fresh(), prm, c["dt0"])   # warm-up
         d, dt = fresh(), c["dt0"]
-        per, itps = [], 0
+        per, itps, caps, reruns = [], 0, 0, 0
         for _ in range(steps):   # the step returns with its stream idle
             t = time.perf_counter()
             dt, r = simplec.intflow_time_step(dev9, dev5, d, prm, dt)
             per.append(time.perf_counter() - t)
             itps += r["itp"]
+            used, rr = simplec.pl_loop_cap_info(dev5)
+            caps += used
+            reruns += rr
         ok = all(bool(torch.isfinite(d[k]).all()) for k in ("u", "v", "p", "T"))
         out[name] = {"ms_per_step": round(sum(per) / steps * 1e3, 3),
                      "itp_per_step": round(itps / steps, 2),
                      "ms_200_steps": round(sum(per) * 1e3, 1),
                      "ms_first_steps": round(sum(per[:cpu_steps]) * 1e3, 2),
+                     "p_solve_iterations_enqueued": round(caps / steps, 2),
+                     "p_loop_reruns": reruns,
                      "finite": ok}
         final[name] = d
         for h in (dev9, dev5):

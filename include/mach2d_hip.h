@@ -517,6 +517,18 @@ int m2d_intflow_pl_loop(m2d_system *s5, const m2d_pl_loop_fields *f, int modvis,
  * m2d_intflow_pl_loop call on this handle. */
 int m2d_set_pl_graph(m2d_system *s5, int use_graph);
 int m2d_pl_loop_info(const m2d_system *s5, int *host_reads, int *batches);
+/* No reference counterpart.  MSI iterations enqueued per p' solve of the
+ * loop above (the solve's own cap stays nitm_p): 0 (default) the previous
+ * call's longest solve plus 2, rounded up to a multiple of 4 (all nitm_p on
+ * a handle's first call); cap > 0 that many; cap < 0 all nitm_p.  A solve
+ * still iterating after its enqueued iterations ends the loop early, and
+ * the call runs again from the updated fields as they were at its start
+ * with all nitm_p enqueued: the results are the same bit for bit, the
+ * iterations past a solve's exit no longer cost launches.
+ * m2d_pl_loop_cap_info: the last call's enqueued iterations per solve and
+ * whether it ran again (0 or 1). */
+int m2d_set_pl_solve_cap(m2d_system *s5, int cap);
+int m2d_pl_loop_cap_info(const m2d_system *s5, int *cap, int *reruns);
 
 /* Device arrays of one internal-flow time step (src/mach2d_main.f90:94-
  * 404): N-vectors in the reference layout unless stated. */

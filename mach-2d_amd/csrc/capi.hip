@@ -83,6 +83,8 @@ struct PlCtl {
     double mx;           // max |pl| of the latest iteration
     double normpl;       // max |pl| / p_avg of the latest iteration that ended normally
     int itp;             // iterations run
+    int overrun;         // a p' solve was still iterating after its enqueued iterations
+    int maxit;           // most MSI iterations of one p' solve in this call
     int pad;
 };
 
@@ -90,7 +92,7 @@ struct PlCtl {
 // captured for (pointers, loop parameters and the solver's launch settings).
 struct PlGraph {
     m2d_pl_loop_fields f;
-    int modvis, nitm, iters, xcd_stride, staged;
+    int modvis, nitm, iters, xcd_stride, staged, solve_cap;
     double tol;
     hipGraphExec_t exec;
 };
@@ -157,12 +159,18 @@ struct m2d_system {
     // pressure-correction loop: control block (device) and its pinned
     // mirror, the stream batches are captured on, the captured batches,
     // the previous call's iteration count (the next call's first batch)
-    // and the last call's host reads and batches
+    // and the last call's host reads and batches; the p' solves' enqueued
+    // iterations (setting: 0 from the previous call's longest solve, > 0
+    // fixed, < 0 all nitm_p), the previous call's longest solve, the last
+    // call's enqueued iterations and reruns, and the saved fields a rerun
+    // starts from
     PlCtl *pl = nullptr, *h_pl = nullptr;
     hipStream_t cap = nullptr;
     std::vector<PlGraph> plg;
     bool pl_graph = true;
     int pl_last = 0, pl_reads = 0, pl_batches = 0;
+    int pl_cap_set = 0, pl_maxit = 0, pl_cap_used = 0, pl_reruns = 0;
+    double *pl_save = nullptr;
 
     // time step (m2d_intflow_time_step): a second stream for the in-order
     // statistics that run beside the step, its events, the device scratch
@@ -265,6 +273,7 @@ extern "C" int m2d_destroy(m2d_system *s)
     if (s->h_ts) (void)hipHostFree(s->h_ts);
     if (s->pl) (void)hipFree(s->pl);
     if (s->h_pl) (void)hipHostFree(s->h_pl);
+    if (s->pl_save) (void)hipFree(s->pl_save);
     if (s->h_st) (void)hipHostFree(s->h_st);
     if (s->h_chunk) (void)hipHostFree(s->h_chunk);
     for (hipEvent_t e : s->ev_chunk)
@@ -855,6 +864,50 @@ __global__ void k_pl_start(SolveState *st, PlCtl *c)
     c->itp = 0;
     c->mx = 0.0;
     c->normpl = 0.0;
+    c->overrun = 0;
+    c->maxit = 0;
+}
+
+// After a p' solve enqueued with fewer iterations than nitm_p: a solve
+// whose stopping rule has not fired ran out of enqueued iterations.  The
+// loop ends there (every later kernel no-ops) and the host runs the call
+// again from the saved fields with all nitm_p iterations enqueued.
+__global__ void k_pl_solve_check(SolveState *st, PlCtl *c)
+{
+    if (threadIdx.x != 0 || st->skip) return;
+    if (!st->done) {
+        c->overrun = 1;
+        st->skip = 1;
+        st->done = 1;
+    }
+}
+
+// The fields the loop updates in place (ue, ve, un, vn, Uce, Vcn, roe, ron,
+// ro, p) to (save) or from the handle's copy, one launch.
+struct PlFields10 {
+    double *f[10];
+};
+constexpr int kPlSaved = 10;
+__global__ __launch_bounds__(256) void k_pl_fields_copy(PlFields10 p, double *save, long n,
+                                                        int to_save)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    double *slot = save + (long)blockIdx.y * n;
+    double *fld = p.f[blockIdx.y];
+    if (to_save)
+        slot[e] = fld[e];
+    else
+        fld[e] = slot[e];
+}
+
+static void pl_fields_copy(const m2d_pl_loop_fields *f, double *save, long n, bool to_save,
+                           hipStream_t s)
+{
+    const PlFields10 p = {{f->ue, f->ve, f->un, f->vn, f->Uce, f->Vcn, f->roe, f->ron, f->ro,
+                           f->p}};
+    hipLaunchKernelGGL(k_pl_fields_copy, dim3((unsigned)((n + 255) / 256), kPlSaved), dim3(256),
+                       0, s, p, save, n, to_save ? 1 : 0);
 }
 
 // Exit test at the end of one iteration (mach2d_main.f90:306-308): count
@@ -865,6 +918,7 @@ __global__ void k_pl_check(SolveState *st, PlCtl *c)
 {
     if (threadIdx.x != 0 || st->skip) return;
     c->itp += 1;
+    if (st->nit > c->maxit) c->maxit = st->nit;
     const bool err = st->outside || st->err_pivot || st->abort ||
                      st->status == M2D_ERR_NONFINITE || st->status == M2D_ERR_SYNC_TIMEOUT;
     if (!err) c->normpl = c->mx / c->p_avg;
@@ -892,14 +946,17 @@ static void enqueue_factor64(m2d_system *s, const double *d_a, bool gated = fals
 
 // fb2d5 / fb2d9 with at most nitm iterations, enqueued without host reads:
 // the device stopping rule turns the iterations after it into no-ops.
+// enq < nitm enqueues only the first enq of them (the stopping rule keeps
+// nitm).
 static void enqueue_solve64(m2d_system *s, const double *a, const double *rhs, double *x,
-                            double gamma, int nitm)
+                            double gamma, int nitm, int enq = -1)
 {
     Work<double> &W = s->w64;
     launch_residual_norm<double, double, double>(a, x, rhs, W.r, s->nx, s->ny, s->partial,
                                                  s->st, &W.sk, NormRule{nullptr, 0, 1, gamma, nitm},
                                                  s->stream, s->nd);
-    for (int k = 1; k <= nitm; ++k)
+    const int n = enq < 0 ? nitm : std::min(enq, nitm);
+    for (int k = 1; k <= n; ++k)
         enqueue_iteration<double>(s, W, s->st, a, rhs, x, nullptr, 0, k, gamma, nitm);
     launch_settle<double>(x, W.x2, s->N, s->st, s->stream);
 }
@@ -907,9 +964,11 @@ static void enqueue_solve64(m2d_system *s, const double *a, const double *rhs, d
 // `count` iterations of the pressure-correction loop (mach2d_main.f90:261-309)
 // enqueued on s->stream with no host read: every kernel is gated by
 // s->st->skip, which the exit test of the iteration that ends the loop sets.
-// The assembly calls read dt and beta from the control block.
+// The assembly calls read dt and beta from the control block.  Each p'
+// solve gets solve_cap of its nitm_p iterations (checked by
+// k_pl_solve_check when fewer).
 static int enqueue_pl_iterations(m2d_system *s, const m2d_pl_loop_fields *f, int modvis,
-                                 int nitm_p, double tol_p, int count)
+                                 int nitm_p, double tol_p, int count, int solve_cap)
 {
     const int nx = s->nx, ny = s->ny;
     const double dt = 0.0, beta = 0.0;  // placeholders: the kernels read s->pl
@@ -933,7 +992,9 @@ static int enqueue_pl_iterations(m2d_system *s, const m2d_pl_loop_fields *f, int
             break;
         }
         enqueue_factor64(s, f->ap, true);
-        enqueue_solve64(s, f->ap, f->bp, f->pl, tol_p, nitm_p);
+        enqueue_solve64(s, f->ap, f->bp, f->pl, tol_p, nitm_p, solve_cap);
+        if (solve_cap < nitm_p)
+            hipLaunchKernelGGL(k_pl_solve_check, dim3(1), dim3(64), 0, s->stream, s->st, s->pl);
         m2d_get_velocity_correction_at_faces_with_pl(&nx, &ny, f->xe, f->ye, f->xk, f->yk,
                                                      f->due, f->dve, f->dun, f->dvn, f->de,
                                                      f->dn, f->pl, f->ue, f->ve, f->un, f->vn,
@@ -964,12 +1025,12 @@ static int enqueue_pl_iterations(m2d_system *s, const m2d_pl_loop_fields *f, int
 // scalars live in the control block, so a batch serves every call with the
 // same arrays and loop parameters).
 static int pl_graph_for(m2d_system *s, const m2d_pl_loop_fields *f, int modvis, int nitm_p,
-                        double tol_p, int count, hipGraphExec_t *out)
+                        double tol_p, int count, int solve_cap, hipGraphExec_t *out)
 {
     for (const PlGraph &g : s->plg)
         if (!memcmp(&g.f, f, sizeof(*f)) && g.modvis == modvis && g.nitm == nitm_p &&
             g.tol == tol_p && g.iters == count && g.xcd_stride == s->xcd_stride &&
-            g.staged == (int)s->staged) {
+            g.staged == (int)s->staged && g.solve_cap == solve_cap) {
             *out = g.exec;
             return M2D_OK;
         }
@@ -979,7 +1040,7 @@ static int pl_graph_for(m2d_system *s, const m2d_pl_loop_fields *f, int modvis, 
     s->stream = s->cap;
     s->timing = false;
     HIP_OK(hipStreamBeginCapture(s->cap, hipStreamCaptureModeRelaxed));
-    int rc = enqueue_pl_iterations(s, f, modvis, nitm_p, tol_p, count);
+    int rc = enqueue_pl_iterations(s, f, modvis, nitm_p, tol_p, count, solve_cap);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s->cap, &g);
     s->stream = user;
@@ -993,12 +1054,13 @@ static int pl_graph_for(m2d_system *s, const m2d_pl_loop_fields *f, int modvis, 
     pg.iters = count;
     pg.xcd_stride = s->xcd_stride;
     pg.staged = (int)s->staged;
+    pg.solve_cap = solve_cap;
     pg.exec = nullptr;
     if (!rc && hipGraphInstantiate(&pg.exec, g, nullptr, nullptr, 0) != hipSuccess)
         rc = M2D_ERR_HIP;
     if (g) (void)hipGraphDestroy(g);
     if (rc) return rc;
-    if (s->plg.size() >= 4) {  // a few shapes of batch at most; drop the oldest
+    if (s->plg.size() >= 8) {  // a few shapes of batch at most; drop the oldest
         (void)hipGraphExecDestroy(s->plg.front().exec);
         s->plg.erase(s->plg.begin());
     }
@@ -1040,58 +1102,110 @@ extern "C" int m2d_intflow_pl_loop(m2d_system *s, const m2d_pl_loop_fields *f, i
         HIP_OK(hipMalloc((void **)&s->pl, sizeof(PlCtl)));
         HIP_OK(hipHostMalloc((void **)&s->h_pl, sizeof(PlCtl), hipHostMallocDefault));
     }
-    // pl = 0 (:257) and the per-call scalars, ahead of the iterations
+    // MSI iterations enqueued per p' solve: by default the previous call's
+    // longest solve plus kPlCapMargin, rounded up to a multiple of 4 (few
+    // distinct captured batches); the first call on a handle enqueues all
+    // nitm_p.  A solve that needs more ends the loop early, and the call
+    // runs again from the fields saved here with every iteration enqueued,
+    // so the results are those of nitm_p enqueued iterations either way.
+    constexpr int kPlCapMargin = 2;
+    int cap = nitm_p;
+    if (s->pl_cap_set > 0)
+        cap = std::min(nitm_p, s->pl_cap_set);
+    else if (s->pl_cap_set == 0 && s->pl_maxit > 0)
+        cap = std::min(nitm_p, (s->pl_maxit + kPlCapMargin + 3) / 4 * 4);
+    s->pl_cap_used = cap;
+    s->pl_reruns = 0;
+    if (cap < nitm_p) {
+        if (!s->pl_save)
+            HIP_OK(hipMalloc((void **)&s->pl_save, sizeof(double) * kPlSaved * s->N));
+        pl_fields_copy(f, s->pl_save, s->N, true, s->stream);
+        HIP_OK(hipGetLastError());
+    }
     PlCtl *h = s->h_pl;
-    memset(h, 0, sizeof(*h));
-    h->dt = dt;
-    h->beta = beta;
-    h->p_avg = p_avg;
-    h->tolm = tolm;
-    HIP_OK(hipMemsetAsync(f->pl, 0, sizeof(double) * s->N, s->stream));
-    HIP_OK(hipMemcpyAsync(s->pl, h, sizeof(PlCtl), hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_pl_start, dim3(1), dim3(64), 0, s->stream, s->st, s->pl);
-    HIP_OK(hipGetLastError());
-    // batches of iterations, each followed by one read of the control block
-    // and the solver state.  The first batch is as long as the previous
-    // call's loop, so a steady run reads the host once per call; the ones
-    // after it hold one iteration each (an iteration past the exit costs
-    // its launches, about 18 + 5 nitm_p of them, as no-ops).
-    int rc = M2D_OK, run = 0;
     s->pl_reads = s->pl_batches = 0;
-    const int first = std::min(itpmax, std::max(1, s->pl_last));
-    bool ended = false;
-    while (!rc && !ended && run < itpmax) {
-        const int count = run == 0 ? first : 1;
-        if (s->pl_graph) {
-            hipGraphExec_t exec = nullptr;
-            rc = pl_graph_for(s, f, modvis, nitm_p, tol_p, count, &exec);
-            if (!rc && hipGraphLaunch(exec, s->stream) != hipSuccess) rc = M2D_ERR_HIP;
-        } else {
-            rc = enqueue_pl_iterations(s, f, modvis, nitm_p, tol_p, count);
+    int rc = M2D_OK;
+    // one run of the loop with `solve_cap` iterations enqueued per solve
+    auto run_loop = [&](int solve_cap) -> int {
+        // pl = 0 (:257) and the per-call scalars, ahead of the iterations
+        memset(h, 0, sizeof(*h));
+        h->dt = dt;
+        h->beta = beta;
+        h->p_avg = p_avg;
+        h->tolm = tolm;
+        if (hipMemsetAsync(f->pl, 0, sizeof(double) * s->N, s->stream) != hipSuccess ||
+            hipMemcpyAsync(s->pl, h, sizeof(PlCtl), hipMemcpyHostToDevice, s->stream) !=
+                hipSuccess)
+            return M2D_ERR_HIP;
+        hipLaunchKernelGGL(k_pl_start, dim3(1), dim3(64), 0, s->stream, s->st, s->pl);
+        if (hipGetLastError() != hipSuccess) return M2D_ERR_HIP;
+        // batches of iterations, each followed by one read of the control
+        // block and the solver state.  The first batch is as long as the
+        // previous call's loop, so a steady run reads the host once per
+        // call; the ones after it hold one iteration each (an iteration past
+        // the exit costs its launches, about 18 + 2 solve_cap of them, as
+        // no-ops).
+        int r = M2D_OK, run = 0;
+        const int first = std::min(itpmax, std::max(1, s->pl_last));
+        bool ended = false;
+        while (!r && !ended && run < itpmax) {
+            const int count = run == 0 ? first : 1;
+            if (s->pl_graph) {
+                hipGraphExec_t exec = nullptr;
+                r = pl_graph_for(s, f, modvis, nitm_p, tol_p, count, solve_cap, &exec);
+                if (!r && hipGraphLaunch(exec, s->stream) != hipSuccess) r = M2D_ERR_HIP;
+            } else {
+                r = enqueue_pl_iterations(s, f, modvis, nitm_p, tol_p, count, solve_cap);
+            }
+            if (r) break;
+            run += count;
+            ++s->pl_batches;
+            if (hipMemcpyAsync(h, s->pl, sizeof(PlCtl), hipMemcpyDeviceToHost, s->stream) !=
+                hipSuccess) {
+                r = M2D_ERR_HIP;
+                break;
+            }
+            r = read_state(s, s->st);  // also completes the copy of the control block
+            ++s->pl_reads;
+            if (r) break;
+            const SolveState &st = *s->h_st;
+            if (st.outside) r = M2D_ERR_OUTSIDE_COUPLING;
+            else if (st.err_pivot || st.status == M2D_ERR_NONFINITE) r = M2D_ERR_NONFINITE;
+            else if (st.abort || st.status == M2D_ERR_SYNC_TIMEOUT) r = M2D_ERR_SYNC_TIMEOUT;
+            ended = st.skip != 0;
         }
-        if (rc) break;
-        run += count;
-        ++s->pl_batches;
-        if (hipMemcpyAsync(h, s->pl, sizeof(PlCtl), hipMemcpyDeviceToHost, s->stream) !=
-            hipSuccess) {
-            rc = M2D_ERR_HIP;
-            break;
-        }
-        rc = read_state(s, s->st);  // also completes the copy of the control block
-        ++s->pl_reads;
-        if (rc) break;
-        const SolveState &st = *s->h_st;
-        if (st.outside) rc = M2D_ERR_OUTSIDE_COUPLING;
-        else if (st.err_pivot || st.status == M2D_ERR_NONFINITE) rc = M2D_ERR_NONFINITE;
-        else if (st.abort || st.status == M2D_ERR_SYNC_TIMEOUT) rc = M2D_ERR_SYNC_TIMEOUT;
-        ended = st.skip != 0;
+        return r;
+    };
+    rc = run_loop(cap);
+    if (!rc && h->overrun) {
+        pl_fields_copy(f, s->pl_save, s->N, false, s->stream);
+        rc = hipGetLastError() == hipSuccess ? run_loop(nitm_p) : M2D_ERR_HIP;
+        s->pl_reruns = 1;
     }
     const int itp = std::min(std::max(h->itp, 1), itpmax);
-    if (!rc) s->pl_last = h->itp;
+    if (!rc) {
+        s->pl_last = h->itp;
+        s->pl_maxit = h->maxit;
+    }
     if (itp_out) *itp_out = itp;
     if (normpl_out) *normpl_out = h->normpl;
     set_last(rc);
     return rc;
+}
+
+extern "C" int m2d_set_pl_solve_cap(m2d_system *s, int cap)
+{
+    if (!s) return M2D_ERR_ARG;
+    s->pl_cap_set = cap;
+    return M2D_OK;
+}
+
+extern "C" int m2d_pl_loop_cap_info(const m2d_system *s, int *cap, int *reruns)
+{
+    if (!s || !cap || !reruns) return M2D_ERR_ARG;
+    *cap = s->pl_cap_used;
+    *reruns = s->pl_reruns;
+    return M2D_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -63,7 +63,7 @@ EXPORTS = [
     "m2d_initialize_boundary_faces_flags", "m2d_check_mass_conservation",
     "m2d_intflow_set_bcu", "m2d_intflow_set_bcv", "m2d_intflow_set_bcpl", "m2d_intflow_set_bcT",
     "m2d_intflow_velocities_at_boundary_faces", "m2d_intflow_pl_loop",
-    "m2d_set_pl_graph", "m2d_pl_loop_info",
+    "m2d_set_pl_graph", "m2d_pl_loop_info", "m2d_set_pl_solve_cap", "m2d_pl_loop_cap_info",
     "m2d_get_mean_cc_9d", "m2d_get_mean_cc_5d", "m2d_tstepper_dt0_scarborough",
     "m2d_tstepper_dt_scarborough",
     "m2d_get_real_centroids_xy", "m2d_get_metrics", "m2d_get_radius",
@@ -194,6 +194,10 @@ def load():
     L.m2d_set_pl_graph.restype = C.c_int
     L.m2d_pl_loop_info.argtypes = [_vp, _ip, _ip]
     L.m2d_pl_loop_info.restype = C.c_int
+    L.m2d_set_pl_solve_cap.argtypes = [_vp, C.c_int]
+    L.m2d_set_pl_solve_cap.restype = C.c_int
+    L.m2d_pl_loop_cap_info.argtypes = [_vp, _ip, _ip]
+    L.m2d_pl_loop_cap_info.restype = C.c_int
     for f in (L.m2d_get_mean_cc_9d, L.m2d_get_mean_cc_5d):
         f.argtypes = [_ip, _ip, _vp]
         f.restype = C.c_double

@@ -73,6 +73,25 @@ def pl_loop_info(dev):
     return reads.value, batches.value
 
 
+def set_pl_solve_cap(dev, cap):
+    """MSI iterations enqueued per p' solve of the loop (m2d_set_pl_solve_cap):
+    0 from the previous call's longest solve (the default), > 0 that many,
+    < 0 all nitm_p; a solve that needs more makes the call run again with
+    all of them, so the results do not depend on it."""
+    rc = _lib.load().m2d_set_pl_solve_cap(dev.h, int(cap))
+    if rc:
+        raise SolverError(rc, "m2d_set_pl_solve_cap")
+
+
+def pl_loop_cap_info(dev):
+    """(iterations enqueued per p' solve, reruns) of the last call on `dev`."""
+    cap, reruns = C.c_int(), C.c_int()
+    rc = _lib.load().m2d_pl_loop_cap_info(dev.h, C.byref(cap), C.byref(reruns))
+    if rc:
+        raise SolverError(rc, "m2d_pl_loop_cap_info")
+    return cap.value, reruns.value
+
+
 # ---------------------------------------------------------------------------
 # One internal-flow time step (src/mach2d_main.f90:94-404), m2d_intflow_time_step
 # ---------------------------------------------------------------------------

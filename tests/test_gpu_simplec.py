@@ -125,3 +125,32 @@ def test_pl_loop_exit_inside_a_batch(gpu, orc):
         got = t[key].cpu().numpy()
         assert np.array_equal(got.reshape(want[key].shape), want[key]), key
     dev.close()
+
+
+@pytest.mark.parametrize("cap", [1, 19])
+def test_pl_loop_short_solve_capture_reruns_same_bits(gpu, orc, cap):
+    """Each p' solve enqueued with `cap` of its nitm_p = 20 iterations
+    (m2d_set_pl_solve_cap): these solves take 18-20 iterations, so the
+    first one runs out, the loop ends early and the call runs again from
+    the saved fields with all 20 enqueued.  Every field, itp and
+    max|pl| / p_avg equal the oracle's loop bit for bit, on the call that
+    captures the batches and on the one that replays them."""
+    from mach2d import simplec
+    nx, ny = 70, 38
+    f = PL.loop_fields(nx, ny, seed=11)
+    p_avg = float(np.mean(f["p"]))
+    kw = dict(modvis=0, dt=2e-3, beta=0.3, p_avg=p_avg, tolm=0.0, itpmax=4, nitm_p=20,
+              tol_p=1e-2)
+    want, itp0, n0 = PL.run(orc, f, nx, ny, kw["modvis"], kw["dt"], kw["beta"], p_avg,
+                            kw["tolm"], kw["itpmax"], kw["nitm_p"], kw["tol_p"])
+    dev = gpu.DeviceMSI(nx, ny, "fp64", ndiag=5)
+    simplec.set_pl_solve_cap(dev, cap)
+    for call in range(2):
+        t = _tensors(f)
+        itp, normpl = simplec.intflow_pl_loop(dev, t, **kw)
+        assert simplec.pl_loop_cap_info(dev) == (cap, 1), call
+        assert itp == itp0 and normpl == n0, call
+        for key in want:
+            got = t[key].cpu().numpy()
+            assert np.array_equal(got.reshape(want[key].shape), want[key]), (call, key)
+    dev.close()

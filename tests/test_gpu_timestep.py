@@ -105,6 +105,42 @@ def test_time_step_pcr_tdma_within_1e9_of_oracle(gpu, orc):
         dev5.close()
 
 
+@pytest.mark.parametrize("cap", [0, 1])
+def test_time_step_p_solve_capture_same_bits(gpu, orc, cap):
+    """The 258x130 nozzle at inlet Mach 0.2, u and v by TDMA, three steps
+    with the p' solves' enqueued iterations trimmed (cap 0: from the
+    previous call's longest solve, fewer than nitm_p = 20 after the first
+    step, no rerun needed) or cut to one (cap 1: a solve that needs more
+    makes its loop run again with all 20).  Every field, dt and the p'
+    iteration counts bit for bit the reference's."""
+    import torch
+
+    from mach2d import simplec
+    nx, ny, dt0 = 258, 130, 5e-6
+    f = TS.step_fields(nx, ny, seed=nx, mach_in=0.2)
+    prm = dict(TS.PARAMS, modvis=0, solver9d=1)
+    d = {k: torch.tensor(np.ascontiguousarray(v).reshape(-1), device="cuda")
+         for k, v in f.items()}
+    dev5 = gpu.DeviceMSI(nx, ny, ndiag=5)
+    simplec.set_pl_solve_cap(dev5, cap)
+    torch.cuda.synchronize()
+    s, dt_o, dt_g = f, dt0, dt0
+    for it in range(3):
+        s, dt_o, ro = TS.step(orc, s, nx, ny, prm, dt_o)
+        dt_g, rg = simplec.intflow_time_step(None, dev5, d, prm, dt_g)
+        used, reruns = simplec.pl_loop_cap_info(dev5)
+        if cap == 0:
+            assert (used < 20 and reruns == 0) if it else used == 20, (it, used, reruns)
+        else:
+            assert used == 1, (it, used)
+        assert rg["itp"] == ro["itp"] and dt_g == dt_o, (it, rg, ro)
+        assert rg["normpl"] == ro["normpl"] and rg["rmass"] == ro["rmass"], it
+        for k in simplec.TS_STATE + simplec.TS_LEVEL + ("pin", "plin", "Tin"):
+            got = d[k].cpu().numpy().reshape(np.asarray(s[k]).shape)
+            assert np.array_equal(got, s[k]), (it, k, np.abs(got - s[k]).max())
+    dev5.close()
+
+
 @pytest.mark.parametrize("bad", [dict(itpmax=0), dict(nitm_p=0), dict(modvis=2),
                                  dict(itmmax=0), dict(solver9d=3)])
 def test_time_step_rejects_bad_parameters_before_changing_fields(gpu, bad):
