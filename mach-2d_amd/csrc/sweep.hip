@@ -1005,8 +1005,12 @@ __device__ __forceinline__ void factor_band(const T *__restrict__ a, T *__restri
         for (int k = 0; k < NU; ++k)
             vstore<T, H>(du + k * L + e + h * H, reinterpret_cast<const T(&)[H]>(oU[k][h * H]));
     };
+    // operands are loaded Ahead chunks before use: two for lu2d5's five
+    // arrays (a ring of three register buffers), one for nine (whose
+    // registers do not allow a third buffer)
+    constexpr int Ahead = Five ? 2 : 1;
     auto chunk = [&](T(&cb)[NA][F], T(&nbuf)[NA][F], int c) -> bool {
-        load_data(nbuf, c + 1);
+        load_data(nbuf, c + Ahead);
         load_bnd(bn, c + 1);
         T oL[NL][F], oU[NU][F];
         const int i0 = c * F - pad - 2 * lane;
@@ -1058,14 +1062,26 @@ __device__ __forceinline__ void factor_band(const T *__restrict__ a, T *__restri
     };
 
     load_data(A, 0);
+    if constexpr (Five) load_data(B, 1);
     load_bnd(bn, 0);
     if (!ready(bn, 0)) return;
     take(bn, 0);
 #pragma unroll
     for (int k = 0; k < NU; ++k) se[k] = lane == 0 ? val[k] : T(0);
-    for (int c = 0; c < nf; c += 2) {
-        if (!chunk(A, B, c) || c + 1 >= nf) break;
-        if (!chunk(B, A, c + 1)) break;
+    if constexpr (Five) {
+        // chunk c computes from buffer c mod 3 and loads chunk c + 2 into
+        // the buffer chunk c - 1 used
+        T C[NA][F];
+        for (int c = 0; c < nf; c += 3) {
+            if (!chunk(A, C, c) || c + 1 >= nf) break;
+            if (!chunk(B, A, c + 1) || c + 2 >= nf) break;
+            if (!chunk(C, B, c + 2)) break;
+        }
+    } else {
+        for (int c = 0; c < nf; c += 2) {
+            if (!chunk(A, B, c) || c + 1 >= nf) break;
+            if (!chunk(B, A, c + 1)) break;
+        }
     }
 }
 
