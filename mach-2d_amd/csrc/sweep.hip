@@ -906,14 +906,18 @@ __device__ __forceinline__ void backward_band_staged(T *__restrict__ wz, const T
 // neighbour, and the step needs no masks.  Pivots are checked afterwards,
 // by prepare_entry.  Per step the wave issues the point's arithmetic, three
 // divisions (dl4, dl2, 1/dl1; the four du quotients through 1/dl1) and two
-// lane shifts per du value; the band below's edge values reach lane 0
-// through LDS once per chunk, and the edge row goes out the same way.
+// lane shifts per du value; the band below's edge values reach lane 0 by a
+// row-local lane shift once per chunk.  Results leave through LDS as the
+// sweeps' do: wave 0 computes and writes each half chunk of dl and du into
+// one of two ring slots, wave 1 stores the chunk and publishes the band's
+// last row (the du values of each column) to the band above, so the
+// computing wave issues no global store (whose registers would stay busy
+// until it completes, behind the operand loads in the in-order counter).
 // --------------------------------------------------------------------------
-// this band's last row over the F columns of a chunk, on its way from lane
-// 63 to lanes 0..F-1 (which publish one column each)
-template <typename T, int F>
-struct FactorEdge {
-    T out[4][F];
+template <typename T, int F, int NO>
+struct FactorRing {
+    T v[2][NO][kWave * F];
+    int stop;
 };
 
 // fn(integral_constant<int, I>) for I = 0 .. N-1: the step index as a
@@ -935,20 +939,43 @@ __device__ __forceinline__ void factor_band(const T *__restrict__ a, T *__restri
     // Five (lu2d5): the S, W, P, E, N arrays of the skewed 9-array copy in,
     // dl1..dl4 and du1..du3 out (dl5 and du4 of the embedding are zero and
     // are neither read nor written)
-    constexpr int NA = Five ? 5 : 9, NL = Five ? 4 : 5, NU = Five ? 3 : 4;
+    constexpr int NA = Five ? 5 : 9, NL = Five ? 4 : 5, NU = Five ? 3 : 4, NO = NL + NU;
     constexpr int S = 2 * F;  // skew chunk: 64 bytes per lane and array
-    __shared__ FactorEdge<T, F> E;
-    const int lane = threadIdx.x;
+    constexpr int H = F / 2;  // results go to the ring per 16-byte half chunk
+    __shared__ FactorRing<T, F, NO> ring;
+    const int lane = threadIdx.x & (kWave - 1);
     const int nx = sk.nx, ny = sk.ny, pad = sk.pad(band);
     const int nf = sk.nst / F;
     const long L = sk.size();
-    const int j = band * kWave + lane;
     const bool has_up = band > 0, has_down = band < sk.nb - 1;
     constexpr int NW = TagWords<T>::n;
     constexpr int CW = 4 * NW;
+    const unsigned lofs = (unsigned)lane * F;
+    if (threadIdx.x >= kWave) {  // the storing wave
+        u64 *bout = lk.bnd + (long)band * nx * CW;
+        for (int c = 0; c < nf; ++c) {
+            handoff();
+            if (ring_stopped(ring)) return;
+            const long e = sk.template index_c<S>(band, lane, c * F);
+#pragma unroll
+            for (int k = 0; k < NO; ++k) {
+                T vals[F];
+                vload<T, F>(vals, ring.v[c & 1][k] + lofs);
+                vstore<T, F>((k < NL ? dl + k * L : du + (k - NL) * L) + e, vals);
+            }
+            // lane q < F: lane 63's du values of step q, column c*F + q - pad - 126
+            const int col = c * F + lane - pad - 2 * (kWave - 1);
+            if (has_down && lane < F && col >= 0 && col < nx) {
+#pragma unroll
+                for (int k = 0; k < NU; ++k)
+                    put_tagged(bout + (long)col * CW + k * NW, lk.gen,
+                               ring.v[c & 1][NL + k][(kWave - 1) * F + lane]);
+            }
+        }
+        return;
+    }
+    const int j = band * kWave + lane;
     const u64 *bin = lk.bnd + (long)(has_up ? band - 1 : 0) * nx * CW;
-    u64 *bout = lk.bnd + (long)band * nx * CW;
-    u64 *scratch = lk.scratch + ((long)band * kWave + lane) * 8;
     // alpha of this row's real volumes (alpha_at: 1 <= i <= nx-2 as well)
     const T arow = (j >= 1 && j <= ny - 2) ? alpha0 : T(0);
     auto aslot = [](int k) { return Five ? (k == 0 ? kS : k == 1 ? kW : k == 2 ? kP : k == 3 ? kE : kN) : k; };
@@ -994,16 +1021,11 @@ __device__ __forceinline__ void factor_band(const T *__restrict__ a, T *__restri
 #pragma unroll
         for (int k = 0; k < NU; ++k) val[k] = bneed(c) ? raw_value(b[k]) : T(0);
     };
-    // the values of steps h*H .. h*H+H-1 (H = F/2: 16-byte halves) go out as
-    // soon as they exist, so only half a chunk of results is held
-    auto put_half = [&](const T(&oL)[NL][F], const T(&oU)[NU][F], long e, int h) {
-        constexpr int H = F / 2;
-#pragma unroll
-        for (int k = 0; k < NL; ++k)
-            vstore<T, H>(dl + k * L + e + h * H, reinterpret_cast<const T(&)[H]>(oL[k][h * H]));
-#pragma unroll
-        for (int k = 0; k < NU; ++k)
-            vstore<T, H>(du + k * L + e + h * H, reinterpret_cast<const T(&)[H]>(oU[k][h * H]));
+    // Giving up: tell the storing wave, meet it at the barrier it waits at.
+    auto bail = [&]() -> bool {
+        if (lane == 0) ring.stop = 1;
+        handoff();
+        return false;
     };
     // operands are loaded Ahead chunks before use: two for lu2d5's five
     // arrays (a ring of three register buffers), one for nine (whose
@@ -1014,7 +1036,7 @@ __device__ __forceinline__ void factor_band(const T *__restrict__ a, T *__restri
         load_bnd(bn, c + 1);
         T oL[NL][F], oU[NU][F];
         const int i0 = c * F - pad - 2 * lane;
-        const long e = sk.template index_c<S>(band, lane, c * F);
+        auto &slot = ring.v[c & 1];
         static_for<0, F>([&](auto St) {
             constexpr int s = decltype(St)::value;
             static_assert(s + 1 <= 15, "lane 0's edge value comes by row_shl");
@@ -1040,31 +1062,34 @@ __device__ __forceinline__ void factor_band(const T *__restrict__ a, T *__restri
                 oU[k][s] = Uv[k];
                 cur[k] = Uv[k];
             }
-            if constexpr (s % (F / 2) == F / 2 - 1) put_half(oL, oU, e, s / (F / 2));
-        });
-        if (c + 1 < nf) {
-            if (!ready(bn, c + 1)) return false;
-            take(bn, c + 1);
-        }
-        if (has_down) {
-            if (lane == kWave - 1) {
+            if constexpr (s % H == H - 1) {
+                constexpr int h = s / H;
 #pragma unroll
-                for (int k = 0; k < NU; ++k) vstore<T, F>(E.out[k], oU[k]);
+                for (int k = 0; k < NL; ++k)
+                    vstore<T, H>(slot[k] + lofs + h * H,
+                                 reinterpret_cast<const T(&)[H]>(oL[k][h * H]));
+#pragma unroll
+                for (int k = 0; k < NU; ++k)
+                    vstore<T, H>(slot[NL + k] + lofs + h * H,
+                                 reinterpret_cast<const T(&)[H]>(oU[k][h * H]));
             }
-            const int col = c * F + lane - pad - 2 * (kWave - 1);
-            const bool ok = lane < F && col >= 0 && col < nx;
-            u64 *p = ok ? bout + (long)col * CW : scratch;
-#pragma unroll
-            for (int k = 0; k < NU; ++k)
-                put_tagged(p + (ok ? k * NW : 0), lk.gen, E.out[k][lane < F ? lane : 0]);
+        });
+        handoff();
+        if (c + 1 < nf) {
+            if (!ready(bn, c + 1)) return bail();
+            take(bn, c + 1);
         }
         return true;
     };
 
+    if (lane == 0) ring.stop = 0;
     load_data(A, 0);
     if constexpr (Five) load_data(B, 1);
     load_bnd(bn, 0);
-    if (!ready(bn, 0)) return;
+    if (!ready(bn, 0)) {
+        bail();
+        return;
+    }
     take(bn, 0);
 #pragma unroll
     for (int k = 0; k < NU; ++k) se[k] = lane == 0 ? val[k] : T(0);
@@ -1156,7 +1181,7 @@ __global__ __launch_bounds__(2 * kWave) void k_backward_staged(T *__restrict__ w
 }
 
 template <typename T, int F, bool Five>
-__global__ __launch_bounds__(kWave) void k_factor(const T *__restrict__ a, T *__restrict__ dl,
+__global__ __launch_bounds__(2 * kWave) void k_factor(const T *__restrict__ a, T *__restrict__ dl,
                                                   T *__restrict__ du, T alpha0, Skew sk, Link lk0,
                                                   int xcd_stride)
 {
@@ -1232,7 +1257,7 @@ __global__ __launch_bounds__(2 * kWave) void k_backward_batch_staged(
 }
 
 template <typename T, int F>
-__global__ __launch_bounds__(kWave) void k_factor_batch(const BatchSys<T> *__restrict__ sys,
+__global__ __launch_bounds__(2 * kWave) void k_factor_batch(const BatchSys<T> *__restrict__ sys,
                                                         const int2 *__restrict__ map)
 {
     const int2 m = map[blockIdx.x];
@@ -1425,7 +1450,7 @@ void launch_factor(const T *a_sk, T *dl, T *du, T alpha0, const Skew &sk, Link l
                    hipStream_t s, bool five)
 {
     constexpr int F = factor_steps<T>();
-    const dim3 g(sk.nb * xcd_stride), b(kWave);
+    const dim3 g(sk.nb * xcd_stride), b(2 * kWave);  // computing and storing wave
     if (five)
         hipLaunchKernelGGL((k_factor<T, F, true>), g, b, 0, s, a_sk, dl, du, alpha0, sk, lk,
                            xcd_stride);
@@ -1482,8 +1507,8 @@ void launch_prepare_factor(T *dl, T *du, const Skew &sk, hipStream_t s, SolveSta
 template <typename T>
 void launch_factor_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_factor_batch<T, factor_steps<T>()>), dim3(nblk), dim3(kWave), 0, s,
-                       sys, map);
+    hipLaunchKernelGGL((k_factor_batch<T, factor_steps<T>()>), dim3(nblk), dim3(2 * kWave), 0,
+                       s, sys, map);
 }
 template <typename T>
 void launch_forward_batch(const BatchSys<T> *sys, const int2 *map, int nblk, hipStream_t s,
