@@ -69,8 +69,8 @@ BYTES_PER_POINT = {
 # profiles/msi_1026x514_hbm_traffic.md).  "residual" is the update and the
 # residual of one launch.
 MEASURED_TRAFFIC = {
-    "sweeps": 84.73e6, "residual": 67.14e6, "factor": 152.32e6,
-    "sweeps32": 39.27e6, "residual32": 33.75e6, "factor32": 65.42e6,
+    "sweeps": 84.95e6, "residual": 67.13e6, "factor": 111.72e6,
+    "sweeps32": 39.25e6, "residual32": 33.75e6, "factor32": 55.94e6,
 }
 
 
