@@ -301,6 +301,9 @@ __global__ __launch_bounds__(256) void k_tdma_d(TdmaBatch B)
 
 typedef __attribute__((address_space(1))) const void tdma_gvoid;
 typedef __attribute__((address_space(3))) void tdma_lvoid;
+// ring slots as LDS pointers: a generic pointer cast to LDS per load costs a
+// null test and a select each
+typedef __attribute__((address_space(3))) double tdma_ldouble;
 
 // s_waitcnt vmcnt(N) alone (expcnt and lgkmcnt left alone)
 template <int N>
@@ -319,7 +322,7 @@ __device__ __forceinline__ void tdma_wait_vmem()
 // aligned since LP, l0 and f % R are even).  step(s) maps the chunk's step
 // index to the line step (clamped into the line by the caller's mapping).
 template <int NA, int R, typename STEP>
-__device__ __forceinline__ void tdma_issue(double *slot, const double *const (&arr)[NA], long LP,
+__device__ __forceinline__ void tdma_issue(tdma_ldouble *slot, const double *const (&arr)[NA], long LP,
                                            int l0, int lane, STEP step)
 {
 #pragma unroll
@@ -339,7 +342,7 @@ __device__ __forceinline__ void tdma_issue(double *slot, const double *const (&a
 // array is the lane's fixed offset goff[ii] = (+-(f / R)) * LP + l0 + f % R
 // plus the wave-uniform t0 * LP, so no lane recomputes a 64-bit product.
 template <int NA>
-__device__ __forceinline__ void tdma_issue_at(double *slot, const double *const (&arr)[NA],
+__device__ __forceinline__ void tdma_issue_at(tdma_ldouble *slot, const double *const (&arr)[NA],
                                               long base, const long (&goff)[kTdmaNI])
 {
 #pragma unroll
@@ -397,6 +400,7 @@ __device__ __forceinline__ void tdma_pre_body(const TdmaBatch &B, int blk, doubl
     const int len = k.len;
     const long T = (long)RS * tdma_rows(len, R);
     double *ring = lds, *tden = lds + kTdmaSlots * NA * kTdmaChunk, *tP = tden + T;
+    tdma_ldouble *lring = (tdma_ldouble *)lds;
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         const int wl = lane < R ? lane : R + (lane - R) * RS;  // idle lanes (R >= 32)
@@ -405,7 +409,7 @@ __device__ __forceinline__ void tdma_pre_body(const TdmaBatch &B, int blk, doubl
                                        tdma_arr(B, S, XPASS, kTUp)};
         const int nch = (len + K - 1) / K;
         auto issue = [&](int c) {
-            tdma_issue<NA, R>(ring + (c % kTdmaSlots) * NA * kTdmaChunk, arr, k.LP, k.l0, lane,
+            tdma_issue<NA, R>(lring + (c % kTdmaSlots) * NA * kTdmaChunk, arr, k.LP, k.l0, lane,
                               [&](int s) { return min(c * K + s, len - 1); });
         };
         for (int c = 0; c < kTdmaSlots - 1; ++c) issue(c);
@@ -470,6 +474,7 @@ __global__ __launch_bounds__(256) void k_tdma_line(TdmaBatch B)
     const TdmaSys S = B.s[k.q];
     const int len = k.len;
     double *ring = lds, *tile = lds + kTdmaSlots * NA * kTdmaChunk;
+    tdma_ldouble *lring = (tdma_ldouble *)lds;
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         const int wl = lane < R ? lane : R + (lane - R) * RS;  // idle lanes (R >= 32)
@@ -484,7 +489,7 @@ __global__ __launch_bounds__(256) void k_tdma_line(TdmaBatch B)
             long goff[kTdmaNI];
             tdma_lane_offsets<R>(goff, k.LP, k.l0, lane, 1);
             auto issue = [&](int c) {
-                double *slot = ring + (c % kTdmaSlots) * NA * kTdmaChunk;
+                tdma_ldouble *slot = lring + (c % kTdmaSlots) * NA * kTdmaChunk;
                 if (c * K + K <= len)
                     tdma_issue_at<NA>(slot, arr, (long)(c * K) * k.LP, goff);
                 else
@@ -548,7 +553,7 @@ __global__ __launch_bounds__(256) void k_tdma_line(TdmaBatch B)
             long goff[kTdmaNI];
             tdma_lane_offsets<R>(goff, k.LP, k.l0, lane, -1);
             auto issue = [&](int c) {
-                double *slot = ring + (c % kTdmaSlots) * kTdmaChunk;
+                tdma_ldouble *slot = lring + (c % kTdmaSlots) * kTdmaChunk;
                 if (len - 2 - c * K - (K - 1) >= 0)
                     tdma_issue_at<1>(slot, arr, (long)(len - 2 - c * K) * k.LP, goff);
                 else
