@@ -1562,8 +1562,14 @@ namespace m2d {
 int norm_l1_device(const double *a, const double *b, const double *var, int nx, int ny,
                    double *d_work, double *h_out, hipStream_t s);
 int norm_l1_work_size(int nx, int ny);
+int norm_l1_enqueue(const double *a, const double *b, const double *var, int nx, int ny,
+                    double *d_work, double *d_res, hipStream_t s);
+void norm5_enqueue(const double *var, const double *b, const double *a, int nx, int ny,
+                   double *d_scr, double *d_res, hipStream_t s);
 int tdma_device_many(int count, int nx, int ny, int nitm, const double *const *a,
                      const double *const *b, double *const *x, hipStream_t s);
+int tdma_enqueue_many(int count, int nx, int ny, int nitm, const double *const *a,
+                      const double *const *b, double *const *x, hipStream_t s);
 }  // namespace m2d
 
 // g = 1 / (Rg T) (mach2d_main.f90:153)
@@ -1607,16 +1613,20 @@ static int solve9(m2d_system *s9, const m2d_time_step_params *P, const double *a
 // The time step's side stream and its scratch: the residual norms' work
 // (norm_l1_work_size), then the in-order sums': the average of p (n + 2
 // doubles), the terms of the four mean coefficients and of the mass check
-// (5 n) and their five sums, n = (nx-2)(ny-2); h_ts receives p's average and
-// the five sums.  Created with the first step on the handle.
+// (5 n) and their five sums, n = (nx-2)(ny-2), then kTsNormSlots sums of the
+// step's residual norms; h_ts receives p's average, the five sums and the
+// norms' sums.  Created with the first step on the handle.
+constexpr int kTsNormSlots = 8;
 static int ts_side_ready(m2d_system *s)
 {
     if (s->aux) return M2D_OK;
     const long n = (long)(s->nx - 2) * (s->ny - 2);
     HIP_OK(hipMalloc((void **)&s->ts_scr,
                      sizeof(double) * (size_t)(norm_l1_work_size(s->nx, s->ny) +
-                                               average_scratch(s->nx, s->ny) + 5 * n + 5)));
-    HIP_OK(hipHostMalloc((void **)&s->h_ts, sizeof(double) * 6, hipHostMallocDefault));
+                                               average_scratch(s->nx, s->ny) + 5 * n + 5 +
+                                               kTsNormSlots)));
+    HIP_OK(hipHostMalloc((void **)&s->h_ts, sizeof(double) * (6 + kTsNormSlots),
+                         hipHostMallocDefault));
     HIP_OK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
     HIP_OK(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
@@ -1675,14 +1685,30 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
     }
     double *const work = s5->ts_scr;
     double *const side = work + norm_l1_work_size(nx, ny);
+    const long nin = (long)(nx - 2) * (ny - 2);
     double norm = 0.0, rmass = 0.0, normpl = 0.0;
     int itp = 0, stage = 1;
+    // The residual norms the step adds up (mach2d_main.f90:229-236, 311,
+    // 380): their sums are enqueued into device slots as the step goes and
+    // read once with the step's last wait, then added in the same order.
+    // kind 9: norm_l1_9d relative (two sums, slots k and k+1); kind 5:
+    // norm_l1_5d (one sum).  A new mass cycle starts the list again.
+    double *const nres = side + average_scratch(nx, ny) + 5 * nin + 5;
+    int nkind[kTsNormSlots], nslot[kTsNormSlots], nops = 0, nused = 0;
     auto norm9_relative = [&](const double *a, const double *b, const double *x) {
-        double o[2] = {0.0, 0.0};
         if (rc) return;
-        rc = norm_l1_device(a, b, x, nx, ny, work, o, st);
-        const double eps = 10.0 * 2.220446049250313080847e-16;  // 10*epsilon(1d0)
-        if (!rc) norm = (o[1] > eps) ? norm + o[0] / o[1] : norm + o[0];
+        rc = norm_l1_enqueue(a, b, x, nx, ny, work, nres + nused, st);
+        nkind[nops] = 9;
+        nslot[nops++] = nused;
+        nused += 2;
+    };
+    auto norm5 = [&](const double *a, const double *b, const double *x) {
+        if (rc) return;
+        norm5_enqueue(x, b, a, nx, ny, work, nres + nused, st);
+        if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
+        nkind[nops] = 5;
+        nslot[nops++] = nused;
+        nused += 1;
     };
     // time levels (mach2d_main.f90:96-104)
     copy(f->roa, f->ro, N);
@@ -1723,7 +1749,7 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
         if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
     }
     for (int itm = 1; itm <= P->itmmax && !rc; ++itm) {  // mass cycle (:155-344)
-        norm = 0.0;
+        nops = nused = 0;  // norm = 0
         stage = 4;
         m2d_get_u_coefficients(&nx, &ny, &modvis, &dt, f->rp, f->re, f->rn, f->Jp, f->Je, f->Jn,
                                f->ye, f->yk, f->alphae, f->betae, f->betan, f->gamman, f->vle,
@@ -1763,7 +1789,7 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
             // solved in the same launches; each result is its solo one
             const double *as[2] = {f->au, f->av}, *bs[2] = {f->bu, f->bv};
             double *xs[2] = {f->u, f->v};
-            rc = tdma_device_many(2, nx, ny, P->nitm_u, as, bs, xs, st);
+            rc = tdma_enqueue_many(2, nx, ny, P->nitm_u, as, bs, xs, st);
             if (rc) break;
             norm9_relative(f->au, f->bu, f->u);
             norm9_relative(f->av, f->bv, f->v);
@@ -1806,7 +1832,7 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
                                  P->nitm_p, P->tol_p, &itp, &normpl);
         if (rc) break;
         stage = 9;
-        m2d_norm_l1_5d(&nx, &ny, f->pl, f->bp, f->ap, &norm);
+        norm5(f->ap, f->bp, f->pl);
         if (ok())
             m2d_get_u_v_at_real_nodes_with_pl(&nx, &ny, f->xe, f->ye, f->xk, f->yk, f->rp, f->pl,
                                               f->au, f->av, f->u, f->v);
@@ -1821,7 +1847,6 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
     // at, ap are final once the mass cycles end, the four mean coefficients
     // of the next time step (:394).  Scratch: terms of the four means at
     // t[0, 4n), of the mass check at t[4n, 5n), the sums at t[5n, 5n + 5).
-    const long nin = (long)(nx - 2) * (ny - 2);
     double *tsum = side + average_scratch(nx, ny);
     bool side_pending = false;
     if (!rc && euler) {
@@ -1895,7 +1920,22 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
                                                       &P->h0, &P->mincc, &P->maxcc);
         if (ok()) *dt_io = nd;
     }
+    if (!rc && nused &&
+        hipMemcpyAsync(s5->h_ts + 6, nres, sizeof(double) * nused, hipMemcpyDeviceToHost, st) !=
+            hipSuccess)
+        rc = M2D_ERR_HIP;
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = M2D_ERR_HIP;
+    if (!rc) {  // the norms, added in the step's order
+        const double eps = 10.0 * 2.220446049250313080847e-16;  // 10*epsilon(1d0)
+        const double *o = s5->h_ts + 6;
+        for (int k = 0; k < nops; ++k) {
+            const double *q = o + nslot[k];
+            if (nkind[k] == 9)
+                norm = (q[1] > eps) ? norm + q[0] / q[1] : norm + q[0];
+            else
+                norm = norm + q[0];
+        }
+    }
     // nothing of the step stays in flight on the side stream, on any path
     if (s5->aux && hipStreamSynchronize(s5->aux) != hipSuccess && !rc) rc = M2D_ERR_HIP;
     assembly_stream(keep);

@@ -575,6 +575,19 @@ extern "C" void m2d_norm_l1_5d(const int *nx, const int *ny, const double *var, 
     m2d_set_last_status_code(rc);
 }
 
+namespace m2d {
+// norm_l1_5d's sum enqueued on `s` into d_res[0] (partials in d_scr, one per
+// block of 256 columns of a real row)
+void norm5_enqueue(const double *var, const double *b, const double *a, int nx, int ny,
+                   double *d_scr, double *d_res, hipStream_t s)
+{
+    const dim3 grid((unsigned)((nx - 2 + kT - 1) / kT), (unsigned)(ny > 2 ? ny - 2 : 1));
+    const long np = (long)grid.x * grid.y;
+    hipLaunchKernelGGL(k_norm5_partial, grid, dim3(kT), 0, s, a, b, var, nx, ny, d_scr);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(kT), 0, s, d_scr, np, d_res);
+}
+}  // namespace m2d
+
 // get_average, mod_field_operations.f90:115-142
 extern "C" double m2d_get_average(const int *nx, const int *ny, const double *F)
 {

@@ -828,8 +828,10 @@ bool tdma_size_ok(int nx, int ny)
            tdma_lanes(ny, kTdmaLineTiles, kTdmaLineArrays) >= 2;
 }
 
-int tdma_device_many(int count, int nx, int ny, int nitm, const double *const *a,
-                     const double *const *b, double *const *x, hipStream_t s)
+// tdma_device_many without the final wait: the solutions are complete
+// when `s` reaches them (the caller's later work on `s` sees them).
+int tdma_enqueue_many(int count, int nx, int ny, int nitm, const double *const *a,
+                      const double *const *b, double *const *x, hipStream_t s)
 {
     if (count < 1 || !tdma_size_ok(nx, ny) || !a || !b || !x) return M2D_ERR_ARG;
     for (int q = 0; q < count; ++q)
@@ -855,7 +857,14 @@ int tdma_device_many(int count, int nx, int ny, int nitm, const double *const *a
         // the next group reuses the scratch: wait for this one
         if (q0 + kTdmaMaxSys < count && hipStreamSynchronize(s) != hipSuccess) return M2D_ERR_HIP;
     }
-    if (hipGetLastError() != hipSuccess) return M2D_ERR_HIP;
+    return hipGetLastError() == hipSuccess ? M2D_OK : M2D_ERR_HIP;
+}
+
+int tdma_device_many(int count, int nx, int ny, int nitm, const double *const *a,
+                     const double *const *b, double *const *x, hipStream_t s)
+{
+    const int rc = tdma_enqueue_many(count, nx, ny, nitm, a, b, x, s);
+    if (rc) return rc;
     return hipStreamSynchronize(s) == hipSuccess ? M2D_OK : M2D_ERR_HIP;
 }
 
@@ -928,20 +937,32 @@ __global__ __launch_bounds__(256) void k_sum2(const double *__restrict__ pa,
 }
 
 // out[0] = sum |A.var - b|, out[1] = sum |b| over real volumes.
-int norm_l1_device(const double *a, const double *b, const double *var, int nx, int ny,
-                   double *d_work, double *h_out, hipStream_t s)
+// The two sums of the L1 norm enqueued on `s` into d_res[0..1] (partials in
+// d_work, norm_l1_work_size doubles).
+int norm_l1_enqueue(const double *a, const double *b, const double *var, int nx, int ny,
+                    double *d_work, double *d_res, hipStream_t s)
 {
     const int gx = (nx - 2 + 255) / 256, gy = ny - 2;
     const int np = gx * gy;
-    double *pa = d_work, *pb = d_work + np, *res = d_work + 2 * np;
+    double *pa = d_work, *pb = d_work + np;
     hipLaunchKernelGGL(k_norm_l1_partial, dim3(gx, gy), dim3(256), 0, s, a, b, var, nx, ny, pa,
                        pb);
-    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(256), 0, s, pa, pb, np, res);
+    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(256), 0, s, pa, pb, np, d_res);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         fprintf(stderr, "mach2d_hip: norm kernels failed to launch: %s\n", hipGetErrorString(e));
         return M2D_ERR_HIP;
     }
+    return M2D_OK;
+}
+
+int norm_l1_device(const double *a, const double *b, const double *var, int nx, int ny,
+                   double *d_work, double *h_out, hipStream_t s)
+{
+    const int np = ((nx - 2 + 255) / 256) * (ny - 2);
+    double *res = d_work + 2 * np;
+    const int rc = norm_l1_enqueue(a, b, var, nx, ny, d_work, res, s);
+    if (rc) return rc;
     if (hipMemcpyAsync(h_out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
         return M2D_ERR_HIP;
     if (hipStreamSynchronize(s) != hipSuccess) return M2D_ERR_HIP;
