@@ -840,20 +840,31 @@ extern "C" int m2d_solve(m2d_system *s, const double *d_a, const double *d_rhs, 
 // ---------------------------------------------------------------------------
 // max |x| over n values into out[0] (one workgroup, fixed reduction order);
 // no-op while *gate is set.
-__global__ __launch_bounds__(1024) void k_absmax(const double *__restrict__ x, long n,
-                                                 double *__restrict__ out, const int *gate)
+// max |x| over n values into *out, which holds 0 on entry (k_pl_start, then
+// k_pl_check after each use): per-block maxima combined by an unsigned
+// 64-bit atomic maximum of their bit patterns, which order as the values do
+// for non-negative doubles (fmax drops NaN and the maxima start at 0, so
+// none is NaN); the maximum does not depend on the order.
+constexpr int kAbsmaxThreads = 256;
+__global__ __launch_bounds__(kAbsmaxThreads) void k_absmax(const double *__restrict__ x, long n,
+                                                           double *__restrict__ out,
+                                                           const int *gate)
 {
     if (*gate) return;
-    __shared__ double red[1024];
+    __shared__ double red[kAbsmaxThreads];
     double m = 0.0;
-    for (long q = threadIdx.x; q < n; q += 1024) m = fmax(m, fabs(x[q]));
+    for (long q = (long)blockIdx.x * kAbsmaxThreads + threadIdx.x; q < n;
+         q += (long)gridDim.x * kAbsmaxThreads)
+        m = fmax(m, fabs(x[q]));
     red[threadIdx.x] = m;
     __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
+    for (int o = kAbsmaxThreads / 2; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = red[0];
+    if (threadIdx.x == 0)
+        atomicMax(reinterpret_cast<unsigned long long *>(out),
+                  (unsigned long long)__double_as_longlong(red[0]));
 }
 
 // Start of a pressure-correction loop: iterations armed, none run.
@@ -922,6 +933,7 @@ __global__ void k_pl_check(SolveState *st, PlCtl *c)
     const bool err = st->outside || st->err_pivot || st->abort ||
                      st->status == M2D_ERR_NONFINITE || st->status == M2D_ERR_SYNC_TIMEOUT;
     if (!err) c->normpl = c->mx / c->p_avg;
+    c->mx = 0.0;  // k_absmax's next maximum starts from 0
     if (err || c->normpl < c->tolm) {
         st->skip = 1;
         st->done = 1;
@@ -1010,7 +1022,10 @@ static int enqueue_pl_iterations(m2d_system *s, const m2d_pl_loop_fields *f, int
             rc = M2D_ERR_ARG;
             break;
         }
-        hipLaunchKernelGGL(k_absmax, dim3(1), dim3(1024), 0, s->stream, f->pl, s->N, &s->pl->mx,
+        hipLaunchKernelGGL(k_absmax,
+                           dim3((unsigned)std::min<long>(1024, (s->N + kAbsmaxThreads - 1) /
+                                                                   kAbsmaxThreads)),
+                           dim3(kAbsmaxThreads), 0, s->stream, f->pl, s->N, &s->pl->mx,
                            &s->st->skip);
         hipLaunchKernelGGL(k_pl_check, dim3(1), dim3(64), 0, s->stream, s->st, s->pl);
         if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
