@@ -1587,6 +1587,18 @@ int tdma_enqueue_many(int count, int nx, int ny, int nitm, const double *const *
                       const double *const *b, double *const *x, hipStream_t s);
 }  // namespace m2d
 
+// dst[k] = src[k] for the K arrays of n values of a list, one launch
+// (grid row k copies array k)
+struct CopyList {
+    const double *src[9];
+    double *dst[9];
+};
+__global__ __launch_bounds__(256) void k_copy_list(CopyList c, long n)
+{
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e < n) c.dst[blockIdx.y][e] = c.src[blockIdx.y][e];
+}
+
 // g = 1 / (Rg T) (mach2d_main.f90:153)
 __global__ __launch_bounds__(256) void k_g_nodes(const double *__restrict__ T, double Rg, long n,
                                                  double *__restrict__ g)
@@ -1688,11 +1700,6 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
         if (q.status && !rc) rc = q.status;
         return rc == M2D_OK;
     };
-    auto copy = [&](double *dst, const double *src, long n) {
-        if (!rc && hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, st) !=
-                       hipSuccess)
-            rc = M2D_ERR_HIP;
-    };
     const unsigned nb = (unsigned)((N + 255) / 256);
     if (ts_side_ready(s5) != M2D_OK) {
         assembly_stream(keep);
@@ -1725,16 +1732,13 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
         nslot[nops++] = nused;
         nused += 1;
     };
-    // time levels (mach2d_main.f90:96-104)
-    copy(f->roa, f->ro, N);
-    copy(f->Ta, f->T, N);
-    copy(f->pa, f->p, N);
-    copy(f->ua, f->u, N);
-    copy(f->va, f->v, N);
-    copy(f->uea, f->ue, N);
-    copy(f->vea, f->ve, N);
-    copy(f->una, f->un, N);
-    copy(f->vna, f->vn, N);
+    // time levels (mach2d_main.f90:96-104), one launch
+    {
+        const CopyList cl = {{f->ro, f->T, f->p, f->u, f->v, f->ue, f->ve, f->un, f->vn},
+                             {f->roa, f->Ta, f->pa, f->ua, f->va, f->uea, f->vea, f->una, f->vna}};
+        hipLaunchKernelGGL(k_copy_list, dim3(nb, 9), dim3(256), 0, st, cl, N);
+        if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
+    }
     // inflow update and fictitious pressures (:107)
     stage = 2;
     if (!rc)
