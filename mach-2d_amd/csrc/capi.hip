@@ -1581,6 +1581,10 @@ int norm_l1_enqueue(const double *a, const double *b, const double *var, int nx,
                     double *d_work, double *d_res, hipStream_t s);
 void norm5_enqueue(const double *var, const double *b, const double *a, int nx, int ny,
                    double *d_scr, double *d_res, hipStream_t s);
+int time_cycle_enqueue(int NX, int NY, double g, double rg, double po, double t0,
+                       const double *x, const double *xp, const double *u, double *uin,
+                       double *vin, double *pin, double *Tin, double *Mw, double *plin,
+                       double *pina, double *p, double *d_scr, double *h_buf, hipStream_t s);
 int tdma_device_many(int count, int nx, int ny, int nitm, const double *const *a,
                      const double *const *b, double *const *x, hipStream_t s);
 int tdma_enqueue_many(int count, int nx, int ny, int nitm, const double *const *a,
@@ -1641,18 +1645,19 @@ static int solve9(m2d_system *s9, const m2d_time_step_params *P, const double *a
 // (norm_l1_work_size), then the in-order sums': the average of p (n + 2
 // doubles), the terms of the four mean coefficients and of the mass check
 // (5 n) and their five sums, n = (nx-2)(ny-2), then kTsNormSlots sums of the
-// step's residual norms; h_ts receives p's average, the five sums and the
-// norms' sums.  Created with the first step on the handle.
+// step's residual norms, then the inflow update's 3 (ny-2) values; h_ts
+// receives p's average, the five sums and the norms' sums, and holds the
+// inflow update's host values.  Created with the first step on the handle.
 constexpr int kTsNormSlots = 8;
 static int ts_side_ready(m2d_system *s)
 {
     if (s->aux) return M2D_OK;
-    const long n = (long)(s->nx - 2) * (s->ny - 2);
+    const long n = (long)(s->nx - 2) * (s->ny - 2), nr = 3L * (s->ny - 2);
     HIP_OK(hipMalloc((void **)&s->ts_scr,
                      sizeof(double) * (size_t)(norm_l1_work_size(s->nx, s->ny) +
                                                average_scratch(s->nx, s->ny) + 5 * n + 5 +
-                                               kTsNormSlots)));
-    HIP_OK(hipHostMalloc((void **)&s->h_ts, sizeof(double) * (6 + kTsNormSlots),
+                                               kTsNormSlots + nr)));
+    HIP_OK(hipHostMalloc((void **)&s->h_ts, sizeof(double) * (size_t)(6 + kTsNormSlots + nr),
                          hipHostMallocDefault));
     HIP_OK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
@@ -1687,6 +1692,7 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
     for (const void *q : need)
         if (!q) return M2D_ERR_ARG;
     if (!euler && (!f->cp || !f->ke || !f->kn || !f->fbe || !f->fbn)) return M2D_ERR_ARG;
+    set_last(M2D_OK);  // the stage checks below read the thread's last status
     const int nx = s5->nx, ny = s5->ny;
     const long N = s5->N;
     const hipStream_t st = s5->stream;
@@ -1742,9 +1748,9 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
     // inflow update and fictitious pressures (:107)
     stage = 2;
     if (!rc)
-        m2d_intflow_time_cycle_update(&nx, &ny, &P->gamma, &P->Rg, &P->P0, &P->T0, f->x, f->xp,
-                                      f->u, f->uin, f->vin, f->pin, f->Tin, f->Mw, f->plin,
-                                      f->pina, f->p);
+        rc = time_cycle_enqueue(nx, ny, P->gamma, P->Rg, P->P0, P->T0, f->x, f->xp, f->u, f->uin,
+                                f->vin, f->pin, f->Tin, f->Mw, f->plin, f->pina, f->p,
+                                nres + kTsNormSlots, s5->h_ts + 6 + kTsNormSlots, st);
     // get_average(p) (:151) on the side stream, beside the u and v stages:
     // only the p' loop changes p, and it waits for the value
     double p_avg = 0.0;

@@ -576,6 +576,35 @@ extern "C" void m2d_norm_l1_5d(const int *nx, const int *ny, const double *var, 
 }
 
 namespace m2d {
+// intflow_time_cycle_update on device arrays for the time step, with the
+// caller's scratch: d_scr and the pinned h_buf, 3 max(ny-2, 1) doubles
+// each.  One wait, for the west u values the C library's pow needs; the
+// pressures go back from h_buf, which the caller leaves alone until its
+// stream has passed this point.
+int time_cycle_enqueue(int NX, int NY, double g, double rg, double po, double t0,
+                       const double *x, const double *xp, const double *u, double *uin,
+                       double *vin, double *pin, double *Tin, double *Mw, double *plin,
+                       double *pina, double *p, double *d_scr, double *h_buf, hipStream_t s)
+{
+    if (NX < 3 || NY < 3) return M2D_ERR_ARG;
+    const int nr = NY - 2;  // inflow rows j = 2..ny-1
+    double *duw = d_scr, *dp = d_scr + 2 * nr, *hu = h_buf, *hp = h_buf + 2 * nr;
+    hipLaunchKernelGGL(k_inflow_u, dim3((unsigned)((nr + kT - 1) / kT)), dim3(kT), 0, s, NX, NY,
+                       u, duw);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(hu, duw, sizeof(double) * 2 * nr, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return M2D_ERR_HIP;
+    for (int q = 0; q < nr; ++q) hp[q] = inflow_pressure(g, rg, po, t0, hu[2 * q], hu[2 * q + 1]);
+    if (hipMemcpyAsync(dp, hp, sizeof(double) * nr, hipMemcpyHostToDevice, s) != hipSuccess)
+        return M2D_ERR_HIP;
+    const int L = NX > NY ? NX : NY;
+    hipLaunchKernelGGL(k_time_cycle, dim3((unsigned)((L + kT - 1) / kT)), dim3(kT), 0, s, NX, NY,
+                       g, rg, t0, x, xp, u, dp, uin, vin, pin, Tin, Mw, plin, pina, p);
+    return hipGetLastError() == hipSuccess ? M2D_OK : M2D_ERR_HIP;
+}
+
 // norm_l1_5d's sum enqueued on `s` into d_res[0] (partials in d_scr, one per
 // block of 256 columns of a real row)
 void norm5_enqueue(const double *var, const double *b, const double *a, int nx, int ny,
