@@ -829,7 +829,10 @@ bool tdma_size_ok(int nx, int ny)
 }
 
 // tdma_device_many without the final wait: the solutions are complete
-// when `s` reaches them (the caller's later work on `s` sees them).
+// when `s` reaches them (the caller's later work on `s` sees them).  The
+// scratch is this host thread's, reused by its next call: a next call on
+// another stream must wait for `s` first (the time step, its one caller,
+// ends with a wait on `s`).
 int tdma_enqueue_many(int count, int nx, int ny, int nitm, const double *const *a,
                       const double *const *b, double *const *x, hipStream_t s)
 {
