@@ -1753,22 +1753,34 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
                                 nres + kTsNormSlots, s5->h_ts + 6 + kTsNormSlots, st);
     // get_average(p) (:151) on the side stream, beside the u and v stages:
     // only the p' loop changes p, and it waits for the value
+    // The fork is recorded here, at p after the inflow update; the side
+    // stream's calls are issued once the u and v stages are queued, so the
+    // device is not left without work while the host makes them.
     double p_avg = 0.0;
-    bool avg_pending = false;
+    bool avg_forked = false, avg_pending = false;
     if (ok()) stage = 3;
     if (!rc) {
-        double *scr = side;
-        if (hipEventRecord(s5->ev_fork, st) != hipSuccess ||
-            hipStreamWaitEvent(s5->aux, s5->ev_fork, 0) != hipSuccess)
-            rc = M2D_ERR_HIP;
-        if (!rc) average_enqueue(nx, ny, f->p, scr, s5->aux);
-        if (!rc && (hipGetLastError() != hipSuccess ||
-                    hipMemcpyAsync(s5->h_ts, scr, sizeof(double), hipMemcpyDeviceToHost,
-                                   s5->aux) != hipSuccess ||
-                    hipEventRecord(s5->ev_join, s5->aux) != hipSuccess))
-            rc = M2D_ERR_HIP;
-        avg_pending = !rc;
+        if (hipEventRecord(s5->ev_fork, st) != hipSuccess) rc = M2D_ERR_HIP;
+        avg_forked = !rc;
     }
+    auto avg_enqueue = [&]() {
+        if (!avg_forked || rc) return;
+        avg_forked = false;
+        double *scr = side;
+        if (hipStreamWaitEvent(s5->aux, s5->ev_fork, 0) != hipSuccess) {
+            rc = M2D_ERR_HIP;
+            return;
+        }
+        average_enqueue(nx, ny, f->p, scr, s5->aux);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(s5->h_ts, scr, sizeof(double), hipMemcpyDeviceToHost, s5->aux) !=
+                hipSuccess ||
+            hipEventRecord(s5->ev_join, s5->aux) != hipSuccess) {
+            rc = M2D_ERR_HIP;
+            return;
+        }
+        avg_pending = true;
+    };
     if (ok()) {
         hipLaunchKernelGGL(k_g_nodes, dim3(nb), dim3(256), 0, st, f->T, P->Rg, N, f->g);
         if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
@@ -1808,6 +1820,8 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
             m2d_intflow_boundary_simplec(&nx, &ny, &modvis, f->due, f->dve, f->dun, f->dvn,
                                          f->de, f->dn);
         if (!ok()) break;
+        avg_enqueue();
+        if (rc) break;
         stage = 5;
         if (P->solver9d == 1) {
             // TDMA: u and v are independent systems (both assembled above),
@@ -1845,6 +1859,8 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
                                 f->de,  f->dn,  f->g,   f->roa, f->u,   f->v,   f->plin,
                                 f->ue,  f->ve,  f->un,  f->vn,  f->Uce, f->Vcn, f->roe,
                                 f->ron, f->ro,  f->p,   f->pl,  f->ap,  f->bp};
+        avg_enqueue();  // (already issued after the u and v stages)
+        if (rc) break;
         if (avg_pending) {
             if (hipEventSynchronize(s5->ev_join) != hipSuccess) {
                 rc = M2D_ERR_HIP;
