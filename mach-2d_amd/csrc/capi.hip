@@ -1785,6 +1785,44 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
         hipLaunchKernelGGL(k_g_nodes, dim3(nb), dim3(256), 0, st, f->T, P->Rg, N, f->g);
         if (hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
     }
+    // The in-order sums that end the step run on the side stream: the mass
+    // check's (:346-348), whose terms are taken before the densities change,
+    // and, for Euler steps, where au, av, at, ap are final once the last p'
+    // loop has run, the four mean coefficients of the next time step (:394).
+    // Their terms are taken on st right after the last p' loop (neither
+    // reads u or v, which the rest of that cycle changes), so the sums, the
+    // step's longest tail, start as early as they can.  Scratch: terms of
+    // the four means at t[0, 4n), of the mass check at t[4n, 5n), the sums
+    // at t[5n, 5n + 5).
+    double *tsum = side + average_scratch(nx, ny);
+    bool side_forked = false, side_pending = false;
+    auto sums_enqueue = [&]() {
+        if (side_forked || rc) return;
+        side_forked = true;
+        if (euler) {
+            const double *a4[4] = {f->au, f->av, f->at, f->ap};
+            const int nd4[4] = {9, 9, 9, 5};
+            cc_terms_enqueue(nx, ny, 4, a4, nd4, tsum, st);
+        }
+        const double *mi[10] = {f->rp, f->re, f->rn,  f->Jp,  f->ro,
+                                f->roa, f->ron, f->roe, f->Uce, f->Vcn};
+        mass_terms_enqueue(nx, ny, dt, mi, tsum + 4 * nin, st);
+        const int first = euler ? 0 : 4, K = 5 - first;
+        if (hipGetLastError() != hipSuccess || hipEventRecord(s5->ev_fork, st) != hipSuccess ||
+            hipStreamWaitEvent(s5->aux, s5->ev_fork, 0) != hipSuccess) {
+            rc = M2D_ERR_HIP;
+            return;
+        }
+        seq_sums_enqueue(tsum + first * nin, nin, K, tsum + 5 * nin + first, s5->aux);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(s5->h_ts + 1 + first, tsum + 5 * nin + first, sizeof(double) * K,
+                           hipMemcpyDeviceToHost, s5->aux) != hipSuccess ||
+            hipEventRecord(s5->ev_join, s5->aux) != hipSuccess) {
+            rc = M2D_ERR_HIP;
+            return;
+        }
+        side_pending = true;
+    };
     for (int itm = 1; itm <= P->itmmax && !rc; ++itm) {  // mass cycle (:155-344)
         nops = nused = 0;  // norm = 0
         stage = 4;
@@ -1872,6 +1910,8 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
         rc = m2d_intflow_pl_loop(s5, &L, modvis, dt, beta, p_avg, P->tolm, P->itpmax,
                                  P->nitm_p, P->tol_p, &itp, &normpl);
         if (rc) break;
+        if (itm == P->itmmax) sums_enqueue();
+        if (rc) break;
         stage = 9;
         norm5(f->ap, f->bp, f->pl);
         if (ok())
@@ -1882,41 +1922,8 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
                                                       f->yk, f->u, f->v);
         ok();
     }
-    // The in-order sums that end the step run on the side stream, beside T
-    // and the densities: the mass check's (:346-348), whose terms are taken
-    // on st before the densities change, and, for Euler steps, where au, av,
-    // at, ap are final once the mass cycles end, the four mean coefficients
-    // of the next time step (:394).  Scratch: terms of the four means at
-    // t[0, 4n), of the mass check at t[4n, 5n), the sums at t[5n, 5n + 5).
-    double *tsum = side + average_scratch(nx, ny);
-    bool side_pending = false;
-    if (!rc && euler) {
-        const double *a4[4] = {f->au, f->av, f->at, f->ap};
-        const int nd4[4] = {9, 9, 9, 5};
-        if (hipEventRecord(s5->ev_fork, st) != hipSuccess ||
-            hipStreamWaitEvent(s5->aux, s5->ev_fork, 0) != hipSuccess)
-            rc = M2D_ERR_HIP;
-        if (!rc) cc_terms_enqueue(nx, ny, 4, a4, nd4, tsum, s5->aux);
-        if (!rc && hipGetLastError() != hipSuccess) rc = M2D_ERR_HIP;
-    }
     if (!rc) stage = 10;
-    if (!rc) {  // mass conservation (:346-348)
-        const double *mi[10] = {f->rp, f->re, f->rn,  f->Jp,  f->ro,
-                                f->roa, f->ron, f->roe, f->Uce, f->Vcn};
-        mass_terms_enqueue(nx, ny, dt, mi, tsum + 4 * nin, st);
-        const int first = euler ? 0 : 4, K = 5 - first;
-        if (hipGetLastError() != hipSuccess || hipEventRecord(s5->ev_fork, st) != hipSuccess ||
-            hipStreamWaitEvent(s5->aux, s5->ev_fork, 0) != hipSuccess)
-            rc = M2D_ERR_HIP;
-        if (!rc) seq_sums_enqueue(tsum + first * nin, nin, K, tsum + 5 * nin + first, s5->aux);
-        if (!rc && (hipGetLastError() != hipSuccess ||
-                    hipMemcpyAsync(s5->h_ts + 1 + first, tsum + 5 * nin + first,
-                                   sizeof(double) * K, hipMemcpyDeviceToHost,
-                                   s5->aux) != hipSuccess ||
-                    hipEventRecord(s5->ev_join, s5->aux) != hipSuccess))
-            rc = M2D_ERR_HIP;
-        side_pending = !rc;
-    }
+    sums_enqueue();  // (issued after the last p' loop)
     if (!rc) stage = 11;
     if (!rc && euler) {  // Euler, constant properties: T from H (:349-362)
         hipLaunchKernelGGL(k_T_nodes, dim3(nb), dim3(256), 0, st, f->u, f->v, P->Href, P->Cpref,
