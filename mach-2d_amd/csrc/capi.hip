@@ -176,7 +176,7 @@ struct m2d_system {
     // statistics that run beside the step, its events, the device scratch
     // of those sums and the pinned copy of their results (p_avg, 4 means)
     hipStream_t aux = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_dep = nullptr;
     double *ts_scr = nullptr, *h_ts = nullptr;
 };
 
@@ -269,6 +269,7 @@ extern "C" int m2d_destroy(m2d_system *s)
     if (s->aux) (void)hipStreamDestroy(s->aux);
     if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
     if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+    if (s->ev_dep) (void)hipEventDestroy(s->ev_dep);
     if (s->ts_scr) (void)hipFree(s->ts_scr);
     if (s->h_ts) (void)hipHostFree(s->h_ts);
     if (s->pl) (void)hipFree(s->pl);
@@ -1623,13 +1624,15 @@ __global__ __launch_bounds__(256) void k_T_nodes(const double *__restrict__ u,
 // solver9d%solve (mod_solver_msi.f90:96-133 or mod_solver_tdma.f90) on the
 // 9-diagonal system a x = b from the current x
 static int solve9(m2d_system *s9, const m2d_time_step_params *P, const double *a,
-                  const double *b, double *x, int nx, int ny, hipStream_t stream)
+                  const double *b, double *x, int nx, int ny, hipStream_t stream, hipEvent_t dep)
 {
     if (P->solver9d == 1) return m2d_tdma2d9_device(nx, ny, P->nitm_u, a, b, x, stream);
-    // the 9-diagonal handle has its own stream: the system assembled on
-    // `stream` must be complete before its factorization starts (read_state
-    // below completes the solve before `stream` uses x)
-    if (hipStreamSynchronize(stream) != hipSuccess) return M2D_ERR_HIP;
+    // the 9-diagonal handle has its own stream: it waits for the system
+    // assembled on `stream` through `dep` (no host wait); read_state below
+    // completes the solve before `stream` uses x
+    if (hipEventRecord(dep, stream) != hipSuccess ||
+        hipStreamWaitEvent(s9->stream, dep, 0) != hipSuccess)
+        return M2D_ERR_HIP;
     enqueue_factor64(s9, a);
     enqueue_solve64(s9, a, b, x, P->tol_u, P->nitm_u);
     int rc = read_state(s9, s9->st);
@@ -1661,6 +1664,7 @@ static int ts_side_ready(m2d_system *s)
                          hipHostMallocDefault));
     HIP_OK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&s->ev_dep, hipEventDisableTiming));
     HIP_OK(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking));
     return M2D_OK;
 }
@@ -1872,11 +1876,11 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
             norm9_relative(f->av, f->bv, f->v);
             if (rc) break;
         } else {
-            rc = solve9(s9, P, f->au, f->bu, f->u, nx, ny, st);
+            rc = solve9(s9, P, f->au, f->bu, f->u, nx, ny, st, s5->ev_dep);
             norm9_relative(f->au, f->bu, f->u);
             if (rc) break;
             stage = 6;
-            rc = solve9(s9, P, f->av, f->bv, f->v, nx, ny, st);
+            rc = solve9(s9, P, f->av, f->bv, f->v, nx, ny, st, s5->ev_dep);
             norm9_relative(f->av, f->bv, f->v);
             if (rc) break;
         }
@@ -1942,7 +1946,7 @@ extern "C" int m2d_intflow_time_step(m2d_system *s9, m2d_system *s5,
             f->ke, f->kn, f->Uce, f->Vcn, f->u, f->v, f->ue, f->ve, f->un, f->vn, f->ua, f->va,
             f->T, f->Ta, f->fbe, f->fbn, f->at, f->bt);
         if (ok()) m2d_intflow_set_bcT(&nx, &ny, &P->Twall, f->x, f->xp, f->T, f->Tin, f->at, f->bt);
-        if (ok()) rc = solve9(s9, P, f->at, f->bt, f->T, nx, ny, st);
+        if (ok()) rc = solve9(s9, P, f->at, f->bt, f->T, nx, ny, st, s5->ev_dep);
         norm9_relative(f->at, f->bt, f->T);
     }
     if (!rc) stage = 12;
