@@ -14,6 +14,7 @@ place, as the reference updates its module arrays.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 from . import _lib
 from ._lib import SolverError
@@ -138,15 +139,24 @@ def ts_size(k, nx, ny):
     return 9 * N if k in ("au", "av", "at") else 5 * N if k == "ap" else N
 
 
-def intflow_time_step(dev9, dev5, f, params, dt):
-    """One time step on device tensors `f` (names of TS_ALL; TS_OPTIONAL may
-    be missing for Euler) with the FP64 handles dev9 (9 diagonals; None for
-    solver9d = 1) and dev5 (5 diagonals).  Returns (next dt, result dict)."""
+# the last checked field set per grid size: weak references to its tensors,
+# their data pointers and the filled structure (a step's Python overhead was
+# mostly these checks)
+_TS_LAST = {}
+
+
+def _ts_fields(f, nx, ny):
     import torch
-    nx, ny = dev5.nx, dev5.ny
+    ts = [f.get(k) for k in TS_ALL]
+    last = _TS_LAST.get((nx, ny))
+    if last is not None:
+        refs, ptrs, flds = last
+        if all((r is None) if t is None else (r is not None and r() is t)
+               for r, t in zip(refs, ts)) and \
+                ptrs == tuple(t.data_ptr() for t in ts if t is not None):
+            return flds
     flds = TimeStepFields()
-    for k in TS_ALL:
-        t = f.get(k)
+    for k, t in zip(TS_ALL, ts):
         if t is None and k in TS_OPTIONAL:
             continue
         n = ts_size(k, nx, ny)
@@ -154,6 +164,16 @@ def intflow_time_step(dev9, dev5, f, params, dt):
                 and t.is_contiguous() and t.numel() == n):
             raise TypeError("%s must be a contiguous float64 CUDA tensor of %d values" % (k, n))
         setattr(flds, k, t.data_ptr())
+    _TS_LAST[(nx, ny)] = (tuple(None if t is None else weakref.ref(t) for t in ts),
+                          tuple(t.data_ptr() for t in ts if t is not None), flds)
+    return flds
+
+
+def intflow_time_step(dev9, dev5, f, params, dt):
+    """One time step on device tensors `f` (names of TS_ALL; TS_OPTIONAL may
+    be missing for Euler) with the FP64 handles dev9 (9 diagonals; None for
+    solver9d = 1) and dev5 (5 diagonals).  Returns (next dt, result dict)."""
+    flds = _ts_fields(f, dev5.nx, dev5.ny)
     prm = TimeStepParams(**params)
     d = C.c_double(dt)
     res = TimeStepResult()
