@@ -336,6 +336,25 @@ def test_time_step_entry_rejects_bad_arguments():
     assert dt.value == 1e-5
 
 
+def test_tuning_entries_check_their_arguments():
+    """The host-side setters added beside the reference's interface check
+    their arguments without touching the GPU: the TDMA line solver takes
+    modes 0 and 1 only (and the default is restored), the p' solve-cap
+    setter and query reject a missing handle or output."""
+    import ctypes as C
+
+    from mach2d import _lib
+    L = _lib.load()
+    assert L.m2d_set_tdma_mode(2) != 0
+    assert L.m2d_set_tdma_mode(-1) != 0
+    assert L.m2d_set_tdma_mode(1) == 0
+    assert L.m2d_set_tdma_mode(0) == 0
+    a, b = C.c_int(), C.c_int()
+    assert L.m2d_set_pl_solve_cap(None, 4) == _lib.M2D_ERR_ARG
+    assert L.m2d_pl_loop_cap_info(None, C.byref(a), C.byref(b)) == _lib.M2D_ERR_ARG
+    assert L.m2d_pl_loop_info(None, C.byref(a), C.byref(b)) == _lib.M2D_ERR_ARG
+
+
 def test_m2dt_round_trip_and_checks(tmp_path):
     """The training-set format: records and metadata come back as written
     (float32), the header carries grid, diagonals, count, gamma and source,
