@@ -99,6 +99,32 @@ def test_tdma_device_many_equals_single(gpu, orc):
     assert all(torch.equal(u, v) for u, v in zip(keep, tx))
 
 
+def test_tdma_pcr_device_many_equals_single(gpu, orc):
+    """PCR mode: systems solved in the same launches (more than one group,
+    so the scratch is reused between groups) give each its solo result bit
+    for bit, each within 1e-12 of the oracle's recurrence; the scratch of
+    one system never leaks into another."""
+    import torch
+    nx, ny, count = 258, 130, 10  # two groups of at most 8 systems
+    systems = [P.random_f3_system(nx, ny, seed=50 + q) for q in range(count)]
+    x0 = [np.random.default_rng(q + 9).uniform(-1, 1, nx * ny) for q in range(count)]
+    ta = [torch.tensor(s.a.reshape(-1), device="cuda") for s in systems]
+    tb = [torch.tensor(s.rhs, device="cuda") for s in systems]
+    tx = [torch.tensor(v, device="cuda") for v in x0]
+    gpu.set_tdma_mode(1)
+    try:
+        gpu.tdma2d9_device_many(nx, ny, 5, ta, tb, tx)
+        for q, s in enumerate(systems):
+            want = orc.tdma2d9(s.a, s.rhs, x0[q], nx, ny, 5)
+            got = tx[q].cpu().numpy()
+            assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), q
+            one = torch.tensor(x0[q], device="cuda")
+            gpu.tdma2d9_device_many(nx, ny, 5, [ta[q]], [tb[q]], [one])
+            assert torch.equal(one, tx[q]), q
+    finally:
+        gpu.set_tdma_mode(0)
+
+
 def test_tdma_solver_objects(gpu, orc):
     s = P.pcorr_system(66, 34, seed=2)
     x = np.zeros(s.N)
