@@ -127,14 +127,15 @@ def test_pl_loop_exit_inside_a_batch(gpu, orc):
     dev.close()
 
 
-@pytest.mark.parametrize("cap", [1, 19])
-def test_pl_loop_short_solve_capture_reruns_same_bits(gpu, orc, cap):
+@pytest.mark.parametrize("cap,graph", [(1, 1), (19, 1), (1, 0)])
+def test_pl_loop_short_solve_capture_reruns_same_bits(gpu, orc, cap, graph):
     """Each p' solve enqueued with `cap` of its nitm_p = 20 iterations
     (m2d_set_pl_solve_cap): these solves take 18-20 iterations, so the
     first one runs out, the loop ends early and the call runs again from
     the saved fields with all 20 enqueued.  Every field, itp and
     max|pl| / p_avg equal the oracle's loop bit for bit, on the call that
-    captures the batches and on the one that replays them."""
+    captures the batches and on the one that replays them (graph 1), and
+    with the kernels enqueued directly (graph 0)."""
     from mach2d import simplec
     nx, ny = 70, 38
     f = PL.loop_fields(nx, ny, seed=11)
@@ -145,6 +146,7 @@ def test_pl_loop_short_solve_capture_reruns_same_bits(gpu, orc, cap):
                             kw["tolm"], kw["itpmax"], kw["nitm_p"], kw["tol_p"])
     dev = gpu.DeviceMSI(nx, ny, "fp64", ndiag=5)
     simplec.set_pl_solve_cap(dev, cap)
+    simplec.set_pl_graph(dev, graph)
     for call in range(2):
         t = _tensors(f)
         itp, normpl = simplec.intflow_pl_loop(dev, t, **kw)
