@@ -869,10 +869,20 @@ __global__ __launch_bounds__(kAbsmaxThreads) void k_absmax(const double *__restr
 }
 
 // Start of a pressure-correction loop: iterations armed, none run.
-__global__ void k_pl_start(SolveState *st, PlCtl *c)
+// pl = 0 (mach2d_main.f90:257) by every thread; the per-call scalars into
+// the control block and the loop armed by the first.
+__global__ __launch_bounds__(256) void k_pl_start(SolveState *st, PlCtl *c, double *pl, long n,
+                                                  double dt, double beta, double p_avg,
+                                                  double tolm)
 {
-    if (threadIdx.x != 0) return;
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e < n) pl[e] = 0.0;
+    if (e != 0) return;
     st->skip = 0;
+    c->dt = dt;
+    c->beta = beta;
+    c->p_avg = p_avg;
+    c->tolm = tolm;
     c->itp = 0;
     c->mx = 0.0;
     c->normpl = 0.0;
@@ -1145,15 +1155,8 @@ extern "C" int m2d_intflow_pl_loop(m2d_system *s, const m2d_pl_loop_fields *f, i
     auto run_loop = [&](int solve_cap) -> int {
         // pl = 0 (:257) and the per-call scalars, ahead of the iterations
         memset(h, 0, sizeof(*h));
-        h->dt = dt;
-        h->beta = beta;
-        h->p_avg = p_avg;
-        h->tolm = tolm;
-        if (hipMemsetAsync(f->pl, 0, sizeof(double) * s->N, s->stream) != hipSuccess ||
-            hipMemcpyAsync(s->pl, h, sizeof(PlCtl), hipMemcpyHostToDevice, s->stream) !=
-                hipSuccess)
-            return M2D_ERR_HIP;
-        hipLaunchKernelGGL(k_pl_start, dim3(1), dim3(64), 0, s->stream, s->st, s->pl);
+        hipLaunchKernelGGL(k_pl_start, dim3((unsigned)((s->N + 255) / 256)), dim3(256), 0,
+                           s->stream, s->st, s->pl, f->pl, s->N, dt, beta, p_avg, tolm);
         if (hipGetLastError() != hipSuccess) return M2D_ERR_HIP;
         // batches of iterations, each followed by one read of the control
         // block and the solver state.  The first batch is as long as the
